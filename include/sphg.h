@@ -1,0 +1,187 @@
+/*
+ * sphg.h — C-ABI of the MI355X-native per-step particle loop of the multi-phase SPH solver.
+ *
+ * Every entry point replaces one step function of the reference OpenMP solver; the citation
+ * above each names the reference interface (file:line) it stands in for. All reference file
+ * citations are into the wodexinxin2015/MultiPhase-SPH-Solver sources.
+ *
+ * Conventions
+ *   - Plain C types only (no HIP or torch types). One context = one simulation (or one batch of
+ *     realisations) resident in device memory, driven on one HIP stream the context owns.
+ *   - Return value: 0 = OK; the reference's own Err_Deal codes (CPP_0-ErrorDeal.cpp:44-277) where
+ *     the reference raises one: 3 = particle outside the cell grid (CPP_3-NNPS.cpp:250-257),
+ *     5 = more than 99 neighbours (CPP_3-NNPS.cpp:313-319), 2 = no particles (CPP_1-SPHLoops.cpp:376);
+ *     codes >= 100 are this library's own (SPHG_E_*).
+ *   - Step functions are asynchronous on the context stream; device-side error codes accumulate
+ *     in a sticky error word that sphg_sync() (and every sphg_step()) returns.
+ *   - Contexts are independent; a context must not be used from two host threads at once.
+ */
+#ifndef SPHG_H_INCLUDED
+#define SPHG_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHG_MAX_NB 99       /* neighbour cap of Par_Cell::influ (Class_NNPS.h:72, CPP_3-NNPS.cpp:313) */
+#define SPHG_NSOIL 30        /* soil parameter slots: layer*10 + model (CPP_2-initialf.cpp:487-795) */
+
+enum {
+    SPHG_OK = 0,
+    SPHG_E_NOPARTICLES = 2,
+    SPHG_E_CELLRANGE = 3,
+    SPHG_E_NEIGHBOURS = 5,
+    SPHG_E_ARG = 100,        /* bad argument (null pointer, unknown field, size mismatch) */
+    SPHG_E_HIP = 101,        /* HIP runtime error */
+    SPHG_E_UNSUPPORTED = 102,/* flag combination this build does not implement */
+    SPHG_E_CAPACITY = 103    /* exceeds the context's capacity */
+};
+
+enum { SPHG_FP64 = 0, SPHG_MIXED = 1 };
+
+/* Problem parameters: Para_FL (Class_Problem.h:10-42), Para_Pro (:44-84), the free-surface
+ * thresholds read with Para_Pro (CPP_2-initialf.cpp:75), Para_GF (:150-162), Para_Boundary
+ * (:118-133), Para_Inter (:104-116), Para_SSInter (:176-192), Para_Model flags
+ * (Class_ParticleProperty.h:10-28) and the Cell_Con domain box (Class_NNPS.h:12-36). */
+typedef struct {
+    int fla, fls, flr, fli, flv, fleob, flts, flspv, flknl, ntd, flbndy;
+    int ndim, fop, inip, loop, ntotal, step_regu, dttype;
+    double dr, dt, tt;
+    double xsph[3];
+    double art_vis[3][2];
+    double art_stre_coe;
+    double threshold_min, threshold_max;
+    double gx, gy, gz;
+    int if_cal, if_move;
+    double coe_bndy, move_vx[3];
+    double interc;
+    int permtype;
+    double pers, perl, perm;
+    int ss_type;
+    double ss_myu, ss_zeta;
+    int ss_acc_type;
+    int flagem, flagsb, flagdp, flagum, flagam, flagbui, flagpl, flagve, flagfocc, flagep;
+    double alphats;
+    /* Cell_Con: particle bounding box widened by 2*dr (CPP_2-initialf.cpp:976-999) */
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    double soil_min[3], soil_max[3], water_min[3], water_max[3];
+} sphg_problem;
+
+/* Para_Soil, field for field (Class_ParticleProperty.h:33-101). */
+typedef struct {
+    double e, poi, eps0, dens, damp1, damp2, cop, ramda, kapa, zmf, ann, bnn, beta;
+    double sitas, sitar, epsr, epss, sd, sw, kse, c1, c2, c3, fai, c, ocr, sme0;
+    double mzm, azm, r0, bl, bz, zet, vpu, porosity, fs, etao, g0, ds, damp_coe;
+} sphg_soil;
+
+/* Para_Fluid (Class_ParticleProperty.h:104-133). */
+typedef struct {
+    double vis, vpu, dens, paa0, porosity, fai, c;
+    int bhtype, press_type, acc_type;
+    double pre_coe, strain_min;
+} sphg_fluid;
+
+/* Para_Structure (Class_ParticleProperty.h:135-151). */
+typedef struct {
+    double dens, eps0, e, poi, vpu, porosity, cop;
+} sphg_structure;
+
+/* Material tables as initial_mat leaves them (CPP_2-initialf.cpp:325-852). */
+typedef struct {
+    sphg_soil soil[SPHG_NSOIL];
+    sphg_fluid water[3];
+    sphg_fluid air;
+    sphg_structure structure;
+} sphg_materials;
+
+typedef struct sphg_ctx sphg_ctx;
+
+/* Context creation: capacity in particles and the arithmetic form (SPHG_FP64 or SPHG_MIXED).
+ * Replaces the allocation block of SPH_Calculation (CPP_1-SPHLoops.cpp:354-378) and the
+ * cell-list set-up celllist_ini1/celllist_ini2 (CPP_3-NNPS.cpp:96-171, called at :446-450). */
+int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat,
+                int capacity, int precision);
+void sphg_destroy(sphg_ctx *ctx);
+int sphg_device_bytes(const sphg_ctx *ctx, size_t *bytes);
+
+/* Host <-> device state transfer, one named field at a time. Names are the Particle members
+ * (Class_particle.h:13-103): x xp vx vxp ax axp ux strep divx interf interfss rho rhop mass hl
+ * pre prep e eps deps sig dsig weps adps satu dsatu suct cop vsig veps divq divr meps roue rous
+ * prew prea porosity cd poro_cop beta beta_ts type matype permtype etype; the per-particle
+ * material copies pParti_ConsPara / pVFluid as "ps_<field>" / "pf_<field>" (only ps_fai, ps_c,
+ * ps_cop, ps_ds differ between particles, CPP_17-1_K-L_expansion_non.cpp:575-600); and the
+ * boundary records clVar_Boundary vx_water vx_soil vx_air vx_struct (Class_particle.h:190-207)
+ * and cl_bndy_pair vel_bndy (:168-177). Values are row-major [n][ncomp] (ncomp 1, 3, 6 or 9);
+ * int fields (type matype permtype etype) use the *_i variants. sphg_upload_done() derives the
+ * per-particle material slots and must follow the uploads before the first step. */
+int sphg_set_field(sphg_ctx *ctx, const char *name, const double *host, int n, int ncomp);
+int sphg_get_field(sphg_ctx *ctx, const char *name, double *host, int n, int ncomp);
+int sphg_set_field_i(sphg_ctx *ctx, const char *name, const int *host, int n);
+int sphg_get_field_i(sphg_ctx *ctx, const char *name, int *host, int n);
+int sphg_upload_done(sphg_ctx *ctx, int n);
+
+/* Host-side initial state: setting_vis_cons + setting_par (CPP_2-initialf.cpp:1021-1380) on host
+ * arrays (row-major, n particles): fills rho rhop mass hl satu e porosity axp cd roue sig pre prep
+ * suct and the per-particle material copies; pro->nboun etc. are not needed. Reads xp, type,
+ * matype, etype. Runs on the host only (no device needed). */
+int sphg_setting_par(const sphg_problem *pro, const sphg_materials *mat, int n,
+                     const double *xp, const int *type, const int *matype, const int *etype,
+                     double *rho, double *rhop, double *mass, double *hl, double *satu, double *e,
+                     double *porosity, double *axp, double *cd, double *roue, double *sig,
+                     double *pre, double *prep, double *suct);
+
+/* ---- step functions: same order and semantics as the reference loop body ---------------- */
+
+/* clNNPS_Fun::parttocell (CPP_3-NNPS.cpp:175-263) incl. Binary_Sort (:597-630). */
+int sphg_parttocell(sphg_ctx *ctx);
+/* clNNPS_Fun::partisearching with weight() (CPP_3-NNPS.cpp:265-374, :17-48). */
+int sphg_partisearching(sphg_ctx *ctx);
+/* clDensity_Fun::density (CPP_6-density.cpp:11-189); normalisation is Jacobi (see DESIGN). */
+int sphg_density(sphg_ctx *ctx);
+/* clStraStre_Fun::strain_noreg (CPP_7-1-strain.cpp:20-220) fused with the pair sums of eos loop 2
+ * (CPP_7-1-eos_CR.cpp:45-177) and of free_surface_check (CPP_7-2-constitutive.cpp:453-480);
+ * eos loop 1 (:20-43) runs at the end of sphg_density. */
+int sphg_strain(sphg_ctx *ctx);
+/* clStraStre_Fun::strain_to_stress (CPP_7-2-constitutive.cpp:13-321) + free-surface stress
+ * correction (:481-505) + Stress_eigValue (CPP_7-1-strain.cpp:283-332). */
+int sphg_strain_to_stress(sphg_ctx *ctx);
+/* clAcce_Fun fluid/soil/structure accelerations (CPP_9-4-derivation_stress.cpp:12-505),
+ * artificial_viscosity (CPP_9-3-artificialvis.cpp:12-99), boundary stress effect
+ * (CPP_9-4-derivation_stress.cpp:631-721), interact1 (CPP_10-interact1.cpp:15-110),
+ * soil_damping_geostress (:850-897) and veocity_update (CPP_11-xsph.cpp:11-142). */
+int sphg_accel(sphg_ctx *ctx);
+/* clVel_Fun::xsph (CPP_11-xsph.cpp:145-256) + velocity_set_domain (CPP_13-boundary.cpp:71-122)
+ * + tm_boundary_2d/3d first loop (CPP_13-boundary.cpp:179-283, :388-520). */
+int sphg_xsph(sphg_ctx *ctx);
+/* tm_boundary_2d/3d second loop (CPP_13-boundary.cpp:284-386) + check_domain (:20-68) +
+ * clUpdate_Fun::update (CPP_12-1-update.cpp:11-55). */
+int sphg_update(sphg_ctx *ctx);
+
+/* One whole reference step (CPP_1-SPHLoops.cpp:478-760) nsteps times; advances pro.l. The step
+ * sequence is captured once into a hipGraph and replayed. Returns the sticky error word. */
+int sphg_step(sphg_ctx *ctx, int nsteps);
+int sphg_sync(sphg_ctx *ctx);
+int sphg_current_step(const sphg_ctx *ctx);
+/* Sets the step counter pPPro->l for the next step (CPP_1-SPHLoops.cpp:478). */
+int sphg_set_step(sphg_ctx *ctx, int l);
+
+/* Parity tap: neighbour lists in reference order (Par_Cell::ninflu / influ[1..], Class_NNPS.h:65-77).
+ * counts[n]; ids[n][SPHG_MAX_NB] (entries past counts[i] are -1). Counts above 99 are reported
+ * as the true count (the reference keeps counting past the cap before raising error 5). */
+int sphg_get_neighbors(sphg_ctx *ctx, int *counts, int *ids, int n);
+
+/* Per-kernel timing of the last sphg_step call set (HIP events on the context stream):
+ * fills up to cap (name, total ms, launches) rows, returns the row count. */
+int sphg_kernel_times(sphg_ctx *ctx, char (*names)[32], double *ms, int *launches, int cap);
+int sphg_enable_timing(sphg_ctx *ctx, int on);
+
+const char *sphg_error_string(int code);
+const char *sphg_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHG_H_INCLUDED */

@@ -1,0 +1,331 @@
+"""ctypes view of include/sphg.h: parameter structs, the product library and a thin context wrapper.
+
+The structs mirror the reference parameter classes (Class_Problem.h, Class_ParticleProperty.h)
+field for field; ``problem_from_deck``/``materials_from_deck`` fill them the way the reference's
+deck reader does (CPP_2-initialf.cpp:22-1018).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .deck import Deck, SOIL_FIELDS, FLUID_FIELDS, STRUCT_FIELDS
+
+_D = C.c_double
+_I = C.c_int
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libsphg.so")
+MAX_NB = 99
+
+
+class Problem(C.Structure):
+    _fields_ = ([(n, _I) for n in "fla fls flr fli flv fleob flts flspv flknl ntd flbndy".split()] +
+                [(n, _I) for n in "ndim fop inip loop ntotal step_regu dttype".split()] +
+                [("dr", _D), ("dt", _D), ("tt", _D), ("xsph", _D * 3), ("art_vis", (_D * 2) * 3),
+                 ("art_stre_coe", _D), ("threshold_min", _D), ("threshold_max", _D),
+                 ("gx", _D), ("gy", _D), ("gz", _D), ("if_cal", _I), ("if_move", _I),
+                 ("coe_bndy", _D), ("move_vx", _D * 3), ("interc", _D), ("permtype", _I),
+                 ("pers", _D), ("perl", _D), ("perm", _D), ("ss_type", _I), ("ss_myu", _D),
+                 ("ss_zeta", _D), ("ss_acc_type", _I)] +
+                [(n, _I) for n in "flagem flagsb flagdp flagum flagam flagbui flagpl flagve flagfocc flagep".split()] +
+                [("alphats", _D)] +
+                [(n, _D) for n in "xmin xmax ymin ymax zmin zmax".split()] +
+                [("soil_min", _D * 3), ("soil_max", _D * 3), ("water_min", _D * 3), ("water_max", _D * 3)])
+
+
+class Soil(C.Structure):
+    _fields_ = [(n, _D) for n in SOIL_FIELDS]
+
+
+class Fluid(C.Structure):
+    _fields_ = [(n, _D) for n in "vis vpu dens paa0 porosity fai c".split()] + \
+               [(n, _I) for n in "bhtype press_type acc_type".split()] + \
+               [("pre_coe", _D), ("strain_min", _D)]
+
+
+class Structure(C.Structure):
+    _fields_ = [(n, _D) for n in STRUCT_FIELDS]
+
+
+class Materials(C.Structure):
+    _fields_ = [("soil", Soil * 30), ("water", Fluid * 3), ("air", Fluid), ("structure", Structure)]
+
+
+def problem_from_deck(d: Deck) -> Problem:
+    p = Problem()
+    for k in "fla fls flr fli flv fleob flts flspv flknl ntd flbndy ndim fop inip loop step_regu dttype".split():
+        setattr(p, k, int(getattr(d, k)))
+    p.ntotal = d.n
+    p.dr = d.dr
+    p.dt = d.dt
+    p.tt = d.dt * (d.loop + 1)  # CPP_2-initialf.cpp:277
+    for i in range(3):
+        p.xsph[i] = d.xsph[i]
+        for j in range(2):
+            p.art_vis[i][j] = d.art_vis[i][j]
+    p.art_stre_coe = d.art_stre_coe
+    p.threshold_min, p.threshold_max = d.threshold_min, d.threshold_max
+    p.gx, p.gy, p.gz = d.gravity
+    p.if_cal, p.if_move, p.coe_bndy = d.if_cal, d.if_move, d.coe_bndy
+    for i in range(3):
+        p.move_vx[i] = d.move_vx[i]
+    p.interc, p.permtype, p.pers, p.perl, p.perm = d.interc, d.permtype, d.pers, d.perl, d.perm
+    p.ss_type, p.ss_myu, p.ss_zeta, p.ss_acc_type = d.ss_type, d.ss_myu, d.ss_zeta, d.ss_acc_type
+    for name, v in zip("flagem flagsb flagdp flagum flagam flagbui flagpl flagve flagfocc flagep".split(),
+                       d.model_flags):
+        setattr(p, name, int(v) if d.fls == 1 else 0)
+    p.alphats = d.alphats
+    bmin, bmax, ph = d.domain()
+    p.xmin, p.ymin, p.zmin = bmin
+    p.xmax, p.ymax, p.zmax = bmax
+    for i in range(3):
+        p.soil_min[i], p.soil_max[i] = ph["soil"][0][i], ph["soil"][1][i]
+        p.water_min[i], p.water_max[i] = ph["water"][0][i], ph["water"][1][i]
+    return p
+
+
+def materials_from_deck(d: Deck) -> Materials:
+    m = Materials()
+    for i, s in enumerate(d.soil_table()):
+        for k in SOIL_FIELDS:
+            setattr(m.soil[i], k, s[k])
+    water, air = d.fluid_tables()
+    for dst, src in list(zip(m.water, water)) + [(m.air, air)]:
+        for k in FLUID_FIELDS:
+            setattr(dst, k, int(src[k]) if k in ("bhtype", "press_type", "acc_type") else float(src[k]))
+    s = d.structure_table()
+    for k in STRUCT_FIELDS:
+        setattr(m.structure, k, s[k])
+    return m
+
+
+# Particle member layout used by both the product and the oracle (ncomp per field)
+PARTICLE_FIELDS = {
+    "x": 3, "xp": 3, "vx": 3, "vxp": 3, "ax": 3, "axp": 3, "ux": 3, "strep": 3, "divx": 9, "interf": 3,
+    "interfss": 3, "rho": 1, "rhop": 1, "mass": 1, "hl": 1, "pre": 1, "prep": 1, "e": 1, "eps": 6,
+    "deps": 6, "sig": 6, "dsig": 6, "weps": 9, "adps": 6, "satu": 1, "dsatu": 1, "suct": 1, "cop": 1,
+    "vsig": 1, "veps": 1, "divq": 1, "divr": 1, "meps": 1, "roue": 1, "rous": 1, "prew": 1, "prea": 1,
+    "porosity": 1, "cd": 1, "poro_cop": 1, "beta": 9, "beta_ts": 9,
+}
+INT_FIELDS = ("type", "matype", "permtype", "etype")
+# fields every step reads or writes (what must be uploaded for a restart)
+STATE_FIELDS = ["xp", "vxp", "vx", "ax", "axp", "ux", "interf", "interfss", "rho", "rhop", "mass", "hl",
+                "pre", "prep", "e", "eps", "sig", "adps", "satu", "dsatu", "suct", "cop", "vsig", "veps",
+                "divq", "divr", "meps", "roue", "prew", "prea", "porosity", "cd", "poro_cop", "x"]
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class _Lib:
+    """Binds one shared library exporting the context API under a prefix (sphg_ or or_)."""
+
+    def __init__(self, path: str, prefix: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built")
+        self.lib = C.CDLL(path)
+        self.prefix = prefix
+        L = self.lib
+        for nm in ("set_field", "get_field"):
+            f = getattr(L, prefix + nm)
+            f.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), _I, _I]
+            f.restype = _I
+        for nm in ("set_field_i", "get_field_i"):
+            f = getattr(L, prefix + nm)
+            f.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), _I]
+            f.restype = _I
+        f = getattr(L, prefix + "get_neighbors")
+        f.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _I]
+        f.restype = _I
+        f = getattr(L, prefix + "step")
+        f.argtypes = [C.c_void_p, _I]
+        f.restype = _I
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+
+class Sim:
+    """A context of either library with the same field-level interface."""
+
+    def __init__(self, lib: _Lib, ctx, n: int):
+        self.L = lib
+        self.ctx = ctx
+        self.n = n
+
+    def set(self, name: str, arr) -> None:
+        if name in INT_FIELDS:
+            a = np.ascontiguousarray(arr, dtype=np.int32)
+            rc = self.L.fn("set_field_i")(self.ctx, name.encode(), _iptr(a), self.n)
+        else:
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            nc = 1 if a.ndim == 1 else a.shape[1]
+            rc = self.L.fn("set_field")(self.ctx, name.encode(), _dptr(a), self.n, nc)
+        if rc:
+            raise RuntimeError(f"set_field {name}: error {rc}")
+
+    def get(self, name: str, ncomp: Optional[int] = None) -> np.ndarray:
+        if name in INT_FIELDS:
+            a = np.zeros(self.n, np.int32)
+            rc = self.L.fn("get_field_i")(self.ctx, name.encode(), _iptr(a), self.n)
+        else:
+            nc = ncomp or PARTICLE_FIELDS.get(name, 3 if name.startswith("vx_") or name == "vel_bndy" else 1)
+            a = np.zeros((self.n, nc) if nc > 1 else self.n, np.float64)
+            rc = self.L.fn("get_field")(self.ctx, name.encode(), _dptr(a), self.n, nc)
+        if rc:
+            raise RuntimeError(f"get_field {name}: error {rc}")
+        return a
+
+    def load_state(self, state: Dict[str, np.ndarray], names=None) -> None:
+        for k in INT_FIELDS:
+            self.set(k, state[k])
+        for k in (names or list(PARTICLE_FIELDS)):
+            if k in state:
+                self.set(k, state[k])
+        for k, v in state.items():
+            if k.startswith("ps_") and k in ("ps_fai", "ps_c", "ps_cop", "ps_ds"):
+                self.set(k, v)
+
+    def step(self, nsteps: int = 1) -> int:
+        return int(self.L.fn("step")(self.ctx, nsteps))
+
+    def neighbors(self):
+        cnt = np.zeros(self.n, np.int32)
+        ids = np.zeros((self.n, MAX_NB), np.int32)
+        rc = self.L.fn("get_neighbors")(self.ctx, _iptr(cnt), _iptr(ids), self.n)
+        if rc:
+            raise RuntimeError(f"get_neighbors: error {rc}")
+        return cnt, ids
+
+
+# ---------------------------------------------------------------------- the product library
+_product: Optional[_Lib] = None
+
+
+def product_lib() -> _Lib:
+    """Load the HIP library built in-tree; raises if it is missing (no fallback path exists)."""
+    global _product
+    if _product is None:
+        lib = _Lib(LIB_PATH, "sphg_")
+        L = lib.lib
+        L.sphg_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Problem), C.POINTER(Materials), _I, _I]
+        L.sphg_create.restype = _I
+        L.sphg_destroy.argtypes = [C.c_void_p]
+        L.sphg_upload_done.argtypes = [C.c_void_p, _I]
+        L.sphg_upload_done.restype = _I
+        L.sphg_sync.argtypes = [C.c_void_p]
+        L.sphg_sync.restype = _I
+        L.sphg_set_step.argtypes = [C.c_void_p, _I]
+        L.sphg_set_step.restype = _I
+        L.sphg_error_string.restype = C.c_char_p
+        L.sphg_build_info.restype = C.c_char_p
+        L.sphg_enable_timing.argtypes = [C.c_void_p, _I]
+        L.sphg_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), _I]
+        L.sphg_kernel_times.restype = _I
+        L.sphg_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+        L.sphg_setting_par.argtypes = [C.POINTER(Problem), C.POINTER(Materials), _I] + \
+            [C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)] + \
+            [C.POINTER(C.c_double)] * 14
+        L.sphg_setting_par.restype = _I
+        for nm in ("parttocell", "partisearching", "density", "strain", "strain_to_stress", "accel", "xsph",
+                   "update"):
+            f = getattr(L, "sphg_" + nm)
+            f.argtypes = [C.c_void_p]
+            f.restype = _I
+        _product = lib
+    return _product
+
+
+class GpuSim(Sim):
+    """One simulation resident on the GPU (sphg_create ... sphg_destroy)."""
+
+    def __init__(self, deck: Deck, precision: int = 0, capacity: Optional[int] = None):
+        lib = product_lib()
+        self._pro = problem_from_deck(deck)
+        self._mat = materials_from_deck(deck)
+        ctx = C.c_void_p()
+        rc = lib.lib.sphg_create(C.byref(ctx), C.byref(self._pro), C.byref(self._mat), capacity or deck.n,
+                                 precision)
+        if rc:
+            raise RuntimeError(f"sphg_create: error {rc} ({lib.lib.sphg_error_string(rc).decode()})")
+        super().__init__(lib, ctx, deck.n)
+
+    def upload_done(self):
+        rc = self.L.lib.sphg_upload_done(self.ctx, self.n)
+        if rc:
+            raise RuntimeError(f"sphg_upload_done: error {rc}")
+
+    def sync(self) -> int:
+        return int(self.L.lib.sphg_sync(self.ctx))
+
+    def set_step(self, l: int):
+        self.L.lib.sphg_set_step(self.ctx, l)
+
+    def call(self, name: str) -> int:
+        return int(getattr(self.L.lib, "sphg_" + name)(self.ctx))
+
+    def enable_timing(self, on=True):
+        self.L.lib.sphg_enable_timing(self.ctx, 1 if on else 0)
+
+    def kernel_times(self):
+        cap = 64
+        names = (C.c_char * 32 * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_int * cap)()
+        k = self.L.lib.sphg_kernel_times(self.ctx, names, ms, cnt, cap)
+        return [(bytes(names[i]).split(b"\0", 1)[0].decode(), ms[i], cnt[i]) for i in range(k)]
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        self.L.lib.sphg_device_bytes(self.ctx, C.byref(b))
+        return int(b.value)
+
+    def close(self):
+        if self.ctx:
+            self.L.lib.sphg_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def setting_par(deck: Deck) -> Dict[str, np.ndarray]:
+    """Initial particle state via the library's host restatement of setting_vis_cons +
+    setting_par (CPP_2-initialf.cpp:1021-1380). Host only; no GPU needed."""
+    L = product_lib().lib
+    pro = problem_from_deck(deck)
+    mat = materials_from_deck(deck)
+    n = deck.n
+    xp = np.ascontiguousarray(deck.xp, dtype=np.float64)
+    ty = np.ascontiguousarray(deck.type, np.int32)
+    mt = np.ascontiguousarray(deck.matype, np.int32)
+    et = np.ascontiguousarray(deck.etype, np.int32)
+    out = {k: np.zeros(n) for k in ("rho", "rhop", "mass", "hl", "satu", "e", "porosity", "cd", "roue", "pre",
+                                     "prep", "suct")}
+    out["axp"] = np.zeros((n, 3))
+    out["sig"] = np.zeros((n, 6))
+    rc = L.sphg_setting_par(C.byref(pro), C.byref(mat), n, _dptr(xp), _iptr(ty), _iptr(mt), _iptr(et),
+                            _dptr(out["rho"]), _dptr(out["rhop"]), _dptr(out["mass"]), _dptr(out["hl"]),
+                            _dptr(out["satu"]), _dptr(out["e"]), _dptr(out["porosity"]), _dptr(out["axp"]),
+                            _dptr(out["cd"]), _dptr(out["roue"]), _dptr(out["sig"]), _dptr(out["pre"]),
+                            _dptr(out["prep"]), _dptr(out["suct"]))
+    if rc:
+        raise RuntimeError(f"sphg_setting_par: error {rc}")
+    out["xp"] = xp.copy()
+    out["vxp"] = np.ascontiguousarray(deck.vxp, dtype=np.float64).copy()
+    out["type"], out["matype"], out["etype"] = ty.copy(), mt.copy(), et.copy()
+    out["permtype"] = np.ascontiguousarray(deck.permtype_p, np.int32).copy()
+    return out

@@ -1,0 +1,372 @@
+"""Input decks of the multi-phase SPH solver: parameters, materials and particles.
+
+A ``Deck`` holds what the reference reads from ``input.dat`` (grammar: CPP_2-initialf.cpp:22-1018,
+sections FL PR XP AT GF IP BP WP AP MS <model blocks> SM PA) and can write that file, so the
+same seeded deck drives the reference solver and this library. The generators build the
+workloads the benchmarks are quoted on (2D dry granular column, 3D soil-water column, 3D slope
+with pile, 2D random-field slope), each scalable to test sizes.
+"""
+from __future__ import annotations
+
+import copy
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+SOIL_FIELDS = ("e poi eps0 dens damp1 damp2 cop ramda kapa zmf ann bnn beta sitas sitar epsr epss "
+               "sd sw kse c1 c2 c3 fai c ocr sme0 mzm azm r0 bl bz zet vpu porosity fs etao g0 ds "
+               "damp_coe").split()
+FLUID_FIELDS = "vis vpu dens paa0 porosity fai c bhtype press_type acc_type pre_coe strain_min".split()
+STRUCT_FIELDS = "dens eps0 e poi vpu porosity cop".split()
+
+# model index -> input.dat block tag and the per-layer field lines (CPP_2-initialf.cpp:487-795)
+MODEL_BLOCKS = {
+    0: ("EM", [["kapa", "poi", "dens"], ["damp1", "damp2", "cop", "eps0"], ["e", "ds"]]),
+    1: ("SB", [["poi", "dens", "eps0", "zmf", "sme0", "fs"], ["ramda", "kapa", "damp1", "damp2", "cop"],
+               ["ann", "ocr"], ["e", "ds"]]),
+    2: ("DP", [["kapa", "poi", "dens", "c", "fai", "fs"], ["damp1", "damp2", "cop", "eps0"], ["e", "ds"]]),
+    3: ("UM", [["poi", "dens", "eps0", "zmf", "sme0", "fs"], ["ramda", "kapa", "damp1", "damp2", "cop"],
+               ["ann", "bnn", "beta", "epss", "epsr", "sitas", "sitar"],
+               ["sd", "sw", "kse", "c1", "c2", "c3"], ["e", "ds"]]),
+    4: ("AM", [["poi", "dens", "eps0", "zmf", "sme0", "fs"], ["ramda", "kapa", "damp1", "damp2", "cop"],
+               ["mzm", "azm", "r0", "bl", "bz", "zet"], ["e", "ds"]]),
+    5: ("BU", [["kapa", "poi", "dens", "c", "fai", "beta", "fs"], ["damp1", "damp2", "cop", "eps0"], ["e", "ds"]]),
+    6: ("PP", [["kapa", "poi", "dens"], ["damp1", "damp2", "cop", "eps0"], ["e", "ds"]]),
+    7: ("VE", [["kapa", "poi", "dens", "c"], ["damp1", "damp2", "cop", "eps0"], ["e", "ds"]]),
+    8: ("FC", [["dens", "c", "eps0", "cop", "sme0"], ["g0", "poi", "zmf", "ramda", "etao"],
+               ["zet", "ann", "bnn", "damp1", "damp2"], ["ds"]]),
+    9: ("EP", [["kapa", "poi", "dens", "c", "fai", "fs"], ["damp1", "damp2", "cop", "eps0"], ["e", "ds"]]),
+}
+MODEL_FLAG_NAMES = ["flagem", "flagsb", "flagdp", "flagum", "flagam", "flagbui", "flagpl", "flagve",
+                    "flagfocc", "flagep"]
+
+
+def _g(v) -> str:
+    """Shortest round-tripping decimal of a double (the reference parses it back with %lf)."""
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    return repr(float(v))
+
+
+def default_soil() -> Dict[str, float]:
+    """Para_Soil::Para_Soil (Class_ParticleProperty.cpp:30-83)."""
+    d = {k: 0.0 for k in SOIL_FIELDS}
+    d["fs"] = 1.0
+    d["ds"] = 1.0
+    return d
+
+
+def default_fluid() -> Dict[str, float]:
+    """Para_Fluid::Para_Fluid (Class_ParticleProperty.cpp:89-106)."""
+    d = {k: 0.0 for k in FLUID_FIELDS}
+    d["strain_min"] = 1.0
+    for k in ("bhtype", "press_type", "acc_type"):
+        d[k] = 0
+    return d
+
+
+@dataclass
+class Deck:
+    ndim: int = 2
+    # Para_FL
+    fla: int = 0
+    fls: int = 1
+    flr: int = 0
+    fli: int = 0
+    flv: int = 0
+    fleob: int = 0
+    flts: int = 0
+    flspv: int = 0
+    flknl: int = 1
+    ntd: int = 1
+    flbndy: int = 2
+    # Para_Pro
+    threshold_min: float = 0.1
+    threshold_max: float = 0.12
+    fop: int = 1
+    inip: int = 0
+    loop: int = 1
+    step_regu: int = 1000000
+    dttype: int = 1
+    dr: float = 0.002
+    dt: float = 1e-6
+    xsph: tuple = (0.5, 0.5, 0.5)
+    art_vis: tuple = ((0.0, 0.0), (0.1, 0.0), (0.0, 0.0))
+    art_stre_coe: float = -1.0
+    gravity: tuple = (0.0, -9.81, 0.0)
+    # Para_Inter / Para_SSInter
+    interc: float = 1.0
+    permtype: int = 0
+    pers: float = 0.0
+    perl: float = 0.0
+    perm: float = 0.0
+    ss_type: int = 0
+    ss_myu: float = 0.0
+    ss_zeta: float = 0.5
+    ss_acc_type: int = 1
+    # Para_Boundary
+    if_cal: int = 0
+    if_move: int = 0
+    coe_bndy: float = 0.1
+    move_vx: tuple = (0.0, 0.0, 0.0)
+    # materials
+    water: List[Dict[str, float]] = field(default_factory=lambda: [default_fluid() for _ in range(3)])
+    air: Dict[str, float] = field(default_factory=default_fluid)
+    model_flags: List[int] = field(default_factory=lambda: [0] * 10)
+    alphats: float = 0.0
+    damp_coe: float = 0.0
+    soil_layers: Dict[int, List[Dict[str, float]]] = field(default_factory=dict)
+    structure: Dict[str, float] = field(default_factory=lambda: {k: 0.0 for k in STRUCT_FIELDS})
+    # particles
+    xp: np.ndarray = None            # (n, 3) float64
+    vxp: np.ndarray = None           # (n, 3) float64
+    type: np.ndarray = None          # (n,) int32
+    matype: np.ndarray = None
+    permtype_p: np.ndarray = None
+    etype: np.ndarray = None
+    name: str = "deck"
+
+    @property
+    def n(self) -> int:
+        return int(self.xp.shape[0])
+
+    def copy(self) -> "Deck":
+        return copy.deepcopy(self)
+
+    # ------------------------------------------------------------------ input.dat writer
+    def write_input_dat(self, path: str) -> None:
+        """Write the reference deck file (grammar of CPP_2-initialf.cpp:22-1018).
+
+        Every line has at least three characters: the section scanners use "%c%c %*[^\\n]%*c",
+        which swallows the following line after a one-character line."""
+        L: List[str] = []
+
+        def sec(tag, rows):
+            L.append(f"{tag} section")
+            L.append("A data")
+            for r in rows:
+                L.append(" ".join(_g(v) for v in r))
+            L.append("E end")
+
+        sec("FL", [[self.fla, self.fls, self.flr, self.fli, self.flv, self.fleob, self.flts, self.flspv,
+                    self.flknl], [self.ntd, self.flbndy]])
+        sec("PR", [[self.ndim, self.threshold_min, self.threshold_max],
+                   [self.fop, self.inip, self.loop, self.n, self.step_regu],
+                   [self.dttype, self.dr, self.dt]])
+        sec("XP", [list(self.xsph)])
+        sec("AT", [list(self.art_vis[0]), [self.art_vis[1][0], self.art_vis[1][1], self.art_stre_coe],
+                   list(self.art_vis[2])])
+        sec("GF", [list(self.gravity)])
+        sec("IP", [[self.interc], [self.permtype, self.pers, self.perl, self.perm],
+                   [self.ss_type, self.ss_myu, self.ss_zeta, self.ss_acc_type]])
+        brow = [self.if_cal, self.if_move, self.coe_bndy]
+        rows = [brow] + ([list(self.move_vx)] if self.if_move >= 1 else [])
+        sec("BP", rows)
+        w0 = self.water[0]
+        sec("WP", [[w0["dens"], w0["vis"], w0["fai"], w0["c"], w0["vpu"], w0["press_type"], w0["acc_type"],
+                    w0["bhtype"], w0["pre_coe"], w0["strain_min"]]] +
+            [[w["dens"], w["vis"], w["fai"], w["c"], w["vpu"]] for w in self.water[1:]])
+        if self.fla == 1:
+            a = self.air
+            sec("AP", [[a["dens"], a["paa0"], a["vis"], a["vpu"], a["press_type"], a["acc_type"], a["pre_coe"]]])
+        if self.fls == 1:
+            ms = list(self.model_flags)
+            rows = [ms] + ([[self.alphats, self.damp_coe]] if self.flts >= 1 else [])
+            sec("MS", rows)
+            for model, layers in sorted(self.soil_layers.items()):
+                tag, lines = MODEL_BLOCKS[model]
+                rows = []
+                for lay in layers:
+                    for ln in lines:
+                        rows.append([lay[k] for k in ln])
+                sec(tag, rows)
+        s = self.structure
+        sec("SM", [[s["dens"], s["eps0"], s["cop"]], [s["e"], s["poi"]]])
+        rows = []
+        for i in range(self.n):
+            if self.ndim == 2:
+                rows.append([self.xp[i, 0], self.xp[i, 1], self.vxp[i, 0], self.vxp[i, 1],
+                             int(self.type[i]), int(self.matype[i]), int(self.permtype_p[i]), int(self.etype[i])])
+            else:
+                rows.append([self.xp[i, 0], self.xp[i, 1], self.xp[i, 2], self.vxp[i, 0], self.vxp[i, 1],
+                             self.vxp[i, 2], int(self.type[i]), int(self.matype[i]), int(self.permtype_p[i]),
+                             int(self.etype[i])])
+        sec("PA", rows)
+        with open(path, "w") as f:
+            f.write("\n".join(L) + "\n")
+
+    # ------------------------------------------------------------------ material tables
+    def soil_table(self) -> List[Dict[str, float]]:
+        """pSoil[30] after initial_mat (CPP_2-initialf.cpp:487-852): block values, vpu=600
+        (:797-802), damp_coe, porosity = eps0/(1+eps0) when fls==1, FOCC e=5e6 (:738)."""
+        tab = [default_soil() for _ in range(30)]
+        if self.fls == 1:
+            for model, layers in self.soil_layers.items():
+                for lay_i, lay in enumerate(layers[: self.model_flags[model]]):
+                    d = tab[lay_i * 10 + model]
+                    for ln in MODEL_BLOCKS[model][1]:
+                        for k in ln:
+                            d[k] = float(lay[k])
+                    if model == 8:
+                        d["e"] = 5.0e6
+        for d in tab:
+            d["vpu"] = 600.0
+            d["damp_coe"] = float(self.damp_coe)
+        if self.fls == 1:
+            for d in tab:
+                d["porosity"] = d["eps0"] / (1 + d["eps0"])
+        return tab
+
+    def fluid_tables(self):
+        """pWater[3] and pAir after initial_mat (CPP_2-initialf.cpp:410-440, :803-835)."""
+        water = [dict(w) for w in self.water]
+        for k in ("press_type", "bhtype", "pre_coe", "strain_min"):
+            water[1][k] = water[0][k]
+            water[2][k] = water[0][k]
+        air = dict(self.air) if self.fla == 1 else default_fluid()
+        soil = self.soil_table()
+        if self.fls == 0:
+            por = 1.0
+        else:
+            por = 0.0
+            for d in soil:
+                por = max(por, d["porosity"])
+        air["porosity"] = por
+        for w in water:
+            w["porosity"] = por
+        for w in water:
+            if w["dens"] <= 0.0:
+                w["dens"] = 1000.0
+        if air["dens"] <= 0.0:
+            air["dens"] = 1.293
+        return water, air
+
+    def structure_table(self) -> Dict[str, float]:
+        s = dict(self.structure)
+        s["vpu"] = 600.0
+        s["porosity"] = s["eps0"] / (1 + s["eps0"])
+        return s
+
+    def domain(self):
+        """Cell_Con box and per-phase extents as input_par computes them (CPP_2-initialf.cpp:937-1017),
+        including the fmax(water_min, x) slip of the phase maxima."""
+        xp = self.xp
+        bmax = np.full(3, -10000.0)
+        bmin = np.full(3, 10000.0)
+        for j in range(3):
+            bmax[j] = max(bmax[j], float(xp[:, j].max()))
+            bmin[j] = min(bmin[j], float(xp[:, j].min()))
+        for j in range(self.ndim):
+            bmax[j] = bmax[j] + 2.0 * self.dr
+            bmin[j] = bmin[j] - 2.0 * self.dr
+        ph = {}
+        for t, nm in ((1, "water"), (2, "soil")):
+            mn = np.full(3, 100000.0)
+            mx = np.full(3, -100000.0)
+            for i in np.nonzero(self.type == t)[0]:
+                for j in range(3):
+                    mn[j] = min(mn[j], xp[i, j])
+                    mx[j] = max(mn[j], xp[i, j])
+            ph[nm] = (mn, mx)
+        return bmin, bmax, ph
+
+
+# ---------------------------------------------------------------------- workload generators
+
+def _jitter(rng, n, ndim, dx):
+    j = np.zeros((n, 3))
+    j[:, :ndim] = rng.uniform(-1e-6 * dx, 1e-6 * dx, size=(n, ndim))
+    return j
+
+
+def _lattice2d(ix0, ix1, iy0, iy1, dx):
+    xs = (np.arange(ix0, ix1) + 0.5) * dx
+    ys = (np.arange(iy0, iy1) + 0.5) * dx
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    p = np.zeros((X.size, 3))
+    p[:, 0] = X.ravel()
+    p[:, 1] = Y.ravel()
+    return p
+
+
+def _lattice3d(ix0, ix1, iy0, iy1, iz0, iz1, dx):
+    xs = (np.arange(ix0, ix1) + 0.5) * dx
+    ys = (np.arange(iy0, iy1) + 0.5) * dx
+    zs = (np.arange(iz0, iz1) + 0.5) * dx
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+
+def _assemble(deck: Deck, parts, rng):
+    xp = np.concatenate([p[0] for p in parts])
+    n = xp.shape[0]
+    deck.xp = xp + _jitter(rng, n, deck.ndim, deck.dr)
+    deck.vxp = np.zeros((n, 3))
+    deck.type = np.concatenate([np.full(p[0].shape[0], p[1], np.int32) for p in parts])
+    deck.matype = np.concatenate([np.full(p[0].shape[0], p[2], np.int32) for p in parts])
+    deck.permtype_p = np.concatenate([np.full(p[0].shape[0], p[3] if len(p) > 3 else 0, np.int32)
+                                      for p in parts])
+    deck.etype = np.ones(n, np.int32)
+    return deck
+
+
+def dp_layer(c, fai, e, poi, dens, eps0=0.0, kapa=0.01, fs=1.0, cop=0.0):
+    d = default_soil()
+    d.update(kapa=kapa, poi=poi, dens=dens, c=c, fai=fai, fs=fs, damp1=0.0, damp2=0.0, cop=cop, eps0=eps0,
+             e=e, ds=1.0)
+    return d
+
+
+def column_collapse_2d(nx=60, ny=120, dx=0.002, wall_h=130, floor_len=180, steps=5000, seed=1,
+                       c_sound=600.0, xsph=0.5) -> Deck:
+    """Plain FP64 2D dry granular column collapse: nx*ny soil lattice at (i+1/2)dx, Drucker-Prager
+    c=0, phi=30, psi=0, E=1.8 MPa, nu=0.3, rho=2040; 3-layer rigid floor (x from -3dx to
+    floor_len*dx) and left wall (y up to wall_h*dx); AV alpha=0.1; dt = 0.2 dx / c_sound;
+    seeded +-1e-6 dx jitter. Defaults give 7,200 soil + 939 wall = 8,139 particles."""
+    rng = np.random.default_rng(seed)
+    d = Deck(ndim=2, dr=dx, dt=0.2 * dx / c_sound, loop=steps, fop=steps, gravity=(0.0, -9.81, 0.0),
+             xsph=(xsph, xsph, xsph), art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), flbndy=2,
+             name=f"column2d_{nx}x{ny}")
+    d.model_flags[2] = 1
+    d.soil_layers[2] = [dp_layer(c=0.0, fai=30.0, e=1.8e6, poi=0.3, dens=2040.0)]
+    soil = _lattice2d(0, nx, 0, ny, dx)
+    floor = _lattice2d(-3, floor_len, -3, 0, dx)
+    wall = _lattice2d(-3, 0, 0, wall_h, dx)
+    return _assemble(d, [(soil, 2, 2), (floor, 0, 0), (wall, 0, 0)], rng)
+
+
+def _box_walls_3d(nx, ny, nz_wall, dx, layers=3):
+    """Floor under [0,nx)x[0,ny) plus four side walls up to nz_wall, `layers` thick."""
+    floor = _lattice3d(-layers, nx + layers, -layers, ny + layers, -layers, 0, dx)
+    xm = _lattice3d(-layers, 0, -layers, ny + layers, 0, nz_wall, dx)
+    xp_ = _lattice3d(nx, nx + layers, -layers, ny + layers, 0, nz_wall, dx)
+    ym = _lattice3d(0, nx, -layers, 0, 0, nz_wall, dx)
+    yp = _lattice3d(0, nx, ny, ny + layers, 0, nz_wall, dx)
+    return np.concatenate([floor, xm, xp_, ym, yp])
+
+
+def soil_water_column_3d(nx=64, ny=32, nz_soil=64, nz_water=16, dx=0.005, steps=2000, seed=2,
+                         c_water=30.0, eps0=0.6, perm=1e-4) -> Deck:
+    """3D saturated soil-water column: nx*ny*nz_soil Drucker-Prager soil (c=1 kPa, phi=25) under an
+    nx*ny*nz_water water layer; walls on four sides and a floor; Darcy drag (fli=1, permtype=1,
+    pers=perm); water EOS with vpu=c_water. dt = 0.2 dx / 600."""
+    rng = np.random.default_rng(seed)
+    d = Deck(ndim=3, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=steps, gravity=(0.0, 0.0, -9.81),
+             fli=1, permtype=1, pers=perm, perl=perm, perm=perm, interc=1.0, flbndy=2,
+             art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), xsph=(0.5, 0.5, 0.5),
+             name=f"soilwater3d_{nx}x{ny}x{nz_soil}+{nz_water}")
+    d.model_flags[2] = 1
+    d.soil_layers[2] = [dp_layer(c=1000.0, fai=25.0, e=1.8e6, poi=0.3, dens=2650.0, eps0=eps0, cop=perm)]
+    w = default_fluid()
+    por = eps0 / (1 + eps0)
+    # Tait stiffness B = rho0 c^2 / 7 carried by the initial pressure prep (CPP_7-1-eos_CR.cpp:28-29)
+    B = 1000.0 * c_water ** 2 / 7.0
+    w.update(dens=1000.0, vis=1e-3, vpu=c_water, press_type=0, acc_type=0, bhtype=0,
+             pre_coe=B / (por * 1000.0 * 9.81 * dx), strain_min=1.0)
+    d.water = [dict(w), dict(w), dict(w)]
+    soil = _lattice3d(0, nx, 0, ny, 0, nz_soil, dx)
+    water = _lattice3d(0, nx, 0, ny, nz_soil, nz_soil + nz_water, dx)
+    walls = _box_walls_3d(nx, ny, nz_soil + nz_water + 4, dx)
+    return _assemble(d, [(soil, 2, 2), (water, 1, 0), (walls, 0, 0)], rng)

@@ -1,0 +1,1690 @@
+/*
+ * Test infrastructure only — the oracle (see sph_oracle.h). A serial/OpenMP FP64 restatement
+ * of the reference step loop; every function names the reference lines it restates. Quirks of
+ * the reference are reproduced on purpose and marked "quirk".
+ *
+ * Parity status of each function is pinned in tests/test_oracle_ref.py against full-step dumps
+ * of the reference link harness (oracle/_ref) on seeded decks: bit-equal at one thread in
+ * OR_DENSITY_GS mode for every function exercised by the 2D column and 3D soil-water decks.
+ */
+#include "sph_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NBMAX 100
+
+typedef struct {
+    double x[3], xp[3], vx[3], vxp[3], ax[3], axp[3], ux[3], strep[3];
+    double divx[3][3], interf[3], interfss[3];
+    double rho, rhop, mass, hl, pre, prep, e;
+    double eps[6], deps[6], sig[6], dsig[6], weps[3][3], adps[6];
+    double satu, dsatu, suct, cop, vsig, veps, divq, divr, meps, roue, rous, prew, prea;
+    double porosity, cd, poro_cop;
+    double beta[3][3], beta_ts[3][3];
+    int type, matype, permtype, etype;
+} OPar;
+
+typedef struct {
+    int cell_id, ninflu;
+    int influ[NBMAX];
+    double wij[NBMAX][4];
+} OCell;
+
+typedef struct {
+    double vx_water[3], vx_soil[3], vx_air[3], vx_struct[3];
+} OVariB;
+
+struct or_ctx {
+    sphg_problem pro;
+    sphg_materials mat;
+    int n;
+    OPar *p;
+    OCell *pc;
+    sphg_soil *ps;   /* pParti_ConsPara */
+    sphg_fluid *pf;  /* pVFluid */
+    OVariB *vb;      /* pParti_VariBndy */
+    double (*vel_bndy)[3];
+    double (*vx_temp)[3];
+    double *rho_tmp; /* Jacobi normalisation input */
+    /* cell list */
+    int cxm, cym, czm, ctotal;
+    int *cstart, *cend; /* cell_info[ctotal+4] */
+    int (*nncell)[27];  /* cell_link[ctotal+4] */
+    int size_m, klog;
+    int *s_cell, *s_id, *t_cell, *t_id; /* parti_cellid sorted/temp [size_m] */
+    int l;
+    double t;
+    int density_mode, nthreads;
+};
+
+/* ------------------------------------------------------------------ kernel function */
+/* weight(): cubic B-spline value and gradient (CPP_3-NNPS.cpp:17-48); the truncated
+ * 0.66666667 is the reference's constant. */
+static void or_weight(double dst, double ra, const double dx[], double *w, int ndim) {
+    double a1 = 0.0, s = dst / ra;
+    int k;
+    if (ndim == 2) {
+        a1 = 0.682092613250980 / ra / ra;
+        w[ndim] = 0.0;
+    }
+    if (ndim == 3) a1 = 0.477464829275686 / ra / ra / ra;
+    if (s == 0.0) {
+        for (k = 0; k < ndim; k++) w[k] = 0.0;
+        w[3] = a1 * (0.66666667 - s * s + 0.5 * s * s * s);
+    }
+    if (s > 0.0 && s <= 1.0) {
+        for (k = 0; k < ndim; k++) w[k] = a1 * dx[k] * (-2 * s + 1.5 * s * s) / (ra * dst);
+        w[3] = a1 * (0.66666667 - s * s + 0.5 * s * s * s);
+    } else if (s > 1.0 && s <= 2.0) {
+        for (k = 0; k < ndim; k++) w[k] = a1 * (-0.5) * dx[k] * (2.0 - s) * (2.0 - s) / (ra * dst);
+        w[3] = a1 * (2.0 - s) * (2.0 - s) * (2.0 - s) / 6.0;
+    }
+}
+
+/* weight2(): Yang & Liu kernel (CPP_3-NNPS.cpp:51-82), flknl 2/4. */
+static void or_weight2(double dst, double ra, const double dx[], double *w, int ndim) {
+    double a1 = 0.0, s = dst / ra;
+    int k;
+    if (ndim == 2) {
+        a1 = 0.106103295394597 / (ra * ra);
+        w[ndim] = 0.0;
+    }
+    if (ndim == 3) a1 = 0.077010456334788 / (ra * ra * ra);
+    if (s == 0.0) {
+        for (k = 0; k < ndim; k++) w[k] = 0.0;
+        w[3] = a1 * (6.0 - 6.0 * s + 1.0 * s * s * s);
+    }
+    if (s > 0.0 && s <= 1.0) {
+        for (k = 0; k < ndim; k++) w[k] = a1 * dx[k] * (-6.0 + 3.0 * s * s) / (ra * dst);
+        w[3] = a1 * (6.0 - 6.0 * s + 1.0 * s * s * s);
+    } else if (s > 1.0 && s <= 2.0) {
+        for (k = 0; k < ndim; k++) w[k] = a1 * (-3.0) * dx[k] * (2.0 - s) * (2.0 - s) / (ra * dst);
+        w[3] = a1 * (2.0 - s) * (2.0 - s) * (2.0 - s);
+    }
+}
+
+/* ------------------------------------------------------------------ context */
+static void par_init(OPar *q) { memset(q, 0, sizeof *q); }
+
+static void soil_default(sphg_soil *s) {
+    memset(s, 0, sizeof *s);
+    s->fs = 1.0;
+    s->ds = 1.0;
+}
+static void fluid_default(sphg_fluid *f) {
+    memset(f, 0, sizeof *f);
+    f->strain_min = 1.0;
+}
+
+or_ctx *or_create(const sphg_problem *pro, const sphg_materials *mat, int n) {
+    or_ctx *c = (or_ctx *)calloc(1, sizeof *c);
+    int i;
+    c->pro = *pro;
+    c->mat = *mat;
+    c->n = n;
+    c->p = (OPar *)calloc(n, sizeof(OPar));
+    c->pc = (OCell *)calloc(n, sizeof(OCell));
+    c->ps = (sphg_soil *)calloc(n, sizeof(sphg_soil));
+    c->pf = (sphg_fluid *)calloc(n, sizeof(sphg_fluid));
+    c->vb = (OVariB *)calloc(n, sizeof(OVariB));
+    c->vel_bndy = (double(*)[3])calloc(n, sizeof(double[3]));
+    c->vx_temp = (double(*)[3])calloc(n, sizeof(double[3]));
+    c->rho_tmp = (double *)calloc(n, sizeof(double));
+    for (i = 0; i < n; i++) {
+        par_init(&c->p[i]);
+        soil_default(&c->ps[i]);
+        fluid_default(&c->pf[i]);
+        c->pc[i].cell_id = -1;
+    }
+    /* allocation sizes of SPH_Calculation (CPP_1-SPHLoops.cpp:353-356) */
+    c->klog = (int)log2((double)n) + 1;
+    c->size_m = 1 << c->klog;
+    c->s_cell = (int *)malloc(sizeof(int) * c->size_m);
+    c->s_id = (int *)malloc(sizeof(int) * c->size_m);
+    c->t_cell = (int *)malloc(sizeof(int) * c->size_m);
+    c->t_id = (int *)malloc(sizeof(int) * c->size_m);
+    for (i = 0; i < c->size_m; i++) { /* parti_cellid() (Class_NNPS.cpp:48-52) */
+        c->s_cell[i] = c->t_cell[i] = 10000000;
+        c->s_id[i] = c->t_id[i] = 10000000;
+    }
+    c->l = 1;
+    c->t = 0.0;
+    c->density_mode = OR_DENSITY_GS;
+    c->nthreads = 1;
+    return c;
+}
+
+void or_destroy(or_ctx *c) {
+    if (!c) return;
+    free(c->p); free(c->pc); free(c->ps); free(c->pf); free(c->vb); free(c->vel_bndy);
+    free(c->vx_temp); free(c->rho_tmp); free(c->cstart); free(c->cend); free(c->nncell);
+    free(c->s_cell); free(c->s_id); free(c->t_cell); free(c->t_id);
+    free(c);
+}
+
+void or_set_mode(or_ctx *c, int density_mode, int nthreads) {
+    c->density_mode = density_mode;
+    c->nthreads = nthreads > 0 ? nthreads : 1;
+}
+void or_set_step(or_ctx *c, int l) { c->l = l; }
+
+/* field access by Particle member name ------------------------------------------------ */
+typedef struct { const char *name; size_t off; int ncomp; } FieldDesc;
+#define FD(nm, nc) {#nm, offsetof(OPar, nm), nc}
+static const FieldDesc PFIELDS[] = {
+    FD(x, 3), FD(xp, 3), FD(vx, 3), FD(vxp, 3), FD(ax, 3), FD(axp, 3), FD(ux, 3), FD(strep, 3),
+    FD(divx, 9), FD(interf, 3), FD(interfss, 3), FD(rho, 1), FD(rhop, 1), FD(mass, 1), FD(hl, 1),
+    FD(pre, 1), FD(prep, 1), FD(e, 1), FD(eps, 6), FD(deps, 6), FD(sig, 6), FD(dsig, 6),
+    FD(weps, 9), FD(adps, 6), FD(satu, 1), FD(dsatu, 1), FD(suct, 1), FD(cop, 1), FD(vsig, 1),
+    FD(veps, 1), FD(divq, 1), FD(divr, 1), FD(meps, 1), FD(roue, 1), FD(rous, 1), FD(prew, 1),
+    FD(prea, 1), FD(porosity, 1), FD(cd, 1), FD(poro_cop, 1), FD(beta, 9), FD(beta_ts, 9),
+};
+#undef FD
+#define SD(nm) {"ps_" #nm, offsetof(sphg_soil, nm), 1}
+static const FieldDesc SFIELDS[] = {
+    SD(e), SD(poi), SD(eps0), SD(dens), SD(damp1), SD(damp2), SD(cop), SD(ramda), SD(kapa), SD(zmf),
+    SD(ann), SD(bnn), SD(beta), SD(sitas), SD(sitar), SD(epsr), SD(epss), SD(sd), SD(sw), SD(kse),
+    SD(c1), SD(c2), SD(c3), SD(fai), SD(c), SD(ocr), SD(sme0), SD(mzm), SD(azm), SD(r0), SD(bl),
+    SD(bz), SD(zet), SD(vpu), SD(porosity), SD(fs), SD(etao), SD(g0), SD(ds), SD(damp_coe),
+};
+#undef SD
+#define FDF(nm) {"pf_" #nm, offsetof(sphg_fluid, nm), 1}
+static const FieldDesc FFIELDS[] = {
+    FDF(vis), FDF(vpu), FDF(dens), FDF(paa0), FDF(porosity), FDF(fai), FDF(c), FDF(pre_coe), FDF(strain_min),
+};
+#undef FDF
+
+static int field_rw(or_ctx *c, const char *name, double *v, int n, int ncomp, int write) {
+    size_t i, k;
+    if (n != c->n) return SPHG_E_ARG;
+    for (i = 0; i < sizeof PFIELDS / sizeof PFIELDS[0]; i++) {
+        if (strcmp(PFIELDS[i].name, name)) continue;
+        if (ncomp != PFIELDS[i].ncomp) return SPHG_E_ARG;
+        for (k = 0; k < (size_t)n; k++) {
+            double *f = (double *)((char *)&c->p[k] + PFIELDS[i].off);
+            if (write) memcpy(f, v + k * ncomp, sizeof(double) * ncomp);
+            else memcpy(v + k * ncomp, f, sizeof(double) * ncomp);
+        }
+        return 0;
+    }
+    for (i = 0; i < sizeof SFIELDS / sizeof SFIELDS[0]; i++) {
+        if (strcmp(SFIELDS[i].name, name)) continue;
+        if (ncomp != 1) return SPHG_E_ARG;
+        for (k = 0; k < (size_t)n; k++) {
+            double *f = (double *)((char *)&c->ps[k] + SFIELDS[i].off);
+            if (write) *f = v[k];
+            else v[k] = *f;
+        }
+        return 0;
+    }
+    for (i = 0; i < sizeof FFIELDS / sizeof FFIELDS[0]; i++) {
+        if (strcmp(FFIELDS[i].name, name)) continue;
+        if (ncomp != 1) return SPHG_E_ARG;
+        for (k = 0; k < (size_t)n; k++) {
+            double *f = (double *)((char *)&c->pf[k] + FFIELDS[i].off);
+            if (write) *f = v[k];
+            else v[k] = *f;
+        }
+        return 0;
+    }
+    {
+        const char *inm[3] = {"pf_bhtype", "pf_press_type", "pf_acc_type"};
+        for (i = 0; i < 3; i++) {
+            if (strcmp(inm[i], name)) continue;
+            for (k = 0; k < (size_t)n; k++) {
+                int *f = i == 0 ? &c->pf[k].bhtype : i == 1 ? &c->pf[k].press_type : &c->pf[k].acc_type;
+                if (write) *f = (int)v[k];
+                else v[k] = *f;
+            }
+            return 0;
+        }
+    }
+    {
+        struct { const char *nm; size_t off; } vbf[] = {
+            {"vx_water", offsetof(OVariB, vx_water)}, {"vx_soil", offsetof(OVariB, vx_soil)},
+            {"vx_air", offsetof(OVariB, vx_air)}, {"vx_struct", offsetof(OVariB, vx_struct)}};
+        for (i = 0; i < 4; i++) {
+            if (strcmp(vbf[i].nm, name)) continue;
+            if (ncomp != 3) return SPHG_E_ARG;
+            for (k = 0; k < (size_t)n; k++) {
+                double *f = (double *)((char *)&c->vb[k] + vbf[i].off);
+                if (write) memcpy(f, v + 3 * k, 24);
+                else memcpy(v + 3 * k, f, 24);
+            }
+            return 0;
+        }
+    }
+    if (!strcmp(name, "vel_bndy")) {
+        if (ncomp != 3) return SPHG_E_ARG;
+        if (write) memcpy(c->vel_bndy, v, 24 * (size_t)n);
+        else memcpy(v, c->vel_bndy, 24 * (size_t)n);
+        return 0;
+    }
+    return SPHG_E_ARG;
+}
+
+int or_set_field(or_ctx *c, const char *name, const double *v, int n, int ncomp) {
+    return field_rw(c, name, (double *)v, n, ncomp, 1);
+}
+int or_get_field(or_ctx *c, const char *name, double *v, int n, int ncomp) {
+    return field_rw(c, name, v, n, ncomp, 0);
+}
+
+static int *ifield(or_ctx *c, const char *name, int k) {
+    if (!strcmp(name, "type")) return &c->p[k].type;
+    if (!strcmp(name, "matype")) return &c->p[k].matype;
+    if (!strcmp(name, "permtype")) return &c->p[k].permtype;
+    if (!strcmp(name, "etype")) return &c->p[k].etype;
+    return NULL;
+}
+int or_set_field_i(or_ctx *c, const char *name, const int *v, int n) {
+    int k;
+    if (n != c->n || !ifield(c, name, 0)) return SPHG_E_ARG;
+    for (k = 0; k < n; k++) *ifield(c, name, k) = v[k];
+    return 0;
+}
+int or_get_field_i(or_ctx *c, const char *name, int *v, int n) {
+    int k;
+    if (n != c->n || !ifield(c, name, 0)) return SPHG_E_ARG;
+    for (k = 0; k < n; k++) v[k] = *ifield(c, name, k);
+    return 0;
+}
+
+/* celllist_ini1 / celllist_ini2 (CPP_3-NNPS.cpp:96-171). */
+static void or_celllist_ini(or_ctx *c) {
+    const sphg_problem *P = &c->pro;
+    double dr = 2.0 * P->dr;
+    int i, m, p, kx, ky, kz;
+    if (P->ndim == 2) {
+        c->cxm = (int)((P->xmax - P->xmin) / dr + 3.0);
+        c->cym = (int)((P->ymax - P->ymin) / dr + 3.0);
+        c->czm = (int)(1.0);
+    } else {
+        c->cxm = (int)((P->xmax - P->xmin) / dr + 3.0);
+        c->cym = (int)((P->ymax - P->ymin) / dr + 3.0);
+        c->czm = (int)((P->zmax - P->zmin) / dr + 3.0);
+    }
+    c->ctotal = c->cxm * c->cym * c->czm;
+    m = c->ctotal;
+    free(c->cstart); free(c->cend); free(c->nncell);
+    c->cstart = (int *)malloc(sizeof(int) * (m + 4));
+    c->cend = (int *)malloc(sizeof(int) * (m + 4));
+    c->nncell = (int(*)[27])malloc(sizeof(int[27]) * (m + 4));
+    for (i = 0; i < m + 4; i++) { /* cell_info() and cell_link() constructors */
+        c->cstart[i] = 2000000;
+        c->cend[i] = -2000000;
+        for (p = 0; p < 27; p++) c->nncell[i][p] = -1;
+    }
+    for (i = 0; i <= m; i++) {
+        p = 0;
+        if (P->ndim == 2) {
+            for (ky = 1; ky <= 3; ky++)
+                for (kx = 1; kx <= 3; kx++) {
+                    int v = (kx - 2) * c->cym + ky - 2 + i;
+                    c->nncell[i][p++] = (v < 0 || v > m) ? 0 : v;
+                }
+        } else {
+            for (kz = 1; kz <= 3; kz++)
+                for (kx = 1; kx <= 3; kx++)
+                    for (ky = 1; ky <= 3; ky++) {
+                        int v = (ky - 2) * c->cxm * c->czm + (kx - 2) * c->czm + kz - 2 + i;
+                        c->nncell[i][p++] = (v < 0 || v > m) ? 0 : v;
+                    }
+        }
+    }
+}
+
+/* setting_vis_cons (CPP_2-initialf.cpp:1021-1058) material slots, when the per-particle copies
+ * were not uploaded explicitly the caller relies on these. */
+int or_upload_done(or_ctx *c) {
+    or_celllist_ini(c);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ NNPS */
+/* Merge_CellID (CPP_3-NNPS.cpp:561-595): ties take the right half first. */
+static void or_merge(or_ctx *c, int start, int mid, int end) {
+    int i, j, k;
+    for (i = start; i <= end; i++) {
+        c->t_cell[i] = c->s_cell[i];
+        c->t_id[i] = c->s_id[i];
+    }
+    i = start;
+    j = mid + 1;
+    k = start;
+    while (i < mid + 1 && j <= end) {
+        if (c->t_cell[i] < c->t_cell[j]) {
+            c->s_cell[k] = c->t_cell[i]; c->s_id[k++] = c->t_id[i++];
+        } else {
+            c->s_cell[k] = c->t_cell[j]; c->s_id[k++] = c->t_id[j++];
+        }
+    }
+    while (i < mid + 1) { c->s_cell[k] = c->t_cell[i]; c->s_id[k++] = c->t_id[i++]; }
+    while (j <= end) { c->s_cell[k] = c->t_cell[j]; c->s_id[k++] = c->t_id[j++]; }
+}
+
+/* parttocell + Binary_Sort (CPP_3-NNPS.cpp:175-263, :597-630). */
+static int or_parttocell(or_ctx *c) {
+    const sphg_problem *P = &c->pro;
+    int n = c->n, m = c->ctotal, i, j, t = 0;
+    double xmin[3] = {P->xmin, P->ymin, P->zmin}, dr = 2.0 * P->dr;
+    for (i = 0; i < m; i++) {
+        c->cstart[i] = 10000000;
+        c->cend[i] = -10000000;
+    }
+    for (i = 0; i < n; i++) {
+        int nt[3];
+        for (j = 0; j < P->ndim; j++) {
+            double v = fabs(c->p[i].xp[j] - xmin[j]) / dr + 2.0; /* quirk: fabs mirrors (A6) */
+            nt[j] = (int)v;
+            if (nt[j] < 2) nt[j] = 2;
+        }
+        if (P->ndim == 2) c->s_cell[i] = c->cym * (nt[0] - 1) + nt[1];
+        else c->s_cell[i] = c->cxm * c->czm * (nt[1] - 1) + c->czm * (nt[0] - 1) + nt[2];
+        c->s_id[i] = i;
+    }
+    for (i = 0; i < n; i++) {
+        if (c->s_cell[i] > 0 && c->s_cell[i] <= m) c->pc[i].cell_id = c->s_cell[i];
+        else t = 3;
+    }
+    if (t) return t;
+    for (int ik = 1; ik <= c->klog; ik++) {
+        int interval = 1 << ik, groups = c->size_m / interval, g;
+        for (g = 0; g < groups; g++) {
+            int start = g * interval, end = (g + 1) * interval - 1;
+            or_merge(c, start, (start + end) >> 1, end);
+        }
+    }
+    for (i = 0; i < n; i++) {
+        int cid = c->s_cell[i];
+        if (i < c->cstart[cid]) c->cstart[cid] = i;
+        if (i > c->cend[cid]) c->cend[cid] = i;
+    }
+    return 0;
+}
+
+/* partisearching (CPP_3-NNPS.cpp:265-374). */
+static int or_partisearching(or_ctx *c) {
+    const sphg_problem *P = &c->pro;
+    int n = c->n, dim = P->ndim, nst = dim == 2 ? 9 : 27, t = 0, i;
+    int use_w1 = (P->flknl == 1 || P->flknl == 3);
+    for (i = 0; i < n; i++) {
+        c->pc[i].ninflu = 0;
+        memset(c->pc[i].influ, 0, sizeof c->pc[i].influ);
+        memset(c->pc[i].wij, 0, sizeof c->pc[i].wij);
+    }
+#pragma omp parallel for schedule(static) num_threads(c->nthreads) reduction(max : t)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        int nc, nj, np, k;
+        if (pi->type == 7) continue;
+        nc = c->pc[i].cell_id;
+        for (nj = 0; nj < nst; nj++) {
+            int nl = c->nncell[nc][nj];
+            if (nl <= 0) continue;
+            for (np = c->cstart[nl]; np <= c->cend[nl]; np++) {
+                int j = c->s_id[np];
+                double dst = 0.0, dx[3], ra;
+                for (k = 0; k < dim; k++) {
+                    dst = dst + (pi->xp[k] - c->p[j].xp[k]) * (pi->xp[k] - c->p[j].xp[k]);
+                    dx[k] = pi->xp[k] - c->p[j].xp[k];
+                }
+                dst = sqrt(dst);
+                ra = (pi->hl + c->p[j].hl) / 2;
+                if (dst <= 2 * ra && i != j) {
+                    int p = ++c->pc[i].ninflu;
+                    if (p <= 99) {
+                        c->pc[i].influ[p] = j;
+                        if (use_w1) or_weight(dst, ra, dx, c->pc[i].wij[p], dim);
+                        else or_weight2(dst, ra, dx, c->pc[i].wij[p], dim);
+                    } else
+                        t = 5;
+                }
+            }
+        }
+    }
+    return t;
+}
+
+/* neighbours actually stored (ninflu may exceed the 99 slots before error 5 is raised) */
+static inline int nbcount(const OCell *pc) { return pc->ninflu < 99 ? pc->ninflu : 99; }
+
+/* virtual velocity of boundary neighbour j as seen by particle i (CPP_6-density.cpp:86-120;
+ * the same selection is repeated in CPP_7-1-strain.cpp:92-125 and CPP_11-xsph.cpp:187-219). */
+static void bndy_vel(const or_ctx *c, int i, int j, int use_vx, double vxj[3]) {
+    const OPar *pi = &c->p[i];
+    int bt = c->pro.flbndy;
+    vxj[0] = vxj[1] = vxj[2] = 0.0;
+    if (bt == 1) {
+        const double *v = use_vx ? pi->vx : pi->vxp;
+        vxj[0] = v[0]; vxj[1] = v[1]; vxj[2] = v[2];
+    } else if (bt == 2 || bt == 3 || bt == 4 || bt == 13 || bt == 14) {
+        const double *v = NULL;
+        if (pi->type == 1) v = c->vb[j].vx_water;
+        else if (pi->type == 2) v = c->vb[j].vx_soil;
+        else if (pi->type == 3) v = c->vb[j].vx_air;
+        else if (pi->type == 4) v = c->vb[j].vx_struct;
+        if (v) { vxj[0] = v[0]; vxj[1] = v[1]; vxj[2] = v[2]; }
+    }
+}
+
+/* ------------------------------------------------------------------ density (CPP_6-density.cpp:11-189) */
+static void or_density(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, l = c->l, i;
+    double dt = c->pro.dt;
+    /* loop 1: water porosity from soil neighbours (:22-54) */
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double up = 0.0, dn = 0.0;
+        int nc = nbcount(&c->pc[i]), nj;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pi->type != pj->type && pi->type == 1 && pj->type == 2) {
+                up = up + pj->mass * pj->porosity / pj->rho * c->pc[i].wij[nj][3];
+                dn = dn + pj->mass / pj->rho * c->pc[i].wij[nj][3];
+            }
+        }
+        if (pi->type == 1 && fabs(dn) > 1.0e-7) {
+            pi->poro_cop = pi->porosity;
+            pi->porosity = up / dn;
+            if (l == 1) pi->poro_cop = pi->porosity;
+        } else if (pi->type == 1 && fabs(dn) <= 1.0e-7) {
+            pi->poro_cop = pi->porosity;
+        }
+    }
+    /* loop 2: continuity (:56-166) */
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double drhot = 0.0, up[3] = {0, 0, 0}, dn[3] = {0, 0, 0}, vxj[3];
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            const double *w = c->pc[i].wij[nj];
+            if (pi->type == pj->type && pi->type != 7 && pi->type != 0) {
+                if (pi->type == 2 || pi->type == 4) {
+                    for (k = 0; k < ndim; k++) {
+                        up[k] = up[k] + pj->mass * (pi->vxp[k] - pj->vxp[k]) * w[k];
+                        dn[k] = dn[k] + pj->mass * (pi->xp[k] - pj->xp[k]) * w[k];
+                    }
+                }
+                if (pi->type == 1 || pi->type == 3)
+                    for (k = 0; k < ndim; k++) up[k] = up[k] + pj->mass * (pi->vxp[k] - pj->vxp[k]) * w[k];
+            } else if (pi->type >= 1 && pi->type <= 4 && pj->type == 0) {
+                bndy_vel(c, i, j, 0, vxj);
+                if (pi->type == 2 || pi->type == 4) {
+                    for (k = 0; k < ndim; k++) {
+                        up[k] = up[k] + pj->mass * (pi->vxp[k] - vxj[k]) * w[k];
+                        dn[k] = dn[k] + pj->mass * (pi->xp[k] - pj->xp[k]) * w[k];
+                    }
+                }
+                if (pi->type == 1 || pi->type == 3)
+                    for (k = 0; k < ndim; k++) up[k] = up[k] + pj->mass * (pi->vxp[k] - vxj[k]) * w[k];
+            }
+        }
+        if (pi->type != 7) {
+            if (pi->type == 2 || pi->type == 4) {
+                for (k = 0; k < ndim; k++) {
+                    if (fabs(dn[k]) > 1.0e-7) drhot -= up[k] / dn[k];
+                    else drhot += up[k];
+                }
+            }
+            if (pi->type == 1 || pi->type == 3)
+                for (k = 0; k < ndim; k++) drhot += up[k];
+            if (pi->type == 1 && fabs(pi->porosity) > 0.0001)
+                pi->rho = pi->rhop + drhot * dt +
+                          pi->rhop * (pi->porosity - pi->poro_cop) / (1.0 - pi->poro_cop);
+            else
+                pi->rho = pi->rhop + drhot * dt;
+        }
+    }
+    /* loop 3: normalisation over same-type neighbours (:168-188). Serial GS reads densities
+     * already normalised for lower indices (quirk A1); JACOBI reads all pre-normalisation. */
+    if (c->density_mode == OR_DENSITY_JACOBI)
+        for (i = 0; i < n; i++) c->rho_tmp[i] = c->p[i].rho;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double up = 0.0, dn = 0.0;
+        int nc = nbcount(&c->pc[i]), nj;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pi->type == pj->type) {
+                double rj = c->density_mode == OR_DENSITY_JACOBI ? c->rho_tmp[j] : pj->rho;
+                up = up + pj->mass * c->pc[i].wij[nj][3];
+                dn = dn + pj->mass / rj * c->pc[i].wij[nj][3];
+            }
+        }
+        if (pi->type != 7 && fabs(dn) > 1.0e-7) pi->rho = up / dn;
+    }
+}
+
+/* ------------------------------------------------------------------ strain (CPP_7-1-strain.cpp:20-220) */
+static void or_strain(or_ctx *c) {
+    int n = c->n, dim = c->pro.ndim, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double up[9], dn[3], dvx[3], vxj[3];
+        int nc = nbcount(&c->pc[i]), nj, k, a, b;
+        if (c->l == c->pro.inip)
+            for (k = 0; k < 6; k++) pi->eps[k] = 0.0;
+        for (k = 0; k < 6; k++) {
+            pi->deps[k] = 0.0;
+            pi->dsig[k] = 0.0;
+        }
+        for (k = 0; k < 9; k++) up[k] = 0.0;
+        for (a = 0; a < 3; a++) {
+            dn[a] = 0.0;
+            for (b = 0; b < 3; b++) {
+                pi->weps[a][b] = 0.0;
+                pi->divx[a][b] = 0.0;
+            }
+        }
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj], active = 0;
+            const OPar *pj = &c->p[j];
+            const double *w = c->pc[i].wij[nj];
+            if (pi->type == pj->type && pi->type != 7 && pi->type != 0) {
+                for (k = 0; k < 3; k++) dvx[k] = pi->vxp[k] - pj->vxp[k];
+                active = 1;
+            } else if (pi->type >= 1 && pi->type <= 4 && pj->type == 0) {
+                bndy_vel(c, i, j, 0, vxj);
+                for (k = 0; k < 3; k++) dvx[k] = pi->vxp[k] - vxj[k];
+                active = 1;
+            }
+            if (!active) continue;
+            for (a = 0; a < 3; a++)
+                for (b = 0; b < 3; b++) up[3 * a + b] = up[3 * a + b] + pj->mass / pj->rho * dvx[a] * w[b];
+            if (pi->type == 2 || pi->type == 4)
+                for (k = 0; k < 3; k++) dn[k] = dn[k] + pj->mass / pj->rho * (pi->xp[k] - pj->xp[k]) * w[k];
+        }
+        if (pi->type == 1 || pi->type == 3) {
+            for (a = 0; a < 3; a++)
+                for (b = 0; b < 3; b++) pi->divx[a][b] = up[3 * a + b];
+        } else {
+            for (b = 0; b < 3; b++)
+                if (fabs(dn[b]) > 1.0e-7)
+                    for (a = 0; a < 3; a++) pi->divx[a][b] = up[3 * a + b] / dn[b];
+        }
+        pi->deps[0] = pi->divx[0][0];
+        pi->deps[1] = pi->divx[1][1];
+        pi->deps[2] = pi->divx[2][2];
+        pi->deps[3] = pi->divx[0][1] + pi->divx[1][0];
+        pi->deps[4] = pi->divx[1][2] + pi->divx[2][1];
+        pi->deps[5] = pi->divx[2][0] + pi->divx[0][2];
+        if (dim == 2) pi->deps[2] = pi->deps[4] = pi->deps[5] = 0.0;
+        for (a = 0; a < 3; a++)
+            for (b = 0; b < 3; b++)
+                pi->weps[a][b] = a == b ? 0.0 : 0.5 * (pi->divx[a][b] - pi->divx[b][a]);
+        if (dim == 2) {
+            pi->weps[0][2] = pi->weps[1][2] = pi->weps[2][0] = pi->weps[2][1] = pi->weps[2][2] = 0.0;
+        }
+    }
+}
+
+/* cal_permeability (CPP_7-2-constitutive.cpp:355-382) with permeablel/permeabletl
+ * (Class_particle.cpp:147-173). */
+static void or_permeability(or_ctx *c) {
+    int i;
+    for (i = 0; i < c->n; i++) {
+        OPar *pi = &c->p[i];
+        const sphg_problem *P = &c->pro;
+        if (pi->type != 2) continue;
+        if (P->permtype == 0) pi->cop = c->ps[i].cop;
+        else if (P->permtype == 1) pi->cop = P->pers;
+        else if (P->permtype == 2) {
+            double sitas = c->ps[i].sitas, sitar = c->ps[i].sitar;
+            if (pi->satu < sitar) pi->cop = P->pers;
+            else if (pi->satu > sitas) pi->cop = P->perl;
+            else pi->cop = (P->perl - P->pers) * pi->satu / (sitas - sitar) + P->pers;
+        } else if (P->permtype == 3) {
+            if (pi->permtype == 1) pi->cop = P->pers;
+            if (pi->permtype == 2) pi->cop = P->perm;
+            if (pi->permtype == 3) pi->cop = P->perl;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ eos (CPP_7-1-eos_CR.cpp:12-178) */
+static void or_eos(or_ctx *c) {
+    int n = c->n, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        if (pi->type == 1) {
+            pi->pre = pi->prep * (pow(pi->rho / c->pf[i].dens / c->pf[i].porosity, 7.0) - 1.0);
+            pi->pre = pi->pre + pi->prep;
+        }
+        if (pi->type == 3) { /* quirk A9: the comma operator discards the density ratio */
+            pi->pre = 1.403 * pi->prep * (-1.0);
+            pi->pre = pi->pre + pi->prep;
+        }
+        if (pi->type == 0 && pi->matype == 1) {
+            pi->pre = pi->prep * (pow(pi->rho / pi->rhop, 7.0) - 1.0);
+            pi->pre = pi->pre + pi->prep;
+        }
+    }
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double u1 = 0, u2 = 0, u3 = 0, u4 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, us[6] = {0}, ds = 0;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double w3 = c->pc[i].wij[nj][3];
+            if (pi->type == pj->type) continue;
+            if (pi->type == 2 && pj->type == 1) {
+                u1 = u1 + pj->mass * pj->pre * w3 / pj->rho;
+                d1 = d1 + pj->mass * w3 / pj->rho;
+            }
+            if (pi->type == 2 && pj->type == 3) {
+                u2 = u2 + pj->mass * pj->pre * w3 / pj->rho;
+                d2 = d2 + pj->mass * w3 / pj->rho;
+            }
+            if (pi->type == 4 && pj->type == 1) {
+                u3 = u3 + pj->mass * pj->pre * w3 / pj->rho;
+                d3 = d3 + pj->mass * w3 / pj->rho;
+            }
+            if (pi->type == 4 && pj->type == 3) {
+                u4 = u4 + pj->mass * pj->pre * w3 / pj->rho;
+                d4 = d4 + pj->mass * w3 / pj->rho;
+            }
+            if (pi->type == 0 && pj->type == 1) {
+                u1 = u1 + pj->mass * pj->pre * w3 / pj->rho;
+                d1 = d1 + pj->mass * w3 / pj->rho;
+            }
+            if (pi->type == 0 && pj->type == 3) {
+                u2 = u2 + pj->mass * pj->pre * w3 / pj->rho;
+                d2 = d2 + pj->mass * w3 / pj->rho;
+            }
+            if (pi->type == 0 && pj->type == 2) {
+                for (k = 0; k < 6; k++) us[k] = us[k] + pj->mass * pj->sig[k] * w3 / pj->rho;
+                ds = ds + pj->mass * w3 / pj->rho;
+            }
+        }
+        if (pi->type == 2) {
+            pi->prew = d1 != 0.0 ? u1 / d1 : u1;
+            pi->prea = d2 != 0.0 ? u2 / d2 : u2;
+        } else if (pi->type == 0) {
+            pi->prew = d1 != 0.0 ? u1 / d1 : u1;
+            pi->prea = d2 != 0.0 ? u2 / d2 : u2;
+            for (k = 0; k < 6; k++) pi->sig[k] = ds != 0.0 ? us[k] / ds : us[k];
+        } else if (pi->type == 4) {
+            pi->prew = d3 != 0.0 ? u3 / d3 : u3;
+            pi->prea = d4 != 0.0 ? u4 / d4 : u4;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ constitutive */
+/* deq(), shearstrain(), poro_cal(), stress_adjust() (Class_particle.cpp:121-215) */
+static void or_deq(OPar *p) {
+    double sxx, syy, szz, j2;
+    p->veps = p->eps[0] + p->eps[1] + p->eps[2];
+    p->vsig = (p->sig[0] + p->sig[1] + p->sig[2]) / 3;
+    p->divq = 0.0;
+    sxx = p->sig[0] - p->vsig;
+    syy = p->sig[1] - p->vsig;
+    szz = p->sig[2] - p->vsig;
+    j2 = 0.5 * (sxx * sxx + syy * syy + szz * szz) + p->sig[3] * p->sig[3] + p->sig[4] * p->sig[4] +
+         p->sig[5] * p->sig[5];
+    p->divq = sqrt(3 * fabs(j2));
+    p->divr = p->vsig != 0.0 ? fabs(p->divq / p->vsig) : 0.0;
+}
+static void or_shearstrain(OPar *p) {
+    double sxx = p->eps[0] - p->veps / 3.0, syy = p->eps[1] - p->veps / 3.0, szz = p->eps[2] - p->veps / 3.0;
+    double j2 = 0.5 * (sxx * sxx + syy * syy + szz * szz) + p->eps[3] * p->eps[3] + p->eps[4] * p->eps[4] +
+                p->eps[5] * p->eps[5];
+    p->meps = sqrt(2.0 * fabs(j2));
+}
+static void or_poro_cal(OPar *p, double dt) {
+    double de = (1 + p->e) * (p->deps[0] + p->deps[1] + p->deps[2]) * dt, et = p->e + de;
+    if (et < 0.0) et = 0.0;
+    p->porosity = et / (1 + et);
+    p->e = et;
+}
+static void or_stress_adjust(OPar *p) {
+    if (p->vsig > 0.0) {
+        double t = -p->vsig;
+        p->sig[0] = p->sig[0] + t;
+        p->sig[1] = p->sig[1] + t;
+        p->sig[2] = p->sig[2] + t;
+        p->sig[3] = p->sig[4] = p->sig[5] = 0.0;
+    }
+}
+
+static void iso_tangent(double d[6][6], double e0, double poi) {
+    double dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi), dmu = e0 * 0.5 / (1.0 + poi), dmu2 = dmu * 2.0;
+    d[0][0] = d[1][1] = d[2][2] = dlam + dmu2;
+    d[0][1] = d[0][2] = d[1][0] = d[1][2] = d[2][0] = d[2][1] = dlam;
+    d[3][3] = d[4][4] = d[5][5] = dmu;
+}
+
+/* mean-stress dependent Young's modulus shared by the soil models (CPP_8-1-consfactor.cpp:30-41) */
+static double soil_modulus(const OPar *p, const sphg_soil *s, double *sm_out) {
+    double sm = ((-p->sig[0]) + (-p->sig[1]) + (-p->sig[2])) * 0.3333333333;
+    if (sm < 1.0) sm = 1.0;
+    if (sm_out) *sm_out = sm;
+    if (s->e > 1.0) return s->e;
+    return 3 * (1 - 2 * s->poi) * (1 + s->eps0) * fabs(sm) / s->kapa;
+}
+
+/* elastic (CPP_8-1-consfactor.cpp:17-59) */
+static void or_elastic(double d[6][6], const OPar *p, const sphg_soil *s) {
+    iso_tangent(d, soil_modulus(p, s, NULL), s->poi);
+}
+
+/* rank-one plastic correction D -= d * w (x) z and plastic strain rate (CPP_8-1-consfactor.cpp:236-273) */
+static void plastic_update(double D[6][6], double d, const double w[6], const double z[6]) {
+    int a, b;
+    for (a = 0; a < 6; a++)
+        for (b = 0; b < 6; b++) D[a][b] = D[a][b] - d * w[a] * z[b];
+}
+
+/* sub_camclay (CPP_8-1-consfactor.cpp:129-281) */
+static void or_sub_camclay(double D[6][6], const OPar *p, const sphg_soil *s, double adps[6]) {
+    double sm, e0, poi = s->poi, zmf = s->zmf, ann = s->ann, eps0 = s->eps0;
+    double sig[6], ymf, hee, dlam, dmu, dmu2, dx, dy, dz, dxy, dyz, dzx, sjj, tt1, tt2;
+    double f11, f22, f33, f12, f23, f31, roue, ttt, hdd, fkk, fij, cc7, d, df, dg, ramda;
+    int k;
+    for (k = 0; k < 6; k++) sig[k] = -p->sig[k];
+    ymf = 3.0 * (zmf - 1.0) / (zmf + 2.0) / s->fs;
+    sm = (sig[0] + sig[1] + sig[2]) * 0.3333333333;
+    if (sm < 1.0) sm = 1.0;
+    e0 = s->e > 1.0 ? s->e : 3 * (1 - 2 * poi) * (1 + eps0) * fabs(sm) / s->kapa;
+    hee = (s->ramda - s->kapa) / (1.0 + eps0);
+    dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi);
+    dmu = e0 * 0.5 / (1.0 + poi);
+    dmu2 = dmu * 2.0;
+    iso_tangent(D, e0, poi);
+    dx = sig[0] - sm; dy = sig[1] - sm; dz = sig[2] - sm;
+    dxy = sig[3]; dyz = sig[4]; dzx = sig[5];
+    sjj = 0.5 * (dx * dx + dy * dy + dz * dz) + dxy * dxy + dyz * dyz + dzx * dzx;
+    tt1 = 1.0 / (ymf * ymf * sm * sm + 3.0 * sjj);
+    tt2 = 2.0 * ymf * ymf * sm * tt1 - 1.0 / sm;
+    f11 = tt2 / 3.0 + dx * tt1 * 3.0;
+    f22 = tt2 / 3.0 + dy * tt1 * 3.0;
+    f33 = tt2 / 3.0 + dz * tt1 * 3.0;
+    f12 = 3.0 * dxy * tt1;
+    f23 = 3.0 * dyz * tt1;
+    f31 = 3.0 * dzx * tt1;
+    roue = (1 + eps0) * log(s->sme0 / sm);
+    if (roue < 0.0) roue = 0.0;
+    ttt = ann * roue * roue;
+    hdd = f11 + f22 + f33 + ttt / sm;
+    fkk = f11 + f22 + f33;
+    fij = 2.0 * (f11 * f11 + f22 * f22 + f33 * f33) + f12 * f12 + f23 * f23 + f31 * f31;
+    cc7 = dlam * (fkk * fkk) + dmu * fij + hdd / hee;
+    d = 1.0 / cc7;
+    df = dlam * fkk;
+    dg = dlam * fkk;
+    ramda = (dmu2 * f11 + dlam * fkk) * p->deps[0] + (dmu2 * f22 + dlam * fkk) * p->deps[1] +
+            (dmu2 * f33 + dlam * fkk) * p->deps[2] +
+            dmu * (f12 * p->deps[3] + f23 * p->deps[4] + f31 * p->deps[5]);
+    if (ramda < 0.0) {
+        double w[6] = {dg + dmu2 * f11, dg + dmu2 * f22, dg + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+        double z[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+        plastic_update(D, d, w, z);
+        adps[0] = ramda * d * f11; adps[1] = ramda * d * f22; adps[2] = ramda * d * f33;
+        adps[3] = ramda * d * f12; adps[4] = ramda * d * f23; adps[5] = ramda * d * f31;
+    }
+}
+
+/* dpmodel1 (CPP_8-1-consfactor.cpp:284-448) */
+static void or_dpmodel1(double D[6][6], const OPar *p, const sphg_soil *s, double adps[6]) {
+    double fai = atan(tan(s->fai * 0.0174532925166667) / s->fs);
+    double ci = s->c / s->fs, poi = s->poi, eps0 = s->eps0;
+    double sig[6], sm, e0, dlam, dmu, dmu2, dx, dy, dz, dxy, dyz, dzx, sjj, sj, yalf, ykap;
+    int k, a, b;
+    for (k = 0; k < 6; k++) sig[k] = -p->sig[k];
+    for (a = 0; a < 6; a++)
+        for (b = 0; b < 6; b++) D[a][b] = 0.0;
+    sm = (sig[0] + sig[1] + sig[2]) * 0.3333333333;
+    if (sm < 1.0) sm = 1.0;
+    e0 = s->e > 1.0 ? s->e : 3 * (1 - 2 * poi) * (1 + eps0) * fabs(sm) / s->kapa;
+    dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi);
+    dmu = e0 * 0.5 / (1.0 + poi);
+    dmu2 = dmu * 2.0;
+    iso_tangent(D, e0, poi);
+    dx = sig[0] - sm; dy = sig[1] - sm; dz = sig[2] - sm;
+    dxy = sig[3]; dyz = sig[4]; dzx = sig[5];
+    sjj = 0.5 * (dx * dx + dy * dy + dz * dz) + dxy * dxy + dyz * dyz + dzx * dzx;
+    sj = sqrt(sjj);
+    yalf = sin(fai) / sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
+    ykap = 3.0 * ci * cos(fai) / sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
+    if ((sj - yalf * sm * 3.0 - ykap) >= 0.0) {
+        double f11 = -yalf + dx / sj * 0.5, f22 = -yalf + dy / sj * 0.5, f33 = -yalf + dz / sj * 0.5;
+        double f12 = dxy / sj * 0.5, f23 = dyz / sj * 0.5, f31 = dzx / sj * 0.5;
+        double fkk = f11 + f22 + f33, yd = 0.0;
+        double g11 = -yd + dx / sj * 0.5, g22 = -yd + dy / sj * 0.5, g33 = -yd + dz / sj * 0.5;
+        double g12 = dxy / sj * 0.5, g23 = dyz / sj * 0.5, g31 = dzx / sj * 0.5;
+        double hdd = g11 + g22 + g33;
+        double fij = 2.0 * (f11 * g11 + f22 * g22 + f33 * g33) + f12 * g12 + f23 * g23 + f31 * g31;
+        double cc7 = dlam * (fkk * hdd) + dmu * fij + hdd, d = 1.0 / cc7, df = dlam * fkk, dg = dlam * hdd;
+        double ramda = (dmu2 * f11 - 3.0 * dlam * yalf) * p->deps[0] +
+                       (dmu2 * f22 - 3.0 * dlam * yalf) * p->deps[1] +
+                       (dmu2 * f33 - 3.0 * dlam * yalf) * p->deps[2] +
+                       dmu * (f12 * p->deps[3] + f23 * p->deps[4] + f31 * p->deps[5]);
+        if (ramda * d < 0.0) {
+            double w[6] = {dg + dmu2 * g11, dg + dmu2 * g22, dg + dmu2 * g33, dmu * g12, dmu * g23, dmu * g31};
+            double z[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+            plastic_update(D, d, w, z);
+            adps[0] = ramda * d * f11; adps[1] = ramda * d * f22; adps[2] = ramda * d * f33;
+            adps[3] = ramda * d * f12; adps[4] = ramda * d * f23; adps[5] = ramda * d * f31;
+        }
+    }
+}
+
+/* DPModel_check: stress-scaling return onto the DP cone (CPP_8-1-consfactor.cpp:779-819); sj is
+ * taken from the stress before the increment (as the reference does). */
+static void or_dp_check(OPar *p, const sphg_soil *s, double dt) {
+    double fai = atan(tan(s->fai * 0.0174532925166667) / s->fs), ci = s->c / s->fs;
+    double sm = (p->sig[0] + p->sig[1] + p->sig[2]) / 3.0, ts[6], i1, af, kc, rn, sj;
+    double dx = p->sig[0] - sm, dy = p->sig[1] - sm, dz = p->sig[2] - sm;
+    int k;
+    sj = sqrt(0.5 * (dx * dx + dy * dy + dz * dz) + p->sig[3] * p->sig[3] + p->sig[4] * p->sig[4] +
+              p->sig[5] * p->sig[5]);
+    for (k = 0; k < 6; k++) ts[k] = p->sig[k] + p->dsig[k] * dt;
+    i1 = ts[0] + ts[1] + ts[2];
+    af = sin(fai) / sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
+    kc = 3.0 * ci * cos(fai) / sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
+    rn = (-af * i1 + kc) / sj;
+    if (rn < 1.0) {
+        for (k = 0; k < 3; k++) ts[k] = rn * (ts[k] - i1 / 3.0) + i1 / 3.0;
+        for (k = 3; k < 6; k++) ts[k] = rn * ts[k];
+    }
+    for (k = 0; k < 6; k++) p->dsig[k] = (ts[k] - p->sig[k]) / dt;
+}
+
+/* Jaumann correction of the stress rate (CPP_7-2-constitutive.cpp:236-253) */
+static void jaumann(OPar *p) {
+    double (*w)[3] = p->weps, *s = p->sig;
+    p->dsig[0] = p->dsig[0] + (w[0][0] * s[0] + w[0][1] * s[3] + w[0][2] * s[5]) -
+                 (s[0] * w[0][0] + s[3] * w[1][0] + s[5] * w[2][0]);
+    p->dsig[1] = p->dsig[1] + (w[1][0] * s[3] + w[1][1] * s[1] + w[1][2] * s[4]) -
+                 (s[3] * w[0][1] + s[1] * w[1][1] + s[4] * w[2][1]);
+    p->dsig[2] = p->dsig[2] + (w[2][0] * s[5] + w[2][1] * s[4] + w[2][2] * s[2]) -
+                 (s[5] * w[0][2] + s[4] * w[1][2] + s[2] * w[2][2]);
+    p->dsig[3] = p->dsig[3] + (w[0][0] * s[3] + w[0][1] * s[1] + w[0][2] * s[4]) -
+                 (s[0] * w[0][1] + s[3] * w[1][1] + s[5] * w[2][1]);
+    p->dsig[4] = p->dsig[4] + (w[1][0] * s[5] + w[1][1] * s[4] + w[1][2] * s[2]) -
+                 (s[3] * w[0][2] + s[1] * w[1][2] + s[4] * w[2][2]);
+    p->dsig[5] = p->dsig[5] + (w[0][0] * s[5] + w[0][1] * s[4] + w[0][2] * s[2]) -
+                 (s[0] * w[0][2] + s[3] * w[1][2] + s[5] * w[2][2]);
+}
+
+/* the matype tests of strain_to_stress (CPP_7-2-constitutive.cpp:104-235): model m for
+ * matype m, 10+m, 20+m and 50+m; any other matype selects no model (only Jaumann acts). */
+static int model_of(int matype) {
+    if ((matype >= 0 && matype < 30) || (matype >= 50 && matype < 60)) return matype % 10;
+    return -1;
+}
+
+static void apply_tangent(OPar *p, double D[6][6]) {
+    int a, b;
+    for (a = 0; a < 6; a++)
+        for (b = 0; b < 6; b++) p->dsig[a] += D[a][b] * p->deps[b];
+}
+
+/* strain_to_stress (CPP_7-2-constitutive.cpp:13-321) for the models the workloads use:
+ * Newtonian/Bingham fluid, elastic (0, 6), SB (1), DP (2), AMCC no-op (4, quirk A13), structure. */
+static int or_strain_to_stress(or_ctx *c) {
+    int n = c->n, i, bad = 0;
+    double dt = c->pro.dt;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads) reduction(max : bad)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double D[6][6], erfs[6], adps[6], deps[6];
+        int k, a, b;
+        if (pi->type == 1 || pi->type == 3) {
+            const sphg_fluid *f = &c->pf[i];
+            double vis = f->vis;
+            if (pi->type == 1) {
+                double dm = (pi->deps[0] + pi->deps[1] + pi->deps[2]) / 3.0, de;
+                pi->eps[0] = pi->deps[0] - dm;
+                pi->eps[1] = pi->deps[1] - dm;
+                pi->eps[2] = pi->deps[2] - dm;
+                de = 0.666666666666667 * (pi->eps[0] * pi->eps[0] + pi->eps[1] * pi->eps[1] +
+                                          pi->eps[2] * pi->eps[2] + pi->deps[3] * pi->deps[3] * 0.25 +
+                                          pi->deps[4] * pi->deps[4] * 0.25 + pi->deps[5] * pi->deps[5] * 0.25);
+                de = sqrt(de);
+                if (f->bhtype == 1) {
+                    double fai = f->fai * 0.0174532925199, tao = pi->pre * tan(fai) + f->c;
+                    vis = de < f->strain_min ? f->vis + tao / f->strain_min : f->vis + tao / de;
+                }
+            }
+            for (k = 0; k < 3; k++)
+                deps[k] = 2.0 * pi->deps[k] - 2.0 * (pi->deps[0] + pi->deps[1] + pi->deps[2]) / 3.0;
+            for (k = 3; k < 6; k++) deps[k] = pi->deps[k];
+            for (k = 0; k < 6; k++) pi->sig[k] = k < 3 ? -pi->pre - vis * deps[k] : -vis * deps[k];
+        } else if (pi->type == 2) {
+            const sphg_soil *s = &c->ps[i];
+            int mt = model_of(pi->matype), known = 1;
+            for (a = 0; a < 6; a++) {
+                erfs[a] = adps[a] = 0.0;
+                for (b = 0; b < 6; b++) D[a][b] = 0.0;
+            }
+            if (c->pro.flts >= 1) known = 0; /* TS transform not restated */
+            if (c->l < c->pro.inip) {
+                or_elastic(D, pi, s);
+                apply_tangent(pi, D);
+            } else if (mt == 0 || mt == 6) {
+                or_elastic(D, pi, s);
+                apply_tangent(pi, D);
+            } else if (mt == 1) {
+                or_sub_camclay(D, pi, s, adps);
+                apply_tangent(pi, D);
+            } else if (mt == 2) {
+                or_dpmodel1(D, pi, s, adps);
+                apply_tangent(pi, D);
+                or_dp_check(pi, s, dt);
+            } else if (mt == 4) {
+                apply_tangent(pi, D); /* AMCC body is commented out: zero tangent, no erfs */
+            } else if (mt >= 0) {
+                known = 0;
+            }
+            if (!known) bad = SPHG_E_UNSUPPORTED;
+            jaumann(pi);
+            for (k = 0; k < 6; k++) {
+                pi->adps[k] = pi->adps[k] + adps[k] * dt;
+                pi->eps[k] = pi->eps[k] + pi->deps[k] * dt;
+                pi->sig[k] = pi->sig[k] + pi->dsig[k] * dt + erfs[k];
+            }
+            or_deq(pi);
+            or_shearstrain(pi);
+            or_poro_cal(pi, dt);
+        } else if (pi->type == 4) {
+            for (a = 0; a < 6; a++)
+                for (b = 0; b < 6; b++) D[a][b] = 0.0;
+            iso_tangent(D, c->mat.structure.e, c->mat.structure.poi);
+            apply_tangent(pi, D);
+            jaumann(pi);
+            for (k = 0; k < 6; k++) {
+                pi->eps[k] = pi->eps[k] + pi->deps[k] * dt;
+                pi->sig[k] = pi->sig[k] + pi->dsig[k] * dt;
+            }
+            or_deq(pi);
+            or_shearstrain(pi);
+            or_poro_cal(pi, dt);
+        }
+    }
+    return bad;
+}
+
+/* free_surface_check (CPP_7-2-constitutive.cpp:453-506) */
+static void or_free_surface(or_ctx *c) {
+    int n = c->n, i;
+    double tmax = c->pro.threshold_max, tmin = c->pro.threshold_min, incl = tmax - tmin;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double color = 0.0, xs;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pi->type == pj->type && pi->type != 0 && pi->type != 7)
+                color = color + pj->mass / pj->rho * 1.0 * c->pc[i].wij[nj][3];
+            else if (pi->type != pj->type && pi->type > 0 && pi->type < 5 && pj->type == 0)
+                color = color + pj->mass / pj->rho * 1.0 * c->pc[i].wij[nj][3];
+        }
+        if (pi->type == 1) {
+            if (color < tmin / 2.0) {
+                for (k = 0; k < 3; k++) pi->sig[k] = -1.0;
+                for (k = 3; k < 6; k++) pi->sig[k] = 0.0;
+            } else if (color >= tmin / 2.0 && color < tmax / 2.0) {
+                xs = (2.0 * color - tmin) / incl;
+                for (k = 0; k < 6; k++) pi->sig[k] = xs * pi->sig[k];
+            }
+        } else if (pi->type == 2) {
+            if (color < tmin) {
+                for (k = 0; k < 3; k++) pi->sig[k] = -1.0;
+                for (k = 3; k < 6; k++) pi->sig[k] = 0.0;
+            } else if (color >= tmin && color < tmax) {
+                xs = (color - tmin) / incl * 1.0;
+                for (k = 0; k < 6; k++) pi->sig[k] = xs * pi->sig[k];
+            }
+            or_stress_adjust(pi);
+            or_deq(pi);
+        }
+    }
+}
+
+/* eejcb: cyclic Jacobi eigen-solver (CPP_7-1-strain.cpp:341-437) */
+static int or_eejcb(double a[9], double v[9], double eps, int jt) {
+    int i, j, p = 0, q = 0, u, w, t, s, l = 1;
+    double fm, cn, sn, omega, x, y, d;
+    for (i = 0; i < 3; i++)
+        for (j = 0; j < 3; j++) v[i * 3 + j] = i == j ? 1.0 : 0.0;
+    for (;;) {
+        fm = 0.0;
+        for (i = 0; i < 3; i++)
+            for (j = 0; j < 3; j++) {
+                d = fabs(a[i * 3 + j]);
+                if (i != j && d > fm) { fm = d; p = i; q = j; }
+            }
+        if (fm < eps) return 1;
+        if (l > jt) return -1;
+        l = l + 1;
+        u = p * 3 + q; w = p * 3 + p; t = q * 3 + p; s = q * 3 + q;
+        x = -a[u];
+        y = (a[s] - a[w]) / 2.0;
+        omega = x / sqrt(x * x + y * y);
+        if (y < 0.0) omega = -omega;
+        sn = 1.0 + sqrt(1.0 - omega * omega);
+        sn = omega / sqrt(2.0 * sn);
+        cn = sqrt(1.0 - sn * sn);
+        fm = a[w];
+        a[w] = fm * cn * cn + a[s] * sn * sn + a[u] * omega;
+        a[s] = fm * sn * sn + a[s] * cn * cn - a[u] * omega;
+        a[u] = 0.0;
+        a[t] = 0.0;
+        for (j = 0; j < 3; j++)
+            if (j != p && j != q) {
+                u = p * 3 + j; w = q * 3 + j;
+                fm = a[u];
+                a[u] = fm * cn + a[w] * sn;
+                a[w] = -fm * sn + a[w] * cn;
+            }
+        for (i = 0; i < 3; i++)
+            if (i != p && i != q) {
+                u = i * 3 + p; w = i * 3 + q;
+                fm = a[u];
+                a[u] = fm * cn + a[w] * sn;
+                a[w] = -fm * sn + a[w] * cn;
+            }
+        for (i = 0; i < 3; i++) {
+            u = i * 3 + p; w = i * 3 + q;
+            fm = v[u];
+            v[u] = fm * cn + v[w] * sn;
+            v[w] = -fm * sn + v[w] * cn;
+        }
+    }
+}
+
+/* Stress_eigValue (CPP_7-1-strain.cpp:283-332) */
+static void or_eig(or_ctx *c) {
+    int n = c->n, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double a[9] = {pi->sig[0], pi->sig[3], pi->sig[5], pi->sig[3], pi->sig[1], pi->sig[4],
+                       pi->sig[5], pi->sig[4], pi->sig[2]}, v[9];
+        int k;
+        or_eejcb(a, v, 0.001, 7);
+        for (k = 0; k < 3; k++) {
+            pi->strep[k] = a[k * 3 + k];
+            pi->beta_ts[k][0] = v[k * 3]; pi->beta_ts[k][1] = v[k * 3 + 1]; pi->beta_ts[k][2] = v[k * 3 + 2];
+            pi->beta[0][k] = v[k * 3]; pi->beta[1][k] = v[k * 3 + 1]; pi->beta[2][k] = v[k * 3 + 2];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ accelerations */
+/* the nine stress-divergence sums of CPP_9-4-derivation_stress.cpp (e.g. :47-64): component
+ * (a,b) pairs stress index SIDX[a][b] with kernel-gradient component b. sgn=+1 adds the i term,
+ * sgn=-1 subtracts it (the *_minor* variants). */
+static const int SIDX[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+static void sdiv(double t[3][3], double mj, const double *aj, double rj, const double *ai, double ri,
+                 int sgn, const double *w) {
+    int a, b;
+    for (a = 0; a < 3; a++)
+        for (b = 0; b < 3; b++) {
+            int s = SIDX[a][b];
+            if (sgn > 0) t[a][b] = t[a][b] + mj * (aj[s] / rj / rj + ai[s] / ri / ri) * w[b];
+            else t[a][b] = t[a][b] + mj * (aj[s] / rj / rj - ai[s] / ri / ri) * w[b];
+        }
+}
+static void add_sum(double ax[3], double t[3][3]) {
+    ax[0] += t[0][0] + t[0][1] + t[0][2];
+    ax[1] += t[1][0] + t[1][1] + t[1][2];
+    ax[2] += t[2][0] + t[2][1] + t[2][2];
+}
+
+/* fluid_acceleration_water/air and the minor* variants (CPP_9-4-derivation_stress.cpp:12-351) */
+static void or_fluid_accel(or_ctx *c, int ftype, int other, int sgn) {
+    int n = c->n, i;
+    double g[3] = {c->pro.gx, c->pro.gy, c->pro.gz};
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double t[3][3] = {{0}}, por, ai[6], aj[6];
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (pi->type != ftype) continue;
+        por = c->pf[i].porosity;
+        for (k = 0; k < 3; k++) pi->ax[k] = g[k];
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pi->type == pj->type || pj->type == other) {
+                for (k = 0; k < 6; k++) {
+                    ai[k] = por * pi->sig[k];
+                    aj[k] = por * pj->sig[k];
+                }
+                sdiv(t, pj->mass, aj, pj->rho, ai, pi->rho, sgn, c->pc[i].wij[nj]);
+            }
+        }
+        add_sum(pi->ax, t);
+    }
+}
+
+/* soil_acceleration_noreg (CPP_9-4-derivation_stress.cpp:354-428) and
+ * structure_acceleration_noreg (:431-505) */
+static void or_solid_accel(or_ctx *c, int stype) {
+    int n = c->n, i;
+    double g[3] = {c->pro.gx, c->pro.gy, c->pro.gz};
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double t[3][3] = {{0}}, por = c->ps[i].porosity;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (pi->type != stype) continue;
+        for (k = 0; k < 3; k++) pi->ax[k] = g[k];
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            const double *w = c->pc[i].wij[nj];
+            double satu, prei, prej;
+            if (pi->type != pj->type) continue;
+            satu = pj->satu; /* quirk: the neighbour's saturation is used for both terms */
+            sdiv(t, pj->mass, pj->sig, pj->rho, pi->sig, pi->rho, 1, w);
+            if (stype == 2) {
+                double di = c->ps[i].dens; /* quirk A12: parameter density, not rho_i */
+                prei = (1 - por) * (-satu * pi->prew + (1 - satu) * pi->prea);
+                prej = (1 - por) * (-satu * pj->prew + (1 - satu) * pj->prea);
+                for (k = 0; k < 3; k++)
+                    t[k][k] = t[k][k] + pj->mass * (prej / pj->rho / pj->rho + prei / di / di) * w[k];
+            } else {
+                prei = -satu * pi->prew + (1 - satu) * pi->prea;
+                prej = -satu * pj->prew + (1 - satu) * pj->prea;
+                for (k = 0; k < 3; k++)
+                    t[k][k] = t[k][k] + pj->mass * (prej / pj->rho / pj->rho - prei / pi->rho / pi->rho) * w[k];
+            }
+        }
+        add_sum(pi->ax, t);
+    }
+}
+
+/* artificial_viscosity (CPP_9-3-artificialvis.cpp:12-99) */
+static void or_art_visc(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double tu[3] = {0, 0, 0};
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double a = 0, b = 0, cs = 0, inprod = 0, dst = 0, myu, paix, hl, rou, dvx[3], dx[3];
+            if (!(pi->type == pj->type && pi->type != 7)) continue;
+            for (k = 0; k < ndim; k++) {
+                dvx[k] = pi->vxp[k] - pj->vxp[k];
+                dx[k] = pi->xp[k] - pj->xp[k];
+                dst = dst + dx[k] * dx[k];
+                inprod = inprod + dvx[k] * dx[k];
+            }
+            hl = (pj->hl + pi->hl) * 0.5;
+            rou = (pj->rho + pi->rho) * 0.5;
+            if (pi->type == 1) { cs = c->pf[i].vpu; a = c->pro.art_vis[0][0]; b = c->pro.art_vis[0][1]; }
+            if (pi->type == 2) { cs = c->ps[i].vpu; a = c->pro.art_vis[1][0]; b = c->pro.art_vis[1][1]; }
+            if (pi->type == 3) { cs = c->pf[i].vpu; a = c->pro.art_vis[2][0]; b = c->pro.art_vis[2][1]; }
+            if (inprod < 0.0) {
+                dst = sqrt(dst);
+                myu = hl * inprod / (dst * dst + 0.01 * hl * hl);
+                paix = (-a * cs * myu + b * myu * myu) / rou;
+            } else
+                paix = 0.0;
+            for (k = 0; k < ndim; k++) tu[k] = tu[k] + pj->mass * paix * c->pc[i].wij[nj][k];
+        }
+        if (pi->type != 7)
+            for (k = 0; k < ndim; k++) pi->ax[k] = pi->ax[k] - tu[k];
+    }
+}
+
+/* boundary_moving_stress_effect, flbndy 1/2 (CPP_9-4-derivation_stress.cpp:631-721) */
+static void or_bndy_stress(or_ctx *c) {
+    int n = c->n, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double t[3][3] = {{0}}, as[6];
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (pi->type == 0 || pi->type == 7) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pj->type != 0) continue;
+            for (k = 0; k < 6; k++) as[k] = 0.0;
+            if (pi->type == 1 || pi->type == 3) {
+                for (k = 0; k < 6; k++) as[k] = pi->sig[k];
+            } else if (pi->type == 2 || pi->type == 4) {
+                double satu = pi->satu;
+                for (k = 0; k < 3; k++) as[k] = pi->sig[k] - satu * pi->prew + (1 - satu) * pi->prea;
+                for (k = 3; k < 6; k++) as[k] = pi->sig[k];
+            }
+            sdiv(t, pj->mass, as, pj->rho, as, pi->rho, 1, c->pc[i].wij[nj]);
+        }
+        add_sum(pi->ax, t);
+    }
+}
+
+/* interact1: Darcy drag (CPP_10-interact1.cpp:15-110) */
+static void or_interact1(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+    double xishu = c->pro.interc, dt = c->pro.dt;
+    double gacc = sqrt(c->pro.gx * c->pro.gx + c->pro.gy * c->pro.gy + c->pro.gz * c->pro.gz);
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double tu[3] = {0, 0, 0}, dx[3], dvx[3], inprod, vx1, vx2;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double w3 = c->pc[i].wij[nj][3];
+            if (pi->type == pj->type) continue;
+            inprod = 0.0;
+            for (k = 0; k < ndim; k++) {
+                dx[k] = pi->xp[k] - pj->xp[k];
+                dvx[k] = pi->vxp[k] - pj->vxp[k];
+                inprod = inprod + dx[k] * dvx[k];
+            }
+            if (!(inprod < 0)) continue;
+            if (pi->type == 2 && pj->type == 1) {
+                for (k = 0; k < ndim; k++)
+                    tu[k] = tu[k] + pi->satu * pi->satu * pi->porosity * pi->porosity * c->pf[j].dens * gacc *
+                                        dvx[k] / pi->cop * pj->mass / pj->rho * w3;
+            } else if (pi->type == 1 && pj->type == 2) {
+                for (k = 0; k < ndim; k++)
+                    tu[k] = tu[k] + pj->satu * pj->satu * pj->porosity * pj->porosity * c->pf[i].dens * gacc *
+                                        dvx[k] / pj->cop * pj->mass / pj->rho * w3;
+            } else if (pi->type == 2 && pj->type == 3) {
+                for (k = 0; k < ndim; k++)
+                    tu[k] = tu[k] + (1 - pi->satu) * (1 - pi->satu) * pi->porosity * pi->porosity * c->pf[j].dens *
+                                        gacc * dvx[k] * 2000 * pj->mass / pj->rho * w3;
+            } else if (pi->type == 3 && pj->type == 2) {
+                for (k = 0; k < ndim; k++)
+                    tu[k] = tu[k] + (1 - pj->satu) * (1 - pj->satu) * pj->porosity * pj->porosity * c->pf[i].dens *
+                                        gacc * dvx[k] * 2000 * pj->mass / pj->rho * w3;
+            }
+        }
+        if (pi->type != 0 && pi->type != 7) {
+            for (k = 0; k < ndim; k++) {
+                pi->interf[k] = tu[k];
+                vx1 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k]) * dt;
+                vx2 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k] - pi->interf[k] * xishu / pi->rho) * dt;
+                if (vx1 * vx2 <= 0.0) {
+                    pi->interf[k] = 0.0;
+                    pi->ax[k] = -2.0 * pi->vxp[k] / dt - pi->axp[k];
+                } else
+                    pi->ax[k] -= pi->interf[k] * xishu / pi->rho;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ velocity */
+/* veocity_update, if_move == 0 (CPP_11-xsph.cpp:11-57) */
+static void or_velocity(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i, k;
+    double dt = c->pro.dt, ath;
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        if (pi->type == 0 && pi->matype < 2) {
+            for (k = 0; k < ndim; k++) {
+                pi->interf[k] = 0.0;
+                ath = (pi->interf[k] + pi->interf[k]) * 0.5;
+                pi->interfss[k] = pi->interfss[k] + ath * dt;
+            }
+        } else if (pi->type == 0 && pi->matype == 3) {
+            for (k = 0; k < ndim; k++) {
+                ath = (pi->interf[k] + pi->interf[k]) * 0.5;
+                pi->interfss[k] = pi->interfss[k] + ath * dt;
+            }
+        } else if (pi->type == 0 && pi->matype == 5 && c->pro.flv == 1) {
+            for (k = 0; k < ndim; k++) {
+                ath = (pi->interf[k] + pi->interf[k]) * 0.5;
+                pi->interfss[k] = pi->interfss[k] + ath * dt;
+            }
+        } else if (pi->type != 0) {
+            for (k = 0; k < ndim; k++) {
+                ath = (pi->axp[k] + pi->ax[k]) * 0.5;
+                pi->vx[k] = pi->vxp[k] + ath * dt;
+            }
+        }
+    }
+}
+
+/* xsph (CPP_11-xsph.cpp:145-256); boundary neighbours use pPar[j].vx (quirk A16) */
+static void or_xsph(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double xs = 0.0, tu[3] = {0, 0, 0}, dn = 0.0;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (k = 0; k < ndim; k++) c->vx_temp[i][k] = 0.0;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj], on = 0;
+            const OPar *pj = &c->p[j];
+            double w3 = c->pc[i].wij[nj][3];
+            if (pi->type == pj->type && pi->type != 0 && pi->type != 7) {
+                if (pi->type == 1) xs = c->pro.xsph[0];
+                else if (pi->type == 2) xs = c->pro.xsph[1];
+                else if (pi->type == 3) xs = c->pro.xsph[2];
+                else if (pi->type == 4) xs = c->pro.xsph[1];
+                on = 1;
+            } else if (pi->type >= 1 && pi->type <= 4 && pj->type == 0) {
+                if (pi->type == 1) xs = 0.006;
+                else if (pi->type == 2) xs = c->pro.xsph[1] / 5.0;
+                else if (pi->type == 3) xs = 0.006;
+                else if (pi->type == 4) xs = c->pro.xsph[1] / 5.0;
+                on = 1;
+            }
+            if (!on) continue;
+            for (k = 0; k < ndim; k++) tu[k] = tu[k] + xs * pj->mass * (pj->vx[k] - pi->vx[k]) * w3 / pj->rho;
+            dn = dn + pj->mass * w3 / pj->rho;
+        }
+        if (pi->type != 0 && pi->type != 7) {
+            if (fabs(dn) > 1.0e-7 && (pi->type == 2 || pi->type == 4))
+                for (k = 0; k < ndim; k++) c->vx_temp[i][k] = pi->vx[k] + tu[k] / dn;
+            else
+                for (k = 0; k < ndim; k++) c->vx_temp[i][k] = pi->vx[k] + tu[k];
+        }
+    }
+    for (i = 0; i < n; i++)
+        if (c->p[i].type != 0 && c->p[i].type != 7)
+            for (int k = 0; k < ndim; k++) c->p[i].vx[k] = c->vx_temp[i][k];
+}
+
+/* ------------------------------------------------------------------ boundary conditions */
+/* velocity_set_domain / tmboundary repulsion (CPP_13-boundary.cpp:71-177) */
+static void or_vel_repulse(or_ctx *c, double coe, int fluid_only) {
+    int n = c->n, ndim = c->pro.ndim, i;
+    double dr = c->pro.dr;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double tu[3] = {0, 0, 0};
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (fluid_only ? !(pi->type == 1 || pi->type == 3) : (pi->type == 0 || pi->type == 7)) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double inprod = 0, dst = 0, db, beta;
+            if (pj->type != 0) continue;
+            for (k = 0; k < ndim; k++) {
+                inprod = inprod + (pi->vx[k] - pj->interfss[k]) * (pi->xp[k] - pj->xp[k]);
+                dst = dst + (pi->xp[k] - pj->xp[k]) * (pi->xp[k] - pj->xp[k]);
+            }
+            if (inprod < 0.0) {
+                dst = sqrt(dst);
+                db = coe * dr;
+                beta = 1 + db / (dst - coe * dr);
+                beta = fmin(1.5, beta);
+                for (k = 0; k < ndim; k++) tu[k] = tu[k] + (1 - beta) * (pi->xp[k] - pj->xp[k]) * inprod / dst / dst;
+            }
+        }
+        for (k = 0; k < ndim; k++) pi->vx[k] = pi->vx[k] + tu[k];
+    }
+}
+
+/* the first three (2D: two) ids of clParti_Pair::dist_sorting (Class_particle.cpp:343-359):
+ * quirk A18, only ids are swapped, distances stay in their original slots. */
+static void dist_sorting(int total, int *id, const double *dst) {
+    int i, j;
+    for (i = 0; i < 4; i++)
+        for (j = i + 1; j < total; j++)
+            if (dst[i] > dst[j]) {
+                int t = id[i];
+                id[i] = id[j];
+                id[j] = t;
+            }
+}
+
+typedef struct { double x, y, z; } OPt;
+typedef struct { double A, B, C, D; } OPlane;
+
+/* cal_plane (CPP_13-boundary.cpp:654-666) */
+static void cal_plane(OPt p1, OPt p2, OPt p3, OPlane *o) {
+    double dst = -p1.x * (p2.y * p3.z - p3.y * p2.z) - p2.x * (p3.y * p1.z - p1.y * p3.z) -
+                 p3.x * (p1.y * p2.z - p2.y * p1.z);
+    o->A = p1.y * (p2.z - p3.z) + p2.y * (p3.z - p1.z) + p3.y * (p1.z - p2.z);
+    o->B = p1.z * (p2.x - p3.x) + p2.z * (p3.x - p1.x) + p3.z * (p1.x - p2.x);
+    o->C = p1.x * (p2.y - p3.y) + p2.x * (p3.y - p1.y) + p3.x * (p1.y - p2.y);
+    o->A = o->A / dst;
+    o->B = o->B / dst;
+    o->C = o->C / dst;
+}
+/* plane_two_points (CPP_13-boundary.cpp:684-708) */
+static void plane_two_points(OPt p1, OPt p2, OPlane *o) {
+    double a = p1.x - p2.x, b = p1.y - p2.y, cc = p1.z - p2.z, t = -a * p2.x - b * p2.y - cc * p2.z;
+    if (fabs(t) > 1.0e-6) {
+        o->A = a / t; o->B = b / t; o->C = cc / t; o->D = 1.0;
+    } else {
+        o->A = a; o->B = b; o->C = cc; o->D = 0.0;
+    }
+}
+
+/* tm_boundary_2d / tm_boundary_3d (CPP_13-boundary.cpp:179-614) */
+static void or_tm_boundary(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+    double dr = c->pro.dr;
+#pragma omp parallel num_threads(c->nthreads)
+    {
+    /* quirk: plane1 is thread-private and default-constructed once per parallel region, and
+     * cal_plane never writes D, so D keeps the last plane_two_points value of the same thread */
+    double sticky_D = 1.0;
+#pragma omp for schedule(static)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        int ids[NBMAX], total = 0, id = 0, nc = nbcount(&c->pc[i]), nj, k;
+        double dsts[NBMAX], dst, da, db, beta;
+        if (pi->type == 0 || pi->type == 7) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pj->type != 0) continue;
+            ids[total] = j;
+            dst = (pi->xp[0] - pj->xp[0]) * (pi->xp[0] - pj->xp[0]) + (pi->xp[1] - pj->xp[1]) * (pi->xp[1] - pj->xp[1]) +
+                  (pi->xp[2] - pj->xp[2]) * (pi->xp[2] - pj->xp[2]);
+            dsts[total++] = sqrt(dst);
+            if (pj->matype > 2) id = j;
+        }
+        for (k = total; k < NBMAX; k++) { ids[k] = -1; dsts[k] = 500000.0; }
+        dist_sorting(total, ids, dsts);
+        if ((ndim == 2 && total >= 2) || (ndim == 3 && total >= 3)) {
+            OPt p4 = {pi->xp[0], pi->xp[1], pi->xp[2]};
+            OPt p1 = {c->p[ids[0]].xp[0], c->p[ids[0]].xp[1], c->p[ids[0]].xp[2]};
+            OPt p2 = {c->p[ids[1]].xp[0], c->p[ids[1]].xp[1], c->p[ids[1]].xp[2]};
+            if (ndim == 2) {
+                /* cal_line (A,B with C=1, quirk A20) + cal_dpline (CPP_13-boundary.cpp:617-652) */
+                double den = p2.y * p1.x - p1.y * p2.x, A = (p1.y - p2.y) / den, B = (p2.x - p1.x) / den, tmp;
+                dst = fabs(A * p4.x + B * p4.y + 1.0);
+                tmp = sqrt(A * A + B * B);
+                dst = tmp > 1.0e-3 ? dst / tmp : 0.0;
+            } else {
+                OPt p3 = {c->p[ids[2]].xp[0], c->p[ids[2]].xp[1], c->p[ids[2]].xp[2]};
+                OPlane c1 = {0, 0, 0, 1.0}, c2 = {0, 0, 0, 1.0}, pl = {0, 0, 0, sticky_D};
+                double cs, tmp;
+                cal_plane(p1, p2, p4, &c1);
+                cal_plane(p1, p3, p4, &c2);
+                cs = c1.A * c2.A + c1.B * c2.B + c1.C * c2.C;
+                cs = cs / sqrt(c1.A * c1.A + c1.B * c1.B + c1.C * c1.C) / sqrt(c2.A * c2.A + c2.B * c2.B + c2.C * c2.C);
+                if (fabs(cs) > 0.999999) plane_two_points(p4, p1, &pl);
+                else cal_plane(p1, p2, p3, &pl);
+                sticky_D = pl.D;
+                /* cal_dpplane (CPP_13-boundary.cpp:631-642) */
+                dst = fabs(pl.A * p4.x + pl.B * p4.y + pl.C * p4.z + pl.D);
+                tmp = sqrt(pl.A * pl.A + pl.B * pl.B + pl.C * pl.C);
+                dst = tmp > 1.0e-3 ? dst / tmp : 0.0;
+            }
+            da = dst > 0.6 * dr ? dst - 0.5 * dr : 0.1 * dr;
+            db = 0.5 * dr;
+            if (fabs(da) > 1.0e-6) {
+                beta = 1.0 + db / da;
+                beta = fmin(1.5, beta);
+            } else
+                beta = 1.5;
+            c->vel_bndy[i][0] = (1.0 - beta) * pi->vx[0] + beta * c->p[id].interfss[0];
+            c->vel_bndy[i][1] = (1.0 - beta) * pi->vx[1] + beta * c->p[id].interfss[1];
+            c->vel_bndy[i][2] = ndim == 2 ? 0.0 : (1.0 - beta) * pi->vx[2] + beta * c->p[id].interfss[2];
+        } else {
+            for (k = 0; k < 3; k++) c->vel_bndy[i][k] = pi->vx[k];
+        }
+    }
+    }
+    /* second loop: Shepard average per phase with the tempup_vel[t][0] quirk (A17) */
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double tv[4][3] = {{0}}, td[4] = {0, 0, 0, 0};
+        int nc = nbcount(&c->pc[i]), nj, t, k;
+        double *out[4];
+        if (pi->type != 0) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double w3 = c->pc[i].wij[nj][3];
+            if (pj->type < 1 || pj->type > 4) continue;
+            t = pj->type - 1;
+            tv[t][0] = tv[t][0] + pj->mass * c->vel_bndy[j][0] * w3 / pj->rho;
+            tv[t][1] = tv[t][0] + pj->mass * c->vel_bndy[j][1] * w3 / pj->rho;
+            tv[t][2] = tv[t][0] + pj->mass * c->vel_bndy[j][2] * w3 / pj->rho;
+            td[t] = td[t] + pj->mass * w3 / pj->rho;
+        }
+        out[0] = c->vb[i].vx_water; out[1] = c->vb[i].vx_soil; out[2] = c->vb[i].vx_air; out[3] = c->vb[i].vx_struct;
+        for (t = 0; t < 4; t++)
+            for (k = 0; k < 3; k++) out[t][k] = fabs(td[t]) > 1.0e-6 ? tv[t][k] / td[t] : 0.0;
+    }
+}
+
+/* check_domain (CPP_13-boundary.cpp:20-68), serial semantics of the shared flag/vbx (A3) */
+static void or_check_domain(or_ctx *c) {
+    const sphg_problem *P = &c->pro;
+    double bmax[3] = {P->xmax, P->ymax, P->zmax}, bmin[3] = {P->xmin, P->ymin, P->zmin}, vbx[3] = {0, 0, 0};
+    double dr = P->dr;
+    int flag = 0, i, k;
+    for (i = 0; i < c->n; i++) {
+        OPar *pi = &c->p[i];
+        if (pi->type != 0 && pi->type != 7) {
+            int hit = 0;
+            if (pi->xp[0] - bmax[0] >= -0.5 * dr) hit = 1;
+            if (pi->xp[0] - bmin[0] <= 0.5 * dr) hit = 1;
+            if (pi->xp[1] - bmax[1] >= -0.5 * dr) hit = 1;
+            if (pi->xp[1] - bmin[1] <= 0.5 * dr) hit = 1;
+            if (pi->xp[2] - bmax[2] >= -0.5 * dr && P->ndim == 3) hit = 1;
+            if (pi->xp[2] - bmin[2] <= 0.5 * dr && P->ndim == 3) hit = 1;
+            if (hit)
+                for (k = 0; k < 3; k++) pi->vx[k] = vbx[k];
+        } else if (pi->type == 0 && pi->matype > 2 && flag == 0) {
+            flag = 1;
+            for (k = 0; k < 3; k++) vbx[k] = pi->interfss[k];
+        }
+    }
+}
+
+/* update (CPP_12-1-update.cpp:11-55) */
+static void or_update(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i, k, loop = c->l - c->pro.inip;
+    double dt = c->pro.dt;
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        for (k = 0; k < ndim; k++) {
+            if (pi->type != 0) {
+                pi->x[k] = pi->xp[k] + (pi->vx[k] + pi->vxp[k]) * dt * 0.5;
+                if (loop == 0) pi->ux[k] = 0.0;
+                else pi->ux[k] = pi->ux[k] + (pi->vx[k] + pi->vxp[k]) * dt * 0.5;
+                pi->xp[k] = pi->x[k];
+                pi->vxp[k] = pi->vx[k];
+                pi->axp[k] = pi->ax[k];
+                pi->rhop = pi->rho;
+            } else if (pi->matype >= 2 && pi->matype <= 5) {
+                pi->x[k] = pi->xp[k] + (pi->interfss[k] + pi->interfss[k]) * dt * 0.5;
+                pi->ux[k] = pi->ux[k] + (pi->interfss[k] + pi->interfss[k]) * dt * 0.5;
+                pi->xp[k] = pi->x[k];
+                pi->rhop = pi->rho;
+                pi->prep = pi->pre;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ one step */
+static int unsupported_flags(const or_ctx *c) {
+    const sphg_problem *P = &c->pro;
+    if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
+    if (P->flagum >= 1) return 1;
+    if (P->fli > 1 || P->ss_type != 0) return 1;
+    if (P->art_stre_coe > 0.0 || P->step_regu <= P->loop) return 1;
+    if (P->flbndy != 1 && P->flbndy != 2) return 1;
+    if (P->if_move != 0) return 1;
+    if (P->inip >= c->l) return 1; /* soil_damping_geostress not restated */
+    return 0;
+}
+
+int or_step(or_ctx *c, int nsteps) {
+    int s, e;
+    if (unsupported_flags(c)) return SPHG_E_UNSUPPORTED;
+    for (s = 0; s < nsteps; s++) {
+        c->t = c->t + c->pro.dt;
+        if ((e = or_parttocell(c))) return e;
+        if ((e = or_partisearching(c))) return e;
+        or_density(c);
+        or_strain(c);
+        if (c->pro.fli == 1) or_permeability(c);
+        or_eos(c);
+        if ((e = or_strain_to_stress(c))) return e;
+        or_free_surface(c);
+        or_eig(c);
+        /* CPP_1-SPHLoops.cpp:586-600: the water/air routine by acc_type of water[0] / air */
+        or_fluid_accel(c, 1, 3, c->mat.water[0].acc_type == 0 ? 1 : -1);
+        or_fluid_accel(c, 3, 1, c->mat.air.acc_type == 0 ? 1 : -1);
+        or_solid_accel(c, 2);
+        or_solid_accel(c, 4);
+        or_art_visc(c);
+        or_bndy_stress(c);
+        if (c->pro.fli == 1) or_interact1(c);
+        or_velocity(c);
+        if (c->pro.if_move != 2) or_xsph(c);
+        if (c->pro.flbndy == 1) {
+            or_vel_repulse(c, 0.30, 0);
+        } else if (c->pro.flbndy == 2) {
+            or_vel_repulse(c, c->pro.coe_bndy, 1);
+            or_tm_boundary(c);
+        }
+        or_check_domain(c);
+        or_update(c);
+        c->l++;
+    }
+    return 0;
+}
+
+int or_get_neighbors(or_ctx *c, int *counts, int *ids, int n) {
+    int i, k;
+    if (n != c->n) return SPHG_E_ARG;
+    for (i = 0; i < n; i++) {
+        int nc = nbcount(&c->pc[i]);
+        counts[i] = c->pc[i].ninflu;
+        for (k = 0; k < SPHG_MAX_NB; k++) ids[(size_t)i * SPHG_MAX_NB + k] = k < nc ? c->pc[i].influ[k + 1] : -1;
+    }
+    return 0;
+}
+
+long long or_pair_count(or_ctx *c) {
+    long long s = 0;
+    for (int i = 0; i < c->n; i++) s += c->pc[i].ninflu;
+    return s;
+}

@@ -1,0 +1,45 @@
+/*
+ * Test infrastructure only — the oracle. A plain-C FP64 restatement of the reference solver's
+ * per-step particle loop (SPH_Calculation, CPP_1-SPHLoops.cpp:478-760), used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's CPU-baseline leg as the checker. Nothing in the
+ * product links or calls it.
+ *
+ * Layout follows the reference: an array of particle records, a per-particle neighbour table
+ * with 99 kernel-value slots, merge-sort binning, OpenMP static loops. With density mode
+ * OR_DENSITY_GS and one thread it reproduces the reference's serial run bit for bit (pinned
+ * against the reference link harness, oracle/_ref); OR_DENSITY_JACOBI evaluates the density
+ * normalisation from the pre-normalisation densities of all neighbours, the order-free form the
+ * GPU computes.
+ */
+#ifndef SPH_ORACLE_H
+#define SPH_ORACLE_H
+
+#include "../include/sphg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_DENSITY_GS = 0, OR_DENSITY_JACOBI = 1 };
+
+typedef struct or_ctx or_ctx;
+
+or_ctx *or_create(const sphg_problem *pro, const sphg_materials *mat, int n);
+void or_destroy(or_ctx *c);
+int or_set_field(or_ctx *c, const char *name, const double *v, int n, int ncomp);
+int or_get_field(or_ctx *c, const char *name, double *v, int n, int ncomp);
+int or_set_field_i(or_ctx *c, const char *name, const int *v, int n);
+int or_get_field_i(or_ctx *c, const char *name, int *v, int n);
+int or_upload_done(or_ctx *c);
+void or_set_mode(or_ctx *c, int density_mode, int nthreads);
+void or_set_step(or_ctx *c, int l);
+/* Runs nsteps reference steps; returns 0 or the reference error code (3, 5). */
+int or_step(or_ctx *c, int nsteps);
+int or_get_neighbors(or_ctx *c, int *counts, int *ids, int n);
+/* Sum over particles of ninflu in the last step (directed pairs). */
+long long or_pair_count(or_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

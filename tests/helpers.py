@@ -1,0 +1,119 @@
+"""Test-side helpers: the oracle library binding and the reference link harness runner.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use these (the oracle is
+the checker, never the product)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from typing import Dict, Iterable, Optional
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "multiphase-sph-solver_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from sphgpy.capi import (Problem, Materials, Sim, _Lib, problem_from_deck, materials_from_deck,  # noqa: E402
+                         PARTICLE_FIELDS, INT_FIELDS)
+from sphgpy.dumpio import read_dump  # noqa: E402
+
+ORACLE_SO = os.path.join(REPO, "oracle", "_build", "libsph_oracle.so")
+REF_HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+_or: Optional[_Lib] = None
+
+
+def oracle_lib() -> _Lib:
+    global _or
+    if _or is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "oracle"], check=True,
+                           capture_output=True)
+        lib = _Lib(ORACLE_SO, "or_")
+        L = lib.lib
+        L.or_create.argtypes = [C.POINTER(Problem), C.POINTER(Materials), C.c_int]
+        L.or_create.restype = C.c_void_p
+        L.or_destroy.argtypes = [C.c_void_p]
+        L.or_upload_done.argtypes = [C.c_void_p]
+        L.or_set_mode.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.or_set_step.argtypes = [C.c_void_p, C.c_int]
+        L.or_pair_count.argtypes = [C.c_void_p]
+        L.or_pair_count.restype = C.c_longlong
+        _or = lib
+    return _or
+
+
+GS, JACOBI = 0, 1
+
+
+class OracleSim(Sim):
+    def __init__(self, deck, state: Dict[str, np.ndarray], mode: int = GS, nthreads: int = 1, step: int = 1):
+        lib = oracle_lib()
+        self._pro = problem_from_deck(deck)
+        self._mat = materials_from_deck(deck)
+        ctx = lib.lib.or_create(C.byref(self._pro), C.byref(self._mat), deck.n)
+        super().__init__(lib, ctx, deck.n)
+        self.load_state(state)
+        for k, v in state.items():
+            if k.startswith("ps_") or k.startswith("pf_") or k in ("vx_water", "vx_soil", "vx_air", "vx_struct", "vel_bndy"):
+                self.set(k, v)
+        lib.lib.or_upload_done(ctx)
+        lib.lib.or_set_mode(ctx, mode, nthreads)
+        lib.lib.or_set_step(ctx, step)
+
+    def pair_count(self) -> int:
+        return int(self.L.lib.or_pair_count(self.ctx))
+
+    def close(self):
+        if self.ctx:
+            self.L.lib.or_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_HARNESS)
+
+
+def run_ref(deck, steps: int, fop: int, workdir: Optional[str] = None, ntd: int = 1) -> Dict[int, dict]:
+    """Run the reference solver (link harness) on `deck` for `steps` steps, dumping every `fop`
+    steps (and step 1). Returns {step: fields} including step 0 = state after setting_par."""
+    d = deck.copy()
+    d.loop, d.fop, d.ntd = steps, fop, ntd
+    wd = workdir or tempfile.mkdtemp(prefix="sphref_")
+    os.makedirs(wd, exist_ok=True)
+    d.write_input_dat(os.path.join(wd, "input.dat"))
+    r = subprocess.run([REF_HARNESS, wd], capture_output=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference harness failed rc={r.returncode}: {r.stdout[-400:]!r}")
+    out = {}
+    for fn in sorted(os.listdir(wd)):
+        if fn.startswith("dump_") and fn.endswith(".bin"):
+            f, step, _ = read_dump(os.path.join(wd, fn))
+            out[step] = f
+    if workdir is None:
+        shutil.rmtree(wd, ignore_errors=True)
+    return out
+
+
+def max_rel(a: np.ndarray, b: np.ndarray) -> float:
+    """Field-max normalised difference max|a-b| / max|b| (0 when both are all zero)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.max(np.abs(b))
+    num = np.max(np.abs(a - b)) if a.size else 0.0
+    if den == 0.0:
+        return float(num)
+    return float(num / den)
