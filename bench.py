@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of the per-step SPH particle loop on one MI355X.
+
+Default workload: 3D saturated soil-water column, 64x32x64 soil lattice under a 64x32x16 water
+layer at dx = 0.005 m plus 3-layer walls and floor (172,032 particles), Drucker-Prager soil,
+Darcy drag, plain FP64. Synthetic seeded lattice (no external data).
+
+One step = one whole reference time step (NNPS + all pair passes + constitutive + update) over
+all particles, state resident in HBM before the timed region starts. The timed region launches
+the step kernels eagerly with a HIP event pair around every kernel on the library's stream, so
+per-kernel durations are measured live; `value` is particle-steps/s over exactly --steps steps.
+
+Extra fields on the JSON line:
+  roofline      dominant kernel: algorithmic bytes per launch / measured average duration vs
+                8 TB/s; `traffic` = HBM bytes per launch from the committed rocprofv3 PMC summary
+                (profiles/pmc_<workload>.json) when present, else null.
+  cpu_baseline  the FP64 oracle (a restatement of the reference loop, validated bit-exact against
+                it) timed on this host's cores on a bounded sample of the same deck.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "multiphase-sph-solver_amd"))
+
+import numpy as np  # noqa: E402
+
+from sphgpy import column_collapse_2d, soil_water_column_3d  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) bytes per particle per kernel launch: every per-particle array the
+# kernel reads or writes counted once per particle (perfect reuse of gathered neighbour data),
+# plus 4 B per list entry for each pass over the neighbour list; W and grad W are recomputed.
+# (F64 reads, F64 writes, int32 reads/writes, neighbour-list passes), 3D / 2D.
+KERNEL_TRAFFIC = {
+    3: {
+        "bin": (3, 0, 2, 0), "scatter": (0, 0, 3, 0), "search": (4, 0, 3, 1),
+        "density": (15, 3, 3, 1), "norm_eos": (8, 2, 3, 1), "strain": (20, 13, 3, 1),
+        "stress": (31, 35, 3, 0), "accel": (26, 9, 4, 1), "xsph": (12, 6, 3, 1), "update": (21, 13, 2, 1),
+    },
+    2: {
+        "bin": (2, 0, 2, 0), "scatter": (0, 0, 3, 0), "search": (3, 0, 3, 1),
+        "density": (11, 3, 3, 1), "norm_eos": (6, 2, 3, 1), "strain": (15, 10, 3, 1),
+        "stress": (31, 35, 3, 0), "accel": (22, 7, 4, 1), "xsph": (9, 5, 3, 1), "update": (15, 10, 2, 1),
+    },
+}
+
+
+def kernel_bytes(ndim: int, name: str, n: int, nbar: float) -> float:
+    fr, fw, ii, passes = KERNEL_TRAFFIC[ndim].get(name, (0, 0, 0, 0))
+    return n * (8.0 * (fr + fw) + 4.0 * ii + passes * 4.0 * (nbar + 1))
+
+
+def step_bytes(ndim: int, n: int, nbar: float) -> float:
+    return sum(kernel_bytes(ndim, k, n, nbar) for k in KERNEL_TRAFFIC[ndim])
+
+
+def make_deck(workload: str):
+    if workload == "soilwater3d":
+        return soil_water_column_3d(steps=2000)
+    if workload == "column2d":
+        return column_collapse_2d(steps=5000)
+    raise SystemExit(f"unknown workload {workload}")
+
+
+WORKLOAD_DESC = {
+    "soilwater3d": "3D saturated soil-water column: soil 64x32x64 + water 64x32x16 lattice, dx=0.005 m, "
+                   "3-layer walls/floor, DP c=1 kPa phi=25, Darcy drag, water vpu=30 m/s",
+    "column2d": "2D dry granular column collapse: soil 60x120 at dx=0.002 m + 3-layer floor/left wall, "
+                "DP c=0 phi=30 E=1.8 MPa, AV alpha=0.1",
+}
+
+
+def cpu_baseline(deck, state, budget_s: float):
+    """The oracle restatement of the reference loop (Jacobi density form, OpenMP static loops)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import OracleSim, JACOBI  # the checker, used here only as the CPU baseline
+
+    threads = min(os.cpu_count() or 1, 16)
+    sim = OracleSim(deck, state, mode=JACOBI, nthreads=threads)
+    sim.step(1)  # warm caches / first-touch
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        rc = sim.step(1)
+        if rc:
+            raise RuntimeError(f"oracle step failed: {rc}")
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    return {"value": deck.n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {deck.n}-particle deck after 1 warm-up step, "
+                      f"{el:.1f} s wall, OpenMP static schedule"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="soilwater3d", choices=sorted(WORKLOAD_DESC))
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="time the hipGraph replay path (no per-kernel events)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from sphgpy.capi import GpuSim, setting_par
+
+    deck = make_deck(args.workload)
+    state = setting_par(deck)
+    t_setup = time.perf_counter()
+    g = GpuSim(deck)
+    g.load_state(state)
+    g.upload_done()
+    g.set_step(1)
+    rc = g.step(args.warmup)
+    if rc:
+        raise SystemExit(f"warm-up failed with error {rc}")
+    cnt, _ = g.neighbors()
+    nbar = float(cnt.mean())
+    g.enable_timing(not args.graph)
+    g.sync()
+    t0 = time.perf_counter()
+    rc = g.step(args.steps)  # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    if rc:
+        raise SystemExit(f"timed steps failed with error {rc}")
+    el = t1 - t0
+    n = deck.n
+    value = n * args.steps / el
+    ms_per_step = el * 1e3 / args.steps
+
+    rows = g.kernel_times() if not args.graph else []
+    roof = None
+    kernels = {}
+    if rows:
+        for name, ms, launches in rows:
+            kernels[name] = {"avg_us": ms * 1e3 / launches, "launches": launches}
+        dom = max((r for r in rows if r[0] in KERNEL_TRAFFIC[deck.ndim]), key=lambda r: r[1])
+        dname, dms, dl = dom
+        avg_s = dms / dl / 1e3
+        alg = kernel_bytes(deck.ndim, dname, n, nbar)
+        traffic = None
+        pmc_path = os.path.join(REPO, "profiles", f"pmc_{args.workload}.json")
+        if os.path.exists(pmc_path):
+            with open(pmc_path) as f:
+                pmc = json.load(f)
+            traffic = pmc.get("kernels", {}).get(dname, {}).get("hbm_bytes_per_launch")
+        achieved = alg / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dname,
+                "alg_bytes_per_launch": int(alg), "avg_launch_us": round(avg_s * 1e6, 2)}
+    step_alg = step_bytes(deck.ndim, n, nbar)
+
+    line = {
+        "metric": "particle-steps/s at 1 GPU, mixed vs FP64 at 1e-4 rel err; p99 time-to-result",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic seeded lattice deck",
+        "config": {"workload": args.workload, "description": WORKLOAD_DESC[args.workload], "particles": n,
+                   "precision": "FP64", "mean_neighbours": round(nbar, 2),
+                   "timing": "graph replay" if args.graph else "eager launches with per-kernel HIP events"},
+        "roofline": roof,
+        "pair_interactions_per_s": round(float(cnt.sum()) * args.steps / el, 1),
+        "step_alg_bytes": int(step_alg),
+        "step_roofline_frac": round(step_alg * args.steps / el / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(deck, state, args.cpu_budget)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

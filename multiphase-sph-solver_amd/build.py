@@ -1,0 +1,44 @@
+"""Builds the in-tree HIP step library multiphase-sph-solver_amd/_build/libsphg.so for gfx950.
+
+Run directly (python build.py) or through __graft_entry__.build(). FP contraction is off for all
+device and host code: the FP64 kernels are meant to reproduce the reference's CPU arithmetic.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT_DIR = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT_DIR, "libsphg.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OUT_DIR, exist_ok=True)
+    obj_cpp = os.path.join(OUT_DIR, "sph_init.o")
+    obj_hip = os.path.join(OUT_DIR, "sphg_capi.o")
+    cmds = [
+        ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-c", os.path.join(CSRC, "sph_init.cpp"),
+         "-o", obj_cpp],
+        [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "sphg_capi.hip"), "-o", obj_hip],
+        [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp],
+    ]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        r = subprocess.run(c, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"build failed: {' '.join(c[:3])} ...")
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

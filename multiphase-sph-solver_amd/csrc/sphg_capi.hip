@@ -1,0 +1,734 @@
+// Host side of the C-ABI (include/sphg.h): context, HBM layout, field transfer, step scheduling.
+//
+// One context owns one HBM pool holding every particle array (SoA), the cell list and the
+// neighbour lists, one HIP stream, and two captured hipGraphs (one per position/velocity buffer
+// parity) that replay a whole reference step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sphg.h"
+#include "sphg_kernels.hip"
+
+using namespace sphg;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Timing {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct sphg_ctx {
+    sphg_problem pro;
+    sphg_materials mat;
+    int cap = 0, n = 0, precision = 0;
+    Params P{};
+    State S{};
+    char *pool = nullptr;
+    size_t pool_bytes = 0;
+    hipStream_t stream = nullptr;
+    int cur = 0;        // which xp / vx buffer holds the current state
+    int step = 1;       // reference step counter l of the next step
+    bool uploaded = false;
+    bool graph_ok[2] = {false, false};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    std::vector<int> h_type, h_matype, h_etype;
+    bool timing = false;
+    std::vector<Timing> tq;
+    std::vector<std::pair<std::string, std::pair<double, int>>> tsum;
+    SoilSlot *d_soil = nullptr;
+    FluidSlot *d_fluid = nullptr;
+    double *d_psc = nullptr;
+    bool has_psc = false;
+    int scan_blocks = 0;
+    bool stepped = false;
+};
+
+#define HIPCHK(x)                                                               \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) {                                                 \
+            fprintf(stderr, "sphg: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+            return SPHG_E_HIP;                                                  \
+        }                                                                       \
+    } while (0)
+
+static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+// -------------------------------------------------------------------------------- creation
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat, int capacity,
+                           int precision) {
+    if (!out || !pro || !mat) return SPHG_E_ARG;
+    if (capacity <= 0) return SPHG_E_NOPARTICLES;
+    if (pro->ndim != 2 && pro->ndim != 3) return SPHG_E_ARG;
+    if (precision != SPHG_FP64) return SPHG_E_UNSUPPORTED;
+    sphg_ctx *c = new sphg_ctx();
+    c->pro = *pro;
+    c->mat = *mat;
+    c->cap = capacity;
+    c->precision = precision;
+    // cell grid (celllist_ini1, CPP_3-NNPS.cpp:96-122)
+    Params &P = c->P;
+    double cs = 2.0 * pro->dr;
+    P.cxm = (int)((pro->xmax - pro->xmin) / cs + 3.0);
+    P.cym = (int)((pro->ymax - pro->ymin) / cs + 3.0);
+    P.czm = pro->ndim == 2 ? 1 : (int)((pro->zmax - pro->zmin) / cs + 3.0);
+    long long ct = (long long)P.cxm * P.cym * P.czm;
+    if (ct <= 0 || ct > (1LL << 30)) {
+        delete c;
+        return SPHG_E_CAPACITY;
+    }
+    P.ncell = (int)ct;
+    P.cell_size = cs;
+    // nncell offsets in reference order (celllist_ini2, CPP_3-NNPS.cpp:125-171)
+    int p = 0;
+    if (pro->ndim == 2) {
+        for (int ky = 1; ky <= 3; ky++)
+            for (int kx = 1; kx <= 3; kx++) P.nb_off[p++] = (kx - 2) * P.cym + ky - 2;
+    } else {
+        for (int kz = 1; kz <= 3; kz++)
+            for (int kx = 1; kx <= 3; kx++)
+                for (int ky = 1; ky <= 3; ky++) P.nb_off[p++] = (ky - 2) * P.cxm * P.czm + (kx - 2) * P.czm + kz - 2;
+    }
+    P.nstencil = p;
+
+    // HBM layout
+    const size_t cap = (size_t)capacity;
+    const int ncellp = P.ncell + 2;
+    c->scan_blocks = (ncellp + kScanTile - 1) / kScanTile;
+    std::vector<std::pair<void **, size_t>> plan;
+    State &S = c->S;
+    auto D = [&](double *&ptr) { plan.push_back({(void **)&ptr, cap * 8}); };
+    auto I = [&](int *&ptr, size_t cnt) { plan.push_back({(void **)&ptr, cnt * 4}); };
+    for (int b = 0; b < 2; b++)
+        for (int k = 0; k < 3; k++) {
+            D(S.xp[b][k]);
+            D(S.vx[b][k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        D(S.vxp[k]); D(S.ax[k]); D(S.axp[k]); D(S.ux[k]); D(S.interf[k]); D(S.interfss[k]);
+        D(S.strep[k]); D(S.weps[k]); D(S.vel_bndy[k]);
+        for (int t = 0; t < 4; t++) D(S.vb[t][k]);
+    }
+    for (int k = 0; k < 6; k++) {
+        D(S.eps[k]); D(S.deps[k]); D(S.sig[k]); D(S.adps[k]);
+    }
+    for (int k = 0; k < 9; k++) D(S.beta[k]);
+    for (double **f : {&S.rho, &S.rho_mid, &S.rhop, &S.mass, &S.hl, &S.pre, &S.prep, &S.e, &S.satu, &S.dsatu,
+                       &S.suct, &S.cop, &S.vsig, &S.veps, &S.divq, &S.divr, &S.meps, &S.roue, &S.prew, &S.prea,
+                       &S.porosity, &S.cd, &S.poro_cop, &S.color})
+        D(*f);
+    plan.push_back({(void **)&c->d_psc, cap * 8});
+    I(S.type, cap); I(S.matype, cap); I(S.permtype, cap); I(S.etype, cap);
+    plan.push_back({(void **)&S.soil_slot, cap});
+    plan.push_back({(void **)&S.fluid_slot, cap});
+    I(S.cell, cap); I(S.sorted, cap); I(S.nb_cnt, cap);
+    I(S.nbr, cap * kMaxNb);
+    I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
+    I(S.block_sums, (size_t)c->scan_blocks + 1);
+    I(S.err, 1); I(S.step, 1);
+    plan.push_back({(void **)&c->d_soil, sizeof(SoilSlot) * kNSlots});
+    plan.push_back({(void **)&c->d_fluid, sizeof(FluidSlot) * 8});
+    size_t total = 0;
+    for (auto &e : plan) total += align256(e.second);
+    if (hipMalloc(&c->pool, total) != hipSuccess) {
+        delete c;
+        return SPHG_E_CAPACITY;
+    }
+    c->pool_bytes = total;
+    size_t off = 0;
+    for (auto &e : plan) {
+        *e.first = c->pool + off;
+        off += align256(e.second);
+    }
+    S.soil = c->d_soil;
+    S.fluid = c->d_fluid;
+    S.ps_c = nullptr;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        hipFree(c->pool);
+        delete c;
+        return SPHG_E_HIP;
+    }
+    // Particle() zero-initialises every member (Class_particle.cpp:12-99); do the same.
+    if (hipMemsetAsync(c->pool, 0, total, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        sphg_destroy(c);
+        return SPHG_E_HIP;
+    }
+    c->h_type.assign(cap, 0);
+    c->h_matype.assign(cap, 0);
+    c->h_etype.assign(cap, 0);
+    *out = c;
+    return SPHG_OK;
+}
+
+extern "C" void sphg_destroy(sphg_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int b = 0; b < 2; b++)
+        if (c->graph[b]) hipGraphExecDestroy(c->graph[b]);
+    for (auto &t : c->tq) {
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    if (c->pool) hipFree(c->pool);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int sphg_device_bytes(const sphg_ctx *c, size_t *bytes) {
+    if (!c || !bytes) return SPHG_E_ARG;
+    *bytes = c->pool_bytes;
+    return SPHG_OK;
+}
+
+// -------------------------------------------------------------------------------- fields
+// Returns the device component arrays of a named field (nullptr if unknown).
+static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, bool for_write,
+                      std::vector<double *> *mirror) {
+    State &S = c->S;
+    const int cur = c->cur;
+    std::string nm(name);
+    auto v3 = [&](double *const a[3]) { for (int k = 0; k < 3; k++) out[k] = a[k]; ncomp = 3; return 0; };
+    auto v6 = [&](double *const a[6]) { for (int k = 0; k < 6; k++) out[k] = a[k]; ncomp = 6; return 0; };
+    auto s1 = [&](double *a) { out[0] = a; ncomp = 1; return 0; };
+    if (nm == "xp") {
+        if (mirror) for (int k = 0; k < 3; k++) mirror->push_back(S.xp[cur ^ 1][k]);
+        return v3(S.xp[cur]);
+    }
+    if (nm == "vx") {
+        if (mirror) for (int k = 0; k < 3; k++) mirror->push_back(S.vx[cur ^ 1][k]);
+        return v3(S.vx[cur]);
+    }
+    if (nm == "vxp") return v3(S.vxp);
+    if (nm == "ax") return v3(S.ax);
+    if (nm == "axp") return v3(S.axp);
+    if (nm == "ux") return v3(S.ux);
+    if (nm == "interf") return v3(S.interf);
+    if (nm == "interfss") return v3(S.interfss);
+    if (nm == "strep") return v3(S.strep);
+    if (nm == "vel_bndy") return v3(S.vel_bndy);
+    if (nm == "vx_water") return v3(S.vb[0]);
+    if (nm == "vx_soil") return v3(S.vb[1]);
+    if (nm == "vx_air") return v3(S.vb[2]);
+    if (nm == "vx_struct") return v3(S.vb[3]);
+    if (nm == "eps") return v6(S.eps);
+    if (nm == "deps") return v6(S.deps);
+    if (nm == "sig") return v6(S.sig);
+    if (nm == "adps") return v6(S.adps);
+    if (nm == "beta") {
+        for (int k = 0; k < 9; k++) out[k] = S.beta[k];
+        ncomp = 9;
+        return 0;
+    }
+    struct { const char *n; double *p; } sc[] = {
+        {"rho", S.rho}, {"rhop", S.rhop}, {"mass", S.mass}, {"hl", S.hl}, {"pre", S.pre}, {"prep", S.prep},
+        {"e", S.e}, {"satu", S.satu}, {"dsatu", S.dsatu}, {"suct", S.suct}, {"cop", S.cop}, {"vsig", S.vsig},
+        {"veps", S.veps}, {"divq", S.divq}, {"divr", S.divr}, {"meps", S.meps}, {"roue", S.roue},
+        {"prew", S.prew}, {"prea", S.prea}, {"porosity", S.porosity}, {"cd", S.cd}, {"poro_cop", S.poro_cop},
+        {"color", S.color}, {"ps_c", c->d_psc}};
+    for (auto &e : sc)
+        if (nm == e.n) return s1(e.p);
+    return -1;
+}
+
+// Fields derived on download from what the device keeps (weps: 3 stored components; beta_ts:
+// transpose of beta; x: equal to xp for every particle the update moves).
+extern "C" int sphg_get_field(sphg_ctx *c, const char *name, double *host, int n, int ncomp) {
+    if (!c || !name || !host || n != c->n) return SPHG_E_ARG;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return SPHG_E_HIP;
+    std::string nm(name);
+    const size_t N = (size_t)n;
+    std::vector<double> buf(N);
+    auto fetch = [&](double *d, std::vector<double> &h) {
+        h.resize(N);
+        return hipMemcpy(h.data(), d, N * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    if (nm == "weps" || nm == "beta_ts" || nm == "x" || nm == "divx" || nm == "dsig") {
+        if (nm == "weps") {
+            if (ncomp != 9) return SPHG_E_ARG;
+            std::vector<double> w[3];
+            for (int k = 0; k < 3; k++)
+                if (!fetch(c->S.weps[k], w[k])) return SPHG_E_HIP;
+            for (size_t i = 0; i < N; i++) {
+                double a = w[0][i], b = w[1][i], d = w[2][i];
+                auto ng = [](double x) { return x == 0.0 ? 0.0 : -x; };
+                double m[9] = {0.0, a, b, ng(a), 0.0, d, ng(b), ng(d), 0.0};
+                memcpy(host + 9 * i, m, 72);
+            }
+            return SPHG_OK;
+        }
+        if (nm == "beta_ts") {
+            if (ncomp != 9) return SPHG_E_ARG;
+            std::vector<double> b[9];
+            for (int k = 0; k < 9; k++)
+                if (!fetch(c->S.beta[k], b[k])) return SPHG_E_HIP;
+            for (size_t i = 0; i < N; i++)
+                for (int r = 0; r < 3; r++)
+                    for (int q = 0; q < 3; q++) host[9 * i + 3 * r + q] = b[3 * q + r][i];
+            return SPHG_OK;
+        }
+        if (nm == "x") {
+            if (ncomp != 3) return SPHG_E_ARG;
+            std::vector<double> x[3];
+            for (int k = 0; k < 3; k++)
+                if (!fetch(c->S.xp[c->cur][k], x[k])) return SPHG_E_HIP;
+            for (size_t i = 0; i < N; i++) {
+                int t = c->h_type[i], m = c->h_matype[i];
+                bool moved = c->stepped && (t != 0 || (m >= 2 && m <= 5));
+                for (int k = 0; k < 3; k++)
+                    host[3 * i + k] = (moved && k < c->pro.ndim) ? x[k][i] : 0.0;
+            }
+            return SPHG_OK;
+        }
+        return SPHG_E_ARG;  // divx / dsig are per-step temporaries not kept in HBM
+    }
+    double *ptr[9];
+    int nc = 0;
+    if (field_ptrs(c, name, ptr, nc, false, nullptr) != 0 || nc != ncomp) return SPHG_E_ARG;
+    for (int k = 0; k < nc; k++) {
+        if (!fetch(ptr[k], buf)) return SPHG_E_HIP;
+        for (size_t i = 0; i < N; i++) host[nc * i + k] = buf[i];
+    }
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host, int n, int ncomp) {
+    if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
+    if (c->n == 0) c->n = n;
+    if (n != c->n) return SPHG_E_ARG;
+    std::string nm(name);
+    const size_t N = (size_t)n;
+    if (nm == "x" || nm == "divx" || nm == "dsig" || nm == "beta_ts" || nm == "weps" || nm == "rous" ||
+        nm.rfind("pf_", 0) == 0 || (nm.rfind("ps_", 0) == 0 && nm != "ps_c"))
+        return SPHG_OK;  // derived, per-step temporary, or taken from the material slots
+    double *ptr[9];
+    int nc = 0;
+    std::vector<double *> mirror;
+    if (field_ptrs(c, name, ptr, nc, true, &mirror) != 0 || nc != ncomp) return SPHG_E_ARG;
+    std::vector<double> buf(N);
+    for (int k = 0; k < nc; k++) {
+        for (size_t i = 0; i < N; i++) buf[i] = host[nc * i + k];
+        HIPCHK(hipMemcpy(ptr[k], buf.data(), N * 8, hipMemcpyHostToDevice));
+        if (!mirror.empty()) HIPCHK(hipMemcpy(mirror[k], buf.data(), N * 8, hipMemcpyHostToDevice));
+    }
+    if (nm == "ps_c") c->has_psc = true;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_field_i(sphg_ctx *c, const char *name, const int *host, int n) {
+    if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
+    if (c->n == 0) c->n = n;
+    if (n != c->n) return SPHG_E_ARG;
+    std::string nm(name);
+    int *d = nullptr;
+    if (nm == "type") { d = c->S.type; std::copy(host, host + n, c->h_type.begin()); }
+    else if (nm == "matype") { d = c->S.matype; std::copy(host, host + n, c->h_matype.begin()); }
+    else if (nm == "permtype") d = c->S.permtype;
+    else if (nm == "etype") { d = c->S.etype; std::copy(host, host + n, c->h_etype.begin()); }
+    else return SPHG_E_ARG;
+    HIPCHK(hipMemcpy(d, host, (size_t)n * 4, hipMemcpyHostToDevice));
+    c->uploaded = false;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_get_field_i(sphg_ctx *c, const char *name, int *host, int n) {
+    if (!c || !name || !host || n != c->n) return SPHG_E_ARG;
+    std::string nm(name);
+    int *d = nm == "type" ? c->S.type : nm == "matype" ? c->S.matype : nm == "permtype" ? c->S.permtype
+             : nm == "etype" ? c->S.etype : nullptr;
+    if (!d) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, d, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return SPHG_OK;
+}
+
+// setting_vis_cons slot selection (CPP_2-initialf.cpp:1021-1058): the per-particle Para_Soil /
+// Para_Fluid copies of the reference are rows of the material tables, referenced by index.
+static int soil_slot_of(int type, int matype, int etype) {
+    if (type == 2) {
+        if (matype < 30 && matype >= 0) return matype;
+        if (matype >= 50) return matype % 10;
+        return 30;
+    }
+    if (type == 7 && etype == 2 && matype >= 0 && matype < 30) return matype;
+    return 30;  // default-constructed Para_Soil
+}
+static int fluid_slot_of(int type, int matype, int etype) {
+    if ((type == 1 || (type == 7 && etype == 1)) && matype >= 0 && matype < 3) return matype;
+    if (type == 3) return 3;
+    return 4;  // default-constructed Para_Fluid
+}
+
+static int check_supported(const sphg_problem &p) {
+    if (p.flknl > 2) return SPHG_E_UNSUPPORTED;  // kernel_gradient_correction
+    if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
+    if (p.flagum >= 1) return SPHG_E_UNSUPPORTED;
+    if (p.fli > 1 || p.ss_type != 0) return SPHG_E_UNSUPPORTED;
+    if (p.art_stre_coe > 0.0 || p.step_regu <= p.loop) return SPHG_E_UNSUPPORTED;
+    if (p.flbndy != 1 && p.flbndy != 2) return SPHG_E_UNSUPPORTED;
+    if (p.if_move != 0) return SPHG_E_UNSUPPORTED;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
+    if (!c || n != c->n) return SPHG_E_ARG;
+    int rc = check_supported(c->pro);
+    if (rc) return rc;
+    Params &P = c->P;
+    const sphg_problem &pro = c->pro;
+    P.n = n;
+    P.cap = c->cap;
+    P.ndim = pro.ndim;
+    P.flbndy = pro.flbndy;
+    P.flknl = pro.flknl;
+    P.fli = pro.fli;
+    P.if_move = pro.if_move;
+    P.flv = pro.flv;
+    P.inip = pro.inip;
+    P.ss_type = pro.ss_type;
+    P.flts = pro.flts;
+    P.dt = pro.dt;
+    P.dr = pro.dr;
+    P.xmin = pro.xmin;
+    P.ymin = pro.ymin;
+    P.zmin = pro.zmin;
+    P.bmin[0] = pro.xmin; P.bmin[1] = pro.ymin; P.bmin[2] = pro.zmin;
+    P.bmax[0] = pro.xmax; P.bmax[1] = pro.ymax; P.bmax[2] = pro.zmax;
+    P.gx = pro.gx; P.gy = pro.gy; P.gz = pro.gz;
+    P.gacc = sqrt(pro.gx * pro.gx + pro.gy * pro.gy + pro.gz * pro.gz);
+    for (int k = 0; k < 3; k++) {
+        P.xsph[k] = pro.xsph[k];
+        P.art_vis[k][0] = pro.art_vis[k][0];
+        P.art_vis[k][1] = pro.art_vis[k][1];
+    }
+    P.coe_bndy = pro.coe_bndy;
+    P.interc = pro.interc;
+    P.permtype = pro.permtype;
+    P.pers = pro.pers; P.perl = pro.perl; P.perm = pro.perm;
+    P.tmin = pro.threshold_min;
+    P.tmax = pro.threshold_max;
+    P.struct_e = c->mat.structure.e;
+    P.struct_poi = c->mat.structure.poi;
+    P.water_acc_minor = c->mat.water[0].acc_type != 0;
+    P.air_acc_minor = c->mat.air.acc_type != 0;
+    P.first_moving_bndy = n;
+    for (int i = 0; i < n; i++)
+        if (c->h_type[i] == 0 && c->h_matype[i] > 2) {
+            P.first_moving_bndy = i;
+            break;
+        }
+    // material slots with host-evaluated transcendental constants
+    std::vector<SoilSlot> soil(kNSlots);
+    for (int s = 0; s < kNSlots; s++) {
+        sphg_soil d{};
+        if (s < SPHG_NSOIL) d = c->mat.soil[s];
+        else { d.fs = 1.0; d.ds = 1.0; }
+        SoilSlot &o = soil[s];
+        o.e = d.e; o.poi = d.poi; o.eps0 = d.eps0; o.dens = d.dens; o.kapa = d.kapa; o.zmf = d.zmf; o.ann = d.ann;
+        o.ramda = d.ramda; o.fs = d.fs; o.c = d.c; o.sme0 = d.sme0; o.porosity = d.porosity; o.vpu = d.vpu;
+        o.cop = d.cop; o.sitas = d.sitas; o.sitar = d.sitar; o.damp_coe = d.damp_coe; o.beta = d.beta;
+        double fai = atan(tan(d.fai * 0.0174532925166667) / d.fs);  // CPP_8-1-consfactor.cpp:294
+        o.dp_sq = sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
+        o.dp_yalf = sin(fai) / o.dp_sq;
+        o.dp_cosf = cos(fai);
+    }
+    std::vector<FluidSlot> fl(8);
+    for (int s = 0; s < 8; s++) {
+        sphg_fluid d{};
+        d.strain_min = 1.0;
+        if (s < 3) d = c->mat.water[s];
+        else if (s == 3) d = c->mat.air;
+        FluidSlot &o = fl[s];
+        o.vis = d.vis; o.vpu = d.vpu; o.dens = d.dens; o.porosity = d.porosity; o.c = d.c; o.pre_coe = d.pre_coe;
+        o.strain_min = d.strain_min;
+        o.tan_fai = tan(d.fai * 0.0174532925199);  // CPP_7-2-constitutive.cpp:42-43
+        o.bhtype = d.bhtype;
+        o.acc_type = d.acc_type;
+    }
+    HIPCHK(hipMemcpy(c->d_soil, soil.data(), sizeof(SoilSlot) * kNSlots, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_fluid, fl.data(), sizeof(FluidSlot) * 8, hipMemcpyHostToDevice));
+    std::vector<int8_t> ss(n), fs(n);
+    for (int i = 0; i < n; i++) {
+        ss[i] = (int8_t)soil_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
+        fs[i] = (int8_t)fluid_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
+    }
+    HIPCHK(hipMemcpy(c->S.soil_slot, ss.data(), n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->S.fluid_slot, fs.data(), n, hipMemcpyHostToDevice));
+    c->S.ps_c = c->has_psc ? c->d_psc : nullptr;
+    int zero = 0;
+    HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->S.step, &c->step, 4, hipMemcpyHostToDevice));
+    for (int b = 0; b < 2; b++) {
+        if (c->graph[b]) hipGraphExecDestroy(c->graph[b]);
+        c->graph[b] = nullptr;
+        c->graph_ok[b] = false;
+    }
+    c->uploaded = true;
+    return SPHG_OK;
+}
+
+// -------------------------------------------------------------------------------- launches
+static void tbegin(sphg_ctx *c, const char *name, Timing &t) {
+    if (!c->timing) return;
+    t.name = name;
+    hipEventCreate(&t.a);
+    hipEventCreate(&t.b);
+    hipEventRecord(t.a, c->stream);
+}
+static void tend(sphg_ctx *c, Timing &t) {
+    if (!c->timing) return;
+    hipEventRecord(t.b, c->stream);
+    c->tq.push_back(t);
+}
+
+#define LAUNCH(name, kern, grid, ...)                                          \
+    do {                                                                       \
+        Timing t_;                                                             \
+        tbegin(c, name, t_);                                                   \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c->stream, __VA_ARGS__); \
+        tend(c, t_);                                                           \
+    } while (0)
+
+template <int NDIM>
+static void enqueue_nnps(sphg_ctx *c) {
+    const int cur = c->cur, n = c->n, M = c->P.ncell + 2;
+    LAUNCH("fill_counts", k_fill_int, grid_for(M), c->S.cell_count, M, 0);
+    LAUNCH("fill_cursor", k_fill_int, grid_for(M), c->S.cell_cursor, M, 0);
+    LAUNCH("bin", k_bin<NDIM>, grid_for(n), c->S, c->P, cur);
+    LAUNCH("scan_tiles", k_scan_tiles, c->scan_blocks, c->S.cell_count, c->S.cell_start, c->S.block_sums, M);
+    {
+        Timing t_;
+        tbegin(c, "scan_top", t_);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, c->stream, c->S.block_sums, c->scan_blocks);
+        tend(c, t_);
+    }
+    LAUNCH("scan_add", k_scan_add, c->scan_blocks, c->S.cell_start, c->S.block_sums, M);
+    LAUNCH("scatter", k_scatter, grid_for(n), c->S, c->P);
+    LAUNCH("cell_order", k_cell_order, grid_for(M), c->S, M);
+    LAUNCH("search", k_search<NDIM>, grid_for(n), c->S, c->P, cur);
+}
+
+template <int NDIM>
+static void enqueue_step(sphg_ctx *c, int phase_mask) {
+    const int cur = c->cur, n = c->n;
+    if (phase_mask & 1) enqueue_nnps<NDIM>(c);
+    if (phase_mask & 2) {
+        LAUNCH("density", k_density<NDIM>, grid_for(n), c->S, c->P, cur);
+        LAUNCH("norm_eos", k_norm<NDIM>, grid_for(n), c->S, c->P, cur);
+    }
+    if (phase_mask & 4) LAUNCH("strain", k_strain<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (phase_mask & 8) LAUNCH("stress", k_stress, grid_for(n), c->S, c->P);
+    if (phase_mask & 16) LAUNCH("accel", k_accel<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (phase_mask & 32) LAUNCH("xsph", k_xsph<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (phase_mask & 64) {
+        LAUNCH("update", k_update<NDIM>, grid_for(n), c->S, c->P, cur);
+        LAUNCH("step_inc", k_step_inc, 1, c->S);
+    }
+}
+
+static void enqueue(sphg_ctx *c, int mask) {
+    if (c->P.ndim == 2) enqueue_step<2>(c, mask);
+    else enqueue_step<3>(c, mask);
+}
+
+static int read_err(sphg_ctx *c) {
+    int e = 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&e, c->S.err, 4, hipMemcpyDeviceToHost));
+    if (e & SPHG_ERR_CELL) return SPHG_E_CELLRANGE;
+    if (e & SPHG_ERR_NB) return SPHG_E_NEIGHBOURS;
+    if (e & 4) return SPHG_E_UNSUPPORTED;
+    return SPHG_OK;
+}
+
+static int finish_timing(sphg_ctx *c) {
+    if (!c->timing || c->tq.empty()) return 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto &t : c->tq) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        bool found = false;
+        for (auto &s : c->tsum)
+            if (s.first == t.name) {
+                s.second.first += ms;
+                s.second.second += 1;
+                found = true;
+            }
+        if (!found) c->tsum.push_back({t.name, {ms, 1}});
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    c->tq.clear();
+    return 0;
+}
+
+// Individual phases (each advances nothing but its own outputs; sphg_update advances l).
+static int phase(sphg_ctx *c, int mask) {
+    if (!c || !c->uploaded) return SPHG_E_ARG;
+    enqueue(c, mask);
+    if (mask & 64) {
+        c->cur ^= 1;
+        c->step++;
+        c->stepped = true;
+    }
+    finish_timing(c);
+    return read_err(c);
+}
+
+extern "C" int sphg_parttocell(sphg_ctx *c) {
+    if (!c || !c->uploaded) return SPHG_E_ARG;
+    // binning only: the search is issued by sphg_partisearching
+    const int n = c->n, M = c->P.ncell + 2, cur = c->cur;
+    auto go = [&](auto ndim_tag) {
+        constexpr int NDIM = decltype(ndim_tag)::value;
+        LAUNCH("fill_counts", k_fill_int, grid_for(M), c->S.cell_count, M, 0);
+        LAUNCH("fill_cursor", k_fill_int, grid_for(M), c->S.cell_cursor, M, 0);
+        LAUNCH("bin", k_bin<NDIM>, grid_for(n), c->S, c->P, cur);
+        LAUNCH("scan_tiles", k_scan_tiles, c->scan_blocks, c->S.cell_count, c->S.cell_start, c->S.block_sums, M);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, c->stream, c->S.block_sums, c->scan_blocks);
+        LAUNCH("scan_add", k_scan_add, c->scan_blocks, c->S.cell_start, c->S.block_sums, M);
+        LAUNCH("scatter", k_scatter, grid_for(n), c->S, c->P);
+        LAUNCH("cell_order", k_cell_order, grid_for(M), c->S, M);
+    };
+    if (c->P.ndim == 2) go(std::integral_constant<int, 2>());
+    else go(std::integral_constant<int, 3>());
+    finish_timing(c);
+    return read_err(c);
+}
+extern "C" int sphg_partisearching(sphg_ctx *c) {
+    if (!c || !c->uploaded) return SPHG_E_ARG;
+    const int n = c->n, cur = c->cur;
+    if (c->P.ndim == 2) LAUNCH("search", k_search<2>, grid_for(n), c->S, c->P, cur);
+    else LAUNCH("search", k_search<3>, grid_for(n), c->S, c->P, cur);
+    finish_timing(c);
+    return read_err(c);
+}
+extern "C" int sphg_density(sphg_ctx *c) { return phase(c, 2); }
+extern "C" int sphg_strain(sphg_ctx *c) { return phase(c, 4); }
+extern "C" int sphg_strain_to_stress(sphg_ctx *c) { return phase(c, 8); }
+extern "C" int sphg_accel(sphg_ctx *c) { return phase(c, 16); }
+extern "C" int sphg_xsph(sphg_ctx *c) { return phase(c, 32); }
+extern "C" int sphg_update(sphg_ctx *c) { return phase(c, 64); }
+
+static int capture(sphg_ctx *c, int parity) {
+    hipGraph_t g = nullptr;
+    int saved = c->cur;
+    c->cur = parity;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue(c, 127);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->cur = saved;
+    if (e != hipSuccess) return SPHG_E_HIP;
+    e = hipGraphInstantiate(&c->graph[parity], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return SPHG_E_HIP;
+    c->graph_ok[parity] = true;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
+    if (!c || !c->uploaded || nsteps < 0) return SPHG_E_ARG;
+    if (c->timing) {
+        for (int s = 0; s < nsteps; s++) {
+            enqueue(c, 127);
+            c->cur ^= 1;
+            c->step++;
+        }
+        c->stepped = c->stepped || nsteps > 0;
+        finish_timing(c);
+        return read_err(c);
+    }
+    for (int s = 0; s < nsteps; s++) {
+        int par = c->cur;
+        if (!c->graph_ok[par]) {
+            int rc = capture(c, par);
+            if (rc) return rc;
+        }
+        HIPCHK(hipGraphLaunch(c->graph[par], c->stream));
+        c->cur ^= 1;
+        c->step++;
+    }
+    c->stepped = c->stepped || nsteps > 0;
+    return read_err(c);
+}
+
+extern "C" int sphg_sync(sphg_ctx *c) {
+    if (!c) return SPHG_E_ARG;
+    return read_err(c);
+}
+
+extern "C" int sphg_current_step(const sphg_ctx *c) { return c ? c->step : -1; }
+
+extern "C" int sphg_set_step(sphg_ctx *c, int l) {
+    if (!c) return SPHG_E_ARG;
+    c->step = l;
+    HIPCHK(hipMemcpy(c->S.step, &l, 4, hipMemcpyHostToDevice));
+    return SPHG_OK;
+}
+
+extern "C" int sphg_get_neighbors(sphg_ctx *c, int *counts, int *ids, int n) {
+    if (!c || !counts || !ids || n != c->n) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(counts, c->S.nb_cnt, (size_t)n * 4, hipMemcpyDeviceToHost));
+    std::vector<int> col((size_t)n);
+    for (int k = 0; k < kMaxNb; k++) {
+        HIPCHK(hipMemcpy(col.data(), c->S.nbr + (size_t)k * c->cap, (size_t)n * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) {
+            int cnt = std::min(counts[i], kMaxNb);
+            ids[(size_t)i * kMaxNb + k] = k < cnt ? col[i] : -1;
+        }
+    }
+    return SPHG_OK;
+}
+
+extern "C" int sphg_enable_timing(sphg_ctx *c, int on) {
+    if (!c) return SPHG_E_ARG;
+    c->timing = on != 0;
+    c->tsum.clear();
+    return SPHG_OK;
+}
+
+extern "C" int sphg_kernel_times(sphg_ctx *c, char (*names)[32], double *ms, int *launches, int cap) {
+    if (!c) return 0;
+    finish_timing(c);
+    int k = 0;
+    for (auto &s : c->tsum) {
+        if (k >= cap) break;
+        memset(names[k], 0, 32);
+        strncpy(names[k], s.first.c_str(), 31);
+        ms[k] = s.second.first;
+        launches[k] = s.second.second;
+        k++;
+    }
+    return k;
+}
+
+extern "C" const char *sphg_error_string(int code) {
+    switch (code) {
+    case SPHG_OK: return "ok";
+    case SPHG_E_NOPARTICLES: return "no particles (reference error 2)";
+    case SPHG_E_CELLRANGE: return "particle outside the cell grid (reference error 3)";
+    case SPHG_E_NEIGHBOURS: return "more than 99 neighbours (reference error 5)";
+    case SPHG_E_ARG: return "bad argument";
+    case SPHG_E_HIP: return "HIP runtime error";
+    case SPHG_E_UNSUPPORTED: return "flag combination not implemented";
+    case SPHG_E_CAPACITY: return "capacity exceeded";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char *sphg_build_info(void) {
+    return "sphg gfx950 FP64 step library (contraction off)";
+}

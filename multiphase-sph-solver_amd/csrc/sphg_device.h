@@ -1,0 +1,131 @@
+// Device-side data layout and helpers of the SPH step kernels (gfx950).
+//
+// State lives in HBM as structure-of-arrays: one contiguous FP64 array per scalar component
+// (a 3-vector is three arrays), so a wavefront of 64 consecutive particles reads 512 B per
+// component, fully coalesced. Neighbour lists are stored column-major, nbr[k * cap + i], for
+// the same reason. Kernel values W and grad W are never stored: every pair loop recomputes them
+// from the FP64 positions with the reference's own expression, which costs ~40 FP64 operations
+// per pair against 32 B per pair per pass of HBM traffic that a stored table would add.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sphg {
+
+constexpr int kMaxNb = 99;
+constexpr int kNSlots = 32;  // 30 soil slots + default row (index 30) + padding
+
+// Per-slot material constants. Loop-invariant transcendental terms (atan/tan/sin/cos of the
+// friction angle) are evaluated once on the host with the C library so that the device
+// arithmetic that consumes them reproduces the reference's FP64 values exactly.
+struct SoilSlot {
+    double e, poi, eps0, dens, kapa, zmf, ann, ramda, fs, c, sme0, porosity, vpu, cop, sitas, sitar;
+    double damp_coe, beta;
+    double dp_yalf;   // sin(phi') / sqrt(9 + 3 sin^2 phi'), phi' = atan(tan(phi)/fs)  (CPP_8-1-consfactor.cpp:346)
+    double dp_cosf;   // cos(phi')
+    double dp_sq;     // sqrt(9 + 3 sin^2 phi')
+};
+
+struct FluidSlot {
+    double vis, vpu, dens, porosity, c, pre_coe, strain_min;
+    double tan_fai;   // tan(fai * 0.0174532925199) for the Bingham branch (CPP_7-2-constitutive.cpp:42-43)
+    int bhtype, acc_type;
+};
+
+struct Params {
+    int n, cap, ndim, flbndy, flknl, fli, if_move, flv, inip, ss_type, flts;
+    int ncell;            // ctotal (reference cell ids run 1..ctotal)
+    int cxm, cym, czm;
+    int first_moving_bndy;  // lowest index of a type-0 particle with matype > 2, or n
+    int water_acc_minor, air_acc_minor;
+    double dt, dr, cell_size, xmin, ymin, zmin;
+    double bmin[3], bmax[3];
+    double gx, gy, gz, gacc;
+    double xsph[3];
+    double art_vis[3][2];
+    double coe_bndy, interc;
+    int permtype;
+    double pers, perl, perm;
+    double tmin, tmax;
+    double struct_e, struct_poi;
+    int nb_off[27];       // stencil offsets in the reference's nncell order
+    int nstencil;
+};
+
+// All device arrays of one context (pointers into one HBM allocation).
+struct State {
+    // kinematics (ping-pong xp / vx by step parity)
+    double *xp[2][3], *vx[2][3];
+    double *vxp[3], *ax[3], *axp[3], *ux[3], *interf[3], *interfss[3];
+    double *rho, *rho_mid, *rhop, *mass, *hl, *pre, *prep, *e;
+    double *eps[6], *deps[6], *sig[6], *adps[6];
+    double *weps[3];      // w01, w02, w12 of the antisymmetric spin tensor
+    double *satu, *dsatu, *suct, *cop, *vsig, *veps, *divq, *divr, *meps, *roue, *prew, *prea;
+    double *porosity, *cd, *poro_cop, *color;
+    double *strep[3], *beta[9];
+    double *vb[4][3];     // clVar_Boundary vx_water/soil/air/struct
+    double *vel_bndy[3];  // cl_bndy_pair
+    double *ps_c;         // per-particle cohesion (random fields), or null -> slot value
+    int *type, *matype, *permtype, *etype;
+    int8_t *soil_slot, *fluid_slot;
+    // cell list
+    int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *block_sums;
+    int *nb_cnt, *nbr;
+    int *err, *step;      // sticky error word; current step number l
+    const SoilSlot *soil;
+    const FluidSlot *fluid;
+};
+
+#define SPHG_ERR_CELL 1
+#define SPHG_ERR_NB 2
+
+// weight(): cubic B-spline, CPP_3-NNPS.cpp:17-48 (same operation order; 0.66666667 as written there).
+__device__ __forceinline__ void weight_b(double dst, double ra, const double dx[3], double w[4], int ndim) {
+    double a1 = 0.0, s = dst / ra;
+    if (ndim == 2) {
+        a1 = 0.682092613250980 / ra / ra;
+        w[2] = 0.0;
+    } else {
+        a1 = 0.477464829275686 / ra / ra / ra;
+    }
+    if (s == 0.0) {
+        for (int k = 0; k < ndim; k++) w[k] = 0.0;
+        w[3] = a1 * (0.66666667 - s * s + 0.5 * s * s * s);
+    }
+    if (s > 0.0 && s <= 1.0) {
+        double c = (-2 * s + 1.5 * s * s), rd = ra * dst;
+        for (int k = 0; k < ndim; k++) w[k] = a1 * dx[k] * c / rd;
+        w[3] = a1 * (0.66666667 - s * s + 0.5 * s * s * s);
+    } else if (s > 1.0 && s <= 2.0) {
+        double rd = ra * dst;
+        for (int k = 0; k < ndim; k++) w[k] = a1 * (-0.5) * dx[k] * (2.0 - s) * (2.0 - s) / rd;
+        w[3] = a1 * (2.0 - s) * (2.0 - s) * (2.0 - s) / 6.0;
+    }
+}
+
+// weight2(): Yang & Liu kernel, CPP_3-NNPS.cpp:51-82 (flknl 2).
+__device__ __forceinline__ void weight_y(double dst, double ra, const double dx[3], double w[4], int ndim) {
+    double a1 = 0.0, s = dst / ra;
+    if (ndim == 2) {
+        a1 = 0.106103295394597 / (ra * ra);
+        w[2] = 0.0;
+    } else {
+        a1 = 0.077010456334788 / (ra * ra * ra);
+    }
+    if (s == 0.0) {
+        for (int k = 0; k < ndim; k++) w[k] = 0.0;
+        w[3] = a1 * (6.0 - 6.0 * s + 1.0 * s * s * s);
+    }
+    if (s > 0.0 && s <= 1.0) {
+        double rd = ra * dst;
+        for (int k = 0; k < ndim; k++) w[k] = a1 * dx[k] * (-6.0 + 3.0 * s * s) / rd;
+        w[3] = a1 * (6.0 - 6.0 * s + 1.0 * s * s * s);
+    } else if (s > 1.0 && s <= 2.0) {
+        double rd = ra * dst;
+        for (int k = 0; k < ndim; k++) w[k] = a1 * (-3.0) * dx[k] * (2.0 - s) * (2.0 - s) / rd;
+        w[3] = a1 * (2.0 - s) * (2.0 - s) * (2.0 - s);
+    }
+}
+
+}  // namespace sphg

@@ -1,0 +1,1327 @@
+// HIP kernels of the per-step particle loop (gfx950, FP64 form).
+//
+// One step = NNPS (bin, scan, scatter, in-cell order, list build) + six fused pair passes +
+// one per-particle pass, in the reference's phase order (CPP_1-SPHLoops.cpp:524-753):
+//   k_density   density loops 1-2               CPP_6-density.cpp:22-166
+//   k_norm      density loop 3 (Jacobi) + eos 1 CPP_6-density.cpp:168-188, CPP_7-1-eos_CR.cpp:20-43
+//   k_strain    strain + eos 2 + colour sums    CPP_7-1-strain.cpp:20-220, CPP_7-1-eos_CR.cpp:45-177,
+//                                               CPP_7-2-constitutive.cpp:453-480, :355-382
+//   k_stress    constitutive + free surface + eig  CPP_7-2-constitutive.cpp:13-321, :481-505,
+//                                               CPP_7-1-strain.cpp:283-437
+//   k_accel     accelerations + AV + boundary stress + drag + velocity
+//                                               CPP_9-4-derivation_stress.cpp:12-721,
+//                                               CPP_9-3-artificialvis.cpp, CPP_10-interact1.cpp:15-110,
+//                                               CPP_11-xsph.cpp:11-57
+//   k_xsph      XSPH + velocity_set_domain + tm_boundary loop 1   CPP_11-xsph.cpp:145-256,
+//                                               CPP_13-boundary.cpp:71-283, :388-520
+//   k_update    tm_boundary loop 2 + check_domain + update  CPP_13-boundary.cpp:284-386, :20-68,
+//                                               CPP_12-1-update.cpp:11-55
+// Every per-particle accumulation runs over the neighbour list in the reference's order with the
+// reference's operation order; the file is compiled with FP contraction off, so with the same
+// inputs each sum is bit-identical to the CPU result. Passes that the reference runs in place
+// (density normalisation, position update) read a previous buffer here (Jacobi form).
+#include "sphg_device.h"
+
+#include <math.h>
+
+namespace sphg {
+
+__device__ __forceinline__ int step_of(const State &S) { return *S.step; }
+
+__device__ __forceinline__ void set_err(const State &S, int bit) { atomicOr(S.err, bit); }
+
+template <int NDIM>
+__device__ __forceinline__ void load_x(const State &S, int cur, int i, double x[3]) {
+    x[0] = S.xp[cur][0][i];
+    x[1] = S.xp[cur][1][i];
+    x[2] = NDIM == 3 ? S.xp[cur][2][i] : 0.0;
+}
+
+// Pair geometry and kernel values exactly as partisearching computes them (CPP_3-NNPS.cpp:300-316).
+template <int NDIM>
+__device__ __forceinline__ void pair_w(const State &S, const Params &P, int cur, const double xi[3], double hli,
+                                       int j, double xj[3], double dx[3], double w[4]) {
+    xj[0] = S.xp[cur][0][j];
+    xj[1] = S.xp[cur][1][j];
+    xj[2] = NDIM == 3 ? S.xp[cur][2][j] : 0.0;
+    double dst = 0.0;
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        dst = dst + (xi[k] - xj[k]) * (xi[k] - xj[k]);
+        dx[k] = xi[k] - xj[k];
+    }
+    if (NDIM == 2) dx[2] = 0.0;
+    dst = sqrt(dst);
+    double ra = (hli + S.hl[j]) / 2;
+    w[0] = w[1] = w[2] = w[3] = 0.0;
+    if (P.flknl == 1 || P.flknl == 3) weight_b(dst, ra, dx, w, NDIM);
+    else weight_y(dst, ra, dx, w, NDIM);
+}
+
+__device__ __forceinline__ int nb_count(const State &S, int i) {
+    int c = S.nb_cnt[i];
+    return c < kMaxNb ? c : kMaxNb;
+}
+
+// ------------------------------------------------------------------------------------------
+// NNPS: cell binning (parttocell, CPP_3-NNPS.cpp:175-263)
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_int(int *a, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_bin(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double xmin[3] = {P.xmin, P.ymin, P.zmin};
+    long long nt[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        double v = fabs(S.xp[cur][k][i] - xmin[k]) / P.cell_size + 2.0;  // fabs mirrors (quirk A6)
+        if (!(v < 1.0e9)) v = 1.0e9;                                      // keeps the cast defined
+        nt[k] = (long long)v;
+        if (nt[k] < 2) nt[k] = 2;
+    }
+    long long id = NDIM == 2 ? (long long)P.cym * (nt[0] - 1) + nt[1]
+                             : (long long)P.cxm * P.czm * (nt[1] - 1) + (long long)P.czm * (nt[0] - 1) + nt[2];
+    int c;
+    if (id > 0 && id <= P.ncell) {
+        c = (int)id;
+    } else {
+        set_err(S, SPHG_ERR_CELL);  // error 3; the particle is parked in cell 0, which no search visits
+        c = 0;
+    }
+    S.cell[i] = c;
+    atomicAdd(&S.cell_count[c], 1);
+}
+
+// Exclusive scan of cell counts: block-local pass, top pass over block sums, add pass.
+constexpr int kScanBlock = 256, kScanItems = 8, kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int *total) {
+    // Hillis-Steele in LDS over 256 threads; adequate for a pass that is <1% of the step.
+    int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        int x = t >= off ? lds[t - off] : 0;
+        __syncthreads();
+        lds[t] += x;
+        __syncthreads();
+    }
+    int incl = lds[t];
+    *total = lds[kScanBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const int *in, int *out, int *block_sums, int m) {
+    __shared__ int lds[kScanBlock];
+    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int loc[kScanItems], s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int idx = base + k;
+        loc[k] = idx < m ? in[idx] : 0;
+        s += loc[k];
+    }
+    int tot;
+    int pre = block_exclusive_scan(s, lds, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int idx = base + k;
+        if (idx < m) out[idx] = pre;
+        pre += loc[k];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(int *block_sums, int nb) {
+    __shared__ int lds[kScanBlock];
+    int carry = 0;
+    for (int base = 0; base < nb; base += kScanBlock) {
+        int idx = base + threadIdx.x;
+        int v = idx < nb ? block_sums[idx] : 0;
+        int tot;
+        int ex = block_exclusive_scan(v, lds, &tot);
+        if (idx < nb) block_sums[idx] = ex + carry;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(int *out, const int *block_sums, int m) {
+    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int add = block_sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int idx = base + k;
+        if (idx < m) out[idx] += add;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(State S, Params P) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    int c = S.cell[i];
+    int pos = S.cell_start[c] + atomicAdd(&S.cell_cursor[c], 1);
+    S.sorted[pos] = i;
+}
+
+// Within a cell the reference's bottom-up merge sort (ties take the right half, CPP_3-NNPS.cpp:576)
+// leaves particles in descending index order; the atomic scatter order is arbitrary, so each
+// cell's run is re-ordered here. Runs are short (a few to a few tens of particles).
+__global__ __launch_bounds__(256) void k_cell_order(State S, int ncells) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int s = S.cell_start[c], cnt = S.cell_count[c];
+    int *a = S.sorted + s;
+    for (int x = 1; x < cnt; x++) {
+        int v = a[x], y = x - 1;
+        while (y >= 0 && a[y] < v) {
+            a[y + 1] = a[y];
+            y--;
+        }
+        a[y + 1] = v;
+    }
+}
+
+// partisearching (CPP_3-NNPS.cpp:265-374): stencil cells in nncell order, candidates in cell
+// order, predicate dst <= 2*ra && i != j evaluated in FP64 exactly as the reference.
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_search(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    int cnt = 0;
+    if (S.type[i] != 7) {
+        double xi[3];
+        load_x<NDIM>(S, cur, i, xi);
+        double hli = S.hl[i];
+        int c = S.cell[i];
+        for (int s = 0; s < P.nstencil; s++) {
+            int nl = c + P.nb_off[s];
+            if (nl <= 0 || nl > P.ncell || c == 0) continue;
+            int st = S.cell_start[nl], en = st + S.cell_count[nl];
+            for (int q = st; q < en; q++) {
+                int j = S.sorted[q];
+                double dst = 0.0;
+#pragma unroll
+                for (int k = 0; k < NDIM; k++) {
+                    double d = xi[k] - S.xp[cur][k][j];
+                    dst = dst + d * d;
+                }
+                dst = sqrt(dst);
+                double ra = (hli + S.hl[j]) / 2;
+                if (dst <= 2 * ra && i != j) {
+                    if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = j;
+                    cnt++;
+                }
+            }
+        }
+        if (cnt > kMaxNb) set_err(S, SPHG_ERR_NB);
+    }
+    S.nb_cnt[i] = cnt;
+}
+
+// Virtual velocity of boundary neighbour j seen by particle i of type ti (CPP_6-density.cpp:86-120).
+__device__ __forceinline__ void bndy_vel(const State &S, const Params &P, int ti, int j, const double own[3],
+                                         double v[3]) {
+    v[0] = v[1] = v[2] = 0.0;
+    int bt = P.flbndy;
+    if (bt == 1) {
+        v[0] = own[0]; v[1] = own[1]; v[2] = own[2];
+    } else if (bt == 2 || bt == 3 || bt == 4 || bt == 13 || bt == 14) {
+        if (ti >= 1 && ti <= 4) {
+            int ph = ti - 1;
+            v[0] = S.vb[ph][0][j]; v[1] = S.vb[ph][1][j]; v[2] = S.vb[ph][2][j];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// density loops 1-2 (CPP_6-density.cpp:22-166)
+// ------------------------------------------------------------------------------------------
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    const int l = step_of(S);
+    double xi[3], xj[3], dx[3], w[4];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const int nc = nb_count(S, i);
+    double por = S.porosity[i], pcop = S.poro_cop[i];
+    if (ti == 1) {  // loop 1: water porosity from soil neighbours
+        double up = 0.0, dn = 0.0;
+        for (int nj = 0; nj < nc; nj++) {
+            int j = S.nbr[(size_t)nj * P.cap + i];
+            if (S.type[j] != 2) continue;
+            pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+            double mj = S.mass[j], rj = S.rho[j];
+            up = up + mj * S.porosity[j] / rj * w[3];
+            dn = dn + mj / rj * w[3];
+        }
+        if (fabs(dn) > 1.0e-7) {
+            pcop = por;
+            por = up / dn;
+            if (l == 1) pcop = por;
+        } else {
+            pcop = por;
+        }
+        S.porosity[i] = por;
+        S.poro_cop[i] = pcop;
+    }
+    if (ti == 7 || ti == 0) {
+        S.rho_mid[i] = S.rho[i];
+        return;
+    }
+    double vi[3] = {S.vxp[0][i], S.vxp[1][i], NDIM == 3 ? S.vxp[2][i] : 0.0};
+    if (NDIM == 2) vi[2] = S.vxp[2][i];
+    double up[3] = {0.0, 0.0, 0.0}, dn[3] = {0.0, 0.0, 0.0};
+    const bool solid = (ti == 2 || ti == 4);
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        int tj = S.type[j];
+        double vj[3];
+        if (tj == ti) {
+            vj[0] = S.vxp[0][j]; vj[1] = S.vxp[1][j]; vj[2] = NDIM == 3 ? S.vxp[2][j] : 0.0;
+        } else if (tj == 0) {
+            bndy_vel(S, P, ti, j, vi, vj);
+        } else {
+            continue;
+        }
+        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        double mj = S.mass[j];
+#pragma unroll
+        for (int k = 0; k < NDIM; k++) {
+            up[k] = up[k] + mj * (vi[k] - vj[k]) * w[k];
+            if (solid) dn[k] = dn[k] + mj * (xi[k] - xj[k]) * w[k];
+        }
+    }
+    double drhot = 0.0;
+    if (solid) {
+#pragma unroll
+        for (int k = 0; k < NDIM; k++) {
+            if (fabs(dn[k]) > 1.0e-7) drhot -= up[k] / dn[k];
+            else drhot += up[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NDIM; k++) drhot += up[k];
+    }
+    double rhop = S.rhop[i], r;
+    if (ti == 1 && fabs(por) > 0.0001) r = rhop + drhot * P.dt + rhop * (por - pcop) / (1.0 - pcop);
+    else r = rhop + drhot * P.dt;
+    S.rho_mid[i] = r;
+}
+
+// density loop 3 (Jacobi form of CPP_6-density.cpp:168-188) and eos loop 1 (CPP_7-1-eos_CR.cpp:20-43)
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    double xi[3], xj[3], dx[3], w[4];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const int nc = nb_count(S, i);
+    double up = 0.0, dn = 0.0;
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        if (S.type[j] != ti) continue;
+        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        double mj = S.mass[j];
+        up = up + mj * w[3];
+        dn = dn + mj / S.rho_mid[j] * w[3];
+    }
+    double r = S.rho_mid[i];
+    if (ti != 7 && fabs(dn) > 1.0e-7) r = up / dn;
+    S.rho[i] = r;
+    // eos loop 1
+    if (ti == 1) {
+        const FluidSlot &f = S.fluid[S.fluid_slot[i]];
+        double prep = S.prep[i];
+        double p = prep * (pow(r / f.dens / f.porosity, 7.0) - 1.0);
+        S.pre[i] = p + prep;
+    } else if (ti == 3) {
+        double prep = S.prep[i];
+        double p = 1.403 * prep * (-1.0);  // quirk A9: comma operator drops the density ratio
+        S.pre[i] = p + prep;
+    } else if (ti == 0 && S.matype[i] == 1) {
+        double prep = S.prep[i];
+        double p = prep * (pow(r / S.rhop[i], 7.0) - 1.0);
+        S.pre[i] = p + prep;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// strain rate + eos loop 2 + free-surface colour + permeability
+// ------------------------------------------------------------------------------------------
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    const int l = step_of(S);
+    double xi[3], xj[3], dx[3], w[4];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const int nc = nb_count(S, i);
+    if (l == P.inip)
+        for (int k = 0; k < 6; k++) S.eps[k][i] = 0.0;
+    double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    double up[9], dnm[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 9; k++) up[k] = 0.0;
+    double u1 = 0, u2 = 0, u3 = 0, u4 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, us[6] = {0, 0, 0, 0, 0, 0}, ds = 0;
+    double color = 0.0;
+    const bool solid = (ti == 2 || ti == 4);
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        int tj = S.type[j];
+        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        double mj = S.mass[j], rj = S.rho[j];
+        // strain (CPP_7-1-strain.cpp:67-150)
+        bool act = false;
+        double dvx[3];
+        if (ti == tj && ti != 7 && ti != 0) {
+            dvx[0] = vi[0] - S.vxp[0][j];
+            dvx[1] = vi[1] - S.vxp[1][j];
+            dvx[2] = vi[2] - S.vxp[2][j];
+            act = true;
+        } else if (ti >= 1 && ti <= 4 && tj == 0) {
+            double vj[3];
+            bndy_vel(S, P, ti, j, vi, vj);
+            dvx[0] = vi[0] - vj[0];
+            dvx[1] = vi[1] - vj[1];
+            dvx[2] = vi[2] - vj[2];
+            act = true;
+        }
+        if (act) {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) up[3 * a + b] = up[3 * a + b] + mj / rj * dvx[a] * w[b];
+            if (solid) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) dnm[k] = dnm[k] + mj / rj * (xi[k] - xj[k]) * w[k];
+            }
+        }
+        // eos loop 2 (CPP_7-1-eos_CR.cpp:62-118)
+        if (ti != tj) {
+            double w3 = w[3];
+            if (tj == 1 && (ti == 2 || ti == 0)) {
+                double pj = S.pre[j];
+                u1 = u1 + mj * pj * w3 / rj;
+                d1 = d1 + mj * w3 / rj;
+            } else if (tj == 3 && (ti == 2 || ti == 0)) {
+                double pj = S.pre[j];
+                u2 = u2 + mj * pj * w3 / rj;
+                d2 = d2 + mj * w3 / rj;
+            } else if (tj == 1 && ti == 4) {
+                double pj = S.pre[j];
+                u3 = u3 + mj * pj * w3 / rj;
+                d3 = d3 + mj * w3 / rj;
+            } else if (tj == 3 && ti == 4) {
+                double pj = S.pre[j];
+                u4 = u4 + mj * pj * w3 / rj;
+                d4 = d4 + mj * w3 / rj;
+            } else if (ti == 0 && tj == 2) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) us[k] = us[k] + mj * S.sig[k][j] * w3 / rj;
+                ds = ds + mj * w3 / rj;
+            }
+        }
+        // colour (CPP_7-2-constitutive.cpp:467-478)
+        if (ti == tj && ti != 0 && ti != 7) color = color + mj / rj * 1.0 * w[3];
+        else if (ti != tj && ti > 0 && ti < 5 && tj == 0) color = color + mj / rj * 1.0 * w[3];
+    }
+    // velocity gradient -> strain rate and spin (CPP_7-1-strain.cpp:153-219)
+    double divx[9];
+    if (ti == 1 || ti == 3) {
+        for (int k = 0; k < 9; k++) divx[k] = up[k];
+    } else {
+        for (int k = 0; k < 9; k++) divx[k] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            if (fabs(dnm[b]) > 1.0e-7)
+#pragma unroll
+                for (int a = 0; a < 3; a++) divx[3 * a + b] = up[3 * a + b] / dnm[b];
+    }
+    double d0 = divx[0], d1_ = divx[4], d2_ = divx[8];
+    double d3_ = divx[1] + divx[3], d4_ = divx[5] + divx[7], d5_ = divx[6] + divx[2];
+    if (NDIM == 2) d2_ = d4_ = d5_ = 0.0;
+    S.deps[0][i] = d0; S.deps[1][i] = d1_; S.deps[2][i] = d2_;
+    S.deps[3][i] = d3_; S.deps[4][i] = d4_; S.deps[5][i] = d5_;
+    double w01 = 0.5 * (divx[1] - divx[3]);
+    double w02 = NDIM == 2 ? 0.0 : 0.5 * (divx[2] - divx[6]);
+    double w12 = NDIM == 2 ? 0.0 : 0.5 * (divx[5] - divx[7]);
+    S.weps[0][i] = w01; S.weps[1][i] = w02; S.weps[2][i] = w12;
+    // eos loop 2 results (CPP_7-1-eos_CR.cpp:121-176)
+    if (ti == 2 || ti == 0) {
+        S.prew[i] = d1 != 0.0 ? u1 / d1 : u1;
+        S.prea[i] = d2 != 0.0 ? u2 / d2 : u2;
+        if (ti == 0)
+            for (int k = 0; k < 6; k++) S.sig[k][i] = ds != 0.0 ? us[k] / ds : us[k];
+    } else if (ti == 4) {
+        S.prew[i] = d3 != 0.0 ? u3 / d3 : u3;
+        S.prea[i] = d4 != 0.0 ? u4 / d4 : u4;
+    }
+    S.color[i] = color;
+    // cal_permeability (CPP_7-2-constitutive.cpp:355-382), fli == 1 only
+    if (P.fli == 1 && ti == 2) {
+        const SoilSlot &sl = S.soil[S.soil_slot[i]];
+        double cop = S.cop[i];
+        if (P.permtype == 0) cop = sl.cop;
+        else if (P.permtype == 1) cop = P.pers;
+        else if (P.permtype == 2) {
+            double sa = S.satu[i];
+            if (sa < sl.sitar) cop = P.pers;
+            else if (sa > sl.sitas) cop = P.perl;
+            else cop = (P.perl - P.pers) * sa / (sl.sitas - sl.sitar) + P.pers;
+        } else if (P.permtype == 3) {
+            int pt = S.permtype[i];
+            if (pt == 1) cop = P.pers;
+            if (pt == 2) cop = P.perm;
+            if (pt == 3) cop = P.perl;
+        }
+        S.cop[i] = cop;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// per-particle constitutive update, free-surface correction and principal stresses
+// ------------------------------------------------------------------------------------------
+struct Sig6 {
+    double s[6];
+};
+
+__device__ __forceinline__ void deq6(const double sig[6], const double eps[6], double &vsig, double &veps,
+                                     double &divq, double &divr) {
+    veps = eps[0] + eps[1] + eps[2];
+    vsig = (sig[0] + sig[1] + sig[2]) / 3;
+    double sxx = sig[0] - vsig, syy = sig[1] - vsig, szz = sig[2] - vsig;
+    double j2 = 0.5 * (sxx * sxx + syy * syy + szz * szz) + sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5];
+    divq = sqrt(3 * fabs(j2));
+    divr = vsig != 0.0 ? fabs(divq / vsig) : 0.0;
+}
+
+__device__ __forceinline__ void iso_tangent(double D[6][6], double e0, double poi, double &dlam, double &dmu) {
+    dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi);
+    dmu = e0 * 0.5 / (1.0 + poi);
+    double dmu2 = dmu * 2.0;
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) D[a][b] = 0.0;
+    D[0][0] = D[1][1] = D[2][2] = dlam + dmu2;
+    D[0][1] = D[0][2] = D[1][0] = D[1][2] = D[2][0] = D[2][1] = dlam;
+    D[3][3] = D[4][4] = D[5][5] = dmu;
+}
+
+__device__ __forceinline__ double soil_e0(const SoilSlot &s, double sm) {
+    return s.e > 1.0 ? s.e : 3 * (1 - 2 * s.poi) * (1 + s.eps0) * fabs(sm) / s.kapa;
+}
+
+// the (ki, kj) loop dsig += D deps (CPP_7-2-constitutive.cpp:107-113), rank-one update applied
+// on the fly: D[a][b] = De[a][b] - d * w[a] * z[b]
+__device__ __forceinline__ void tangent_apply(double dsig[6], const double De[6][6], bool plastic, double d,
+                                              const double wv[6], const double zv[6], const double deps[6]) {
+#pragma unroll
+    for (int a = 0; a < 6; a++) {
+        double acc = dsig[a];
+#pragma unroll
+        for (int b = 0; b < 6; b++) {
+            double Dab = De[a][b];
+            if (plastic) Dab = Dab - d * wv[a] * zv[b];
+            acc += Dab * deps[b];
+        }
+        dsig[a] = acc;
+    }
+}
+
+__device__ int eejcb3(double a[9], double v[9], double eps, int jt) {
+    int p = 0, q = 0, l = 1;
+    for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) v[x * 3 + y] = x == y ? 1.0 : 0.0;
+    for (;;) {
+        double fm = 0.0;
+        for (int x = 0; x < 3; x++)
+            for (int y = 0; y < 3; y++) {
+                double d = fabs(a[x * 3 + y]);
+                if (x != y && d > fm) {
+                    fm = d;
+                    p = x;
+                    q = y;
+                }
+            }
+        if (fm < eps) return 1;
+        if (l > jt) return -1;
+        l = l + 1;
+        int u = p * 3 + q, w = p * 3 + p, t = q * 3 + p, s = q * 3 + q;
+        double x = -a[u];
+        double y = (a[s] - a[w]) / 2.0;
+        double omega = x / sqrt(x * x + y * y);
+        if (y < 0.0) omega = -omega;
+        double sn = 1.0 + sqrt(1.0 - omega * omega);
+        sn = omega / sqrt(2.0 * sn);
+        double cn = sqrt(1.0 - sn * sn);
+        fm = a[w];
+        a[w] = fm * cn * cn + a[s] * sn * sn + a[u] * omega;
+        a[s] = fm * sn * sn + a[s] * cn * cn - a[u] * omega;
+        a[u] = 0.0;
+        a[t] = 0.0;
+        for (int jj = 0; jj < 3; jj++)
+            if (jj != p && jj != q) {
+                int uu = p * 3 + jj, ww = q * 3 + jj;
+                fm = a[uu];
+                a[uu] = fm * cn + a[ww] * sn;
+                a[ww] = -fm * sn + a[ww] * cn;
+            }
+        for (int ii = 0; ii < 3; ii++)
+            if (ii != p && ii != q) {
+                int uu = ii * 3 + p, ww = ii * 3 + q;
+                fm = a[uu];
+                a[uu] = fm * cn + a[ww] * sn;
+                a[ww] = -fm * sn + a[ww] * cn;
+            }
+        for (int ii = 0; ii < 3; ii++) {
+            int uu = ii * 3 + p, ww = ii * 3 + q;
+            fm = v[uu];
+            v[uu] = fm * cn + v[ww] * sn;
+            v[ww] = -fm * sn + v[ww] * cn;
+        }
+    }
+}
+
+__device__ __forceinline__ int model_of(int matype) {
+    if ((matype >= 0 && matype < 30) || (matype >= 50 && matype < 60)) return matype % 10;
+    return -1;
+}
+
+__global__ __launch_bounds__(256) void k_stress(State S, Params P) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    const int l = step_of(S);
+    const double dt = P.dt;
+    double sig[6], deps[6];
+    for (int k = 0; k < 6; k++) {
+        sig[k] = S.sig[k][i];
+        deps[k] = S.deps[k][i];
+    }
+    // weps[a][b] from the three stored components (antisymmetric, zero diagonal)
+    const double w01 = S.weps[0][i], w02 = S.weps[1][i], w12 = S.weps[2][i];
+    // the reference forms weps[b][a] = 0.5*(divx[b][a]-divx[a][b]) separately; that equals -weps[a][b]
+    // bit for bit except that a zero difference is +0 on both sides, which neg0 reproduces
+    auto neg0 = [](double x) { return x == 0.0 ? 0.0 : -x; };
+    const double W[3][3] = {{0.0, w01, w02}, {neg0(w01), 0.0, w12}, {neg0(w02), neg0(w12), 0.0}};
+    if (ti == 1 || ti == 3) {
+        // fluids (CPP_7-2-constitutive.cpp:26-86)
+        const FluidSlot &f = S.fluid[S.fluid_slot[i]];
+        double vis = f.vis, pre = S.pre[i];
+        if (ti == 1) {
+            double dm = (deps[0] + deps[1] + deps[2]) / 3.0;
+            double e0 = deps[0] - dm, e1 = deps[1] - dm, e2 = deps[2] - dm;
+            S.eps[0][i] = e0; S.eps[1][i] = e1; S.eps[2][i] = e2;
+            double de = 0.666666666666667 * (e0 * e0 + e1 * e1 + e2 * e2 + deps[3] * deps[3] * 0.25 +
+                                             deps[4] * deps[4] * 0.25 + deps[5] * deps[5] * 0.25);
+            de = sqrt(de);
+            if (f.bhtype == 1) {
+                double tao = pre * f.tan_fai + f.c;
+                vis = de < f.strain_min ? f.vis + tao / f.strain_min : f.vis + tao / de;
+            }
+        }
+        double dd[6];
+        for (int k = 0; k < 3; k++) dd[k] = 2.0 * deps[k] - 2.0 * (deps[0] + deps[1] + deps[2]) / 3.0;
+        for (int k = 3; k < 6; k++) dd[k] = deps[k];
+        for (int k = 0; k < 6; k++) sig[k] = k < 3 ? -pre - vis * dd[k] : -vis * dd[k];
+    } else if (ti == 2 || ti == 4) {
+        double dsig[6] = {0, 0, 0, 0, 0, 0}, adps[6] = {0, 0, 0, 0, 0, 0}, erfs[6] = {0, 0, 0, 0, 0, 0};
+        double De[6][6], dlam, dmu;
+        if (ti == 2) {
+            const SoilSlot &s = S.soil[S.soil_slot[i]];
+            int mt = P.inip > l ? 0 : model_of(S.matype[i]);
+            double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * 0.3333333333;
+            if (sm < 1.0) sm = 1.0;
+            if (mt == 0 || mt == 6) {  // elastic (CPP_8-1-consfactor.cpp:17-59)
+                iso_tangent(De, soil_e0(s, sm), s.poi, dlam, dmu);
+                double z6[6] = {0, 0, 0, 0, 0, 0};
+                tangent_apply(dsig, De, false, 0.0, z6, z6, deps);
+            } else if (mt == 2) {  // dpmodel1 + DPModel_check (CPP_8-1-consfactor.cpp:284-448, :779-819)
+                double e0 = soil_e0(s, sm);
+                iso_tangent(De, e0, s.poi, dlam, dmu);
+                double dmu2 = dmu * 2.0;
+                double ci = (S.ps_c ? S.ps_c[i] : s.c) / s.fs;
+                double dxv = -sig[0] - sm, dyv = -sig[1] - sm, dzv = -sig[2] - sm;
+                double dxy = -sig[3], dyz = -sig[4], dzx = -sig[5];
+                double sjj = 0.5 * (dxv * dxv + dyv * dyv + dzv * dzv) + dxy * dxy + dyz * dyz + dzx * dzx;
+                double sj = sqrt(sjj);
+                double yalf = s.dp_yalf;
+                double ykap = 3.0 * ci * s.dp_cosf / s.dp_sq;
+                bool plastic = false;
+                double d = 0.0, wv[6], zv[6], ramda = 0.0, fv[6];
+                if ((sj - yalf * sm * 3.0 - ykap) >= 0.0) {
+                    double f11 = -yalf + dxv / sj * 0.5, f22 = -yalf + dyv / sj * 0.5, f33 = -yalf + dzv / sj * 0.5;
+                    double f12 = dxy / sj * 0.5, f23 = dyz / sj * 0.5, f31 = dzx / sj * 0.5;
+                    double fkk = f11 + f22 + f33, yd = 0.0;
+                    double g11 = -yd + dxv / sj * 0.5, g22 = -yd + dyv / sj * 0.5, g33 = -yd + dzv / sj * 0.5;
+                    double g12 = dxy / sj * 0.5, g23 = dyz / sj * 0.5, g31 = dzx / sj * 0.5;
+                    double hdd = g11 + g22 + g33;
+                    double fij = 2.0 * (f11 * g11 + f22 * g22 + f33 * g33) + f12 * g12 + f23 * g23 + f31 * g31;
+                    double cc7 = dlam * (fkk * hdd) + dmu * fij + hdd;
+                    d = 1.0 / cc7;
+                    double df = dlam * fkk, dg = dlam * hdd;
+                    ramda = (dmu2 * f11 - 3.0 * dlam * yalf) * deps[0] + (dmu2 * f22 - 3.0 * dlam * yalf) * deps[1] +
+                            (dmu2 * f33 - 3.0 * dlam * yalf) * deps[2] +
+                            dmu * (f12 * deps[3] + f23 * deps[4] + f31 * deps[5]);
+                    if (ramda * d < 0.0) {
+                        plastic = true;
+                        wv[0] = dg + dmu2 * g11; wv[1] = dg + dmu2 * g22; wv[2] = dg + dmu2 * g33;
+                        wv[3] = dmu * g12; wv[4] = dmu * g23; wv[5] = dmu * g31;
+                        zv[0] = df + dmu2 * f11; zv[1] = df + dmu2 * f22; zv[2] = df + dmu2 * f33;
+                        zv[3] = dmu * f12; zv[4] = dmu * f23; zv[5] = dmu * f31;
+                        fv[0] = f11; fv[1] = f22; fv[2] = f33; fv[3] = f12; fv[4] = f23; fv[5] = f31;
+                    }
+                }
+                tangent_apply(dsig, De, plastic, d, wv, zv, deps);
+                if (plastic)
+                    for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
+                // DPModel_check: stress-scaling return; sj from the stress before the increment
+                double smc = (sig[0] + sig[1] + sig[2]) / 3.0;
+                double ex = sig[0] - smc, ey = sig[1] - smc, ez = sig[2] - smc;
+                double sjc = sqrt(0.5 * (ex * ex + ey * ey + ez * ez) + sig[3] * sig[3] + sig[4] * sig[4] +
+                                  sig[5] * sig[5]);
+                double ts[6];
+                for (int k = 0; k < 6; k++) ts[k] = sig[k] + dsig[k] * dt;
+                double i1 = ts[0] + ts[1] + ts[2];
+                double kc = 3.0 * ci * s.dp_cosf / s.dp_sq;
+                double rn = (-yalf * i1 + kc) / sjc;
+                if (rn < 1.0) {
+                    for (int k = 0; k < 3; k++) ts[k] = rn * (ts[k] - i1 / 3.0) + i1 / 3.0;
+                    for (int k = 3; k < 6; k++) ts[k] = rn * ts[k];
+                }
+                for (int k = 0; k < 6; k++) dsig[k] = (ts[k] - sig[k]) / dt;
+            } else if (mt == 1) {  // sub_camclay (CPP_8-1-consfactor.cpp:129-281)
+                double poi = s.poi, eps0 = s.eps0, zmf = s.zmf;
+                double ymf = 3.0 * (zmf - 1.0) / (zmf + 2.0) / s.fs;
+                double e0 = soil_e0(s, sm);
+                double hee = (s.ramda - s.kapa) / (1.0 + eps0);
+                iso_tangent(De, e0, poi, dlam, dmu);
+                double dmu2 = dmu * 2.0;
+                double dxv = -sig[0] - sm, dyv = -sig[1] - sm, dzv = -sig[2] - sm;
+                double dxy = -sig[3], dyz = -sig[4], dzx = -sig[5];
+                double sjj = 0.5 * (dxv * dxv + dyv * dyv + dzv * dzv) + dxy * dxy + dyz * dyz + dzx * dzx;
+                double tt1 = 1.0 / (ymf * ymf * sm * sm + 3.0 * sjj);
+                double tt2 = 2.0 * ymf * ymf * sm * tt1 - 1.0 / sm;
+                double f11 = tt2 / 3.0 + dxv * tt1 * 3.0, f22 = tt2 / 3.0 + dyv * tt1 * 3.0,
+                       f33 = tt2 / 3.0 + dzv * tt1 * 3.0;
+                double f12 = 3.0 * dxy * tt1, f23 = 3.0 * dyz * tt1, f31 = 3.0 * dzx * tt1;
+                double roue = (1 + eps0) * log(s.sme0 / sm);
+                if (roue < 0.0) roue = 0.0;
+                double ttt = s.ann * roue * roue;
+                double hdd = f11 + f22 + f33 + ttt / sm;
+                double fkk = f11 + f22 + f33;
+                double fij = 2.0 * (f11 * f11 + f22 * f22 + f33 * f33) + f12 * f12 + f23 * f23 + f31 * f31;
+                double cc7 = dlam * (fkk * fkk) + dmu * fij + hdd / hee;
+                double d = 1.0 / cc7, df = dlam * fkk, dg = dlam * fkk;
+                double ramda = (dmu2 * f11 + dlam * fkk) * deps[0] + (dmu2 * f22 + dlam * fkk) * deps[1] +
+                               (dmu2 * f33 + dlam * fkk) * deps[2] +
+                               dmu * (f12 * deps[3] + f23 * deps[4] + f31 * deps[5]);
+                bool plastic = ramda < 0.0;
+                double wv[6] = {dg + dmu2 * f11, dg + dmu2 * f22, dg + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+                double zv[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+                tangent_apply(dsig, De, plastic, d, wv, zv, deps);
+                if (plastic) {
+                    double fv[6] = {f11, f22, f33, f12, f23, f31};
+                    for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
+                }
+            } else if (mt == 4) {
+                // sub_camanisotropy: body commented out in the reference (quirk A13): zero tangent
+                double Z[6][6];
+                for (int a = 0; a < 6; a++)
+                    for (int b = 0; b < 6; b++) Z[a][b] = 0.0;
+                double z6[6] = {0, 0, 0, 0, 0, 0};
+                tangent_apply(dsig, Z, false, 0.0, z6, z6, deps);
+            } else if (mt >= 0) {
+                set_err(S, 4);  // model not implemented in this build
+            }
+        } else {
+            // structure (CPP_7-2-constitutive.cpp:276-319)
+            iso_tangent(De, P.struct_e, P.struct_poi, dlam, dmu);
+            double z6[6] = {0, 0, 0, 0, 0, 0};
+            tangent_apply(dsig, De, false, 0.0, z6, z6, deps);
+        }
+        // Jaumann correction (CPP_7-2-constitutive.cpp:236-253)
+        const double *s = sig;
+        dsig[0] = dsig[0] + (W[0][0] * s[0] + W[0][1] * s[3] + W[0][2] * s[5]) - (s[0] * W[0][0] + s[3] * W[1][0] + s[5] * W[2][0]);
+        dsig[1] = dsig[1] + (W[1][0] * s[3] + W[1][1] * s[1] + W[1][2] * s[4]) - (s[3] * W[0][1] + s[1] * W[1][1] + s[4] * W[2][1]);
+        dsig[2] = dsig[2] + (W[2][0] * s[5] + W[2][1] * s[4] + W[2][2] * s[2]) - (s[5] * W[0][2] + s[4] * W[1][2] + s[2] * W[2][2]);
+        dsig[3] = dsig[3] + (W[0][0] * s[3] + W[0][1] * s[1] + W[0][2] * s[4]) - (s[0] * W[0][1] + s[3] * W[1][1] + s[5] * W[2][1]);
+        dsig[4] = dsig[4] + (W[1][0] * s[5] + W[1][1] * s[4] + W[1][2] * s[2]) - (s[3] * W[0][2] + s[1] * W[1][2] + s[4] * W[2][2]);
+        dsig[5] = dsig[5] + (W[0][0] * s[5] + W[0][1] * s[4] + W[0][2] * s[2]) - (s[0] * W[0][2] + s[3] * W[1][2] + s[5] * W[2][2]);
+        double eps[6];
+        for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
+        for (int k = 0; k < 6; k++) {
+            if (ti == 2) S.adps[k][i] = S.adps[k][i] + adps[k] * dt;
+            eps[k] = eps[k] + deps[k] * dt;
+            sig[k] = sig[k] + dsig[k] * dt + erfs[k];
+        }
+        for (int k = 0; k < 6; k++) S.eps[k][i] = eps[k];
+        double vsig, veps, divq, divr;
+        deq6(sig, eps, vsig, veps, divq, divr);
+        // shearstrain (Class_particle.cpp:175-192)
+        double sxx = eps[0] - veps / 3.0, syy = eps[1] - veps / 3.0, szz = eps[2] - veps / 3.0;
+        double j2 = 0.5 * (sxx * sxx + syy * syy + szz * szz) + eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5];
+        S.meps[i] = sqrt(2.0 * fabs(j2));
+        // poro_cal (Class_particle.cpp:194-203)
+        double e = S.e[i];
+        double de = (1 + e) * (deps[0] + deps[1] + deps[2]) * dt, et = e + de;
+        if (et < 0.0) et = 0.0;
+        S.porosity[i] = et / (1 + et);
+        S.e[i] = et;
+        S.veps[i] = veps;
+        S.vsig[i] = vsig;
+        S.divq[i] = divq;
+        S.divr[i] = divr;
+    }
+    // free-surface correction (CPP_7-2-constitutive.cpp:481-505)
+    const double color = S.color[i], tmin = P.tmin, tmax = P.tmax, incl = tmax - tmin;
+    if (ti == 1) {
+        if (color < tmin / 2.0) {
+            for (int k = 0; k < 3; k++) sig[k] = -1.0;
+            for (int k = 3; k < 6; k++) sig[k] = 0.0;
+        } else if (color >= tmin / 2.0 && color < tmax / 2.0) {
+            double xs = (2.0 * color - tmin) / incl;
+            for (int k = 0; k < 6; k++) sig[k] = xs * sig[k];
+        }
+    } else if (ti == 2) {
+        if (color < tmin) {
+            for (int k = 0; k < 3; k++) sig[k] = -1.0;
+            for (int k = 3; k < 6; k++) sig[k] = 0.0;
+        } else if (color >= tmin && color < tmax) {
+            double xs = (color - tmin) / incl * 1.0;
+            for (int k = 0; k < 6; k++) sig[k] = xs * sig[k];
+        }
+        double vsig = S.vsig[i];
+        if (vsig > 0.0) {  // stress_adjust (Class_particle.cpp:205-215)
+            double t = -vsig;
+            sig[0] = sig[0] + t;
+            sig[1] = sig[1] + t;
+            sig[2] = sig[2] + t;
+            sig[3] = sig[4] = sig[5] = 0.0;
+        }
+        double eps[6], veps, divq, divr;
+        for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
+        deq6(sig, eps, vsig, veps, divq, divr);
+        S.vsig[i] = vsig;
+        S.veps[i] = veps;
+        S.divq[i] = divq;
+        S.divr[i] = divr;
+    }
+    if (ti != 0 && ti != 7) {
+        for (int k = 0; k < 6; k++) S.sig[k][i] = sig[k];
+    }
+    // principal stresses (CPP_7-1-strain.cpp:283-332)
+    double a[9] = {sig[0], sig[3], sig[5], sig[3], sig[1], sig[4], sig[5], sig[4], sig[2]}, v[9];
+    eejcb3(a, v, 0.001, 7);
+    for (int k = 0; k < 3; k++) {
+        S.strep[k][i] = a[k * 3 + k];
+        for (int c = 0; c < 3; c++) S.beta[c * 3 + k][i] = v[k * 3 + c];  // beta[c][k] = v[3k+c]
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// accelerations, artificial viscosity, boundary stress, Darcy drag, velocity update
+// ------------------------------------------------------------------------------------------
+__device__ __constant__ int kSidx[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    double xi[3], xj[3], dx[3], w[4];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const int nc = nb_count(S, i);
+    double sigi[6];
+    for (int k = 0; k < 6; k++) sigi[k] = S.sig[k][i];
+    const double rhoi = S.rho[i];
+    const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    // per-type coefficients
+    double fpor = 0.0;
+    int fsgn = 1, fother = -1;
+    if (ti == 1) {
+        fpor = S.fluid[S.fluid_slot[i]].porosity;
+        fsgn = P.water_acc_minor ? -1 : 1;
+        fother = 3;
+    } else if (ti == 3) {
+        fpor = S.fluid[S.fluid_slot[i]].porosity;
+        fsgn = P.air_acc_minor ? -1 : 1;
+        fother = 1;
+    }
+    double spor = 0.0, sdens = 0.0, prewi = S.prew[i], preai = S.prea[i], satui = S.satu[i];
+    if (ti == 2) {
+        const SoilSlot &sl = S.soil[S.soil_slot[i]];
+        spor = sl.porosity;
+        sdens = sl.dens;
+    }
+    // AV coefficients (CPP_9-3-artificialvis.cpp:60-77)
+    double av_a = 0.0, av_b = 0.0, av_c = 0.0;
+    if (ti == 1) { av_c = S.fluid[S.fluid_slot[i]].vpu; av_a = P.art_vis[0][0]; av_b = P.art_vis[0][1]; }
+    if (ti == 2) { av_c = S.soil[S.soil_slot[i]].vpu; av_a = P.art_vis[1][0]; av_b = P.art_vis[1][1]; }
+    if (ti == 3) { av_c = S.fluid[S.fluid_slot[i]].vpu; av_a = P.art_vis[2][0]; av_b = P.art_vis[2][1]; }
+    // boundary-effect stress of particle i (CPP_9-4-derivation_stress.cpp:667-697)
+    double as[6];
+    for (int k = 0; k < 6; k++) as[k] = 0.0;
+    if (ti == 1 || ti == 3) {
+        for (int k = 0; k < 6; k++) as[k] = sigi[k];
+    } else if (ti == 2 || ti == 4) {
+        for (int k = 0; k < 3; k++) as[k] = sigi[k] - satui * prewi + (1 - satui) * preai;
+        for (int k = 3; k < 6; k++) as[k] = sigi[k];
+    }
+    double fluid_t[3][3], solid_t[3][3], bnd_t[3][3], av[3] = {0, 0, 0}, drag[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) fluid_t[a][b] = solid_t[a][b] = bnd_t[a][b] = 0.0;
+    double aif[6];
+    for (int k = 0; k < 6; k++) aif[k] = fpor * sigi[k];
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        int tj = S.type[j];
+        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        double mj = S.mass[j], rj = S.rho[j];
+        if ((ti == 1 || ti == 3) && (tj == ti || tj == fother)) {
+            // fluid_acceleration_water/air (+ minor variants), CPP_9-4-derivation_stress.cpp:12-351
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    int s = kSidx[a][b];
+                    double aj = fpor * S.sig[s][j];
+                    if (fsgn > 0) fluid_t[a][b] = fluid_t[a][b] + mj * (aj / rj / rj + aif[s] / rhoi / rhoi) * w[b];
+                    else fluid_t[a][b] = fluid_t[a][b] + mj * (aj / rj / rj - aif[s] / rhoi / rhoi) * w[b];
+                }
+        }
+        if ((ti == 2 || ti == 4) && tj == ti) {
+            // soil_acceleration_noreg / structure_acceleration_noreg (:354-505)
+            double satu = S.satu[j];  // quirk: neighbour saturation used for both terms
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    int s = kSidx[a][b];
+                    solid_t[a][b] = solid_t[a][b] + mj * (S.sig[s][j] / rj / rj + sigi[s] / rhoi / rhoi) * w[b];
+                }
+            double prewj = S.prew[j], preaj = S.prea[j];
+            if (ti == 2) {
+                double prei = (1 - spor) * (-satu * prewi + (1 - satu) * preai);
+                double prej = (1 - spor) * (-satu * prewj + (1 - satu) * preaj);
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj + prei / sdens / sdens) * w[k];  // quirk A12
+            } else {
+                double prei = -satu * prewi + (1 - satu) * preai;
+                double prej = -satu * prewj + (1 - satu) * preaj;
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj - prei / rhoi / rhoi) * w[k];
+            }
+        }
+        if (tj == ti && ti != 7) {
+            // artificial_viscosity (CPP_9-3-artificialvis.cpp:36-89)
+            double inprod = 0.0, dst = 0.0, dv[3], dxx[3];
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                dv[k] = vi[k] - S.vxp[k][j];
+                dxx[k] = xi[k] - xj[k];
+                dst = dst + dxx[k] * dxx[k];
+                inprod = inprod + dv[k] * dxx[k];
+            }
+            double hl = (S.hl[j] + hli) * 0.5;
+            double rou = (rj + rhoi) * 0.5;
+            double paix;
+            if (inprod < 0.0) {
+                dst = sqrt(dst);
+                double myu = hl * inprod / (dst * dst + 0.01 * hl * hl);
+                paix = (-av_a * av_c * myu + av_b * myu * myu) / rou;
+            } else {
+                paix = 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) av[k] = av[k] + mj * paix * w[k];
+        }
+        if (tj == 0 && ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2)) {
+            // boundary_moving_stress_effect (CPP_9-4-derivation_stress.cpp:631-721)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    int s = kSidx[a][b];
+                    bnd_t[a][b] = bnd_t[a][b] + mj * (as[s] / rj / rj + as[s] / rhoi / rhoi) * w[b];
+                }
+        }
+        if (P.fli == 1 && ti != tj) {
+            // interact1 pair sum (CPP_10-interact1.cpp:40-88)
+            double inprod = 0.0, dxx[3], dv[3];
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                dxx[k] = xi[k] - xj[k];
+                dv[k] = vi[k] - S.vxp[k][j];
+                inprod = inprod + dxx[k] * dv[k];
+            }
+            if (inprod < 0) {
+                double w3 = w[3];
+                if (ti == 2 && tj == 1) {
+                    double fd = S.fluid[S.fluid_slot[j]].dens, copi = S.cop[i], pori = S.porosity[i];
+#pragma unroll
+                    for (int k = 0; k < NDIM; k++)
+                        drag[k] = drag[k] + satui * satui * pori * pori * fd * P.gacc * dv[k] / copi * mj / rj * w3;
+                } else if (ti == 1 && tj == 2) {
+                    double fd = S.fluid[S.fluid_slot[i]].dens, sj = S.satu[j], pj = S.porosity[j], cj = S.cop[j];
+#pragma unroll
+                    for (int k = 0; k < NDIM; k++)
+                        drag[k] = drag[k] + sj * sj * pj * pj * fd * P.gacc * dv[k] / cj * mj / rj * w3;
+                } else if (ti == 2 && tj == 3) {
+                    double fd = S.fluid[S.fluid_slot[j]].dens, pori = S.porosity[i];
+#pragma unroll
+                    for (int k = 0; k < NDIM; k++)
+                        drag[k] = drag[k] + (1 - satui) * (1 - satui) * pori * pori * fd * P.gacc * dv[k] * 2000 * mj / rj * w3;
+                } else if (ti == 3 && tj == 2) {
+                    double fd = S.fluid[S.fluid_slot[i]].dens, sj = S.satu[j], pj = S.porosity[j];
+#pragma unroll
+                    for (int k = 0; k < NDIM; k++)
+                        drag[k] = drag[k] + (1 - sj) * (1 - sj) * pj * pj * fd * P.gacc * dv[k] * 2000 * mj / rj * w3;
+                }
+            }
+        }
+    }
+    // combine in the reference's call order (CPP_1-SPHLoops.cpp:586-660)
+    double ax[3] = {S.ax[0][i], S.ax[1][i], S.ax[2][i]};
+    const double g[3] = {P.gx, P.gy, P.gz};
+    if (ti >= 1 && ti <= 4) {
+        double (*t)[3] = (ti == 1 || ti == 3) ? fluid_t : solid_t;
+        for (int k = 0; k < 3; k++) ax[k] = g[k];
+        ax[0] += t[0][0] + t[0][1] + t[0][2];
+        ax[1] += t[1][0] + t[1][1] + t[1][2];
+        ax[2] += t[2][0] + t[2][1] + t[2][2];
+    }
+    if (ti != 7)
+        for (int k = 0; k < NDIM; k++) ax[k] = ax[k] - av[k];
+    if (ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2)) {
+        ax[0] += bnd_t[0][0] + bnd_t[0][1] + bnd_t[0][2];
+        ax[1] += bnd_t[1][0] + bnd_t[1][1] + bnd_t[1][2];
+        ax[2] += bnd_t[2][0] + bnd_t[2][1] + bnd_t[2][2];
+    }
+    const double dt = P.dt;
+    const double axp[3] = {S.axp[0][i], S.axp[1][i], S.axp[2][i]};
+    if (P.fli == 1 && ti != 0 && ti != 7) {
+        // interact1 clamp (CPP_10-interact1.cpp:93-108)
+        for (int k = 0; k < NDIM; k++) {
+            double itf = drag[k];
+            double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * dt;
+            double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - itf * P.interc / rhoi) * dt;
+            if (vx1 * vx2 <= 0.0) {
+                itf = 0.0;
+                ax[k] = -2.0 * vi[k] / dt - axp[k];
+            } else {
+                ax[k] -= itf * P.interc / rhoi;
+            }
+            S.interf[k][i] = itf;
+        }
+    }
+    for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
+    // veocity_update, if_move == 0 (CPP_11-xsph.cpp:11-57)
+    if (ti == 0) {
+        int mt = S.matype[i];
+        if (mt < 2) {
+            for (int k = 0; k < NDIM; k++) {
+                S.interf[k][i] = 0.0;
+                double ath = (0.0 + 0.0) * 0.5;
+                S.interfss[k][i] = S.interfss[k][i] + ath * dt;
+            }
+        } else if (mt == 3 || (mt == 5 && P.flv == 1)) {
+            for (int k = 0; k < NDIM; k++) {
+                double f = S.interf[k][i];
+                double ath = (f + f) * 0.5;
+                S.interfss[k][i] = S.interfss[k][i] + ath * dt;
+            }
+        }
+    } else {
+        for (int k = 0; k < NDIM; k++) {
+            double ath = (axp[k] + ax[k]) * 0.5;
+            S.vx[cur][k][i] = vi[k] + ath * dt;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// XSPH + velocity_set_domain + tm_boundary first loop
+// ------------------------------------------------------------------------------------------
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    const int nxt = cur ^ 1;
+    double vi[3] = {S.vx[cur][0][i], S.vx[cur][1][i], S.vx[cur][2][i]};
+    if (ti == 0 || ti == 7) {
+        for (int k = 0; k < 3; k++) S.vx[nxt][k][i] = vi[k];
+        return;
+    }
+    double xi[3], xj[3], dx[3], w[4];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const int nc = nb_count(S, i);
+    double tu[3] = {0, 0, 0}, dn = 0.0;
+    const double xsame = ti == 1 ? P.xsph[0] : ti == 2 ? P.xsph[1] : ti == 3 ? P.xsph[2] : P.xsph[1];
+    const double xbnd = (ti == 1 || ti == 3) ? 0.006 : P.xsph[1] / 5.0;
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        int tj = S.type[j];
+        double xs;
+        if (tj == ti) xs = xsame;
+        else if (tj == 0) xs = xbnd;
+        else continue;
+        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
+#pragma unroll
+        for (int k = 0; k < NDIM; k++) tu[k] = tu[k] + xs * mj * (S.vx[cur][k][j] - vi[k]) * w3 / rj;
+        dn = dn + mj * w3 / rj;
+    }
+    double v[3] = {vi[0], vi[1], vi[2]};
+    if (fabs(dn) > 1.0e-7 && (ti == 2 || ti == 4))
+        for (int k = 0; k < NDIM; k++) v[k] = vi[k] + tu[k] / dn;
+    else
+        for (int k = 0; k < NDIM; k++) v[k] = vi[k] + tu[k];
+    const double dr = P.dr;
+    // velocity_set_domain (flbndy 2, fluids) / tmboundary (flbndy 1), CPP_13-boundary.cpp:71-177
+    if ((P.flbndy == 2 && (ti == 1 || ti == 3)) || P.flbndy == 1) {
+        const double coe = P.flbndy == 1 ? 0.30 : P.coe_bndy;
+        double t1[3] = {0, 0, 0};
+        for (int nj = 0; nj < nc; nj++) {
+            int j = S.nbr[(size_t)nj * P.cap + i];
+            if (S.type[j] != 0) continue;
+            double inprod = 0.0, dst = 0.0, xjj[3];
+            for (int k = 0; k < NDIM; k++) xjj[k] = S.xp[cur][k][j];
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                inprod = inprod + (v[k] - S.interfss[k][j]) * (xi[k] - xjj[k]);
+                dst = dst + (xi[k] - xjj[k]) * (xi[k] - xjj[k]);
+            }
+            if (inprod < 0.0) {
+                dst = sqrt(dst);
+                double db = coe * dr;
+                double beta = 1 + db / (dst - coe * dr);
+                beta = fmin(1.5, beta);
+#pragma unroll
+                for (int k = 0; k < NDIM; k++) t1[k] = t1[k] + (1 - beta) * (xi[k] - xjj[k]) * inprod / dst / dst;
+            }
+        }
+        for (int k = 0; k < NDIM; k++) v[k] = v[k] + t1[k];
+    }
+    for (int k = 0; k < 3; k++) S.vx[nxt][k][i] = v[k];
+    if (P.flbndy != 2) return;
+    // tm_boundary_2d / 3d first loop (CPP_13-boundary.cpp:196-283, :411-520)
+    // ids of the first NDIM slots after clParti_Pair::dist_sorting (quirk A18: ids move, the
+    // distances stay in their slots), computed in one streaming pass over the boundary neighbours.
+    int cur_id[3] = {-1, -1, -1};
+    double d0[3] = {0, 0, 0};
+    int total = 0, idm = 0;
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        if (S.type[j] != 0) continue;
+        double xjj[3] = {S.xp[cur][0][j], S.xp[cur][1][j], NDIM == 3 ? S.xp[cur][2][j] : 0.0};
+        double dst = (xi[0] - xjj[0]) * (xi[0] - xjj[0]) + (xi[1] - xjj[1]) * (xi[1] - xjj[1]) +
+                     (xi[2] - xjj[2]) * (xi[2] - xjj[2]);
+        dst = sqrt(dst);
+        if (S.matype[j] > 2) idm = j;
+        int x = j;
+        if (total == 0) {
+            cur_id[0] = x;
+            d0[0] = dst;
+        } else {
+            int passes = total < NDIM ? total : NDIM;
+            for (int p = 0; p < passes; p++)
+                if (d0[p] > dst) {
+                    int t = cur_id[p];
+                    cur_id[p] = x;
+                    x = t;
+                }
+            if (total < NDIM) {
+                cur_id[total] = x;
+                d0[total] = dst;
+            }
+        }
+        total++;
+    }
+    double vb[3];
+    if (total >= NDIM) {
+        double dst;
+        const double p4[3] = {xi[0], xi[1], xi[2]};
+        double p1[3], p2[3];
+        for (int k = 0; k < 3; k++) {
+            p1[k] = (k < NDIM) ? S.xp[cur][k][cur_id[0]] : 0.0;
+            p2[k] = (k < NDIM) ? S.xp[cur][k][cur_id[1]] : 0.0;
+        }
+        if (NDIM == 2) {
+            // cal_line (C = 1, quirk A20) + cal_dpline, CPP_13-boundary.cpp:617-652
+            double den = p2[1] * p1[0] - p1[1] * p2[0];
+            double A = (p1[1] - p2[1]) / den, B = (p2[0] - p1[0]) / den;
+            dst = fabs(A * p4[0] + B * p4[1] + 1.0);
+            double tmp = sqrt(A * A + B * B);
+            dst = tmp > 1.0e-3 ? dst / tmp : 0.0;
+        } else {
+            double p3[3] = {S.xp[cur][0][cur_id[2]], S.xp[cur][1][cur_id[2]], S.xp[cur][2][cur_id[2]]};
+            auto plane = [](const double *a, const double *b, const double *c, double o[3]) {
+                double den = -a[0] * (b[1] * c[2] - c[1] * b[2]) - b[0] * (c[1] * a[2] - a[1] * c[2]) -
+                             c[0] * (a[1] * b[2] - b[1] * a[2]);
+                o[0] = a[1] * (b[2] - c[2]) + b[1] * (c[2] - a[2]) + c[1] * (a[2] - b[2]);
+                o[1] = a[2] * (b[0] - c[0]) + b[2] * (c[0] - a[0]) + c[2] * (a[0] - b[0]);
+                o[2] = a[0] * (b[1] - c[1]) + b[0] * (c[1] - a[1]) + c[0] * (a[1] - b[1]);
+                o[0] = o[0] / den;
+                o[1] = o[1] / den;
+                o[2] = o[2] / den;
+            };
+            double c1[3], c2[3], pl[4];
+            plane(p1, p2, p4, c1);
+            plane(p1, p3, p4, c2);
+            double cs = c1[0] * c2[0] + c1[1] * c2[1] + c1[2] * c2[2];
+            cs = cs / sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]) / sqrt(c2[0] * c2[0] + c2[1] * c2[1] + c2[2] * c2[2]);
+            // plane D: the reference's thread-private clPlane keeps D from an earlier
+            // plane_two_points call; this kernel uses the constructed value 1.0
+            pl[3] = 1.0;
+            if (fabs(cs) > 0.999999) {
+                double a = p4[0] - p1[0], b = p4[1] - p1[1], c = p4[2] - p1[2];
+                double t = -a * p1[0] - b * p1[1] - c * p1[2];
+                if (fabs(t) > 1.0e-6) { pl[0] = a / t; pl[1] = b / t; pl[2] = c / t; pl[3] = 1.0; }
+                else { pl[0] = a; pl[1] = b; pl[2] = c; pl[3] = 0.0; }
+            } else {
+                plane(p1, p2, p3, pl);
+            }
+            dst = fabs(pl[0] * p4[0] + pl[1] * p4[1] + pl[2] * p4[2] + pl[3]);
+            double tmp = sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+            dst = tmp > 1.0e-3 ? dst / tmp : 0.0;
+        }
+        double da = dst > 0.6 * dr ? dst - 0.5 * dr : 0.1 * dr;
+        double db = 0.5 * dr, beta;
+        if (fabs(da) > 1.0e-6) {
+            beta = 1.0 + db / da;
+            beta = fmin(1.5, beta);
+        } else {
+            beta = 1.5;
+        }
+        vb[0] = (1.0 - beta) * v[0] + beta * S.interfss[0][idm];
+        vb[1] = (1.0 - beta) * v[1] + beta * S.interfss[1][idm];
+        vb[2] = NDIM == 2 ? 0.0 : (1.0 - beta) * v[2] + beta * S.interfss[2][idm];
+    } else {
+        vb[0] = v[0]; vb[1] = v[1]; vb[2] = v[2];
+    }
+    for (int k = 0; k < 3; k++) S.vel_bndy[k][i] = vb[k];
+}
+
+// ------------------------------------------------------------------------------------------
+// tm_boundary second loop + check_domain + update
+// ------------------------------------------------------------------------------------------
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    const int nxt = cur ^ 1;
+    const int l = step_of(S);
+    const double dt = P.dt;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    if (ti == 0 && P.flbndy == 2) {
+        double xj[3], dx[3], w[4];
+        const double hli = S.hl[i];
+        const int nc = nb_count(S, i);
+        double tv[4][3], td[4] = {0, 0, 0, 0};
+        for (int a = 0; a < 4; a++) tv[a][0] = tv[a][1] = tv[a][2] = 0.0;
+        for (int nj = 0; nj < nc; nj++) {
+            int j = S.nbr[(size_t)nj * P.cap + i];
+            int tj = S.type[j];
+            if (tj < 1 || tj > 4) continue;
+            pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+            double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
+            int t = tj - 1;
+            tv[t][0] = tv[t][0] + mj * S.vel_bndy[0][j] * w3 / rj;
+            tv[t][1] = tv[t][0] + mj * S.vel_bndy[1][j] * w3 / rj;  // quirk A17
+            tv[t][2] = tv[t][0] + mj * S.vel_bndy[2][j] * w3 / rj;
+            td[t] = td[t] + mj * w3 / rj;
+        }
+        for (int t = 0; t < 4; t++)
+            for (int k = 0; k < 3; k++) S.vb[t][k][i] = fabs(td[t]) > 1.0e-6 ? tv[t][k] / td[t] : 0.0;
+    }
+    double v[3] = {S.vx[nxt][0][i], S.vx[nxt][1][i], S.vx[nxt][2][i]};
+    if (ti != 0 && ti != 7) {
+        // check_domain with the serial meaning of its shared flag (quirk A3)
+        bool hit = false;
+        if (xi[0] - P.bmax[0] >= -0.5 * P.dr) hit = true;
+        if (xi[0] - P.bmin[0] <= 0.5 * P.dr) hit = true;
+        if (xi[1] - P.bmax[1] >= -0.5 * P.dr) hit = true;
+        if (xi[1] - P.bmin[1] <= 0.5 * P.dr) hit = true;
+        if (NDIM == 3 && xi[2] - P.bmax[2] >= -0.5 * P.dr) hit = true;
+        if (NDIM == 3 && xi[2] - P.bmin[2] <= 0.5 * P.dr) hit = true;
+        if (hit) {
+            int b = P.first_moving_bndy;
+            for (int k = 0; k < 3; k++) v[k] = (b < i) ? S.interfss[k][b] : 0.0;
+            for (int k = 0; k < 3; k++) S.vx[nxt][k][i] = v[k];
+        }
+    }
+    const int loop = l - P.inip;
+    if (ti != 0) {
+        for (int k = 0; k < NDIM; k++) {
+            double vp = S.vxp[k][i];
+            double xn = xi[k] + (v[k] + vp) * dt * 0.5;
+            double u = S.ux[k][i];
+            if (loop == 0) u = 0.0;
+            else u = u + (v[k] + vp) * dt * 0.5;
+            S.ux[k][i] = u;
+            S.xp[nxt][k][i] = xn;
+            S.vxp[k][i] = v[k];
+            S.axp[k][i] = S.ax[k][i];
+        }
+        if (NDIM == 2) S.xp[nxt][2][i] = S.xp[cur][2][i];
+        S.rhop[i] = S.rho[i];
+    } else {
+        int mt = S.matype[i];
+        if (mt >= 2 && mt <= 5) {
+            for (int k = 0; k < NDIM; k++) {
+                double f = S.interfss[k][i];
+                S.xp[nxt][k][i] = xi[k] + (f + f) * dt * 0.5;
+                S.ux[k][i] = S.ux[k][i] + (f + f) * dt * 0.5;
+            }
+            if (NDIM == 2) S.xp[nxt][2][i] = S.xp[cur][2][i];
+            S.rhop[i] = S.rho[i];
+            S.prep[i] = S.pre[i];
+        } else {
+            for (int k = 0; k < 3; k++) S.xp[nxt][k][i] = S.xp[cur][k][i];
+        }
+    }
+}
+
+__global__ void k_step_inc(State S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *S.step += 1;
+}
+
+// explicit instantiations used by the host side
+template __global__ void k_bin<2>(State, Params, int);
+template __global__ void k_bin<3>(State, Params, int);
+template __global__ void k_search<2>(State, Params, int);
+template __global__ void k_search<3>(State, Params, int);
+template __global__ void k_density<2>(State, Params, int);
+template __global__ void k_density<3>(State, Params, int);
+template __global__ void k_norm<2>(State, Params, int);
+template __global__ void k_norm<3>(State, Params, int);
+template __global__ void k_strain<2>(State, Params, int);
+template __global__ void k_strain<3>(State, Params, int);
+template __global__ void k_accel<2>(State, Params, int);
+template __global__ void k_accel<3>(State, Params, int);
+template __global__ void k_xsph<2>(State, Params, int);
+template __global__ void k_xsph<3>(State, Params, int);
+template __global__ void k_update<2>(State, Params, int);
+template __global__ void k_update<3>(State, Params, int);
+
+}  // namespace sphg

@@ -1,0 +1,72 @@
+"""Pins the oracle (oracle/sph_oracle.c) to the reference solver.
+
+1. Against the committed golden fixtures (reference output, serial run): bit-equal at every stored
+   step in Gauss-Seidel density mode.
+2. Where the reference link harness is built (oracle/_ref), against fresh reference runs on larger
+   decks and more steps.
+"""
+import numpy as np
+import pytest
+
+from fixtures import load_golden, bitequal
+from helpers import OracleSim, GS, JACOBI, have_ref, run_ref, max_rel
+from sphgpy import column_collapse_2d, soil_water_column_3d
+from sphgpy.capi import PARTICLE_FIELDS
+
+COMPARE = [k for k in PARTICLE_FIELDS]
+
+
+def _mismatches(sim, ref):
+    bad = []
+    for k in COMPARE:
+        if k not in ref:
+            continue
+        got = sim.get(k)
+        if not bitequal(got, ref[k]):
+            bad.append((k, max_rel(got, ref[k])))
+    return bad
+
+
+@pytest.mark.parametrize("name", ["column2d_small", "soilwater3d_small"])
+def test_oracle_matches_golden_bitwise(name):
+    deck, states = load_golden(name)
+    sim = OracleSim(deck, states[0], mode=GS)
+    done = 0
+    for s in sorted(k for k in states if k > 0):
+        assert sim.step(s - done) == 0
+        done = s
+        assert _mismatches(sim, states[s]) == [], f"step {s}"
+
+
+def test_oracle_fixture_is_not_trivial():
+    # the compared fields actually move over the fixture horizon
+    deck, states = load_golden("column2d_small")
+    last = max(states)
+    assert np.max(np.abs(states[last]["vx"])) > 1e-6
+    assert np.max(np.abs(states[last]["sig"] - states[0]["sig"])) > 1.0
+
+
+def test_jacobi_differs_from_gs_only_slightly():
+    # the GPU's Jacobi density normalisation vs the reference's serial in-place sweep (quirk A1)
+    deck, states = load_golden("column2d_small")
+    a = OracleSim(deck, states[0], mode=GS)
+    b = OracleSim(deck, states[0], mode=JACOBI)
+    a.step(8)
+    b.step(8)
+    assert max_rel(b.get("rho"), a.get("rho")) < 1e-6
+    assert max_rel(b.get("xp"), a.get("xp")) < 1e-9
+    assert not bitequal(a.get("rho"), b.get("rho"))
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference link harness not built")
+@pytest.mark.parametrize("gen,steps", [
+    (lambda: column_collapse_2d(nx=24, ny=40, wall_h=50, floor_len=80, steps=40, seed=7), 40),
+    (lambda: soil_water_column_3d(nx=6, ny=5, nz_soil=6, nz_water=3, steps=12, seed=3), 12),
+])
+def test_oracle_matches_reference_every_step(gen, steps):
+    deck = gen()
+    dumps = run_ref(deck, steps, 1)
+    sim = OracleSim(deck, dumps[0], mode=GS)
+    for s in range(1, steps + 1):
+        assert sim.step(1) == 0
+        assert _mismatches(sim, dumps[s]) == [], f"step {s}"
