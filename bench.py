@@ -162,6 +162,9 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dname,
                 "alg_bytes_per_launch": int(alg), "avg_launch_us": round(avg_s * 1e6, 2)}
     step_alg = step_bytes(deck.ndim, n, nbar)
+    finite = bool(all(np.isfinite(g.get(k)).all() for k in ("xp", "vx", "sig", "rho")))
+    if not finite:
+        raise SystemExit("state became non-finite during the timed steps")
 
     line = {
         "metric": "particle-steps/s at 1 GPU, mixed vs FP64 at 1e-4 rel err; p99 time-to-result",

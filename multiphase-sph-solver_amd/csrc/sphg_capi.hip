@@ -237,7 +237,7 @@ static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, b
         {"e", S.e}, {"satu", S.satu}, {"dsatu", S.dsatu}, {"suct", S.suct}, {"cop", S.cop}, {"vsig", S.vsig},
         {"veps", S.veps}, {"divq", S.divq}, {"divr", S.divr}, {"meps", S.meps}, {"roue", S.roue},
         {"prew", S.prew}, {"prea", S.prea}, {"porosity", S.porosity}, {"cd", S.cd}, {"poro_cop", S.poro_cop},
-        {"color", S.color}, {"ps_c", c->d_psc}};
+        {"color", S.color}, {"ps_c", c->d_psc}, {"rho_mid", S.rho_mid}};
     for (auto &e : sc)
         if (nm == e.n) return s1(e.p);
     return -1;

@@ -274,8 +274,13 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
         S.porosity[i] = por;
         S.poro_cop[i] = pcop;
     }
-    if (ti == 7 || ti == 0) {
+    if (ti == 7) {
         S.rho_mid[i] = S.rho[i];
+        return;
+    }
+    if (ti == 0) {  // no continuity sum, but the reference still assigns rho = rhop + drhot*dt (:160-163)
+        double drhot = 0.0;
+        S.rho_mid[i] = S.rhop[i] + drhot * P.dt;
         return;
     }
     double vi[3] = {S.vxp[0][i], S.vxp[1][i], NDIM == 3 ? S.vxp[2][i] : 0.0};

@@ -339,8 +339,27 @@ static void or_celllist_ini(or_ctx *c) {
     }
 }
 
-/* setting_vis_cons (CPP_2-initialf.cpp:1021-1058) material slots, when the per-particle copies
- * were not uploaded explicitly the caller relies on these. */
+/* setting_vis_cons (CPP_2-initialf.cpp:1021-1058): per-particle copies of the material rows.
+ * Reads type/matype/etype; per-particle overrides (random fields) are set afterwards. */
+int or_setting_vis_cons(or_ctx *c) {
+    int i;
+    for (i = 0; i < c->n; i++) {
+        const OPar *p = &c->p[i];
+        int m = p->matype;
+        soil_default(&c->ps[i]);
+        fluid_default(&c->pf[i]);
+        if (p->type == 1 && m >= 0 && m < 3) c->pf[i] = c->mat.water[m];
+        if (p->type == 2) {
+            if (m < 30 && m >= 0) c->ps[i] = c->mat.soil[m];
+            else if (m >= 50) c->ps[i] = c->mat.soil[m % 10];
+        }
+        if (p->type == 3) c->pf[i] = c->mat.air;
+        if (p->type == 7 && p->etype == 1 && m >= 0 && m < 3) c->pf[i] = c->mat.water[m];
+        if (p->type == 7 && p->etype == 2 && m >= 0 && m < 30) c->ps[i] = c->mat.soil[m];
+    }
+    return 0;
+}
+
 int or_upload_done(or_ctx *c) {
     or_celllist_ini(c);
     return 0;
@@ -413,11 +432,7 @@ static int or_partisearching(or_ctx *c) {
     const sphg_problem *P = &c->pro;
     int n = c->n, dim = P->ndim, nst = dim == 2 ? 9 : 27, t = 0, i;
     int use_w1 = (P->flknl == 1 || P->flknl == 3);
-    for (i = 0; i < n; i++) {
-        c->pc[i].ninflu = 0;
-        memset(c->pc[i].influ, 0, sizeof c->pc[i].influ);
-        memset(c->pc[i].wij, 0, sizeof c->pc[i].wij);
-    }
+    for (i = 0; i < n; i++) c->pc[i].ninflu = 0; /* Par_Cell::initial (Class_NNPS.cpp:94-97) */
 #pragma omp parallel for schedule(static) num_threads(c->nthreads) reduction(max : t)
     for (i = 0; i < n; i++) {
         OPar *pi = &c->p[i];
@@ -656,7 +671,8 @@ static void or_permeability(or_ctx *c) {
 }
 
 /* ------------------------------------------------------------------ eos (CPP_7-1-eos_CR.cpp:12-178) */
-static void or_eos(or_ctx *c) {
+/* loop 1 (:20-43): per-particle pressure from density */
+static void or_eos1(or_ctx *c) {
     int n = c->n, i;
 #pragma omp parallel for schedule(static) num_threads(c->nthreads)
     for (i = 0; i < n; i++) {
@@ -674,6 +690,11 @@ static void or_eos(or_ctx *c) {
             pi->pre = pi->pre + pi->prep;
         }
     }
+}
+
+/* loop 2 (:45-177): pore pressures of solids and boundary stress from neighbours */
+static void or_eos2(or_ctx *c) {
+    int n = c->n, i;
 #pragma omp parallel for schedule(static) num_threads(c->nthreads)
     for (i = 0; i < n; i++) {
         OPar *pi = &c->p[i];
@@ -1635,20 +1656,32 @@ static int unsupported_flags(const or_ctx *c) {
     return 0;
 }
 
-int or_step(or_ctx *c, int nsteps) {
-    int s, e;
+/* One reference step as phase groups. The grouping only moves eos loop 1 (which reads the
+ * final densities and writes pressures nothing in between reads) ahead of the strain pass, so the
+ * arithmetic is the reference's loop body unchanged (CPP_1-SPHLoops.cpp:524-753). */
+int or_phase(or_ctx *c, int mask) {
+    int e;
     if (unsupported_flags(c)) return SPHG_E_UNSUPPORTED;
-    for (s = 0; s < nsteps; s++) {
+    if (mask & 1) {
         c->t = c->t + c->pro.dt;
         if ((e = or_parttocell(c))) return e;
         if ((e = or_partisearching(c))) return e;
+    }
+    if (mask & 2) {
         or_density(c);
+        or_eos1(c);
+    }
+    if (mask & 4) {
         or_strain(c);
         if (c->pro.fli == 1) or_permeability(c);
-        or_eos(c);
+        or_eos2(c);
+    }
+    if (mask & 8) {
         if ((e = or_strain_to_stress(c))) return e;
         or_free_surface(c);
         or_eig(c);
+    }
+    if (mask & 16) {
         /* CPP_1-SPHLoops.cpp:586-600: the water/air routine by acc_type of water[0] / air */
         or_fluid_accel(c, 1, 3, c->mat.water[0].acc_type == 0 ? 1 : -1);
         or_fluid_accel(c, 3, 1, c->mat.air.acc_type == 0 ? 1 : -1);
@@ -1658,17 +1691,27 @@ int or_step(or_ctx *c, int nsteps) {
         or_bndy_stress(c);
         if (c->pro.fli == 1) or_interact1(c);
         or_velocity(c);
+    }
+    if (mask & 32) {
         if (c->pro.if_move != 2) or_xsph(c);
-        if (c->pro.flbndy == 1) {
-            or_vel_repulse(c, 0.30, 0);
-        } else if (c->pro.flbndy == 2) {
+        if (c->pro.flbndy == 1) or_vel_repulse(c, 0.30, 0);
+        else if (c->pro.flbndy == 2) {
             or_vel_repulse(c, c->pro.coe_bndy, 1);
             or_tm_boundary(c);
         }
+    }
+    if (mask & 64) {
         or_check_domain(c);
         or_update(c);
         c->l++;
     }
+    return 0;
+}
+
+int or_step(or_ctx *c, int nsteps) {
+    int s, e;
+    for (s = 0; s < nsteps; s++)
+        if ((e = or_phase(c, 127))) return e;
     return 0;
 }
 
@@ -1681,6 +1724,13 @@ int or_get_neighbors(or_ctx *c, int *counts, int *ids, int n) {
         for (k = 0; k < SPHG_MAX_NB; k++) ids[(size_t)i * SPHG_MAX_NB + k] = k < nc ? c->pc[i].influ[k + 1] : -1;
     }
     return 0;
+}
+
+/* kernel values stored by the last list build for particle i (Par_Cell::wij[1..ninflu]) */
+int or_get_wij(or_ctx *c, int i, double *out, int cap) {
+    int nc = nbcount(&c->pc[i]), k;
+    for (k = 0; k < nc && k < cap; k++) memcpy(out + 4 * k, c->pc[i].wij[k + 1], 32);
+    return nc;
 }
 
 long long or_pair_count(or_ctx *c) {

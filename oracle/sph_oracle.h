@@ -30,11 +30,15 @@ int or_set_field(or_ctx *c, const char *name, const double *v, int n, int ncomp)
 int or_get_field(or_ctx *c, const char *name, double *v, int n, int ncomp);
 int or_set_field_i(or_ctx *c, const char *name, const int *v, int n);
 int or_get_field_i(or_ctx *c, const char *name, int *v, int n);
+int or_setting_vis_cons(or_ctx *c);
 int or_upload_done(or_ctx *c);
 void or_set_mode(or_ctx *c, int density_mode, int nthreads);
 void or_set_step(or_ctx *c, int l);
 /* Runs nsteps reference steps; returns 0 or the reference error code (3, 5). */
 int or_step(or_ctx *c, int nsteps);
+/* Phase groups of one step (bit 1 NNPS, 2 density+eos1, 4 strain+eos2, 8 stress, 16 accel+velocity,
+ * 32 xsph+boundary, 64 update); bits in increasing order make one step. */
+int or_phase(or_ctx *c, int mask);
 int or_get_neighbors(or_ctx *c, int *counts, int *ids, int n);
 /* Sum over particles of ninflu in the last step (directed pairs). */
 long long or_pair_count(or_ctx *c);

@@ -42,8 +42,11 @@ def oracle_lib() -> _Lib:
         L.or_create.restype = C.c_void_p
         L.or_destroy.argtypes = [C.c_void_p]
         L.or_upload_done.argtypes = [C.c_void_p]
+        L.or_setting_vis_cons.argtypes = [C.c_void_p]
         L.or_set_mode.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.or_set_step.argtypes = [C.c_void_p, C.c_int]
+        L.or_phase.argtypes = [C.c_void_p, C.c_int]
+        L.or_phase.restype = C.c_int
         L.or_pair_count.argtypes = [C.c_void_p]
         L.or_pair_count.restype = C.c_longlong
         _or = lib
@@ -60,6 +63,9 @@ class OracleSim(Sim):
         self._mat = materials_from_deck(deck)
         ctx = lib.lib.or_create(C.byref(self._pro), C.byref(self._mat), deck.n)
         super().__init__(lib, ctx, deck.n)
+        for k in INT_FIELDS:
+            self.set(k, state[k])
+        lib.lib.or_setting_vis_cons(ctx)
         self.load_state(state)
         for k, v in state.items():
             if k.startswith("ps_") or k.startswith("pf_") or k in ("vx_water", "vx_soil", "vx_air", "vx_struct", "vel_bndy"):
@@ -67,6 +73,9 @@ class OracleSim(Sim):
         lib.lib.or_upload_done(ctx)
         lib.lib.or_set_mode(ctx, mode, nthreads)
         lib.lib.or_set_step(ctx, step)
+
+    def phase(self, mask: int) -> int:
+        return int(self.L.lib.or_phase(self.ctx, mask))
 
     def pair_count(self) -> int:
         return int(self.L.lib.or_pair_count(self.ctx))
