@@ -1,0 +1,75 @@
+"""Summarises rocprofv3 runs of bench.py into a per-kernel table and profiles/pmc_<workload>.json.
+
+    python multiphase-sph-solver_amd/pmc_summary.py <kt_dir> <fetch_dir> <write_dir> <cfetch_dir> <cwrite_dir> \
+        <workload> <out.md> <out.json>
+
+Inputs are the output directories of: rocprofv3 --kernel-trace --stats (bench), --pmc FETCH_SIZE
+(bench), --pmc WRITE_SIZE (bench), and the two PMC passes over copy_calibration.py (1 GiB copy).
+"""
+import collections
+import csv
+import glob
+import json
+import os
+import sys
+
+
+def _csv(d, suffix):
+    f = glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)
+    return list(csv.DictReader(open(f[0]))) if f else []
+
+
+def counters(d):
+    agg = collections.defaultdict(list)
+    meta = {}
+    for r in _csv(d, "counter_collection.csv"):
+        agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        meta[r["Kernel_Name"]] = (int(r["VGPR_Count"]), int(r["Accum_VGPR_Count"]), int(r["Scratch_Size"]))
+    return agg, meta
+
+
+def short(name):
+    s = name.split("(")[0].replace("void ", "").replace("sphg::", "")
+    return s.split("<")[0].replace("k_", "")
+
+
+def main(kt, fe, wr, cfe, cwr, workload, out_md, out_json):
+    stats = _csv(kt, "kernel_stats.csv")
+    f, meta = counters(fe)
+    w, _ = counters(wr)
+    cf, _ = counters(cfe)
+    cw, _ = counters(cwr)
+    gib_kib = 1024.0 * 1024.0
+    copy_name = [k for k in cf if "copyBuffer" in k][0]
+    fscale = gib_kib / (sum(cf[copy_name]) / len(cf[copy_name]))
+    wscale = gib_kib / (sum(cw[copy_name]) / len(cw[copy_name]))
+    rows = []
+    js = {"workload": workload, "fetch_scale": fscale, "write_scale": wscale, "kernels": {}}
+    for r in stats:
+        name = r["Name"]
+        if "sphg" not in name:
+            continue
+        k = short(name)
+        kname = {"norm": "norm_eos"}.get(k, k)
+        fk = f.get(name, [0.0])
+        wk = w.get(name, [0.0])
+        fb = sum(fk) / len(fk) * 1024 * fscale
+        wb = sum(wk) / len(wk) * 1024 * wscale
+        vg = meta.get(name, (0, 0, 0))
+        rows.append((kname, int(r["Calls"]), float(r["AverageNs"]) / 1e3, float(r["Percentage"]), fb, wb, vg))
+        js["kernels"][kname] = {"avg_us": float(r["AverageNs"]) / 1e3, "fetch_bytes_per_launch": fb,
+                                "write_bytes_per_launch": wb, "hbm_bytes_per_launch": fb + wb,
+                                "vgpr": vg[0], "agpr": vg[1], "scratch": vg[2]}
+    with open(out_json, "w") as fo:
+        json.dump(js, fo, indent=1)
+    with open(out_md, "a") as fo:
+        fo.write(f"\nFETCH_SIZE scale {fscale:.4f}, WRITE_SIZE scale {wscale:.4f} "
+                 f"(1 GiB device copy read back as {gib_kib / fscale:.0f} / {gib_kib / wscale:.0f} KiB)\n\n")
+        fo.write("| kernel | calls | avg us | % time | HBM fetch MB/launch | HBM write MB/launch | VGPR | scratch |\n")
+        fo.write("|---|---|---|---|---|---|---|---|\n")
+        for k, c, us, pct, fb, wb, vg in rows:
+            fo.write(f"| {k} | {c} | {us:.1f} | {pct:.1f} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {vg[0]} | {vg[2]} |\n")
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:9])
