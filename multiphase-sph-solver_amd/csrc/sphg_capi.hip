@@ -136,6 +136,14 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
     plan.push_back({(void **)&S.fluid_slot, cap});
     I(S.cell, cap); I(S.sorted, cap); I(S.nb_cnt, cap);
     I(S.nbr, cap * kMaxNb);
+    for (int k = 0; k < 4; k++) {
+        if (k == 2 && pro->ndim == 2) {
+            S.wij[2] = nullptr;  // 2D: the z gradient is identically zero and is not stored
+            continue;
+        }
+        plan.push_back({(void **)&S.wij[k], cap * kMaxNb * 8});
+    }
+    for (int k = 0; k < 6; k++) D(S.qsig[k]);
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);

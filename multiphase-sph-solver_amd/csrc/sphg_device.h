@@ -3,9 +3,9 @@
 // State lives in HBM as structure-of-arrays: one contiguous FP64 array per scalar component
 // (a 3-vector is three arrays), so a wavefront of 64 consecutive particles reads 512 B per
 // component, fully coalesced. Neighbour lists are stored column-major, nbr[k * cap + i], for
-// the same reason. Kernel values W and grad W are never stored: every pair loop recomputes them
-// from the FP64 positions with the reference's own expression, which costs ~40 FP64 operations
-// per pair against 32 B per pair per pass of HBM traffic that a stored table would add.
+// the same reason. Kernel values W and grad W are computed once per step by the list build (with
+// the reference's own expression) and stored beside the ids in the same column-major layout; the
+// six pair passes read them back coalesced instead of re-deriving them from gathered positions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -72,6 +72,8 @@ struct State {
     // cell list
     int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *block_sums;
     int *nb_cnt, *nbr;
+    double *wij[4];       // kernel gradient (3) and value per list entry, column-major like nbr
+    double *qsig[6];      // sig / rho / rho per particle (the j term of the solid momentum sums)
     int *err, *step;      // sticky error word; current step number l
     const SoilSlot *soil;
     const FluidSlot *fluid;

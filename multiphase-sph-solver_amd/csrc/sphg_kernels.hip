@@ -30,6 +30,17 @@ __device__ __forceinline__ int step_of(const State &S) { return *S.step; }
 
 __device__ __forceinline__ void set_err(const State &S, int bit) { atomicOr(S.err, bit); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each with
+// its own L2. Remapping b so that XCD x processes one contiguous 1/8 of the particles keeps a
+// workgroup's neighbour gathers (which reach a few thousand indices either side) in the L2 that
+// already holds them instead of re-fetching them from HBM into eight different L2s.
+__device__ __forceinline__ int pidx() {
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    const unsigned x = b & 7u, r = b >> 3, q = nb >> 3, rem = nb & 7u;
+    const unsigned start = x * q + (x < rem ? x : rem);
+    return (int)((start + r) * blockDim.x + threadIdx.x);
+}
+
 template <int NDIM>
 __device__ __forceinline__ void load_x(const State &S, int cur, int i, double x[3]) {
     x[0] = S.xp[cur][0][i];
@@ -37,25 +48,21 @@ __device__ __forceinline__ void load_x(const State &S, int cur, int i, double x[
     x[2] = NDIM == 3 ? S.xp[cur][2][i] : 0.0;
 }
 
-// Pair geometry and kernel values exactly as partisearching computes them (CPP_3-NNPS.cpp:300-316).
+// Kernel values of list entry nj of particle i, written by k_search for this step.
 template <int NDIM>
-__device__ __forceinline__ void pair_w(const State &S, const Params &P, int cur, const double xi[3], double hli,
-                                       int j, double xj[3], double dx[3], double w[4]) {
+__device__ __forceinline__ void load_w(const State &S, const Params &P, int nj, int i, double w[4]) {
+    const size_t o = (size_t)nj * P.cap + i;
+    w[0] = S.wij[0][o];
+    w[1] = S.wij[1][o];
+    w[2] = NDIM == 3 ? S.wij[2][o] : 0.0;
+    w[3] = S.wij[3][o];
+}
+
+template <int NDIM>
+__device__ __forceinline__ void load_xj(const State &S, int cur, int j, double xj[3]) {
     xj[0] = S.xp[cur][0][j];
     xj[1] = S.xp[cur][1][j];
     xj[2] = NDIM == 3 ? S.xp[cur][2][j] : 0.0;
-    double dst = 0.0;
-#pragma unroll
-    for (int k = 0; k < NDIM; k++) {
-        dst = dst + (xi[k] - xj[k]) * (xi[k] - xj[k]);
-        dx[k] = xi[k] - xj[k];
-    }
-    if (NDIM == 2) dx[2] = 0.0;
-    dst = sqrt(dst);
-    double ra = (hli + S.hl[j]) / 2;
-    w[0] = w[1] = w[2] = w[3] = 0.0;
-    if (P.flknl == 1 || P.flknl == 3) weight_b(dst, ra, dx, w, NDIM);
-    else weight_y(dst, ra, dx, w, NDIM);
 }
 
 __device__ __forceinline__ int nb_count(const State &S, int i) {
@@ -73,7 +80,7 @@ __global__ void k_fill_int(int *a, int n, int v) {
 
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_bin(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const double xmin[3] = {P.xmin, P.ymin, P.zmin};
     long long nt[3] = {0, 0, 0};
@@ -163,7 +170,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(int *out, const int *bl
 }
 
 __global__ __launch_bounds__(256) void k_scatter(State S, Params P) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     int c = S.cell[i];
     int pos = S.cell_start[c] + atomicAdd(&S.cell_cursor[c], 1);
@@ -189,33 +196,46 @@ __global__ __launch_bounds__(256) void k_cell_order(State S, int ncells) {
 }
 
 // partisearching (CPP_3-NNPS.cpp:265-374): stencil cells in nncell order, candidates in cell
-// order, predicate dst <= 2*ra && i != j evaluated in FP64 exactly as the reference.
+// order, predicate sqrt(d2) <= 2*ra && i != j with d2 accumulated in FP64 exactly as the reference.
+// sqrt is monotone and correctly rounded, so candidates whose d2 lies outside a +-1e-14 relative
+// band around (2*ra)^2 are decided without it; only the band (on-cut-off pairs of the jittered
+// lattice) takes the exact sqrt test. Phase A stages the ids of each thread in LDS; phase B walks
+// the lists in lock-step over k so that ids and kernel values are written coalesced, column-major.
+constexpr int kSearchBlock = 256, kStage = 32;
+
 template <int NDIM>
-__global__ __launch_bounds__(256) void k_search(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int cur) {
+    __shared__ int stage[kStage][kSearchBlock];
+    const int i = pidx();
+    const int t = threadIdx.x;
     if (i >= P.n) return;
     int cnt = 0;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
     if (S.type[i] != 7) {
-        double xi[3];
-        load_x<NDIM>(S, cur, i, xi);
-        double hli = S.hl[i];
-        int c = S.cell[i];
+        const int c = S.cell[i];
         for (int s = 0; s < P.nstencil; s++) {
             int nl = c + P.nb_off[s];
             if (nl <= 0 || nl > P.ncell || c == 0) continue;
             int st = S.cell_start[nl], en = st + S.cell_count[nl];
             for (int q = st; q < en; q++) {
                 int j = S.sorted[q];
-                double dst = 0.0;
+                double d2 = 0.0;
 #pragma unroll
                 for (int k = 0; k < NDIM; k++) {
                     double d = xi[k] - S.xp[cur][k][j];
-                    dst = dst + d * d;
+                    d2 = d2 + d * d;
                 }
-                dst = sqrt(dst);
-                double ra = (hli + S.hl[j]) / 2;
-                if (dst <= 2 * ra && i != j) {
-                    if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = j;
+                double R = 2 * ((hli + S.hl[j]) / 2);
+                double R2 = R * R;
+                bool in;
+                if (d2 < R2 * 0.99999999999999) in = true;
+                else if (d2 > R2 * 1.00000000000001) in = false;
+                else in = sqrt(d2) <= R;
+                if (in && i != j) {
+                    if (cnt < kStage) stage[cnt][t] = j;
+                    else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = j;
                     cnt++;
                 }
             }
@@ -223,6 +243,37 @@ __global__ __launch_bounds__(256) void k_search(State S, Params P, int cur) {
         if (cnt > kMaxNb) set_err(S, SPHG_ERR_NB);
     }
     S.nb_cnt[i] = cnt;
+    const int nk = cnt < kMaxNb ? cnt : kMaxNb;
+    for (int k = 0; k < nk; k++) {
+        const size_t o = (size_t)k * P.cap + i;
+        int j;
+        if (k < kStage) {
+            j = stage[k][t];
+            S.nbr[o] = j;
+        } else {
+            j = S.nbr[o];
+        }
+        double xj[3], dx[3], w[4];
+        xj[0] = S.xp[cur][0][j];
+        xj[1] = S.xp[cur][1][j];
+        xj[2] = NDIM == 3 ? S.xp[cur][2][j] : 0.0;
+        double dst = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < NDIM; kk++) {
+            dst = dst + (xi[kk] - xj[kk]) * (xi[kk] - xj[kk]);
+            dx[kk] = xi[kk] - xj[kk];
+        }
+        if (NDIM == 2) dx[2] = 0.0;
+        dst = sqrt(dst);
+        double ra = (hli + S.hl[j]) / 2;
+        w[0] = w[1] = w[2] = w[3] = 0.0;
+        if (P.flknl == 1 || P.flknl == 3) weight_b(dst, ra, dx, w, NDIM);
+        else weight_y(dst, ra, dx, w, NDIM);
+        S.wij[0][o] = w[0];
+        S.wij[1][o] = w[1];
+        if (NDIM == 3) S.wij[2][o] = w[2];
+        S.wij[3][o] = w[3];
+    }
 }
 
 // Virtual velocity of boundary neighbour j seen by particle i of type ti (CPP_6-density.cpp:86-120).
@@ -245,7 +296,7 @@ __device__ __forceinline__ void bndy_vel(const State &S, const Params &P, int ti
 // ------------------------------------------------------------------------------------------
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
     const int l = step_of(S);
@@ -259,7 +310,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
         for (int nj = 0; nj < nc; nj++) {
             int j = S.nbr[(size_t)nj * P.cap + i];
             if (S.type[j] != 2) continue;
-            pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+            load_w<NDIM>(S, P, nj, i, w);
             double mj = S.mass[j], rj = S.rho[j];
             up = up + mj * S.porosity[j] / rj * w[3];
             dn = dn + mj / rj * w[3];
@@ -298,7 +349,8 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
         } else {
             continue;
         }
-        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        load_w<NDIM>(S, P, nj, i, w);
+        if (solid) load_xj<NDIM>(S, cur, j, xj);
         double mj = S.mass[j];
 #pragma unroll
         for (int k = 0; k < NDIM; k++) {
@@ -326,7 +378,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
 // density loop 3 (Jacobi form of CPP_6-density.cpp:168-188) and eos loop 1 (CPP_7-1-eos_CR.cpp:20-43)
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
     double xi[3], xj[3], dx[3], w[4];
@@ -337,7 +389,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     for (int nj = 0; nj < nc; nj++) {
         int j = S.nbr[(size_t)nj * P.cap + i];
         if (S.type[j] != ti) continue;
-        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        load_w<NDIM>(S, P, nj, i, w);
         double mj = S.mass[j];
         up = up + mj * w[3];
         dn = dn + mj / S.rho_mid[j] * w[3];
@@ -367,7 +419,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
     const int l = step_of(S);
@@ -386,7 +438,8 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     for (int nj = 0; nj < nc; nj++) {
         int j = S.nbr[(size_t)nj * P.cap + i];
         int tj = S.type[j];
-        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        load_w<NDIM>(S, P, nj, i, w);
+        if (solid) load_xj<NDIM>(S, cur, j, xj);
         double mj = S.mass[j], rj = S.rho[j];
         // strain (CPP_7-1-strain.cpp:67-150)
         bool act = false;
@@ -605,7 +658,7 @@ __device__ __forceinline__ int model_of(int matype) {
 }
 
 __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
     const int l = step_of(S);
@@ -827,6 +880,15 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
     if (ti != 0 && ti != 7) {
         for (int k = 0; k < 6; k++) S.sig[k][i] = sig[k];
     }
+    // sig / rho / rho, the per-neighbour factor of the solid momentum sums (same operation order)
+    {
+        const double r = S.rho[i];
+        const int keep = (ti != 0 && ti != 7);
+        for (int k = 0; k < 6; k++) {
+            double sk = keep ? sig[k] : S.sig[k][i];
+            S.qsig[k][i] = sk / r / r;
+        }
+    }
     // principal stresses (CPP_7-1-strain.cpp:283-332)
     double a[9] = {sig[0], sig[3], sig[5], sig[3], sig[1], sig[4], sig[5], sig[4], sig[2]}, v[9];
     eejcb3(a, v, 0.001, 7);
@@ -843,7 +905,7 @@ __device__ __constant__ int kSidx[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
 
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
     double xi[3], xj[3], dx[3], w[4];
@@ -889,12 +951,14 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     double fluid_t[3][3], solid_t[3][3], bnd_t[3][3], av[3] = {0, 0, 0}, drag[3] = {0, 0, 0};
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) fluid_t[a][b] = solid_t[a][b] = bnd_t[a][b] = 0.0;
-    double aif[6];
+    double aif[6], qi[6];
     for (int k = 0; k < 6; k++) aif[k] = fpor * sigi[k];
+    for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
     for (int nj = 0; nj < nc; nj++) {
         int j = S.nbr[(size_t)nj * P.cap + i];
         int tj = S.type[j];
-        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        load_w<NDIM>(S, P, nj, i, w);
+        load_xj<NDIM>(S, cur, j, xj);
         double mj = S.mass[j], rj = S.rho[j];
         if ((ti == 1 || ti == 3) && (tj == ti || tj == fother)) {
             // fluid_acceleration_water/air (+ minor variants), CPP_9-4-derivation_stress.cpp:12-351
@@ -911,12 +975,15 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         if ((ti == 2 || ti == 4) && tj == ti) {
             // soil_acceleration_noreg / structure_acceleration_noreg (:354-505)
             double satu = S.satu[j];  // quirk: neighbour saturation used for both terms
+            double qj[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) qj[k] = S.qsig[k][j];  // sig_j / rho_j / rho_j, precomputed
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
                     int s = kSidx[a][b];
-                    solid_t[a][b] = solid_t[a][b] + mj * (S.sig[s][j] / rj / rj + sigi[s] / rhoi / rhoi) * w[b];
+                    solid_t[a][b] = solid_t[a][b] + mj * (qj[s] + qi[s]) * w[b];
                 }
             double prewj = S.prew[j], preaj = S.prea[j];
             if (ti == 2) {
@@ -1065,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
     const int nxt = cur ^ 1;
@@ -1088,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
         if (tj == ti) xs = xsame;
         else if (tj == 0) xs = xbnd;
         else continue;
-        pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+        load_w<NDIM>(S, P, nj, i, w);
         double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
 #pragma unroll
         for (int k = 0; k < NDIM; k++) tu[k] = tu[k] + xs * mj * (S.vx[cur][k][j] - vi[k]) * w3 / rj;
@@ -1230,7 +1297,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
     const int nxt = cur ^ 1;
@@ -1248,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
             int j = S.nbr[(size_t)nj * P.cap + i];
             int tj = S.type[j];
             if (tj < 1 || tj > 4) continue;
-            pair_w<NDIM>(S, P, cur, xi, hli, j, xj, dx, w);
+            load_w<NDIM>(S, P, nj, i, w);
             double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
             int t = tj - 1;
             tv[t][0] = tv[t][0] + mj * S.vel_bndy[0][j] * w3 / rj;
