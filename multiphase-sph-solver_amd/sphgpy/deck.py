@@ -370,3 +370,51 @@ def soil_water_column_3d(nx=64, ny=32, nz_soil=64, nz_water=16, dx=0.005, steps=
     water = _lattice3d(0, nx, 0, ny, nz_soil, nz_soil + nz_water, dx)
     walls = _box_walls_3d(nx, ny, nz_soil + nz_water + 4, dx)
     return _assemble(d, [(soil, 2, 2), (water, 1, 0), (walls, 0, 0)], rng)
+
+
+def _layer(model: int, **kw) -> Dict[str, float]:
+    d = default_soil()
+    base = dict(poi=0.3, dens=2000.0, eps0=0.6, kapa=0.04, fs=1.0, cop=1e-4, e=2.0e6, ds=1.0, sitas=1.0, sitar=0.2,
+                sd=1.0, sw=1.0, c1=1.0, c2=1.0)
+    base.update(kw)
+    d.update(base)
+    return d
+
+
+def model_zoo_2d(nx=12, nbands=7, band_h=3, water_h=3, dx=0.002, steps=12, seed=5, art_stress=0.3,
+                 step_regu=3, inip=3) -> Deck:
+    """2D column of horizontal soil bands, one per constitutive model (elastic, SB Cam-Clay, DP,
+    UMCC, AMCC, Bui DP, FOCC), under a water layer, with artificial stress, periodic
+    strain/stress regularisation and initial geostress damping switched on: a deck that drives
+    every per-particle branch of the step loop the benchmark decks leave idle."""
+    rng = np.random.default_rng(seed)
+    d = Deck(ndim=2, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=(0.0, -9.81, 0.0),
+             art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), art_stre_coe=art_stress, step_regu=step_regu,
+             inip=inip, flbndy=2, fli=1, permtype=1, pers=1e-4, perl=1e-3, perm=5e-4, interc=1.0, damp_coe=0.05,
+             name="model_zoo_2d")
+    models = [0, 1, 2, 3, 4, 5, 8][:nbands]
+    d.model_flags = [1 if m in models else 0 for m in range(10)]
+    d.soil_layers = {
+        0: [_layer(0, e=3.0e6)],
+        1: [_layer(1, zmf=1.2, sme0=4000.0, ramda=0.2, kapa=0.04, ann=0.5, ocr=1.5, e=0.5)],
+        2: [_layer(2, c=500.0, fai=30.0, e=1.8e6)],
+        3: [_layer(3, zmf=1.2, sme0=4000.0, ramda=0.2, kapa=0.04, ann=0.5, bnn=0.5, beta=1.0, epss=0.6,
+                   epsr=0.9, sitas=1.0, sitar=0.3, sd=1.0, sw=1.0, kse=1.0, c1=1.0, c2=1.0, c3=1.0, e=0.5)],
+        4: [_layer(4, zmf=1.2, sme0=4000.0, ramda=0.2, kapa=0.04, mzm=1.0, azm=0.5, r0=1.0, bl=0.95, bz=1.0,
+                   zet=0.0, e=0.5)],
+        5: [_layer(5, c=300.0, fai=28.0, beta=5.0, e=1.5e6)],
+        8: [_layer(8, c=0.7, eps0=0.8, sme0=4000.0, g0=120.0, zmf=1.2, ramda=0.1, etao=1.6, zet=0.5, ann=1.0,
+                   bnn=1.0, damp1=1.0, damp2=0.1)],
+    }
+    w = default_fluid()
+    w.update(dens=1000.0, vis=1e-3, vpu=30.0, press_type=0, acc_type=0, bhtype=0, pre_coe=10.0, strain_min=1.0)
+    d.water = [dict(w), dict(w), dict(w)]
+    parts = []
+    for b, m in enumerate(models):
+        parts.append((_lattice2d(0, nx, b * band_h, (b + 1) * band_h, dx), 2, m, 1 + (b % 3)))
+    top = len(models) * band_h
+    parts.append((_lattice2d(0, nx, top, top + water_h, dx), 1, 0))
+    parts.append((_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0))
+    parts.append((_lattice2d(-3, 0, 0, top + water_h + 3, dx), 0, 0))
+    parts.append((_lattice2d(nx, nx + 3, 0, top + water_h + 3, dx), 0, 0))
+    return _assemble(d, parts, rng)

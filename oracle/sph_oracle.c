@@ -26,6 +26,7 @@ typedef struct {
     double satu, dsatu, suct, cop, vsig, veps, divq, divr, meps, roue, rous, prew, prea;
     double porosity, cd, poro_cop;
     double beta[3][3], beta_ts[3][3];
+    double Rij[3][3]; /* clRaij (Class_particle.h:181-187) */
     int type, matype, permtype, etype;
 } OPar;
 
@@ -182,7 +183,7 @@ static const FieldDesc PFIELDS[] = {
     FD(pre, 1), FD(prep, 1), FD(e, 1), FD(eps, 6), FD(deps, 6), FD(sig, 6), FD(dsig, 6),
     FD(weps, 9), FD(adps, 6), FD(satu, 1), FD(dsatu, 1), FD(suct, 1), FD(cop, 1), FD(vsig, 1),
     FD(veps, 1), FD(divq, 1), FD(divr, 1), FD(meps, 1), FD(roue, 1), FD(rous, 1), FD(prew, 1),
-    FD(prea, 1), FD(porosity, 1), FD(cd, 1), FD(poro_cop, 1), FD(beta, 9), FD(beta_ts, 9),
+    FD(prea, 1), FD(porosity, 1), FD(cd, 1), FD(poro_cop, 1), FD(beta, 9), FD(beta_ts, 9), FD(Rij, 9),
 };
 #undef FD
 #define SD(nm) {"ps_" #nm, offsetof(sphg_soil, nm), 1}
@@ -907,6 +908,250 @@ static void or_dpmodel1(double D[6][6], const OPar *p, const sphg_soil *s, doubl
     }
 }
 
+/* dpmodel_bui: DP with a separate dilatancy angle (CPP_8-1-consfactor.cpp:451-615) */
+static void or_dpmodel_bui(double D[6][6], const OPar *p, const sphg_soil *s, double adps[6]) {
+    double fai = atan(tan(s->fai * 0.0174532925166667) / s->fs);
+    double fai_dil = atan(tan(s->beta * 0.0174532925166667) / s->fs);
+    double ci = s->c / s->fs, poi = s->poi, eps0 = s->eps0;
+    double sig[6], sm, e0, dlam, dmu, dmu2, dx, dy, dz, dxy, dyz, dzx, sjj, sj, yalf, ykap;
+    int k, a, b;
+    for (k = 0; k < 6; k++) sig[k] = -p->sig[k];
+    for (a = 0; a < 6; a++)
+        for (b = 0; b < 6; b++) D[a][b] = 0.0;
+    sm = (sig[0] + sig[1] + sig[2]) * 0.3333333333;
+    if (sm < 1.0) sm = 1.0;
+    e0 = s->e > 1.0 ? s->e : 3 * (1 - 2 * poi) * (1 + eps0) * fabs(sm) / s->kapa;
+    dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi);
+    dmu = e0 * 0.5 / (1.0 + poi);
+    dmu2 = dmu * 2.0;
+    iso_tangent(D, e0, poi);
+    dx = sig[0] - sm; dy = sig[1] - sm; dz = sig[2] - sm;
+    dxy = sig[3]; dyz = sig[4]; dzx = sig[5];
+    sjj = 0.5 * (dx * dx + dy * dy + dz * dz) + dxy * dxy + dyz * dyz + dzx * dzx;
+    sj = sqrt(sjj);
+    yalf = sin(fai) / sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
+    ykap = 3.0 * ci * cos(fai) / sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
+    if ((sj - yalf * sm * 3.0 - ykap) >= 0.0) {
+        double f11 = -yalf + dx / sj * 0.5, f22 = -yalf + dy / sj * 0.5, f33 = -yalf + dz / sj * 0.5;
+        double f12 = dxy / sj * 0.5, f23 = dyz / sj * 0.5, f31 = dzx / sj * 0.5;
+        double fkk = f11 + f22 + f33;
+        double yd = sin(fai_dil) / sqrt(9.0 + 3.0 * sin(fai_dil) * sin(fai_dil));
+        double g11 = -yd + dx / sj * 0.5, g22 = -yd + dy / sj * 0.5, g33 = -yd + dz / sj * 0.5;
+        double g12 = dxy / sj * 0.5, g23 = dyz / sj * 0.5, g31 = dzx / sj * 0.5;
+        double hdd = g11 + g22 + g33;
+        double fij = 2.0 * (f11 * g11 + f22 * g22 + f33 * g33) + f12 * g12 + f23 * g23 + f31 * g31;
+        double cc7 = dlam * (fkk * hdd) + dmu * fij + hdd, d = 1.0 / cc7, df = dlam * fkk, dg = dlam * hdd;
+        double ramda = (dmu2 * f11 - 3.0 * dlam * yalf) * p->deps[0] + (dmu2 * f22 - 3.0 * dlam * yalf) * p->deps[1] +
+                       (dmu2 * f33 - 3.0 * dlam * yalf) * p->deps[2] +
+                       dmu * (f12 * p->deps[3] + f23 * p->deps[4] + f31 * p->deps[5]);
+        if (ramda * d < 0.0) {
+            double w[6] = {dg + dmu2 * g11, dg + dmu2 * g22, dg + dmu2 * g33, dmu * g12, dmu * g23, dmu * g31};
+            double z[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+            plastic_update(D, d, w, z);
+            adps[0] = ramda * d * f11; adps[1] = ramda * d * f22; adps[2] = ramda * d * f33;
+            adps[3] = ramda * d * f12; adps[4] = ramda * d * f23; adps[5] = ramda * d * f31;
+        }
+    }
+}
+
+/* sub_camunsatu: unsaturated Cam-Clay (CPP_8-4-consfactor-umcc.cpp:17-180); erfs before the
+ * tangent update, 0.333333333 as written there */
+static void or_sub_camunsatu(double D[6][6], double erfs[6], const OPar *p, const sphg_soil *s, double adps[6]) {
+    double poi = s->poi, eps0 = s->eps0, sm, e0, hee, dlam, dmu, dmu2, sig[6];
+    double qq = (s->epsr - s->epss) / (s->sitas - s->sitar), ymf, dx, dy, dz, dxy, dyz, dzx, sjj, tt1, tt2;
+    double f11, f22, f33, f12, f23, f31, rous, roue, ttt, hdd, fkk, fij, cc7, d, df, dg, ramda, omake;
+    int k;
+    for (k = 0; k < 6; k++) sig[k] = -p->sig[k];
+    ymf = 3.0 * (s->zmf - 1.0) / (s->zmf + 2.0) / s->fs;
+    sm = (sig[0] + sig[1] + sig[2]) * 0.333333333;
+    if (sm < 1.0) sm = 1.0;
+    e0 = s->e > 1.0 ? s->e : 3 * (1 - 2 * poi) * (1 + eps0) * fabs(sm) / s->kapa;
+    hee = (s->ramda - s->kapa) / (1.0 + eps0);
+    dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi);
+    dmu = e0 * 0.5 / (1.0 + poi);
+    dmu2 = dmu * 2.0;
+    iso_tangent(D, e0, poi);
+    dx = sig[0] - sm; dy = sig[1] - sm; dz = sig[2] - sm;
+    dxy = sig[3]; dyz = sig[4]; dzx = sig[5];
+    sjj = 0.5 * (dx * dx + dy * dy + dz * dz) + dxy * dxy + dyz * dyz + dzx * dzx;
+    tt1 = 1.0 / (ymf * ymf * sm * sm + 3.0 * sjj);
+    tt2 = 2.0 * ymf * ymf * sm * tt1 - 1.0 / sm;
+    f11 = tt2 * 0.333333333 + dx * tt1 * 3.0;
+    f22 = tt2 * 0.333333333 + dy * tt1 * 3.0;
+    f33 = tt2 * 0.333333333 + dz * tt1 * 3.0;
+    f12 = 3.0 * dxy * tt1;
+    f23 = 3.0 * dyz * tt1;
+    f31 = 3.0 * dzx * tt1;
+    rous = qq * (s->sitas - p->satu);
+    roue = hee * (1.0 + s->eps0) * log(s->sme0 / sm);
+    ttt = s->ann * roue + s->bnn * rous;
+    ttt = pow(ttt, s->beta);
+    hdd = f11 + f22 + f33 + ttt / sm;
+    fkk = f11 + f22 + f33;
+    fij = 2.0 * (f11 * f11 + f22 * f22 + f33 * f33) + f12 * f12 + f23 * f23 + f31 * f31;
+    cc7 = dlam * (fkk * fkk) + dmu * fij + hdd / hee;
+    d = 1.0 / cc7;
+    df = dlam * fkk;
+    dg = dlam * fkk;
+    ramda = (dmu2 * f11 + dlam * fkk) * p->deps[0] + (dmu2 * f22 + dlam * fkk) * p->deps[1] +
+            (dmu2 * f33 + dlam * fkk) * p->deps[2] + dmu * (f12 * p->deps[3] + f23 * p->deps[4] + f31 * p->deps[5]);
+    omake = qq * p->dsatu / (s->ramda - s->kapa);
+    ramda = ramda + omake;
+    if (ramda < 0.0) {
+        erfs[0] = omake * d * (D[0][0] * f11 + D[0][1] * f22 + D[0][2] * f33);
+        erfs[1] = omake * d * (D[1][0] * f11 + D[1][1] * f22 + D[1][2] * f33);
+        erfs[2] = omake * d * (D[2][0] * f11 + D[2][1] * f22 + D[2][2] * f33);
+        erfs[3] = omake * d * (D[3][3] * f12);
+        erfs[4] = omake * d * (D[4][4] * f23);
+        erfs[5] = omake * d * (D[5][5] * f31);
+        {
+            double w[6] = {dg + dmu2 * f11, dg + dmu2 * f22, dg + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+            double z[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+            plastic_update(D, d, w, z);
+        }
+        adps[0] = ramda * d * f11; adps[1] = ramda * d * f22; adps[2] = ramda * d * f33;
+        adps[3] = ramda * d * f12; adps[4] = ramda * d * f23; adps[5] = ramda * d * f31;
+    }
+}
+
+/* detA: recursive cofactor determinant (CPP_8-5-consfactor-focc.cpp:275-302) */
+static double or_detA(double arr[6][6], int n) {
+    double bak[6][6], sum = 0;
+    int i, j, k, cc;
+    if (n == 1) return arr[0][0];
+    for (i = 0; i < n; i++) {
+        for (j = 1; j < n; j++) {
+            for (cc = 0, k = 0; k < n; k++) {
+                if (k == i) continue;
+                bak[j - 1][cc++] = arr[j][k];
+            }
+        }
+        sum += (i % 2 == 0 ? 1 : -1) * arr[0][i] * or_detA(bak, n - 1);
+    }
+    return sum;
+}
+
+/* getAStart: adjugate by cofactors (CPP_8-5-consfactor-focc.cpp:305-336) */
+static void or_adjugate(double a[6][6], int n, double ans[6][6]) {
+    int i, j, k, t;
+    double tmp[6][6];
+    if (n == 1) {
+        ans[0][0] = 1.0;
+        return;
+    }
+    for (i = 0; i < n; i++)
+        for (j = 0; j < n; j++) {
+            for (k = 0; k < n - 1; k++)
+                for (t = 0; t < n - 1; t++) tmp[k][t] = a[k >= i ? k + 1 : k][t >= j ? t + 1 : t];
+            ans[j][i] = or_detA(tmp, n - 1);
+            if ((i + j) % 2 == 1) ans[j][i] = -ans[j][i];
+        }
+}
+
+/* fractional_mcc (CPP_8-5-consfactor-focc.cpp:13-240); quirks: sm = sum / 0.33333333 (A13),
+ * e approximated by 2.71828, p0 = pa = 98000 */
+static void or_fractional_mcc(double D[6][6], double C[6][6], const OPar *p, const sphg_soil *s) {
+    double sig[6], sij[6], sm, q, s_val, skl, smn[3][3], sita, poi = s->poi, h1 = s->damp1, h2 = s->damp2;
+    double ramda = s->ramda, mc = s->zmf, alpha = s->ann, beta = s->bnn, c = s->c, zeta = s->zet, etao = s->etao;
+    double g0 = s->g0, p0 = 98000, pa = 98000, m, ita, df, dg, pc, qc, e = p->e, mf, order, mij[6], nij[6], hp;
+    double P[6][6], T[6][6], adj[6][6], g, e0, det_a, dlam, dmu, dmu2, ramda_cc, ymf, tt1, tt2;
+    double f11, f22, f33, f12, f23, f31, fkk;
+    int a, b, kk;
+    for (a = 0; a < 6; a++) {
+        sig[a] = -p->sig[a];
+        for (b = 0; b < 6; b++) C[a][b] = P[a][b] = 0.0;
+    }
+    sm = (sig[0] + sig[1] + sig[2]) / 0.33333333;
+    for (a = 0; a < 6; a++) sij[a] = a < 3 ? sig[a] - sm : sig[a];
+    q = 0.5 * (sij[0] * sij[0] + sij[1] * sij[1] + sij[2] * sij[2]) + sij[3] * sij[3] + sij[4] * sij[4] + sij[5] * sij[5];
+    q = sqrt(3.0 * q);
+    ita = q / sm;
+    if (ita == 0.0) {
+        ita = 0.000001;
+        q = ita * sm;
+    }
+    g = g0 * (2.97 - e) * (2.97 - e) / (1.0 + e) * sqrt(sm * pa);
+    e0 = 2.0 * g * (1.0 + poi);
+    ramda_cc = 0.0;
+    ymf = mc;
+    dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi);
+    dmu = e0 * 0.5 / (1.0 + poi);
+    dmu2 = dmu * 2.0;
+    tt1 = 1.0 / (ymf * ymf * sm * sm + q * q);
+    tt2 = 2.0 * ymf * ymf * sm * tt1 - 1.0 / sm;
+    f11 = tt2 * 0.33333333 + sij[0] * tt1 * 3.0;
+    f22 = tt2 * 0.33333333 + sij[1] * tt1 * 3.0;
+    f33 = tt2 * 0.33333333 + sij[2] * tt1 * 3.0;
+    f12 = 3.0 * sij[3] * tt1;
+    f23 = 3.0 * sij[4] * tt1;
+    f31 = 3.0 * sij[5] * tt1;
+    fkk = f11 + f22 + f33;
+    ramda_cc = (dmu2 * f11 + dlam * fkk) * p->deps[0] + (dmu2 * f22 + dlam * fkk) * p->deps[1] +
+               (dmu2 * f33 + dlam * fkk) * p->deps[2] + dmu * (f12 * p->deps[3] + f23 * p->deps[4] + f31 * p->deps[5]);
+    smn[0][0] = sij[0]; smn[0][1] = sij[3]; smn[0][2] = sij[5];
+    smn[1][0] = sij[3]; smn[1][1] = sij[1]; smn[1][2] = sij[4];
+    smn[2][0] = sij[5]; smn[2][1] = sij[4]; smn[2][2] = sij[2];
+    s_val = 0.0;
+    for (a = 0; a < 3; a++)
+        for (b = 0; b < 3; b++)
+            for (kk = 0; kk < 3; kk++) s_val = s_val + smn[a][b] * smn[b][kk] * smn[kk][a];
+    skl = 0.0;
+    for (a = 0; a < 3; a++)
+        for (b = 0; b < 3; b++) skl = skl + smn[a][b] * smn[b][a];
+    if (skl == 0.0) skl = 0.000001;
+    sita = 9.0 * s_val * 0.5 / pow(1.5 * skl, 1.5);
+    if (sita < -1.0) sita = -1.0;
+    else if (sita > 1.0) sita = 1.0;
+    sita = asin(sita) * 0.33333333;
+    c = pow(c, 4);
+    m = 2 * c / (1 + c + (1 - c) * sin(3.0 * sita));
+    m = pow(m, 0.25);
+    m = mc * m;
+    pc = pa * pow((etao - e) / ramda, 1 / zeta);
+    qc = q + m * (sm - pc);
+    df = (sm - p0 / 2) * mc * mc / q;
+    dg = ((sm - pc) + (2 - alpha) * (pc - sm * 0.5 - q * q / mc / mc / sm / 2)) / ((q - qc) + (2 - alpha) * qc);
+    dg = pow(m, alpha + 1) * dg;
+    for (a = 0; a < 6; a++) {
+        double base = 3.0 * sij[a] * 0.5 / q;
+        mij[a] = a < 3 ? 1.0 / sqrt(1.0 + df * df) * (base + df * 0.33333333) : 1.0 / sqrt(1.0 + df * df) * base;
+        nij[a] = a < 3 ? 1.0 / sqrt(1.0 + dg * dg) * (base + dg * 0.33333333) : 1.0 / sqrt(1.0 + dg * dg) * base;
+    }
+    order = beta * (e - etao + ramda * pow(sm / pa, zeta));
+    mf = mc * pow(2.71828, -order);
+    hp = (h1 - h2 * e) * (1 - e) * g * (mf - ita) / ita * pow(2.71828, order);
+    C[0][0] = C[1][1] = C[2][2] = 1.0 / e0;
+    C[0][1] = C[0][2] = C[1][0] = C[1][2] = C[2][0] = C[2][1] = -poi / e0;
+    C[3][3] = C[4][4] = C[5][5] = 1.0 / g;
+    for (a = 0; a < 6; a++)
+        for (b = 0; b < 6; b++) P[a][b] = nij[a] * mij[b] / hp;
+    for (a = 0; a < 6; a++)
+        for (b = 0; b < 6; b++) T[a][b] = ramda_cc < 0.0 ? C[a][b] + P[a][b] : C[a][b];
+    det_a = or_detA(T, 6);
+    or_adjugate(T, 6, adj);
+    for (a = 0; a < 6; a++)
+        for (b = 0; b < 6; b++) {
+            if (det_a != 0.0)
+                D[a][b] = adj[a][b] / det_a;
+            else {
+                det_a = or_detA(C, 6);
+                or_adjugate(C, 6, adj);
+                D[a][b] = adj[a][b] / det_a;
+            }
+        }
+}
+
+/* fractional_plastic (CPP_8-5-consfactor-focc.cpp:243-259) */
+static void or_fractional_plastic(const double dsig[6], const double deps[6], double adps[6], double C[6][6]) {
+    double ee[6];
+    int a, b;
+    for (a = 0; a < 6; a++) {
+        ee[a] = 0.0;
+        for (b = 0; b < 6; b++) ee[a] = ee[a] + C[a][b] * dsig[b];
+    }
+    for (a = 0; a < 6; a++) adps[a] = deps[a] - ee[a];
+}
+
 /* DPModel_check: stress-scaling return onto the DP cone (CPP_8-1-consfactor.cpp:779-819); sj is
  * taken from the stress before the increment (as the reference does). */
 static void or_dp_check(OPar *p, const sphg_soil *s, double dt) {
@@ -966,7 +1211,7 @@ static int or_strain_to_stress(or_ctx *c) {
 #pragma omp parallel for schedule(static) num_threads(c->nthreads) reduction(max : bad)
     for (i = 0; i < n; i++) {
         OPar *pi = &c->p[i];
-        double D[6][6], erfs[6], adps[6], deps[6];
+        double D[6][6], erfs[6], adps[6], deps[6], Cf[6][6];
         int k, a, b;
         if (pi->type == 1 || pi->type == 3) {
             const sphg_fluid *f = &c->pf[i];
@@ -1010,13 +1255,25 @@ static int or_strain_to_stress(or_ctx *c) {
                 or_dpmodel1(D, pi, s, adps);
                 apply_tangent(pi, D);
                 or_dp_check(pi, s, dt);
+            } else if (mt == 3) {
+                or_sub_camunsatu(D, erfs, pi, s, adps);
+                apply_tangent(pi, D);
             } else if (mt == 4) {
                 apply_tangent(pi, D); /* AMCC body is commented out: zero tangent, no erfs */
+            } else if (mt == 5) {
+                or_dpmodel_bui(D, pi, s, adps);
+                apply_tangent(pi, D);
+                or_dp_check(pi, s, dt);
+            } else if (mt == 8) {
+                or_fractional_mcc(D, Cf, pi, s);
+                apply_tangent(pi, D);
             } else if (mt >= 0) {
-                known = 0;
+                known = 0; /* 7 viscous-elastic and 9 Mohr-Coulomb are not restated */
             }
             if (!known) bad = SPHG_E_UNSUPPORTED;
             jaumann(pi);
+            if (pi->matype == 8 || pi->matype == 18 || pi->matype == 28)
+                or_fractional_plastic(pi->dsig, pi->deps, adps, Cf);
             for (k = 0; k < 6; k++) {
                 pi->adps[k] = pi->adps[k] + adps[k] * dt;
                 pi->eps[k] = pi->eps[k] + pi->deps[k] * dt;
@@ -1356,30 +1613,245 @@ static void or_interact1(or_ctx *c) {
     }
 }
 
+
+/* ------------------------------------------------------------------ regularisation, saturation */
+/* strain_regu / stress_regu: Shepard smoothing of eps or sig over same-type neighbours
+ * (CPP_7-2-constitutive.cpp:385-450). In place in the reference (quirk A2, Gauss-Seidel in a
+ * serial run); OR_DENSITY_JACOBI smooths from the values before the pass. */
+static void or_regu(or_ctx *c, int which_sig) {
+    int n = c->n, i;
+    double (*prev)[6] = NULL;
+    if (c->density_mode == OR_DENSITY_JACOBI) {
+        prev = (double(*)[6])malloc(sizeof(double[6]) * (size_t)n);
+        for (i = 0; i < n; i++) memcpy(prev[i], which_sig ? c->p[i].sig : c->p[i].eps, 48);
+    }
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double up[6] = {0, 0, 0, 0, 0, 0}, dn = 0.0;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            const double *v = prev ? prev[j] : (which_sig ? pj->sig : pj->eps);
+            if (pi->type != pj->type) continue;
+            for (k = 0; k < 6; k++) up[k] = up[k] + pj->mass / pj->rho * c->pc[i].wij[nj][3] * v[k];
+            dn = dn + pj->mass / pj->rho * c->pc[i].wij[nj][3];
+        }
+        if (fabs(dn) > 1.0e-6) {
+            double *out = which_sig ? pi->sig : pi->eps;
+            for (k = 0; k < 6; k++) out[k] = up[k] / dn;
+        }
+    }
+    free(prev);
+}
+
+/* moisturesuct (CPP_7-2-constitutive.cpp:324-338); pi = 3.141592653 (Header_Parameters.h:10) */
+static double or_moisturesuct(double satu, double dsatu, const sphg_soil *s) {
+    const double pi = 3.141592653;
+    double temp;
+    if ((satu - s->sitar) < 0.0001) temp = tan(pi * (s->sitas - satu - 0.0001) / (s->sitas - s->sitar) / 2.0);
+    temp = tan(pi * (s->sitas - satu) / (s->sitas - s->sitar) / 2.0);
+    if (dsatu > 0.0) return log(1 + exp(s->c2 * s->sw) * temp) / s->c2;
+    return log(1 + exp(s->c1 * s->sd) * temp) / s->c1;
+}
+
+/* saturation (CPP_7-1-strain.cpp:222-280) */
+static void or_saturation(or_ctx *c) {
+    int n = c->n, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        const sphg_soil *s = &c->ps[i];
+        double cof = s->sitas - s->sitar, sitas = s->sitas, sitar = s->sitar, up = 0.0, dn = 0.0, ps;
+        int nc = nbcount(&c->pc[i]), nj;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pi->type != pj->type && pi->type == 2 && pj->type == 1) {
+                up = up + pj->mass / pj->rho * cof * c->pc[i].wij[nj][3];
+                dn = dn + pj->mass / pj->rho * c->pc[i].wij[nj][3];
+            }
+        }
+        if (pi->type != 2) continue;
+        ps = pi->satu;
+        pi->satu = dn != 0.0 ? up / dn + sitar : up + sitar;
+        pi->dsatu = pi->satu - ps;
+        if (pi->dsatu < 0) {
+            pi->satu = pi->satu - pi->dsatu;
+            pi->dsatu = 0.0;
+        }
+        if (pi->satu > sitas) pi->satu = sitas;
+        else if (pi->satu < sitar) pi->satu = sitar;
+        pi->suct = or_moisturesuct(pi->satu, pi->dsatu, s);
+    }
+}
+
+/* ------------------------------------------------------------------ artificial stress */
+/* artificial_stress_2d / _3d (CPP_9-5-artificialstress.cpp:28-317); pi*0.78539816325 (quirk A15) */
+static void or_inverse_mat(double A[3][3], double B[3][3]) {
+    double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[1][0] * (A[0][1] * A[2][2] - A[0][2] * A[2][1]) +
+                 A[2][0] * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+    if (fabs(det) > 1.0e-5) {
+        B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+        B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+        B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+        B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+        B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+        B[1][2] = (A[1][0] * A[0][2] - A[0][0] * A[1][2]) / det;
+        B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+        B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+        B[2][2] = (A[0][0] * A[1][1] - A[1][0] * A[0][1]) / det;
+    }
+}
+static void mat3mul(double a[3][3], double b[3][3], double c[3][3]) {
+    int r, q;
+    for (r = 0; r < 3; r++)
+        for (q = 0; q < 3; q++) c[r][q] = a[r][0] * b[0][q] + a[r][1] * b[1][q] + a[r][2] * b[2][q];
+}
+
+static void or_art_stress(or_ctx *c) {
+    const double pi = 3.141592653;
+    int n = c->n, ndim = c->pro.ndim, i;
+    double xs = c->pro.art_stre_coe;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *p = &c->p[i];
+        double R[3][3] = {{0}};
+        if (p->type != 2) continue;
+        if (ndim == 2) {
+            double sita, sxx, syy, rxx, ryy;
+            if (fabs(p->sig[0] - p->sig[1]) < 1.0e-6) {
+                if (p->sig[3] > 0.0) sita = -pi * 0.78539816325;
+                else if (p->sig[3] < 0.0) sita = pi * 0.78539816325;
+                else sita = 0.0;
+            } else
+                sita = 0.5 * atan(2.0 * p->sig[3] / (p->sig[0] - p->sig[1]));
+            sxx = cos(sita) * cos(sita) * p->sig[0] + 2.0 * sin(sita) * cos(sita) * p->sig[3] + sin(sita) * sin(sita) * p->sig[1];
+            syy = sin(sita) * sin(sita) * p->sig[0] - 2.0 * sin(sita) * cos(sita) * p->sig[3] + cos(sita) * cos(sita) * p->sig[1];
+            rxx = sxx > 0.0 ? -xs * sxx / p->rho / p->rho : 0.0;
+            ryy = syy > 0.0 ? -xs * syy / p->rho / p->rho : 0.0;
+            R[0][0] = rxx * cos(sita) * cos(sita) + ryy * sin(sita) * sin(sita);
+            R[1][1] = rxx * sin(sita) * sin(sita) + ryy * cos(sita) * cos(sita);
+            R[0][1] = (rxx - ryy) * sin(sita) * cos(sita);
+            R[1][0] = R[0][1];
+        } else {
+            double rr[3], bi[3][3] = {{0}}, bti[3][3] = {{0}}, st[3][3] = {{0}}, tmp[3][3];
+            int k;
+            for (k = 0; k < 3; k++) rr[k] = p->strep[k] > 0.0 ? -xs * p->strep[k] / p->rho / p->rho : 0.0;
+            or_inverse_mat(p->beta, bi);
+            or_inverse_mat(p->beta_ts, bti);
+            st[0][0] = rr[0]; st[1][1] = rr[1]; st[2][2] = rr[2];
+            mat3mul(bi, st, tmp);
+            mat3mul(tmp, bti, R);
+        }
+        memcpy(p->Rij, R, sizeof R);
+    }
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double tu[3] = {0, 0, 0};
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (pi->type != 2) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            const double *w = c->pc[i].wij[nj];
+            double dst = 0.0, hl, q, f;
+            if (pi->type != pj->type) continue;
+            for (k = 0; k < 3; k++) dst = dst + (pi->xp[k] - pj->xp[k]) * (pi->xp[k] - pj->xp[k]);
+            dst = sqrt(dst);
+            hl = ndim == 2 ? (pi->hl + pj->hl) * 0.5 : (pi->hl + pj->hl) / 2;
+            q = dst / hl;
+            if (q >= 0.0 && q < 1.0) f = pow((0.666666666666667 - q * q + 0.5 * q * q * q) * 3.82300884955752, 2.55);
+            else if (q >= 1.0 && q < 2.0) f = pow((2.0 - q) * (2.0 - q) * (2.0 - q) * 0.629737609329447, 2.55);
+            else f = 0.0;
+            if (ndim == 2) {
+                tu[0] = tu[0] + pj->mass * f * (pi->Rij[0][0] + pj->Rij[0][0]) * w[0] + pj->mass * f * (pi->Rij[0][1] + pj->Rij[0][1]) * w[1];
+                tu[1] = tu[1] + pj->mass * f * (pi->Rij[1][0] + pj->Rij[1][0]) * w[0] + pj->mass * f * (pi->Rij[1][1] + pj->Rij[1][1]) * w[1];
+            } else {
+                for (k = 0; k < 3; k++)
+                    tu[k] = tu[k] + pj->mass * f * (pi->Rij[k][0] + pj->Rij[k][0]) * w[0] +
+                            pj->mass * f * (pi->Rij[k][1] + pj->Rij[k][1]) * w[1] +
+                            pj->mass * f * (pi->Rij[k][2] + pj->Rij[k][2]) * w[2];
+            }
+        }
+        for (k = 0; k < ndim; k++) pi->ax[k] = pi->ax[k] + tu[k];
+    }
+}
+
+/* soil_damping_geostress (CPP_9-4-derivation_stress.cpp:850-897) */
+static void or_damping_geostress(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+    double dt = c->pro.dt;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double damp[3] = {0, 0, 0}, cd = pi->cd, vx1, vx2;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (!(pi->type == 2 || pi->type == 4)) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pi->type != pj->type) continue;
+            for (k = 0; k < ndim; k++)
+                damp[k] = damp[k] + pj->mass / pj->rho * cd * (pi->vxp[k] - pj->vxp[k]) * c->pc[i].wij[nj][3];
+        }
+        for (k = 0; k < ndim; k++) {
+            vx1 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k]) * dt;
+            vx2 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k] - damp[k]) * dt;
+            if (vx1 * vx2 <= 0.0) pi->ax[k] = -2.0 * pi->vxp[k] / dt - pi->axp[k];
+            else pi->ax[k] -= damp[k];
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ velocity */
-/* veocity_update, if_move == 0 (CPP_11-xsph.cpp:11-57) */
+/* veocity_update (CPP_11-xsph.cpp:11-142) */
 static void or_velocity(or_ctx *c) {
-    int n = c->n, ndim = c->pro.ndim, i, k;
+    int n = c->n, ndim = c->pro.ndim, i, k, mv = c->pro.if_move, l = c->l, inip = c->pro.inip;
     double dt = c->pro.dt, ath;
     for (i = 0; i < n; i++) {
         OPar *pi = &c->p[i];
-        if (pi->type == 0 && pi->matype < 2) {
-            for (k = 0; k < ndim; k++) {
-                pi->interf[k] = 0.0;
-                ath = (pi->interf[k] + pi->interf[k]) * 0.5;
-                pi->interfss[k] = pi->interfss[k] + ath * dt;
+        if (mv == 0) {
+            if (pi->type == 0 && pi->matype < 2) {
+                for (k = 0; k < ndim; k++) {
+                    pi->interf[k] = 0.0;
+                    ath = (pi->interf[k] + pi->interf[k]) * 0.5;
+                    pi->interfss[k] = pi->interfss[k] + ath * dt;
+                }
+            } else if ((pi->type == 0 && pi->matype == 3) || (pi->type == 0 && pi->matype == 5 && c->pro.flv == 1)) {
+                for (k = 0; k < ndim; k++) {
+                    ath = (pi->interf[k] + pi->interf[k]) * 0.5;
+                    pi->interfss[k] = pi->interfss[k] + ath * dt;
+                }
+            } else if (pi->type != 0) {
+                for (k = 0; k < ndim; k++) {
+                    ath = (pi->axp[k] + pi->ax[k]) * 0.5;
+                    pi->vx[k] = pi->vxp[k] + ath * dt;
+                }
             }
-        } else if (pi->type == 0 && pi->matype == 3) {
-            for (k = 0; k < ndim; k++) {
-                ath = (pi->interf[k] + pi->interf[k]) * 0.5;
-                pi->interfss[k] = pi->interfss[k] + ath * dt;
+        } else if (mv == 1) {
+            if (pi->type == 0 && pi->matype == 2 && l > inip) {
+                for (k = 0; k < ndim; k++) pi->interfss[k] = c->pro.move_vx[k];
+            } else if (pi->type == 0) {
+                for (k = 0; k < ndim; k++) pi->interfss[k] = 0.0;
+            } else {
+                for (k = 0; k < ndim; k++) {
+                    ath = (pi->axp[k] + pi->ax[k]) * 0.5;
+                    pi->vx[k] = pi->vxp[k] + ath * dt;
+                }
             }
-        } else if (pi->type == 0 && pi->matype == 5 && c->pro.flv == 1) {
-            for (k = 0; k < ndim; k++) {
-                ath = (pi->interf[k] + pi->interf[k]) * 0.5;
-                pi->interfss[k] = pi->interfss[k] + ath * dt;
+        } else if (mv == 2 || mv == 3) {
+            int mt = mv == 2 ? 2 : 4;
+            if (pi->type == mt && pi->etype == 5 && l > inip) {
+                for (k = 0; k < ndim; k++) {
+                    pi->axp[k] = 0.0;
+                    pi->vxp[k] = c->pro.move_vx[k];
+                    pi->vx[k] = c->pro.move_vx[k];
+                    pi->ax[k] = 0.0;
+                }
             }
-        } else if (pi->type != 0) {
             for (k = 0; k < ndim; k++) {
                 ath = (pi->axp[k] + pi->ax[k]) * 0.5;
                 pi->vx[k] = pi->vxp[k] + ath * dt;
@@ -1647,12 +2119,8 @@ static void or_update(or_ctx *c) {
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
     if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
-    if (P->flagum >= 1) return 1;
     if (P->fli > 1 || P->ss_type != 0) return 1;
-    if (P->art_stre_coe > 0.0 || P->step_regu <= P->loop) return 1;
     if (P->flbndy != 1 && P->flbndy != 2) return 1;
-    if (P->if_move != 0) return 1;
-    if (P->inip >= c->l) return 1; /* soil_damping_geostress not restated */
     return 0;
 }
 
@@ -1673,11 +2141,14 @@ int or_phase(or_ctx *c, int mask) {
     }
     if (mask & 4) {
         or_strain(c);
+        if (c->l % c->pro.step_regu == 0) or_regu(c, 0);
+        if (c->pro.flagum == 1) or_saturation(c);
         if (c->pro.fli == 1) or_permeability(c);
         or_eos2(c);
     }
     if (mask & 8) {
         if ((e = or_strain_to_stress(c))) return e;
+        if (c->l % c->pro.step_regu == 0) or_regu(c, 1);
         or_free_surface(c);
         or_eig(c);
     }
@@ -1688,8 +2159,10 @@ int or_phase(or_ctx *c, int mask) {
         or_solid_accel(c, 2);
         or_solid_accel(c, 4);
         or_art_visc(c);
+        if (c->pro.art_stre_coe > 0.0) or_art_stress(c);
         or_bndy_stress(c);
         if (c->pro.fli == 1) or_interact1(c);
+        if (c->l <= c->pro.inip) or_damping_geostress(c);
         or_velocity(c);
     }
     if (mask & 32) {

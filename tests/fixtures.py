@@ -22,10 +22,20 @@ def load_golden(name: str):
 
 
 def bitequal(a: np.ndarray, b: np.ndarray) -> bool:
+    """Bit-for-bit equality; NaNs compare equal to NaNs (their payloads are platform specific)."""
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)
     if a.shape != b.shape:
         return False
     if a.dtype == np.float64:
-        return bool(np.array_equal(a.view(np.int64), b.view(np.int64)))
+        na, nb = np.isnan(a), np.isnan(b)
+        if not np.array_equal(na, nb):
+            return False
+        return bool(np.array_equal(np.where(na, 0.0, a).view(np.int64), np.where(nb, 0.0, b).view(np.int64)))
     return bool(np.array_equal(a, b))
+
+
+def flush_denormal(a: np.ndarray) -> np.ndarray:
+    """The reference's FOCC plastic strain rate carries run-to-run-varying denormal residues
+    (|x| < 1e-300) from uninitialised memory; they are compared as zero."""
+    return np.where(np.abs(a) < 1e-300, 0.0, a)

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 from helpers import run_ref, have_ref  # noqa: E402
-from sphgpy import column_collapse_2d, soil_water_column_3d  # noqa: E402
+from sphgpy import column_collapse_2d, soil_water_column_3d, model_zoo_2d  # noqa: E402
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype"})
@@ -27,6 +27,7 @@ def decks():
     return {
         "column2d_small": (column_collapse_2d(nx=10, ny=20, wall_h=30, floor_len=30, steps=8), 8, (1, 4, 8)),
         "soilwater3d_small": (soil_water_column_3d(nx=5, ny=4, nz_soil=5, nz_water=3, steps=5), 5, (1, 5)),
+        "model_zoo_2d": (model_zoo_2d(steps=9), 9, (1, 3, 6, 9)),
     }
 
 
