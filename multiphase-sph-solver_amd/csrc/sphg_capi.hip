@@ -52,6 +52,9 @@ struct sphg_ctx {
     bool has_psc = false;
     int scan_blocks = 0;
     bool stepped = false;
+    bool regu_on = false;  // regularisation passes are in the launch sequence
+    bool pre_on = false;   // artificial-stress / geostress-damping pair pass is in the launch sequence
+    bool full_models = false;  // some soil particle uses Cam-Clay, unsaturated Cam-Clay or FOCC
 };
 
 #define HIPCHK(x)                                                               \
@@ -144,6 +147,12 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
         plan.push_back({(void **)&S.wij[k], cap * kMaxNb * 8});
     }
     for (int k = 0; k < 6; k++) D(S.qsig[k]);
+    // artificial-stress tensor (kept between steps) and the Jacobi regularisation copy
+    for (int k = 0; k < 9; k++) {
+        if (pro->art_stre_coe > 0.0) D(S.rij[k]);
+        else S.rij[k] = nullptr;
+    }
+    for (int k = 0; k < 6; k++) D(S.tmp6[k]);
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
@@ -237,6 +246,12 @@ static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, b
     if (nm == "adps") return v6(S.adps);
     if (nm == "beta") {
         for (int k = 0; k < 9; k++) out[k] = S.beta[k];
+        ncomp = 9;
+        return 0;
+    }
+    if (nm == "Rij") {
+        if (!S.rij[0]) return -1;
+        for (int k = 0; k < 9; k++) out[k] = S.rij[k];
         ncomp = 9;
         return 0;
     }
@@ -382,12 +397,44 @@ static int fluid_slot_of(int type, int matype, int etype) {
 static int check_supported(const sphg_problem &p) {
     if (p.flknl > 2) return SPHG_E_UNSUPPORTED;  // kernel_gradient_correction
     if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
-    if (p.flagum >= 1) return SPHG_E_UNSUPPORTED;
     if (p.fli > 1 || p.ss_type != 0) return SPHG_E_UNSUPPORTED;
-    if (p.art_stre_coe > 0.0 || p.step_regu <= p.loop) return SPHG_E_UNSUPPORTED;
     if (p.flbndy != 1 && p.flbndy != 2) return SPHG_E_UNSUPPORTED;
-    if (p.if_move != 0) return SPHG_E_UNSUPPORTED;
+    if (p.if_move < 0 || p.if_move > 3) return SPHG_E_UNSUPPORTED;
+    if (p.step_regu <= 0) return SPHG_E_ARG;  // l % step_regu
     return SPHG_OK;
+}
+
+// True when some step l in [first, first + count) runs the regularisation passes (l % step_regu == 0).
+static bool regu_due(int step_regu, long long first, int count) {
+    if (count <= 0) return false;
+    if (first <= 0) return true;
+    long long next = ((first + step_regu - 1) / step_regu) * step_regu;
+    return next <= first + count - 1;
+}
+
+static void drop_graphs(sphg_ctx *c) {
+    if (c->graph[0] || c->graph[1]) hipStreamSynchronize(c->stream);  // no launched graph in flight
+    for (int b = 0; b < 2; b++) {
+        if (c->graph[b]) hipGraphExecDestroy(c->graph[b]);
+        c->graph[b] = nullptr;
+        c->graph_ok[b] = false;
+    }
+}
+
+// Adds the regularisation passes to the launch sequence before a batch of steps that needs them
+// (they stay in; the kernels test l % step_regu themselves).
+static void ensure_regu(sphg_ctx *c, int count) {
+    if (c->regu_on || !regu_due(c->P.step_regu, c->step, count)) return;
+    c->regu_on = true;
+    drop_graphs(c);
+}
+
+// The artificial-stress / damping pair pass runs while artificial stress is on or l <= inip.
+static void plan_pre(sphg_ctx *c) {
+    bool want = c->P.art_stre_coe > 0.0 || c->step <= c->P.inip;
+    if (want == c->pre_on) return;
+    c->pre_on = want;
+    drop_graphs(c);
 }
 
 extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
@@ -425,6 +472,10 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.interc = pro.interc;
     P.permtype = pro.permtype;
     P.pers = pro.pers; P.perl = pro.perl; P.perm = pro.perm;
+    P.flagum = pro.flagum;
+    P.step_regu = pro.step_regu;
+    P.art_stre_coe = pro.art_stre_coe;
+    for (int k = 0; k < 3; k++) P.move_vx[k] = pro.move_vx[k];
     P.tmin = pro.threshold_min;
     P.tmax = pro.threshold_max;
     P.struct_e = c->mat.structure.e;
@@ -437,6 +488,11 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
             P.first_moving_bndy = i;
             break;
         }
+    c->full_models = false;
+    for (int i = 0; i < n; i++) {
+        int m = c->h_matype[i], mt = ((m >= 0 && m < 30) || (m >= 50 && m < 60)) ? m % 10 : -1;
+        if (c->h_type[i] == 2 && (mt == 1 || mt == 3 || mt == 8)) c->full_models = true;
+    }
     // material slots with host-evaluated transcendental constants
     std::vector<SoilSlot> soil(kNSlots);
     for (int s = 0; s < kNSlots; s++) {
@@ -447,10 +503,17 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
         o.e = d.e; o.poi = d.poi; o.eps0 = d.eps0; o.dens = d.dens; o.kapa = d.kapa; o.zmf = d.zmf; o.ann = d.ann;
         o.ramda = d.ramda; o.fs = d.fs; o.c = d.c; o.sme0 = d.sme0; o.porosity = d.porosity; o.vpu = d.vpu;
         o.cop = d.cop; o.sitas = d.sitas; o.sitar = d.sitar; o.damp_coe = d.damp_coe; o.beta = d.beta;
+        o.bnn = d.bnn; o.epsr = d.epsr; o.epss = d.epss; o.sd = d.sd; o.sw = d.sw; o.c1 = d.c1; o.c2 = d.c2;
+        o.g0 = d.g0; o.etao = d.etao; o.zet = d.zet; o.damp1 = d.damp1; o.damp2 = d.damp2;
+        o.exp_c2sw = exp(d.c2 * d.sw);  // CPP_7-2-constitutive.cpp:333-336
+        o.exp_c1sd = exp(d.c1 * d.sd);
         double fai = atan(tan(d.fai * 0.0174532925166667) / d.fs);  // CPP_8-1-consfactor.cpp:294
         o.dp_sq = sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
         o.dp_yalf = sin(fai) / o.dp_sq;
         o.dp_cosf = cos(fai);
+        // dilatancy angle of dpmodel_bui (CPP_8-1-consfactor.cpp:462-463, :511)
+        double fai_dil = atan(tan(d.beta * 0.0174532925166667) / d.fs);
+        o.dp_yalf_dil = sin(fai_dil) / sqrt(9.0 + 3.0 * sin(fai_dil) * sin(fai_dil));
     }
     std::vector<FluidSlot> fl(8);
     for (int s = 0; s < 8; s++) {
@@ -478,11 +541,9 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     int zero = 0;
     HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->S.step, &c->step, 4, hipMemcpyHostToDevice));
-    for (int b = 0; b < 2; b++) {
-        if (c->graph[b]) hipGraphExecDestroy(c->graph[b]);
-        c->graph[b] = nullptr;
-        c->graph_ok[b] = false;
-    }
+    drop_graphs(c);
+    c->regu_on = false;
+    c->pre_on = false;
     c->uploaded = true;
     return SPHG_OK;
 }
@@ -536,9 +597,29 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         LAUNCH("density", k_density<NDIM>, grid_for(n), c->S, c->P, cur);
         LAUNCH("norm_eos", k_norm<NDIM>, grid_for(n), c->S, c->P, cur);
     }
-    if (phase_mask & 4) LAUNCH("strain", k_strain<NDIM>, grid_for(n), c->S, c->P, cur);
-    if (phase_mask & 8) LAUNCH("stress", k_stress, grid_for(n), c->S, c->P);
-    if (phase_mask & 16) LAUNCH("accel", k_accel<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (phase_mask & 4) {
+        if (c->P.flagum == 1) LAUNCH("saturation", k_saturation<NDIM>, grid_for(n), c->S, c->P, cur);
+        LAUNCH("strain", k_strain<NDIM>, grid_for(n), c->S, c->P, cur);
+        if (c->regu_on) {
+            LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 0);
+            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, 0);
+        }
+    }
+    if (phase_mask & 8) {
+        auto ks = c->full_models ? k_stress<true> : k_stress<false>;
+        if (c->regu_on) {
+            LAUNCH("stress", ks, grid_for(n), c->S, c->P, 1);
+            LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 1);
+            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, 1);
+            LAUNCH("stress_post", ks, grid_for(n), c->S, c->P, 2);
+        } else {
+            LAUNCH("stress", ks, grid_for(n), c->S, c->P, 0);
+        }
+    }
+    if (phase_mask & 16) {
+        if (c->pre_on) LAUNCH("accel_pre", k_accel_pre<NDIM>, grid_for(n), c->S, c->P, cur);
+        LAUNCH("accel", k_accel<NDIM>, grid_for(n), c->S, c->P, cur);
+    }
     if (phase_mask & 32) LAUNCH("xsph", k_xsph<NDIM>, grid_for(n), c->S, c->P, cur);
     if (phase_mask & 64) {
         LAUNCH("update", k_update<NDIM>, grid_for(n), c->S, c->P, cur);
@@ -585,6 +666,8 @@ static int finish_timing(sphg_ctx *c) {
 // Individual phases (each advances nothing but its own outputs; sphg_update advances l).
 static int phase(sphg_ctx *c, int mask) {
     if (!c || !c->uploaded) return SPHG_E_ARG;
+    ensure_regu(c, 1);
+    plan_pre(c);
     enqueue(c, mask);
     if (mask & 64) {
         c->cur ^= 1;
@@ -648,8 +731,10 @@ static int capture(sphg_ctx *c, int parity) {
 
 extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
     if (!c || !c->uploaded || nsteps < 0) return SPHG_E_ARG;
+    ensure_regu(c, nsteps);
     if (c->timing) {
         for (int s = 0; s < nsteps; s++) {
+            plan_pre(c);
             enqueue(c, 127);
             c->cur ^= 1;
             c->step++;
@@ -660,6 +745,7 @@ extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
     }
     for (int s = 0; s < nsteps; s++) {
         int par = c->cur;
+        plan_pre(c);
         if (!c->graph_ok[par]) {
             int rc = capture(c, par);
             if (rc) return rc;

@@ -22,6 +22,9 @@ constexpr int kNSlots = 32;  // 30 soil slots + default row (index 30) + padding
 struct SoilSlot {
     double e, poi, eps0, dens, kapa, zmf, ann, ramda, fs, c, sme0, porosity, vpu, cop, sitas, sitar;
     double damp_coe, beta;
+    double bnn, epsr, epss, sd, sw, c1, c2, g0, etao, zet, damp1, damp2;  // UMCC / FOCC parameters
+    double exp_c2sw, exp_c1sd;  // exp(c2 * sw), exp(c1 * sd) of moisturesuct (CPP_7-2-constitutive.cpp:333-336)
+    double dp_yalf_dil;  // sin(psi') / sqrt(9 + 3 sin^2 psi'), psi' = atan(tan(beta)/fs) (Bui DP dilatancy)
     double dp_yalf;   // sin(phi') / sqrt(9 + 3 sin^2 phi'), phi' = atan(tan(phi)/fs)  (CPP_8-1-consfactor.cpp:346)
     double dp_cosf;   // cos(phi')
     double dp_sq;     // sqrt(9 + 3 sin^2 phi')
@@ -51,6 +54,9 @@ struct Params {
     double struct_e, struct_poi;
     int nb_off[27];       // stencil offsets in the reference's nncell order
     int nstencil;
+    int flagum, step_regu;
+    double art_stre_coe;
+    double move_vx[3];
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -74,6 +80,8 @@ struct State {
     int *nb_cnt, *nbr;
     double *wij[4];       // kernel gradient (3) and value per list entry, column-major like nbr
     double *qsig[6];      // sig / rho / rho per particle (the j term of the solid momentum sums)
+    double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
+    double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
     int *err, *step;      // sticky error word; current step number l
     const SoilSlot *soil;
     const FluidSlot *fluid;

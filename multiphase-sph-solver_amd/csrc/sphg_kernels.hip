@@ -417,6 +417,48 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 // strain rate + eos loop 2 + free-surface colour + permeability
 // ------------------------------------------------------------------------------------------
+// saturation (CPP_7-1-strain.cpp:222-280) with moisturesuct (CPP_7-2-constitutive.cpp:324-338),
+// flagum == 1. It reads only masses, densities and kernel values and writes satu / dsatu / suct,
+// which the strain pass reads only for the permeability of the same particle, so it runs
+// just before that pass.
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) {
+    int i = pidx();
+    if (i >= P.n || S.type[i] != 2) return;
+    double w[4];
+    const int nc = nb_count(S, i);
+    const SoilSlot &sl = S.soil[S.soil_slot[i]];
+    const double sat_cof = sl.sitas - sl.sitar;
+    double sat_up = 0.0, sat_dn = 0.0;
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        if (S.type[j] != 1) continue;
+        load_w<NDIM>(S, P, nj, i, w);
+        double mj = S.mass[j], rj = S.rho[j];
+        sat_up = sat_up + mj / rj * sat_cof * w[3];
+        sat_dn = sat_dn + mj / rj * w[3];
+    }
+    // saturation update and moisturesuct (CPP_7-1-strain.cpp:252-278, CPP_7-2-constitutive.cpp:324-338)
+    {
+        const double sitas = sl.sitas, sitar = sl.sitar;
+        double ps = S.satu[i];
+        double sa = sat_dn != 0.0 ? sat_up / sat_dn + sitar : sat_up + sitar;
+        double dsa = sa - ps;
+        if (dsa < 0) {
+            sa = sa - dsa;
+            dsa = 0.0;
+        }
+        if (sa > sitas) sa = sitas;
+        else if (sa < sitar) sa = sitar;
+        const double pi = 3.141592653;
+        double temp = tan(pi * (sitas - sa) / (sitas - sitar) / 2.0);
+        double suct = dsa > 0.0 ? log(1 + sl.exp_c2sw * temp) / sl.c2 : log(1 + sl.exp_c1sd * temp) / sl.c1;
+        S.satu[i] = sa;
+        S.dsatu[i] = dsa;
+        S.suct[i] = suct;
+    }
+}
+
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     int i = pidx();
@@ -657,17 +699,261 @@ __device__ __forceinline__ int model_of(int matype) {
     return -1;
 }
 
-__global__ __launch_bounds__(256) void k_stress(State S, Params P) {
-    int i = pidx();
-    if (i >= P.n) return;
-    const int ti = S.type[i];
-    const int l = step_of(S);
-    const double dt = P.dt;
-    double sig[6], deps[6];
-    for (int k = 0; k < 6; k++) {
-        sig[k] = S.sig[k][i];
-        deps[k] = S.deps[k][i];
+// detA: recursive cofactor expansion along the first row (CPP_8-5-consfactor-focc.cpp:275-302),
+// instantiated per size so the recursion is resolved at compile time; same term order.
+template <int N>
+struct Det {
+    __device__ static double of(const double (&a)[N][N]) {
+        double sum = 0;
+        for (int i = 0; i < N; i++) {
+            double bak[N - 1][N - 1];
+            for (int j = 1; j < N; j++) {
+                int cc = 0;
+                for (int k = 0; k < N; k++) {
+                    if (k == i) continue;
+                    bak[j - 1][cc++] = a[j][k];
+                }
+            }
+            sum += (i % 2 == 0 ? 1 : -1) * a[0][i] * Det<N - 1>::of(bak);
+        }
+        return sum;
     }
+};
+template <>
+struct Det<1> {
+    __device__ static double of(const double (&a)[1][1]) { return a[0][0]; }
+};
+
+// getAStart: adjugate of a 6x6 matrix by cofactors (CPP_8-5-consfactor-focc.cpp:305-336)
+__device__ void adjugate6(const double (&a)[6][6], double (&ans)[6][6]) {
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double t[5][5];
+            for (int k = 0; k < 5; k++)
+                for (int q = 0; q < 5; q++) t[k][q] = a[k >= i ? k + 1 : k][q >= j ? q + 1 : q];
+            double v = Det<5>::of(t);
+            ans[j][i] = ((i + j) % 2 == 1) ? -v : v;
+        }
+}
+
+// fractional_mcc (CPP_8-5-consfactor-focc.cpp:13-240): tangent D = adj(T)/det(T) and the elastic
+// compliance C used afterwards by fractional_plastic; quirks kept (sm = sum / 0.33333333,
+// e approximated by 2.71828, p0 = pa = 98000).
+__device__ void focc_tangent(const double sigin[6], const double deps[6], double e, const SoilSlot &s,
+                             double (&D)[6][6], double (&C)[6][6]) {
+    const double poi = s.poi, h1 = s.damp1, h2 = s.damp2, ramda = s.ramda, mc = s.zmf, alpha = s.ann,
+                 beta = s.bnn, zeta = s.zet, etao = s.etao, g0 = s.g0, p0 = 98000, pa = 98000;
+    double c = s.c, sg[6], sij[6];
+    for (int a = 0; a < 6; a++) sg[a] = -sigin[a];
+    double sm = (sg[0] + sg[1] + sg[2]) / 0.33333333;
+    for (int a = 0; a < 6; a++) sij[a] = a < 3 ? sg[a] - sm : sg[a];
+    double q = 0.5 * (sij[0] * sij[0] + sij[1] * sij[1] + sij[2] * sij[2]) + sij[3] * sij[3] + sij[4] * sij[4] +
+               sij[5] * sij[5];
+    q = sqrt(3.0 * q);
+    double ita = q / sm;
+    if (ita == 0.0) {
+        ita = 0.000001;
+        q = ita * sm;
+    }
+    double g = g0 * (2.97 - e) * (2.97 - e) / (1.0 + e) * sqrt(sm * pa);
+    double e0 = 2.0 * g * (1.0 + poi);
+    double ymf = mc;
+    double dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi), dmu = e0 * 0.5 / (1.0 + poi), dmu2 = dmu * 2.0;
+    double tt1 = 1.0 / (ymf * ymf * sm * sm + q * q);
+    double tt2 = 2.0 * ymf * ymf * sm * tt1 - 1.0 / sm;
+    double f11 = tt2 * 0.33333333 + sij[0] * tt1 * 3.0, f22 = tt2 * 0.33333333 + sij[1] * tt1 * 3.0,
+           f33 = tt2 * 0.33333333 + sij[2] * tt1 * 3.0;
+    double f12 = 3.0 * sij[3] * tt1, f23 = 3.0 * sij[4] * tt1, f31 = 3.0 * sij[5] * tt1;
+    double fkk = f11 + f22 + f33;
+    double ramda_cc = (dmu2 * f11 + dlam * fkk) * deps[0] + (dmu2 * f22 + dlam * fkk) * deps[1] +
+                      (dmu2 * f33 + dlam * fkk) * deps[2] + dmu * (f12 * deps[3] + f23 * deps[4] + f31 * deps[5]);
+    const double smn[3][3] = {{sij[0], sij[3], sij[5]}, {sij[3], sij[1], sij[4]}, {sij[5], sij[4], sij[2]}};
+    double s_val = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (int k = 0; k < 3; k++) s_val = s_val + smn[a][b] * smn[b][k] * smn[k][a];
+    double skl = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) skl = skl + smn[a][b] * smn[b][a];
+    if (skl == 0.0) skl = 0.000001;
+    double sita = 9.0 * s_val * 0.5 / pow(1.5 * skl, 1.5);
+    if (sita < -1.0) sita = -1.0;
+    else if (sita > 1.0) sita = 1.0;
+    sita = asin(sita) * 0.33333333;
+    c = pow(c, 4);
+    double m = 2 * c / (1 + c + (1 - c) * sin(3.0 * sita));
+    m = pow(m, 0.25);
+    m = mc * m;
+    double pc = pa * pow((etao - e) / ramda, 1 / zeta);
+    double qc = q + m * (sm - pc);
+    double df = (sm - p0 / 2) * mc * mc / q;
+    double dg = ((sm - pc) + (2 - alpha) * (pc - sm * 0.5 - q * q / mc / mc / sm / 2)) / ((q - qc) + (2 - alpha) * qc);
+    dg = pow(m, alpha + 1) * dg;
+    double mij[6], nij[6];
+    for (int a = 0; a < 6; a++) {
+        double base = 3.0 * sij[a] * 0.5 / q;
+        mij[a] = a < 3 ? 1.0 / sqrt(1.0 + df * df) * (base + df * 0.33333333) : 1.0 / sqrt(1.0 + df * df) * base;
+        nij[a] = a < 3 ? 1.0 / sqrt(1.0 + dg * dg) * (base + dg * 0.33333333) : 1.0 / sqrt(1.0 + dg * dg) * base;
+    }
+    double order = beta * (e - etao + ramda * pow(sm / pa, zeta));
+    double mf = mc * pow(2.71828, -order);
+    double hp = (h1 - h2 * e) * (1 - e) * g * (mf - ita) / ita * pow(2.71828, order);
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) C[a][b] = 0.0;
+    C[0][0] = C[1][1] = C[2][2] = 1.0 / e0;
+    C[0][1] = C[0][2] = C[1][0] = C[1][2] = C[2][0] = C[2][1] = -poi / e0;
+    C[3][3] = C[4][4] = C[5][5] = 1.0 / g;
+    double T[6][6], adj[6][6];
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) T[a][b] = ramda_cc < 0.0 ? C[a][b] + nij[a] * mij[b] / hp : C[a][b];
+    double det_a = Det<6>::of(T);
+    adjugate6(T, adj);
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) {
+            if (det_a != 0.0) {
+                D[a][b] = adj[a][b] / det_a;
+            } else {
+                det_a = Det<6>::of(C);
+                adjugate6(C, adj);
+                D[a][b] = adj[a][b] / det_a;
+            }
+        }
+}
+
+// one DP-family stress update (dpmodel1 / dpmodel_bui + DPModel_check), CPP_8-1-consfactor.cpp:284-615, :779-819
+__device__ __forceinline__ void dp_update(const double sig[6], const double deps[6], const SoilSlot &s, double ci,
+                                          double yalf_dil, bool bui, double dt, double dsig[6], double adps[6]) {
+    double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * 0.3333333333;
+    if (sm < 1.0) sm = 1.0;
+    double e0 = soil_e0(s, sm);
+    double De[6][6], dlam, dmu;
+    iso_tangent(De, e0, s.poi, dlam, dmu);
+    double dmu2 = dmu * 2.0;
+    double dxv = -sig[0] - sm, dyv = -sig[1] - sm, dzv = -sig[2] - sm;
+    double dxy = -sig[3], dyz = -sig[4], dzx = -sig[5];
+    double sjj = 0.5 * (dxv * dxv + dyv * dyv + dzv * dzv) + dxy * dxy + dyz * dyz + dzx * dzx;
+    double sj = sqrt(sjj);
+    double yalf = s.dp_yalf;
+    double ykap = 3.0 * ci * s.dp_cosf / s.dp_sq;
+    bool plastic = false;
+    double d = 0.0, wv[6], zv[6], ramda = 0.0, fv[6];
+    if ((sj - yalf * sm * 3.0 - ykap) >= 0.0) {
+        double f11 = -yalf + dxv / sj * 0.5, f22 = -yalf + dyv / sj * 0.5, f33 = -yalf + dzv / sj * 0.5;
+        double f12 = dxy / sj * 0.5, f23 = dyz / sj * 0.5, f31 = dzx / sj * 0.5;
+        double fkk = f11 + f22 + f33, yd = bui ? yalf_dil : 0.0;
+        double g11 = -yd + dxv / sj * 0.5, g22 = -yd + dyv / sj * 0.5, g33 = -yd + dzv / sj * 0.5;
+        double g12 = dxy / sj * 0.5, g23 = dyz / sj * 0.5, g31 = dzx / sj * 0.5;
+        double hdd = g11 + g22 + g33;
+        double fij = 2.0 * (f11 * g11 + f22 * g22 + f33 * g33) + f12 * g12 + f23 * g23 + f31 * g31;
+        double cc7 = dlam * (fkk * hdd) + dmu * fij + hdd;
+        d = 1.0 / cc7;
+        double df = dlam * fkk, dg = dlam * hdd;
+        ramda = (dmu2 * f11 - 3.0 * dlam * yalf) * deps[0] + (dmu2 * f22 - 3.0 * dlam * yalf) * deps[1] +
+                (dmu2 * f33 - 3.0 * dlam * yalf) * deps[2] + dmu * (f12 * deps[3] + f23 * deps[4] + f31 * deps[5]);
+        if (ramda * d < 0.0) {
+            plastic = true;
+            wv[0] = dg + dmu2 * g11; wv[1] = dg + dmu2 * g22; wv[2] = dg + dmu2 * g33;
+            wv[3] = dmu * g12; wv[4] = dmu * g23; wv[5] = dmu * g31;
+            zv[0] = df + dmu2 * f11; zv[1] = df + dmu2 * f22; zv[2] = df + dmu2 * f33;
+            zv[3] = dmu * f12; zv[4] = dmu * f23; zv[5] = dmu * f31;
+            fv[0] = f11; fv[1] = f22; fv[2] = f33; fv[3] = f12; fv[4] = f23; fv[5] = f31;
+        }
+    }
+    tangent_apply(dsig, De, plastic, d, wv, zv, deps);
+    if (plastic)
+        for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
+    // DPModel_check: stress-scaling return; sj from the stress before the increment
+    double smc = (sig[0] + sig[1] + sig[2]) / 3.0;
+    double ex = sig[0] - smc, ey = sig[1] - smc, ez = sig[2] - smc;
+    double sjc = sqrt(0.5 * (ex * ex + ey * ey + ez * ez) + sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5]);
+    double ts[6];
+    for (int k = 0; k < 6; k++) ts[k] = sig[k] + dsig[k] * dt;
+    double i1 = ts[0] + ts[1] + ts[2];
+    double rn = (-yalf * i1 + ykap) / sjc;
+    if (rn < 1.0) {
+        for (int k = 0; k < 3; k++) ts[k] = rn * (ts[k] - i1 / 3.0) + i1 / 3.0;
+        for (int k = 3; k < 6; k++) ts[k] = rn * ts[k];
+    }
+    for (int k = 0; k < 6; k++) dsig[k] = (ts[k] - sig[k]) / dt;
+}
+
+// Cam-Clay family (sub_camclay, sub_camunsatu): CPP_8-1-consfactor.cpp:129-281, CPP_8-4-consfactor-umcc.cpp:17-180
+__device__ __forceinline__ void camclay_update(const double sig[6], const double deps[6], const SoilSlot &s, bool unsat,
+                                               double satu, double dsatu, double dsig[6], double adps[6],
+                                               double erfs[6]) {
+    const double third = unsat ? 0.333333333 : 0.3333333333;
+    double poi = s.poi, eps0 = s.eps0, zmf = s.zmf;
+    double ymf = 3.0 * (zmf - 1.0) / (zmf + 2.0) / s.fs;
+    double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * third;
+    if (sm < 1.0) sm = 1.0;
+    double e0 = soil_e0(s, sm);
+    double hee = (s.ramda - s.kapa) / (1.0 + eps0);
+    double De[6][6], dlam, dmu;
+    iso_tangent(De, e0, poi, dlam, dmu);
+    double dmu2 = dmu * 2.0;
+    double dxv = -sig[0] - sm, dyv = -sig[1] - sm, dzv = -sig[2] - sm;
+    double dxy = -sig[3], dyz = -sig[4], dzx = -sig[5];
+    double sjj = 0.5 * (dxv * dxv + dyv * dyv + dzv * dzv) + dxy * dxy + dyz * dyz + dzx * dzx;
+    double tt1 = 1.0 / (ymf * ymf * sm * sm + 3.0 * sjj);
+    double tt2 = 2.0 * ymf * ymf * sm * tt1 - 1.0 / sm;
+    double f11, f22, f33;
+    if (unsat) {
+        f11 = tt2 * 0.333333333 + dxv * tt1 * 3.0;
+        f22 = tt2 * 0.333333333 + dyv * tt1 * 3.0;
+        f33 = tt2 * 0.333333333 + dzv * tt1 * 3.0;
+    } else {
+        f11 = tt2 / 3.0 + dxv * tt1 * 3.0;
+        f22 = tt2 / 3.0 + dyv * tt1 * 3.0;
+        f33 = tt2 / 3.0 + dzv * tt1 * 3.0;
+    }
+    double f12 = 3.0 * dxy * tt1, f23 = 3.0 * dyz * tt1, f31 = 3.0 * dzx * tt1;
+    double ttt, omake = 0.0;
+    if (unsat) {
+        double qq = (s.epsr - s.epss) / (s.sitas - s.sitar);
+        double rous = qq * (s.sitas - satu);
+        double roue = hee * (1.0 + s.eps0) * log(s.sme0 / sm);
+        ttt = s.ann * roue + s.bnn * rous;
+        ttt = pow(ttt, s.beta);
+        omake = qq * dsatu / (s.ramda - s.kapa);
+    } else {
+        double roue = (1 + eps0) * log(s.sme0 / sm);
+        if (roue < 0.0) roue = 0.0;
+        ttt = s.ann * roue * roue;
+    }
+    double hdd = f11 + f22 + f33 + ttt / sm;
+    double fkk = f11 + f22 + f33;
+    double fij = 2.0 * (f11 * f11 + f22 * f22 + f33 * f33) + f12 * f12 + f23 * f23 + f31 * f31;
+    double cc7 = dlam * (fkk * fkk) + dmu * fij + hdd / hee;
+    double d = 1.0 / cc7, df = dlam * fkk, dg = dlam * fkk;
+    double ramda = (dmu2 * f11 + dlam * fkk) * deps[0] + (dmu2 * f22 + dlam * fkk) * deps[1] +
+                   (dmu2 * f33 + dlam * fkk) * deps[2] + dmu * (f12 * deps[3] + f23 * deps[4] + f31 * deps[5]);
+    if (unsat) ramda = ramda + omake;
+    bool plastic = ramda < 0.0;
+    if (plastic && unsat) {
+        erfs[0] = omake * d * (De[0][0] * f11 + De[0][1] * f22 + De[0][2] * f33);
+        erfs[1] = omake * d * (De[1][0] * f11 + De[1][1] * f22 + De[1][2] * f33);
+        erfs[2] = omake * d * (De[2][0] * f11 + De[2][1] * f22 + De[2][2] * f33);
+        erfs[3] = omake * d * (De[3][3] * f12);
+        erfs[4] = omake * d * (De[4][4] * f23);
+        erfs[5] = omake * d * (De[5][5] * f31);
+    }
+    double wv[6] = {dg + dmu2 * f11, dg + dmu2 * f22, dg + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+    double zv[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
+    tangent_apply(dsig, De, plastic, d, wv, zv, deps);
+    if (plastic) {
+        double fv[6] = {f11, f22, f33, f12, f23, f31};
+        for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
+    }
+}
+
+// strain_to_stress for particle i (CPP_7-2-constitutive.cpp:13-321): updates sig (in the
+// caller's registers), eps, adps, e, porosity and the invariants of the soil/structure particles.
+// kFull adds the Cam-Clay family (1, 3) and FOCC (8); the host selects it only when a soil
+// particle uses one of them, so the common elastic / DP instantiation keeps a small register budget.
+template <bool kFull>
+__device__ __forceinline__ void constitutive(const State &S, const Params &P, int i, int ti, int l, double sig[6],
+                                             const double deps[6]) {
+    const double dt = P.dt;
     // weps[a][b] from the three stored components (antisymmetric, zero diagonal)
     const double w01 = S.weps[0][i], w02 = S.weps[1][i], w12 = S.weps[2][i];
     // the reference forms weps[b][a] = 0.5*(divx[b][a]-divx[a][b]) separately; that equals -weps[a][b]
@@ -697,7 +983,15 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
     } else if (ti == 2 || ti == 4) {
         double dsig[6] = {0, 0, 0, 0, 0, 0}, adps[6] = {0, 0, 0, 0, 0, 0}, erfs[6] = {0, 0, 0, 0, 0, 0};
         double De[6][6], dlam, dmu;
+        // fractional_plastic runs for matype 8/18/28 also before inip, where the reference passes
+        // its (never assigned, in practice zero) OpenMP-private compliance; zero is used here
+        bool focc = false;
+        double Cf[6][6];
+        for (int a = 0; a < 6; a++)
+            for (int b = 0; b < 6; b++) Cf[a][b] = 0.0;
         if (ti == 2) {
+            const int m8 = S.matype[i];
+            focc = (m8 == 8 || m8 == 18 || m8 == 28);
             const SoilSlot &s = S.soil[S.soil_slot[i]];
             int mt = P.inip > l ? 0 : model_of(S.matype[i]);
             double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * 0.3333333333;
@@ -706,94 +1000,11 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
                 iso_tangent(De, soil_e0(s, sm), s.poi, dlam, dmu);
                 double z6[6] = {0, 0, 0, 0, 0, 0};
                 tangent_apply(dsig, De, false, 0.0, z6, z6, deps);
-            } else if (mt == 2) {  // dpmodel1 + DPModel_check (CPP_8-1-consfactor.cpp:284-448, :779-819)
-                double e0 = soil_e0(s, sm);
-                iso_tangent(De, e0, s.poi, dlam, dmu);
-                double dmu2 = dmu * 2.0;
+            } else if (mt == 2 || mt == 5) {
                 double ci = (S.ps_c ? S.ps_c[i] : s.c) / s.fs;
-                double dxv = -sig[0] - sm, dyv = -sig[1] - sm, dzv = -sig[2] - sm;
-                double dxy = -sig[3], dyz = -sig[4], dzx = -sig[5];
-                double sjj = 0.5 * (dxv * dxv + dyv * dyv + dzv * dzv) + dxy * dxy + dyz * dyz + dzx * dzx;
-                double sj = sqrt(sjj);
-                double yalf = s.dp_yalf;
-                double ykap = 3.0 * ci * s.dp_cosf / s.dp_sq;
-                bool plastic = false;
-                double d = 0.0, wv[6], zv[6], ramda = 0.0, fv[6];
-                if ((sj - yalf * sm * 3.0 - ykap) >= 0.0) {
-                    double f11 = -yalf + dxv / sj * 0.5, f22 = -yalf + dyv / sj * 0.5, f33 = -yalf + dzv / sj * 0.5;
-                    double f12 = dxy / sj * 0.5, f23 = dyz / sj * 0.5, f31 = dzx / sj * 0.5;
-                    double fkk = f11 + f22 + f33, yd = 0.0;
-                    double g11 = -yd + dxv / sj * 0.5, g22 = -yd + dyv / sj * 0.5, g33 = -yd + dzv / sj * 0.5;
-                    double g12 = dxy / sj * 0.5, g23 = dyz / sj * 0.5, g31 = dzx / sj * 0.5;
-                    double hdd = g11 + g22 + g33;
-                    double fij = 2.0 * (f11 * g11 + f22 * g22 + f33 * g33) + f12 * g12 + f23 * g23 + f31 * g31;
-                    double cc7 = dlam * (fkk * hdd) + dmu * fij + hdd;
-                    d = 1.0 / cc7;
-                    double df = dlam * fkk, dg = dlam * hdd;
-                    ramda = (dmu2 * f11 - 3.0 * dlam * yalf) * deps[0] + (dmu2 * f22 - 3.0 * dlam * yalf) * deps[1] +
-                            (dmu2 * f33 - 3.0 * dlam * yalf) * deps[2] +
-                            dmu * (f12 * deps[3] + f23 * deps[4] + f31 * deps[5]);
-                    if (ramda * d < 0.0) {
-                        plastic = true;
-                        wv[0] = dg + dmu2 * g11; wv[1] = dg + dmu2 * g22; wv[2] = dg + dmu2 * g33;
-                        wv[3] = dmu * g12; wv[4] = dmu * g23; wv[5] = dmu * g31;
-                        zv[0] = df + dmu2 * f11; zv[1] = df + dmu2 * f22; zv[2] = df + dmu2 * f33;
-                        zv[3] = dmu * f12; zv[4] = dmu * f23; zv[5] = dmu * f31;
-                        fv[0] = f11; fv[1] = f22; fv[2] = f33; fv[3] = f12; fv[4] = f23; fv[5] = f31;
-                    }
-                }
-                tangent_apply(dsig, De, plastic, d, wv, zv, deps);
-                if (plastic)
-                    for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
-                // DPModel_check: stress-scaling return; sj from the stress before the increment
-                double smc = (sig[0] + sig[1] + sig[2]) / 3.0;
-                double ex = sig[0] - smc, ey = sig[1] - smc, ez = sig[2] - smc;
-                double sjc = sqrt(0.5 * (ex * ex + ey * ey + ez * ez) + sig[3] * sig[3] + sig[4] * sig[4] +
-                                  sig[5] * sig[5]);
-                double ts[6];
-                for (int k = 0; k < 6; k++) ts[k] = sig[k] + dsig[k] * dt;
-                double i1 = ts[0] + ts[1] + ts[2];
-                double kc = 3.0 * ci * s.dp_cosf / s.dp_sq;
-                double rn = (-yalf * i1 + kc) / sjc;
-                if (rn < 1.0) {
-                    for (int k = 0; k < 3; k++) ts[k] = rn * (ts[k] - i1 / 3.0) + i1 / 3.0;
-                    for (int k = 3; k < 6; k++) ts[k] = rn * ts[k];
-                }
-                for (int k = 0; k < 6; k++) dsig[k] = (ts[k] - sig[k]) / dt;
-            } else if (mt == 1) {  // sub_camclay (CPP_8-1-consfactor.cpp:129-281)
-                double poi = s.poi, eps0 = s.eps0, zmf = s.zmf;
-                double ymf = 3.0 * (zmf - 1.0) / (zmf + 2.0) / s.fs;
-                double e0 = soil_e0(s, sm);
-                double hee = (s.ramda - s.kapa) / (1.0 + eps0);
-                iso_tangent(De, e0, poi, dlam, dmu);
-                double dmu2 = dmu * 2.0;
-                double dxv = -sig[0] - sm, dyv = -sig[1] - sm, dzv = -sig[2] - sm;
-                double dxy = -sig[3], dyz = -sig[4], dzx = -sig[5];
-                double sjj = 0.5 * (dxv * dxv + dyv * dyv + dzv * dzv) + dxy * dxy + dyz * dyz + dzx * dzx;
-                double tt1 = 1.0 / (ymf * ymf * sm * sm + 3.0 * sjj);
-                double tt2 = 2.0 * ymf * ymf * sm * tt1 - 1.0 / sm;
-                double f11 = tt2 / 3.0 + dxv * tt1 * 3.0, f22 = tt2 / 3.0 + dyv * tt1 * 3.0,
-                       f33 = tt2 / 3.0 + dzv * tt1 * 3.0;
-                double f12 = 3.0 * dxy * tt1, f23 = 3.0 * dyz * tt1, f31 = 3.0 * dzx * tt1;
-                double roue = (1 + eps0) * log(s.sme0 / sm);
-                if (roue < 0.0) roue = 0.0;
-                double ttt = s.ann * roue * roue;
-                double hdd = f11 + f22 + f33 + ttt / sm;
-                double fkk = f11 + f22 + f33;
-                double fij = 2.0 * (f11 * f11 + f22 * f22 + f33 * f33) + f12 * f12 + f23 * f23 + f31 * f31;
-                double cc7 = dlam * (fkk * fkk) + dmu * fij + hdd / hee;
-                double d = 1.0 / cc7, df = dlam * fkk, dg = dlam * fkk;
-                double ramda = (dmu2 * f11 + dlam * fkk) * deps[0] + (dmu2 * f22 + dlam * fkk) * deps[1] +
-                               (dmu2 * f33 + dlam * fkk) * deps[2] +
-                               dmu * (f12 * deps[3] + f23 * deps[4] + f31 * deps[5]);
-                bool plastic = ramda < 0.0;
-                double wv[6] = {dg + dmu2 * f11, dg + dmu2 * f22, dg + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
-                double zv[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
-                tangent_apply(dsig, De, plastic, d, wv, zv, deps);
-                if (plastic) {
-                    double fv[6] = {f11, f22, f33, f12, f23, f31};
-                    for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
-                }
+                dp_update(sig, deps, s, ci, s.dp_yalf_dil, mt == 5, dt, dsig, adps);
+            } else if (kFull && (mt == 1 || mt == 3)) {
+                camclay_update(sig, deps, s, mt == 3, S.satu[i], S.dsatu[i], dsig, adps, erfs);
             } else if (mt == 4) {
                 // sub_camanisotropy: body commented out in the reference (quirk A13): zero tangent
                 double Z[6][6];
@@ -801,8 +1012,15 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
                     for (int b = 0; b < 6; b++) Z[a][b] = 0.0;
                 double z6[6] = {0, 0, 0, 0, 0, 0};
                 tangent_apply(dsig, Z, false, 0.0, z6, z6, deps);
+            } else if (kFull && mt == 8) {
+                double D[6][6];
+                focc_tangent(sig, deps, S.e[i], s, D, Cf);
+                double z6[6] = {0, 0, 0, 0, 0, 0};
+                tangent_apply(dsig, D, false, 0.0, z6, z6, deps);
             } else if (mt >= 0) {
-                set_err(S, 4);  // model not implemented in this build
+                // viscous-elastic (7) and Mohr-Coulomb (9) are not implemented (the host also
+                // refuses 1 / 3 / 8 in the reduced instantiation)
+                set_err(S, 4);
             }
         } else {
             // structure (CPP_7-2-constitutive.cpp:276-319)
@@ -818,6 +1036,14 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
         dsig[3] = dsig[3] + (W[0][0] * s[3] + W[0][1] * s[1] + W[0][2] * s[4]) - (s[0] * W[0][1] + s[3] * W[1][1] + s[5] * W[2][1]);
         dsig[4] = dsig[4] + (W[1][0] * s[5] + W[1][1] * s[4] + W[1][2] * s[2]) - (s[3] * W[0][2] + s[1] * W[1][2] + s[4] * W[2][2]);
         dsig[5] = dsig[5] + (W[0][0] * s[5] + W[0][1] * s[4] + W[0][2] * s[2]) - (s[0] * W[0][2] + s[3] * W[1][2] + s[5] * W[2][2]);
+        if (kFull && focc) {
+            // fractional_plastic (CPP_8-5-consfactor-focc.cpp:243-259)
+            for (int a = 0; a < 6; a++) {
+                double ee = 0.0;
+                for (int b = 0; b < 6; b++) ee = ee + Cf[a][b] * dsig[b];
+                adps[a] = deps[a] - ee;
+            }
+        }
         double eps[6];
         for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
         for (int k = 0; k < 6; k++) {
@@ -843,6 +1069,11 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
         S.divq[i] = divq;
         S.divr[i] = divr;
     }
+}
+
+// free_surface_check (+ stress_adjust), the per-particle factor of the solid momentum sums,
+// principal stresses, and the artificial-stress tensor of particle i.
+__device__ __forceinline__ void stress_post(const State &S, const Params &P, int i, int ti, double sig[6]) {
     // free-surface correction (CPP_7-2-constitutive.cpp:481-505)
     const double color = S.color[i], tmin = P.tmin, tmax = P.tmax, incl = tmax - tmin;
     if (ti == 1) {
@@ -883,11 +1114,7 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
     // sig / rho / rho, the per-neighbour factor of the solid momentum sums (same operation order)
     {
         const double r = S.rho[i];
-        const int keep = (ti != 0 && ti != 7);
-        for (int k = 0; k < 6; k++) {
-            double sk = keep ? sig[k] : S.sig[k][i];
-            S.qsig[k][i] = sk / r / r;
-        }
+        for (int k = 0; k < 6; k++) S.qsig[k][i] = sig[k] / r / r;
     }
     // principal stresses (CPP_7-1-strain.cpp:283-332)
     double a[9] = {sig[0], sig[3], sig[5], sig[3], sig[1], sig[4], sig[5], sig[4], sig[2]}, v[9];
@@ -896,12 +1123,184 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P) {
         S.strep[k][i] = a[k * 3 + k];
         for (int c = 0; c < 3; c++) S.beta[c * 3 + k][i] = v[k * 3 + c];  // beta[c][k] = v[3k+c]
     }
+    // artificial-stress tensor (CPP_9-5-artificialstress.cpp:42-89 in 2D, :161-227 in 3D)
+    if (P.art_stre_coe > 0.0 && ti == 2) {
+        const double pi = 3.141592653, xs = P.art_stre_coe, rho = S.rho[i];
+        double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (P.ndim == 2) {
+            double sita;
+            if (fabs(sig[0] - sig[1]) < 1.0e-6) {
+                if (sig[3] > 0.0) sita = -pi * 0.78539816325;  // quirk A15
+                else if (sig[3] < 0.0) sita = pi * 0.78539816325;
+                else sita = 0.0;
+            } else {
+                sita = 0.5 * atan(2.0 * sig[3] / (sig[0] - sig[1]));
+            }
+            double cs = cos(sita), sn = sin(sita);
+            double sxx = cs * cs * sig[0] + 2.0 * sn * cs * sig[3] + sn * sn * sig[1];
+            double syy = sn * sn * sig[0] - 2.0 * sn * cs * sig[3] + cs * cs * sig[1];
+            double rxx = sxx > 0.0 ? -xs * sxx / rho / rho : 0.0;
+            double ryy = syy > 0.0 ? -xs * syy / rho / rho : 0.0;
+            R[0][0] = rxx * cs * cs + ryy * sn * sn;
+            R[1][1] = rxx * sn * sn + ryy * cs * cs;
+            R[0][1] = (rxx - ryy) * sn * cs;
+            R[1][0] = R[0][1];
+        } else {
+            double rr[3], B[3][3], Bt[3][3], bi[3][3], bti[3][3], tmp[3][3];
+            for (int k = 0; k < 3; k++) rr[k] = a[k * 3 + k] > 0.0 ? -xs * a[k * 3 + k] / rho / rho : 0.0;
+            for (int r = 0; r < 3; r++)
+                for (int q = 0; q < 3; q++) {
+                    B[r][q] = v[q * 3 + r];   // beta
+                    Bt[r][q] = v[r * 3 + q];  // beta_ts
+                    bi[r][q] = bti[r][q] = 0.0;
+                }
+            auto inv3 = [](const double A[3][3], double Bo[3][3]) {
+                double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                             A[1][0] * (A[0][1] * A[2][2] - A[0][2] * A[2][1]) +
+                             A[2][0] * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+                if (fabs(det) > 1.0e-5) {
+                    Bo[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+                    Bo[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+                    Bo[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+                    Bo[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+                    Bo[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+                    Bo[1][2] = (A[1][0] * A[0][2] - A[0][0] * A[1][2]) / det;
+                    Bo[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+                    Bo[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+                    Bo[2][2] = (A[0][0] * A[1][1] - A[1][0] * A[0][1]) / det;
+                }
+            };
+            inv3(B, bi);
+            inv3(Bt, bti);
+            double st[3][3] = {{rr[0], 0.0, 0.0}, {0.0, rr[1], 0.0}, {0.0, 0.0, rr[2]}};
+            for (int r = 0; r < 3; r++)
+                for (int q = 0; q < 3; q++) tmp[r][q] = bi[r][0] * st[0][q] + bi[r][1] * st[1][q] + bi[r][2] * st[2][q];
+            for (int r = 0; r < 3; r++)
+                for (int q = 0; q < 3; q++) R[r][q] = tmp[r][0] * bti[0][q] + tmp[r][1] * bti[1][q] + tmp[r][2] * bti[2][q];
+        }
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) S.rij[r * 3 + q][i] = R[r][q];
+    }
+}
+
+// part 0: constitutive update and post-processing in one pass; part 1: constitutive update only;
+// part 2: post-processing of the stress in memory (a stress regularisation pass sits between 1 and 2)
+template <bool kFull>
+__global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
+    int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    double sig[6];
+    for (int k = 0; k < 6; k++) sig[k] = S.sig[k][i];
+    if (part != 2) {
+        double deps[6];
+        for (int k = 0; k < 6; k++) deps[k] = S.deps[k][i];
+        constitutive<kFull>(S, P, i, ti, step_of(S), sig, deps);
+        if (part == 1) {
+            if (ti != 0 && ti != 7)
+                for (int k = 0; k < 6; k++) S.sig[k][i] = sig[k];
+            return;
+        }
+    }
+    stress_post(S, P, i, ti, sig);
+}
+
+// Jacobi form of strain_regu / stress_regu (CPP_7-2-constitutive.cpp:385-450): every step the
+// kernels check l % step_regu themselves so the captured step graph stays fixed.
+__global__ __launch_bounds__(256) void k_regu_copy(State S, Params P, int which_sig) {
+    int i = pidx();
+    if (i >= P.n || (step_of(S) % P.step_regu) != 0) return;
+    for (int k = 0; k < 6; k++) S.tmp6[k][i] = which_sig ? S.sig[k][i] : S.eps[k][i];
+}
+
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_regu(State S, Params P, int which_sig) {
+    int i = pidx();
+    if (i >= P.n || (step_of(S) % P.step_regu) != 0) return;
+    const int ti = S.type[i];
+    const int nc = nb_count(S, i);
+    double up[6] = {0, 0, 0, 0, 0, 0}, dn = 0.0, w[4];
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        if (S.type[j] != ti) continue;
+        load_w<NDIM>(S, P, nj, i, w);
+        double mj = S.mass[j], rj = S.rho[j];
+        for (int k = 0; k < 6; k++) up[k] = up[k] + mj / rj * w[3] * S.tmp6[k][j];
+        dn = dn + mj / rj * w[3];
+    }
+    if (fabs(dn) > 1.0e-6)
+        for (int k = 0; k < 6; k++) (which_sig ? S.sig : S.eps)[k][i] = up[k] / dn;
 }
 
 // ------------------------------------------------------------------------------------------
 // accelerations, artificial viscosity, boundary stress, Darcy drag, velocity update
 // ------------------------------------------------------------------------------------------
 __device__ __constant__ int kSidx[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+
+// Pair sums that the momentum kernel adds in the reference's order when they are switched on:
+// artificial stress (soil, same-type pairs) into tmp6[0..2] and the geostress damping sum
+// (soil / structure, same-type pairs, l <= inip) into tmp6[3..5]. Split out so the momentum
+// kernel keeps its register budget when neither is active.
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
+    int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
+    const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
+    if (!art_on && !damp_on) return;
+    double xi[3], xj[3], w[4];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const int nc = nb_count(S, i);
+    const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    double Ri[9], tu_art[3] = {0, 0, 0}, damp[3] = {0, 0, 0};
+    if (art_on)
+        for (int k = 0; k < 9; k++) Ri[k] = S.rij[k][i];
+    const double cdi = damp_on ? S.cd[i] : 0.0;
+    for (int nj = 0; nj < nc; nj++) {
+        int j = S.nbr[(size_t)nj * P.cap + i];
+        int tj = S.type[j];
+        if (tj != ti) continue;
+        load_w<NDIM>(S, P, nj, i, w);
+        double mj = S.mass[j], rj = S.rho[j];
+        if (art_on) {
+            load_xj<NDIM>(S, cur, j, xj);
+            // artificial_stress_2d / _3d pair sum (CPP_9-5-artificialstress.cpp:92-140, :230-317)
+            double dst = 0.0;
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) dst = dst + (xi[k] - xj[k]) * (xi[k] - xj[k]);
+            dst = sqrt(dst);
+            double hl = (hli + S.hl[j]) * 0.5;
+            double q = dst / hl, f;
+            if (q >= 0.0 && q < 1.0) f = pow((0.666666666666667 - q * q + 0.5 * q * q * q) * 3.82300884955752, 2.55);
+            else if (q >= 1.0 && q < 2.0) f = pow((2.0 - q) * (2.0 - q) * (2.0 - q) * 0.629737609329447, 2.55);
+            else f = 0.0;
+            double Rj[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) Rj[k] = S.rij[k][j];
+            if (NDIM == 2) {
+                tu_art[0] = tu_art[0] + mj * f * (Ri[0] + Rj[0]) * w[0] + mj * f * (Ri[1] + Rj[1]) * w[1];
+                tu_art[1] = tu_art[1] + mj * f * (Ri[3] + Rj[3]) * w[0] + mj * f * (Ri[4] + Rj[4]) * w[1];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    tu_art[k] = tu_art[k] + mj * f * (Ri[3 * k] + Rj[3 * k]) * w[0] +
+                                mj * f * (Ri[3 * k + 1] + Rj[3 * k + 1]) * w[1] +
+                                mj * f * (Ri[3 * k + 2] + Rj[3 * k + 2]) * w[2];
+            }
+        }
+        if (damp_on) {
+            // soil_damping_geostress pair sum (CPP_9-4-derivation_stress.cpp:858-872)
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) damp[k] = damp[k] + mj / rj * cdi * (vi[k] - S.vxp[k][j]) * w[3];
+        }
+    }
+    if (art_on)
+        for (int k = 0; k < 3; k++) S.tmp6[k][i] = tu_art[k];
+    if (damp_on)
+        for (int k = 0; k < 3; k++) S.tmp6[3 + k][i] = damp[k];
+}
 
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
@@ -951,6 +1350,9 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     double fluid_t[3][3], solid_t[3][3], bnd_t[3][3], av[3] = {0, 0, 0}, drag[3] = {0, 0, 0};
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) fluid_t[a][b] = solid_t[a][b] = bnd_t[a][b] = 0.0;
+    // artificial stress (ti == 2) and geostress damping (ti 2 / 4, l <= inip): pair sums from k_accel_pre
+    const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
+    const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
     double aif[6], qi[6];
     for (int k = 0; k < 6; k++) aif[k] = fpor * sigi[k];
     for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
@@ -1080,6 +1482,8 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     }
     if (ti != 7)
         for (int k = 0; k < NDIM; k++) ax[k] = ax[k] - av[k];
+    if (art_on)
+        for (int k = 0; k < NDIM; k++) ax[k] = ax[k] + S.tmp6[k][i];
     if (ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2)) {
         ax[0] += bnd_t[0][0] + bnd_t[0][1] + bnd_t[0][2];
         ax[1] += bnd_t[1][0] + bnd_t[1][1] + bnd_t[1][2];
@@ -1102,8 +1506,53 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             S.interf[k][i] = itf;
         }
     }
+    if (damp_on) {
+        // soil_damping_geostress clamp (CPP_9-4-derivation_stress.cpp:874-895)
+        double damp[3];
+        for (int k = 0; k < NDIM; k++) damp[k] = S.tmp6[3 + k][i];
+        for (int k = 0; k < NDIM; k++) {
+            double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * dt;
+            double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - damp[k]) * dt;
+            if (vx1 * vx2 <= 0.0) ax[k] = -2.0 * vi[k] / dt - axp[k];
+            else ax[k] -= damp[k];
+        }
+    }
+    // veocity_update (CPP_11-xsph.cpp:11-142)
+    if (P.if_move == 1) {
+        if (ti == 0) {
+            const bool mv = S.matype[i] == 2 && step_of(S) > P.inip;
+            for (int k = 0; k < NDIM; k++) S.interfss[k][i] = mv ? P.move_vx[k] : 0.0;
+        } else {
+            for (int k = 0; k < NDIM; k++) {
+                double ath = (axp[k] + ax[k]) * 0.5;
+                S.vx[cur][k][i] = vi[k] + ath * dt;
+            }
+        }
+        for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
+        return;
+    }
+    if (P.if_move == 2 || P.if_move == 3) {
+        // the driven particles take the prescribed velocity; their vxp is replaced in the update
+        // kernel (neighbours still read the old vxp during this pass)
+        const int mt = P.if_move == 2 ? 2 : 4;
+        double vp[3] = {vi[0], vi[1], vi[2]}, ap[3] = {axp[0], axp[1], axp[2]};
+        if (ti == mt && S.etype[i] == 5 && step_of(S) > P.inip) {
+            for (int k = 0; k < NDIM; k++) {
+                ap[k] = 0.0;
+                vp[k] = P.move_vx[k];
+                ax[k] = 0.0;
+                S.axp[k][i] = 0.0;
+            }
+        }
+        for (int k = 0; k < NDIM; k++) {
+            double ath = (ap[k] + ax[k]) * 0.5;
+            S.vx[cur][k][i] = vp[k] + ath * dt;
+        }
+        for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
+        return;
+    }
     for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
-    // veocity_update, if_move == 0 (CPP_11-xsph.cpp:11-57)
+    // if_move == 0
     if (ti == 0) {
         int mt = S.matype[i];
         if (mt < 2) {
@@ -1148,7 +1597,8 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     double tu[3] = {0, 0, 0}, dn = 0.0;
     const double xsame = ti == 1 ? P.xsph[0] : ti == 2 ? P.xsph[1] : ti == 3 ? P.xsph[2] : P.xsph[1];
     const double xbnd = (ti == 1 || ti == 3) ? 0.006 : P.xsph[1] / 5.0;
-    for (int nj = 0; nj < nc; nj++) {
+    const int nxs = P.if_move == 2 ? 0 : nc;  // xsph is skipped when soil is driven (CPP_1-SPHLoops.cpp:714)
+    for (int nj = 0; nj < nxs; nj++) {
         int j = S.nbr[(size_t)nj * P.cap + i];
         int tj = S.type[j];
         double xs;
@@ -1162,10 +1612,13 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
         dn = dn + mj * w3 / rj;
     }
     double v[3] = {vi[0], vi[1], vi[2]};
-    if (fabs(dn) > 1.0e-7 && (ti == 2 || ti == 4))
+    if (P.if_move == 2) {
+        // v stays the velocity of the momentum update
+    } else if (fabs(dn) > 1.0e-7 && (ti == 2 || ti == 4)) {
         for (int k = 0; k < NDIM; k++) v[k] = vi[k] + tu[k] / dn;
-    else
+    } else {
         for (int k = 0; k < NDIM; k++) v[k] = vi[k] + tu[k];
+    }
     const double dr = P.dr;
     // velocity_set_domain (flbndy 2, fluids) / tmboundary (flbndy 1), CPP_13-boundary.cpp:71-177
     if ((P.flbndy == 2 && (ti == 1 || ti == 3)) || P.flbndy == 1) {
@@ -1343,9 +1796,11 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
         }
     }
     const int loop = l - P.inip;
+    const bool driven = (P.if_move == 2 || P.if_move == 3) && ti == (P.if_move == 2 ? 2 : 4) &&
+                        S.etype[i] == 5 && l > P.inip;
     if (ti != 0) {
         for (int k = 0; k < NDIM; k++) {
-            double vp = S.vxp[k][i];
+            double vp = driven ? P.move_vx[k] : S.vxp[k][i];
             double xn = xi[k] + (v[k] + vp) * dt * 0.5;
             double u = S.ux[k][i];
             if (loop == 0) u = 0.0;
@@ -1395,5 +1850,13 @@ template __global__ void k_xsph<2>(State, Params, int);
 template __global__ void k_xsph<3>(State, Params, int);
 template __global__ void k_update<2>(State, Params, int);
 template __global__ void k_update<3>(State, Params, int);
+template __global__ void k_regu<2>(State, Params, int);
+template __global__ void k_stress<false>(State, Params, int);
+template __global__ void k_stress<true>(State, Params, int);
+template __global__ void k_saturation<2>(State, Params, int);
+template __global__ void k_saturation<3>(State, Params, int);
+template __global__ void k_accel_pre<2>(State, Params, int);
+template __global__ void k_accel_pre<3>(State, Params, int);
+template __global__ void k_regu<3>(State, Params, int);
 
 }  // namespace sphg

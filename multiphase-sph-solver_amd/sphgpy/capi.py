@@ -110,7 +110,7 @@ PARTICLE_FIELDS = {
     "interfss": 3, "rho": 1, "rhop": 1, "mass": 1, "hl": 1, "pre": 1, "prep": 1, "e": 1, "eps": 6,
     "deps": 6, "sig": 6, "dsig": 6, "weps": 9, "adps": 6, "satu": 1, "dsatu": 1, "suct": 1, "cop": 1,
     "vsig": 1, "veps": 1, "divq": 1, "divr": 1, "meps": 1, "roue": 1, "rous": 1, "prew": 1, "prea": 1,
-    "porosity": 1, "cd": 1, "poro_cop": 1, "beta": 9, "beta_ts": 9,
+    "porosity": 1, "cd": 1, "poro_cop": 1, "beta": 9, "beta_ts": 9, "Rij": 9,
 }
 INT_FIELDS = ("type", "matype", "permtype", "etype")
 # fields every step reads or writes (what must be uploaded for a restart)

@@ -418,3 +418,61 @@ def model_zoo_2d(nx=12, nbands=7, band_h=3, water_h=3, dx=0.002, steps=12, seed=
     parts.append((_lattice2d(-3, 0, 0, top + water_h + 3, dx), 0, 0))
     parts.append((_lattice2d(nx, nx + 3, 0, top + water_h + 3, dx), 0, 0))
     return _assemble(d, parts, rng)
+
+
+def moving_box_2d(if_move=1, nx=10, ny=8, dx=0.002, steps=12, seed=13, inip=2, speed=0.05) -> Deck:
+    """2D Drucker-Prager soil block in a three-sided box whose driven part follows the
+    Para_Boundary velocity (CPP_11-xsph.cpp:58-142):
+      if_move 1: the right wall (type 0, matype 2) moves inwards at `speed`;
+      if_move 2: the two top soil rows (etype 5) are pushed down at `speed`;
+      if_move 3: an elastic structure plate (type 4, etype 5) on top is pushed down at `speed`."""
+    rng = np.random.default_rng(seed)
+    move = (-speed, 0.0, 0.0) if if_move == 1 else (0.0, -speed, 0.0)
+    d = Deck(ndim=2, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=(0.0, -9.81, 0.0),
+             art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), inip=inip, flbndy=2, if_move=if_move, move_vx=move,
+             name=f"moving_box_2d_m{if_move}")
+    d.model_flags[2] = 1
+    d.soil_layers[2] = [dp_layer(c=200.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0)]
+    d.structure = {"dens": 2500.0, "eps0": 0.0, "cop": 0.0, "e": 2.0e7, "poi": 0.25}
+    parts = [(_lattice2d(0, nx, 0, ny, dx), 2, 2),
+             (_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0),
+             (_lattice2d(-3, 0, 0, ny + 4, dx), 0, 0),
+             (_lattice2d(nx, nx + 3, 0, ny + 4, dx), 0, 2 if if_move == 1 else 0)]
+    if if_move == 3:
+        parts.append((_lattice2d(0, nx, ny, ny + 2, dx), 4, 0))
+    deck = _assemble(d, parts, rng)
+    if if_move == 2:
+        top = (deck.type == 2) & (deck.xp[:, 1] > (ny - 2) * dx)
+        deck.etype[top] = 5
+    elif if_move == 3:
+        deck.etype[deck.type == 4] = 5
+    return deck
+
+
+def art_stress_block(ndim=3, nx=5, ny=4, nz=6, dx=0.004, steps=10, seed=17, art_stress=0.3, step_regu=2,
+                     inip=2, stretch=150.0) -> Deck:
+    """Soil block of an elastic lower half and a Drucker-Prager upper half in a box (2D: nx by nz,
+    3D: nx by ny by nz), with the artificial stress (CPP_9-5-artificialstress.cpp), strain/stress
+    regularisation every `step_regu` steps and initial geostress damping switched on. The soil
+    starts with the horizontal stretching velocity `stretch` * (x - x_centre), so that tensile
+    principal stresses, the only ones the artificial stress acts on, appear within a few steps."""
+    rng = np.random.default_rng(seed)
+    grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
+    d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav,
+             art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), art_stre_coe=art_stress, step_regu=step_regu,
+             inip=inip, flbndy=2, damp_coe=0.05, name=f"art_stress_block_{ndim}d")
+    d.model_flags[0] = 1
+    d.model_flags[2] = 1
+    d.soil_layers = {0: [_layer(0, e=3.0e6)], 2: [_layer(2, c=400.0, fai=30.0, e=1.8e6)]}
+    h = nz // 2
+    if ndim == 2:
+        parts = [(_lattice2d(0, nx, 0, h, dx), 2, 0), (_lattice2d(0, nx, h, nz, dx), 2, 2),
+                 (_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0), (_lattice2d(-3, 0, 0, nz + 3, dx), 0, 0),
+                 (_lattice2d(nx, nx + 3, 0, nz + 3, dx), 0, 0)]
+    else:
+        parts = [(_lattice3d(0, nx, 0, ny, 0, h, dx), 2, 0), (_lattice3d(0, nx, 0, ny, h, nz, dx), 2, 2),
+                 (_box_walls_3d(nx, ny, nz + 3, dx), 0, 0)]
+    deck = _assemble(d, parts, rng)
+    soil = deck.type == 2
+    deck.vxp[soil, 0] = stretch * (deck.xp[soil, 0] - 0.5 * nx * dx)
+    return deck

@@ -1211,7 +1211,11 @@ static int or_strain_to_stress(or_ctx *c) {
 #pragma omp parallel for schedule(static) num_threads(c->nthreads) reduction(max : bad)
     for (i = 0; i < n; i++) {
         OPar *pi = &c->p[i];
-        double D[6][6], erfs[6], adps[6], deps[6], Cf[6][6];
+        /* Cf: the reference's OpenMP-private cijkl is never assigned before fractional_plastic
+         * runs for matype 8/18/28 while l < inip (CPP_7-2-constitutive.cpp:17, :24, :255-258);
+         * fresh thread stacks hold zeros there in practice, so zero is used (denormal residues
+         * of the reference's run-to-run garbage aside) */
+        double D[6][6], erfs[6], adps[6], deps[6], Cf[6][6] = {{0}};
         int k, a, b;
         if (pi->type == 1 || pi->type == 3) {
             const sphg_fluid *f = &c->pf[i];

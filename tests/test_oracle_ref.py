@@ -10,7 +10,7 @@ import pytest
 
 from fixtures import load_golden, bitequal, flush_denormal
 from helpers import OracleSim, GS, JACOBI, have_ref, run_ref, max_rel
-from sphgpy import column_collapse_2d, soil_water_column_3d, model_zoo_2d
+from sphgpy import column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d, art_stress_block
 from sphgpy.capi import PARTICLE_FIELDS
 
 COMPARE = [k for k in PARTICLE_FIELDS]
@@ -30,7 +30,11 @@ def _mismatches(sim, ref):
     return bad
 
 
-@pytest.mark.parametrize("name", ["column2d_small", "soilwater3d_small", "model_zoo_2d"])
+GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1", "moving_box_m2", "moving_box_m3",
+          "art_stress_2d", "art_stress_3d"]
+
+
+@pytest.mark.parametrize("name", GOLDEN)
 def test_oracle_matches_golden_bitwise(name):
     deck, states = load_golden(name)
     sim = OracleSim(deck, states[0], mode=GS)
@@ -66,6 +70,11 @@ def test_jacobi_differs_from_gs_only_slightly():
     (lambda: column_collapse_2d(nx=24, ny=40, wall_h=50, floor_len=80, steps=40, seed=7), 40),
     (lambda: soil_water_column_3d(nx=6, ny=5, nz_soil=6, nz_water=3, steps=12, seed=3), 12),
     (lambda: model_zoo_2d(nx=14, steps=20, seed=11), 20),
+    (lambda: moving_box_2d(1, steps=12), 12),
+    (lambda: moving_box_2d(2, steps=12), 12),
+    (lambda: moving_box_2d(3, steps=12), 12),
+    (lambda: art_stress_block(2, steps=10), 10),
+    (lambda: art_stress_block(3, steps=10), 10),
 ])
 def test_oracle_matches_reference_every_step(gen, steps):
     deck = gen()
