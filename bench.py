@@ -14,8 +14,9 @@ Extra fields on the JSON line:
   roofline      dominant kernel: algorithmic bytes per launch / measured average duration vs
                 8 TB/s; `traffic` = HBM bytes per launch from the committed rocprofv3 PMC summary
                 (profiles/pmc_<workload>.json) when present, else null.
-  cpu_baseline  the FP64 oracle (a restatement of the reference loop, validated bit-exact against
-                it) timed on this host's cores on a bounded sample of the same deck.
+  cpu_baseline  the reference solver itself (compiled from its sources into oracle/_ref) timed on
+                this host's cores on a bounded sample of the same deck; where that build is absent,
+                the FP64 oracle (a restatement of the reference loop, validated bit-exact against it).
 """
 from __future__ import annotations
 
@@ -75,6 +76,41 @@ WORKLOAD_DESC = {
     "column2d": "2D dry granular column collapse: soil 60x120 at dx=0.002 m + 3-layer floor/left wall, "
                 "DP c=0 phi=30 E=1.8 MPa, AV alpha=0.1",
 }
+
+
+REF_HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+
+
+def _ref_wall(deck, steps: int, threads: int) -> float:
+    import subprocess
+    import tempfile
+    d = deck.copy()
+    d.loop, d.fop, d.ntd = steps, 10 ** 8, threads
+    with tempfile.TemporaryDirectory(prefix="sphref_") as wd:
+        d.write_input_dat(os.path.join(wd, "input.dat"))
+        t0 = time.perf_counter()
+        r = subprocess.run([REF_HARNESS, wd], capture_output=True)
+        el = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(f"reference harness failed: rc={r.returncode}")
+    return el
+
+
+def cpu_baseline_reference(deck, budget_s: float):
+    """The reference solver itself (its own sources compiled into oracle/_ref, OpenMP with the
+    deck's ntd threads). Set-up, the initial and step-1 dumps are the same in every run, so the
+    per-step time is the wall-time difference of a short and a long run."""
+    threads = min(os.cpu_count() or 1, 16)
+    t2 = _ref_wall(deck, 2, threads)
+    t4 = _ref_wall(deck, 4, threads)
+    est = max((t4 - t2) / 2.0, 1e-4)
+    k = int(min(max(budget_s / est, 4), 400))
+    tk = _ref_wall(deck, 2 + k, threads)
+    per_step = (tk - t2) / k
+    return {"value": deck.n / per_step, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"reference solver (its own sources, g++ -O2 -fopenmp, ntd={threads}) on the same "
+                      f"{deck.n}-particle deck: ({k + 2}-step run - 2-step run) / {k} steps = "
+                      f"{per_step * 1e3:.1f} ms/step"}
 
 
 def cpu_baseline(deck, state, budget_s: float):
@@ -189,7 +225,10 @@ def main():
         "kernels": kernels,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(deck, state, args.cpu_budget)
+        if os.path.exists(REF_HARNESS):
+            line["cpu_baseline"] = cpu_baseline_reference(deck, args.cpu_budget)
+        else:
+            line["cpu_baseline"] = cpu_baseline(deck, state, args.cpu_budget)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -139,22 +139,33 @@ int sphg_setting_par(const sphg_problem *pro, const sphg_materials *mat, int n,
 int sphg_parttocell(sphg_ctx *ctx);
 /* clNNPS_Fun::partisearching with weight() (CPP_3-NNPS.cpp:265-374, :17-48). */
 int sphg_partisearching(sphg_ctx *ctx);
-/* clDensity_Fun::density (CPP_6-density.cpp:11-189); normalisation is Jacobi (see DESIGN). */
+/* clDensity_Fun::density (CPP_6-density.cpp:11-189). The normalisation pass reads every
+ * neighbour's pre-normalisation density (Jacobi form), where the reference's serial loop reads
+ * the already normalised densities of lower-index neighbours in place (:168-188). */
 int sphg_density(sphg_ctx *ctx);
 /* clStraStre_Fun::strain_noreg (CPP_7-1-strain.cpp:20-220) fused with the pair sums of eos loop 2
  * (CPP_7-1-eos_CR.cpp:45-177) and of free_surface_check (CPP_7-2-constitutive.cpp:453-480);
- * eos loop 1 (:20-43) runs at the end of sphg_density. */
+ * strain_regu every step_regu steps (CPP_7-2-constitutive.cpp:419-450, Jacobi form),
+ * saturation + moisturesuct when flagum == 1 (CPP_7-1-strain.cpp:222-280) and cal_permeability
+ * when fli == 1 (CPP_7-2-constitutive.cpp:355-382). eos loop 1 (CPP_7-1-eos_CR.cpp:20-43) runs
+ * at the end of sphg_density. */
 int sphg_strain(sphg_ctx *ctx);
-/* clStraStre_Fun::strain_to_stress (CPP_7-2-constitutive.cpp:13-321) + free-surface stress
- * correction (:481-505) + Stress_eigValue (CPP_7-1-strain.cpp:283-332). */
+/* clStraStre_Fun::strain_to_stress (CPP_7-2-constitutive.cpp:13-321; models elastic, SB, DP,
+ * UMCC, AMCC, Bui DP, FOCC: CPP_8-1/8-2/8-4/8-5) + stress_regu every step_regu steps (:385-418,
+ * Jacobi form) + free-surface stress correction (:481-505) + Stress_eigValue
+ * (CPP_7-1-strain.cpp:283-332) + the per-particle tensor of artificial_stress_2d/3d
+ * (CPP_9-5-artificialstress.cpp:42-89, :161-227). */
 int sphg_strain_to_stress(sphg_ctx *ctx);
 /* clAcce_Fun fluid/soil/structure accelerations (CPP_9-4-derivation_stress.cpp:12-505),
- * artificial_viscosity (CPP_9-3-artificialvis.cpp:12-99), boundary stress effect
+ * artificial_viscosity (CPP_9-3-artificialvis.cpp:12-99), the artificial-stress pair sum
+ * (CPP_9-5-artificialstress.cpp:92-140, :230-317), boundary stress effect
  * (CPP_9-4-derivation_stress.cpp:631-721), interact1 (CPP_10-interact1.cpp:15-110),
- * soil_damping_geostress (:850-897) and veocity_update (CPP_11-xsph.cpp:11-142). */
+ * soil_damping_geostress (CPP_9-4-derivation_stress.cpp:850-897) and veocity_update for
+ * if_move 0-3 (CPP_11-xsph.cpp:11-142). */
 int sphg_accel(sphg_ctx *ctx);
-/* clVel_Fun::xsph (CPP_11-xsph.cpp:145-256) + velocity_set_domain (CPP_13-boundary.cpp:71-122)
- * + tm_boundary_2d/3d first loop (CPP_13-boundary.cpp:179-283, :388-520). */
+/* clVel_Fun::xsph (CPP_11-xsph.cpp:145-256; skipped when if_move == 2, CPP_1-SPHLoops.cpp:714)
+ * + velocity_set_domain (CPP_13-boundary.cpp:71-122) + tm_boundary_2d/3d first loop
+ * (CPP_13-boundary.cpp:179-283, :388-520). */
 int sphg_xsph(sphg_ctx *ctx);
 /* tm_boundary_2d/3d second loop (CPP_13-boundary.cpp:284-386) + check_domain (:20-68) +
  * clUpdate_Fun::update (CPP_12-1-update.cpp:11-55). */

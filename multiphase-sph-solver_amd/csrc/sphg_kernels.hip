@@ -1713,8 +1713,10 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
             plane(p1, p3, p4, c2);
             double cs = c1[0] * c2[0] + c1[1] * c2[1] + c1[2] * c2[2];
             cs = cs / sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]) / sqrt(c2[0] * c2[0] + c2[1] * c2[1] + c2[2] * c2[2]);
-            // plane D: the reference's thread-private clPlane keeps D from an earlier
-            // plane_two_points call; this kernel uses the constructed value 1.0
+            // plane D: cal_plane (CPP_13-boundary.cpp:653) never writes D, so the reference's
+            // OpenMP-private clPlane carries D from that thread's last two-point plane (1, or 0
+            // for a plane through the origin) or its constructor; this kernel uses 1.0, which is
+            // the reference's value unless such a through-origin plane occurred earlier on the thread
             pl[3] = 1.0;
             if (fabs(cs) > 0.999999) {
                 double a = p4[0] - p1[0], b = p4[1] - p1[1], c = p4[2] - p1[2];
