@@ -75,6 +75,7 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
                            int precision) {
     if (!out || !pro || !mat) return SPHG_E_ARG;
     if (capacity <= 0) return SPHG_E_NOPARTICLES;
+    if (capacity > (1 << 28)) return SPHG_E_CAPACITY;  // list entries pack the type above 28 id bits
     if (pro->ndim != 2 && pro->ndim != 3) return SPHG_E_ARG;
     if (precision != SPHG_FP64) return SPHG_E_UNSUPPORTED;
     sphg_ctx *c = new sphg_ctx();
@@ -139,14 +140,8 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
     plan.push_back({(void **)&S.fluid_slot, cap});
     I(S.cell, cap); I(S.sorted, cap); I(S.nb_cnt, cap);
     I(S.nbr, cap * kMaxNb);
-    for (int k = 0; k < 4; k++) {
-        if (k == 2 && pro->ndim == 2) {
-            S.wij[2] = nullptr;  // 2D: the z gradient is identically zero and is not stored
-            continue;
-        }
-        plan.push_back({(void **)&S.wij[k], cap * kMaxNb * 8});
-    }
     for (int k = 0; k < 6; k++) D(S.qsig[k]);
+    D(S.fdens);
     // artificial-stress tensor (kept between steps) and the Jacobi regularisation copy
     for (int k = 0; k < 9; k++) {
         if (pro->art_stre_coe > 0.0) D(S.rij[k]);
@@ -537,6 +532,28 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     }
     HIPCHK(hipMemcpy(c->S.soil_slot, ss.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->S.fluid_slot, fs.data(), n, hipMemcpyHostToDevice));
+    // per-particle fluid density (drag partner), and the uniformity flags the pair passes use
+    std::vector<double> fd(n), hl(n);
+    P.fluid_por_uniform = 1;
+    double por0 = 0.0;
+    bool seen = false;
+    for (int i = 0; i < n; i++) {
+        fd[i] = fl[fs[i]].dens;
+        int t = c->h_type[i];
+        if (t == 1 || t == 3) {
+            double pr = fl[fs[i]].porosity;
+            if (!seen) { por0 = pr; seen = true; }
+            else if (memcmp(&pr, &por0, 8) != 0) P.fluid_por_uniform = 0;
+        }
+    }
+    HIPCHK(hipMemcpy(c->S.fdens, fd.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(hl.data(), c->S.hl, (size_t)n * 8, hipMemcpyDeviceToHost));
+    P.hl_uniform = 1;
+    for (int i = 1; i < n; i++)
+        if (memcmp(&hl[i], &hl[0], 8) != 0) P.hl_uniform = 0;
+    // the pair passes form the kernel normalisation once per particle from a common smoothing
+    // length; setting_par gives every particle hl = dr and the step never changes it
+    if (!P.hl_uniform) return SPHG_E_UNSUPPORTED;
     c->S.ps_c = c->has_psc ? c->d_psc : nullptr;
     int zero = 0;
     HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));
@@ -602,7 +619,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         LAUNCH("strain", k_strain<NDIM>, grid_for(n), c->S, c->P, cur);
         if (c->regu_on) {
             LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 0);
-            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, 0);
+            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, cur, 0);
         }
     }
     if (phase_mask & 8) {
@@ -610,7 +627,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         if (c->regu_on) {
             LAUNCH("stress", ks, grid_for(n), c->S, c->P, 1);
             LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 1);
-            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, 1);
+            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, cur, 1);
             LAUNCH("stress_post", ks, grid_for(n), c->S, c->P, 2);
         } else {
             LAUNCH("stress", ks, grid_for(n), c->S, c->P, 0);
@@ -781,7 +798,7 @@ extern "C" int sphg_get_neighbors(sphg_ctx *c, int *counts, int *ids, int n) {
         HIPCHK(hipMemcpy(col.data(), c->S.nbr + (size_t)k * c->cap, (size_t)n * 4, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; i++) {
             int cnt = std::min(counts[i], kMaxNb);
-            ids[(size_t)i * kMaxNb + k] = k < cnt ? col[i] : -1;
+            ids[(size_t)i * kMaxNb + k] = k < cnt ? (col[i] & kIdMask) : -1;
         }
     }
     return SPHG_OK;

@@ -3,9 +3,9 @@
 // State lives in HBM as structure-of-arrays: one contiguous FP64 array per scalar component
 // (a 3-vector is three arrays), so a wavefront of 64 consecutive particles reads 512 B per
 // component, fully coalesced. Neighbour lists are stored column-major, nbr[k * cap + i], for
-// the same reason. Kernel values W and grad W are computed once per step by the list build (with
-// the reference's own expression) and stored beside the ids in the same column-major layout; the
-// six pair passes read them back coalesced instead of re-deriving them from gathered positions.
+// the same reason, with the neighbour's type packed beside its id. Kernel values are not stored:
+// every pair pass recomputes W and grad W from the gathered positions with the reference's own
+// expression, which moves 4 B per pair and pass instead of 36 B.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -57,6 +57,8 @@ struct Params {
     int flagum, step_regu;
     double art_stre_coe;
     double move_vx[3];
+    int hl_uniform;          // every particle has the same smoothing length (hl_j == hl_i bit for bit)
+    int fluid_por_uniform;   // every fluid particle's slot has the same porosity
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -77,9 +79,10 @@ struct State {
     int8_t *soil_slot, *fluid_slot;
     // cell list
     int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *block_sums;
-    int *nb_cnt, *nbr;
-    double *wij[4];       // kernel gradient (3) and value per list entry, column-major like nbr
-    double *qsig[6];      // sig / rho / rho per particle (the j term of the solid momentum sums)
+    int *nb_cnt, *nbr;    // list entries: id | type << 28, column-major nbr[k * cap + i]
+    double *qsig[6];      // the j factor of the momentum sums: sig / rho / rho (solids),
+                          // (porosity * sig) / rho / rho (fluids)
+    double *fdens;        // density of the particle's fluid slot (drag partner density)
     double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
     double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
     int *err, *step;      // sticky error word; current step number l

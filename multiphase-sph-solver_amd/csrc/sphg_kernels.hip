@@ -48,16 +48,6 @@ __device__ __forceinline__ void load_x(const State &S, int cur, int i, double x[
     x[2] = NDIM == 3 ? S.xp[cur][2][i] : 0.0;
 }
 
-// Kernel values of list entry nj of particle i, written by k_search for this step.
-template <int NDIM>
-__device__ __forceinline__ void load_w(const State &S, const Params &P, int nj, int i, double w[4]) {
-    const size_t o = (size_t)nj * P.cap + i;
-    w[0] = S.wij[0][o];
-    w[1] = S.wij[1][o];
-    w[2] = NDIM == 3 ? S.wij[2][o] : 0.0;
-    w[3] = S.wij[3][o];
-}
-
 template <int NDIM>
 __device__ __forceinline__ void load_xj(const State &S, int cur, int j, double xj[3]) {
     xj[0] = S.xp[cur][0][j];
@@ -69,6 +59,122 @@ __device__ __forceinline__ int nb_count(const State &S, int i) {
     int c = S.nb_cnt[i];
     return c < kMaxNb ? c : kMaxNb;
 }
+
+// List entries (and the cell-ordered ids) carry the particle type in bits 28-30 beside the id,
+// so a pair pass knows which branch a neighbour takes, and which of its fields to gather, in the
+// same load that yields its index.
+constexpr int kTypeShift = 28;
+constexpr int kIdMask = (1 << kTypeShift) - 1;
+__device__ __forceinline__ int nb_id(int e) { return e & kIdMask; }
+__device__ __forceinline__ int nb_type(int e) { return e >> kTypeShift; }
+
+// Walks the column-major list of particle i with the following entry already in flight.
+struct NbCursor {
+    const int *p;
+    size_t stride;
+    int n, k, e;
+    __device__ __forceinline__ NbCursor(const State &S, const Params &P, int i, int nc)
+        : p(S.nbr + i), stride((size_t)P.cap), n(nc), k(0), e(nc > 0 ? S.nbr[i] : 0) {}
+    __device__ __forceinline__ int next() {
+        const int c = e;
+        k++;
+        if (k < n) e = p[(size_t)k * stride];
+        return c;
+    }
+};
+
+// Walks the list of particle i in batches of B entries. The pair passes are latency bound (a
+// wave owns 64 particles and each neighbour costs a dependent gather), so the body of a batch
+// first issues the gathers of all B neighbours and only then consumes them in list order: one
+// memory round trip serves B neighbours, and the summation order is the list order as before.
+// The ids of the next batch are loaded while a batch is processed. Slots past the end of the
+// list repeat the last entry (valid addresses) and are reported as inactive (b >= act).
+template <int B, class F>
+__device__ __forceinline__ void nb_batches(const State &S, const Params &P, int i, int nc, F &&f) {
+    if (nc <= 0) return;
+    const int *lst = S.nbr + i;
+    const size_t cap = (size_t)P.cap;
+    int e[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) e[b] = lst[(size_t)(b < nc ? b : nc - 1) * cap];
+    for (int base = 0; base < nc; base += B) {
+        int en[B];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int k = base + B + b;
+            en[b] = lst[(size_t)(k < nc ? k : nc - 1) * cap];
+        }
+        f(e, nc - base);
+#pragma unroll
+        for (int b = 0; b < B; b++) e[b] = en[b];
+    }
+}
+
+// Kernel value and gradient of a listed pair, recomputed from the positions with the expressions
+// of partisearching and weight() / weight2() (CPP_3-NNPS.cpp:300-314, :17-82): the FP64 values are
+// those of the reference's wij[p] for the same xp. Two rearrangements leave every result bit for
+// bit unchanged and remove divisions, the dominant cost of the pair passes:
+//   - a1 depends on the pair only through ra = (hl_i + hl_j) / 2; every particle carries the same
+//     smoothing length (setting_par gives all of them dr, CPP_2-initialf.cpp:1095-1272, and the
+//     loop never changes it; the host refuses other input), so ra and a1 are formed once per
+//     particle with the same expressions (KernelCtx);
+//   - the two radial branches share their final division by rd: each forms its own numerator with
+//     its own operation order, and one division by rd follows.
+// d2 (the squared distance before the root) and dst are returned for the passes that reuse them.
+struct KernelCtx {
+    double ra, a1;
+};
+
+template <int NDIM>
+__device__ __forceinline__ double kernel_a1(const Params &P, double ra) {
+    if (P.flknl == 1 || P.flknl == 3) return NDIM == 2 ? 0.682092613250980 / ra / ra : 0.477464829275686 / ra / ra / ra;
+    return NDIM == 2 ? 0.106103295394597 / (ra * ra) : 0.077010456334788 / (ra * ra * ra);
+}
+
+template <int NDIM>
+__device__ __forceinline__ KernelCtx kernel_ctx(const Params &P, double hli) {
+    KernelCtx k;
+    k.ra = (hli + hli) / 2;
+    k.a1 = kernel_a1<NDIM>(P, k.ra);
+    return k;
+}
+
+template <int NDIM>
+__device__ __forceinline__ void pair_kernel(const Params &P, const KernelCtx &kc, const double xi[3],
+                                            const double xj[3], double hli, double hlj, double w[4], double &d2,
+                                            double &dst) {
+    double dx[3];
+    d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        d2 = d2 + (xi[k] - xj[k]) * (xi[k] - xj[k]);
+        dx[k] = xi[k] - xj[k];
+    }
+    dst = sqrt(d2);
+    const double ra = kc.ra, a1 = kc.a1;
+    const double s = dst / ra;
+    const bool bspline = P.flknl == 1 || P.flknl == 3;
+    const bool near = s > 0.0 && s <= 1.0, far = s > 1.0 && s <= 2.0;
+    const double t = 2.0 - s, rd = ra * dst;
+    // gradient: weight(): a1*dx*c/rd | a1*(-0.5)*dx*t*t/rd;  weight2(): a1*dx*(-6+3s^2)/rd | a1*(-3)*dx*t*t/rd
+    const double cn = bspline ? (-2 * s + 1.5 * s * s) : (-6.0 + 3.0 * s * s);
+    const double cf = bspline ? -0.5 : -3.0;
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        const double num = near ? a1 * dx[k] * cn : a1 * cf * dx[k] * t * t;
+        w[k] = (near || far) ? num / rd : 0.0;
+    }
+    if (NDIM == 2) w[2] = 0.0;
+    // value: weight(): a1*(0.66666667-s^2+0.5s^3) | a1*t^3/6;  weight2(): a1*(6-6s+s^3) | a1*t^3
+    const double vn = bspline ? a1 * (0.66666667 - s * s + 0.5 * s * s * s) : a1 * (6.0 - 6.0 * s + 1.0 * s * s * s);
+    const double vf = a1 * t * t * t;
+    if (s == 0.0 || near) w[3] = vn;
+    else if (far) w[3] = bspline ? vf / 6.0 : vf;
+    else w[3] = 0.0;
+}
+
+// hl of neighbour j: equal to hl_i for every pair (uniform smoothing length, checked by the host)
+__device__ __forceinline__ double hl_of(const State &, const Params &, int, double hli) { return hli; }
 
 // ------------------------------------------------------------------------------------------
 // NNPS: cell binning (parttocell, CPP_3-NNPS.cpp:175-263)
@@ -174,7 +280,7 @@ __global__ __launch_bounds__(256) void k_scatter(State S, Params P) {
     if (i >= P.n) return;
     int c = S.cell[i];
     int pos = S.cell_start[c] + atomicAdd(&S.cell_cursor[c], 1);
-    S.sorted[pos] = i;
+    S.sorted[pos] = i | (S.type[i] << kTypeShift);
 }
 
 // Within a cell the reference's bottom-up merge sort (ties take the right half, CPP_3-NNPS.cpp:576)
@@ -187,7 +293,7 @@ __global__ __launch_bounds__(256) void k_cell_order(State S, int ncells) {
     int *a = S.sorted + s;
     for (int x = 1; x < cnt; x++) {
         int v = a[x], y = x - 1;
-        while (y >= 0 && a[y] < v) {
+        while (y >= 0 && (a[y] & kIdMask) < (v & kIdMask)) {
             a[y + 1] = a[y];
             y--;
         }
@@ -199,8 +305,10 @@ __global__ __launch_bounds__(256) void k_cell_order(State S, int ncells) {
 // order, predicate sqrt(d2) <= 2*ra && i != j with d2 accumulated in FP64 exactly as the reference.
 // sqrt is monotone and correctly rounded, so candidates whose d2 lies outside a +-1e-14 relative
 // band around (2*ra)^2 are decided without it; only the band (on-cut-off pairs of the jittered
-// lattice) takes the exact sqrt test. Phase A stages the ids of each thread in LDS; phase B walks
-// the lists in lock-step over k so that ids and kernel values are written coalesced, column-major.
+// lattice) takes the exact sqrt test. The first kStage entries of each thread are staged in LDS
+// and written out row by row afterwards, so the list rows are stored coalesced. Kernel values are
+// not stored: each pair pass recomputes them from the positions (pair_kernel), which costs a
+// square root and a few divisions per pair instead of 32 B of HBM traffic per pair and pass.
 constexpr int kSearchBlock = 256, kStage = 32;
 
 template <int NDIM>
@@ -213,6 +321,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     if (S.type[i] != 7) {
         const int c = S.cell[i];
         for (int s = 0; s < P.nstencil; s++) {
@@ -220,22 +329,23 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
             if (nl <= 0 || nl > P.ncell || c == 0) continue;
             int st = S.cell_start[nl], en = st + S.cell_count[nl];
             for (int q = st; q < en; q++) {
-                int j = S.sorted[q];
+                const int e = S.sorted[q];
+                const int j = nb_id(e);
                 double d2 = 0.0;
 #pragma unroll
                 for (int k = 0; k < NDIM; k++) {
                     double d = xi[k] - S.xp[cur][k][j];
                     d2 = d2 + d * d;
                 }
-                double R = 2 * ((hli + S.hl[j]) / 2);
+                double R = 2 * ((hli + hl_of(S, P, j, hli)) / 2);
                 double R2 = R * R;
                 bool in;
                 if (d2 < R2 * 0.99999999999999) in = true;
                 else if (d2 > R2 * 1.00000000000001) in = false;
                 else in = sqrt(d2) <= R;
                 if (in && i != j) {
-                    if (cnt < kStage) stage[cnt][t] = j;
-                    else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = j;
+                    if (cnt < kStage) stage[cnt][t] = e;
+                    else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = e;
                     cnt++;
                 }
             }
@@ -243,56 +353,42 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
         if (cnt > kMaxNb) set_err(S, SPHG_ERR_NB);
     }
     S.nb_cnt[i] = cnt;
-    const int nk = cnt < kMaxNb ? cnt : kMaxNb;
-    for (int k = 0; k < nk; k++) {
-        const size_t o = (size_t)k * P.cap + i;
-        int j;
-        if (k < kStage) {
-            j = stage[k][t];
-            S.nbr[o] = j;
-        } else {
-            j = S.nbr[o];
-        }
-        double xj[3], dx[3], w[4];
-        xj[0] = S.xp[cur][0][j];
-        xj[1] = S.xp[cur][1][j];
-        xj[2] = NDIM == 3 ? S.xp[cur][2][j] : 0.0;
-        double dst = 0.0;
-#pragma unroll
-        for (int kk = 0; kk < NDIM; kk++) {
-            dst = dst + (xi[kk] - xj[kk]) * (xi[kk] - xj[kk]);
-            dx[kk] = xi[kk] - xj[kk];
-        }
-        if (NDIM == 2) dx[2] = 0.0;
-        dst = sqrt(dst);
-        double ra = (hli + S.hl[j]) / 2;
-        w[0] = w[1] = w[2] = w[3] = 0.0;
-        if (P.flknl == 1 || P.flknl == 3) weight_b(dst, ra, dx, w, NDIM);
-        else weight_y(dst, ra, dx, w, NDIM);
-        S.wij[0][o] = w[0];
-        S.wij[1][o] = w[1];
-        if (NDIM == 3) S.wij[2][o] = w[2];
-        S.wij[3][o] = w[3];
-    }
+    const int nk = cnt < kStage ? cnt : kStage;
+    for (int k = 0; k < nk; k++) S.nbr[(size_t)k * P.cap + i] = stage[k][t];
 }
 
-// Virtual velocity of boundary neighbour j seen by particle i of type ti (CPP_6-density.cpp:86-120).
-__device__ __forceinline__ void bndy_vel(const State &S, const Params &P, int ti, int j, const double own[3],
-                                         double v[3]) {
-    v[0] = v[1] = v[2] = 0.0;
-    int bt = P.flbndy;
-    if (bt == 1) {
-        v[0] = own[0]; v[1] = own[1]; v[2] = own[2];
-    } else if (bt == 2 || bt == 3 || bt == 4 || bt == 13 || bt == 14) {
-        if (ti >= 1 && ti <= 4) {
-            int ph = ti - 1;
-            v[0] = S.vb[ph][0][j]; v[1] = S.vb[ph][1][j]; v[2] = S.vb[ph][2][j];
-        }
+// Source of the velocity that a boundary neighbour shows to a particle of phase ti
+// (CPP_6-density.cpp:86-120 and its copies in the strain pass): flbndy 1 the particle's own
+// velocity (mode 1), flbndy 2/3/4/13/14 the boundary's per-phase virtual velocity (mode 2),
+// otherwise zero (mode 0).
+__device__ __forceinline__ int bvel_mode(const Params &P, int ti) {
+    if (P.flbndy == 1) return 1;
+    const int bt = P.flbndy;
+    if ((bt == 2 || bt == 3 || bt == 4 || bt == 13 || bt == 14) && ti >= 1 && ti <= 4) return 2;
+    return 0;
+}
+
+// Velocity of neighbour j (of type tj) as seen by particle i: own-phase neighbours their vxp,
+// boundary neighbours the mode above. The gather address is chosen per lane, so one load serves
+// both kinds of neighbour.
+template <int NDIM>
+__device__ __forceinline__ void nb_vel(const State &S, int j, bool bnd, int mode, int ph, const double vi[3],
+                                       double vj[3]) {
+    const bool virt = bnd && mode == 2;
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        const double *src = virt ? S.vb[ph][k] : S.vxp[k];
+        vj[k] = src[j];
     }
+    if (bnd && mode == 1)
+        for (int k = 0; k < NDIM; k++) vj[k] = vi[k];
+    else if (bnd && mode == 0)
+        for (int k = 0; k < NDIM; k++) vj[k] = 0.0;
 }
 
 // ------------------------------------------------------------------------------------------
-// density loops 1-2 (CPP_6-density.cpp:22-166)
+// density loops 1-2 (CPP_6-density.cpp:22-166); water porosity (loop 1) and continuity (loop 2)
+// accumulate side by side in one walk over the list.
 // ------------------------------------------------------------------------------------------
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
@@ -300,31 +396,6 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     if (i >= P.n) return;
     const int ti = S.type[i];
     const int l = step_of(S);
-    double xi[3], xj[3], dx[3], w[4];
-    load_x<NDIM>(S, cur, i, xi);
-    const double hli = S.hl[i];
-    const int nc = nb_count(S, i);
-    double por = S.porosity[i], pcop = S.poro_cop[i];
-    if (ti == 1) {  // loop 1: water porosity from soil neighbours
-        double up = 0.0, dn = 0.0;
-        for (int nj = 0; nj < nc; nj++) {
-            int j = S.nbr[(size_t)nj * P.cap + i];
-            if (S.type[j] != 2) continue;
-            load_w<NDIM>(S, P, nj, i, w);
-            double mj = S.mass[j], rj = S.rho[j];
-            up = up + mj * S.porosity[j] / rj * w[3];
-            dn = dn + mj / rj * w[3];
-        }
-        if (fabs(dn) > 1.0e-7) {
-            pcop = por;
-            por = up / dn;
-            if (l == 1) pcop = por;
-        } else {
-            pcop = por;
-        }
-        S.porosity[i] = por;
-        S.poro_cop[i] = pcop;
-    }
     if (ti == 7) {
         S.rho_mid[i] = S.rho[i];
         return;
@@ -334,29 +405,64 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
         S.rho_mid[i] = S.rhop[i] + drhot * P.dt;
         return;
     }
-    double vi[3] = {S.vxp[0][i], S.vxp[1][i], NDIM == 3 ? S.vxp[2][i] : 0.0};
-    if (NDIM == 2) vi[2] = S.vxp[2][i];
-    double up[3] = {0.0, 0.0, 0.0}, dn[3] = {0.0, 0.0, 0.0};
-    const bool solid = (ti == 2 || ti == 4);
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        int tj = S.type[j];
-        double vj[3];
-        if (tj == ti) {
-            vj[0] = S.vxp[0][j]; vj[1] = S.vxp[1][j]; vj[2] = NDIM == 3 ? S.vxp[2][j] : 0.0;
-        } else if (tj == 0) {
-            bndy_vel(S, P, ti, j, vi, vj);
-        } else {
-            continue;
-        }
-        load_w<NDIM>(S, P, nj, i, w);
-        if (solid) load_xj<NDIM>(S, cur, j, xj);
-        double mj = S.mass[j];
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const int nc = nb_count(S, i);
+    const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    const bool solid = (ti == 2 || ti == 4), water = ti == 1;
+    const int mode = bvel_mode(P, ti), ph = ti - 1;
+    double up[3] = {0.0, 0.0, 0.0}, dn[3] = {0.0, 0.0, 0.0}, pup = 0.0, pdn = 0.0;
+    constexpr int B = 4;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xj[B][3], vj[B][3], hlj[B], mj[B], porj[B], rj[B];
 #pragma unroll
-        for (int k = 0; k < NDIM; k++) {
-            up[k] = up[k] + mj * (vi[k] - vj[k]) * w[k];
-            if (solid) dn[k] = dn[k] + mj * (xi[k] - xj[k]) * w[k];
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xj[b]);
+            hlj[b] = hl_of(S, P, j, hli);
+            mj[b] = S.mass[j];
+            nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
+            porj[b] = 0.0;
+            rj[b] = 1.0;
+            if (water) {
+                porj[b] = S.porosity[j];
+                rj[b] = S.rho[j];
+            }
         }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b >= act) continue;
+            const int tj = nb_type(e[b]);
+            const bool same = tj == ti, bnd = tj == 0;
+            if (!(same || bnd || (water && tj == 2))) continue;
+            double w[4], d2, dst;
+            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            if (water && tj == 2) {  // loop 1: water porosity from soil neighbours
+                pup = pup + mj[b] * porj[b] / rj[b] * w[3];
+                pdn = pdn + mj[b] / rj[b] * w[3];
+            }
+            if (same || bnd) {  // loop 2: continuity
+#pragma unroll
+                for (int k = 0; k < NDIM; k++) {
+                    up[k] = up[k] + mj[b] * (vi[k] - vj[b][k]) * w[k];
+                    if (solid) dn[k] = dn[k] + mj[b] * (xi[k] - xj[b][k]) * w[k];
+                }
+            }
+        }
+    });
+    double por = S.porosity[i], pcop = S.poro_cop[i];
+    if (water) {
+        if (fabs(pdn) > 1.0e-7) {
+            pcop = por;
+            por = pup / pdn;
+            if (l == 1) pcop = por;
+        } else {
+            pcop = por;
+        }
+        S.porosity[i] = por;
+        S.poro_cop[i] = pcop;
     }
     double drhot = 0.0;
     if (solid) {
@@ -381,19 +487,32 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
-    double xi[3], xj[3], dx[3], w[4];
+    double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     double up = 0.0, dn = 0.0;
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        if (S.type[j] != ti) continue;
-        load_w<NDIM>(S, P, nj, i, w);
-        double mj = S.mass[j];
-        up = up + mj * w[3];
-        dn = dn + mj / S.rho_mid[j] * w[3];
-    }
+    constexpr int B = 8;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xj[B][3], hlj[B], mj[B], rmj[B];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xj[b]);
+            hlj[b] = hl_of(S, P, j, hli);
+            mj[b] = S.mass[j];
+            rmj[b] = S.rho_mid[j];
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b >= act || nb_type(e[b]) != ti) continue;
+            double w[4], d2, dst;
+            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            up = up + mj[b] * w[3];
+            dn = dn + mj[b] / rmj[b] * w[3];
+        }
+    });
     double r = S.rho_mid[i];
     if (ti != 7 && fabs(dn) > 1.0e-7) r = up / dn;
     S.rho[i] = r;
@@ -425,19 +544,34 @@ template <int NDIM>
 __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n || S.type[i] != 2) return;
-    double w[4];
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     const SoilSlot &sl = S.soil[S.soil_slot[i]];
     const double sat_cof = sl.sitas - sl.sitar;
     double sat_up = 0.0, sat_dn = 0.0;
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        if (S.type[j] != 1) continue;
-        load_w<NDIM>(S, P, nj, i, w);
-        double mj = S.mass[j], rj = S.rho[j];
-        sat_up = sat_up + mj / rj * sat_cof * w[3];
-        sat_dn = sat_dn + mj / rj * w[3];
-    }
+    constexpr int B = 8;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xj[B][3], hlj[B], mj[B], rj[B];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xj[b]);
+            hlj[b] = hl_of(S, P, j, hli);
+            mj[b] = S.mass[j];
+            rj[b] = S.rho[j];
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b >= act || nb_type(e[b]) != 1) continue;
+            double w[4], d2, dst;
+            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            sat_up = sat_up + mj[b] / rj[b] * sat_cof * w[3];
+            sat_dn = sat_dn + mj[b] / rj[b] * w[3];
+        }
+    });
     // saturation update and moisturesuct (CPP_7-1-strain.cpp:252-278, CPP_7-2-constitutive.cpp:324-338)
     {
         const double sitas = sl.sitas, sitar = sl.sitar;
@@ -465,89 +599,127 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     if (i >= P.n) return;
     const int ti = S.type[i];
     const int l = step_of(S);
-    double xi[3], xj[3], dx[3], w[4];
+    double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     if (l == P.inip)
         for (int k = 0; k < 6; k++) S.eps[k][i] = 0.0;
-    double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
     double up[9], dnm[3] = {0.0, 0.0, 0.0};
+    #pragma unroll
     for (int k = 0; k < 9; k++) up[k] = 0.0;
-    double u1 = 0, u2 = 0, u3 = 0, u4 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, us[6] = {0, 0, 0, 0, 0, 0}, ds = 0;
+    // eos loop 2 sums: water (uw, dw) and air (ua, da) pressure of the particle's fluid neighbours,
+    // soil stress (us, ds) of a boundary particle's soil neighbours
+    double uw = 0, dw = 0, ua = 0, da = 0, us[6] = {0, 0, 0, 0, 0, 0}, ds = 0;
     double color = 0.0;
     const bool solid = (ti == 2 || ti == 4);
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        int tj = S.type[j];
-        load_w<NDIM>(S, P, nj, i, w);
-        if (solid) load_xj<NDIM>(S, cur, j, xj);
-        double mj = S.mass[j], rj = S.rho[j];
-        // strain (CPP_7-1-strain.cpp:67-150)
-        bool act = false;
-        double dvx[3];
-        if (ti == tj && ti != 7 && ti != 0) {
-            dvx[0] = vi[0] - S.vxp[0][j];
-            dvx[1] = vi[1] - S.vxp[1][j];
-            dvx[2] = vi[2] - S.vxp[2][j];
-            act = true;
-        } else if (ti >= 1 && ti <= 4 && tj == 0) {
-            double vj[3];
-            bndy_vel(S, P, ti, j, vi, vj);
-            dvx[0] = vi[0] - vj[0];
-            dvx[1] = vi[1] - vj[1];
-            dvx[2] = vi[2] - vj[2];
-            act = true;
-        }
-        if (act) {
+    const int mode = bvel_mode(P, ti), ph = ti - 1;
+    if (ti == 0) {
+        // boundary particles: eos loop 2 only (CPP_7-1-eos_CR.cpp:62-118): water / air pressure and
+        // soil stress interpolated onto the wall
+        constexpr int B = 4;
+        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+            double xj[B][3], hlj[B], mj[B], rj[B], pj[B], sgj[B][6];
 #pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) up[3 * a + b] = up[3 * a + b] + mj / rj * dvx[a] * w[b];
-            if (solid) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) dnm[k] = dnm[k] + mj / rj * (xi[k] - xj[k]) * w[k];
+            for (int b = 0; b < B; b++) {
+                const int j = nb_id(e[b]);
+                load_xj<NDIM>(S, cur, j, xj[b]);
+                hlj[b] = hl_of(S, P, j, hli);
+                mj[b] = S.mass[j];
+                rj[b] = S.rho[j];
+                pj[b] = S.pre[j];
+                #pragma unroll
+                for (int k = 0; k < 6; k++) sgj[b][k] = S.sig[k][j];
             }
-        }
-        // eos loop 2 (CPP_7-1-eos_CR.cpp:62-118)
-        if (ti != tj) {
-            double w3 = w[3];
-            if (tj == 1 && (ti == 2 || ti == 0)) {
-                double pj = S.pre[j];
-                u1 = u1 + mj * pj * w3 / rj;
-                d1 = d1 + mj * w3 / rj;
-            } else if (tj == 3 && (ti == 2 || ti == 0)) {
-                double pj = S.pre[j];
-                u2 = u2 + mj * pj * w3 / rj;
-                d2 = d2 + mj * w3 / rj;
-            } else if (tj == 1 && ti == 4) {
-                double pj = S.pre[j];
-                u3 = u3 + mj * pj * w3 / rj;
-                d3 = d3 + mj * w3 / rj;
-            } else if (tj == 3 && ti == 4) {
-                double pj = S.pre[j];
-                u4 = u4 + mj * pj * w3 / rj;
-                d4 = d4 + mj * w3 / rj;
-            } else if (ti == 0 && tj == 2) {
 #pragma unroll
-                for (int k = 0; k < 6; k++) us[k] = us[k] + mj * S.sig[k][j] * w3 / rj;
-                ds = ds + mj * w3 / rj;
+            for (int b = 0; b < B; b++) {
+                const int tj = nb_type(e[b]);
+                if (b >= act || (tj != 1 && tj != 2 && tj != 3)) continue;
+                double w[4], d2, dst;
+                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                const double w3 = w[3];
+                if (tj == 1 || tj == 3) {
+                    const double tu = mj[b] * pj[b] * w3 / rj[b], td = mj[b] * w3 / rj[b];
+                    if (tj == 1) {
+                        uw = uw + tu;
+                        dw = dw + td;
+                    } else {
+                        ua = ua + tu;
+                        da = da + td;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 6; k++) us[k] = us[k] + mj[b] * sgj[b][k] * w3 / rj[b];
+                    ds = ds + mj[b] * w3 / rj[b];
+                }
             }
-        }
-        // colour (CPP_7-2-constitutive.cpp:467-478)
-        if (ti == tj && ti != 0 && ti != 7) color = color + mj / rj * 1.0 * w[3];
-        else if (ti != tj && ti > 0 && ti < 5 && tj == 0) color = color + mj / rj * 1.0 * w[3];
+        });
+    } else if (ti != 7) {
+        constexpr int B = 4;
+        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+            double xj[B][3], vj[B][3], hlj[B], mj[B], rj[B], pj[B];
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const int j = nb_id(e[b]);
+                load_xj<NDIM>(S, cur, j, xj[b]);
+                hlj[b] = hl_of(S, P, j, hli);
+                mj[b] = S.mass[j];
+                rj[b] = S.rho[j];
+                nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
+                pj[b] = solid ? S.pre[j] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                if (b >= act) continue;
+                const int tj = nb_type(e[b]);
+                const bool bnd = tj == 0;
+                double w[4], d2, dst;
+                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                // strain (CPP_7-1-strain.cpp:67-150)
+                if (ti == tj || (bnd && ti <= 4)) {
+                    double dvx[3];
+                    dvx[0] = vi[0] - vj[b][0];
+                    dvx[1] = vi[1] - vj[b][1];
+                    dvx[2] = NDIM == 3 ? vi[2] - vj[b][2] : 0.0;
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) up[3 * a + c] = up[3 * a + c] + mj[b] / rj[b] * dvx[a] * w[c];
+                    if (solid) {
+#pragma unroll
+                        for (int k = 0; k < 3; k++) dnm[k] = dnm[k] + mj[b] / rj[b] * (xi[k] - xj[b][k]) * w[k];
+                    }
+                }
+                // eos loop 2 (CPP_7-1-eos_CR.cpp:62-118): soil and structure particles
+                if (solid && (tj == 1 || tj == 3)) {
+                    const double w3 = w[3];
+                    const double tu = mj[b] * pj[b] * w3 / rj[b], td = mj[b] * w3 / rj[b];
+                    if (tj == 1) {
+                        uw = uw + tu;
+                        dw = dw + td;
+                    } else {
+                        ua = ua + tu;
+                        da = da + td;
+                    }
+                }
+                // colour (CPP_7-2-constitutive.cpp:467-478)
+                if (ti == tj || (bnd && ti <= 4)) color = color + mj[b] / rj[b] * 1.0 * w[3];
+            }
+        });
     }
     // velocity gradient -> strain rate and spin (CPP_7-1-strain.cpp:153-219)
     double divx[9];
     if (ti == 1 || ti == 3) {
+        #pragma unroll
         for (int k = 0; k < 9; k++) divx[k] = up[k];
     } else {
+        #pragma unroll
         for (int k = 0; k < 9; k++) divx[k] = 0.0;
 #pragma unroll
         for (int b = 0; b < 3; b++)
             if (fabs(dnm[b]) > 1.0e-7)
-#pragma unroll
                 for (int a = 0; a < 3; a++) divx[3 * a + b] = up[3 * a + b] / dnm[b];
     }
     double d0 = divx[0], d1_ = divx[4], d2_ = divx[8];
@@ -561,13 +733,13 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     S.weps[0][i] = w01; S.weps[1][i] = w02; S.weps[2][i] = w12;
     // eos loop 2 results (CPP_7-1-eos_CR.cpp:121-176)
     if (ti == 2 || ti == 0) {
-        S.prew[i] = d1 != 0.0 ? u1 / d1 : u1;
-        S.prea[i] = d2 != 0.0 ? u2 / d2 : u2;
+        S.prew[i] = dw != 0.0 ? uw / dw : uw;
+        S.prea[i] = da != 0.0 ? ua / da : ua;
         if (ti == 0)
             for (int k = 0; k < 6; k++) S.sig[k][i] = ds != 0.0 ? us[k] / ds : us[k];
     } else if (ti == 4) {
-        S.prew[i] = d3 != 0.0 ? u3 / d3 : u3;
-        S.prea[i] = d4 != 0.0 ? u4 / d4 : u4;
+        S.prew[i] = dw != 0.0 ? uw / dw : uw;
+        S.prea[i] = da != 0.0 ? ua / da : ua;
     }
     S.color[i] = color;
     // cal_permeability (CPP_7-2-constitutive.cpp:355-382), fli == 1 only
@@ -771,6 +943,7 @@ __device__ void focc_tangent(const double sigin[6], const double deps[6], double
     double s_val = 0.0;
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++)
+            #pragma unroll
             for (int k = 0; k < 3; k++) s_val = s_val + smn[a][b] * smn[b][k] * smn[k][a];
     double skl = 0.0;
     for (int a = 0; a < 3; a++)
@@ -867,13 +1040,16 @@ __device__ __forceinline__ void dp_update(const double sig[6], const double deps
     double ex = sig[0] - smc, ey = sig[1] - smc, ez = sig[2] - smc;
     double sjc = sqrt(0.5 * (ex * ex + ey * ey + ez * ez) + sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5]);
     double ts[6];
+    #pragma unroll
     for (int k = 0; k < 6; k++) ts[k] = sig[k] + dsig[k] * dt;
     double i1 = ts[0] + ts[1] + ts[2];
     double rn = (-yalf * i1 + ykap) / sjc;
     if (rn < 1.0) {
+        #pragma unroll
         for (int k = 0; k < 3; k++) ts[k] = rn * (ts[k] - i1 / 3.0) + i1 / 3.0;
         for (int k = 3; k < 6; k++) ts[k] = rn * ts[k];
     }
+    #pragma unroll
     for (int k = 0; k < 6; k++) dsig[k] = (ts[k] - sig[k]) / dt;
 }
 
@@ -942,6 +1118,7 @@ __device__ __forceinline__ void camclay_update(const double sig[6], const double
     tangent_apply(dsig, De, plastic, d, wv, zv, deps);
     if (plastic) {
         double fv[6] = {f11, f22, f33, f12, f23, f31};
+        #pragma unroll
         for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
     }
 }
@@ -977,8 +1154,10 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
             }
         }
         double dd[6];
+        #pragma unroll
         for (int k = 0; k < 3; k++) dd[k] = 2.0 * deps[k] - 2.0 * (deps[0] + deps[1] + deps[2]) / 3.0;
         for (int k = 3; k < 6; k++) dd[k] = deps[k];
+        #pragma unroll
         for (int k = 0; k < 6; k++) sig[k] = k < 3 ? -pre - vis * dd[k] : -vis * dd[k];
     } else if (ti == 2 || ti == 4) {
         double dsig[6] = {0, 0, 0, 0, 0, 0}, adps[6] = {0, 0, 0, 0, 0, 0}, erfs[6] = {0, 0, 0, 0, 0, 0};
@@ -1045,12 +1224,15 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
             }
         }
         double eps[6];
+        #pragma unroll
         for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
+        #pragma unroll
         for (int k = 0; k < 6; k++) {
             if (ti == 2) S.adps[k][i] = S.adps[k][i] + adps[k] * dt;
             eps[k] = eps[k] + deps[k] * dt;
             sig[k] = sig[k] + dsig[k] * dt + erfs[k];
         }
+        #pragma unroll
         for (int k = 0; k < 6; k++) S.eps[k][i] = eps[k];
         double vsig, veps, divq, divr;
         deq6(sig, eps, vsig, veps, divq, divr);
@@ -1078,18 +1260,22 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
     const double color = S.color[i], tmin = P.tmin, tmax = P.tmax, incl = tmax - tmin;
     if (ti == 1) {
         if (color < tmin / 2.0) {
+            #pragma unroll
             for (int k = 0; k < 3; k++) sig[k] = -1.0;
             for (int k = 3; k < 6; k++) sig[k] = 0.0;
         } else if (color >= tmin / 2.0 && color < tmax / 2.0) {
             double xs = (2.0 * color - tmin) / incl;
+            #pragma unroll
             for (int k = 0; k < 6; k++) sig[k] = xs * sig[k];
         }
     } else if (ti == 2) {
         if (color < tmin) {
+            #pragma unroll
             for (int k = 0; k < 3; k++) sig[k] = -1.0;
             for (int k = 3; k < 6; k++) sig[k] = 0.0;
         } else if (color >= tmin && color < tmax) {
             double xs = (color - tmin) / incl * 1.0;
+            #pragma unroll
             for (int k = 0; k < 6; k++) sig[k] = xs * sig[k];
         }
         double vsig = S.vsig[i];
@@ -1101,6 +1287,7 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
             sig[3] = sig[4] = sig[5] = 0.0;
         }
         double eps[6], veps, divq, divr;
+        #pragma unroll
         for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
         deq6(sig, eps, vsig, veps, divq, divr);
         S.vsig[i] = vsig;
@@ -1109,16 +1296,27 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
         S.divr[i] = divr;
     }
     if (ti != 0 && ti != 7) {
+        #pragma unroll
         for (int k = 0; k < 6; k++) S.sig[k][i] = sig[k];
     }
-    // sig / rho / rho, the per-neighbour factor of the solid momentum sums (same operation order)
+    // the j factor of the momentum sums, in the reference's operation order: sig / rho / rho for
+    // solids, (porosity * sig) / rho / rho for fluids when all fluids share the porosity the
+    // fluid sum multiplies by (the i particle's; otherwise the fluid pass forms it per pair)
     {
         const double r = S.rho[i];
-        for (int k = 0; k < 6; k++) S.qsig[k][i] = sig[k] / r / r;
+        if ((ti == 1 || ti == 3) && P.fluid_por_uniform) {
+            const double fp = S.fluid[S.fluid_slot[i]].porosity;
+            #pragma unroll
+            for (int k = 0; k < 6; k++) S.qsig[k][i] = (fp * sig[k]) / r / r;
+        } else {
+            #pragma unroll
+            for (int k = 0; k < 6; k++) S.qsig[k][i] = sig[k] / r / r;
+        }
     }
     // principal stresses (CPP_7-1-strain.cpp:283-332)
     double a[9] = {sig[0], sig[3], sig[5], sig[3], sig[1], sig[4], sig[5], sig[4], sig[2]}, v[9];
     eejcb3(a, v, 0.001, 7);
+    #pragma unroll
     for (int k = 0; k < 3; k++) {
         S.strep[k][i] = a[k * 3 + k];
         for (int c = 0; c < 3; c++) S.beta[c * 3 + k][i] = v[k * 3 + c];  // beta[c][k] = v[3k+c]
@@ -1147,6 +1345,7 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
             R[1][0] = R[0][1];
         } else {
             double rr[3], B[3][3], Bt[3][3], bi[3][3], bti[3][3], tmp[3][3];
+            #pragma unroll
             for (int k = 0; k < 3; k++) rr[k] = a[k * 3 + k] > 0.0 ? -xs * a[k * 3 + k] / rho / rho : 0.0;
             for (int r = 0; r < 3; r++)
                 for (int q = 0; q < 3; q++) {
@@ -1191,9 +1390,11 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
     if (i >= P.n) return;
     const int ti = S.type[i];
     double sig[6];
+    #pragma unroll
     for (int k = 0; k < 6; k++) sig[k] = S.sig[k][i];
     if (part != 2) {
         double deps[6];
+        #pragma unroll
         for (int k = 0; k < 6; k++) deps[k] = S.deps[k][i];
         constitutive<kFull>(S, P, i, ti, step_of(S), sig, deps);
         if (part == 1) {
@@ -1210,24 +1411,44 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
 __global__ __launch_bounds__(256) void k_regu_copy(State S, Params P, int which_sig) {
     int i = pidx();
     if (i >= P.n || (step_of(S) % P.step_regu) != 0) return;
+    #pragma unroll
     for (int k = 0; k < 6; k++) S.tmp6[k][i] = which_sig ? S.sig[k][i] : S.eps[k][i];
 }
 
 template <int NDIM>
-__global__ __launch_bounds__(256) void k_regu(State S, Params P, int which_sig) {
+__global__ __launch_bounds__(256) void k_regu(State S, Params P, int cur, int which_sig) {
     int i = pidx();
     if (i >= P.n || (step_of(S) % P.step_regu) != 0) return;
     const int ti = S.type[i];
     const int nc = nb_count(S, i);
-    double up[6] = {0, 0, 0, 0, 0, 0}, dn = 0.0, w[4];
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        if (S.type[j] != ti) continue;
-        load_w<NDIM>(S, P, nj, i, w);
-        double mj = S.mass[j], rj = S.rho[j];
-        for (int k = 0; k < 6; k++) up[k] = up[k] + mj / rj * w[3] * S.tmp6[k][j];
-        dn = dn + mj / rj * w[3];
-    }
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    double up[6] = {0, 0, 0, 0, 0, 0}, dn = 0.0;
+    constexpr int B = 4;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xj[B][3], hlj[B], mj[B], rj[B], vj[B][6];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xj[b]);
+            hlj[b] = hl_of(S, P, j, hli);
+            mj[b] = S.mass[j];
+            rj[b] = S.rho[j];
+            #pragma unroll
+            for (int k = 0; k < 6; k++) vj[b][k] = S.tmp6[k][j];
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b >= act || nb_type(e[b]) != ti) continue;
+            double w[4], d2, dst;
+            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            #pragma unroll
+            for (int k = 0; k < 6; k++) up[k] = up[k] + mj[b] / rj[b] * w[3] * vj[b][k];
+            dn = dn + mj[b] / rj[b] * w[3];
+        }
+    });
     if (fabs(dn) > 1.0e-6)
         for (int k = 0; k < 6; k++) (which_sig ? S.sig : S.eps)[k][i] = up[k] / dn;
 }
@@ -1235,7 +1456,47 @@ __global__ __launch_bounds__(256) void k_regu(State S, Params P, int which_sig) 
 // ------------------------------------------------------------------------------------------
 // accelerations, artificial viscosity, boundary stress, Darcy drag, velocity update
 // ------------------------------------------------------------------------------------------
+
 __device__ __constant__ int kSidx[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+
+// artificial_viscosity pair term (CPP_9-3-artificialvis.cpp:36-89); dst is sqrt of the same
+// squared distance the kernel value was built from.
+template <int NDIM>
+__device__ __forceinline__ void av_pair(const double vi[3], const double vj[3], const double xi[3],
+                                        const double xj[3], double dst, double hli, double hlj, double rhoi,
+                                        double rj, double mj, double av_a, double av_b, double av_c,
+                                        const double w[4], double av[3]) {
+    double inprod = 0.0, dv[3], dxx[3];
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        dv[k] = vi[k] - vj[k];
+        dxx[k] = xi[k] - xj[k];
+        inprod = inprod + dv[k] * dxx[k];
+    }
+    const double hl = (hlj + hli) * 0.5;
+    const double rou = (rj + rhoi) * 0.5;
+    double paix;
+    if (inprod < 0.0) {
+        double myu = hl * inprod / (dst * dst + 0.01 * hl * hl);
+        paix = (-av_a * av_c * myu + av_b * myu * myu) / rou;
+    } else {
+        paix = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) av[k] = av[k] + mj * paix * w[k];
+}
+
+// boundary_moving_stress_effect pair term (CPP_9-4-derivation_stress.cpp:631-721); bi = as / rho_i^2
+__device__ __forceinline__ void bnd_pair(const double as[6], const double bi[6], double mj, double rj,
+                                         const double w[4], double bnd_t[3][3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const int sx = kSidx[a][b];
+            bnd_t[a][b] = bnd_t[a][b] + mj * (as[sx] / rj / rj + bi[sx]) * w[b];
+        }
+}
 
 // Pair sums that the momentum kernel adds in the reference's order when they are switched on:
 // artificial stress (soil, same-type pairs) into tmp6[0..2] and the geostress damping sum
@@ -1249,36 +1510,46 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
     const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
     if (!art_on && !damp_on) return;
-    double xi[3], xj[3], w[4];
+    double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
     double Ri[9], tu_art[3] = {0, 0, 0}, damp[3] = {0, 0, 0};
     if (art_on)
         for (int k = 0; k < 9; k++) Ri[k] = S.rij[k][i];
     const double cdi = damp_on ? S.cd[i] : 0.0;
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        int tj = S.type[j];
-        if (tj != ti) continue;
-        load_w<NDIM>(S, P, nj, i, w);
-        double mj = S.mass[j], rj = S.rho[j];
-        if (art_on) {
-            load_xj<NDIM>(S, cur, j, xj);
-            // artificial_stress_2d / _3d pair sum (CPP_9-5-artificialstress.cpp:92-140, :230-317)
-            double dst = 0.0;
+    constexpr int B = 2;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+    double xjb[B][3], hljb[B], mjb[B], rjb[B], Rjb[B][9], vjb[B][3];
 #pragma unroll
-            for (int k = 0; k < NDIM; k++) dst = dst + (xi[k] - xj[k]) * (xi[k] - xj[k]);
-            dst = sqrt(dst);
-            double hl = (hli + S.hl[j]) * 0.5;
+    for (int b = 0; b < B; b++) {
+        const int j = nb_id(e[b]);
+        load_xj<NDIM>(S, cur, j, xjb[b]);
+        hljb[b] = hl_of(S, P, j, hli);
+        mjb[b] = S.mass[j];
+        rjb[b] = S.rho[j];
+        if (art_on)
+            for (int k = 0; k < 9; k++) Rjb[b][k] = S.rij[k][j];
+        if (damp_on)
+            for (int k = 0; k < NDIM; k++) vjb[b][k] = S.vxp[k][j];
+    }
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+        if (b >= act || nb_type(e[b]) != ti) continue;
+        const double *xj = xjb[b], *Rj = Rjb[b], *vj = vjb[b];
+        const double hlj = hljb[b], mj = mjb[b], rj = rjb[b];
+        double w[4], d2, dst;
+        pair_kernel<NDIM>(P, kc, xi, xj, hli, hlj, w, d2, dst);
+        if (art_on) {
+            // artificial_stress_2d / _3d pair sum (CPP_9-5-artificialstress.cpp:92-140, :230-317);
+            // the distance is the square root of the same sum the kernel value used
+            double hl = (hli + hlj) * 0.5;
             double q = dst / hl, f;
             if (q >= 0.0 && q < 1.0) f = pow((0.666666666666667 - q * q + 0.5 * q * q * q) * 3.82300884955752, 2.55);
             else if (q >= 1.0 && q < 2.0) f = pow((2.0 - q) * (2.0 - q) * (2.0 - q) * 0.629737609329447, 2.55);
             else f = 0.0;
-            double Rj[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) Rj[k] = S.rij[k][j];
             if (NDIM == 2) {
                 tu_art[0] = tu_art[0] + mj * f * (Ri[0] + Rj[0]) * w[0] + mj * f * (Ri[1] + Rj[1]) * w[1];
                 tu_art[1] = tu_art[1] + mj * f * (Ri[3] + Rj[3]) * w[0] + mj * f * (Ri[4] + Rj[4]) * w[1];
@@ -1293,9 +1564,10 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
         if (damp_on) {
             // soil_damping_geostress pair sum (CPP_9-4-derivation_stress.cpp:858-872)
 #pragma unroll
-            for (int k = 0; k < NDIM; k++) damp[k] = damp[k] + mj / rj * cdi * (vi[k] - S.vxp[k][j]) * w[3];
+            for (int k = 0; k < NDIM; k++) damp[k] = damp[k] + mj / rj * cdi * (vi[k] - vj[k]) * w[3];
         }
     }
+    });
     if (art_on)
         for (int k = 0; k < 3; k++) S.tmp6[k][i] = tu_art[k];
     if (damp_on)
@@ -1310,8 +1582,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     double xi[3], xj[3], dx[3], w[4];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     double sigi[6];
+    #pragma unroll
     for (int k = 0; k < 6; k++) sigi[k] = S.sig[k][i];
     const double rhoi = S.rho[i];
     const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
@@ -1340,10 +1614,13 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     if (ti == 3) { av_c = S.fluid[S.fluid_slot[i]].vpu; av_a = P.art_vis[2][0]; av_b = P.art_vis[2][1]; }
     // boundary-effect stress of particle i (CPP_9-4-derivation_stress.cpp:667-697)
     double as[6];
+    #pragma unroll
     for (int k = 0; k < 6; k++) as[k] = 0.0;
     if (ti == 1 || ti == 3) {
+        #pragma unroll
         for (int k = 0; k < 6; k++) as[k] = sigi[k];
     } else if (ti == 2 || ti == 4) {
+        #pragma unroll
         for (int k = 0; k < 3; k++) as[k] = sigi[k] - satui * prewi + (1 - satui) * preai;
         for (int k = 3; k < 6; k++) as[k] = sigi[k];
     }
@@ -1353,128 +1630,202 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     // artificial stress (ti == 2) and geostress damping (ti 2 / 4, l <= inip): pair sums from k_accel_pre
     const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
-    double aif[6], qi[6];
-    for (int k = 0; k < 6; k++) aif[k] = fpor * sigi[k];
-    for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        int tj = S.type[j];
-        load_w<NDIM>(S, P, nj, i, w);
-        load_xj<NDIM>(S, cur, j, xj);
-        double mj = S.mass[j], rj = S.rho[j];
-        if ((ti == 1 || ti == 3) && (tj == ti || tj == fother)) {
-            // fluid_acceleration_water/air (+ minor variants), CPP_9-4-derivation_stress.cpp:12-351
+    const bool bstress = ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2);
+    const bool drag_on = P.fli == 1;
+    // i-side factors of the pair terms (loop invariants of the reference's expressions)
+    double bi[6];
+    #pragma unroll
+    for (int k = 0; k < 6; k++) bi[k] = as[k] / rhoi / rhoi;
+    if (ti == 1 || ti == 3) {
+        // fluid_acceleration_water/air (+ minor variants), CPP_9-4-derivation_stress.cpp:12-351;
+        // qsig of a fluid neighbour holds (porosity * sig) / rho / rho, the j factor of the sum
+        double fi[6];
+        #pragma unroll
+        for (int k = 0; k < 6; k++) fi[k] = (fpor * sigi[k]) / rhoi / rhoi;
+        const double fdi = S.fluid[S.fluid_slot[i]].dens;
+        constexpr int B = 2;
+        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satjb[B], porjb[B], copjb[B];
 #pragma unroll
-            for (int a = 0; a < 3; a++)
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xjb[b]);
+            hljb[b] = hl_of(S, P, j, hli);
+            mjb[b] = S.mass[j];
+            rjb[b] = S.rho[j];
+            #pragma unroll
+            for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
 #pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    int s = kSidx[a][b];
-                    double aj = fpor * S.sig[s][j];
-                    if (fsgn > 0) fluid_t[a][b] = fluid_t[a][b] + mj * (aj / rj / rj + aif[s] / rhoi / rhoi) * w[b];
-                    else fluid_t[a][b] = fluid_t[a][b] + mj * (aj / rj / rj - aif[s] / rhoi / rhoi) * w[b];
+            for (int k = 0; k < 6; k++) qjb[b][k] = P.fluid_por_uniform ? S.qsig[k][j] : S.sig[k][j];
+            satjb[b] = 0.0;
+            porjb[b] = 0.0;
+            copjb[b] = 1.0;
+            if (drag_on) {
+                satjb[b] = S.satu[j];
+                porjb[b] = S.porosity[j];
+                copjb[b] = S.cop[j];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b >= act) continue;
+            const int tj = nb_type(e[b]);
+            const double *xj = xjb[b], *vj = vjb[b];
+            double *qj = qjb[b];
+            const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satj = satjb[b], porj = porjb[b], copj = copjb[b];
+            double w[4], d2, dst;
+            pair_kernel<NDIM>(P, kc, xi, xj, hli, hlj, w, d2, dst);
+            if (tj == ti || tj == fother) {
+                if (!P.fluid_por_uniform)
+                    for (int k = 0; k < 6; k++) qj[k] = (fpor * qj[k]) / rj / rj;
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        const int sx = kSidx[a][b];
+                        if (fsgn > 0) fluid_t[a][b] = fluid_t[a][b] + mj * (qj[sx] + fi[sx]) * w[b];
+                        else fluid_t[a][b] = fluid_t[a][b] + mj * (qj[sx] - fi[sx]) * w[b];
+                    }
+            }
+            if (tj == ti) av_pair<NDIM>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
+            if (tj == 0 && bstress) bnd_pair(as, bi, mj, rj, w, bnd_t);
+            if (drag_on && tj == 2) {
+                // interact1 pair sum (CPP_10-interact1.cpp:40-88), water / air against soil
+                double inprod = 0.0, dxx[3], dv[3];
+#pragma unroll
+                for (int k = 0; k < NDIM; k++) {
+                    dxx[k] = xi[k] - xj[k];
+                    dv[k] = vi[k] - vj[k];
+                    inprod = inprod + dxx[k] * dv[k];
                 }
-        }
-        if ((ti == 2 || ti == 4) && tj == ti) {
-            // soil_acceleration_noreg / structure_acceleration_noreg (:354-505)
-            double satu = S.satu[j];  // quirk: neighbour saturation used for both terms
-            double qj[6];
+                if (inprod < 0) {
+                    const double w3 = w[3];
+                    if (ti == 1) {
 #pragma unroll
-            for (int k = 0; k < 6; k++) qj[k] = S.qsig[k][j];  // sig_j / rho_j / rho_j, precomputed
+                        for (int k = 0; k < NDIM; k++)
+                            drag[k] = drag[k] + satj * satj * porj * porj * fdi * P.gacc * dv[k] / copj * mj / rj * w3;
+                    } else {
 #pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    int s = kSidx[a][b];
-                    solid_t[a][b] = solid_t[a][b] + mj * (qj[s] + qi[s]) * w[b];
-                }
-            double prewj = S.prew[j], preaj = S.prea[j];
-            if (ti == 2) {
-                double prei = (1 - spor) * (-satu * prewi + (1 - satu) * preai);
-                double prej = (1 - spor) * (-satu * prewj + (1 - satu) * preaj);
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-                    solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj + prei / sdens / sdens) * w[k];  // quirk A12
-            } else {
-                double prei = -satu * prewi + (1 - satu) * preai;
-                double prej = -satu * prewj + (1 - satu) * preaj;
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-                    solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj - prei / rhoi / rhoi) * w[k];
-            }
-        }
-        if (tj == ti && ti != 7) {
-            // artificial_viscosity (CPP_9-3-artificialvis.cpp:36-89)
-            double inprod = 0.0, dst = 0.0, dv[3], dxx[3];
-#pragma unroll
-            for (int k = 0; k < NDIM; k++) {
-                dv[k] = vi[k] - S.vxp[k][j];
-                dxx[k] = xi[k] - xj[k];
-                dst = dst + dxx[k] * dxx[k];
-                inprod = inprod + dv[k] * dxx[k];
-            }
-            double hl = (S.hl[j] + hli) * 0.5;
-            double rou = (rj + rhoi) * 0.5;
-            double paix;
-            if (inprod < 0.0) {
-                dst = sqrt(dst);
-                double myu = hl * inprod / (dst * dst + 0.01 * hl * hl);
-                paix = (-av_a * av_c * myu + av_b * myu * myu) / rou;
-            } else {
-                paix = 0.0;
-            }
-#pragma unroll
-            for (int k = 0; k < NDIM; k++) av[k] = av[k] + mj * paix * w[k];
-        }
-        if (tj == 0 && ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2)) {
-            // boundary_moving_stress_effect (CPP_9-4-derivation_stress.cpp:631-721)
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    int s = kSidx[a][b];
-                    bnd_t[a][b] = bnd_t[a][b] + mj * (as[s] / rj / rj + as[s] / rhoi / rhoi) * w[b];
-                }
-        }
-        if (P.fli == 1 && ti != tj) {
-            // interact1 pair sum (CPP_10-interact1.cpp:40-88)
-            double inprod = 0.0, dxx[3], dv[3];
-#pragma unroll
-            for (int k = 0; k < NDIM; k++) {
-                dxx[k] = xi[k] - xj[k];
-                dv[k] = vi[k] - S.vxp[k][j];
-                inprod = inprod + dxx[k] * dv[k];
-            }
-            if (inprod < 0) {
-                double w3 = w[3];
-                if (ti == 2 && tj == 1) {
-                    double fd = S.fluid[S.fluid_slot[j]].dens, copi = S.cop[i], pori = S.porosity[i];
-#pragma unroll
-                    for (int k = 0; k < NDIM; k++)
-                        drag[k] = drag[k] + satui * satui * pori * pori * fd * P.gacc * dv[k] / copi * mj / rj * w3;
-                } else if (ti == 1 && tj == 2) {
-                    double fd = S.fluid[S.fluid_slot[i]].dens, sj = S.satu[j], pj = S.porosity[j], cj = S.cop[j];
-#pragma unroll
-                    for (int k = 0; k < NDIM; k++)
-                        drag[k] = drag[k] + sj * sj * pj * pj * fd * P.gacc * dv[k] / cj * mj / rj * w3;
-                } else if (ti == 2 && tj == 3) {
-                    double fd = S.fluid[S.fluid_slot[j]].dens, pori = S.porosity[i];
-#pragma unroll
-                    for (int k = 0; k < NDIM; k++)
-                        drag[k] = drag[k] + (1 - satui) * (1 - satui) * pori * pori * fd * P.gacc * dv[k] * 2000 * mj / rj * w3;
-                } else if (ti == 3 && tj == 2) {
-                    double fd = S.fluid[S.fluid_slot[i]].dens, sj = S.satu[j], pj = S.porosity[j];
-#pragma unroll
-                    for (int k = 0; k < NDIM; k++)
-                        drag[k] = drag[k] + (1 - sj) * (1 - sj) * pj * pj * fd * P.gacc * dv[k] * 2000 * mj / rj * w3;
+                        for (int k = 0; k < NDIM; k++)
+                            drag[k] = drag[k] + (1 - satj) * (1 - satj) * porj * porj * fdi * P.gacc * dv[k] * 2000 * mj / rj * w3;
+                    }
                 }
             }
         }
+        });
+    } else if (ti == 2 || ti == 4) {
+        // soil_acceleration_noreg / structure_acceleration_noreg (CPP_9-4-derivation_stress.cpp:354-505)
+        double qi[6];
+        #pragma unroll
+        for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
+        const double copi = S.cop[i], pori = S.porosity[i];
+        constexpr int B = 2;
+        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xjb[b]);
+            hljb[b] = hl_of(S, P, j, hli);
+            mjb[b] = S.mass[j];
+            rjb[b] = S.rho[j];
+            #pragma unroll
+            for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; k++) qjb[b][k] = S.qsig[k][j];  // sig_j / rho_j / rho_j
+            satub[b] = S.satu[j];
+            prewb[b] = S.prew[j];
+            preab[b] = S.prea[j];
+            fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b >= act) continue;
+            const int tj = nb_type(e[b]);
+            const double *xj = xjb[b], *vj = vjb[b], *qj = qjb[b];
+            const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satu = satub[b], prewj = prewb[b],
+                         preaj = preab[b], fdj = fdjb[b];
+            double w[4], d2, dst;
+            pair_kernel<NDIM>(P, kc, xi, xj, hli, hlj, w, d2, dst);
+            if (tj == ti) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        const int sx = kSidx[a][b];
+                        solid_t[a][b] = solid_t[a][b] + mj * (qj[sx] + qi[sx]) * w[b];
+                    }
+                // pore-pressure term; quirk: the neighbour's saturation is used for both terms
+                if (ti == 2) {
+                    double prei = (1 - spor) * (-satu * prewi + (1 - satu) * preai);
+                    double prej = (1 - spor) * (-satu * prewj + (1 - satu) * preaj);
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj + prei / sdens / sdens) * w[k];  // quirk A12
+                } else {
+                    double prei = -satu * prewi + (1 - satu) * preai;
+                    double prej = -satu * prewj + (1 - satu) * preaj;
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj - prei / rhoi / rhoi) * w[k];
+                }
+                av_pair<NDIM>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
+            }
+            if (tj == 0 && bstress) bnd_pair(as, bi, mj, rj, w, bnd_t);
+            if (drag_on && ti == 2 && (tj == 1 || tj == 3)) {
+                // interact1 pair sum (CPP_10-interact1.cpp:40-88), soil against water / air
+                double inprod = 0.0, dxx[3], dv[3];
+#pragma unroll
+                for (int k = 0; k < NDIM; k++) {
+                    dxx[k] = xi[k] - xj[k];
+                    dv[k] = vi[k] - vj[k];
+                    inprod = inprod + dxx[k] * dv[k];
+                }
+                if (inprod < 0) {
+                    const double w3 = w[3];
+                    if (tj == 1) {
+#pragma unroll
+                        for (int k = 0; k < NDIM; k++)
+                            drag[k] = drag[k] + satui * satui * pori * pori * fdj * P.gacc * dv[k] / copi * mj / rj * w3;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < NDIM; k++)
+                            drag[k] = drag[k] + (1 - satui) * (1 - satui) * pori * pori * fdj * P.gacc * dv[k] * 2000 * mj / rj * w3;
+                    }
+                }
+            }
+        }
+        });
+    } else if (ti == 0) {
+        // boundary particles: artificial viscosity among themselves only
+        constexpr int B = 4;
+        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+            double xj[B][3], hlj[B], mj[B], rj[B], vj[B][3];
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const int j = nb_id(e[b]);
+                load_xj<NDIM>(S, cur, j, xj[b]);
+                hlj[b] = hl_of(S, P, j, hli);
+                mj[b] = S.mass[j];
+                rj[b] = S.rho[j];
+                #pragma unroll
+                for (int k = 0; k < 3; k++) vj[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                if (b >= act || nb_type(e[b]) != 0) continue;
+                double w[4], d2, dst;
+                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                av_pair<NDIM>(vi, vj[b], xi, xj[b], dst, hli, hlj[b], rhoi, rj[b], mj[b], av_a, av_b, av_c, w, av);
+            }
+        });
     }
     // combine in the reference's call order (CPP_1-SPHLoops.cpp:586-660)
     double ax[3] = {S.ax[0][i], S.ax[1][i], S.ax[2][i]};
     const double g[3] = {P.gx, P.gy, P.gz};
     if (ti >= 1 && ti <= 4) {
         double (*t)[3] = (ti == 1 || ti == 3) ? fluid_t : solid_t;
+        #pragma unroll
         for (int k = 0; k < 3; k++) ax[k] = g[k];
         ax[0] += t[0][0] + t[0][1] + t[0][2];
         ax[1] += t[1][0] + t[1][1] + t[1][2];
@@ -1493,6 +1844,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     const double axp[3] = {S.axp[0][i], S.axp[1][i], S.axp[2][i]};
     if (P.fli == 1 && ti != 0 && ti != 7) {
         // interact1 clamp (CPP_10-interact1.cpp:93-108)
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) {
             double itf = drag[k];
             double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * dt;
@@ -1509,7 +1861,9 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     if (damp_on) {
         // soil_damping_geostress clamp (CPP_9-4-derivation_stress.cpp:874-895)
         double damp[3];
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) damp[k] = S.tmp6[3 + k][i];
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) {
             double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * dt;
             double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - damp[k]) * dt;
@@ -1521,13 +1875,16 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     if (P.if_move == 1) {
         if (ti == 0) {
             const bool mv = S.matype[i] == 2 && step_of(S) > P.inip;
+            #pragma unroll
             for (int k = 0; k < NDIM; k++) S.interfss[k][i] = mv ? P.move_vx[k] : 0.0;
         } else {
+            #pragma unroll
             for (int k = 0; k < NDIM; k++) {
                 double ath = (axp[k] + ax[k]) * 0.5;
                 S.vx[cur][k][i] = vi[k] + ath * dt;
             }
         }
+        #pragma unroll
         for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
         return;
     }
@@ -1537,6 +1894,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         const int mt = P.if_move == 2 ? 2 : 4;
         double vp[3] = {vi[0], vi[1], vi[2]}, ap[3] = {axp[0], axp[1], axp[2]};
         if (ti == mt && S.etype[i] == 5 && step_of(S) > P.inip) {
+            #pragma unroll
             for (int k = 0; k < NDIM; k++) {
                 ap[k] = 0.0;
                 vp[k] = P.move_vx[k];
@@ -1544,24 +1902,29 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 S.axp[k][i] = 0.0;
             }
         }
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) {
             double ath = (ap[k] + ax[k]) * 0.5;
             S.vx[cur][k][i] = vp[k] + ath * dt;
         }
+        #pragma unroll
         for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
         return;
     }
+    #pragma unroll
     for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
     // if_move == 0
     if (ti == 0) {
         int mt = S.matype[i];
         if (mt < 2) {
+            #pragma unroll
             for (int k = 0; k < NDIM; k++) {
                 S.interf[k][i] = 0.0;
                 double ath = (0.0 + 0.0) * 0.5;
                 S.interfss[k][i] = S.interfss[k][i] + ath * dt;
             }
         } else if (mt == 3 || (mt == 5 && P.flv == 1)) {
+            #pragma unroll
             for (int k = 0; k < NDIM; k++) {
                 double f = S.interf[k][i];
                 double ath = (f + f) * 0.5;
@@ -1569,6 +1932,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             }
         }
     } else {
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) {
             double ath = (axp[k] + ax[k]) * 0.5;
             S.vx[cur][k][i] = vi[k] + ath * dt;
@@ -1587,36 +1951,87 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     const int nxt = cur ^ 1;
     double vi[3] = {S.vx[cur][0][i], S.vx[cur][1][i], S.vx[cur][2][i]};
     if (ti == 0 || ti == 7) {
+        #pragma unroll
         for (int k = 0; k < 3; k++) S.vx[nxt][k][i] = vi[k];
         return;
     }
-    double xi[3], xj[3], dx[3], w[4];
+    double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     double tu[3] = {0, 0, 0}, dn = 0.0;
     const double xsame = ti == 1 ? P.xsph[0] : ti == 2 ? P.xsph[1] : ti == 3 ? P.xsph[2] : P.xsph[1];
     const double xbnd = (ti == 1 || ti == 3) ? 0.006 : P.xsph[1] / 5.0;
-    const int nxs = P.if_move == 2 ? 0 : nc;  // xsph is skipped when soil is driven (CPP_1-SPHLoops.cpp:714)
-    for (int nj = 0; nj < nxs; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        int tj = S.type[j];
-        double xs;
-        if (tj == ti) xs = xsame;
-        else if (tj == 0) xs = xbnd;
-        else continue;
-        load_w<NDIM>(S, P, nj, i, w);
-        double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
+    const bool do_xsph = P.if_move != 2;  // xsph is skipped when soil is driven (CPP_1-SPHLoops.cpp:714)
+    const bool tm = P.flbndy == 2;
+    // tm_boundary_2d / 3d first loop (CPP_13-boundary.cpp:196-283, :411-520), walked in the same
+    // pass (it does not depend on the smoothed velocity): ids of the first NDIM slots after
+    // clParti_Pair::dist_sorting (quirk A18: ids move, the distances stay in their slots),
+    // computed in one streaming pass over the boundary neighbours; the distance is the root of the
+    // same squared sum the kernel value uses.
+    int cur_id[3] = {-1, -1, -1};
+    double d0[3] = {0, 0, 0};
+    int total = 0, idm = 0;
+    constexpr int B = 4;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xj[B][3], hlj[B], mj[B], rj[B], vxj[B][3];
+        int mtj[B];
 #pragma unroll
-        for (int k = 0; k < NDIM; k++) tu[k] = tu[k] + xs * mj * (S.vx[cur][k][j] - vi[k]) * w3 / rj;
-        dn = dn + mj * w3 / rj;
-    }
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xj[b]);
+            hlj[b] = hl_of(S, P, j, hli);
+            mj[b] = S.mass[j];
+            rj[b] = S.rho[j];
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) vxj[b][k] = S.vx[cur][k][j];
+            mtj[b] = tm ? S.matype[j] : 0;
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+        const int j = nb_id(e[b]), tj = nb_type(e[b]);
+        const bool bnd = tj == 0;
+        if (b >= act || (tj != ti && !bnd)) continue;
+        double w[4], d2, dst;
+        pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+        if (do_xsph) {
+            const double xs = tj == ti ? xsame : xbnd, w3 = w[3];
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) tu[k] = tu[k] + xs * mj[b] * (vxj[b][k] - vi[k]) * w3 / rj[b];
+            dn = dn + mj[b] * w3 / rj[b];
+        }
+        if (tm && bnd) {
+            if (mtj[b] > 2) idm = j;
+            int x = j;
+            if (total == 0) {
+                cur_id[0] = x;
+                d0[0] = dst;
+            } else {
+                int passes = total < NDIM ? total : NDIM;
+                for (int q = 0; q < passes; q++)
+                    if (d0[q] > dst) {
+                        int t = cur_id[q];
+                        cur_id[q] = x;
+                        x = t;
+                    }
+                if (total < NDIM) {
+                    cur_id[total] = x;
+                    d0[total] = dst;
+                }
+            }
+            total++;
+        }
+        }
+    });
     double v[3] = {vi[0], vi[1], vi[2]};
-    if (P.if_move == 2) {
+    if (!do_xsph) {
         // v stays the velocity of the momentum update
     } else if (fabs(dn) > 1.0e-7 && (ti == 2 || ti == 4)) {
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) v[k] = vi[k] + tu[k] / dn;
     } else {
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) v[k] = vi[k] + tu[k];
     }
     const double dr = P.dr;
@@ -1624,14 +2039,25 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     if ((P.flbndy == 2 && (ti == 1 || ti == 3)) || P.flbndy == 1) {
         const double coe = P.flbndy == 1 ? 0.30 : P.coe_bndy;
         double t1[3] = {0, 0, 0};
-        for (int nj = 0; nj < nc; nj++) {
-            int j = S.nbr[(size_t)nj * P.cap + i];
-            if (S.type[j] != 0) continue;
-            double inprod = 0.0, dst = 0.0, xjj[3];
-            for (int k = 0; k < NDIM; k++) xjj[k] = S.xp[cur][k][j];
+        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xjb[B][3], fjb[B][3];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            #pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                xjb[b][k] = S.xp[cur][k][j];
+                fjb[b][k] = S.interfss[k][j];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if (b >= act || nb_type(e[b]) != 0) continue;
+            const double *xjj = xjb[b], *fj = fjb[b];
+            double inprod = 0.0, dst = 0.0;
 #pragma unroll
             for (int k = 0; k < NDIM; k++) {
-                inprod = inprod + (v[k] - S.interfss[k][j]) * (xi[k] - xjj[k]);
+                inprod = inprod + (v[k] - fj[k]) * (xi[k] - xjj[k]);
                 dst = dst + (xi[k] - xjj[k]) * (xi[k] - xjj[k]);
             }
             if (inprod < 0.0) {
@@ -1643,48 +2069,19 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
                 for (int k = 0; k < NDIM; k++) t1[k] = t1[k] + (1 - beta) * (xi[k] - xjj[k]) * inprod / dst / dst;
             }
         }
+        });
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) v[k] = v[k] + t1[k];
     }
+    #pragma unroll
     for (int k = 0; k < 3; k++) S.vx[nxt][k][i] = v[k];
-    if (P.flbndy != 2) return;
-    // tm_boundary_2d / 3d first loop (CPP_13-boundary.cpp:196-283, :411-520)
-    // ids of the first NDIM slots after clParti_Pair::dist_sorting (quirk A18: ids move, the
-    // distances stay in their slots), computed in one streaming pass over the boundary neighbours.
-    int cur_id[3] = {-1, -1, -1};
-    double d0[3] = {0, 0, 0};
-    int total = 0, idm = 0;
-    for (int nj = 0; nj < nc; nj++) {
-        int j = S.nbr[(size_t)nj * P.cap + i];
-        if (S.type[j] != 0) continue;
-        double xjj[3] = {S.xp[cur][0][j], S.xp[cur][1][j], NDIM == 3 ? S.xp[cur][2][j] : 0.0};
-        double dst = (xi[0] - xjj[0]) * (xi[0] - xjj[0]) + (xi[1] - xjj[1]) * (xi[1] - xjj[1]) +
-                     (xi[2] - xjj[2]) * (xi[2] - xjj[2]);
-        dst = sqrt(dst);
-        if (S.matype[j] > 2) idm = j;
-        int x = j;
-        if (total == 0) {
-            cur_id[0] = x;
-            d0[0] = dst;
-        } else {
-            int passes = total < NDIM ? total : NDIM;
-            for (int p = 0; p < passes; p++)
-                if (d0[p] > dst) {
-                    int t = cur_id[p];
-                    cur_id[p] = x;
-                    x = t;
-                }
-            if (total < NDIM) {
-                cur_id[total] = x;
-                d0[total] = dst;
-            }
-        }
-        total++;
-    }
+    if (!tm) return;
     double vb[3];
     if (total >= NDIM) {
         double dst;
         const double p4[3] = {xi[0], xi[1], xi[2]};
         double p1[3], p2[3];
+        #pragma unroll
         for (int k = 0; k < 3; k++) {
             p1[k] = (k < NDIM) ? S.xp[cur][k][cur_id[0]] : 0.0;
             p2[k] = (k < NDIM) ? S.xp[cur][k][cur_id[1]] : 0.0;
@@ -1744,6 +2141,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     } else {
         vb[0] = v[0]; vb[1] = v[1]; vb[2] = v[2];
     }
+    #pragma unroll
     for (int k = 0; k < 3; k++) S.vel_bndy[k][i] = vb[k];
 }
 
@@ -1761,24 +2159,40 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     if (ti == 0 && P.flbndy == 2) {
-        double xj[3], dx[3], w[4];
         const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
         const int nc = nb_count(S, i);
         double tv[4][3], td[4] = {0, 0, 0, 0};
         for (int a = 0; a < 4; a++) tv[a][0] = tv[a][1] = tv[a][2] = 0.0;
-        for (int nj = 0; nj < nc; nj++) {
-            int j = S.nbr[(size_t)nj * P.cap + i];
-            int tj = S.type[j];
-            if (tj < 1 || tj > 4) continue;
-            load_w<NDIM>(S, P, nj, i, w);
-            double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
-            int t = tj - 1;
-            tv[t][0] = tv[t][0] + mj * S.vel_bndy[0][j] * w3 / rj;
-            tv[t][1] = tv[t][0] + mj * S.vel_bndy[1][j] * w3 / rj;  // quirk A17
-            tv[t][2] = tv[t][0] + mj * S.vel_bndy[2][j] * w3 / rj;
-            td[t] = td[t] + mj * w3 / rj;
-        }
+        constexpr int B = 4;
+        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+            double xj[B][3], hlj[B], mj[B], rj[B], vbj[B][3];
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const int j = nb_id(e[b]);
+                load_xj<NDIM>(S, cur, j, xj[b]);
+                hlj[b] = hl_of(S, P, j, hli);
+                mj[b] = S.mass[j];
+                rj[b] = S.rho[j];
+                #pragma unroll
+                for (int k = 0; k < 3; k++) vbj[b][k] = S.vel_bndy[k][j];
+            }
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const int tj = nb_type(e[b]);
+                if (b >= act || tj < 1 || tj > 4) continue;
+                double w[4], d2, dst;
+                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                const double w3 = w[3];
+                int t = tj - 1;
+                tv[t][0] = tv[t][0] + mj[b] * vbj[b][0] * w3 / rj[b];
+                tv[t][1] = tv[t][0] + mj[b] * vbj[b][1] * w3 / rj[b];  // quirk A17
+                tv[t][2] = tv[t][0] + mj[b] * vbj[b][2] * w3 / rj[b];
+                td[t] = td[t] + mj[b] * w3 / rj[b];
+            }
+        });
         for (int t = 0; t < 4; t++)
+            #pragma unroll
             for (int k = 0; k < 3; k++) S.vb[t][k][i] = fabs(td[t]) > 1.0e-6 ? tv[t][k] / td[t] : 0.0;
     }
     double v[3] = {S.vx[nxt][0][i], S.vx[nxt][1][i], S.vx[nxt][2][i]};
@@ -1793,7 +2207,9 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
         if (NDIM == 3 && xi[2] - P.bmin[2] <= 0.5 * P.dr) hit = true;
         if (hit) {
             int b = P.first_moving_bndy;
+            #pragma unroll
             for (int k = 0; k < 3; k++) v[k] = (b < i) ? S.interfss[k][b] : 0.0;
+            #pragma unroll
             for (int k = 0; k < 3; k++) S.vx[nxt][k][i] = v[k];
         }
     }
@@ -1801,6 +2217,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     const bool driven = (P.if_move == 2 || P.if_move == 3) && ti == (P.if_move == 2 ? 2 : 4) &&
                         S.etype[i] == 5 && l > P.inip;
     if (ti != 0) {
+        #pragma unroll
         for (int k = 0; k < NDIM; k++) {
             double vp = driven ? P.move_vx[k] : S.vxp[k][i];
             double xn = xi[k] + (v[k] + vp) * dt * 0.5;
@@ -1817,6 +2234,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     } else {
         int mt = S.matype[i];
         if (mt >= 2 && mt <= 5) {
+            #pragma unroll
             for (int k = 0; k < NDIM; k++) {
                 double f = S.interfss[k][i];
                 S.xp[nxt][k][i] = xi[k] + (f + f) * dt * 0.5;
@@ -1826,6 +2244,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
             S.rhop[i] = S.rho[i];
             S.prep[i] = S.pre[i];
         } else {
+            #pragma unroll
             for (int k = 0; k < 3; k++) S.xp[nxt][k][i] = S.xp[cur][k][i];
         }
     }
@@ -1852,13 +2271,13 @@ template __global__ void k_xsph<2>(State, Params, int);
 template __global__ void k_xsph<3>(State, Params, int);
 template __global__ void k_update<2>(State, Params, int);
 template __global__ void k_update<3>(State, Params, int);
-template __global__ void k_regu<2>(State, Params, int);
+template __global__ void k_regu<2>(State, Params, int, int);
 template __global__ void k_stress<false>(State, Params, int);
 template __global__ void k_stress<true>(State, Params, int);
 template __global__ void k_saturation<2>(State, Params, int);
 template __global__ void k_saturation<3>(State, Params, int);
 template __global__ void k_accel_pre<2>(State, Params, int);
 template __global__ void k_accel_pre<3>(State, Params, int);
-template __global__ void k_regu<3>(State, Params, int);
+template __global__ void k_regu<3>(State, Params, int, int);
 
 }  // namespace sphg
