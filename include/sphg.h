@@ -37,7 +37,8 @@ enum {
     SPHG_E_ARG = 100,        /* bad argument (null pointer, unknown field, size mismatch) */
     SPHG_E_HIP = 101,        /* HIP runtime error */
     SPHG_E_UNSUPPORTED = 102,/* flag combination this build does not implement */
-    SPHG_E_CAPACITY = 103    /* exceeds the context's capacity */
+    SPHG_E_CAPACITY = 103,   /* exceeds the context's capacity */
+    SPHG_E_NOCONVERGE = 104  /* K-L root bisection cannot reach |f| <= 1e-6 in FP64 (the reference never returns) */
 };
 
 enum { SPHG_FP64 = 0, SPHG_MIXED = 1 };
@@ -188,6 +189,44 @@ int sphg_get_neighbors(sphg_ctx *ctx, int *counts, int *ids, int n);
  * fills up to cap (name, total ms, launches) rows, returns the row count. */
 int sphg_kernel_times(sphg_ctx *ctx, char (*names)[32], double *ms, int *launches, int cap);
 int sphg_enable_timing(sphg_ctx *ctx, int on);
+
+/* ---- K-L random fields (CPP_17-1_K-L_expansion_non.cpp) ---------------------------------- */
+
+#define SPHG_KL_TERMS 200    /* klterm (CPP_17-1_K-L_expansion_non.cpp:20) */
+
+/* One spatially variable property (Para_SpatialVariability, Class_spatial_variability.h:44-66)
+ * plus the K-L modes that sphg_kl_modes() fills in. */
+typedef struct {
+    int ndim;
+    int dist;                /* 1 normal, 2 lognormal */
+    int field;               /* 0 soil fai, 1 soil c, 2 soil cop, 3 soil ds, 4 fluid fai, 5 fluid c */
+    double mean_value, sd_value, coe_depth;
+    double rl[3];            /* correlation lengths rl_value */
+    double a[3], x_trans[3]; /* K-L domain [-a, a] about x_trans */
+    double soil_max, water_max;  /* depth-trend references (Cell_Con soil_max / water_max, y or z) */
+    double wi[SPHG_KL_TERMS][3];
+    double eigval[SPHG_KL_TERMS];
+    int order[SPHG_KL_TERMS][3];
+} sphg_kl_field;
+
+/* K-L domain of SpatialVariables_Generate_2D/3D (CPP_17-1_K-L_expansion_non.cpp:72-90, :175-191):
+ * the extents of the non-boundary particles widened by 20 % per side (input_par's *_nb box,
+ * CPP_2-initialf.cpp:915-1003); returns 13 when a half-width is zero, as the reference. */
+int sphg_kl_domain(int ndim, int n, const double *xp, const int *type, double a[3], double x_trans[3]);
+/* solve_wi + solve_ramdai (CPP_17-1_K-L_expansion_non.cpp:268-438): roots by bisection in FP64
+ * on the host, eigenvalue products, the 200 largest in the reference's order. Needs ndim, a, rl. */
+int sphg_kl_modes(sphg_kl_field *f);
+/* solve_h_2D / solve_h_3D (CPP_17-1_K-L_expansion_non.cpp:499-713) on the device for the
+ * context's particles (FP64), with the standard normals xi[200] supplied by the caller (the
+ * reference draws them from a clock-seeded generator, :24-34). Field 1 (soil cohesion) only: the
+ * result becomes the per-particle cohesion of the soil models. */
+int sphg_kl_synthesize(sphg_ctx *ctx, const sphg_kl_field *f, const double *xi);
+/* Realisation batch in a common K-L domain: for nreal realisations xi[nreal][200] over the same
+ * particles, H = Phi (n x 200) * Xi (200 x nreal) as one FP32 matrix-core GEMM, then the field
+ * values value[nreal][n] (FP64 from the FP32 H; particles of other types keep fill). gemm_ms
+ * (may be null) receives the GEMM kernel time. */
+int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const double *xp, const int *type, int nreal,
+                             const double *xi, double fill, double *value, double *gemm_ms);
 
 const char *sphg_error_string(int code);
 const char *sphg_build_info(void);

@@ -14,12 +14,14 @@
 
 #include "../../include/sphg.h"
 #include "sphg_kernels.hip"
+#include "sphg_kl.hip"
 
 using namespace sphg;
 
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kKlTermsHost = SPHG_KL_TERMS;
 
 struct Timing {
     std::string name;
@@ -836,10 +838,104 @@ extern "C" const char *sphg_error_string(int code) {
     case SPHG_E_HIP: return "HIP runtime error";
     case SPHG_E_UNSUPPORTED: return "flag combination not implemented";
     case SPHG_E_CAPACITY: return "capacity exceeded";
+    case SPHG_E_NOCONVERGE: return "K-L root bisection did not converge";
     default: return "unknown error";
     }
 }
 
 extern "C" const char *sphg_build_info(void) {
     return "sphg gfx950 FP64 step library (contraction off)";
+}
+
+// -------------------------------------------------------------------------------- K-L fields
+extern "C" int sphg_kl_synthesize(sphg_ctx *c, const sphg_kl_field *f, const double *xi) {
+    if (!c || !xi) return SPHG_E_ARG;
+    int rc = kl_check(f);
+    if (rc) return rc;
+    if (f->field != 1 || c->n <= 0) return SPHG_E_UNSUPPORTED;
+    // per-particle cohesion starts from the material slots (setting_vis_cons copies), then the
+    // field overwrites the soil particles
+    if (!c->has_psc) {
+        std::vector<double> base(c->n);
+        for (int i = 0; i < c->n; i++) {
+            int s = soil_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
+            base[i] = s < SPHG_NSOIL ? c->mat.soil[s].c : 0.0;
+        }
+        HIPCHK(hipMemcpy(c->d_psc, base.data(), (size_t)c->n * 8, hipMemcpyHostToDevice));
+        c->has_psc = true;
+    }
+    sphg::KlModes m;
+    kl_modes_device(*f, xi, m);
+    sphg::KlModes *dm = nullptr;
+    HIPCHK(hipMalloc(&dm, sizeof(m)));
+    hipError_t e = hipMemcpy(dm, &m, sizeof(m), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        sphg::KlParams kp = kl_params(*f, c->n);
+        const int cur = c->cur;
+        hipLaunchKernelGGL(sphg::k_kl_fp64, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->S.xp[cur][0],
+                           c->S.xp[cur][1], c->S.xp[cur][2], c->S.type, dm, kp, c->d_psc);
+        e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(dm);
+    if (e != hipSuccess) return SPHG_E_HIP;
+    c->S.ps_c = c->d_psc;
+    for (int b = 0; b < 2; b++) c->graph_ok[b] = false;  // the State copy in the graphs changed
+    return SPHG_OK;
+}
+
+extern "C" int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const double *xp, const int *type, int nreal,
+                                        const double *xi, double fill, double *value, double *gemm_ms) {
+    if (!xp || !type || !xi || !value || n <= 0 || nreal <= 0) return SPHG_E_ARG;
+    int rc = kl_check(f);
+    if (rc) return rc;
+    const int npad = (n + 31) / 32 * 32, rpad = (nreal + 31) / 32 * 32;
+    sphg::KlModes m;
+    kl_modes_device(*f, nullptr, m);
+    std::vector<float> xif((size_t)kKlTermsHost * rpad, 0.0f);
+    for (int r = 0; r < nreal; r++)
+        for (int j = 0; j < kKlTermsHost; j++) xif[(size_t)j * rpad + r] = (float)xi[(size_t)r * kKlTermsHost + j];
+    sphg::KlParams kp = kl_params(*f, n);
+    char *pool = nullptr;
+    const size_t b_xp = align256((size_t)n * 24), b_ty = align256((size_t)n * 4), b_m = align256(sizeof(m)),
+                 b_phi = align256((size_t)npad * kKlTermsHost * 4), b_xi = align256(xif.size() * 4),
+                 b_h = align256((size_t)npad * rpad * 4), b_v = align256((size_t)nreal * n * 8);
+    HIPCHK(hipMalloc(&pool, b_xp + b_ty + b_m + b_phi + b_xi + b_h + b_v));
+    double *dxp = (double *)pool;
+    int *dty = (int *)(pool + b_xp);
+    sphg::KlModes *dm = (sphg::KlModes *)(pool + b_xp + b_ty);
+    float *dphi = (float *)(pool + b_xp + b_ty + b_m);
+    float *dxi = (float *)(pool + b_xp + b_ty + b_m + b_phi);
+    float *dh = (float *)(pool + b_xp + b_ty + b_m + b_phi + b_xi);
+    double *dv = (double *)(pool + b_xp + b_ty + b_m + b_phi + b_xi + b_h);
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemcpyAsync(dxp, xp, (size_t)n * 24, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dty, type, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dm, &m, sizeof(m), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dxi, xif.data(), xif.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const int tot = npad * kKlTermsHost;
+        hipLaunchKernelGGL(sphg::k_kl_phi, dim3((tot + 255) / 256), dim3(256), 0, st, dxp, dty, dm, kp, npad, dphi);
+        hipEventRecord(e0, st);
+        hipLaunchKernelGGL(sphg::k_kl_gemm, dim3(npad / 32, rpad / 32), dim3(64), 0, st, dphi, dxi, rpad, dh);
+        hipEventRecord(e1, st);
+        const int totv = n * nreal;
+        hipLaunchKernelGGL(sphg::k_kl_values, dim3((totv + 255) / 256), dim3(256), 0, st, dh, dxp, dty, kp, nreal,
+                           rpad, fill, dv);
+        e = hipMemcpyAsync(value, dv, (size_t)nreal * n * 8, hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && gemm_ms) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        *gemm_ms = ms;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    hipFree(pool);
+    return e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
 }

@@ -56,6 +56,66 @@ class Materials(C.Structure):
     _fields_ = [("soil", Soil * 30), ("water", Fluid * 3), ("air", Fluid), ("structure", Structure)]
 
 
+KL_TERMS = 200
+
+
+class KLField(C.Structure):
+    """sphg_kl_field: one spatially variable property and its K-L modes."""
+    _fields_ = [("ndim", _I), ("dist", _I), ("field", _I), ("mean_value", _D), ("sd_value", _D),
+                ("coe_depth", _D), ("rl", _D * 3), ("a", _D * 3), ("x_trans", _D * 3), ("soil_max", _D),
+                ("water_max", _D), ("wi", (_D * 3) * KL_TERMS), ("eigval", _D * KL_TERMS),
+                ("order", (_I * 3) * KL_TERMS)]
+
+    def modes(self):
+        """(wi[200][3], eigval[200], order[200][3]) as arrays."""
+        wi = np.ctypeslib.as_array(self.wi).reshape(KL_TERMS, 3).copy()
+        ev = np.ctypeslib.as_array(self.eigval).copy()
+        od = np.ctypeslib.as_array(self.order).reshape(KL_TERMS, 3).copy()
+        return wi, ev, od
+
+
+def kl_field(deck: Deck, dist: int, mean: float, sd: float, rl, field: int = 1, coe_depth: float = 0.0) -> KLField:
+    """K-L field for `deck` over the reference's domain (sphg_kl_domain) with its modes
+    (sphg_kl_modes); the depth-trend references are the deck's soil / water maxima on the
+    vertical axis the reference reads (y in 2D, the z entry in 3D)."""
+    L = product_lib().lib
+    f = KLField()
+    f.ndim, f.dist, f.field = deck.ndim, dist, field
+    f.mean_value, f.sd_value, f.coe_depth = mean, sd, coe_depth
+    for k in range(3):
+        f.rl[k] = float(rl[k]) if k < len(rl) else 0.0
+    xp = np.ascontiguousarray(deck.xp, dtype=np.float64)
+    ty = np.ascontiguousarray(deck.type, np.int32)
+    rc = L.sphg_kl_domain(deck.ndim, deck.n, _dptr(xp), _iptr(ty), f.a, f.x_trans)
+    if rc:
+        raise RuntimeError(f"sphg_kl_domain: error {rc}")
+    _, _, ph = deck.domain()
+    ax = 1 if deck.ndim == 2 else 2
+    f.soil_max = float(ph["soil"][1][ax])
+    f.water_max = float(ph["water"][1][ax])
+    rc = L.sphg_kl_modes(C.byref(f))
+    if rc:
+        raise RuntimeError(f"sphg_kl_modes: error {rc}")
+    return f
+
+
+def kl_synthesize_batch(f: KLField, xp: np.ndarray, ty: np.ndarray, xi: np.ndarray, fill: float = 0.0):
+    """Field values [nreal][n] of a realisation batch in the common domain (FP32 matrix-core GEMM)
+    and the GEMM kernel time in ms."""
+    L = product_lib().lib
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    ty = np.ascontiguousarray(ty, np.int32)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    nreal = xi.shape[0]
+    out = np.zeros((nreal, xp.shape[0]))
+    ms = C.c_double(0.0)
+    rc = L.sphg_kl_synthesize_batch(C.byref(f), xp.shape[0], _dptr(xp), _iptr(ty), nreal, _dptr(xi), fill,
+                                    _dptr(out), C.byref(ms))
+    if rc:
+        raise RuntimeError(f"sphg_kl_synthesize_batch: error {rc}")
+    return out, ms.value
+
+
 def problem_from_deck(d: Deck) -> Problem:
     p = Problem()
     for k in "fla fls flr fli flv fleob flts flspv flknl ntd flbndy ndim fop inip loop step_regu dttype".split():
@@ -237,6 +297,16 @@ def product_lib() -> _Lib:
             [C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)] + \
             [C.POINTER(C.c_double)] * 14
         L.sphg_setting_par.restype = _I
+        L.sphg_kl_domain.argtypes = [_I, _I, C.POINTER(C.c_double), C.POINTER(C.c_int), _D * 3, _D * 3]
+        L.sphg_kl_domain.restype = _I
+        L.sphg_kl_modes.argtypes = [C.POINTER(KLField)]
+        L.sphg_kl_modes.restype = _I
+        L.sphg_kl_synthesize.argtypes = [C.c_void_p, C.POINTER(KLField), C.POINTER(C.c_double)]
+        L.sphg_kl_synthesize.restype = _I
+        L.sphg_kl_synthesize_batch.argtypes = [C.POINTER(KLField), _I, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                               _I, C.POINTER(C.c_double), _D, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double)]
+        L.sphg_kl_synthesize_batch.restype = _I
         for nm in ("parttocell", "partisearching", "density", "strain", "strain_to_stress", "accel", "xsph",
                    "update"):
             f = getattr(L, "sphg_" + nm)
@@ -284,6 +354,13 @@ class GpuSim(Sim):
         cnt = (C.c_int * cap)()
         k = self.L.lib.sphg_kernel_times(self.ctx, names, ms, cnt, cap)
         return [(bytes(names[i]).split(b"\0", 1)[0].decode(), ms[i], cnt[i]) for i in range(k)]
+
+    def kl_synthesize(self, f: "KLField", xi: np.ndarray) -> None:
+        """sphg_kl_synthesize: K-L field (soil cohesion) into the per-particle cohesion."""
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        rc = self.L.lib.sphg_kl_synthesize(self.ctx, C.byref(f), _dptr(xi))
+        if rc:
+            raise RuntimeError(f"sphg_kl_synthesize: error {rc}")
 
     def device_bytes(self) -> int:
         b = C.c_size_t()

@@ -476,3 +476,36 @@ def art_stress_block(ndim=3, nx=5, ny=4, nz=6, dx=0.004, steps=10, seed=17, art_
     soil = deck.type == 2
     deck.vxp[soil, 0] = stretch * (deck.xp[soil, 0] - 0.5 * nx * dx)
     return deck
+
+
+def slope_2d(height=5.0, dx=0.1, steps=3000, seed=0, crest=1.5, toe=1.5, base=4.0, walls=3) -> Deck:
+    """2D soil slope for the random-field batch: a crest of `crest`*H, a 1:2 (vertical:horizontal)
+    face, a toe of `toe`*H and a `base` m layer under all of it, Drucker-Prager soil (mean
+    c = 10 kPa, phi = 25, E = 20 MPa), rigid floor and side walls of `walls` layers, lattice at
+    (i + 1/2) dx with seeded jitter. H = 5..15 m at dx = 0.1 gives about 16k..86k particles."""
+    rng = np.random.default_rng(seed)
+    H = float(height)
+    d = Deck(ndim=2, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=steps, gravity=(0.0, -9.81, 0.0),
+             art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), flbndy=2, name=f"slope2d_H{H:g}")
+    d.model_flags[2] = 1
+    d.soil_layers[2] = [dp_layer(c=10000.0, fai=25.0, e=2.0e7, poi=0.3, dens=2000.0)]
+    L = (crest + 2.0 + toe) * H
+    nx = int(round(L / dx))
+    nb = int(round(base / dx))
+    pts = []
+    for ix in range(nx):
+        x = (ix + 0.5) * dx
+        if x < crest * H:
+            top = H
+        elif x < (crest + 2.0) * H:
+            top = H - (x - crest * H) / 2.0
+        else:
+            top = 0.0
+        ny = nb + int(np.floor(top / dx))
+        for iy in range(ny):
+            pts.append((x, (iy + 0.5) * dx, 0.0))
+    soil = np.array(pts)
+    hw = nb + int(np.ceil(H / dx)) + 2
+    parts = [(soil, 2, 2), (_lattice2d(-walls, nx + walls, -walls, 0, dx), 0, 0),
+             (_lattice2d(-walls, 0, 0, hw, dx), 0, 0), (_lattice2d(nx, nx + walls, 0, nb + 2, dx), 0, 0)]
+    return _assemble(d, parts, rng)

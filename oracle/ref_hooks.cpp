@@ -137,9 +137,92 @@ void clFParaOutput_Fun::paramters_output(Particle *pPar, Para_Fluid *pVFluid, Pa
 
 int SPH_Calculation(int argc, char *argv);
 
+// K-L mode: runs the reference's solve_wi, solve_ramdai and solve_h_2D/3D
+// (CPP_17-1_K-L_expansion_non.cpp:268-713) on injected standard normals.
+// kl_in.bin: int32 ndim, n; double a[3], rl[3], x_trans[3]; int32 dist; double mean, sd,
+//            coe_depth, soil_max, water_max; int32 field; double uu[200]; double xp[n][3];
+//            int32 type[n]; double value[n] (the field before the synthesis)
+// kl_out.bin: double wi[200][3], eigval[200]; int32 order[200][3]; double value[n]; int32 rc
+int kl_mode(const char *dir) {
+    std::string in = std::string(dir) + "/kl_in.bin", out = std::string(dir) + "/kl_out.bin";
+    FILE *f = fopen(in.c_str(), "rb");
+    if (!f) return 2;
+    int ndim, n, dist, field;
+    double a[3], rl[3], xt[3], mean, sd, cdep, smax, wmax, uu[200];
+    bool ok = fread(&ndim, 4, 1, f) == 1 && fread(&n, 4, 1, f) == 1 && fread(a, 8, 3, f) == 3 &&
+              fread(rl, 8, 3, f) == 3 && fread(xt, 8, 3, f) == 3 && fread(&dist, 4, 1, f) == 1 &&
+              fread(&mean, 8, 1, f) == 1 && fread(&sd, 8, 1, f) == 1 && fread(&cdep, 8, 1, f) == 1 &&
+              fread(&smax, 8, 1, f) == 1 && fread(&wmax, 8, 1, f) == 1 && fread(&field, 4, 1, f) == 1 &&
+              fread(uu, 8, 200, f) == 200;
+    std::vector<double> xp((size_t)n * 3), val(n);
+    std::vector<int> type(n);
+    ok = ok && fread(xp.data(), 8, xp.size(), f) == xp.size() && fread(type.data(), 4, n, f) == (size_t)n &&
+         fread(val.data(), 8, n, f) == (size_t)n;
+    fclose(f);
+    if (!ok) return 2;
+    Particle *par = new Particle[n];
+    Para_Soil *ps = new Para_Soil[n];
+    Para_Fluid *pf = new Para_Fluid[n];
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) par[i].xp[k] = xp[3 * i + k];
+        par[i].type = type[i];
+        double v = val[i];
+        if (field == 0) ps[i].fai = v;
+        if (field == 1) ps[i].c = v;
+        if (field == 2) ps[i].cop = v;
+        if (field == 3) ps[i].ds = v;
+        if (field == 4) pf[i].fai = v;
+        if (field == 5) pf[i].c = v;
+    }
+    Para_Pro pro;
+    pro.ntotal = n;
+    pro.ndim = ndim;
+    Para_SpatialVariability psv;
+    psv.type = dist;
+    psv.mean_value = mean;
+    psv.sd_value = sd;
+    psv.coe_depth = cdep;
+    for (int k = 0; k < 3; k++) {
+        psv.rl_value[k] = rl[k];
+        psv.a[k] = a[k];
+    }
+    double wi[200][3] = {{0}}, eig[200];
+    int order[200][3];
+    clFSpatialVariability_Fun fn;
+    fn.solve_wi(wi, psv.a, psv.rl_value, ndim);
+    int rc = fn.solve_ramdai(eig, wi, psv.rl_value, order, ndim);
+    if (!rc) {
+        if (ndim == 2) rc = fn.solve_h_2D(par, pf, ps, pro, wi, eig, uu, xt, order, psv, smax, wmax, field);
+        else rc = fn.solve_h_3D(par, pf, ps, pro, wi, eig, uu, xt, order, psv, smax, wmax, field);
+    }
+    for (int i = 0; i < n; i++) {
+        double v = val[i];
+        if (field == 0) v = ps[i].fai;
+        if (field == 1) v = ps[i].c;
+        if (field == 2) v = ps[i].cop;
+        if (field == 3) v = ps[i].ds;
+        if (field == 4) v = pf[i].fai;
+        if (field == 5) v = pf[i].c;
+        val[i] = v;
+    }
+    FILE *o = fopen(out.c_str(), "wb");
+    if (!o) return 2;
+    fwrite(wi, 8, 600, o);
+    fwrite(eig, 8, 200, o);
+    fwrite(order, 4, 600, o);
+    fwrite(val.data(), 8, n, o);
+    fwrite(&rc, 4, 1, o);
+    fclose(o);
+    delete[] par;
+    delete[] ps;
+    delete[] pf;
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc >= 3 && strcmp(argv[1], "--kl") == 0) return kl_mode(argv[2]);
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <project-dir-with-input.dat>\n", argv[0]);
+        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir>\n", argv[0]);
         return 2;
     }
     return SPH_Calculation(3, argv[1]);

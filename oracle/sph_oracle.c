@@ -398,7 +398,7 @@ static int or_parttocell(or_ctx *c) {
         c->cend[i] = -10000000;
     }
     for (i = 0; i < n; i++) {
-        int nt[3];
+        int nt[3] = {2, 2, 2};
         for (j = 0; j < P->ndim; j++) {
             double v = fabs(c->p[i].xp[j] - xmin[j]) / dr + 2.0; /* quirk: fabs mirrors (A6) */
             nt[j] = (int)v;

@@ -43,6 +43,15 @@ int or_get_neighbors(or_ctx *c, int *counts, int *ids, int n);
 /* Sum over particles of ninflu in the last step (directed pairs). */
 long long or_pair_count(or_ctx *c);
 
+/* K-L random-field synthesis (kl_oracle.c, CPP_17-1_K-L_expansion_non.cpp): 200 modes. */
+int or_kl_solve_wi(double wi[][3], const double a_x[3], const double dirac_r[3], int ndim);
+void or_kl_solve_ramdai(double *eigval, const double wi[][3], const double dirac_r[3], int order_id[][3],
+                        int ndim);
+int or_kl_synthesize(int ndim, int n, const double *xp, const int *type, const double wi[][3],
+                     const double *eigval, const int order_id[][3], const double *uu, const double x_trans[3],
+                     const double a[3], int dist, double mean_value, double sd_value, double coe_depth,
+                     double soil_max, double water_max, int field, double *value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,9 +16,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from helpers import run_ref, have_ref  # noqa: E402
+from helpers import run_ref, run_ref_kl, have_ref  # noqa: E402
 from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d,  # noqa: E402
-                    art_stress_block)
+                    art_stress_block, slope_2d)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype"})
@@ -35,6 +35,40 @@ def decks():
         "art_stress_2d": (art_stress_block(2, steps=6), 6, (1, 2, 4, 6)),
         "art_stress_3d": (art_stress_block(3, nx=4, ny=3, nz=4, steps=6), 6, (1, 3, 6)),
     }
+
+
+def kl_cases():
+    """K-L inputs: (deck, dist, mean, sd, rl, coe_depth, field, xi seed, initial value)."""
+    return {
+        "kl_slope2d": (slope_2d(1.0, dx=0.1, seed=3), 2, 10000.0, 3000.0, (1.0, 0.2, 0.0), 200.0, 1, 11, 10000.0),
+        "kl_block3d": (art_stress_block(3, nx=5, ny=4, nz=6, dx=0.2), 1, 20.0, 3.0, (1.0, 1.0, 0.5), 5.0, 0, 12,
+                       30.0),
+    }
+
+
+def kl_inputs(name):
+    """The K-L case `name` as the arrays and scalars run_ref_kl / oracle_kl take."""
+    from sphgpy.capi import kl_field
+    deck, dist, mean, sd, rl, cdep, field, seed, v0 = kl_cases()[name]
+    f = kl_field(deck, dist=dist, mean=mean, sd=sd, rl=rl, coe_depth=cdep, field=field)
+    xi = np.random.default_rng(seed).standard_normal(200)
+    return deck, f, (deck.ndim, deck.xp, deck.type, list(f.a), list(f.rl), list(f.x_trans), xi, dist, mean, sd,
+                     cdep, f.soil_max, f.water_max, field, np.full(deck.n, v0))
+
+
+def main_kl(names=None):
+    for name in kl_cases():
+        if names and name not in names:
+            continue
+        deck, f, args = kl_inputs(name)
+        wi, ev, od, val, rc = run_ref_kl(*args)
+        assert rc == 0
+        scal = np.array([args[8], args[9], args[10], args[11], args[12]])  # mean sd coe_depth soil_max water_max
+        ints = np.array([args[0], args[7], args[13]], np.int32)  # ndim dist field
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), xp=deck.xp, type=deck.type, a=np.array(args[3]),
+                            rl=np.array(args[4]), x_trans=np.array(args[5]), xi=args[6], value0=args[14], wi=wi,
+                            eigval=ev, order=od, value=val, scal=scal, ints=ints)
+        print(name, deck.n, "particles")
 
 
 def main(names=None):
@@ -54,4 +88,10 @@ def main(names=None):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:])
+    args = sys.argv[1:]
+    step_names = [a for a in args if not a.startswith("kl_")]
+    kl_names = [a for a in args if a.startswith("kl_")]
+    if not args or step_names:
+        main(step_names or None)
+    if not args or kl_names:
+        main_kl(kl_names or None)

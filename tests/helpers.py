@@ -126,3 +126,64 @@ def max_rel(a: np.ndarray, b: np.ndarray) -> float:
     if den == 0.0:
         return float(num)
     return float(num / den)
+
+
+# ---------------------------------------------------------------------------------- K-L
+KL_TERMS = 200
+
+
+def oracle_kl(ndim, xp, ty, a, rl, x_trans, xi, dist, mean, sd, coe_depth, soil_max, water_max, field, value):
+    """Oracle K-L (oracle/kl_oracle.c): returns wi[200][3], eigval[200], order[200][3], value[n], rc."""
+    L = oracle_lib().lib
+    D3 = C.c_double * 3
+    wi = np.zeros((KL_TERMS, 3))
+    ev = np.zeros(KL_TERMS)
+    od = np.zeros((KL_TERMS, 3), np.int32)
+    dp = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    ip = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    rcw = L.or_kl_solve_wi(dp(wi), D3(*a), D3(*rl), ndim)
+    if rcw:
+        return wi, ev, od, np.ascontiguousarray(value, dtype=np.float64).copy(), rcw
+    L.or_kl_solve_ramdai(dp(ev), dp(wi), D3(*rl), ip(od), ndim)
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    ty = np.ascontiguousarray(ty, np.int32)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    out = np.ascontiguousarray(value, dtype=np.float64).copy()
+    L.or_kl_synthesize.restype = C.c_int
+    L.or_kl_synthesize.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), D3, D3, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
+    rc = L.or_kl_synthesize(ndim, xp.shape[0], dp(xp), ip(ty), dp(wi), dp(ev), ip(od), dp(xi), D3(*x_trans),
+                            D3(*a), dist, mean, sd, coe_depth, soil_max, water_max, field, dp(out))
+    return wi, ev, od, out, rc
+
+
+def run_ref_kl(ndim, xp, ty, a, rl, x_trans, xi, dist, mean, sd, coe_depth, soil_max, water_max, field, value):
+    """The reference's own solve_wi / solve_ramdai / solve_h_2D|3D through the link harness."""
+    import struct
+    n = xp.shape[0]
+    wd = tempfile.mkdtemp(prefix="sphkl_")
+    try:
+        with open(os.path.join(wd, "kl_in.bin"), "wb") as f:
+            f.write(struct.pack("<ii", ndim, n))
+            f.write(np.asarray(a, np.float64).tobytes() + np.asarray(rl, np.float64).tobytes() +
+                    np.asarray(x_trans, np.float64).tobytes())
+            f.write(struct.pack("<i", dist) + struct.pack("<5d", mean, sd, coe_depth, soil_max, water_max) +
+                    struct.pack("<i", field))
+            f.write(np.asarray(xi, np.float64).tobytes())
+            f.write(np.ascontiguousarray(xp, np.float64).tobytes() + np.ascontiguousarray(ty, np.int32).tobytes() +
+                    np.ascontiguousarray(value, np.float64).tobytes())
+        r = subprocess.run([REF_HARNESS, "--kl", wd], capture_output=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference harness --kl failed rc={r.returncode}")
+        raw = open(os.path.join(wd, "kl_out.bin"), "rb").read()
+        o = 0
+        wi = np.frombuffer(raw, np.float64, 600, o).reshape(200, 3).copy(); o += 4800
+        ev = np.frombuffer(raw, np.float64, 200, o).copy(); o += 1600
+        od = np.frombuffer(raw, np.int32, 600, o).reshape(200, 3).copy(); o += 2400
+        val = np.frombuffer(raw, np.float64, n, o).copy(); o += 8 * n
+        rc = int(np.frombuffer(raw, np.int32, 1, o)[0])
+        return wi, ev, od, val, rc
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
