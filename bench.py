@@ -31,7 +31,7 @@ sys.path.insert(0, os.path.join(REPO, "multiphase-sph-solver_amd"))
 
 import numpy as np  # noqa: E402
 
-from sphgpy import column_collapse_2d, soil_water_column_3d  # noqa: E402
+from sphgpy import column_collapse_2d, soil_water_column_3d, slope_2d  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -70,7 +70,21 @@ def make_deck(workload: str):
     raise SystemExit(f"unknown workload {workload}")
 
 
+def make_batch(nreal: int, seed: int = 2024):
+    """Monte-Carlo slope batch: nreal 2D slopes, height uniform in [5, 15] m per seed, dx = 0.1 m,
+    lognormal cohesion (mean 10 kPa, COV 0.3, correlation lengths 10 m x 2 m) from a 200-mode K-L
+    expansion per realisation."""
+    rng = np.random.default_rng(seed)
+    heights = rng.uniform(5.0, 15.0, nreal)
+    decks = [slope_2d(float(h), dx=0.1, steps=3000, seed=seed + r) for r, h in enumerate(heights)]
+    xis = rng.standard_normal((nreal, 200))
+    return decks, xis
+
+
 WORKLOAD_DESC = {
+    "slope_batch": "Monte-Carlo slope batch: 2D slopes of height U[5,15] m at dx=0.1 m (crest 1.5H, 1:2 face, "
+                   "toe 1.5H, 4 m base; 17k-89k particles each), DP phi=25, lognormal cohesion (10 kPa, COV 0.3, "
+                   "lx=10 m, ly=2 m, 200 K-L modes) per realisation, all realisations in one context",
     "soilwater3d": "3D saturated soil-water column: soil 64x32x64 + water 64x32x16 lattice, dx=0.005 m, "
                    "3-layer walls/floor, DP c=1 kPa phi=25, Darcy drag, water vpu=30 m/s",
     "column2d": "2D dry granular column collapse: soil 60x120 at dx=0.002 m + 3-layer floor/left wall, "
@@ -96,7 +110,7 @@ def _ref_wall(deck, steps: int, threads: int) -> float:
     return el
 
 
-def cpu_baseline_reference(deck, budget_s: float):
+def cpu_baseline_reference(deck, budget_s: float, what: str = "the same"):
     """The reference solver itself (its own sources compiled into oracle/_ref, OpenMP with the
     deck's ntd threads). Set-up, the initial and step-1 dumps are the same in every run, so the
     per-step time is the wall-time difference of a short and a long run."""
@@ -108,7 +122,7 @@ def cpu_baseline_reference(deck, budget_s: float):
     tk = _ref_wall(deck, 2 + k, threads)
     per_step = (tk - t2) / k
     return {"value": deck.n / per_step, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
-            "sample": f"reference solver (its own sources, g++ -O2 -fopenmp, ntd={threads}) on the same "
+            "sample": f"reference solver (its own sources, g++ -O2 -fopenmp, ntd={threads}) on {what} "
                       f"{deck.n}-particle deck: ({k + 2}-step run - 2-step run) / {k} steps = "
                       f"{per_step * 1e3:.1f} ms/step"}
 
@@ -145,20 +159,36 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time the hipGraph replay path (no per-kernel events)")
+    ap.add_argument("--nreal", type=int, default=256, help="realisations of the slope_batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from sphgpy.capi import GpuSim, setting_par
+    from sphgpy.capi import GpuSim, GpuBatch, batch_state, kl_field, setting_par
 
-    deck = make_deck(args.workload)
-    state = setting_par(deck)
-    t_setup = time.perf_counter()
-    g = GpuSim(deck)
-    g.load_state(state)
-    g.upload_done()
+    extra = {}
+    if args.workload == "slope_batch":
+        decks, xis = make_batch(args.nreal)
+        g = GpuBatch(decks)
+        g.load_state(batch_state(decks))
+        g.upload_done()
+        t_kl = time.perf_counter()
+        for r, d in enumerate(decks):
+            g.kl_synthesize_part(r, kl_field(d, dist=2, mean=10000.0, sd=3000.0, rl=(10.0, 2.0)), xis[r])
+        g.sync()
+        extra["realisations"] = len(decks)
+        extra["kl_synthesis_s"] = round(time.perf_counter() - t_kl, 3)
+        extra["particles_per_realisation"] = [int(min(d.n for d in decks)), int(max(d.n for d in decks))]
+        deck = decks[0]
+        state = None
+    else:
+        deck = make_deck(args.workload)
+        state = setting_par(deck)
+        g = GpuSim(deck)
+        g.load_state(state)
+        g.upload_done()
     g.set_step(1)
     rc = g.step(args.warmup)
     if rc:
@@ -173,7 +203,7 @@ def main():
     if rc:
         raise SystemExit(f"timed steps failed with error {rc}")
     el = t1 - t0
-    n = deck.n
+    n = g.n
     value = n * args.steps / el
     ms_per_step = el * 1e3 / args.steps
 
@@ -215,9 +245,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic seeded lattice deck",
-        "config": {"workload": args.workload, "description": WORKLOAD_DESC[args.workload], "particles": n,
-                   "precision": "FP64", "mean_neighbours": round(nbar, 2),
-                   "timing": "graph replay" if args.graph else "eager launches with per-kernel HIP events"},
+        "config": dict({"workload": args.workload, "description": WORKLOAD_DESC[args.workload], "particles": n,
+                        "precision": "FP64", "mean_neighbours": round(nbar, 2),
+                        "timing": "graph replay" if args.graph else "eager launches with per-kernel HIP events"},
+                       **extra),
         "roofline": roof,
         "pair_interactions_per_s": round(float(cnt.sum()) * args.steps / el, 1),
         "step_alg_bytes": int(step_alg),
@@ -225,10 +256,13 @@ def main():
         "kernels": kernels,
     }
     if not args.no_cpu:
+        # for the batch: the first realisation alone (the reference runs one deck per process)
         if os.path.exists(REF_HARNESS):
-            line["cpu_baseline"] = cpu_baseline_reference(deck, args.cpu_budget)
+            what = "the first realisation's" if args.workload == "slope_batch" else "the same"
+            line["cpu_baseline"] = cpu_baseline_reference(deck, args.cpu_budget, what)
         else:
-            line["cpu_baseline"] = cpu_baseline(deck, state, args.cpu_budget)
+            line["cpu_baseline"] = cpu_baseline(deck, state if state is not None else setting_par(deck),
+                                                args.cpu_budget)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

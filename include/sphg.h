@@ -100,11 +100,23 @@ typedef struct {
 
 typedef struct sphg_ctx sphg_ctx;
 
+/* Cell_Con domain box of one simulation (Class_NNPS.h:12-36). */
+typedef struct {
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+} sphg_box;
+
 /* Context creation: capacity in particles and the arithmetic form (SPHG_FP64 or SPHG_MIXED).
  * Replaces the allocation block of SPH_Calculation (CPP_1-SPHLoops.cpp:354-378) and the
  * cell-list set-up celllist_ini1/celllist_ini2 (CPP_3-NNPS.cpp:96-171, called at :446-450). */
 int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat,
                 int capacity, int precision);
+/* Realisation batch (no counterpart in the reference, which runs one deck per process): nreal
+ * simulations of one problem family (same flags, materials, dr, dt) with their own particles and
+ * domain boxes in one context. Each keeps its own cell grid and check_domain box, so every
+ * realisation computes exactly what it computes alone; one launch per kernel serves all of them.
+ * Particles are uploaded realisation-major, counts[r] each. */
+int sphg_create_batch(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat, int nreal,
+                      const int *counts, const sphg_box *boxes, int precision);
 void sphg_destroy(sphg_ctx *ctx);
 int sphg_device_bytes(const sphg_ctx *ctx, size_t *bytes);
 
@@ -221,6 +233,9 @@ int sphg_kl_modes(sphg_kl_field *f);
  * reference draws them from a clock-seeded generator, :24-34). Field 1 (soil cohesion) only: the
  * result becomes the per-particle cohesion of the soil models. */
 int sphg_kl_synthesize(sphg_ctx *ctx, const sphg_kl_field *f, const double *xi);
+/* The same for the particles [first, first + count) only: one realisation of a batch context
+ * (no counterpart in the reference, which holds one realisation per run). */
+int sphg_kl_synthesize_part(sphg_ctx *ctx, const sphg_kl_field *f, const double *xi, int first, int count);
 /* Realisation batch in a common K-L domain: for nreal realisations xi[nreal][200] over the same
  * particles, H = Phi (n x 200) * Xi (200 x nreal) as one FP32 matrix-core GEMM, then the field
  * values value[nreal][n] (FP64 from the FP32 H; particles of other types keep fill). gemm_ms

@@ -57,6 +57,10 @@ struct sphg_ctx {
     bool regu_on = false;  // regularisation passes are in the launch sequence
     bool pre_on = false;   // artificial-stress / geostress-damping pair pass is in the launch sequence
     bool full_models = false;  // some soil particle uses Cam-Clay, unsaturated Cam-Clay or FOCC
+    std::vector<Grid> grids;   // one per realisation (host copy)
+    std::vector<int> counts;   // particles per realisation (batch contexts)
+    Grid *d_grids = nullptr;
+    int *d_rid = nullptr;
 };
 
 #define HIPCHK(x)                                                               \
@@ -73,8 +77,35 @@ static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 // -------------------------------------------------------------------------------- creation
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat, int capacity,
-                           int precision) {
+// Cell grid of one simulation: celllist_ini1 (CPP_3-NNPS.cpp:96-122) and the nncell stencil of
+// celllist_ini2 (:125-171) in reference order. Returns false when the grid is empty or too large.
+static bool make_grid(const sphg_box &b, int ndim, double dr, Grid &g) {
+    const double cs = 2.0 * dr;
+    g = Grid{};
+    g.xmin[0] = b.xmin; g.xmin[1] = b.ymin; g.xmin[2] = b.zmin;
+    g.bmin[0] = b.xmin; g.bmin[1] = b.ymin; g.bmin[2] = b.zmin;
+    g.bmax[0] = b.xmax; g.bmax[1] = b.ymax; g.bmax[2] = b.zmax;
+    g.cxm = (int)((b.xmax - b.xmin) / cs + 3.0);
+    g.cym = (int)((b.ymax - b.ymin) / cs + 3.0);
+    g.czm = ndim == 2 ? 1 : (int)((b.zmax - b.zmin) / cs + 3.0);
+    long long ct = (long long)g.cxm * g.cym * g.czm;
+    if (ct <= 0 || ct > (1LL << 30)) return false;
+    g.ncell = (int)ct;
+    int p = 0;
+    if (ndim == 2) {
+        for (int ky = 1; ky <= 3; ky++)
+            for (int kx = 1; kx <= 3; kx++) g.nb_off[p++] = (kx - 2) * g.cym + ky - 2;
+    } else {
+        for (int kz = 1; kz <= 3; kz++)
+            for (int kx = 1; kx <= 3; kx++)
+                for (int ky = 1; ky <= 3; ky++) g.nb_off[p++] = (ky - 2) * g.cxm * g.czm + (kx - 2) * g.czm + kz - 2;
+    }
+    g.nstencil = p;
+    return true;
+}
+
+static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat, int capacity,
+                         int precision, int nreal, const int *counts, const sphg_box *boxes) {
     if (!out || !pro || !mat) return SPHG_E_ARG;
     if (capacity <= 0) return SPHG_E_NOPARTICLES;
     if (capacity > (1 << 28)) return SPHG_E_CAPACITY;  // list entries pack the type above 28 id bits
@@ -85,30 +116,25 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
     c->mat = *mat;
     c->cap = capacity;
     c->precision = precision;
-    // cell grid (celllist_ini1, CPP_3-NNPS.cpp:96-122)
     Params &P = c->P;
-    double cs = 2.0 * pro->dr;
-    P.cxm = (int)((pro->xmax - pro->xmin) / cs + 3.0);
-    P.cym = (int)((pro->ymax - pro->ymin) / cs + 3.0);
-    P.czm = pro->ndim == 2 ? 1 : (int)((pro->zmax - pro->zmin) / cs + 3.0);
-    long long ct = (long long)P.cxm * P.cym * P.czm;
-    if (ct <= 0 || ct > (1LL << 30)) {
+    P.cell_size = 2.0 * pro->dr;
+    P.nreal = nreal;
+    long long cells = 0;
+    c->grids.resize(nreal);
+    for (int r = 0; r < nreal; r++) {
+        if (!make_grid(boxes[r], pro->ndim, pro->dr, c->grids[r])) {
+            delete c;
+            return SPHG_E_CAPACITY;
+        }
+        c->grids[r].base = (int)cells;
+        cells += c->grids[r].ncell;
+    }
+    if (cells > (1LL << 30)) {
         delete c;
         return SPHG_E_CAPACITY;
     }
-    P.ncell = (int)ct;
-    P.cell_size = cs;
-    // nncell offsets in reference order (celllist_ini2, CPP_3-NNPS.cpp:125-171)
-    int p = 0;
-    if (pro->ndim == 2) {
-        for (int ky = 1; ky <= 3; ky++)
-            for (int kx = 1; kx <= 3; kx++) P.nb_off[p++] = (kx - 2) * P.cym + ky - 2;
-    } else {
-        for (int kz = 1; kz <= 3; kz++)
-            for (int kx = 1; kx <= 3; kx++)
-                for (int ky = 1; ky <= 3; ky++) P.nb_off[p++] = (ky - 2) * P.cxm * P.czm + (kx - 2) * P.czm + kz - 2;
-    }
-    P.nstencil = p;
+    if (counts) c->counts.assign(counts, counts + nreal);
+    P.ncell = (int)cells;  // all realisations' cells, numbered consecutively
 
     // HBM layout
     const size_t cap = (size_t)capacity;
@@ -153,6 +179,8 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
+    plan.push_back({(void **)&c->d_grids, sizeof(Grid) * (size_t)nreal});
+    I(c->d_rid, cap);
     plan.push_back({(void **)&c->d_soil, sizeof(SoilSlot) * kNSlots});
     plan.push_back({(void **)&c->d_fluid, sizeof(FluidSlot) * 8});
     size_t total = 0;
@@ -169,6 +197,8 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
     }
     S.soil = c->d_soil;
     S.fluid = c->d_fluid;
+    S.grids = c->d_grids;
+    S.rid = c->d_rid;
     S.ps_c = nullptr;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         hipFree(c->pool);
@@ -185,6 +215,25 @@ extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_m
     c->h_etype.assign(cap, 0);
     *out = c;
     return SPHG_OK;
+}
+
+extern "C" int sphg_create(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat, int capacity,
+                           int precision) {
+    if (!pro) return SPHG_E_ARG;
+    sphg_box b{pro->xmin, pro->xmax, pro->ymin, pro->ymax, pro->zmin, pro->zmax};
+    return create_common(out, pro, mat, capacity, precision, 1, nullptr, &b);
+}
+
+extern "C" int sphg_create_batch(sphg_ctx **out, const sphg_problem *pro, const sphg_materials *mat, int nreal,
+                                 const int *counts, const sphg_box *boxes, int precision) {
+    if (!pro || !counts || !boxes || nreal <= 0) return SPHG_E_ARG;
+    long long tot = 0;
+    for (int r = 0; r < nreal; r++) {
+        if (counts[r] <= 0) return SPHG_E_NOPARTICLES;
+        tot += counts[r];
+    }
+    if (tot > (1 << 28)) return SPHG_E_CAPACITY;
+    return create_common(out, pro, mat, (int)tot, precision, nreal, counts, boxes);
 }
 
 extern "C" void sphg_destroy(sphg_ctx *c) {
@@ -453,11 +502,6 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.flts = pro.flts;
     P.dt = pro.dt;
     P.dr = pro.dr;
-    P.xmin = pro.xmin;
-    P.ymin = pro.ymin;
-    P.zmin = pro.zmin;
-    P.bmin[0] = pro.xmin; P.bmin[1] = pro.ymin; P.bmin[2] = pro.zmin;
-    P.bmax[0] = pro.xmax; P.bmax[1] = pro.ymax; P.bmax[2] = pro.zmax;
     P.gx = pro.gx; P.gy = pro.gy; P.gz = pro.gz;
     P.gacc = sqrt(pro.gx * pro.gx + pro.gy * pro.gy + pro.gz * pro.gz);
     for (int k = 0; k < 3; k++) {
@@ -479,12 +523,31 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.struct_poi = c->mat.structure.poi;
     P.water_acc_minor = c->mat.water[0].acc_type != 0;
     P.air_acc_minor = c->mat.air.acc_type != 0;
-    P.first_moving_bndy = n;
-    for (int i = 0; i < n; i++)
-        if (c->h_type[i] == 0 && c->h_matype[i] > 2) {
-            P.first_moving_bndy = i;
-            break;
+    // realisation of every particle (batch contexts: realisation-major ranges of counts[r]) and
+    // each realisation's first type-0 particle with matype > 2 (check_domain's wall velocity)
+    {
+        const int nr = P.nreal;
+        std::vector<int> rid(n, 0), first(nr + 1, 0);
+        if (!c->counts.empty()) {
+            long long tot = 0;
+            for (int r = 0; r < nr; r++) {
+                first[r] = (int)tot;
+                tot += c->counts[r];
+            }
+            if (tot != n) return SPHG_E_ARG;
         }
+        first[nr] = n;
+        for (int r = 0; r < nr; r++) {
+            Grid &g = c->grids[r];
+            g.first_moving_bndy = n;
+            for (int i = first[r]; i < first[r + 1]; i++) {
+                rid[i] = r;
+                if (g.first_moving_bndy == n && c->h_type[i] == 0 && c->h_matype[i] > 2) g.first_moving_bndy = i;
+            }
+        }
+        HIPCHK(hipMemcpy(c->d_rid, rid.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_grids, c->grids.data(), sizeof(Grid) * nr, hipMemcpyHostToDevice));
+    }
     c->full_models = false;
     for (int i = 0; i < n; i++) {
         int m = c->h_matype[i], mt = ((m >= 0 && m < 30) || (m >= 50 && m < 60)) ? m % 10 : -1;
@@ -848,11 +911,12 @@ extern "C" const char *sphg_build_info(void) {
 }
 
 // -------------------------------------------------------------------------------- K-L fields
-extern "C" int sphg_kl_synthesize(sphg_ctx *c, const sphg_kl_field *f, const double *xi) {
-    if (!c || !xi) return SPHG_E_ARG;
+extern "C" int sphg_kl_synthesize_part(sphg_ctx *c, const sphg_kl_field *f, const double *xi, int first,
+                                       int count) {
+    if (!c || !xi || first < 0 || count <= 0 || first + count > c->n) return SPHG_E_ARG;
     int rc = kl_check(f);
     if (rc) return rc;
-    if (f->field != 1 || c->n <= 0) return SPHG_E_UNSUPPORTED;
+    if (f->field != 1) return SPHG_E_UNSUPPORTED;
     // per-particle cohesion starts from the material slots (setting_vis_cons copies), then the
     // field overwrites the soil particles
     if (!c->has_psc) {
@@ -870,10 +934,10 @@ extern "C" int sphg_kl_synthesize(sphg_ctx *c, const sphg_kl_field *f, const dou
     HIPCHK(hipMalloc(&dm, sizeof(m)));
     hipError_t e = hipMemcpy(dm, &m, sizeof(m), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        sphg::KlParams kp = kl_params(*f, c->n);
+        sphg::KlParams kp = kl_params(*f, count);
         const int cur = c->cur;
-        hipLaunchKernelGGL(sphg::k_kl_fp64, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->S.xp[cur][0],
-                           c->S.xp[cur][1], c->S.xp[cur][2], c->S.type, dm, kp, c->d_psc);
+        hipLaunchKernelGGL(sphg::k_kl_fp64, dim3(grid_for(count)), dim3(kBlock), 0, c->stream, c->S.xp[cur][0],
+                           c->S.xp[cur][1], c->S.xp[cur][2], c->S.type, dm, kp, first, c->d_psc);
         e = hipStreamSynchronize(c->stream);
     }
     hipFree(dm);
@@ -881,6 +945,11 @@ extern "C" int sphg_kl_synthesize(sphg_ctx *c, const sphg_kl_field *f, const dou
     c->S.ps_c = c->d_psc;
     for (int b = 0; b < 2; b++) c->graph_ok[b] = false;  // the State copy in the graphs changed
     return SPHG_OK;
+}
+
+extern "C" int sphg_kl_synthesize(sphg_ctx *c, const sphg_kl_field *f, const double *xi) {
+    if (!c) return SPHG_E_ARG;
+    return sphg_kl_synthesize_part(c, f, xi, 0, c->n);
 }
 
 extern "C" int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const double *xp, const int *type, int nreal,

@@ -36,14 +36,23 @@ struct FluidSlot {
     int bhtype, acc_type;
 };
 
+// Cell grid of one simulation (celllist_ini1/2, CPP_3-NNPS.cpp:96-171) and its check_domain box.
+// A context holds one grid per realisation: their cells are numbered consecutively (realisation r
+// owns global cells base + 1 .. base + ncell; global cell 0 parks particles outside their grid),
+// so realisations that overlap in space never see each other and each keeps the cell keys,
+// neighbour lists and arithmetic it has alone.
+struct Grid {
+    double xmin[3];          // Cell_Con xmin, ymin, zmin
+    double bmin[3], bmax[3]; // Cell_Con box for check_domain
+    int cxm, cym, czm, ncell, base, first_moving_bndy, nstencil, pad;
+    int nb_off[27];          // stencil offsets in the reference's nncell order
+};
+
 struct Params {
     int n, cap, ndim, flbndy, flknl, fli, if_move, flv, inip, ss_type, flts;
-    int ncell;            // ctotal (reference cell ids run 1..ctotal)
-    int cxm, cym, czm;
-    int first_moving_bndy;  // lowest index of a type-0 particle with matype > 2, or n
+    int ncell;            // cells of all realisations (each grid's own ids run 1..ctotal)
     int water_acc_minor, air_acc_minor;
-    double dt, dr, cell_size, xmin, ymin, zmin;
-    double bmin[3], bmax[3];
+    double dt, dr, cell_size;
     double gx, gy, gz, gacc;
     double xsph[3];
     double art_vis[3][2];
@@ -52,12 +61,11 @@ struct Params {
     double pers, perl, perm;
     double tmin, tmax;
     double struct_e, struct_poi;
-    int nb_off[27];       // stencil offsets in the reference's nncell order
-    int nstencil;
     int flagum, step_regu;
     double art_stre_coe;
     double move_vx[3];
     int hl_uniform;          // every particle has the same smoothing length (hl_j == hl_i bit for bit)
+    int nreal;               // realisations (cell grids) in the context
     int fluid_por_uniform;   // every fluid particle's slot has the same porosity
 };
 
@@ -86,6 +94,8 @@ struct State {
     double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
     double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
     int *err, *step;      // sticky error word; current step number l
+    const Grid *grids;    // one per realisation
+    const int *rid;       // realisation of each particle
     const SoilSlot *soil;
     const FluidSlot *fluid;
 };

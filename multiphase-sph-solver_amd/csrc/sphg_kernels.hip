@@ -188,20 +188,20 @@ template <int NDIM>
 __global__ __launch_bounds__(256) void k_bin(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
-    const double xmin[3] = {P.xmin, P.ymin, P.zmin};
+    const Grid &g = S.grids[S.rid[i]];
     long long nt[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < NDIM; k++) {
-        double v = fabs(S.xp[cur][k][i] - xmin[k]) / P.cell_size + 2.0;  // fabs mirrors (quirk A6)
-        if (!(v < 1.0e9)) v = 1.0e9;                                      // keeps the cast defined
+        double v = fabs(S.xp[cur][k][i] - g.xmin[k]) / P.cell_size + 2.0;  // fabs mirrors (quirk A6)
+        if (!(v < 1.0e9)) v = 1.0e9;                                        // keeps the cast defined
         nt[k] = (long long)v;
         if (nt[k] < 2) nt[k] = 2;
     }
-    long long id = NDIM == 2 ? (long long)P.cym * (nt[0] - 1) + nt[1]
-                             : (long long)P.cxm * P.czm * (nt[1] - 1) + (long long)P.czm * (nt[0] - 1) + nt[2];
+    long long id = NDIM == 2 ? (long long)g.cym * (nt[0] - 1) + nt[1]
+                             : (long long)g.cxm * g.czm * (nt[1] - 1) + (long long)g.czm * (nt[0] - 1) + nt[2];
     int c;
-    if (id > 0 && id <= P.ncell) {
-        c = (int)id;
+    if (id > 0 && id <= g.ncell) {
+        c = g.base + (int)id;
     } else {
         set_err(S, SPHG_ERR_CELL);  // error 3; the particle is parked in cell 0, which no search visits
         c = 0;
@@ -322,11 +322,13 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
     const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
-    if (S.type[i] != 7) {
-        const int c = S.cell[i];
-        for (int s = 0; s < P.nstencil; s++) {
-            int nl = c + P.nb_off[s];
-            if (nl <= 0 || nl > P.ncell || c == 0) continue;
+    if (S.type[i] != 7 && S.cell[i] != 0) {
+        const Grid &g = S.grids[S.rid[i]];
+        const int c = S.cell[i] - g.base;  // cell id within the particle's own grid
+        for (int s = 0; s < g.nstencil; s++) {
+            const int nll = c + g.nb_off[s];
+            if (nll <= 0 || nll > g.ncell) continue;
+            const int nl = g.base + nll;
             int st = S.cell_start[nl], en = st + S.cell_count[nl];
             for (int q = st; q < en; q++) {
                 const int e = S.sorted[q];
@@ -2198,15 +2200,16 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     double v[3] = {S.vx[nxt][0][i], S.vx[nxt][1][i], S.vx[nxt][2][i]};
     if (ti != 0 && ti != 7) {
         // check_domain with the serial meaning of its shared flag (quirk A3)
+        const Grid &g = S.grids[S.rid[i]];
         bool hit = false;
-        if (xi[0] - P.bmax[0] >= -0.5 * P.dr) hit = true;
-        if (xi[0] - P.bmin[0] <= 0.5 * P.dr) hit = true;
-        if (xi[1] - P.bmax[1] >= -0.5 * P.dr) hit = true;
-        if (xi[1] - P.bmin[1] <= 0.5 * P.dr) hit = true;
-        if (NDIM == 3 && xi[2] - P.bmax[2] >= -0.5 * P.dr) hit = true;
-        if (NDIM == 3 && xi[2] - P.bmin[2] <= 0.5 * P.dr) hit = true;
+        if (xi[0] - g.bmax[0] >= -0.5 * P.dr) hit = true;
+        if (xi[0] - g.bmin[0] <= 0.5 * P.dr) hit = true;
+        if (xi[1] - g.bmax[1] >= -0.5 * P.dr) hit = true;
+        if (xi[1] - g.bmin[1] <= 0.5 * P.dr) hit = true;
+        if (NDIM == 3 && xi[2] - g.bmax[2] >= -0.5 * P.dr) hit = true;
+        if (NDIM == 3 && xi[2] - g.bmin[2] <= 0.5 * P.dr) hit = true;
         if (hit) {
-            int b = P.first_moving_bndy;
+            int b = g.first_moving_bndy;
             #pragma unroll
             for (int k = 0; k < 3; k++) v[k] = (b < i) ? S.interfss[k][b] : 0.0;
             #pragma unroll

@@ -52,9 +52,9 @@ __device__ __forceinline__ double kl_eigvect(const KlModes &M, int ndim, int j, 
 
 // FP64 synthesis into the per-particle cohesion of a context
 __global__ __launch_bounds__(256) void k_kl_fp64(const double *x, const double *y, const double *z, const int *type,
-                                                 const KlModes *M, KlParams kp, double *out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kp.n) return;
+                                                 const KlModes *M, KlParams kp, int first, double *out) {
+    const int i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= first + kp.n) return;
     const int t = type[i];
     if (t == 0 || t == 7) return;
     const double xa[3] = {x[i] - kp.x_trans[0], y[i] - kp.x_trans[1], kp.ndim == 3 ? z[i] - kp.x_trans[2] : 0.0};

@@ -59,6 +59,10 @@ class Materials(C.Structure):
 KL_TERMS = 200
 
 
+class Box(C.Structure):
+    _fields_ = [(n, _D) for n in "xmin xmax ymin ymax zmin zmax".split()]
+
+
 class KLField(C.Structure):
     """sphg_kl_field: one spatially variable property and its K-L modes."""
     _fields_ = [("ndim", _I), ("dist", _I), ("field", _I), ("mean_value", _D), ("sd_value", _D),
@@ -307,6 +311,11 @@ def product_lib() -> _Lib:
                                                _I, C.POINTER(C.c_double), _D, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double)]
         L.sphg_kl_synthesize_batch.restype = _I
+        L.sphg_kl_synthesize_part.argtypes = [C.c_void_p, C.POINTER(KLField), C.POINTER(C.c_double), _I, _I]
+        L.sphg_kl_synthesize_part.restype = _I
+        L.sphg_create_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Problem), C.POINTER(Materials), _I,
+                                        C.POINTER(C.c_int), C.POINTER(Box), _I]
+        L.sphg_create_batch.restype = _I
         for nm in ("parttocell", "partisearching", "density", "strain", "strain_to_stress", "accel", "xsph",
                    "update"):
             f = getattr(L, "sphg_" + nm)
@@ -377,6 +386,49 @@ class GpuSim(Sim):
             self.close()
         except Exception:
             pass
+
+
+class GpuBatch(GpuSim):
+    """A realisation batch in one context (sphg_create_batch): decks of one problem family, each
+    with its own particles, domain box and cell grid; particles concatenated realisation-major."""
+
+    def __init__(self, decks, precision: int = 0):
+        lib = product_lib()
+        self.decks = list(decks)
+        self._pro = problem_from_deck(self.decks[0])
+        self._mat = materials_from_deck(self.decks[0])
+        nr = len(self.decks)
+        counts = (C.c_int * nr)(*[d.n for d in self.decks])
+        boxes = (Box * nr)()
+        for r, d in enumerate(self.decks):
+            bmin, bmax, _ = d.domain()
+            boxes[r].xmin, boxes[r].xmax = bmin[0], bmax[0]
+            boxes[r].ymin, boxes[r].ymax = bmin[1], bmax[1]
+            boxes[r].zmin, boxes[r].zmax = bmin[2], bmax[2]
+        ctx = C.c_void_p()
+        rc = lib.lib.sphg_create_batch(C.byref(ctx), C.byref(self._pro), C.byref(self._mat), nr, counts, boxes,
+                                       precision)
+        if rc:
+            raise RuntimeError(f"sphg_create_batch: error {rc}")
+        self.offsets = np.concatenate([[0], np.cumsum([d.n for d in self.decks])]).astype(np.int64)
+        Sim.__init__(self, lib, ctx, int(self.offsets[-1]))
+
+    def kl_synthesize_part(self, r: int, f: "KLField", xi: np.ndarray) -> None:
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        a, b = int(self.offsets[r]), int(self.offsets[r + 1])
+        rc = self.L.lib.sphg_kl_synthesize_part(self.ctx, C.byref(f), _dptr(xi), a, b - a)
+        if rc:
+            raise RuntimeError(f"sphg_kl_synthesize_part: error {rc}")
+
+    def part(self, name: str, r: int) -> np.ndarray:
+        a, b = int(self.offsets[r]), int(self.offsets[r + 1])
+        return self.get(name)[a:b]
+
+
+def batch_state(decks) -> Dict[str, np.ndarray]:
+    """Concatenated initial states (setting_par of every deck), realisation-major."""
+    parts = [setting_par(d) for d in decks]
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
 
 
 def setting_par(deck: Deck) -> Dict[str, np.ndarray]:

@@ -492,19 +492,14 @@ def slope_2d(height=5.0, dx=0.1, steps=3000, seed=0, crest=1.5, toe=1.5, base=4.
     L = (crest + 2.0 + toe) * H
     nx = int(round(L / dx))
     nb = int(round(base / dx))
-    pts = []
-    for ix in range(nx):
-        x = (ix + 0.5) * dx
-        if x < crest * H:
-            top = H
-        elif x < (crest + 2.0) * H:
-            top = H - (x - crest * H) / 2.0
-        else:
-            top = 0.0
-        ny = nb + int(np.floor(top / dx))
-        for iy in range(ny):
-            pts.append((x, (iy + 0.5) * dx, 0.0))
-    soil = np.array(pts)
+    xs = (np.arange(nx) + 0.5) * dx
+    top = np.where(xs < crest * H, H, np.where(xs < (crest + 2.0) * H, H - (xs - crest * H) / 2.0, 0.0))
+    ny = nb + np.floor(top / dx).astype(np.int64)
+    ix = np.repeat(np.arange(nx), ny)
+    iy = np.concatenate([np.arange(k) for k in ny])
+    soil = np.zeros((ix.size, 3))
+    soil[:, 0] = (ix + 0.5) * dx
+    soil[:, 1] = (iy + 0.5) * dx
     hw = nb + int(np.ceil(H / dx)) + 2
     parts = [(soil, 2, 2), (_lattice2d(-walls, nx + walls, -walls, 0, dx), 0, 0),
              (_lattice2d(-walls, 0, 0, hw, dx), 0, 0), (_lattice2d(nx, nx + walls, 0, nb + 2, dx), 0, 0)]

@@ -261,3 +261,36 @@ def test_phase_entry_points_and_batches_with_all_passes():
     assert compare(a, b) == {}
     assert compare(a, c) == {}
     assert compare(a, e) == {}
+
+
+def test_batch_realisations_equal_standalone_runs():
+    # a realisation batch (one context, one cell grid per realisation) computes, for every
+    # realisation, exactly what that realisation computes alone, K-L cohesion included
+    from sphgpy import slope_2d
+    from sphgpy.capi import GpuBatch, batch_state, kl_field
+    decks = [slope_2d(h, dx=0.25, seed=s) for h, s in ((1.0, 1), (1.6, 2), (0.8, 3))]
+    rng = np.random.default_rng(5)
+    fields = [kl_field(d, dist=2, mean=10000.0, sd=3000.0, rl=(1.0, 0.3)) for d in decks]
+    xis = [rng.standard_normal(200) for _ in decks]
+    b = GpuBatch(decks)
+    b.load_state(batch_state(decks))
+    b.upload_done()
+    for r in range(len(decks)):
+        b.kl_synthesize_part(r, fields[r], xis[r])
+    b.set_step(1)
+    assert b.step(6) == 0
+    cb, ib = b.neighbors()
+    for r, d in enumerate(decks):
+        g = GpuSim(d)
+        g.load_state(setting_par(d))
+        g.upload_done()
+        g.kl_synthesize(fields[r], xis[r])
+        g.set_step(1)
+        assert g.step(6) == 0
+        for k in ("xp", "vx", "sig", "rho", "eps", "ps_c"):
+            assert bitequal(b.part(k, r), g.get(k)), (r, k)
+        a, z = int(b.offsets[r]), int(b.offsets[r + 1])
+        cg, ig = g.neighbors()
+        assert np.array_equal(cb[a:z], cg)
+        ib_r = ib[a:z]
+        assert np.array_equal(np.where(ib_r >= 0, ib_r - a, -1), ig)
