@@ -43,6 +43,8 @@ enum {
 
 enum { SPHG_FP64 = 0, SPHG_MIXED = 1 };
 
+#define SPHG_BUSY (-1)       /* sphg_poll: the context still has work in flight */
+
 /* Problem parameters: Para_FL (Class_Problem.h:10-42), Para_Pro (:44-84), the free-surface
  * thresholds read with Para_Pro (CPP_2-initialf.cpp:75), Para_GF (:150-162), Para_Boundary
  * (:118-133), Para_Inter (:104-116), Para_SSInter (:176-192), Para_Model flags
@@ -188,6 +190,11 @@ int sphg_update(sphg_ctx *ctx);
  * sequence is captured once into a hipGraph and replayed. Returns the sticky error word. */
 int sphg_step(sphg_ctx *ctx, int nsteps);
 int sphg_sync(sphg_ctx *ctx);
+/* Serving (no counterpart in the reference, which runs one simulation per process): enqueue
+ * nsteps steps on the context's stream and return at once; sphg_poll returns SPHG_BUSY while they
+ * run, then the sticky error word. Contexts on one device overlap their work. */
+int sphg_step_async(sphg_ctx *ctx, int nsteps);
+int sphg_poll(sphg_ctx *ctx);
 int sphg_current_step(const sphg_ctx *ctx);
 /* Sets the step counter pPPro->l for the next step (CPP_1-SPHLoops.cpp:478). */
 int sphg_set_step(sphg_ctx *ctx, int l);

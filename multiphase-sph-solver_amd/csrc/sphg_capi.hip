@@ -840,6 +840,34 @@ extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
     return read_err(c);
 }
 
+// Serving: steps are enqueued on the context's stream without waiting, so several contexts'
+// work overlaps on the device; sphg_poll reports whether the stream has drained.
+extern "C" int sphg_step_async(sphg_ctx *c, int nsteps) {
+    if (!c || !c->uploaded || nsteps < 0 || c->timing) return SPHG_E_ARG;
+    ensure_regu(c, nsteps);
+    for (int s = 0; s < nsteps; s++) {
+        int par = c->cur;
+        plan_pre(c);
+        if (!c->graph_ok[par]) {
+            int rc = capture(c, par);
+            if (rc) return rc;
+        }
+        HIPCHK(hipGraphLaunch(c->graph[par], c->stream));
+        c->cur ^= 1;
+        c->step++;
+    }
+    c->stepped = c->stepped || nsteps > 0;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_poll(sphg_ctx *c) {
+    if (!c) return SPHG_E_ARG;
+    hipError_t q = hipStreamQuery(c->stream);
+    if (q == hipErrorNotReady) return SPHG_BUSY;
+    if (q != hipSuccess) return SPHG_E_HIP;
+    return read_err(c);
+}
+
 extern "C" int sphg_sync(sphg_ctx *c) {
     if (!c) return SPHG_E_ARG;
     return read_err(c);

@@ -316,6 +316,10 @@ def product_lib() -> _Lib:
         L.sphg_create_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Problem), C.POINTER(Materials), _I,
                                         C.POINTER(C.c_int), C.POINTER(Box), _I]
         L.sphg_create_batch.restype = _I
+        L.sphg_step_async.argtypes = [C.c_void_p, _I]
+        L.sphg_step_async.restype = _I
+        L.sphg_poll.argtypes = [C.c_void_p]
+        L.sphg_poll.restype = _I
         for nm in ("parttocell", "partisearching", "density", "strain", "strain_to_stress", "accel", "xsph",
                    "update"):
             f = getattr(L, "sphg_" + nm)
