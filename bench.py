@@ -14,6 +14,10 @@ Extra fields on the JSON line:
   roofline      dominant kernel: algorithmic bytes per launch / measured average duration vs
                 8 TB/s; `traffic` = HBM bytes per launch from the committed rocprofv3 PMC summary
                 (profiles/pmc_<workload>.json) when present, else null.
+  mixed         the same workload in the mixed arithmetic form (SPHG_MIXED: FP32 kernel values and
+                hardware reciprocals in the pair terms, FP64 sums and state), same warm-up and
+                steps from the same initial state: its rate, speed-up over FP64 and field-max
+                relative error against the FP64 context after the same number of steps.
   cpu_baseline  the reference solver itself (compiled from its sources into oracle/_ref) timed on
                 this host's cores on a bounded sample of the same deck; where that build is absent,
                 the FP64 oracle (a restatement of the reference loop, validated bit-exact against it).
@@ -163,6 +167,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time the hipGraph replay path (no per-kernel events)")
     ap.add_argument("--nreal", type=int, default=256, help="realisations of the slope_batch workload")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
@@ -170,6 +175,7 @@ def main():
         sys.exit(2)
 
     from sphgpy.capi import GpuSim, GpuBatch, batch_state, kl_field, setting_par
+    SPHG_FP64, SPHG_MIXED = 0, 1
 
     if args.workload == "serve":
         from sphgpy.serve import Server, request_stream
@@ -190,41 +196,48 @@ def main():
         print(json.dumps(line), flush=True)
         return
 
-    extra = {}
-    if args.workload == "slope_batch":
-        decks, xis = make_batch(args.nreal)
-        g = GpuBatch(decks)
-        g.load_state(batch_state(decks))
-        g.upload_done()
-        t_kl = time.perf_counter()
-        for r, d in enumerate(decks):
-            g.kl_synthesize_part(r, kl_field(d, dist=2, mean=10000.0, sd=3000.0, rl=(10.0, 2.0)), xis[r])
+    def build(precision):
+        """Context of the workload in the given arithmetic form, state resident, step counter 1."""
+        extra = {}
+        if args.workload == "slope_batch":
+            decks, xis = make_batch(args.nreal)
+            g = GpuBatch(decks, precision=precision)
+            g.load_state(batch_state(decks))
+            g.upload_done()
+            t_kl = time.perf_counter()
+            for r, d in enumerate(decks):
+                g.kl_synthesize_part(r, kl_field(d, dist=2, mean=10000.0, sd=3000.0, rl=(10.0, 2.0)), xis[r])
+            g.sync()
+            extra["realisations"] = len(decks)
+            extra["kl_synthesis_s"] = round(time.perf_counter() - t_kl, 3)
+            extra["particles_per_realisation"] = [int(min(d.n for d in decks)), int(max(d.n for d in decks))]
+            deck, state = decks[0], None
+        else:
+            deck = make_deck(args.workload)
+            state = setting_par(deck)
+            g = GpuSim(deck, precision=precision)
+            g.load_state(state)
+            g.upload_done()
+        g.set_step(1)
+        return g, deck, state, extra
+
+    def timed(g, per_kernel):
+        rc = g.step(args.warmup)
+        if rc:
+            raise SystemExit(f"warm-up failed with error {rc}")
+        g.enable_timing(per_kernel)
         g.sync()
-        extra["realisations"] = len(decks)
-        extra["kl_synthesis_s"] = round(time.perf_counter() - t_kl, 3)
-        extra["particles_per_realisation"] = [int(min(d.n for d in decks)), int(max(d.n for d in decks))]
-        deck = decks[0]
-        state = None
-    else:
-        deck = make_deck(args.workload)
-        state = setting_par(deck)
-        g = GpuSim(deck)
-        g.load_state(state)
-        g.upload_done()
-    g.set_step(1)
-    rc = g.step(args.warmup)
-    if rc:
-        raise SystemExit(f"warm-up failed with error {rc}")
+        t0 = time.perf_counter()
+        rc = g.step(args.steps)  # returns after the stream is synchronised
+        t1 = time.perf_counter()
+        if rc:
+            raise SystemExit(f"timed steps failed with error {rc}")
+        return t1 - t0
+
+    g, deck, state, extra = build(SPHG_FP64)
+    el = timed(g, not args.graph)
     cnt, _ = g.neighbors()
     nbar = float(cnt.mean())
-    g.enable_timing(not args.graph)
-    g.sync()
-    t0 = time.perf_counter()
-    rc = g.step(args.steps)  # returns after the stream is synchronised
-    t1 = time.perf_counter()
-    if rc:
-        raise SystemExit(f"timed steps failed with error {rc}")
-    el = t1 - t0
     n = g.n
     value = n * args.steps / el
     ms_per_step = el * 1e3 / args.steps
@@ -254,6 +267,24 @@ def main():
     if not finite:
         raise SystemExit("state became non-finite during the timed steps")
 
+    mixed = None
+    if not args.no_mixed:
+        # the same workload in the mixed form, same warm-up and step count from the same initial
+        # state; its error is taken against the FP64 context after the same number of steps
+        gm, *_ = build(SPHG_MIXED)
+        elm = timed(gm, not args.graph)
+        err = {}
+        for k in ("xp", "vx", "sig", "rho"):
+            a, b = gm.get(k), g.get(k)
+            err[k] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+        mrows = gm.kernel_times() if not args.graph else []
+        mixed = {"value": round(n * args.steps / elm, 1), "ms_per_step": round(elm * 1e3 / args.steps, 4),
+                 "speedup_vs_fp64": round(el / elm, 3),
+                 "max_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()},
+                 "err_horizon_steps": args.warmup + args.steps,
+                 "kernels": {name: {"avg_us": ms * 1e3 / la, "launches": la} for name, ms, la in mrows}}
+        gm.close()
+
     line = {
         "metric": "particle-steps/s at 1 GPU, mixed vs FP64 at 1e-4 rel err; p99 time-to-result",
         "value": round(value, 1),
@@ -276,6 +307,7 @@ def main():
         "step_alg_bytes": int(step_alg),
         "step_roofline_frac": round(step_alg * args.steps / el / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernels,
+        "mixed": mixed,
     }
     if not args.no_cpu:
         # for the batch: the first realisation alone (the reference runs one deck per process)

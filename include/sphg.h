@@ -41,6 +41,10 @@ enum {
     SPHG_E_NOCONVERGE = 104  /* K-L root bisection cannot reach |f| <= 1e-6 in FP64 (the reference never returns) */
 };
 
+/* Arithmetic form of a context. SPHG_FP64 reproduces the reference's FP64 operations in their
+ * order. SPHG_MIXED keeps all state and every neighbour sum in FP64 but evaluates distance,
+ * kernel value and gradient of each pair in FP32 (separation formed in FP64 first) and the
+ * divisions inside pair terms with the hardware reciprocal plus one Newton step. */
 enum { SPHG_FP64 = 0, SPHG_MIXED = 1 };
 
 #define SPHG_BUSY (-1)       /* sphg_poll: the context still has work in flight */

@@ -110,7 +110,7 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     if (capacity <= 0) return SPHG_E_NOPARTICLES;
     if (capacity > (1 << 28)) return SPHG_E_CAPACITY;  // list entries pack the type above 28 id bits
     if (pro->ndim != 2 && pro->ndim != 3) return SPHG_E_ARG;
-    if (precision != SPHG_FP64) return SPHG_E_UNSUPPORTED;
+    if (precision != SPHG_FP64 && precision != SPHG_MIXED) return SPHG_E_ARG;
     sphg_ctx *c = new sphg_ctx();
     c->pro = *pro;
     c->mat = *mat;
@@ -671,20 +671,20 @@ static void enqueue_nnps(sphg_ctx *c) {
     LAUNCH("search", k_search<NDIM>, grid_for(n), c->S, c->P, cur);
 }
 
-template <int NDIM>
+template <int NDIM, class M>
 static void enqueue_step(sphg_ctx *c, int phase_mask) {
     const int cur = c->cur, n = c->n;
     if (phase_mask & 1) enqueue_nnps<NDIM>(c);
     if (phase_mask & 2) {
-        LAUNCH("density", k_density<NDIM>, grid_for(n), c->S, c->P, cur);
-        LAUNCH("norm_eos", k_norm<NDIM>, grid_for(n), c->S, c->P, cur);
+        LAUNCH("density", (k_density<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        LAUNCH("norm_eos", (k_norm<NDIM, M>), grid_for(n), c->S, c->P, cur);
     }
     if (phase_mask & 4) {
-        if (c->P.flagum == 1) LAUNCH("saturation", k_saturation<NDIM>, grid_for(n), c->S, c->P, cur);
-        LAUNCH("strain", k_strain<NDIM>, grid_for(n), c->S, c->P, cur);
+        if (c->P.flagum == 1) LAUNCH("saturation", (k_saturation<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        LAUNCH("strain", (k_strain<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->regu_on) {
             LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 0);
-            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, cur, 0);
+            LAUNCH("regu", (k_regu<NDIM, M>), grid_for(n), c->S, c->P, cur, 0);
         }
     }
     if (phase_mask & 8) {
@@ -692,26 +692,27 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         if (c->regu_on) {
             LAUNCH("stress", ks, grid_for(n), c->S, c->P, 1);
             LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 1);
-            LAUNCH("regu", k_regu<NDIM>, grid_for(n), c->S, c->P, cur, 1);
+            LAUNCH("regu", (k_regu<NDIM, M>), grid_for(n), c->S, c->P, cur, 1);
             LAUNCH("stress_post", ks, grid_for(n), c->S, c->P, 2);
         } else {
             LAUNCH("stress", ks, grid_for(n), c->S, c->P, 0);
         }
     }
     if (phase_mask & 16) {
-        if (c->pre_on) LAUNCH("accel_pre", k_accel_pre<NDIM>, grid_for(n), c->S, c->P, cur);
-        LAUNCH("accel", k_accel<NDIM>, grid_for(n), c->S, c->P, cur);
+        if (c->pre_on) LAUNCH("accel_pre", (k_accel_pre<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        LAUNCH("accel", (k_accel<NDIM, M>), grid_for(n), c->S, c->P, cur);
     }
-    if (phase_mask & 32) LAUNCH("xsph", k_xsph<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (phase_mask & 32) LAUNCH("xsph", (k_xsph<NDIM, M>), grid_for(n), c->S, c->P, cur);
     if (phase_mask & 64) {
-        LAUNCH("update", k_update<NDIM>, grid_for(n), c->S, c->P, cur);
+        LAUNCH("update", (k_update<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("step_inc", k_step_inc, 1, c->S);
     }
 }
 
 static void enqueue(sphg_ctx *c, int mask) {
-    if (c->P.ndim == 2) enqueue_step<2>(c, mask);
-    else enqueue_step<3>(c, mask);
+    const bool mixed = c->precision == SPHG_MIXED;
+    if (c->P.ndim == 2) mixed ? enqueue_step<2, Fast>(c, mask) : enqueue_step<2, Exact>(c, mask);
+    else mixed ? enqueue_step<3, Fast>(c, mask) : enqueue_step<3, Exact>(c, mask);
 }
 
 static int read_err(sphg_ctx *c) {

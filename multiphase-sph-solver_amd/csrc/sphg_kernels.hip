@@ -123,6 +123,28 @@ __device__ __forceinline__ void nb_batches(const State &S, const Params &P, int 
 // d2 (the squared distance before the root) and dst are returned for the passes that reuse them.
 struct KernelCtx {
     double ra, a1;
+    double r2;             // (2 ra)^2, the squared support radius (mixed form)
+    float ra_f, inv_ra_f, a1_f;
+};
+
+// Arithmetic of the pair passes.
+//   Exact: the reference's FP64 expressions, IEEE division and square root, reference order.
+//   Fast (SPHG_MIXED): the separation x_i - x_j is formed in FP64 (absolute positions need it)
+//   and rounded to FP32; distance, kernel value and gradient are FP32 with the hardware square
+//   root and reciprocal; divisions inside the pair terms use the FP64 hardware reciprocal with
+//   one Newton step (relative error ~1e-15 instead of correctly rounded). Every sum over the
+//   neighbours, and all per-particle state, stay FP64.
+struct Exact {
+    static constexpr bool kFast = false;
+    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+};
+struct Fast {
+    static constexpr bool kFast = true;
+    __device__ static __forceinline__ double div(double a, double b) {
+        double r = __builtin_amdgcn_rcp(b);
+        r = fma(fma(-b, r, 1.0), r, r);
+        return a * r;
+    }
 };
 
 template <int NDIM>
@@ -136,13 +158,64 @@ __device__ __forceinline__ KernelCtx kernel_ctx(const Params &P, double hli) {
     KernelCtx k;
     k.ra = (hli + hli) / 2;
     k.a1 = kernel_a1<NDIM>(P, k.ra);
+    k.r2 = (2.0 * k.ra) * (2.0 * k.ra);
+    k.ra_f = (float)k.ra;
+    k.inv_ra_f = (float)(1.0 / k.ra);
+    k.a1_f = (float)k.a1;
     return k;
 }
 
+// Mixed form of pair_kernel below: same branches, FP32 arithmetic; d2 stays the FP64 sum.
+// Every listed pair has s <= 2 in FP64 (the search predicate). Pairs on the cut-off, s = 2 - t
+// with t ~ 1e-7 (lattice decks), carry tiny but nonzero FP64 weights, and some ratios of sums
+// (eos and wall-stress interpolation: x / d if d != 0) switch on exactly that; 2 - s formed in
+// FP32 would round such t to 0. So t comes from the FP64 difference of squares,
+// t = (r2 - d2) / (ra (2 ra + dst)), which keeps its relative precision, clamped at 0.
 template <int NDIM>
+__device__ __forceinline__ void pair_kernel_fast(const Params &P, const KernelCtx &kc, const double xi[3],
+                                                 const double xj[3], double w[4], double &d2, double &dst) {
+    float dx[3];
+    float d2f = 0.0f;
+    d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        const double dd = xi[k] - xj[k];
+        d2 = d2 + dd * dd;
+        dx[k] = (float)dd;
+        d2f = d2f + dx[k] * dx[k];
+    }
+    const float df = __builtin_amdgcn_sqrtf(d2f);
+    const float a1 = kc.a1_f;
+    const float s = df * kc.inv_ra_f;
+    const bool bspline = P.flknl == 1 || P.flknl == 3;
+    const bool near = s > 0.0f && s <= 1.0f, far = s > 1.0f && d2 <= kc.r2 * 1.00000000000001;
+    const float tc = (float)(kc.r2 - d2) * kc.inv_ra_f * __builtin_amdgcn_rcpf(2.0f * kc.ra_f + df);
+    const float t = tc > 0.0f ? tc : 0.0f, ird = kc.inv_ra_f * __builtin_amdgcn_rcpf(df);
+    const float cn = bspline ? (-2.0f * s + 1.5f * s * s) : (-6.0f + 3.0f * s * s);
+    const float cf = bspline ? -0.5f : -3.0f;
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) {
+        const float num = near ? a1 * dx[k] * cn : a1 * cf * dx[k] * t * t;
+        w[k] = (near || far) ? (double)(num * ird) : 0.0;
+    }
+    if (NDIM == 2) w[2] = 0.0;
+    const float vn = bspline ? a1 * (0.66666667f - s * s + 0.5f * s * s * s) : a1 * (6.0f - 6.0f * s + s * s * s);
+    const float vf = a1 * t * t * t;
+    float v = 0.0f;
+    if (s == 0.0f || near) v = vn;
+    else if (far) v = bspline ? vf * (1.0f / 6.0f) : vf;
+    w[3] = (double)v;
+    dst = (double)df;
+}
+
+template <int NDIM, class M>
 __device__ __forceinline__ void pair_kernel(const Params &P, const KernelCtx &kc, const double xi[3],
                                             const double xj[3], double hli, double hlj, double w[4], double &d2,
                                             double &dst) {
+    if constexpr (M::kFast) {
+        pair_kernel_fast<NDIM>(P, kc, xi, xj, w, d2, dst);
+        return;
+    }
     double dx[3];
     d2 = 0.0;
 #pragma unroll
@@ -392,7 +465,7 @@ __device__ __forceinline__ void nb_vel(const State &S, int j, bool bnd, int mode
 // density loops 1-2 (CPP_6-density.cpp:22-166); water porosity (loop 1) and continuity (loop 2)
 // accumulate side by side in one walk over the list.
 // ------------------------------------------------------------------------------------------
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -440,10 +513,10 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
             const bool same = tj == ti, bnd = tj == 0;
             if (!(same || bnd || (water && tj == 2))) continue;
             double w[4], d2, dst;
-            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
             if (water && tj == 2) {  // loop 1: water porosity from soil neighbours
-                pup = pup + mj[b] * porj[b] / rj[b] * w[3];
-                pdn = pdn + mj[b] / rj[b] * w[3];
+                pup = pup + M::div(mj[b] * porj[b], rj[b]) * w[3];
+                pdn = pdn + M::div(mj[b], rj[b]) * w[3];
             }
             if (same || bnd) {  // loop 2: continuity
 #pragma unroll
@@ -484,7 +557,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
 }
 
 // density loop 3 (Jacobi form of CPP_6-density.cpp:168-188) and eos loop 1 (CPP_7-1-eos_CR.cpp:20-43)
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -510,9 +583,9 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
         for (int b = 0; b < B; b++) {
             if (b >= act || nb_type(e[b]) != ti) continue;
             double w[4], d2, dst;
-            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
             up = up + mj[b] * w[3];
-            dn = dn + mj[b] / rmj[b] * w[3];
+            dn = dn + M::div(mj[b], rmj[b]) * w[3];
         }
     });
     double r = S.rho_mid[i];
@@ -542,7 +615,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
 // flagum == 1. It reads only masses, densities and kernel values and writes satu / dsatu / suct,
 // which the strain pass reads only for the permeability of the same particle, so it runs
 // just before that pass.
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n || S.type[i] != 2) return;
@@ -569,7 +642,7 @@ __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) 
         for (int b = 0; b < B; b++) {
             if (b >= act || nb_type(e[b]) != 1) continue;
             double w[4], d2, dst;
-            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
             sat_up = sat_up + mj[b] / rj[b] * sat_cof * w[3];
             sat_dn = sat_dn + mj[b] / rj[b] * w[3];
         }
@@ -595,7 +668,7 @@ __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) 
     }
 }
 
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -640,10 +713,10 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                 const int tj = nb_type(e[b]);
                 if (b >= act || (tj != 1 && tj != 2 && tj != 3)) continue;
                 double w[4], d2, dst;
-                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
                 const double w3 = w[3];
                 if (tj == 1 || tj == 3) {
-                    const double tu = mj[b] * pj[b] * w3 / rj[b], td = mj[b] * w3 / rj[b];
+                    const double tu = M::div(mj[b] * pj[b] * w3, rj[b]), td = M::div(mj[b] * w3, rj[b]);
                     if (tj == 1) {
                         uw = uw + tu;
                         dw = dw + td;
@@ -653,8 +726,8 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                     }
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 6; k++) us[k] = us[k] + mj[b] * sgj[b][k] * w3 / rj[b];
-                    ds = ds + mj[b] * w3 / rj[b];
+                    for (int k = 0; k < 6; k++) us[k] = us[k] + M::div(mj[b] * sgj[b][k] * w3, rj[b]);
+                    ds = ds + M::div(mj[b] * w3, rj[b]);
                 }
             }
         });
@@ -678,7 +751,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                 const int tj = nb_type(e[b]);
                 const bool bnd = tj == 0;
                 double w[4], d2, dst;
-                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
                 // strain (CPP_7-1-strain.cpp:67-150)
                 if (ti == tj || (bnd && ti <= 4)) {
                     double dvx[3];
@@ -688,16 +761,16 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
 #pragma unroll
                     for (int a = 0; a < 3; a++)
 #pragma unroll
-                        for (int c = 0; c < 3; c++) up[3 * a + c] = up[3 * a + c] + mj[b] / rj[b] * dvx[a] * w[c];
+                        for (int c = 0; c < 3; c++) up[3 * a + c] = up[3 * a + c] + M::div(mj[b], rj[b]) * dvx[a] * w[c];
                     if (solid) {
 #pragma unroll
-                        for (int k = 0; k < 3; k++) dnm[k] = dnm[k] + mj[b] / rj[b] * (xi[k] - xj[b][k]) * w[k];
+                        for (int k = 0; k < 3; k++) dnm[k] = dnm[k] + M::div(mj[b], rj[b]) * (xi[k] - xj[b][k]) * w[k];
                     }
                 }
                 // eos loop 2 (CPP_7-1-eos_CR.cpp:62-118): soil and structure particles
                 if (solid && (tj == 1 || tj == 3)) {
                     const double w3 = w[3];
-                    const double tu = mj[b] * pj[b] * w3 / rj[b], td = mj[b] * w3 / rj[b];
+                    const double tu = M::div(mj[b] * pj[b] * w3, rj[b]), td = M::div(mj[b] * w3, rj[b]);
                     if (tj == 1) {
                         uw = uw + tu;
                         dw = dw + td;
@@ -707,7 +780,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                     }
                 }
                 // colour (CPP_7-2-constitutive.cpp:467-478)
-                if (ti == tj || (bnd && ti <= 4)) color = color + mj[b] / rj[b] * 1.0 * w[3];
+                if (ti == tj || (bnd && ti <= 4)) color = color + M::div(mj[b], rj[b]) * 1.0 * w[3];
             }
         });
     }
@@ -1417,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_regu_copy(State S, Params P, int which_
     for (int k = 0; k < 6; k++) S.tmp6[k][i] = which_sig ? S.sig[k][i] : S.eps[k][i];
 }
 
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_regu(State S, Params P, int cur, int which_sig) {
     int i = pidx();
     if (i >= P.n || (step_of(S) % P.step_regu) != 0) return;
@@ -1445,7 +1518,7 @@ __global__ __launch_bounds__(256) void k_regu(State S, Params P, int cur, int wh
         for (int b = 0; b < B; b++) {
             if (b >= act || nb_type(e[b]) != ti) continue;
             double w[4], d2, dst;
-            pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+            pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
             #pragma unroll
             for (int k = 0; k < 6; k++) up[k] = up[k] + mj[b] / rj[b] * w[3] * vj[b][k];
             dn = dn + mj[b] / rj[b] * w[3];
@@ -1463,7 +1536,7 @@ __device__ __constant__ int kSidx[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
 
 // artificial_viscosity pair term (CPP_9-3-artificialvis.cpp:36-89); dst is sqrt of the same
 // squared distance the kernel value was built from.
-template <int NDIM>
+template <int NDIM, class M>
 __device__ __forceinline__ void av_pair(const double vi[3], const double vj[3], const double xi[3],
                                         const double xj[3], double dst, double hli, double hlj, double rhoi,
                                         double rj, double mj, double av_a, double av_b, double av_c,
@@ -1479,8 +1552,8 @@ __device__ __forceinline__ void av_pair(const double vi[3], const double vj[3], 
     const double rou = (rj + rhoi) * 0.5;
     double paix;
     if (inprod < 0.0) {
-        double myu = hl * inprod / (dst * dst + 0.01 * hl * hl);
-        paix = (-av_a * av_c * myu + av_b * myu * myu) / rou;
+        double myu = M::div(hl * inprod, dst * dst + 0.01 * hl * hl);
+        paix = M::div(-av_a * av_c * myu + av_b * myu * myu, rou);
     } else {
         paix = 0.0;
     }
@@ -1489,6 +1562,7 @@ __device__ __forceinline__ void av_pair(const double vi[3], const double vj[3], 
 }
 
 // boundary_moving_stress_effect pair term (CPP_9-4-derivation_stress.cpp:631-721); bi = as / rho_i^2
+template <class M>
 __device__ __forceinline__ void bnd_pair(const double as[6], const double bi[6], double mj, double rj,
                                          const double w[4], double bnd_t[3][3]) {
 #pragma unroll
@@ -1496,7 +1570,7 @@ __device__ __forceinline__ void bnd_pair(const double as[6], const double bi[6],
 #pragma unroll
         for (int b = 0; b < 3; b++) {
             const int sx = kSidx[a][b];
-            bnd_t[a][b] = bnd_t[a][b] + mj * (as[sx] / rj / rj + bi[sx]) * w[b];
+            bnd_t[a][b] = bnd_t[a][b] + mj * (M::div(M::div(as[sx], rj), rj) + bi[sx]) * w[b];
         }
 }
 
@@ -1504,7 +1578,7 @@ __device__ __forceinline__ void bnd_pair(const double as[6], const double bi[6],
 // artificial stress (soil, same-type pairs) into tmp6[0..2] and the geostress damping sum
 // (soil / structure, same-type pairs, l <= inip) into tmp6[3..5]. Split out so the momentum
 // kernel keeps its register budget when neither is active.
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -1543,7 +1617,7 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
         const double *xj = xjb[b], *Rj = Rjb[b], *vj = vjb[b];
         const double hlj = hljb[b], mj = mjb[b], rj = rjb[b];
         double w[4], d2, dst;
-        pair_kernel<NDIM>(P, kc, xi, xj, hli, hlj, w, d2, dst);
+        pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
         if (art_on) {
             // artificial_stress_2d / _3d pair sum (CPP_9-5-artificialstress.cpp:92-140, :230-317);
             // the distance is the square root of the same sum the kernel value used
@@ -1576,7 +1650,7 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
         for (int k = 0; k < 3; k++) S.tmp6[3 + k][i] = damp[k];
 }
 
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -1676,10 +1750,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             double *qj = qjb[b];
             const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satj = satjb[b], porj = porjb[b], copj = copjb[b];
             double w[4], d2, dst;
-            pair_kernel<NDIM>(P, kc, xi, xj, hli, hlj, w, d2, dst);
+            pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
             if (tj == ti || tj == fother) {
                 if (!P.fluid_por_uniform)
-                    for (int k = 0; k < 6; k++) qj[k] = (fpor * qj[k]) / rj / rj;
+                    for (int k = 0; k < 6; k++) qj[k] = M::div(M::div(fpor * qj[k], rj), rj);
 #pragma unroll
                 for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -1689,8 +1763,8 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                         else fluid_t[a][b] = fluid_t[a][b] + mj * (qj[sx] - fi[sx]) * w[b];
                     }
             }
-            if (tj == ti) av_pair<NDIM>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
-            if (tj == 0 && bstress) bnd_pair(as, bi, mj, rj, w, bnd_t);
+            if (tj == ti) av_pair<NDIM, M>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
+            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t);
             if (drag_on && tj == 2) {
                 // interact1 pair sum (CPP_10-interact1.cpp:40-88), water / air against soil
                 double inprod = 0.0, dxx[3], dv[3];
@@ -1705,11 +1779,11 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                     if (ti == 1) {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
-                            drag[k] = drag[k] + satj * satj * porj * porj * fdi * P.gacc * dv[k] / copj * mj / rj * w3;
+                            drag[k] = drag[k] + M::div(M::div(satj * satj * porj * porj * fdi * P.gacc * dv[k], copj) * mj, rj) * w3;
                     } else {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
-                            drag[k] = drag[k] + (1 - satj) * (1 - satj) * porj * porj * fdi * P.gacc * dv[k] * 2000 * mj / rj * w3;
+                            drag[k] = drag[k] + M::div((1 - satj) * (1 - satj) * porj * porj * fdi * P.gacc * dv[k] * 2000 * mj, rj) * w3;
                     }
                 }
             }
@@ -1748,7 +1822,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satu = satub[b], prewj = prewb[b],
                          preaj = preab[b], fdj = fdjb[b];
             double w[4], d2, dst;
-            pair_kernel<NDIM>(P, kc, xi, xj, hli, hlj, w, d2, dst);
+            pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
             if (tj == ti) {
 #pragma unroll
                 for (int a = 0; a < 3; a++)
@@ -1763,17 +1837,17 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                     double prej = (1 - spor) * (-satu * prewj + (1 - satu) * preaj);
 #pragma unroll
                     for (int k = 0; k < 3; k++)
-                        solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj + prei / sdens / sdens) * w[k];  // quirk A12
+                        solid_t[k][k] = solid_t[k][k] + mj * (M::div(M::div(prej, rj), rj) + M::div(M::div(prei, sdens), sdens)) * w[k];  // quirk A12
                 } else {
                     double prei = -satu * prewi + (1 - satu) * preai;
                     double prej = -satu * prewj + (1 - satu) * preaj;
 #pragma unroll
                     for (int k = 0; k < 3; k++)
-                        solid_t[k][k] = solid_t[k][k] + mj * (prej / rj / rj - prei / rhoi / rhoi) * w[k];
+                        solid_t[k][k] = solid_t[k][k] + mj * (M::div(M::div(prej, rj), rj) - M::div(M::div(prei, rhoi), rhoi)) * w[k];
                 }
-                av_pair<NDIM>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
+                av_pair<NDIM, M>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
             }
-            if (tj == 0 && bstress) bnd_pair(as, bi, mj, rj, w, bnd_t);
+            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t);
             if (drag_on && ti == 2 && (tj == 1 || tj == 3)) {
                 // interact1 pair sum (CPP_10-interact1.cpp:40-88), soil against water / air
                 double inprod = 0.0, dxx[3], dv[3];
@@ -1788,11 +1862,11 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                     if (tj == 1) {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
-                            drag[k] = drag[k] + satui * satui * pori * pori * fdj * P.gacc * dv[k] / copi * mj / rj * w3;
+                            drag[k] = drag[k] + M::div(M::div(satui * satui * pori * pori * fdj * P.gacc * dv[k], copi) * mj, rj) * w3;
                     } else {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
-                            drag[k] = drag[k] + (1 - satui) * (1 - satui) * pori * pori * fdj * P.gacc * dv[k] * 2000 * mj / rj * w3;
+                            drag[k] = drag[k] + M::div((1 - satui) * (1 - satui) * pori * pori * fdj * P.gacc * dv[k] * 2000 * mj, rj) * w3;
                     }
                 }
             }
@@ -1817,8 +1891,8 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             for (int b = 0; b < B; b++) {
                 if (b >= act || nb_type(e[b]) != 0) continue;
                 double w[4], d2, dst;
-                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
-                av_pair<NDIM>(vi, vj[b], xi, xj[b], dst, hli, hlj[b], rhoi, rj[b], mj[b], av_a, av_b, av_c, w, av);
+                pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                av_pair<NDIM, M>(vi, vj[b], xi, xj[b], dst, hli, hlj[b], rhoi, rj[b], mj[b], av_a, av_b, av_c, w, av);
             }
         });
     }
@@ -1945,7 +2019,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 // XSPH + velocity_set_domain + tm_boundary first loop
 // ------------------------------------------------------------------------------------------
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -1996,30 +2070,32 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
         const bool bnd = tj == 0;
         if (b >= act || (tj != ti && !bnd)) continue;
         double w[4], d2, dst;
-        pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+        pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
         if (do_xsph) {
             const double xs = tj == ti ? xsame : xbnd, w3 = w[3];
 #pragma unroll
-            for (int k = 0; k < NDIM; k++) tu[k] = tu[k] + xs * mj[b] * (vxj[b][k] - vi[k]) * w3 / rj[b];
-            dn = dn + mj[b] * w3 / rj[b];
+            for (int k = 0; k < NDIM; k++) tu[k] = tu[k] + M::div(xs * mj[b] * (vxj[b][k] - vi[k]) * w3, rj[b]);
+            dn = dn + M::div(mj[b] * w3, rj[b]);
         }
         if (tm && bnd) {
+            // the mixed form orders by the FP64 squared distance (its dst is an FP32 root)
+            const double key = M::kFast ? d2 : dst;
             if (mtj[b] > 2) idm = j;
             int x = j;
             if (total == 0) {
                 cur_id[0] = x;
-                d0[0] = dst;
+                d0[0] = key;
             } else {
                 int passes = total < NDIM ? total : NDIM;
                 for (int q = 0; q < passes; q++)
-                    if (d0[q] > dst) {
+                    if (d0[q] > key) {
                         int t = cur_id[q];
                         cur_id[q] = x;
                         x = t;
                     }
                 if (total < NDIM) {
                     cur_id[total] = x;
-                    d0[total] = dst;
+                    d0[total] = key;
                 }
             }
             total++;
@@ -2065,10 +2141,10 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
             if (inprod < 0.0) {
                 dst = sqrt(dst);
                 double db = coe * dr;
-                double beta = 1 + db / (dst - coe * dr);
+                double beta = 1 + M::div(db, dst - coe * dr);
                 beta = fmin(1.5, beta);
 #pragma unroll
-                for (int k = 0; k < NDIM; k++) t1[k] = t1[k] + (1 - beta) * (xi[k] - xjj[k]) * inprod / dst / dst;
+                for (int k = 0; k < NDIM; k++) t1[k] = t1[k] + M::div(M::div((1 - beta) * (xi[k] - xjj[k]) * inprod, dst), dst);
             }
         }
         });
@@ -2150,7 +2226,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 // tm_boundary second loop + check_domain + update
 // ------------------------------------------------------------------------------------------
-template <int NDIM>
+template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -2184,13 +2260,13 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
                 const int tj = nb_type(e[b]);
                 if (b >= act || tj < 1 || tj > 4) continue;
                 double w[4], d2, dst;
-                pair_kernel<NDIM>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
+                pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
                 const double w3 = w[3];
                 int t = tj - 1;
-                tv[t][0] = tv[t][0] + mj[b] * vbj[b][0] * w3 / rj[b];
-                tv[t][1] = tv[t][0] + mj[b] * vbj[b][1] * w3 / rj[b];  // quirk A17
-                tv[t][2] = tv[t][0] + mj[b] * vbj[b][2] * w3 / rj[b];
-                td[t] = td[t] + mj[b] * w3 / rj[b];
+                tv[t][0] = tv[t][0] + M::div(mj[b] * vbj[b][0] * w3, rj[b]);
+                tv[t][1] = tv[t][0] + M::div(mj[b] * vbj[b][1] * w3, rj[b]);  // quirk A17
+                tv[t][2] = tv[t][0] + M::div(mj[b] * vbj[b][2] * w3, rj[b]);
+                td[t] = td[t] + M::div(mj[b] * w3, rj[b]);
             }
         });
         for (int t = 0; t < 4; t++)
@@ -2262,25 +2338,43 @@ template __global__ void k_bin<2>(State, Params, int);
 template __global__ void k_bin<3>(State, Params, int);
 template __global__ void k_search<2>(State, Params, int);
 template __global__ void k_search<3>(State, Params, int);
-template __global__ void k_density<2>(State, Params, int);
-template __global__ void k_density<3>(State, Params, int);
-template __global__ void k_norm<2>(State, Params, int);
-template __global__ void k_norm<3>(State, Params, int);
-template __global__ void k_strain<2>(State, Params, int);
-template __global__ void k_strain<3>(State, Params, int);
-template __global__ void k_accel<2>(State, Params, int);
-template __global__ void k_accel<3>(State, Params, int);
-template __global__ void k_xsph<2>(State, Params, int);
-template __global__ void k_xsph<3>(State, Params, int);
-template __global__ void k_update<2>(State, Params, int);
-template __global__ void k_update<3>(State, Params, int);
-template __global__ void k_regu<2>(State, Params, int, int);
+template __global__ void k_density<2, Exact>(State, Params, int);
+template __global__ void k_density<2, Fast>(State, Params, int);
+template __global__ void k_density<3, Exact>(State, Params, int);
+template __global__ void k_density<3, Fast>(State, Params, int);
+template __global__ void k_norm<2, Exact>(State, Params, int);
+template __global__ void k_norm<2, Fast>(State, Params, int);
+template __global__ void k_norm<3, Exact>(State, Params, int);
+template __global__ void k_norm<3, Fast>(State, Params, int);
+template __global__ void k_strain<2, Exact>(State, Params, int);
+template __global__ void k_strain<2, Fast>(State, Params, int);
+template __global__ void k_strain<3, Exact>(State, Params, int);
+template __global__ void k_strain<3, Fast>(State, Params, int);
+template __global__ void k_accel<2, Exact>(State, Params, int);
+template __global__ void k_accel<2, Fast>(State, Params, int);
+template __global__ void k_accel<3, Exact>(State, Params, int);
+template __global__ void k_accel<3, Fast>(State, Params, int);
+template __global__ void k_xsph<2, Exact>(State, Params, int);
+template __global__ void k_xsph<2, Fast>(State, Params, int);
+template __global__ void k_xsph<3, Exact>(State, Params, int);
+template __global__ void k_xsph<3, Fast>(State, Params, int);
+template __global__ void k_update<2, Exact>(State, Params, int);
+template __global__ void k_update<2, Fast>(State, Params, int);
+template __global__ void k_update<3, Exact>(State, Params, int);
+template __global__ void k_update<3, Fast>(State, Params, int);
+template __global__ void k_regu<2, Exact>(State, Params, int, int);
+template __global__ void k_regu<2, Fast>(State, Params, int, int);
 template __global__ void k_stress<false>(State, Params, int);
 template __global__ void k_stress<true>(State, Params, int);
-template __global__ void k_saturation<2>(State, Params, int);
-template __global__ void k_saturation<3>(State, Params, int);
-template __global__ void k_accel_pre<2>(State, Params, int);
-template __global__ void k_accel_pre<3>(State, Params, int);
-template __global__ void k_regu<3>(State, Params, int, int);
+template __global__ void k_saturation<2, Exact>(State, Params, int);
+template __global__ void k_saturation<2, Fast>(State, Params, int);
+template __global__ void k_saturation<3, Exact>(State, Params, int);
+template __global__ void k_saturation<3, Fast>(State, Params, int);
+template __global__ void k_accel_pre<2, Exact>(State, Params, int);
+template __global__ void k_accel_pre<2, Fast>(State, Params, int);
+template __global__ void k_accel_pre<3, Exact>(State, Params, int);
+template __global__ void k_accel_pre<3, Fast>(State, Params, int);
+template __global__ void k_regu<3, Exact>(State, Params, int, int);
+template __global__ void k_regu<3, Fast>(State, Params, int, int);
 
 }  // namespace sphg

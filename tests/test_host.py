@@ -29,6 +29,19 @@ def test_library_exports_every_declared_symbol():
     assert missing == []
 
 
+def test_create_rejects_unknown_arithmetic_form():
+    # argument checks run before any device call
+    import ctypes as C
+    from sphgpy.capi import product_lib, materials_from_deck
+    deck = column_collapse_2d(nx=4, ny=4, wall_h=6, floor_len=8, steps=1, seed=0)
+    pro, mat = problem_from_deck(deck), materials_from_deck(deck)
+    ctx = C.c_void_p()
+    L = product_lib().lib
+    assert L.sphg_create(C.byref(ctx), C.byref(pro), C.byref(mat), deck.n, 2) == 100  # SPHG_E_ARG
+    assert L.sphg_create(C.byref(ctx), C.byref(pro), C.byref(mat), deck.n, -1) == 100
+    assert not ctx.value
+
+
 @pytest.mark.parametrize("name", ["column2d_small", "soilwater3d_small"])
 def test_setting_par_matches_reference_init(name):
     deck, states = load_golden(name)
