@@ -67,6 +67,7 @@ struct Params {
     int hl_uniform;          // every particle has the same smoothing length (hl_j == hl_i bit for bit)
     int nreal;               // realisations (cell grids) in the context
     int fluid_por_uniform;   // every fluid particle's slot has the same porosity
+    double ss_myu, ss_zeta;  // Para_SSInter friction coefficient and force factor
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -93,6 +94,10 @@ struct State {
     double *fdens;        // density of the particle's fluid slot (drag partner density)
     double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
     double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
+    // soil-structure interaction (ss_type 1-3): per soil / water particle the structure particles
+    // its contact force is passed to (-1: none) and what each receives (f_n[3], f_t[3])
+    int *ssi_id[3];
+    double *ssi_c[3][6];
     int *err, *step;      // sticky error word; current step number l
     const Grid *grids;    // one per realisation
     const int *rid;       // realisation of each particle

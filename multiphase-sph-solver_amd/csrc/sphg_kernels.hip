@@ -1650,6 +1650,95 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
         for (int k = 0; k < 3; k++) S.tmp6[3 + k][i] = damp[k];
 }
 
+// The end of the momentum pass for particle i, from its summed acceleration ax: the geostress
+// damping clamp and veocity_update (CPP_11-xsph.cpp:11-142); stores ax, vx (and the moving
+// boundary's interfss).
+template <int NDIM>
+__device__ __forceinline__ void accel_tail(const State &S, const Params &P, int i, int ti, int cur, double ax[3],
+                                           const double vi[3], const double axp[3], bool damp_on) {
+    const double dt = P.dt;
+    if (damp_on) {
+        // soil_damping_geostress clamp (CPP_9-4-derivation_stress.cpp:874-895)
+        double damp[3];
+        #pragma unroll
+        for (int k = 0; k < NDIM; k++) damp[k] = S.tmp6[3 + k][i];
+        #pragma unroll
+        for (int k = 0; k < NDIM; k++) {
+            double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * dt;
+            double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - damp[k]) * dt;
+            if (vx1 * vx2 <= 0.0) ax[k] = -2.0 * vi[k] / dt - axp[k];
+            else ax[k] -= damp[k];
+        }
+    }
+    // veocity_update (CPP_11-xsph.cpp:11-142)
+    if (P.if_move == 1) {
+        if (ti == 0) {
+            const bool mv = S.matype[i] == 2 && step_of(S) > P.inip;
+            #pragma unroll
+            for (int k = 0; k < NDIM; k++) S.interfss[k][i] = mv ? P.move_vx[k] : 0.0;
+        } else {
+            #pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                double ath = (axp[k] + ax[k]) * 0.5;
+                S.vx[cur][k][i] = vi[k] + ath * dt;
+            }
+        }
+        #pragma unroll
+        for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
+        return;
+    }
+    if (P.if_move == 2 || P.if_move == 3) {
+        // the driven particles take the prescribed velocity; their vxp is replaced in the update
+        // kernel (neighbours still read the old vxp during this pass)
+        const int mt = P.if_move == 2 ? 2 : 4;
+        double vp[3] = {vi[0], vi[1], vi[2]}, ap[3] = {axp[0], axp[1], axp[2]};
+        if (ti == mt && S.etype[i] == 5 && step_of(S) > P.inip) {
+            #pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                ap[k] = 0.0;
+                vp[k] = P.move_vx[k];
+                ax[k] = 0.0;
+                S.axp[k][i] = 0.0;
+            }
+        }
+        #pragma unroll
+        for (int k = 0; k < NDIM; k++) {
+            double ath = (ap[k] + ax[k]) * 0.5;
+            S.vx[cur][k][i] = vp[k] + ath * dt;
+        }
+        #pragma unroll
+        for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
+        return;
+    }
+    #pragma unroll
+    for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
+    // if_move == 0
+    if (ti == 0) {
+        int mt = S.matype[i];
+        if (mt < 2) {
+            #pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                S.interf[k][i] = 0.0;
+                double ath = (0.0 + 0.0) * 0.5;
+                S.interfss[k][i] = S.interfss[k][i] + ath * dt;
+            }
+        } else if (mt == 3 || (mt == 5 && P.flv == 1)) {
+            #pragma unroll
+            for (int k = 0; k < NDIM; k++) {
+                double f = S.interf[k][i];
+                double ath = (f + f) * 0.5;
+                S.interfss[k][i] = S.interfss[k][i] + ath * dt;
+            }
+        }
+    } else {
+        #pragma unroll
+        for (int k = 0; k < NDIM; k++) {
+            double ath = (axp[k] + ax[k]) * 0.5;
+            S.vx[cur][k][i] = vi[k] + ath * dt;
+        }
+    }
+}
+
 template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     int i = pidx();
@@ -1934,86 +2023,14 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             S.interf[k][i] = itf;
         }
     }
-    if (damp_on) {
-        // soil_damping_geostress clamp (CPP_9-4-derivation_stress.cpp:874-895)
-        double damp[3];
-        #pragma unroll
-        for (int k = 0; k < NDIM; k++) damp[k] = S.tmp6[3 + k][i];
-        #pragma unroll
-        for (int k = 0; k < NDIM; k++) {
-            double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * dt;
-            double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - damp[k]) * dt;
-            if (vx1 * vx2 <= 0.0) ax[k] = -2.0 * vi[k] / dt - axp[k];
-            else ax[k] -= damp[k];
-        }
-    }
-    // veocity_update (CPP_11-xsph.cpp:11-142)
-    if (P.if_move == 1) {
-        if (ti == 0) {
-            const bool mv = S.matype[i] == 2 && step_of(S) > P.inip;
-            #pragma unroll
-            for (int k = 0; k < NDIM; k++) S.interfss[k][i] = mv ? P.move_vx[k] : 0.0;
-        } else {
-            #pragma unroll
-            for (int k = 0; k < NDIM; k++) {
-                double ath = (axp[k] + ax[k]) * 0.5;
-                S.vx[cur][k][i] = vi[k] + ath * dt;
-            }
-        }
+    if (P.ss_type != 0) {
+        // soil-structure interaction adds to ax before the damping clamp and the velocity update
+        // (CPP_1-SPHLoops.cpp:662-694): k_ssi_* and k_accel_tail finish the pass
         #pragma unroll
         for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
         return;
     }
-    if (P.if_move == 2 || P.if_move == 3) {
-        // the driven particles take the prescribed velocity; their vxp is replaced in the update
-        // kernel (neighbours still read the old vxp during this pass)
-        const int mt = P.if_move == 2 ? 2 : 4;
-        double vp[3] = {vi[0], vi[1], vi[2]}, ap[3] = {axp[0], axp[1], axp[2]};
-        if (ti == mt && S.etype[i] == 5 && step_of(S) > P.inip) {
-            #pragma unroll
-            for (int k = 0; k < NDIM; k++) {
-                ap[k] = 0.0;
-                vp[k] = P.move_vx[k];
-                ax[k] = 0.0;
-                S.axp[k][i] = 0.0;
-            }
-        }
-        #pragma unroll
-        for (int k = 0; k < NDIM; k++) {
-            double ath = (ap[k] + ax[k]) * 0.5;
-            S.vx[cur][k][i] = vp[k] + ath * dt;
-        }
-        #pragma unroll
-        for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
-        return;
-    }
-    #pragma unroll
-    for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
-    // if_move == 0
-    if (ti == 0) {
-        int mt = S.matype[i];
-        if (mt < 2) {
-            #pragma unroll
-            for (int k = 0; k < NDIM; k++) {
-                S.interf[k][i] = 0.0;
-                double ath = (0.0 + 0.0) * 0.5;
-                S.interfss[k][i] = S.interfss[k][i] + ath * dt;
-            }
-        } else if (mt == 3 || (mt == 5 && P.flv == 1)) {
-            #pragma unroll
-            for (int k = 0; k < NDIM; k++) {
-                double f = S.interf[k][i];
-                double ath = (f + f) * 0.5;
-                S.interfss[k][i] = S.interfss[k][i] + ath * dt;
-            }
-        }
-    } else {
-        #pragma unroll
-        for (int k = 0; k < NDIM; k++) {
-            double ath = (axp[k] + ax[k]) * 0.5;
-            S.vx[cur][k][i] = vi[k] + ath * dt;
-        }
-    }
+    accel_tail<NDIM>(S, P, i, ti, cur, ax, vi, axp, damp_on);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -2119,11 +2119,254 @@ static void or_update(or_ctx *c) {
     }
 }
 
+/* ------------------------------------------------------------------ soil-structure interaction */
+/* interaction_ss_rigid / _deformable / _penalty (CPP_10-1-interact_ss.cpp:17-1038) with the
+ * geometry helpers of the same file (:1039-1369), in the serial (one thread) order: one
+ * clInterf_SS slot per particle, structure slots accumulated in ascending soil-particle order.
+ * The OpenMP-private clLine / clPlane keep their constructor's C = 1 / D = 1 (never assigned)
+ * and their last unit normal when set_normal() finds a degenerate one (Class_Problem.cpp:194-258). */
+typedef struct { double A, B, C, nx, ny; } OLine;
+typedef struct { double x0, y0, z0, a, b, c; } OLine3;
+typedef struct { double A, B, C, D, nx, ny, nz; } OPlaneN;
+typedef struct { double x, y, z; } OVec;
+
+static double vdot(OVec a, OVec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static OVec vsub(OVec a, OVec b) { OVec r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+
+static double ss_dpline(OPt p, const OLine *l) { /* cal_dpline :1224-1236 */
+    double dst = fabs(l->A * p.x + l->B * p.y + l->C), temp = sqrt(l->A * l->A + l->B * l->B);
+    return temp > 1.0e-3 ? dst / temp : 0.0;
+}
+static double ss_dpplane(OPt p, const OPlaneN *q) { /* cal_dpplane :1238-1250 */
+    double dst = fabs(q->A * p.x + q->B * p.y + q->C * p.z + q->D), temp = sqrt(q->A * q->A + q->B * q->B + q->C * q->C);
+    return temp > 1.0e-3 ? dst / temp : 0.0;
+}
+static OPt ss_map_line(OPt r, const OLine *l) { /* cal_mappoint_line :1094-1125 (z of the result 0) */
+    double dA = l->A, dB = l->B, dp = ss_dpline(r, l), temp = sqrt(dA * dA + dB * dB);
+    OPt p1 = {r.x + dp * dA / temp, r.y + dp * dB / temp, 0.0}, p2 = {r.x - dp * dA / temp, r.y - dp * dB / temp, 0.0};
+    return ss_dpline(p1, l) < 1.0e-3 ? p1 : p2;
+}
+static OPt ss_map_plane(OPt r, const OPlaneN *q) { /* cal_mappoint_plane :1127-1160 */
+    double dA = q->A, dB = q->B, dC = q->C, dp = ss_dpplane(r, q), temp = sqrt(dA * dA + dB * dB + dC * dC);
+    OPt p1 = {r.x + dp * dA / temp, r.y + dp * dB / temp, r.z + dp * dC / temp};
+    OPt p2 = {r.x - dp * dA / temp, r.y - dp * dB / temp, r.z - dp * dC / temp};
+    return ss_dpplane(p1, q) < 1.0e-3 ? p1 : p2;
+}
+static void ss_line3(OPt p1, OPt p2, OLine3 *o) { /* cal_line3D :1078-1092 */
+    double dst = 0.0;
+    o->x0 = p1.x; o->y0 = p1.y; o->z0 = p1.z;
+    dst = (p2.x - p1.x) * (p2.x - p1.x) + (p2.y - p1.y) * (p2.y - p1.y) + (p2.z - p1.z) * (p2.z - p1.z);
+    dst = sqrt(dst);
+    o->a = (p2.x - p1.x) / dst; o->b = (p2.y - p1.y) / dst; o->c = (p2.z - p1.z) / dst;
+}
+static int ss_intersect(const OLine3 *l1, const OLine3 *l2, OPt *o) { /* cal_intersect_lines3D :1162-1185 */
+    double flag = l1->a * l2->b - l2->a * l1->b, t;
+    if (!(fabs(flag) > 1.0e-3)) return 1;
+    t = (l2->a * (l1->y0 - l2->y0) - l2->b * (l1->x0 - l2->x0)) / flag;
+    o->x = l1->x0 + l1->a * t; o->y = l1->y0 + l1->b * t; o->z = l1->z0 + l1->c * t;
+    return 0;
+}
+static double pdist(OPt a, OPt b) {
+    return sqrt((b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y) + (b.z - a.z) * (b.z - a.z));
+}
+/* linear_interpolate :1187-1202 */
+static void ss_interp(OPt p1, OPt p2, OPt r, double m1, double m2, const double v1[3], const double v2[3],
+                      double *m, double v[3]) {
+    double xk1 = 0.0, xk2 = pdist(p1, p2), xp = pdist(p1, r);
+    int k;
+    *m = (m1 * fabs(xp - xk2) + m2 * fabs(xp - xk1)) / fabs(xk2 - xk1);
+    for (k = 0; k < 3; k++) v[k] = (v1[k] * fabs(xp - xk2) + v2[k] * fabs(xp - xk1)) / fabs(xk2 - xk1);
+}
+/* linear_interpolate_force :1204-1222 */
+static void ss_interp_force(OPt p1, OPt p2, OPt r, const double f[3], double o1[3], double o2[3]) {
+    double xk1 = 0.0, xk2 = pdist(p1, p2), xp = pdist(p1, r);
+    int k;
+    for (k = 0; k < 3; k++) o1[k] = -fabs(xp - xk2) / fabs(xk1 - xk2) * f[k];
+    for (k = 0; k < 3; k++) o2[k] = -fabs(xp - xk1) / fabs(xk1 - xk2) * f[k];
+}
+
+static void or_ssi(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i, k, var = c->pro.ss_type;
+    double dt = c->pro.dt, dr = c->pro.dr, myu = c->pro.ss_myu, zeta = c->pro.ss_zeta;
+    double reach = var == 1 ? 0.5 * dr : dr;
+    double (*fn)[3] = (double (*)[3])calloc((size_t)n, sizeof *fn), (*ft)[3] = (double (*)[3])calloc((size_t)n, sizeof *ft);
+    OLine line = {0.0, 0.0, 1.0, 0.0, 0.0};
+    OPlaneN plane = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        int ids[SPHG_MAX_NB + 1], total = 0, nc = nbcount(&c->pc[i]), nj, id1, id2, id3;
+        double dsts[SPHG_MAX_NB + 1], vs[3] = {0, 0, 0}, vn[3] = {0, 0, 0}, vt[3] = {0, 0, 0}, ms, dp, mag_fn, mag_ft;
+        double f1n[3], f1t[3], f2n[3], f2t[3], f3n[3], f3t[3];
+        OPt p1, p2, p3, p4, outp;
+        if (!(pi->type == 1 || pi->type == 2)) continue;
+        /* structure neighbours in list order with their distances, then dist_sorting */
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double d;
+            if (pj->type != 4) continue;
+            ids[total] = j;
+            d = (pi->xp[0] - pj->xp[0]) * (pi->xp[0] - pj->xp[0]) + (pi->xp[1] - pj->xp[1]) * (pi->xp[1] - pj->xp[1]) +
+                (pi->xp[2] - pj->xp[2]) * (pi->xp[2] - pj->xp[2]);
+            dsts[total] = sqrt(d);
+            total++;
+        }
+        dist_sorting(total, ids, dsts);
+        p4.x = pi->xp[0]; p4.y = pi->xp[1]; p4.z = pi->xp[2];
+        if (ndim == 2) {
+            OVec a, b, q, v0, v1;
+            double u, den;
+            if (total < 2) continue;
+            id1 = ids[0]; id2 = ids[1];
+            p1.x = c->p[id1].xp[0]; p1.y = c->p[id1].xp[1]; p1.z = c->p[id1].xp[2];
+            p2.x = c->p[id2].xp[0]; p2.y = c->p[id2].xp[1]; p2.z = c->p[id2].xp[2];
+            den = p2.y * p1.x - p1.y * p2.x; /* cal_line :1055-1061 */
+            line.A = (p1.y - p2.y) / den;
+            line.B = (p2.x - p1.x) / den;
+            dp = ss_dpline(p4, &line);
+            outp = ss_map_line(p4, &line);
+            a.x = p1.x; a.y = p1.y; a.z = p1.z; b.x = p2.x; b.y = p2.y; b.z = p2.z;
+            q.x = outp.x; q.y = outp.y; q.z = 0.0;
+            v0 = vsub(b, a); v1 = vsub(q, a); /* check_ifinside_line :1296-1315 */
+            u = 1.0 / vdot(v0, v0) * vdot(v0, v1);
+            if (u < 0 || u > 1) continue;
+            if (!(dp < reach)) continue;
+            {
+                double t = sqrt(line.A * line.A + line.B * line.B), dot;
+                OVec e, nn;
+                if (t > 1.0e-10) { line.nx = line.A / t; line.ny = line.B / t; }
+                e.x = outp.x - p4.x; e.y = outp.y - p4.y; e.z = 0.0;
+                nn.x = line.nx; nn.y = line.ny; nn.z = 0.0;
+                dot = vdot(e, nn);
+                if (!(dot <= 0.0)) { line.nx = -line.nx; line.ny = -line.ny; }
+            }
+            ss_interp(p1, p2, outp, c->p[id1].mass, c->p[id2].mass, c->p[id1].vxp, c->p[id2].vxp, &ms, vs);
+            vs[0] -= pi->vxp[0];
+            vs[1] -= pi->vxp[1];
+            if (var != 1) {
+                vs[0] *= ms / (ms + pi->mass);
+                vs[1] *= ms / (ms + pi->mass);
+            }
+            {
+                double v = vs[0] * line.nx + vs[1] * line.ny; /* decomposition_velocity2D */
+                vn[0] = v * line.nx; vn[1] = v * line.ny;
+                vt[0] = vs[0] - vn[0]; vt[1] = vs[1] - vn[1];
+            }
+            if (var == 3) {
+                mag_fn = (1.0 - zeta) * 2.0 * pi->mass * (dr - dp) / dt / dt;
+                fn[i][0] = mag_fn * line.nx; fn[i][1] = mag_fn * line.ny;
+            } else {
+                fn[i][0] = pi->mass * vn[0] / dt; fn[i][1] = pi->mass * vn[1] / dt;
+            }
+            ft[i][0] = pi->mass * vt[0] / dt; ft[i][1] = pi->mass * vt[1] / dt;
+            mag_fn = sqrt(fn[i][0] * fn[i][0] + fn[i][1] * fn[i][1]);
+            mag_ft = sqrt(ft[i][0] * ft[i][0] + ft[i][1] * ft[i][1]);
+            if (mag_ft > (mag_fn * myu)) {
+                vt[0] = vt[0] * myu * mag_fn / mag_ft; vt[1] = vt[1] * myu * mag_fn / mag_ft;
+                ft[i][0] = pi->mass * vt[0] / dt; ft[i][1] = pi->mass * vt[1] / dt;
+            }
+            ss_interp_force(p1, p2, outp, fn[i], f1n, f2n);
+            ss_interp_force(p1, p2, outp, ft[i], f1t, f2t);
+            for (k = 0; k < 3; k++) {
+                fn[id1][k] += f1n[k]; fn[id2][k] += f2n[k];
+                ft[id1][k] += f1t[k]; ft[id2][k] += f2t[k];
+            }
+        } else {
+            OVec A, B, C, P, v0, v1, v2;
+            double d00, d01, d02, d11, d12, inv, u, v, tms, tvs[3];
+            OLine3 l31, l32;
+            OPt sect;
+            if (total < 3) continue;
+            id1 = ids[0]; id2 = ids[1]; id3 = ids[2];
+            p1.x = c->p[id1].xp[0]; p1.y = c->p[id1].xp[1]; p1.z = c->p[id1].xp[2];
+            p2.x = c->p[id2].xp[0]; p2.y = c->p[id2].xp[1]; p2.z = c->p[id2].xp[2];
+            p3.x = c->p[id3].xp[0]; p3.y = c->p[id3].xp[1]; p3.z = c->p[id3].xp[2];
+            {
+                OPlane q;
+                cal_plane(p1, p2, p3, &q); /* same expressions as CPP_10-1-interact_ss.cpp:1063-1076 */
+                plane.A = q.A; plane.B = q.B; plane.C = q.C;
+            }
+            dp = ss_dpplane(p4, &plane);
+            outp = ss_map_plane(p4, &plane);
+            A.x = p1.x; A.y = p1.y; A.z = p1.z; B.x = p2.x; B.y = p2.y; B.z = p2.z;
+            C.x = p3.x; C.y = p3.y; C.z = p3.z; P.x = outp.x; P.y = outp.y; P.z = outp.z;
+            v0 = vsub(C, A); v1 = vsub(B, A); v2 = vsub(P, A); /* check_ifinside_triangle :1317-1344 */
+            d00 = vdot(v0, v0); d01 = vdot(v0, v1); d02 = vdot(v0, v2); d11 = vdot(v1, v1); d12 = vdot(v1, v2);
+            inv = 1 / (d00 * d11 - d01 * d01);
+            u = (d11 * d02 - d01 * d12) * inv;
+            if (u < 0 || u > 1) continue;
+            v = (d00 * d12 - d01 * d02) * inv;
+            if (v < 0 || v > 1) continue;
+            if (!(u + v <= 1)) continue;
+            if (!(dp < reach)) continue;
+            {
+                double t = sqrt(plane.A * plane.A + plane.B * plane.B + plane.C * plane.C), dot;
+                OVec e, nn;
+                if (t > 1.0e-10) { plane.nx = plane.A / t; plane.ny = plane.B / t; plane.nz = plane.C / t; }
+                e.x = outp.x - p4.x; e.y = outp.y - p4.y; e.z = outp.z - p4.z;
+                nn.x = plane.nx; nn.y = plane.ny; nn.z = plane.nz;
+                dot = vdot(e, nn);
+                if (!(dot <= 0.0)) { plane.nx = -plane.nx; plane.ny = -plane.ny; plane.nz = -plane.nz; }
+            }
+            ss_line3(p1, p2, &l31);
+            ss_line3(p3, outp, &l32);
+            sect.x = sect.y = sect.z = 0.0;
+            if (ss_intersect(&l31, &l32, &sect)) continue;
+            ss_interp(p1, p2, sect, c->p[id1].mass, c->p[id2].mass, c->p[id1].vxp, c->p[id2].vxp, &tms, tvs);
+            ss_interp(p3, sect, outp, c->p[id3].mass, tms, c->p[id3].vxp, tvs, &ms, vs);
+            for (k = 0; k < 3; k++) vs[k] -= pi->vxp[k];
+            if (var != 1)
+                for (k = 0; k < 3; k++) vs[k] *= ms / (ms + pi->mass);
+            {
+                double w = vs[0] * plane.nx + vs[1] * plane.ny + vs[2] * plane.nz; /* decomposition_velocity3D */
+                vn[0] = w * plane.nx; vn[1] = w * plane.ny; vn[2] = w * plane.nz;
+                for (k = 0; k < 3; k++) vt[k] = vs[k] - vn[k];
+            }
+            if (var == 3) {
+                mag_fn = (1.0 - zeta) * 2.0 * pi->mass * (dr - dp) / dt / dt;
+                fn[i][0] = mag_fn * plane.nx; fn[i][1] = mag_fn * plane.ny; fn[i][2] = mag_fn * plane.nz;
+            } else {
+                for (k = 0; k < 3; k++) fn[i][k] = pi->mass * vn[k] / dt;
+            }
+            for (k = 0; k < 3; k++) ft[i][k] = pi->mass * vt[k] / dt;
+            mag_fn = sqrt(fn[i][0] * fn[i][0] + fn[i][1] * fn[i][1] + fn[i][2] * fn[i][2]);
+            mag_ft = sqrt(ft[i][0] * ft[i][0] + ft[i][1] * ft[i][1] + ft[i][2] * ft[i][2]);
+            if (mag_ft > (mag_fn * myu)) {
+                for (k = 0; k < 3; k++) vt[k] = vt[k] * myu * mag_fn / mag_ft;
+                for (k = 0; k < 3; k++) ft[i][k] = pi->mass * vt[k] / dt;
+            }
+            ss_interp_force(p3, sect, outp, fn[i], f1n, f3n);
+            ss_interp_force(p3, sect, outp, ft[i], f1t, f3t);
+            for (k = 0; k < 3; k++) { fn[id3][k] += f1n[k]; ft[id3][k] += f1t[k]; }
+            ss_interp_force(p1, p2, sect, f3n, f1n, f2n);
+            ss_interp_force(p1, p2, sect, f3t, f1t, f2t);
+            for (k = 0; k < 3; k++) {
+                fn[id1][k] -= f1n[k]; fn[id2][k] -= f2n[k];
+                ft[id1][k] -= f1t[k]; ft[id2][k] -= f2t[k];
+            }
+        }
+    }
+    /* thread slots into the particles (one slot here) */
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        if (!(pi->type == 1 || pi->type == 2 || pi->type == 4)) continue;
+        for (k = 0; k < 3; k++) {
+            pi->interfss[k] = 0.0;
+            if (var == 3) pi->interfss[k] += ft[i][k] + fn[i][k];
+            else pi->interfss[k] += zeta * (ft[i][k] + fn[i][k]);
+            pi->ax[k] += pi->interfss[k] / pi->mass;
+        }
+    }
+    free(fn);
+    free(ft);
+}
+
 /* ------------------------------------------------------------------ one step */
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
     if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
-    if (P->fli > 1 || P->ss_type != 0) return 1;
+    if (P->fli > 1 || P->ss_type < 0 || P->ss_type > 3) return 1;
+    if (P->ss_type != 0 && P->ss_acc_type == 2) return 1;
     if (P->flbndy != 1 && P->flbndy != 2) return 1;
     return 0;
 }
@@ -2166,6 +2409,7 @@ int or_phase(or_ctx *c, int mask) {
         if (c->pro.art_stre_coe > 0.0) or_art_stress(c);
         or_bndy_stress(c);
         if (c->pro.fli == 1) or_interact1(c);
+        if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3) or_ssi(c);
         if (c->l <= c->pro.inip) or_damping_geostress(c);
         or_velocity(c);
     }

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 from helpers import run_ref, run_ref_kl, have_ref  # noqa: E402
 from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d,  # noqa: E402
-                    art_stress_block, slope_2d)
+                    art_stress_block, slope_2d, ssi_block)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype"})
@@ -34,6 +34,11 @@ def decks():
         "moving_box_m3": (moving_box_2d(3, steps=6), 6, (1, 3, 6)),
         "art_stress_2d": (art_stress_block(2, steps=6), 6, (1, 2, 4, 6)),
         "art_stress_3d": (art_stress_block(3, nx=4, ny=3, nz=4, steps=6), 6, (1, 3, 6)),
+        "ssi_2d_t1": (ssi_block(2, 1, steps=6), 6, (1, 3, 6)),
+        "ssi_2d_t2": (ssi_block(2, 2, steps=6), 6, (1, 3, 6)),
+        "ssi_2d_t3": (ssi_block(2, 3, steps=6), 6, (1, 3, 6)),
+        "ssi_3d_t1": (ssi_block(3, 1, nx=5, ny=3, nz=3, steps=5), 5, (1, 5)),
+        "ssi_3d_t3": (ssi_block(3, 3, nx=5, ny=3, nz=3, steps=5), 5, (1, 5)),
     }
 
 

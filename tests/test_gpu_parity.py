@@ -225,7 +225,26 @@ def test_artificial_stress_vs_oracle(name):
     assert np.abs(o.get("Rij")).max() > 0.0  # the artificial stress is active in this deck
 
 
-@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"])
+SSI = ["ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3"]
+
+
+@pytest.mark.parametrize("name", SSI)
+def test_soil_structure_interaction_bit_exact_vs_oracle(name):
+    # interaction_ss_rigid / _deformable / _penalty: contact geometry, Coulomb cap, the reaction
+    # gathered onto the structure particles in the reference's serial order, damping clamp and
+    # velocity update after it
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare(g, o) == {}, f"step {s + 1}"
+    f = g.get("interfss")
+    assert np.abs(f[deck.type == 2]).max() > 0.0 and np.abs(f[deck.type == 4]).max() > 0.0
+
+
+@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI)
 def test_gpu_vs_reference_golden_models(name):
     # against the reference itself: within the oracle-Jacobi distance plus the device
     # transcendental allowance

@@ -31,7 +31,7 @@ def _mismatches(sim, ref):
 
 
 GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1", "moving_box_m2", "moving_box_m3",
-          "art_stress_2d", "art_stress_3d"]
+          "art_stress_2d", "art_stress_3d", "ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -83,3 +83,44 @@ def test_oracle_matches_reference_every_step(gen, steps):
     for s in range(1, steps + 1):
         assert sim.step(1) == 0
         assert _mismatches(sim, dumps[s]) == [], f"step {s}"
+
+
+@pytest.mark.parametrize("name", ["ssi_2d_t1", "ssi_2d_t2", "ssi_3d_t1"])
+def test_ssi_contact_force_closed_form(name):
+    # Step 1 of the soil-structure decks: the soil block moves at v0 = (0.4 tangential, 0.6 towards
+    # the plate(, 0.3 tangential)) onto a resting plate. For a particle in contact, the rigid form's
+    # force (interaction_ss_rigid) is m (v_s - v_i) / dt split along the plate normal, with the
+    # tangential part capped at myu |f_n| (Coulomb); interfss = zeta f. The deformable form scales
+    # v_s - v_i by m_s / (m_s + m_i), m_s the mass interpolated between the structure particles
+    # (all 2500 dx^d against the soil's 2000 dx^d). The plate normal is the lattice axis up to the
+    # 1e-6 dx jitter. Momentum: the structure particles receive -f in total.
+    deck, states = load_golden(name)
+    sim = OracleSim(deck, states[0], mode=GS)
+    assert sim.step(1) == 0
+    f = sim.get("interfss")
+    soil, struct = deck.type == 2, deck.type == 4
+    ndim = deck.ndim
+    nrm = 1 if ndim == 2 else 2
+    tan = [0] if ndim == 2 else [0, 1]
+    zeta, myu, dt = deck.ss_zeta, deck.ss_myu, deck.dt
+    m = 2000.0 * deck.dr ** ndim
+    scale = 1.0 if deck.ss_type == 1 else 2500.0 / 4500.0
+    contact = soil & (np.abs(f).sum(axis=1) > 0)
+    assert contact.sum() >= 4
+    vt = np.array([0.4, 0.3])[: len(tan)]
+    for i in np.nonzero(contact)[0]:
+        fn = zeta * m * scale * 0.6 / dt
+        ftan = zeta * m * scale * min(np.linalg.norm(vt), myu * 0.6) / dt
+        assert f[i, nrm] == pytest.approx(fn, rel=2e-5)
+        assert np.linalg.norm(f[i, tan]) == pytest.approx(ftan, rel=2e-5)
+        assert np.all(np.sign(f[i, tan]) == -np.sign(vt))  # friction opposes the sliding
+    tot_soil = f[soil].sum(axis=0)
+    tot_struct = f[struct].sum(axis=0)
+    np.testing.assert_allclose(tot_struct, -tot_soil, rtol=1e-12, atol=1e-12 * np.abs(tot_soil).max())
+    # the contact force reaches ax: ax_ssi - ax_without = interfss / m
+    deck0 = deck.copy()
+    deck0.ss_type = 0
+    sim0 = OracleSim(deck0, states[0], mode=GS)
+    assert sim0.step(1) == 0
+    dax = sim.get("ax") - sim0.get("ax")
+    np.testing.assert_allclose(dax[contact], f[contact] / m, rtol=1e-9)
