@@ -186,6 +186,10 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
             for (int k = 0; k < 6; k++) S.ssi_c[q][k] = nullptr;
         }
     }
+    for (int k = 0; k < 3; k++) {
+        if (pro->ss_type >= 1 && pro->ss_type <= 3 && pro->ss_acc_type == 2) D(S.cpl[k]);
+        else S.cpl[k] = nullptr;
+    }
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
@@ -454,7 +458,6 @@ static int check_supported(const sphg_problem &p) {
     if (p.flknl > 2) return SPHG_E_UNSUPPORTED;  // kernel_gradient_correction
     if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
     if (p.fli > 1 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
-    if (p.ss_type != 0 && p.ss_acc_type == 2) return SPHG_E_UNSUPPORTED;  // coupled_Water_Soil_Structure
     if (p.flbndy != 1 && p.flbndy != 2) return SPHG_E_UNSUPPORTED;
     if (p.if_move < 0 || p.if_move > 3) return SPHG_E_UNSUPPORTED;
     if (p.step_regu <= 0) return SPHG_E_ARG;  // l % step_regu
@@ -511,6 +514,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.inip = pro.inip;
     P.ss_type = pro.ss_type;
     P.ss_myu = pro.ss_myu;
+    P.coupled = (pro.ss_type >= 1 && pro.ss_type <= 3 && pro.ss_acc_type == 2) ? 1 : 0;
     P.ss_zeta = pro.ss_zeta;
     P.flts = pro.flts;
     P.dt = pro.dt;
@@ -713,6 +717,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
     }
     if (phase_mask & 16) {
         if (c->pre_on) LAUNCH("accel_pre", (k_accel_pre<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if (c->P.coupled) LAUNCH("coupled", (k_coupled<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("accel", (k_accel<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.ss_type != 0) {
             LAUNCH("ssi_soil", k_ssi_soil<NDIM>, grid_for(n), c->S, c->P, cur);

@@ -68,6 +68,7 @@ struct Params {
     int nreal;               // realisations (cell grids) in the context
     int fluid_por_uniform;   // every fluid particle's slot has the same porosity
     double ss_myu, ss_zeta;  // Para_SSInter friction coefficient and force factor
+    int coupled;             // Para_SSInter acc_type 2 with an interaction type 1-3
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -98,6 +99,7 @@ struct State {
     // its contact force is passed to (-1: none) and what each receives (f_n[3], f_t[3])
     int *ssi_id[3];
     double *ssi_c[3][6];
+    double *cpl[3];       // coupled_Water_Soil_Structure row sums (Para_SSInter acc_type 2)
     int *err, *step;      // sticky error word; current step number l
     const Grid *grids;    // one per realisation
     const int *rid;       // realisation of each particle

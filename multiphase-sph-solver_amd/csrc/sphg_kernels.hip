@@ -1739,6 +1739,52 @@ __device__ __forceinline__ void accel_tail(const State &S, const Params &P, int 
     }
 }
 
+// coupled_Water_Soil_Structure (CPP_9-4-derivation_stress.cpp:507-629), Para_SSInter acc_type 2:
+// stress-gradient sums across structure-soil and structure-water pairs, kept per row
+// (t[a][0] + t[a][1]) + t[a][2] for the momentum kernel to add after the boundary-stress term
+template <int NDIM, class M>
+__global__ __launch_bounds__(256) void k_coupled(State S, Params P, int cur) {
+    int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    if (ti != 4 && ti != 2 && ti != 1) return;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i], rhoi = S.rho[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const int nc = nb_count(S, i);
+    double si[6], t[3][3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) si[k] = S.sig[k][i];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) t[a][b] = 0.0;
+    for (int k = 0; k < nc; k++) {
+        const int e = S.nbr[(size_t)k * P.cap + i];
+        const int tj = nb_type(e), j = nb_id(e);
+        const bool plus = (ti == 4 && (tj == 2 || tj == 1)) || (ti == 2 && tj == 4);
+        const bool minus = ti == 1 && tj == 4;
+        if (!plus && !minus) continue;
+        double xj[3], w[4], d2, dst, sj[6];
+        load_xj<NDIM>(S, cur, j, xj);
+        pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hl_of(S, P, j, hli), w, d2, dst);
+        const double mj = S.mass[j], rj = S.rho[j];
+#pragma unroll
+        for (int q = 0; q < 6; q++) sj[q] = S.sig[q][j];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const int sx = kSidx[a][b];
+                if (plus) t[a][b] = t[a][b] + mj * (M::div(M::div(sj[sx], rj), rj) + M::div(M::div(si[sx], rhoi), rhoi)) * w[b];
+                else t[a][b] = t[a][b] + mj * (M::div(M::div(sj[sx], rj), rj) - M::div(M::div(si[sx], rhoi), rhoi)) * w[b];
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) S.cpl[a][i] = t[a][0] + t[a][1] + t[a][2];
+}
+
 template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     int i = pidx();
@@ -2004,6 +2050,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         ax[0] += bnd_t[0][0] + bnd_t[0][1] + bnd_t[0][2];
         ax[1] += bnd_t[1][0] + bnd_t[1][1] + bnd_t[1][2];
         ax[2] += bnd_t[2][0] + bnd_t[2][1] + bnd_t[2][2];
+    }
+    if (P.coupled && (ti == 4 || ti == 2 || ti == 1)) {
+        #pragma unroll
+        for (int k = 0; k < 3; k++) ax[k] += S.cpl[k][i];
     }
     const double dt = P.dt;
     const double axp[3] = {S.axp[0][i], S.axp[1][i], S.axp[2][i]};
@@ -2387,6 +2437,10 @@ template __global__ void k_saturation<2, Exact>(State, Params, int);
 template __global__ void k_saturation<2, Fast>(State, Params, int);
 template __global__ void k_saturation<3, Exact>(State, Params, int);
 template __global__ void k_saturation<3, Fast>(State, Params, int);
+template __global__ void k_coupled<2, Exact>(State, Params, int);
+template __global__ void k_coupled<2, Fast>(State, Params, int);
+template __global__ void k_coupled<3, Exact>(State, Params, int);
+template __global__ void k_coupled<3, Fast>(State, Params, int);
 template __global__ void k_accel_pre<2, Exact>(State, Params, int);
 template __global__ void k_accel_pre<2, Fast>(State, Params, int);
 template __global__ void k_accel_pre<3, Exact>(State, Params, int);

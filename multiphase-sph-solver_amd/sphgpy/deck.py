@@ -479,7 +479,7 @@ def art_stress_block(ndim=3, nx=5, ny=4, nz=6, dx=0.004, steps=10, seed=17, art_
 
 
 def ssi_block(ndim=2, ss_type=1, nx=8, ny=4, nz=5, dx=0.002, steps=10, seed=21, myu=0.3, zeta=None, gap=0.4,
-              v0=(0.4, -0.6, 0.3)) -> Deck:
+              v0=(0.4, -0.6, 0.3), acc_type=1, water=0) -> Deck:
     """Soil-structure interaction (Para_SSInter type 1 rigid, 2 deformable, 3 penalty;
     CPP_10-1-interact_ss.cpp): a Drucker-Prager soil block (2D: nx by nz, 3D: nx by ny by nz)
     moving at v0 onto an elastic structure plate (type 4, two rows / layers, 2 dx wider than the
@@ -487,27 +487,38 @@ def ssi_block(ndim=2, ss_type=1, nx=8, ny=4, nz=5, dx=0.002, steps=10, seed=21, 
     above the plate's top row, so soil particles are in contact from the first step; v0 has a
     tangential part so that the Coulomb cap (friction `myu`) acts. zeta defaults to 0.5, and to 0.999
     for the penalty form, whose normal force (1 - zeta) 2 m (dr - dp) / dt^2 is otherwise explosive
-    at this time step."""
+    at this time step. acc_type 2 switches on coupled_Water_Soil_Structure; `water` > 0 adds that
+    many lattice columns of water (type 1) beside the block, on the plate."""
     if zeta is None:
         zeta = 0.999 if ss_type == 3 else 0.5
     rng = np.random.default_rng(seed)
     grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
     d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, ss_type=ss_type, ss_myu=myu,
-             ss_zeta=zeta, ss_acc_type=1, art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), flbndy=2,
+             ss_zeta=zeta, ss_acc_type=acc_type, art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), flbndy=2,
              name=f"ssi_block_{ndim}d_t{ss_type}")
+    if water:
+        w = default_fluid()
+        w.update(dens=1000.0, vis=1e-3, vpu=30.0, press_type=0, acc_type=0, bhtype=0, pre_coe=1.0, strain_min=1.0)
+        d.water = [dict(w), dict(w), dict(w)]
     d.model_flags[2] = 1
-    d.soil_layers[2] = [dp_layer(c=300.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0)]
+    # with water the soil's void ratio sets the fluid porosity of the water EOS (0 would divide by 0)
+    d.soil_layers[2] = [dp_layer(c=300.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0, eps0=0.6 if water else 0.0)]
     d.structure = {"dens": 2500.0, "eps0": 0.0, "cop": 0.0, "e": 2.0e7, "poi": 0.25}
     drop = (1.0 - gap) * dx
     if ndim == 2:
         soil = _lattice2d(0, nx, 2, 2 + nz, dx)
         soil[:, 1] -= drop
-        parts = [(soil, 2, 2), (_lattice2d(-2, nx + 2, 0, 2, dx), 4, 0), (_lattice2d(-5, nx + 5, -3, 0, dx), 0, 0)]
+        parts = [(soil, 2, 2), (_lattice2d(-2, nx + water + 2, 0, 2, dx), 4, 0),
+                 (_lattice2d(-5, nx + water + 5, -3, 0, dx), 0, 0)]
+        if water:
+            parts.append((_lattice2d(nx, nx + water, 2, 2 + nz, dx), 1, 0))
     else:
         soil = _lattice3d(0, nx, 0, ny, 2, 2 + nz, dx)
         soil[:, 2] -= drop
-        parts = [(soil, 2, 2), (_lattice3d(-2, nx + 2, -2, ny + 2, 0, 2, dx), 4, 0),
-                 (_lattice3d(-5, nx + 5, -5, ny + 5, -3, 0, dx), 0, 0)]
+        parts = [(soil, 2, 2), (_lattice3d(-2, nx + water + 2, -2, ny + 2, 0, 2, dx), 4, 0),
+                 (_lattice3d(-5, nx + water + 5, -5, ny + 5, -3, 0, dx), 0, 0)]
+        if water:
+            parts.append((_lattice3d(nx, nx + water, 0, ny, 2, 2 + nz, dx), 1, 0))
     deck = _assemble(d, parts, rng)
     s = deck.type == 2
     # v0 = (tangential x, normal towards the plate, second tangential (3D))

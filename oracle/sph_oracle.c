@@ -1562,6 +1562,27 @@ static void or_bndy_stress(or_ctx *c) {
     }
 }
 
+/* coupled_Water_Soil_Structure (CPP_9-4-derivation_stress.cpp:507-629), Para_SSInter acc_type 2:
+ * stress-gradient sums across the structure / soil and structure / water interfaces */
+static void or_coupled(or_ctx *c) {
+    int n = c->n, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double t[3][3] = {{0}};
+        int nc = nbcount(&c->pc[i]), nj;
+        if (!(pi->type == 4 || pi->type == 2 || pi->type == 1)) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if ((pi->type == 4 && pj->type == 2) || (pi->type == 2 && pj->type == 4) || (pi->type == 4 && pj->type == 1))
+                sdiv(t, pj->mass, pj->sig, pj->rho, pi->sig, pi->rho, 1, c->pc[i].wij[nj]);
+            if (pi->type == 1 && pj->type == 4) sdiv(t, pj->mass, pj->sig, pj->rho, pi->sig, pi->rho, -1, c->pc[i].wij[nj]);
+        }
+        add_sum(pi->ax, t);
+    }
+}
+
 /* interact1: Darcy drag (CPP_10-interact1.cpp:15-110) */
 static void or_interact1(or_ctx *c) {
     int n = c->n, ndim = c->pro.ndim, i;
@@ -2366,7 +2387,6 @@ static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
     if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
     if (P->fli > 1 || P->ss_type < 0 || P->ss_type > 3) return 1;
-    if (P->ss_type != 0 && P->ss_acc_type == 2) return 1;
     if (P->flbndy != 1 && P->flbndy != 2) return 1;
     return 0;
 }
@@ -2408,6 +2428,7 @@ int or_phase(or_ctx *c, int mask) {
         or_art_visc(c);
         if (c->pro.art_stre_coe > 0.0) or_art_stress(c);
         or_bndy_stress(c);
+        if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3 && c->pro.ss_acc_type == 2) or_coupled(c);
         if (c->pro.fli == 1) or_interact1(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3) or_ssi(c);
         if (c->l <= c->pro.inip) or_damping_geostress(c);

@@ -39,6 +39,8 @@ def decks():
         "ssi_2d_t3": (ssi_block(2, 3, steps=6), 6, (1, 3, 6)),
         "ssi_3d_t1": (ssi_block(3, 1, nx=5, ny=3, nz=3, steps=5), 5, (1, 5)),
         "ssi_3d_t3": (ssi_block(3, 3, nx=5, ny=3, nz=3, steps=5), 5, (1, 5)),
+        "ssi_cpl_2d": (ssi_block(2, 1, steps=6, acc_type=2, water=2), 6, (1, 3, 6)),
+        "ssi_cpl_3d": (ssi_block(3, 1, nx=5, ny=3, nz=3, steps=5, acc_type=2, water=2), 5, (1, 5)),
     }
 
 

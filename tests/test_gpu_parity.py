@@ -225,7 +225,7 @@ def test_artificial_stress_vs_oracle(name):
     assert np.abs(o.get("Rij")).max() > 0.0  # the artificial stress is active in this deck
 
 
-SSI = ["ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3"]
+SSI = ["ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3", "ssi_cpl_2d", "ssi_cpl_3d"]
 
 
 @pytest.mark.parametrize("name", SSI)

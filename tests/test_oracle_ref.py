@@ -31,7 +31,8 @@ def _mismatches(sim, ref):
 
 
 GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1", "moving_box_m2", "moving_box_m3",
-          "art_stress_2d", "art_stress_3d", "ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3"]
+          "art_stress_2d", "art_stress_3d", "ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3",
+          "ssi_cpl_2d", "ssi_cpl_3d"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -124,3 +125,12 @@ def test_ssi_contact_force_closed_form(name):
     assert sim0.step(1) == 0
     dax = sim.get("ax") - sim0.get("ax")
     np.testing.assert_allclose(dax[contact], f[contact] / m, rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_golden_fixture_states_are_finite(name):
+    # a deck that drives the reference to NaN would make every comparison vacuous
+    deck, states = load_golden(name)
+    for s, st in states.items():
+        for k in ("xp", "vx", "sig", "rho"):
+            assert np.isfinite(st[k]).all(), (s, k)
