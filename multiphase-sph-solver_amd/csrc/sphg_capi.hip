@@ -190,6 +190,16 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
         if (pro->ss_type >= 1 && pro->ss_type <= 3 && pro->ss_acc_type == 2) D(S.cpl[k]);
         else S.cpl[k] = nullptr;
     }
+    const bool tran = pro->flbndy == 3 || pro->flbndy == 4 || pro->flbndy == 13 || pro->flbndy == 14;
+    for (int t = 0; t < 4; t++)
+        for (int k = 0; k < 6; k++) {
+            if (tran) D(S.sigb[t][k]);
+            else S.sigb[t][k] = nullptr;
+        }
+    for (int k = 0; k < 3; k++) {
+        if (tran) D(S.bndt[k]);
+        else S.bndt[k] = nullptr;
+    }
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
@@ -458,7 +468,8 @@ static int check_supported(const sphg_problem &p) {
     if (p.flknl > 2) return SPHG_E_UNSUPPORTED;  // kernel_gradient_correction
     if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
     if (p.fli > 1 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
-    if (p.flbndy != 1 && p.flbndy != 2) return SPHG_E_UNSUPPORTED;
+    if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 13 && p.flbndy != 14)
+        return SPHG_E_UNSUPPORTED;
     if (p.if_move < 0 || p.if_move > 3) return SPHG_E_UNSUPPORTED;
     if (p.step_regu <= 0) return SPHG_E_ARG;  // l % step_regu
     return SPHG_OK;
@@ -717,6 +728,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
     }
     if (phase_mask & 16) {
         if (c->pre_on) LAUNCH("accel_pre", (k_accel_pre<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if (tran_bndy(c->P)) LAUNCH("bndy_tran", (k_bndy_tran<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.coupled) LAUNCH("coupled", (k_coupled<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("accel", (k_accel<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.ss_type != 0) {
@@ -725,7 +737,10 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
             LAUNCH("accel_tail", k_accel_tail<NDIM>, grid_for(n), c->S, c->P, cur);
         }
     }
-    if (phase_mask & 32) LAUNCH("xsph", (k_xsph<NDIM, M>), grid_for(n), c->S, c->P, cur);
+    if (phase_mask & 32) {
+        LAUNCH("xsph", (k_xsph<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if (tran_bndy(c->P)) LAUNCH("slip_tran", (k_slip_tran<NDIM, M>), grid_for(n), c->S, c->P, cur);
+    }
     if (phase_mask & 64) {
         LAUNCH("update", (k_update<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("step_inc", k_step_inc, 1, c->S);

@@ -100,6 +100,10 @@ struct State {
     int *ssi_id[3];
     double *ssi_c[3][6];
     double *cpl[3];       // coupled_Water_Soil_Structure row sums (Para_SSInter acc_type 2)
+    // boundary treatment of Tran et al. (flbndy 3/4/13/14): per boundary particle and phase
+    // (water, soil, air, structure) the interpolated stress, and the boundary-stress row sums
+    double *sigb[4][6];
+    double *bndt[3];
     int *err, *step;      // sticky error word; current step number l
     const Grid *grids;    // one per realisation
     const int *rid;       // realisation of each particle

@@ -432,6 +432,11 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     for (int k = 0; k < nk; k++) S.nbr[(size_t)k * P.cap + i] = stage[k][t];
 }
 
+// boundary treatment of Tran et al. (no slip 3, free slip 4, each after velocity_set_domain 13 / 14)
+__host__ __device__ __forceinline__ bool tran_bndy(const Params &P) {
+    return P.flbndy == 3 || P.flbndy == 4 || P.flbndy == 13 || P.flbndy == 14;
+}
+
 // Source of the velocity that a boundary neighbour shows to a particle of phase ti
 // (CPP_6-density.cpp:86-120 and its copies in the strain pass): flbndy 1 the particle's own
 // velocity (mode 1), flbndy 2/3/4/13/14 the boundary's per-phase virtual velocity (mode 2),
@@ -1739,6 +1744,60 @@ __device__ __forceinline__ void accel_tail(const State &S, const Params &P, int 
     }
 }
 
+// boundary_stress_effect_tran (CPP_9-4-derivation_stress.cpp:725-820), flbndy 3/4/13/14: each
+// boundary neighbour contributes the stress its slip pass interpolated for the particle's phase;
+// row sums (t[a][0] + t[a][1]) + t[a][2] for the momentum kernel to add in the boundary-stress slot
+template <int NDIM, class M>
+__global__ __launch_bounds__(256) void k_bndy_tran(State S, Params P, int cur) {
+    int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    if (ti == 0 || ti == 7) return;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i], rhoi = S.rho[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const int nc = nb_count(S, i);
+    double ai[6], t[3][3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) ai[k] = 0.0;
+    if (ti == 1 || ti == 3 || ti == 4) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) ai[k] = S.sig[k][i];
+    } else if (ti == 2) {
+        const double satu = S.satu[i], prew = S.prew[i], prea = S.prea[i];
+#pragma unroll
+        for (int k = 0; k < 3; k++) ai[k] = S.sig[k][i] - satu * prew + (1 - satu) * prea;
+#pragma unroll
+        for (int k = 3; k < 6; k++) ai[k] = S.sig[k][i];
+    }
+    const int ph = (ti >= 1 && ti <= 4) ? ti - 1 : -1;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) t[a][b] = 0.0;
+    for (int k = 0; k < nc; k++) {
+        const int e = S.nbr[(size_t)k * P.cap + i];
+        if (nb_type(e) != 0) continue;
+        const int j = nb_id(e);
+        double xj[3], w[4], d2, dst, aj[6];
+        load_xj<NDIM>(S, cur, j, xj);
+        pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hl_of(S, P, j, hli), w, d2, dst);
+        const double mj = S.mass[j], rj = S.rho[j];
+#pragma unroll
+        for (int q = 0; q < 6; q++) aj[q] = ph >= 0 ? S.sigb[ph < 0 ? 0 : ph][q][j] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const int sx = kSidx[a][b];
+                t[a][b] = t[a][b] + mj * (M::div(M::div(aj[sx], rj), rj) + M::div(M::div(ai[sx], rhoi), rhoi)) * w[b];
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) S.bndt[a][i] = t[a][0] + t[a][1] + t[a][2];
+}
+
 // coupled_Water_Soil_Structure (CPP_9-4-derivation_stress.cpp:507-629), Para_SSInter acc_type 2:
 // stress-gradient sums across structure-soil and structure-water pairs, kept per row
 // (t[a][0] + t[a][1]) + t[a][2] for the momentum kernel to add after the boundary-stress term
@@ -2051,6 +2110,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         ax[1] += bnd_t[1][0] + bnd_t[1][1] + bnd_t[1][2];
         ax[2] += bnd_t[2][0] + bnd_t[2][1] + bnd_t[2][2];
     }
+    if (tran_bndy(P) && ti != 0 && ti != 7) {
+        #pragma unroll
+        for (int k = 0; k < 3; k++) ax[k] += S.bndt[k][i];
+    }
     if (P.coupled && (ti == 4 || ti == 2 || ti == 1)) {
         #pragma unroll
         for (int k = 0; k < 3; k++) ax[k] += S.cpl[k][i];
@@ -2180,8 +2243,8 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
         for (int k = 0; k < NDIM; k++) v[k] = vi[k] + tu[k];
     }
     const double dr = P.dr;
-    // velocity_set_domain (flbndy 2, fluids) / tmboundary (flbndy 1), CPP_13-boundary.cpp:71-177
-    if ((P.flbndy == 2 && (ti == 1 || ti == 3)) || P.flbndy == 1) {
+    // velocity_set_domain (flbndy 2 / 13 / 14, fluids) / tmboundary (flbndy 1), CPP_13-boundary.cpp:71-177
+    if (((P.flbndy == 2 || P.flbndy == 13 || P.flbndy == 14) && (ti == 1 || ti == 3)) || P.flbndy == 1) {
         const double coe = P.flbndy == 1 ? 0.30 : P.coe_bndy;
         double t1[3] = {0, 0, 0};
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
@@ -2288,6 +2351,90 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     }
     #pragma unroll
     for (int k = 0; k < 3; k++) S.vel_bndy[k][i] = vb[k];
+}
+
+// ------------------------------------------------------------------------------------------
+// no_slip_tran / free_slip_tran (CPP_13-boundary.cpp:711-958, :960-1237)
+// ------------------------------------------------------------------------------------------
+// Every boundary particle takes, per phase, the Shepard averages of its neighbours' stress (soil:
+// effective stress) and of their velocity after XSPH; its phase velocity is interfss minus the
+// average (no slip) or interfss plus the average with the pair-normal component removed (free
+// slip, pairs within 0.1 dr skipped). The phase densities the reference also stores are read by
+// no other pass and are not kept.
+template <int NDIM, class M>
+__global__ __launch_bounds__(256) void k_slip_tran(State S, Params P, int cur) {
+    int i = pidx();
+    if (i >= P.n || S.type[i] != 0) return;
+    const int nxt = cur ^ 1;
+    const bool free_slip = P.flbndy == 4 || P.flbndy == 14;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double x3[3] = {S.xp[cur][0][i], S.xp[cur][1][i], S.xp[cur][2][i]};
+    const double hli = S.hl[i], dr = P.dr;
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const int nc = nb_count(S, i);
+    double ts[4][6], tv[4][3], td[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        td[t] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) ts[t][k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) tv[t][k] = 0.0;
+    }
+    for (int q = 0; q < nc; q++) {
+        const int e = S.nbr[(size_t)q * P.cap + i];
+        const int tj = nb_type(e);
+        if (tj < 1 || tj > 4) continue;
+        const int j = nb_id(e), t = tj - 1;
+        double xj[3], w[4], d2, dst;
+        load_xj<NDIM>(S, cur, j, xj);
+        pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hl_of(S, P, j, hli), w, d2, dst);
+        const double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
+        double sj[6], vj[3], v1[3];
+#pragma unroll
+        for (int k = 0; k < 6; k++) sj[k] = S.sig[k][j];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            vj[k] = S.vx[nxt][k][j];
+            v1[k] = x3[k] - S.xp[cur][k][j];
+        }
+        if (t == 1) {
+            const double satu = S.satu[j], prew = S.prew[j], prea = S.prea[j];
+#pragma unroll
+            for (int k = 0; k < 3; k++) sj[k] = sj[k] - satu * prew + (1 - satu) * prea;
+        }
+        // the phase index stays uniform in registers: one slot is updated per neighbour
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            if (p != t) continue;
+#pragma unroll
+            for (int k = 0; k < 6; k++) ts[p][k] = ts[p][k] + M::div(mj * sj[k] * w3, rj);
+            if (!free_slip) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) tv[p][k] = tv[p][k] + M::div(mj * vj[k] * w3, rj);
+            } else {
+                const double dd = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+                const double inprod = vj[0] * v1[0] + vj[1] * v1[1] + vj[2] * v1[2];
+                if (dd > 0.1 * dr) {
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        tv[p][k] = tv[p][k] + M::div(mj * (vj[k] - M::div(M::div(inprod * v1[k], dd), dd)) * w3, rj);
+                }
+            }
+            td[p] = td[p] + M::div(mj * w3, rj);
+        }
+    }
+    const double fss[3] = {S.interfss[0][i], S.interfss[1][i], S.interfss[2][i]};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const bool any = fabs(td[t]) > 1.0e-6;
+#pragma unroll
+        for (int k = 0; k < 6; k++) S.sigb[t][k][i] = any ? ts[t][k] / td[t] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            S.vb[t][k][i] = !any ? fss[k] : (free_slip ? fss[k] + tv[t][k] / td[t] : fss[k] - tv[t][k] / td[t]);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2441,6 +2588,14 @@ template __global__ void k_coupled<2, Exact>(State, Params, int);
 template __global__ void k_coupled<2, Fast>(State, Params, int);
 template __global__ void k_coupled<3, Exact>(State, Params, int);
 template __global__ void k_coupled<3, Fast>(State, Params, int);
+template __global__ void k_bndy_tran<2, Exact>(State, Params, int);
+template __global__ void k_bndy_tran<2, Fast>(State, Params, int);
+template __global__ void k_bndy_tran<3, Exact>(State, Params, int);
+template __global__ void k_bndy_tran<3, Fast>(State, Params, int);
+template __global__ void k_slip_tran<2, Exact>(State, Params, int);
+template __global__ void k_slip_tran<2, Fast>(State, Params, int);
+template __global__ void k_slip_tran<3, Exact>(State, Params, int);
+template __global__ void k_slip_tran<3, Fast>(State, Params, int);
 template __global__ void k_accel_pre<2, Exact>(State, Params, int);
 template __global__ void k_accel_pre<2, Fast>(State, Params, int);
 template __global__ void k_accel_pre<3, Exact>(State, Params, int);

@@ -526,6 +526,35 @@ def ssi_block(ndim=2, ss_type=1, nx=8, ny=4, nz=5, dx=0.002, steps=10, seed=21, 
     return deck
 
 
+def tran_box(ndim=2, flbndy=3, nx=8, ny=3, nz_soil=4, nz_water=3, dx=0.002, steps=8, seed=31, v0=0.2) -> Deck:
+    """Soil under water in a box whose walls use the boundary treatment of Tran et al.
+    (flbndy 3 no slip, 4 free slip, 13 / 14 the same after velocity_set_domain;
+    CPP_13-boundary.cpp:711-1237, CPP_9-4-derivation_stress.cpp:725-820). Soil (DP, void ratio
+    0.6) and water start sliding at v0 along the floor, so that the wall velocities and stresses
+    each phase sees differ from zero."""
+    rng = np.random.default_rng(seed)
+    grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
+    d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, flbndy=flbndy, coe_bndy=0.1,
+             art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"tran_box_{ndim}d_b{flbndy}")
+    d.model_flags[2] = 1
+    d.soil_layers[2] = [dp_layer(c=300.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0, eps0=0.6)]
+    w = default_fluid()
+    w.update(dens=1000.0, vis=1e-3, vpu=30.0, press_type=0, acc_type=0, bhtype=0, pre_coe=1.0, strain_min=1.0)
+    d.water = [dict(w), dict(w), dict(w)]
+    top = nz_soil + nz_water
+    if ndim == 2:
+        parts = [(_lattice2d(0, nx, 0, nz_soil, dx), 2, 2), (_lattice2d(0, nx, nz_soil, top, dx), 1, 0),
+                 (_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0), (_lattice2d(-3, 0, 0, top + 3, dx), 0, 0),
+                 (_lattice2d(nx, nx + 3, 0, top + 3, dx), 0, 0)]
+    else:
+        parts = [(_lattice3d(0, nx, 0, ny, 0, nz_soil, dx), 2, 2), (_lattice3d(0, nx, 0, ny, nz_soil, top, dx), 1, 0),
+                 (_box_walls_3d(nx, ny, top + 3, dx), 0, 0)]
+    deck = _assemble(d, parts, rng)
+    moving = (deck.type == 1) | (deck.type == 2)
+    deck.vxp[moving, 0] = v0
+    return deck
+
+
 def slope_2d(height=5.0, dx=0.1, steps=3000, seed=0, crest=1.5, toe=1.5, base=4.0, walls=3) -> Deck:
     """2D soil slope for the random-field batch: a crest of `crest`*H, a 1:2 (vertical:horizontal)
     face, a toe of `toe`*H and a `base` m layer under all of it, Drucker-Prager soil (mean

@@ -38,6 +38,7 @@ typedef struct {
 
 typedef struct {
     double vx_water[3], vx_soil[3], vx_air[3], vx_struct[3];
+    double sig_ph[4][6], rho_ph[4]; /* sig_water/soil/air/struct, rho_water/... (Class_particle.h) */
 } OVariB;
 
 struct or_ctx {
@@ -1562,6 +1563,37 @@ static void or_bndy_stress(or_ctx *c) {
     }
 }
 
+/* boundary_stress_effect_tran (CPP_9-4-derivation_stress.cpp:725-820), flbndy 3/4/13/14: the
+ * boundary neighbour contributes the stress its no-slip / free-slip pass interpolated for the
+ * particle's phase */
+static void or_bndy_stress_tran(or_ctx *c) {
+    int n = c->n, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double t[3][3] = {{0}};
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (pi->type == 0 || pi->type == 7) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double ai[6] = {0, 0, 0, 0, 0, 0}, aj[6] = {0, 0, 0, 0, 0, 0};
+            if (pj->type != 0) continue;
+            if (pi->type == 1 || pi->type == 3 || pi->type == 4) {
+                for (k = 0; k < 6; k++) ai[k] = pi->sig[k];
+            } else if (pi->type == 2) {
+                double satu = pi->satu;
+                for (k = 0; k < 3; k++) ai[k] = pi->sig[k] - satu * pi->prew + (1 - satu) * pi->prea;
+                for (k = 3; k < 6; k++) ai[k] = pi->sig[k];
+            }
+            if (pi->type >= 1 && pi->type <= 4)
+                for (k = 0; k < 6; k++) aj[k] = c->vb[j].sig_ph[pi->type - 1][k];
+            sdiv(t, pj->mass, aj, pj->rho, ai, pi->rho, 1, c->pc[i].wij[nj]);
+        }
+        add_sum(pi->ax, t);
+    }
+}
+
 /* coupled_Water_Soil_Structure (CPP_9-4-derivation_stress.cpp:507-629), Para_SSInter acc_type 2:
  * stress-gradient sums across the structure / soil and structure / water interfaces */
 static void or_coupled(or_ctx *c) {
@@ -2089,6 +2121,65 @@ static void or_tm_boundary(or_ctx *c) {
     }
 }
 
+/* no_slip_tran / free_slip_tran (CPP_13-boundary.cpp:711-958, :960-1237): every boundary particle
+ * takes, per phase (water, soil, air, structure), the Shepard averages of its neighbours' stress
+ * (soil: effective stress), density and velocity; its virtual velocity is interfss minus the
+ * average velocity (no slip) or interfss plus the average of the velocity with the component
+ * along the particle pair removed (free slip, pairs closer than 0.1 dr skipped) */
+static void or_slip_tran(or_ctx *c, int free_slip) {
+    int n = c->n, i;
+    double dr = c->pro.dr;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double ts[4][6] = {{0}}, tv[4][3] = {{0}}, tr[4] = {0, 0, 0, 0}, td[4] = {0, 0, 0, 0};
+        double *vxo[4];
+        int nc = nbcount(&c->pc[i]), nj, k, t;
+        if (pi->type != 0) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double w3 = c->pc[i].wij[nj][3], v1[3], dst, inprod;
+            if (pj->type < 1 || pj->type > 4) continue;
+            t = pj->type - 1;
+            v1[0] = pi->xp[0] - pj->xp[0];
+            v1[1] = pi->xp[1] - pj->xp[1];
+            v1[2] = pi->xp[2] - pj->xp[2];
+            dst = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+            inprod = pj->vx[0] * v1[0] + pj->vx[1] * v1[1] + pj->vx[2] * v1[2];
+            if (t == 1) {
+                double satu = pj->satu;
+                for (k = 0; k < 3; k++)
+                    ts[t][k] = ts[t][k] + pj->mass * (pj->sig[k] - satu * pj->prew + (1 - satu) * pj->prea) * w3 / pj->rho;
+                for (k = 3; k < 6; k++) ts[t][k] = ts[t][k] + pj->mass * pj->sig[k] * w3 / pj->rho;
+            } else {
+                for (k = 0; k < 6; k++) ts[t][k] = ts[t][k] + pj->mass * pj->sig[k] * w3 / pj->rho;
+            }
+            if (!free_slip) {
+                for (k = 0; k < 3; k++) tv[t][k] = tv[t][k] + pj->mass * pj->vx[k] * w3 / pj->rho;
+            } else if (dst > 0.1 * dr) {
+                for (k = 0; k < 3; k++)
+                    tv[t][k] = tv[t][k] + pj->mass * (pj->vx[k] - inprod * v1[k] / dst / dst) * w3 / pj->rho;
+            }
+            tr[t] = tr[t] + pj->mass * pj->rho * w3 / pj->rho;
+            td[t] = td[t] + pj->mass * w3 / pj->rho;
+        }
+        vxo[0] = c->vb[i].vx_water; vxo[1] = c->vb[i].vx_soil; vxo[2] = c->vb[i].vx_air; vxo[3] = c->vb[i].vx_struct;
+        for (t = 0; t < 4; t++) {
+            if (fabs(td[t]) > 1.0e-6) {
+                for (k = 0; k < 6; k++) c->vb[i].sig_ph[t][k] = ts[t][k] / td[t];
+                for (k = 0; k < 3; k++)
+                    vxo[t][k] = free_slip ? pi->interfss[k] + tv[t][k] / td[t] : pi->interfss[k] - tv[t][k] / td[t];
+                c->vb[i].rho_ph[t] = tr[t] / td[t];
+            } else {
+                for (k = 0; k < 6; k++) c->vb[i].sig_ph[t][k] = 0.0;
+                for (k = 0; k < 3; k++) vxo[t][k] = pi->interfss[k];
+                c->vb[i].rho_ph[t] = 0.0;
+            }
+        }
+    }
+}
+
 /* check_domain (CPP_13-boundary.cpp:20-68), serial semantics of the shared flag/vbx (A3) */
 static void or_check_domain(or_ctx *c) {
     const sphg_problem *P = &c->pro;
@@ -2387,7 +2478,8 @@ static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
     if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
     if (P->fli > 1 || P->ss_type < 0 || P->ss_type > 3) return 1;
-    if (P->flbndy != 1 && P->flbndy != 2) return 1;
+    if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 13 && P->flbndy != 14)
+        return 1;
     return 0;
 }
 
@@ -2427,7 +2519,8 @@ int or_phase(or_ctx *c, int mask) {
         or_solid_accel(c, 4);
         or_art_visc(c);
         if (c->pro.art_stre_coe > 0.0) or_art_stress(c);
-        or_bndy_stress(c);
+        if (c->pro.flbndy == 1 || c->pro.flbndy == 2) or_bndy_stress(c);
+        else or_bndy_stress_tran(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3 && c->pro.ss_acc_type == 2) or_coupled(c);
         if (c->pro.fli == 1) or_interact1(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3) or_ssi(c);
@@ -2440,6 +2533,11 @@ int or_phase(or_ctx *c, int mask) {
         else if (c->pro.flbndy == 2) {
             or_vel_repulse(c, c->pro.coe_bndy, 1);
             or_tm_boundary(c);
+        } else if (c->pro.flbndy == 3 || c->pro.flbndy == 4) {
+            or_slip_tran(c, c->pro.flbndy == 4);
+        } else if (c->pro.flbndy == 13 || c->pro.flbndy == 14) {
+            or_vel_repulse(c, c->pro.coe_bndy, 1);
+            or_slip_tran(c, c->pro.flbndy == 14);
         }
     }
     if (mask & 64) {

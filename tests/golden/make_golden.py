@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 from helpers import run_ref, run_ref_kl, have_ref  # noqa: E402
 from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d,  # noqa: E402
-                    art_stress_block, slope_2d, ssi_block)
+                    art_stress_block, slope_2d, ssi_block, tran_box)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype"})
@@ -41,6 +41,12 @@ def decks():
         "ssi_3d_t3": (ssi_block(3, 3, nx=5, ny=3, nz=3, steps=5), 5, (1, 5)),
         "ssi_cpl_2d": (ssi_block(2, 1, steps=6, acc_type=2, water=2), 6, (1, 3, 6)),
         "ssi_cpl_3d": (ssi_block(3, 1, nx=5, ny=3, nz=3, steps=5, acc_type=2, water=2), 5, (1, 5)),
+        "tran_2d_b3": (tran_box(2, 3, steps=6), 6, (1, 3, 6)),
+        "tran_2d_b4": (tran_box(2, 4, steps=6), 6, (1, 3, 6)),
+        "tran_2d_b13": (tran_box(2, 13, steps=6), 6, (1, 3, 6)),
+        "tran_2d_b14": (tran_box(2, 14, steps=6), 6, (1, 3, 6)),
+        "tran_3d_b3": (tran_box(3, 3, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 5, (1, 5)),
+        "tran_3d_b14": (tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 5, (1, 5)),
     }
 
 

@@ -244,7 +244,25 @@ def test_soil_structure_interaction_bit_exact_vs_oracle(name):
     assert np.abs(f[deck.type == 2]).max() > 0.0 and np.abs(f[deck.type == 4]).max() > 0.0
 
 
-@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI)
+TRAN = ["tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3", "tran_3d_b14"]
+
+
+@pytest.mark.parametrize("name", TRAN)
+def test_tran_boundaries_bit_exact_vs_oracle(name):
+    # no-slip / free-slip walls of Tran et al. (flbndy 3, 4; 13, 14 after velocity_set_domain):
+    # per-phase wall stress and velocity, the boundary-stress momentum term that reads them, and
+    # the per-phase wall velocity in the next step's density and strain passes
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare(g, o, GPU_FIELDS + ["vx_water", "vx_soil"]) == {}, f"step {s + 1}"
+    assert np.abs(g.get("vx_soil")[deck.type == 0]).max() > 0.0
+
+
+@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN)
 def test_gpu_vs_reference_golden_models(name):
     # against the reference itself: within the oracle-Jacobi distance plus the device
     # transcendental allowance

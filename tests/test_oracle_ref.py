@@ -32,7 +32,8 @@ def _mismatches(sim, ref):
 
 GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1", "moving_box_m2", "moving_box_m3",
           "art_stress_2d", "art_stress_3d", "ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3",
-          "ssi_cpl_2d", "ssi_cpl_3d"]
+          "ssi_cpl_2d", "ssi_cpl_3d", "tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3",
+          "tran_3d_b14"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
