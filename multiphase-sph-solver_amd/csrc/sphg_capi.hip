@@ -468,7 +468,8 @@ static int check_supported(const sphg_problem &p) {
     if (p.flknl > 2) return SPHG_E_UNSUPPORTED;  // kernel_gradient_correction
     if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
     if (p.fli > 1 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
-    if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 13 && p.flbndy != 14)
+    if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 5 && p.flbndy != 13 &&
+        p.flbndy != 14)
         return SPHG_E_UNSUPPORTED;
     if (p.if_move < 0 || p.if_move > 3) return SPHG_E_UNSUPPORTED;
     if (p.step_regu <= 0) return SPHG_E_ARG;  // l % step_regu
@@ -734,8 +735,9 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         if (c->P.ss_type != 0) {
             LAUNCH("ssi_soil", k_ssi_soil<NDIM>, grid_for(n), c->S, c->P, cur);
             LAUNCH("ssi_struct", k_ssi_struct, grid_for(n), c->S, c->P);
-            LAUNCH("accel_tail", k_accel_tail<NDIM>, grid_for(n), c->S, c->P, cur);
         }
+        if (c->P.ss_type != 0 || c->P.flbndy == 5)
+            LAUNCH("accel_tail", k_accel_tail<NDIM>, grid_for(n), c->S, c->P, cur);
     }
     if (phase_mask & 32) {
         LAUNCH("xsph", (k_xsph<NDIM, M>), grid_for(n), c->S, c->P, cur);

@@ -1569,13 +1569,14 @@ __device__ __forceinline__ void av_pair(const double vi[3], const double vj[3], 
 // boundary_moving_stress_effect pair term (CPP_9-4-derivation_stress.cpp:631-721); bi = as / rho_i^2
 template <class M>
 __device__ __forceinline__ void bnd_pair(const double as[6], const double bi[6], double mj, double rj,
-                                         const double w[4], double bnd_t[3][3]) {
+                                         const double w[4], double bnd_t[3][3], bool neg) {
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
             const int sx = kSidx[a][b];
-            bnd_t[a][b] = bnd_t[a][b] + mj * (M::div(M::div(as[sx], rj), rj) + bi[sx]) * w[b];
+            const double aj = (neg && sx >= 3) ? -as[sx] : as[sx];
+            bnd_t[a][b] = bnd_t[a][b] + mj * (M::div(M::div(aj, rj), rj) + bi[sx]) * w[b];
         }
 }
 
@@ -1656,11 +1657,11 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
 }
 
 // The end of the momentum pass for particle i, from its summed acceleration ax: the geostress
-// damping clamp and veocity_update (CPP_11-xsph.cpp:11-142); stores ax, vx (and the moving
-// boundary's interfss).
+// damping clamp (accel_damp) and veocity_update (accel_vel, CPP_11-xsph.cpp:11-142), which stores
+// ax, vx (and the moving boundary's interfss).
 template <int NDIM>
-__device__ __forceinline__ void accel_tail(const State &S, const Params &P, int i, int ti, int cur, double ax[3],
-                                           const double vi[3], const double axp[3], bool damp_on) {
+__device__ __forceinline__ void accel_damp(const State &S, const Params &P, int i, double ax[3], const double vi[3],
+                                           const double axp[3], bool damp_on) {
     const double dt = P.dt;
     if (damp_on) {
         // soil_damping_geostress clamp (CPP_9-4-derivation_stress.cpp:874-895)
@@ -1675,6 +1676,12 @@ __device__ __forceinline__ void accel_tail(const State &S, const Params &P, int 
             else ax[k] -= damp[k];
         }
     }
+}
+
+template <int NDIM>
+__device__ __forceinline__ void accel_vel(const State &S, const Params &P, int i, int ti, int cur, double ax[3],
+                                          const double vi[3], const double axp[3]) {
+    const double dt = P.dt;
     // veocity_update (CPP_11-xsph.cpp:11-142)
     if (P.if_move == 1) {
         if (ti == 0) {
@@ -1900,7 +1907,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     // artificial stress (ti == 2) and geostress damping (ti 2 / 4, l <= inip): pair sums from k_accel_pre
     const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
-    const bool bstress = ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2);
+    const bool bstress = ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2 || P.flbndy == 5);
+    // boundary_effect_free_struct (flbndy 5, CPP_9-4-derivation_stress.cpp:900-990): the wall's
+    // shear stress is the fluid particle's own with the sign flipped
+    const bool bneg = P.flbndy == 5 && (ti == 1 || ti == 3);
     const bool drag_on = P.fli == 1;
     // i-side factors of the pair terms (loop invariants of the reference's expressions)
     double bi[6];
@@ -1958,7 +1968,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                     }
             }
             if (tj == ti) av_pair<NDIM, M>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
-            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t);
+            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t, bneg);
             if (drag_on && tj == 2) {
                 // interact1 pair sum (CPP_10-interact1.cpp:40-88), water / air against soil
                 double inprod = 0.0, dxx[3], dv[3];
@@ -2041,7 +2051,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 }
                 av_pair<NDIM, M>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
             }
-            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t);
+            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t, bneg);
             if (drag_on && ti == 2 && (tj == 1 || tj == 3)) {
                 // interact1 pair sum (CPP_10-interact1.cpp:40-88), soil against water / air
                 double inprod = 0.0, dxx[3], dv[3];
@@ -2105,7 +2115,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         for (int k = 0; k < NDIM; k++) ax[k] = ax[k] - av[k];
     if (art_on)
         for (int k = 0; k < NDIM; k++) ax[k] = ax[k] + S.tmp6[k][i];
-    if (ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2)) {
+    if (ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2 || P.flbndy == 5)) {
         ax[0] += bnd_t[0][0] + bnd_t[0][1] + bnd_t[0][2];
         ax[1] += bnd_t[1][0] + bnd_t[1][1] + bnd_t[1][2];
         ax[2] += bnd_t[2][0] + bnd_t[2][1] + bnd_t[2][2];
@@ -2136,14 +2146,16 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             S.interf[k][i] = itf;
         }
     }
-    if (P.ss_type != 0) {
-        // soil-structure interaction adds to ax before the damping clamp and the velocity update
-        // (CPP_1-SPHLoops.cpp:662-694): k_ssi_* and k_accel_tail finish the pass
+    if (P.ss_type != 0 || P.flbndy == 5) {
+        // soil-structure interaction adds to ax before the damping clamp, the free-solid wall
+        // contact (flbndy 5) after it, both before the velocity update (CPP_1-SPHLoops.cpp:662-705):
+        // k_ssi_* and k_accel_tail finish the pass
         #pragma unroll
         for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
         return;
     }
-    accel_tail<NDIM>(S, P, i, ti, cur, ax, vi, axp, damp_on);
+    accel_damp<NDIM>(S, P, i, ax, vi, axp, damp_on);
+    accel_vel<NDIM>(S, P, i, ti, cur, ax, vi, axp);
 }
 
 // ------------------------------------------------------------------------------------------

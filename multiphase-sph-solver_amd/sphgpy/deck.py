@@ -526,16 +526,22 @@ def ssi_block(ndim=2, ss_type=1, nx=8, ny=4, nz=5, dx=0.002, steps=10, seed=21, 
     return deck
 
 
-def tran_box(ndim=2, flbndy=3, nx=8, ny=3, nz_soil=4, nz_water=3, dx=0.002, steps=8, seed=31, v0=0.2) -> Deck:
+def tran_box(ndim=2, flbndy=3, nx=8, ny=3, nz_soil=4, nz_water=3, dx=0.002, steps=8, seed=31, v0=0.2,
+             gap=None, myu=0.3, zeta=0.5, vdown=0.0) -> Deck:
     """Soil under water in a box whose walls use the boundary treatment of Tran et al.
     (flbndy 3 no slip, 4 free slip, 13 / 14 the same after velocity_set_domain;
     CPP_13-boundary.cpp:711-1237, CPP_9-4-derivation_stress.cpp:725-820). Soil (DP, void ratio
     0.6) and water start sliding at v0 along the floor, so that the wall velocities and stresses
-    each phase sees differ from zero."""
+    each phase sees differ from zero. flbndy 5 (free-solid walls, CPP_13-boundary.cpp:1239-1515,
+    CPP_9-4-derivation_stress.cpp:900-990) takes the wall friction myu and factor zeta from
+    Para_SSInter; `gap` moves the soil and water down so that their lowest row sits gap dx above
+    the floor's top row (contact needs less than 0.5 dx); vdown adds a velocity towards the floor."""
     rng = np.random.default_rng(seed)
     grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
     d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, flbndy=flbndy, coe_bndy=0.1,
              art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"tran_box_{ndim}d_b{flbndy}")
+    if flbndy == 5:
+        d.ss_myu, d.ss_zeta = myu, zeta
     d.model_flags[2] = 1
     d.soil_layers[2] = [dp_layer(c=300.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0, eps0=0.6)]
     w = default_fluid()
@@ -551,7 +557,10 @@ def tran_box(ndim=2, flbndy=3, nx=8, ny=3, nz_soil=4, nz_water=3, dx=0.002, step
                  (_box_walls_3d(nx, ny, top + 3, dx), 0, 0)]
     deck = _assemble(d, parts, rng)
     moving = (deck.type == 1) | (deck.type == 2)
+    if gap is not None:
+        deck.xp[moving, ndim - 1] -= (1.0 - gap) * dx
     deck.vxp[moving, 0] = v0
+    deck.vxp[moving, ndim - 1] = -vdown
     return deck
 
 

@@ -1537,12 +1537,15 @@ static void or_art_visc(or_ctx *c) {
 }
 
 /* boundary_moving_stress_effect, flbndy 1/2 (CPP_9-4-derivation_stress.cpp:631-721) */
+/* boundary_moving_stress_effect (CPP_9-4-derivation_stress.cpp:631-721, flbndy 1/2): the wall
+ * mirrors the particle's own stress; boundary_effect_free_struct (:900-990, flbndy 5) the same with
+ * the shear components negated for fluids */
 static void or_bndy_stress(or_ctx *c) {
-    int n = c->n, i;
+    int n = c->n, i, free_struct = c->pro.flbndy == 5;
 #pragma omp parallel for schedule(static) num_threads(c->nthreads)
     for (i = 0; i < n; i++) {
         OPar *pi = &c->p[i];
-        double t[3][3] = {{0}}, as[6];
+        double t[3][3] = {{0}}, as[6], aj[6];
         int nc = nbcount(&c->pc[i]), nj, k;
         if (pi->type == 0 || pi->type == 7) continue;
         for (nj = 1; nj <= nc; nj++) {
@@ -1557,7 +1560,10 @@ static void or_bndy_stress(or_ctx *c) {
                 for (k = 0; k < 3; k++) as[k] = pi->sig[k] - satu * pi->prew + (1 - satu) * pi->prea;
                 for (k = 3; k < 6; k++) as[k] = pi->sig[k];
             }
-            sdiv(t, pj->mass, as, pj->rho, as, pi->rho, 1, c->pc[i].wij[nj]);
+            for (k = 0; k < 6; k++) aj[k] = as[k];
+            if (free_struct && (pi->type == 1 || pi->type == 3))
+                for (k = 3; k < 6; k++) aj[k] = -as[k];
+            sdiv(t, pj->mass, aj, pj->rho, as, pi->rho, 1, c->pc[i].wij[nj]);
         }
         add_sum(pi->ax, t);
     }
@@ -2473,12 +2479,150 @@ static void or_ssi(or_ctx *c) {
     free(ft);
 }
 
+/* boundary_free_solid (CPP_13-boundary.cpp:1239-1515), flbndy 5: contact of a particle with the
+ * wall line (2D: types 1-4) or triangle (3D: every particle, as the reference's loop has no type
+ * test) of its nearest boundary particles, with the interaction_ss_rigid geometry, the walls'
+ * interfss as their velocity, and zeta f / m added to ax directly; 2D adds 0.0 to ax[2] */
+static void or_free_solid(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i, k;
+    double dt = c->pro.dt, dr = c->pro.dr, myu = c->pro.ss_myu, zeta = c->pro.ss_zeta;
+    OLine line = {0.0, 0.0, 1.0, 0.0, 0.0};
+    OPlaneN plane = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        int ids[SPHG_MAX_NB + 1], total = 0, nc = nbcount(&c->pc[i]), nj, id1, id2, id3;
+        double dsts[SPHG_MAX_NB + 1], vs[3] = {0, 0, 0}, vn[3] = {0, 0, 0}, vt[3] = {0, 0, 0}, fn[3] = {0, 0, 0},
+                                       ft[3] = {0, 0, 0}, ms, dp, mag_fn, mag_ft;
+        OPt p1, p2, p3, p4, outp;
+        if (ndim == 2 && !(pi->type >= 1 && pi->type <= 4)) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double d;
+            if (pj->type != 0) continue;
+            ids[total] = j;
+            d = (pi->xp[0] - pj->xp[0]) * (pi->xp[0] - pj->xp[0]) + (pi->xp[1] - pj->xp[1]) * (pi->xp[1] - pj->xp[1]) +
+                (pi->xp[2] - pj->xp[2]) * (pi->xp[2] - pj->xp[2]);
+            dsts[total] = sqrt(d);
+            total++;
+        }
+        dist_sorting(total, ids, dsts);
+        p4.x = pi->xp[0]; p4.y = pi->xp[1]; p4.z = pi->xp[2];
+        if (ndim == 2) {
+            OVec a, b, q, v0, v1;
+            double u, den;
+            if (total < 2) continue;
+            id1 = ids[0]; id2 = ids[1];
+            p1.x = c->p[id1].xp[0]; p1.y = c->p[id1].xp[1]; p1.z = c->p[id1].xp[2];
+            p2.x = c->p[id2].xp[0]; p2.y = c->p[id2].xp[1]; p2.z = c->p[id2].xp[2];
+            den = p2.y * p1.x - p1.y * p2.x;
+            line.A = (p1.y - p2.y) / den;
+            line.B = (p2.x - p1.x) / den;
+            dp = ss_dpline(p4, &line);
+            outp = ss_map_line(p4, &line);
+            a.x = p1.x; a.y = p1.y; a.z = p1.z; b.x = p2.x; b.y = p2.y; b.z = p2.z;
+            q.x = outp.x; q.y = outp.y; q.z = 0.0;
+            v0 = vsub(b, a); v1 = vsub(q, a);
+            u = 1.0 / vdot(v0, v0) * vdot(v0, v1);
+            if (u < 0 || u > 1) continue;
+            if (!(dp < 0.5 * dr)) continue;
+            {
+                double t = sqrt(line.A * line.A + line.B * line.B), dot;
+                OVec e, nn;
+                if (t > 1.0e-10) { line.nx = line.A / t; line.ny = line.B / t; }
+                e.x = outp.x - p4.x; e.y = outp.y - p4.y; e.z = 0.0;
+                nn.x = line.nx; nn.y = line.ny; nn.z = 0.0;
+                dot = vdot(e, nn);
+                if (!(dot <= 0.0)) { line.nx = -line.nx; line.ny = -line.ny; }
+            }
+            ss_interp(p1, p2, outp, c->p[id1].mass, c->p[id2].mass, c->p[id1].interfss, c->p[id2].interfss, &ms, vs);
+            vs[0] -= pi->vxp[0];
+            vs[1] -= pi->vxp[1];
+            {
+                double v = vs[0] * line.nx + vs[1] * line.ny;
+                vn[0] = v * line.nx; vn[1] = v * line.ny;
+                vt[0] = vs[0] - vn[0]; vt[1] = vs[1] - vn[1];
+            }
+            fn[0] = pi->mass * vn[0] / dt; fn[1] = pi->mass * vn[1] / dt;
+            ft[0] = pi->mass * vt[0] / dt; ft[1] = pi->mass * vt[1] / dt;
+            mag_fn = sqrt(fn[0] * fn[0] + fn[1] * fn[1]);
+            mag_ft = sqrt(ft[0] * ft[0] + ft[1] * ft[1]);
+            if (mag_ft > (mag_fn * myu)) {
+                vt[0] = vt[0] * myu * mag_fn / mag_ft; vt[1] = vt[1] * myu * mag_fn / mag_ft;
+                ft[0] = pi->mass * vt[0] / dt; ft[1] = pi->mass * vt[1] / dt;
+            }
+            pi->ax[0] += zeta * (ft[0] + fn[0]) / pi->mass;
+            pi->ax[1] += zeta * (ft[1] + fn[1]) / pi->mass;
+            pi->ax[2] += 0.0;
+        } else {
+            OVec A, B, C, P, v0, v1, v2;
+            double d00, d01, d02, d11, d12, inv, u, v, tms, tvs[3];
+            OLine3 l31, l32;
+            OPt sect;
+            if (total < 3) continue;
+            id1 = ids[0]; id2 = ids[1]; id3 = ids[2];
+            p1.x = c->p[id1].xp[0]; p1.y = c->p[id1].xp[1]; p1.z = c->p[id1].xp[2];
+            p2.x = c->p[id2].xp[0]; p2.y = c->p[id2].xp[1]; p2.z = c->p[id2].xp[2];
+            p3.x = c->p[id3].xp[0]; p3.y = c->p[id3].xp[1]; p3.z = c->p[id3].xp[2];
+            {
+                OPlane q;
+                cal_plane(p1, p2, p3, &q);
+                plane.A = q.A; plane.B = q.B; plane.C = q.C;
+            }
+            dp = ss_dpplane(p4, &plane);
+            outp = ss_map_plane(p4, &plane);
+            A.x = p1.x; A.y = p1.y; A.z = p1.z; B.x = p2.x; B.y = p2.y; B.z = p2.z;
+            C.x = p3.x; C.y = p3.y; C.z = p3.z; P.x = outp.x; P.y = outp.y; P.z = outp.z;
+            v0 = vsub(C, A); v1 = vsub(B, A); v2 = vsub(P, A);
+            d00 = vdot(v0, v0); d01 = vdot(v0, v1); d02 = vdot(v0, v2); d11 = vdot(v1, v1); d12 = vdot(v1, v2);
+            inv = 1 / (d00 * d11 - d01 * d01);
+            u = (d11 * d02 - d01 * d12) * inv;
+            if (u < 0 || u > 1) continue;
+            v = (d00 * d12 - d01 * d02) * inv;
+            if (v < 0 || v > 1) continue;
+            if (!(u + v <= 1)) continue;
+            if (!(dp < 0.5 * dr)) continue;
+            {
+                double t = sqrt(plane.A * plane.A + plane.B * plane.B + plane.C * plane.C), dot;
+                OVec e, nn;
+                if (t > 1.0e-10) { plane.nx = plane.A / t; plane.ny = plane.B / t; plane.nz = plane.C / t; }
+                e.x = outp.x - p4.x; e.y = outp.y - p4.y; e.z = outp.z - p4.z;
+                nn.x = plane.nx; nn.y = plane.ny; nn.z = plane.nz;
+                dot = vdot(e, nn);
+                if (!(dot <= 0.0)) { plane.nx = -plane.nx; plane.ny = -plane.ny; plane.nz = -plane.nz; }
+            }
+            ss_line3(p1, p2, &l31);
+            ss_line3(p3, outp, &l32);
+            sect.x = sect.y = sect.z = 0.0;
+            if (ss_intersect(&l31, &l32, &sect)) continue;
+            ss_interp(p1, p2, sect, c->p[id1].mass, c->p[id2].mass, c->p[id1].interfss, c->p[id2].interfss, &tms, tvs);
+            ss_interp(p3, sect, outp, c->p[id3].mass, tms, c->p[id3].interfss, tvs, &ms, vs);
+            for (k = 0; k < 3; k++) vs[k] -= pi->vxp[k];
+            {
+                double w = vs[0] * plane.nx + vs[1] * plane.ny + vs[2] * plane.nz;
+                vn[0] = w * plane.nx; vn[1] = w * plane.ny; vn[2] = w * plane.nz;
+                for (k = 0; k < 3; k++) vt[k] = vs[k] - vn[k];
+            }
+            for (k = 0; k < 3; k++) fn[k] = pi->mass * vn[k] / dt;
+            for (k = 0; k < 3; k++) ft[k] = pi->mass * vt[k] / dt;
+            mag_fn = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+            mag_ft = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+            if (mag_ft > (mag_fn * myu)) {
+                for (k = 0; k < 3; k++) vt[k] = vt[k] * myu * mag_fn / mag_ft;
+                for (k = 0; k < 3; k++) ft[k] = pi->mass * vt[k] / dt;
+            }
+            for (k = 0; k < 3; k++) pi->ax[k] += zeta * (ft[k] + fn[k]) / pi->mass;
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ one step */
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
     if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
     if (P->fli > 1 || P->ss_type < 0 || P->ss_type > 3) return 1;
-    if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 13 && P->flbndy != 14)
+    if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 5 && P->flbndy != 13 &&
+        P->flbndy != 14)
         return 1;
     return 0;
 }
@@ -2519,12 +2663,13 @@ int or_phase(or_ctx *c, int mask) {
         or_solid_accel(c, 4);
         or_art_visc(c);
         if (c->pro.art_stre_coe > 0.0) or_art_stress(c);
-        if (c->pro.flbndy == 1 || c->pro.flbndy == 2) or_bndy_stress(c);
+        if (c->pro.flbndy == 1 || c->pro.flbndy == 2 || c->pro.flbndy == 5) or_bndy_stress(c);
         else or_bndy_stress_tran(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3 && c->pro.ss_acc_type == 2) or_coupled(c);
         if (c->pro.fli == 1) or_interact1(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3) or_ssi(c);
         if (c->l <= c->pro.inip) or_damping_geostress(c);
+        if (c->pro.flbndy == 5) or_free_solid(c);
         or_velocity(c);
     }
     if (mask & 32) {

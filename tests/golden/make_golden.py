@@ -47,6 +47,8 @@ def decks():
         "tran_2d_b14": (tran_box(2, 14, steps=6), 6, (1, 3, 6)),
         "tran_3d_b3": (tran_box(3, 3, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 5, (1, 5)),
         "tran_3d_b14": (tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 5, (1, 5)),
+        "free_solid_2d": (tran_box(2, 5, steps=6, gap=0.4, vdown=0.3), 6, (1, 3, 6)),
+        "free_solid_3d": (tran_box(3, 5, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, gap=0.4, vdown=0.3), 5, (1, 5)),
     }
 
 

@@ -245,6 +245,20 @@ def test_soil_structure_interaction_bit_exact_vs_oracle(name):
 
 
 TRAN = ["tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3", "tran_3d_b14"]
+FREE_SOLID = ["free_solid_2d", "free_solid_3d"]
+
+
+@pytest.mark.parametrize("name", FREE_SOLID)
+def test_free_solid_walls_bit_exact_vs_oracle(name):
+    # flbndy 5: mirrored wall stress with the fluid shear sign flipped, and the rigid contact
+    # against the wall line / triangle after the damping clamp
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare(g, o) == {}, f"step {s + 1}"
 
 
 @pytest.mark.parametrize("name", TRAN)
@@ -262,7 +276,7 @@ def test_tran_boundaries_bit_exact_vs_oracle(name):
     assert np.abs(g.get("vx_soil")[deck.type == 0]).max() > 0.0
 
 
-@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN)
+@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN + FREE_SOLID)
 def test_gpu_vs_reference_golden_models(name):
     # against the reference itself: within the oracle-Jacobi distance plus the device
     # transcendental allowance

@@ -33,7 +33,7 @@ def _mismatches(sim, ref):
 GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1", "moving_box_m2", "moving_box_m3",
           "art_stress_2d", "art_stress_3d", "ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3",
           "ssi_cpl_2d", "ssi_cpl_3d", "tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3",
-          "tran_3d_b14"]
+          "tran_3d_b14", "free_solid_2d", "free_solid_3d"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -135,3 +135,32 @@ def test_golden_fixture_states_are_finite(name):
     for s, st in states.items():
         for k in ("xp", "vx", "sig", "rho"):
             assert np.isfinite(st[k]).all(), (s, k)
+
+
+@pytest.mark.parametrize("name", ["free_solid_2d", "free_solid_3d"])
+def test_free_solid_wall_contact_closed_form(name):
+    # flbndy 5 (boundary_free_solid): the soil and water start 0.4 dx above the floor moving at
+    # (0.2 along, 0.3 towards the floor); a particle in contact with the resting floor gets
+    # zeta m (v_wall - v) / dt split along the floor normal with the tangential part capped at
+    # myu |f_n|, added to ax: the difference to the same deck with zeta = 0 at step 1
+    deck, states = load_golden(name)
+    a = OracleSim(deck, states[0], mode=GS)
+    d0 = deck.copy()
+    d0.ss_zeta = 0.0
+    b = OracleSim(d0, states[0], mode=GS)
+    assert a.step(1) == 0 and b.step(1) == 0
+    dax = a.get("ax") - b.get("ax")
+    hit = np.abs(dax).sum(axis=1) > 0
+    assert hit.sum() >= 4
+    nrm = deck.ndim - 1
+    zeta, myu, dt = deck.ss_zeta, deck.ss_myu, deck.dt
+    # particles next to a side wall may take a tilted wall line (the nearest-two / -three choice
+    # follows dist_sorting's id-only swaps); the closed form holds over the floor's interior
+    xs = states[0]["xp"][:, : deck.ndim - 1] / deck.dr
+    span = xs[deck.type == 2].max(axis=0)
+    interior = np.all((xs > 1.0) & (xs < span - 1.0), axis=1)
+    assert (hit & interior).sum() >= 1
+    for i in np.nonzero(hit & interior)[0]:
+        assert deck.type[i] in (1, 2)
+        assert dax[i, nrm] == pytest.approx(zeta * 0.3 / dt, rel=2e-5)
+        assert dax[i, 0] == pytest.approx(-zeta * min(0.2, myu * 0.3) / dt, rel=2e-5)
