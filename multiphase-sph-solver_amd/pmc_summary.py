@@ -2,6 +2,7 @@
 
     python multiphase-sph-solver_amd/pmc_summary.py <kt_dir> <fetch_dir> <write_dir> <cfetch_dir> <cwrite_dir> \
         <workload> <out.md> <out.json>
+    python multiphase-sph-solver_amd/pmc_summary.py --counters <out.md> <pmc_dir> [<pmc_dir> ...]
 
 Inputs are the output directories of: rocprofv3 --kernel-trace --stats (bench), --pmc FETCH_SIZE
 (bench), --pmc WRITE_SIZE (bench), and the two PMC passes over copy_calibration.py (1 GiB copy).
@@ -71,5 +72,28 @@ def main(kt, fe, wr, cfe, cwr, workload, out_md, out_json):
             fo.write(f"| {k} | {c} | {us:.1f} | {pct:.1f} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {vg[0]} | {vg[2]} |\n")
 
 
+def counter_table(out_md, dirs):
+    """Per-kernel average of every counter in the given --pmc output directories, appended to
+    out_md as one table (rows: step kernels, columns: counters)."""
+    vals = collections.defaultdict(dict)
+    for d in dirs:
+        per = collections.defaultdict(lambda: collections.defaultdict(list))
+        for r in _csv(d, "counter_collection.csv"):
+            if "sphg" in r["Kernel_Name"]:
+                per[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        for k, cs in per.items():
+            for c, v in cs.items():
+                vals[k][c] = sum(v) / len(v)
+    cols = sorted({c for v in vals.values() for c in v})
+    with open(out_md, "a") as fo:
+        fo.write("\n| kernel | " + " | ".join(cols) + " |\n")
+        fo.write("|---" * (len(cols) + 1) + "|\n")
+        for k in sorted(vals):
+            fo.write(f"| {k} | " + " | ".join(f"{vals[k].get(c, float('nan')):.4g}" for c in cols) + " |\n")
+
+
 if __name__ == "__main__":
-    main(*sys.argv[1:9])
+    if sys.argv[1] == "--counters":
+        counter_table(sys.argv[2], sys.argv[3:])
+    else:
+        main(*sys.argv[1:9])
