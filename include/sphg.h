@@ -194,6 +194,16 @@ int sphg_update(sphg_ctx *ctx);
  * sequence is captured once into a hipGraph and replayed. Returns the sticky error word. */
 int sphg_step(sphg_ctx *ctx, int nsteps);
 int sphg_sync(sphg_ctx *ctx);
+/* The time loop of SPH_Calculation (CPP_1-SPHLoops.cpp:478-770) over the context: for
+ * l = current step .. loop, while t < tt, t += dt and one step; output after step 1 and after
+ * every fop-th step (CPP_1-SPHLoops.cpp:743), and once more, then stop, when t has reached tt
+ * (:761-767). Steps between two outputs run as one batch of graph replays. `out` (may be null)
+ * gets the step number and time with the state on the device complete; a nonzero return stops
+ * the loop and is returned. A step error (the reference's Err_Deal codes 2/3/5, ...) ends the loop
+ * and is returned; the state is then that of the end of the batch. *t is the simulated time in
+ * and out (pPPro->t). */
+typedef int (*sphg_output_fn)(void *user, int step, double t);
+int sphg_run(sphg_ctx *ctx, int loop, int fop, double tt, double *t, sphg_output_fn out, void *user);
 /* Serving (no counterpart in the reference, which runs one simulation per process): enqueue
  * nsteps steps on the context's stream and return at once; sphg_poll returns SPHG_BUSY while they
  * run, then the sticky error word. Contexts on one device overlap their work. */

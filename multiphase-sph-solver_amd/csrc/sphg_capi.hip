@@ -883,6 +883,40 @@ extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
 
 // Serving: steps are enqueued on the context's stream without waiting, so several contexts'
 // work overlaps on the device; sphg_poll reports whether the stream has drained.
+extern "C" int sphg_run(sphg_ctx *c, int loop, int fop, double tt, double *t, sphg_output_fn out, void *user) {
+    if (!c || !c->uploaded || !t || fop <= 0) return SPHG_E_ARG;
+    const double dt = c->P.dt;
+    int l = c->step;
+    while (l <= loop) {
+        if (!(*t < tt)) {  // the reference's else branch: output and leave the loop
+            if (out) {
+                int r = out(user, l, *t);
+                if (r) return r;
+            }
+            break;
+        }
+        // steps l .. next, next = the first output step at or after l, all with t < tt at their start
+        int next = l == 1 ? 1 : ((l + fop - 1) / fop) * fop;
+        if (next > loop) next = loop;
+        int k = 0;
+        double tk = *t;
+        while (l + k <= next && tk < tt) {
+            tk = tk + dt;
+            k++;
+        }
+        int rc = sphg_step(c, k);
+        *t = tk;
+        l += k;
+        if (rc) return rc;
+        const int last = l - 1;
+        if (out && (last % fop == 0 || last == 1)) {
+            int r = out(user, last, *t);
+            if (r) return r;
+        }
+    }
+    return SPHG_OK;
+}
+
 extern "C" int sphg_step_async(sphg_ctx *c, int nsteps) {
     if (!c || !c->uploaded || nsteps < 0 || c->timing) return SPHG_E_ARG;
     ensure_regu(c, nsteps);

@@ -320,6 +320,8 @@ def product_lib() -> _Lib:
         L.sphg_step_async.restype = _I
         L.sphg_poll.argtypes = [C.c_void_p]
         L.sphg_poll.restype = _I
+        L.sphg_run.argtypes = [C.c_void_p, _I, _I, C.c_double, C.POINTER(C.c_double), OUTPUT_FN, C.c_void_p]
+        L.sphg_run.restype = _I
         for nm in ("parttocell", "partisearching", "density", "strain", "strain_to_stress", "accel", "xsph",
                    "update"):
             f = getattr(L, "sphg_" + nm)
@@ -327,6 +329,9 @@ def product_lib() -> _Lib:
             f.restype = _I
         _product = lib
     return _product
+
+
+OUTPUT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double)
 
 
 class GpuSim(Sim):
@@ -350,6 +355,14 @@ class GpuSim(Sim):
 
     def sync(self) -> int:
         return int(self.L.lib.sphg_sync(self.ctx))
+
+    def run(self, loop: int, fop: int, tt: float, t: float = 0.0, output=None):
+        """The reference time loop (sphg_run): output(step, t) after step 1, every fop-th step and at
+        t >= tt; returns (error code, simulated time)."""
+        cb = OUTPUT_FN(lambda user, l, tm: int(output(l, tm) or 0)) if output else OUTPUT_FN()
+        tv = C.c_double(t)
+        rc = self.L.lib.sphg_run(self.ctx, loop, fop, tt, C.byref(tv), cb, None)
+        return int(rc), tv.value
 
     def set_step(self, l: int):
         self.L.lib.sphg_set_step(self.ctx, l)

@@ -345,3 +345,35 @@ def test_batch_realisations_equal_standalone_runs():
         assert np.array_equal(cb[a:z], cg)
         ib_r = ib[a:z]
         assert np.array_equal(np.where(ib_r >= 0, ib_r - a, -1), ig)
+
+
+def test_time_loop_driver_outputs_and_matches_stepping():
+    # sphg_run: the reference's loop cadence (output after step 1 and every fop-th step, stop at
+    # t >= tt), steps batched between outputs; the states it reports equal stepping by hand and sit
+    # on the reference's fixture states within the order-free normalisation bound
+    deck, states = load_golden("column2d_small")
+    g = make_gpu(deck, states[0])
+    h = make_gpu(deck, states[0])
+    seen = []
+    done = [0]
+
+    def out(l, t):
+        seen.append((l, t))
+        assert h.step(l - done[0]) == 0
+        done[0] = l
+        for k in ("xp", "vx", "sig", "rho"):
+            assert bitequal(g.get(k), h.get(k)), (l, k)
+        if l in states:
+            oj_state = states[l]
+            assert max_rel(g.get("xp"), oj_state["xp"]) < 1e-9
+        return 0
+
+    rc, t = g.run(8, 4, deck.dt * 9, 0.0, out)
+    assert rc == 0
+    assert [l for l, _ in seen] == [1, 4, 8]
+    assert t == pytest.approx(8 * deck.dt, rel=1e-12)
+    # t reaching tt ends the loop with one more output (CPP_1-SPHLoops.cpp:761-767)
+    g2 = make_gpu(deck, states[0])
+    seen2 = []
+    rc, t2 = g2.run(8, 4, deck.dt * 2.5, 0.0, lambda l, tm: seen2.append(l))
+    assert rc == 0 and seen2 == [1, 4] and g2.L.lib.sphg_current_step(g2.ctx) == 4
