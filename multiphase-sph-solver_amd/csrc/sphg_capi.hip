@@ -167,7 +167,7 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     I(S.type, cap); I(S.matype, cap); I(S.permtype, cap); I(S.etype, cap);
     plan.push_back({(void **)&S.soil_slot, cap});
     plan.push_back({(void **)&S.fluid_slot, cap});
-    I(S.cell, cap); I(S.sorted, cap); I(S.nb_cnt, cap);
+    I(S.cell, cap); I(S.sorted, cap); I(S.nb_cnt, cap); I(S.nb_types, cap);
     I(S.nbr, cap * kMaxNb);
     for (int k = 0; k < 6; k++) D(S.qsig[k]);
     D(S.fdens);
@@ -196,10 +196,6 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
             if (tran) D(S.sigb[t][k]);
             else S.sigb[t][k] = nullptr;
         }
-    for (int k = 0; k < 3; k++) {
-        if (tran) D(S.bndt[k]);
-        else S.bndt[k] = nullptr;
-    }
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
@@ -729,7 +725,6 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
     }
     if (phase_mask & 16) {
         if (c->pre_on) LAUNCH("accel_pre", (k_accel_pre<NDIM, M>), grid_for(n), c->S, c->P, cur);
-        if (tran_bndy(c->P)) LAUNCH("bndy_tran", (k_bndy_tran<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.coupled) LAUNCH("coupled", (k_coupled<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("accel", (k_accel<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.ss_type != 0) {

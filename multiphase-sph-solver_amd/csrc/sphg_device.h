@@ -90,6 +90,7 @@ struct State {
     // cell list
     int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *block_sums;
     int *nb_cnt, *nbr;    // list entries: id | type << 28, column-major nbr[k * cap + i]
+    int *nb_types;        // bit t set when the list holds a neighbour of type t
     double *qsig[6];      // the j factor of the momentum sums: sig / rho / rho (solids),
                           // (porosity * sig) / rho / rho (fluids)
     double *fdens;        // density of the particle's fluid slot (drag partner density)
@@ -101,9 +102,8 @@ struct State {
     double *ssi_c[3][6];
     double *cpl[3];       // coupled_Water_Soil_Structure row sums (Para_SSInter acc_type 2)
     // boundary treatment of Tran et al. (flbndy 3/4/13/14): per boundary particle and phase
-    // (water, soil, air, structure) the interpolated stress, and the boundary-stress row sums
+    // (water, soil, air, structure) the interpolated stress
     double *sigb[4][6];
-    double *bndt[3];
     int *err, *step;      // sticky error word; current step number l
     const Grid *grids;    // one per realisation
     const int *rid;       // realisation of each particle

@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     const int i = pidx();
     const int t = threadIdx.x;
     if (i >= P.n) return;
-    int cnt = 0;
+    int cnt = 0, tmask = 0;
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
@@ -421,6 +421,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
                 if (in && i != j) {
                     if (cnt < kStage) stage[cnt][t] = e;
                     else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = e;
+                    tmask |= 1 << nb_type(e);
                     cnt++;
                 }
             }
@@ -428,6 +429,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
         if (cnt > kMaxNb) set_err(S, SPHG_ERR_NB);
     }
     S.nb_cnt[i] = cnt;
+    S.nb_types[i] = tmask;
     const int nk = cnt < kStage ? cnt : kStage;
     for (int k = 0; k < nk; k++) S.nbr[(size_t)k * P.cap + i] = stage[k][t];
 }
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     const bool solid = (ti == 2 || ti == 4), water = ti == 1;
     const int mode = bvel_mode(P, ti), ph = ti - 1;
     double up[3] = {0.0, 0.0, 0.0}, dn[3] = {0.0, 0.0, 0.0}, pup = 0.0, pdn = 0.0;
-    constexpr int B = 4;
+    constexpr int B = 2;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xj[B][3], vj[B][3], hlj[B], mj[B], porj[B], rj[B];
 #pragma unroll
@@ -573,7 +575,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     double up = 0.0, dn = 0.0;
-    constexpr int B = 8;
+    constexpr int B = 6;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xj[B][3], hlj[B], mj[B], rmj[B];
 #pragma unroll
@@ -699,7 +701,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     if (ti == 0) {
         // boundary particles: eos loop 2 only (CPP_7-1-eos_CR.cpp:62-118): water / air pressure and
         // soil stress interpolated onto the wall
-        constexpr int B = 4;
+        constexpr int B = 2;
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
             double xj[B][3], hlj[B], mj[B], rj[B], pj[B], sgj[B][6];
 #pragma unroll
@@ -737,7 +739,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
             }
         });
     } else if (ti != 7) {
-        constexpr int B = 4;
+        constexpr int B = 2;
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
             double xj[B][3], vj[B][3], hlj[B], mj[B], rj[B], pj[B];
 #pragma unroll
@@ -1566,20 +1568,6 @@ __device__ __forceinline__ void av_pair(const double vi[3], const double vj[3], 
     for (int k = 0; k < NDIM; k++) av[k] = av[k] + mj * paix * w[k];
 }
 
-// boundary_moving_stress_effect pair term (CPP_9-4-derivation_stress.cpp:631-721); bi = as / rho_i^2
-template <class M>
-__device__ __forceinline__ void bnd_pair(const double as[6], const double bi[6], double mj, double rj,
-                                         const double w[4], double bnd_t[3][3], bool neg) {
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-            const int sx = kSidx[a][b];
-            const double aj = (neg && sx >= 3) ? -as[sx] : as[sx];
-            bnd_t[a][b] = bnd_t[a][b] + mj * (M::div(M::div(aj, rj), rj) + bi[sx]) * w[b];
-        }
-}
-
 // Pair sums that the momentum kernel adds in the reference's order when they are switched on:
 // artificial stress (soil, same-type pairs) into tmp6[0..2] and the geostress damping sum
 // (soil / structure, same-type pairs, l <= inip) into tmp6[3..5]. Split out so the momentum
@@ -1751,58 +1739,76 @@ __device__ __forceinline__ void accel_vel(const State &S, const Params &P, int i
     }
 }
 
-// boundary_stress_effect_tran (CPP_9-4-derivation_stress.cpp:725-820), flbndy 3/4/13/14: each
-// boundary neighbour contributes the stress its slip pass interpolated for the particle's phase;
-// row sums (t[a][0] + t[a][1]) + t[a][2] for the momentum kernel to add in the boundary-stress slot
+// Boundary-stress momentum term of every non-boundary particle, summed over its boundary
+// neighbours into row sums (t[a][0] + t[a][1]) + t[a][2] that the momentum kernel adds in the
+// boundary-stress slot. The momentum kernel calls it after its neighbour loops, for particles that
+// list a boundary neighbour, so its accumulators do not add to the register peak of those loops:
+//   flbndy 1/2  boundary_moving_stress_effect (CPP_9-4-derivation_stress.cpp:631-721): the wall
+//               mirrors the particle's own stress;
+//   flbndy 5    boundary_effect_free_struct (:900-990): the same with a fluid's shear negated;
+//   flbndy 3/4/13/14  boundary_stress_effect_tran (:725-820): the stress the wall's slip pass
+//               interpolated for the particle's phase.
+// The particle's own stress is effective stress for soil (and, for flbndy 1/2/5, structure).
 template <int NDIM, class M>
-__global__ __launch_bounds__(256) void k_bndy_tran(State S, Params P, int cur) {
-    int i = pidx();
-    if (i >= P.n) return;
-    const int ti = S.type[i];
-    if (ti == 0 || ti == 7) return;
-    double xi[3];
-    load_x<NDIM>(S, cur, i, xi);
-    const double hli = S.hl[i], rhoi = S.rho[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
-    const int nc = nb_count(S, i);
-    double ai[6], t[3][3];
+__device__ __forceinline__ void bndy_stress_rows(const State &S, const Params &P, int cur, int i, int ti,
+                                                 const double xi[3], const KernelCtx &kc, double rows[3]) {
+    const bool tran = tran_bndy(P);
+    double ai[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) ai[k] = 0.0;
-    if (ti == 1 || ti == 3 || ti == 4) {
+    if (ti == 1 || ti == 3 || (tran && ti == 4)) {
 #pragma unroll
         for (int k = 0; k < 6; k++) ai[k] = S.sig[k][i];
-    } else if (ti == 2) {
+    } else if (ti == 2 || ti == 4) {
         const double satu = S.satu[i], prew = S.prew[i], prea = S.prea[i];
 #pragma unroll
         for (int k = 0; k < 3; k++) ai[k] = S.sig[k][i] - satu * prew + (1 - satu) * prea;
 #pragma unroll
         for (int k = 3; k < 6; k++) ai[k] = S.sig[k][i];
     }
-    const int ph = (ti >= 1 && ti <= 4) ? ti - 1 : -1;
+    const int nc = nb_count(S, i);
+    const double hli = S.hl[i], rhoi = S.rho[i];
+    const int ph = (ti >= 1 && ti <= 4) ? ti - 1 : 0;
+    const bool neg = P.flbndy == 5 && (ti == 1 || ti == 3);
+    double bi[6], t[3][3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) bi[k] = M::div(M::div(ai[k], rhoi), rhoi);
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) t[a][b] = 0.0;
-    for (int k = 0; k < nc; k++) {
-        const int e = S.nbr[(size_t)k * P.cap + i];
-        if (nb_type(e) != 0) continue;
-        const int j = nb_id(e);
-        double xj[3], w[4], d2, dst, aj[6];
-        load_xj<NDIM>(S, cur, j, xj);
-        pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hl_of(S, P, j, hli), w, d2, dst);
-        const double mj = S.mass[j], rj = S.rho[j];
+    constexpr int B = 2;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xj[B][3], mj[B], rj[B];
 #pragma unroll
-        for (int q = 0; q < 6; q++) aj[q] = ph >= 0 ? S.sigb[ph < 0 ? 0 : ph][q][j] : 0.0;
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xj[b]);
+            mj[b] = S.mass[j];
+            rj[b] = S.rho[j];
+        }
 #pragma unroll
-        for (int a = 0; a < 3; a++)
+        for (int b = 0; b < B; b++) {
+            if (b >= act || nb_type(e[b]) != 0) continue;
+            const int j = nb_id(e[b]);
+            double w[4], d2, dst, aj[6];
+            pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hl_of(S, P, j, hli), w, d2, dst);
 #pragma unroll
-            for (int b = 0; b < 3; b++) {
-                const int sx = kSidx[a][b];
-                t[a][b] = t[a][b] + mj * (M::div(M::div(aj[sx], rj), rj) + M::div(M::div(ai[sx], rhoi), rhoi)) * w[b];
+            for (int q = 0; q < 6; q++) {
+                if (tran) aj[q] = (ti >= 1 && ti <= 4) ? S.sigb[ph][q][j] : 0.0;
+                else aj[q] = (neg && q >= 3) ? -ai[q] : ai[q];
             }
-    }
 #pragma unroll
-    for (int a = 0; a < 3; a++) S.bndt[a][i] = t[a][0] + t[a][1] + t[a][2];
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int sx = kSidx[a][c];
+                    t[a][c] = t[a][c] + mj[b] * (M::div(M::div(aj[sx], rj[b]), rj[b]) + bi[sx]) * w[c];
+                }
+        }
+    });
+#pragma unroll
+    for (int a = 0; a < 3; a++) rows[a] = t[a][0] + t[a][1] + t[a][2];
 }
 
 // coupled_Water_Soil_Structure (CPP_9-4-derivation_stress.cpp:507-629), Para_SSInter acc_type 2:
@@ -1814,6 +1820,12 @@ __global__ __launch_bounds__(256) void k_coupled(State S, Params P, int cur) {
     if (i >= P.n) return;
     const int ti = S.type[i];
     if (ti != 4 && ti != 2 && ti != 1) return;
+    const int partners = ti == 4 ? ((1 << 1) | (1 << 2)) : (1 << 4);
+    if (!(S.nb_types[i] & partners)) {  // empty sums
+#pragma unroll
+        for (int a = 0; a < 3; a++) S.cpl[a][i] = 0.0;
+        return;
+    }
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i], rhoi = S.rho[i];
@@ -1889,33 +1901,16 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     if (ti == 1) { av_c = S.fluid[S.fluid_slot[i]].vpu; av_a = P.art_vis[0][0]; av_b = P.art_vis[0][1]; }
     if (ti == 2) { av_c = S.soil[S.soil_slot[i]].vpu; av_a = P.art_vis[1][0]; av_b = P.art_vis[1][1]; }
     if (ti == 3) { av_c = S.fluid[S.fluid_slot[i]].vpu; av_a = P.art_vis[2][0]; av_b = P.art_vis[2][1]; }
-    // boundary-effect stress of particle i (CPP_9-4-derivation_stress.cpp:667-697)
-    double as[6];
-    #pragma unroll
-    for (int k = 0; k < 6; k++) as[k] = 0.0;
-    if (ti == 1 || ti == 3) {
-        #pragma unroll
-        for (int k = 0; k < 6; k++) as[k] = sigi[k];
-    } else if (ti == 2 || ti == 4) {
-        #pragma unroll
-        for (int k = 0; k < 3; k++) as[k] = sigi[k] - satui * prewi + (1 - satui) * preai;
-        for (int k = 3; k < 6; k++) as[k] = sigi[k];
-    }
-    double fluid_t[3][3], solid_t[3][3], bnd_t[3][3], av[3] = {0, 0, 0}, drag[3] = {0, 0, 0};
+    // the boundary-stress term (pair sum over boundary neighbours) follows the neighbour loops
+    // stress-divergence sums: fluid_acceleration_* for fluids, soil/structure_acceleration for
+    // solids (a particle takes one of the two)
+    double mom_t[3][3], av[3] = {0, 0, 0}, drag[3] = {0, 0, 0};
     for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) fluid_t[a][b] = solid_t[a][b] = bnd_t[a][b] = 0.0;
+        for (int b = 0; b < 3; b++) mom_t[a][b] = 0.0;
     // artificial stress (ti == 2) and geostress damping (ti 2 / 4, l <= inip): pair sums from k_accel_pre
     const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
-    const bool bstress = ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2 || P.flbndy == 5);
-    // boundary_effect_free_struct (flbndy 5, CPP_9-4-derivation_stress.cpp:900-990): the wall's
-    // shear stress is the fluid particle's own with the sign flipped
-    const bool bneg = P.flbndy == 5 && (ti == 1 || ti == 3);
     const bool drag_on = P.fli == 1;
-    // i-side factors of the pair terms (loop invariants of the reference's expressions)
-    double bi[6];
-    #pragma unroll
-    for (int k = 0; k < 6; k++) bi[k] = as[k] / rhoi / rhoi;
     if (ti == 1 || ti == 3) {
         // fluid_acceleration_water/air (+ minor variants), CPP_9-4-derivation_stress.cpp:12-351;
         // qsig of a fluid neighbour holds (porosity * sig) / rho / rho, the j factor of the sum
@@ -1923,7 +1918,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         #pragma unroll
         for (int k = 0; k < 6; k++) fi[k] = (fpor * sigi[k]) / rhoi / rhoi;
         const double fdi = S.fluid[S.fluid_slot[i]].dens;
-        constexpr int B = 2;
+        constexpr int B = 1;  // one neighbour per round trip: 3 waves per SIMD instead of 2
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satjb[B], porjb[B], copjb[B];
 #pragma unroll
@@ -1963,12 +1958,11 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 #pragma unroll
                     for (int b = 0; b < 3; b++) {
                         const int sx = kSidx[a][b];
-                        if (fsgn > 0) fluid_t[a][b] = fluid_t[a][b] + mj * (qj[sx] + fi[sx]) * w[b];
-                        else fluid_t[a][b] = fluid_t[a][b] + mj * (qj[sx] - fi[sx]) * w[b];
+                        if (fsgn > 0) mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] + fi[sx]) * w[b];
+                        else mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] - fi[sx]) * w[b];
                     }
             }
             if (tj == ti) av_pair<NDIM, M>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
-            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t, bneg);
             if (drag_on && tj == 2) {
                 // interact1 pair sum (CPP_10-interact1.cpp:40-88), water / air against soil
                 double inprod = 0.0, dxx[3], dv[3];
@@ -1999,7 +1993,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         #pragma unroll
         for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
         const double copi = S.cop[i], pori = S.porosity[i];
-        constexpr int B = 2;
+        constexpr int B = 1;  // one neighbour per round trip: 3 waves per SIMD instead of 2
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B];
 #pragma unroll
@@ -2033,7 +2027,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 #pragma unroll
                     for (int b = 0; b < 3; b++) {
                         const int sx = kSidx[a][b];
-                        solid_t[a][b] = solid_t[a][b] + mj * (qj[sx] + qi[sx]) * w[b];
+                        mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] + qi[sx]) * w[b];
                     }
                 // pore-pressure term; quirk: the neighbour's saturation is used for both terms
                 if (ti == 2) {
@@ -2041,17 +2035,16 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                     double prej = (1 - spor) * (-satu * prewj + (1 - satu) * preaj);
 #pragma unroll
                     for (int k = 0; k < 3; k++)
-                        solid_t[k][k] = solid_t[k][k] + mj * (M::div(M::div(prej, rj), rj) + M::div(M::div(prei, sdens), sdens)) * w[k];  // quirk A12
+                        mom_t[k][k] = mom_t[k][k] + mj * (M::div(M::div(prej, rj), rj) + M::div(M::div(prei, sdens), sdens)) * w[k];  // quirk A12
                 } else {
                     double prei = -satu * prewi + (1 - satu) * preai;
                     double prej = -satu * prewj + (1 - satu) * preaj;
 #pragma unroll
                     for (int k = 0; k < 3; k++)
-                        solid_t[k][k] = solid_t[k][k] + mj * (M::div(M::div(prej, rj), rj) - M::div(M::div(prei, rhoi), rhoi)) * w[k];
+                        mom_t[k][k] = mom_t[k][k] + mj * (M::div(M::div(prej, rj), rj) - M::div(M::div(prei, rhoi), rhoi)) * w[k];
                 }
                 av_pair<NDIM, M>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
             }
-            if (tj == 0 && bstress) bnd_pair<M>(as, bi, mj, rj, w, bnd_t, bneg);
             if (drag_on && ti == 2 && (tj == 1 || tj == 3)) {
                 // interact1 pair sum (CPP_10-interact1.cpp:40-88), soil against water / air
                 double inprod = 0.0, dxx[3], dv[3];
@@ -2100,11 +2093,13 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             }
         });
     }
+    double brow[3] = {0.0, 0.0, 0.0};
+    if (ti != 0 && ti != 7 && (S.nb_types[i] & 1)) bndy_stress_rows<NDIM, M>(S, P, cur, i, ti, xi, kc, brow);
     // combine in the reference's call order (CPP_1-SPHLoops.cpp:586-660)
     double ax[3] = {S.ax[0][i], S.ax[1][i], S.ax[2][i]};
     const double g[3] = {P.gx, P.gy, P.gz};
     if (ti >= 1 && ti <= 4) {
-        double (*t)[3] = (ti == 1 || ti == 3) ? fluid_t : solid_t;
+        double (*t)[3] = mom_t;
         #pragma unroll
         for (int k = 0; k < 3; k++) ax[k] = g[k];
         ax[0] += t[0][0] + t[0][1] + t[0][2];
@@ -2115,14 +2110,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         for (int k = 0; k < NDIM; k++) ax[k] = ax[k] - av[k];
     if (art_on)
         for (int k = 0; k < NDIM; k++) ax[k] = ax[k] + S.tmp6[k][i];
-    if (ti != 0 && ti != 7 && (P.flbndy == 1 || P.flbndy == 2 || P.flbndy == 5)) {
-        ax[0] += bnd_t[0][0] + bnd_t[0][1] + bnd_t[0][2];
-        ax[1] += bnd_t[1][0] + bnd_t[1][1] + bnd_t[1][2];
-        ax[2] += bnd_t[2][0] + bnd_t[2][1] + bnd_t[2][2];
-    }
-    if (tran_bndy(P) && ti != 0 && ti != 7) {
+    if (ti != 0 && ti != 7) {
+        // boundary stress, every built wall type
         #pragma unroll
-        for (int k = 0; k < 3; k++) ax[k] += S.bndt[k][i];
+        for (int k = 0; k < 3; k++) ax[k] += brow[k];
     }
     if (P.coupled && (ti == 4 || ti == 2 || ti == 1)) {
         #pragma unroll
@@ -2191,7 +2182,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     int cur_id[3] = {-1, -1, -1};
     double d0[3] = {0, 0, 0};
     int total = 0, idm = 0;
-    constexpr int B = 4;
+    constexpr int B = 2;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xj[B][3], hlj[B], mj[B], rj[B], vxj[B][3];
         int mtj[B];
@@ -2600,10 +2591,6 @@ template __global__ void k_coupled<2, Exact>(State, Params, int);
 template __global__ void k_coupled<2, Fast>(State, Params, int);
 template __global__ void k_coupled<3, Exact>(State, Params, int);
 template __global__ void k_coupled<3, Fast>(State, Params, int);
-template __global__ void k_bndy_tran<2, Exact>(State, Params, int);
-template __global__ void k_bndy_tran<2, Fast>(State, Params, int);
-template __global__ void k_bndy_tran<3, Exact>(State, Params, int);
-template __global__ void k_bndy_tran<3, Fast>(State, Params, int);
 template __global__ void k_slip_tran<2, Exact>(State, Params, int);
 template __global__ void k_slip_tran<2, Fast>(State, Params, int);
 template __global__ void k_slip_tran<3, Exact>(State, Params, int);

@@ -92,6 +92,7 @@ __device__ Contact find_contact(const State &S, const Params &P, int cur, int i,
                                 double reach) {
     Contact c;
     c.on = false;
+    if (!(S.nb_types[i] & (1 << wall))) return c;  // no wall particle listed
     const SPt p4 = spt_of(S, cur, i);
     // wall neighbours in list order, their distances over all three coordinates, and the first
     // NDIM ids after clParti_Pair::dist_sorting (Class_particle.cpp:343-359), formed while
