@@ -42,6 +42,15 @@ def test_create_rejects_unknown_arithmetic_form():
     assert not ctx.value
 
 
+def test_time_loop_rejects_bad_arguments():
+    # argument checks of sphg_run run before any device call
+    import ctypes as C
+    from sphgpy.capi import product_lib, OUTPUT_FN
+    L = product_lib().lib
+    t = C.c_double(0.0)
+    assert L.sphg_run(None, 10, 1, 1.0, C.byref(t), OUTPUT_FN(), None) == 100
+
+
 @pytest.mark.parametrize("name", ["column2d_small", "soilwater3d_small"])
 def test_setting_par_matches_reference_init(name):
     deck, states = load_golden(name)
