@@ -212,6 +212,35 @@ int sphg_poll(sphg_ctx *ctx);
 int sphg_current_step(const sphg_ctx *ctx);
 /* Sets the step counter pPPro->l for the next step (CPP_1-SPHLoops.cpp:478). */
 int sphg_set_step(sphg_ctx *ctx, int l);
+/* Particles in the context (pPPro->ntotal, all realisations of a batch), -1 for a null context. */
+int sphg_particle_count(const sphg_ctx *ctx);
+
+/* ---- Result files and checkpoints -------------------------------------------------------- */
+
+/* Per-particle fields of one .odb table, host arrays of n particles, row-major (3 or 6 values
+ * per particle where the Particle member is a vector, Class_particle.h:18-83). */
+typedef struct {
+    const double *xp, *ux, *ax, *vx, *interf, *interfss, *strep; /* 3 per particle */
+    const double *sig, *eps, *adps;                               /* 6 per particle */
+    const double *pre, *prea, *prew, *vsig, *veps, *meps, *divq, *divr, *satu, *suct;
+    const int *type, *matype;
+} sphg_odb_fields;
+/* output_total (CPP_14-2-output.cpp:15-135): writes <dir>/data-<l, 10 digits>.odb, the rule
+ * line, "l t", one row per particle (records numbered from 1; boundary rows carry interfss and
+ * interf where other rows carry ax and vx), the rule line. Host only. */
+int sphg_write_odb_fields(const char *dir, int l, double t, int n, const sphg_odb_fields *f);
+/* The same table from the context's current state, particles first .. first+count-1 (one
+ * realisation of a batch; 0 and the particle count for a single simulation). l and t label the
+ * file as the reference's output_total call does (pPPro->l, pPPro->t at the output step). */
+int sphg_write_odb(sphg_ctx *ctx, const char *dir, int l, double t, int first, int count);
+/* Checkpoint and resume (no counterpart in the reference, which restarts from input.dat only):
+ * sphg_save_state writes every device array of the context except the neighbour lists, which the
+ * next step's search rebuilds, together with the step counter and the buffer parity;
+ * sphg_load_state restores them into a context created and uploaded from the same deck (same
+ * flags, capacity, particle count and arithmetic form; SPHG_E_ARG otherwise). Stepping a
+ * restored context gives bit for bit the results of the uninterrupted run. */
+int sphg_save_state(sphg_ctx *ctx, const char *path);
+int sphg_load_state(sphg_ctx *ctx, const char *path);
 
 /* Parity tap: neighbour lists in reference order (Par_Cell::ninflu / influ[1..], Class_NNPS.h:65-77).
  * counts[n]; ids[n][SPHG_MAX_NB] (entries past counts[i] are -1). Counts above 99 are reported

@@ -23,13 +23,15 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     obj_cpp = os.path.join(OUT_DIR, "sph_init.o")
     obj_kl = os.path.join(OUT_DIR, "sph_kl_host.o")
+    obj_io = os.path.join(OUT_DIR, "sph_io.o")
     obj_hip = os.path.join(OUT_DIR, "sphg_capi.o")
     host = ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-c"]
     cmds = [
         host + [os.path.join(CSRC, "sph_init.cpp"), "-o", obj_cpp],
         host + [os.path.join(CSRC, "sph_kl_host.cpp"), "-o", obj_kl],
+        host + [os.path.join(CSRC, "sph_io.cpp"), "-o", obj_io],
         [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "sphg_capi.hip"), "-o", obj_hip],
-        [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp, obj_kl],
+        [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp, obj_kl, obj_io],
     ]
     for c in cmds:
         if verbose:

@@ -572,6 +572,10 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
         }
         HIPCHK(hipMemcpy(c->d_rid, rid.data(), (size_t)n * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_grids, c->grids.data(), sizeof(Grid) * nr, hipMemcpyHostToDevice));
+        // the search walks particles in the binning's cell order; until the first binning that
+        // order is the identity, so a search issued before any binning still visits every particle once
+        for (int i = 0; i < n; i++) rid[i] = i;
+        HIPCHK(hipMemcpy(c->S.sorted, rid.data(), (size_t)n * 4, hipMemcpyHostToDevice));
     }
     c->full_models = false;
     for (int i = 0; i < n; i++) {
@@ -944,6 +948,88 @@ extern "C" int sphg_sync(sphg_ctx *c) {
 }
 
 extern "C" int sphg_current_step(const sphg_ctx *c) { return c ? c->step : -1; }
+extern "C" int sphg_particle_count(const sphg_ctx *c) { return c ? c->n : -1; }
+
+// ------------------------------------------------------------------------------- checkpoints
+// File: the header below, then the device pool byte for byte with the neighbour-list region left
+// out. Everything a step reads is in the pool (state, material slots, grids, per-particle
+// cohesion, sticky error word, device step counter); the host keeps only the step counter, the
+// buffer parity and two flags, which the header carries.
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int32_t version, n, cap, ndim, precision, nreal, step, cur, stepped, has_psc;
+    uint64_t pool_bytes, skip_off, skip_len;
+};
+constexpr char kCkptMagic[8] = {'S', 'P', 'H', 'G', 'C', 'K', 'P', 'T'};
+constexpr size_t kCkptChunk = size_t(64) << 20;
+
+CkptHeader ckpt_header(const sphg_ctx *c) {
+    CkptHeader h{};
+    memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.n = c->n; h.cap = c->cap; h.ndim = c->P.ndim; h.precision = c->precision; h.nreal = c->P.nreal;
+    h.step = c->step; h.cur = c->cur; h.stepped = c->stepped; h.has_psc = c->has_psc;
+    h.pool_bytes = c->pool_bytes;
+    h.skip_off = (uint64_t)((char *)c->S.nbr - c->pool);
+    h.skip_len = (uint64_t)c->cap * kMaxNb * 4;
+    return h;
+}
+
+// copies pool[off, off+len) to or from the file through a bounded host buffer
+bool ckpt_stream(sphg_ctx *c, FILE *f, size_t off, size_t len, bool save) {
+    std::vector<char> buf(std::min(len, kCkptChunk));
+    for (size_t done = 0; done < len;) {
+        const size_t k = std::min(kCkptChunk, len - done);
+        if (save) {
+            if (hipMemcpy(buf.data(), c->pool + off + done, k, hipMemcpyDeviceToHost) != hipSuccess) return false;
+            if (fwrite(buf.data(), 1, k, f) != k) return false;
+        } else {
+            if (fread(buf.data(), 1, k, f) != k) return false;
+            if (hipMemcpy(c->pool + off + done, buf.data(), k, hipMemcpyHostToDevice) != hipSuccess) return false;
+        }
+        done += k;
+    }
+    return true;
+}
+}  // namespace
+
+extern "C" int sphg_save_state(sphg_ctx *c, const char *path) {
+    if (!c || !path || !c->uploaded) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const CkptHeader h = ckpt_header(c);
+    FILE *f = fopen(path, "wb");
+    if (!f) return SPHG_E_ARG;
+    bool ok = fwrite(&h, sizeof h, 1, f) == 1 && ckpt_stream(c, f, 0, h.skip_off, true) &&
+              ckpt_stream(c, f, h.skip_off + h.skip_len, h.pool_bytes - h.skip_off - h.skip_len, true);
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SPHG_OK : SPHG_E_ARG;
+}
+
+extern "C" int sphg_load_state(sphg_ctx *c, const char *path) {
+    if (!c || !path || !c->uploaded) return SPHG_E_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return SPHG_E_ARG;
+    CkptHeader h{};
+    const CkptHeader mine = ckpt_header(c);
+    if (fread(&h, sizeof h, 1, f) != 1 || memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1 ||
+        h.n != mine.n || h.cap != mine.cap || h.ndim != mine.ndim || h.precision != mine.precision ||
+        h.nreal != mine.nreal || h.pool_bytes != mine.pool_bytes || h.skip_off != mine.skip_off ||
+        h.skip_len != mine.skip_len || (h.cur != 0 && h.cur != 1)) {
+        fclose(f);
+        return SPHG_E_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const bool ok = ckpt_stream(c, f, 0, h.skip_off, false) &&
+                    ckpt_stream(c, f, h.skip_off + h.skip_len, h.pool_bytes - h.skip_off - h.skip_len, false);
+    fclose(f);
+    if (!ok) return SPHG_E_ARG;
+    c->step = h.step;
+    c->cur = h.cur;
+    c->stepped = h.stepped != 0;
+    c->has_psc = h.has_psc != 0;
+    return SPHG_OK;
+}
 
 extern "C" int sphg_set_step(sphg_ctx *c, int l) {
     if (!c) return SPHG_E_ARG;

@@ -68,6 +68,16 @@ constexpr int kIdMask = (1 << kTypeShift) - 1;
 __device__ __forceinline__ int nb_id(int e) { return e & kIdMask; }
 __device__ __forceinline__ int nb_type(int e) { return e >> kTypeShift; }
 
+// The search walks the particles in cell order (the binning's sorted permutation, rebuilt every
+// step just before it): the lanes of a wave then sit in a few adjacent cells and scan the same
+// stencil cells, so their candidate loads coincide. The pair passes keep id order: there the
+// scattered own-particle loads and stores cost more than the shared neighbour lines save.
+// Returns P.n past the end.
+__device__ __forceinline__ int cell_pidx(const State &S, const Params &P) {
+    const int t = pidx();
+    return t < P.n ? (S.sorted[t] & kIdMask) : P.n;
+}
+
 // Walks the column-major list of particle i with the following entry already in flight.
 struct NbCursor {
     const int *p;
@@ -387,7 +397,7 @@ constexpr int kSearchBlock = 256, kStage = 32;
 template <int NDIM>
 __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int cur) {
     __shared__ int stage[kStage][kSearchBlock];
-    const int i = pidx();
+    const int i = cell_pidx(S, P);
     const int t = threadIdx.x;
     if (i >= P.n) return;
     int cnt = 0, tmask = 0;

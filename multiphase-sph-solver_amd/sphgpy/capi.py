@@ -322,6 +322,16 @@ def product_lib() -> _Lib:
         L.sphg_poll.restype = _I
         L.sphg_run.argtypes = [C.c_void_p, _I, _I, C.c_double, C.POINTER(C.c_double), OUTPUT_FN, C.c_void_p]
         L.sphg_run.restype = _I
+        L.sphg_particle_count.argtypes = [C.c_void_p]
+        L.sphg_particle_count.restype = _I
+        L.sphg_write_odb_fields.argtypes = [C.c_char_p, _I, C.c_double, _I, C.POINTER(OdbFields)]
+        L.sphg_write_odb_fields.restype = _I
+        L.sphg_write_odb.argtypes = [C.c_void_p, C.c_char_p, _I, C.c_double, _I, _I]
+        L.sphg_write_odb.restype = _I
+        for nm in ("save_state", "load_state"):
+            f = getattr(L, "sphg_" + nm)
+            f.argtypes = [C.c_void_p, C.c_char_p]
+            f.restype = _I
         for nm in ("parttocell", "partisearching", "density", "strain", "strain_to_stress", "accel", "xsph",
                    "update"):
             f = getattr(L, "sphg_" + nm)
@@ -332,6 +342,44 @@ def product_lib() -> _Lib:
 
 
 OUTPUT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double)
+
+ODB_V3 = ("xp", "ux", "ax", "vx", "interf", "interfss", "strep")
+ODB_V6 = ("sig", "eps", "adps")
+ODB_S = ("pre", "prea", "prew", "vsig", "veps", "meps", "divq", "divr", "satu", "suct")
+
+
+class OdbFields(C.Structure):
+    """sphg_odb_fields: host arrays of one .odb table."""
+    _fields_ = [(k, C.POINTER(C.c_double)) for k in ODB_V3 + ODB_V6 + ODB_S] + \
+               [("type", C.POINTER(C.c_int)), ("matype", C.POINTER(C.c_int))]
+
+
+def write_odb_fields(dirname: str, l: int, t: float, state: Dict[str, np.ndarray]) -> str:
+    """sphg_write_odb_fields on host arrays (a state dict as the golden fixtures hold it); returns
+    the path written."""
+    keep = {}
+    f = OdbFields()
+    n = None
+    for k, nc in [(k, 3) for k in ODB_V3] + [(k, 6) for k in ODB_V6] + [(k, 1) for k in ODB_S]:
+        a = np.ascontiguousarray(np.asarray(state[k], dtype=np.float64).reshape(-1, nc))
+        n = a.shape[0] if n is None else n
+        if a.shape[0] != n:
+            raise ValueError(f"{k}: {a.shape[0]} particles, expected {n}")
+        keep[k] = a
+        setattr(f, k, _dptr(a))
+    for k in ("type", "matype"):
+        a = np.ascontiguousarray(np.asarray(state[k], dtype=np.int32).reshape(-1))
+        keep[k] = a
+        setattr(f, k, _iptr(a))
+    rc = product_lib().lib.sphg_write_odb_fields(dirname.encode(), l, t, n, C.byref(f))
+    if rc:
+        raise RuntimeError(f"sphg_write_odb_fields: error {rc}")
+    return odb_path(dirname, l)
+
+
+def odb_path(dirname: str, l: int) -> str:
+    """File name output_total gives the table of step l (CPP_14-2-output.cpp:27-76)."""
+    return os.path.join(dirname, f"data-{l:010d}.odb" if l < 100000000 else "data-.odb")
 
 
 class GpuSim(Sim):
@@ -366,6 +414,22 @@ class GpuSim(Sim):
 
     def set_step(self, l: int):
         self.L.lib.sphg_set_step(self.ctx, l)
+
+    def write_odb(self, dirname: str, l: int, t: float, first: int = 0, count: Optional[int] = None) -> str:
+        """sphg_write_odb: the reference's .odb table of particles first..first+count-1."""
+        count = self.n - first if count is None else count
+        rc = self.L.lib.sphg_write_odb(self.ctx, dirname.encode(), l, t, first, count)
+        if rc:
+            raise RuntimeError(f"sphg_write_odb: error {rc}")
+        return odb_path(dirname, l)
+
+    def save_state(self, path: str) -> int:
+        """sphg_save_state: checkpoint of the device state (returns the error code)."""
+        return int(self.L.lib.sphg_save_state(self.ctx, path.encode()))
+
+    def restore_state(self, path: str) -> int:
+        """sphg_load_state into this (created and uploaded) context; returns the error code."""
+        return int(self.L.lib.sphg_load_state(self.ctx, path.encode()))
 
     def call(self, name: str) -> int:
         return int(getattr(self.L.lib, "sphg_" + name)(self.ctx))
