@@ -215,7 +215,24 @@ int sphg_set_step(sphg_ctx *ctx, int l);
 /* Particles in the context (pPPro->ntotal, all realisations of a batch), -1 for a null context. */
 int sphg_particle_count(const sphg_ctx *ctx);
 
-/* ---- Result files and checkpoints -------------------------------------------------------- */
+/* ---- Deck and result files, checkpoints --------------------------------------------------- */
+
+/* initial_pro + initial_mat + input_par (CPP_2-initialf.cpp:22-1018): reads an input.dat deck
+ * into the problem and material structs (constructor defaults, then the FL PR XP AT GF IP BP WP
+ * AP MS <soil model blocks> SM sections, each found from the top of the file as the reference
+ * finds it, then the reference's material post-processing: vpu 600, porosities, default
+ * densities, FOCC e) and the particle arrays of the PA section. Host only. Called with any array
+ * null it fills pro, mat and *n (pPPro->ntotal) and stops; called again with arrays of cap >= *n
+ * particles (xp, vxp: 3 per particle, unused dimensions 0) it reads the particles and fills the
+ * Cell_Con box and phase extents of pro. A section that is absent, or that ends (or reaches the
+ * end of the file, where the reference would scan forever) before its values, leaves the
+ * defaults. Returns 2 when ntotal <= 0 (the reference's error 2), SPHG_E_CAPACITY when
+ * cap < ntotal, SPHG_E_ARG for a file that cannot be opened, a missing PA section or a particle
+ * row that does not parse (the reference would go on with zeroed particles), or more than three
+ * layers of one soil model (beyond the 30 slots). The rain, excavation, vibration and
+ * spatial-variability sections belong to flags this library does not run and are not read. */
+int sphg_read_input(const char *path, sphg_problem *pro, sphg_materials *mat, int *n, int cap,
+                    double *xp, double *vxp, int *type, int *matype, int *permtype, int *etype);
 
 /* Per-particle fields of one .odb table, host arrays of n particles, row-major (3 or 6 values
  * per particle where the Particle member is a vector, Class_particle.h:18-83). */

@@ -143,7 +143,7 @@ def problem_from_deck(d: Deck) -> Problem:
     for name, v in zip("flagem flagsb flagdp flagum flagam flagbui flagpl flagve flagfocc flagep".split(),
                        d.model_flags):
         setattr(p, name, int(v) if d.fls == 1 else 0)
-    p.alphats = d.alphats
+    p.alphats = d.alphats if (d.fls == 1 and d.flts >= 1) else 0.0  # read with damp_coe (MS section)
     bmin, bmax, ph = d.domain()
     p.xmin, p.ymin, p.zmin = bmin
     p.xmax, p.ymax, p.zmax = bmax
@@ -328,6 +328,9 @@ def product_lib() -> _Lib:
         L.sphg_write_odb_fields.restype = _I
         L.sphg_write_odb.argtypes = [C.c_void_p, C.c_char_p, _I, C.c_double, _I, _I]
         L.sphg_write_odb.restype = _I
+        L.sphg_read_input.argtypes = [C.c_char_p, C.POINTER(Problem), C.POINTER(Materials), C.POINTER(C.c_int), _I,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)] + [C.POINTER(C.c_int)] * 4
+        L.sphg_read_input.restype = _I
         for nm in ("save_state", "load_state"):
             f = getattr(L, "sphg_" + nm)
             f.argtypes = [C.c_void_p, C.c_char_p]
@@ -375,6 +378,26 @@ def write_odb_fields(dirname: str, l: int, t: float, state: Dict[str, np.ndarray
     if rc:
         raise RuntimeError(f"sphg_write_odb_fields: error {rc}")
     return odb_path(dirname, l)
+
+
+def read_input(path: str):
+    """sphg_read_input: (Problem, Materials, particles) of an input.dat deck; particles holds xp, vxp
+    (n x 3) and type, matype, permtype, etype (n)."""
+    L = product_lib().lib
+    pro, mat, n = Problem(), Materials(), C.c_int(0)
+    rc = L.sphg_read_input(path.encode(), C.byref(pro), C.byref(mat), C.byref(n), 0, None, None, None, None,
+                           None, None)
+    if rc:
+        raise RuntimeError(f"sphg_read_input: error {rc}")
+    nn = n.value
+    parts = {"xp": np.zeros((nn, 3)), "vxp": np.zeros((nn, 3))}
+    for k in ("type", "matype", "permtype", "etype"):
+        parts[k] = np.zeros(nn, np.int32)
+    rc = L.sphg_read_input(path.encode(), C.byref(pro), C.byref(mat), C.byref(n), nn, _dptr(parts["xp"]),
+                           _dptr(parts["vxp"]), *[_iptr(parts[k]) for k in ("type", "matype", "permtype", "etype")])
+    if rc:
+        raise RuntimeError(f"sphg_read_input: error {rc}")
+    return pro, mat, parts
 
 
 def odb_path(dirname: str, l: int) -> str:

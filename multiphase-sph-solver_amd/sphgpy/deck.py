@@ -211,9 +211,11 @@ class Deck:
                             d[k] = float(lay[k])
                     if model == 8:
                         d["e"] = 5.0e6
+        # damp_coe is read from the MS section's second line, which exists only when flts >= 1
+        damp = float(self.damp_coe) if (self.fls == 1 and self.flts >= 1) else 0.0
         for d in tab:
             d["vpu"] = 600.0
-            d["damp_coe"] = float(self.damp_coe)
+            d["damp_coe"] = damp
         if self.fls == 1:
             for d in tab:
                 d["porosity"] = d["eps0"] / (1 + d["eps0"])

@@ -65,3 +65,110 @@ def test_odb_writer_every_column_and_boundary_rows():
     st2["divq"], st2["divr"] = st["divr"], st["divq"]
     with tempfile.TemporaryDirectory() as tmp:
         assert open(write_odb_fields(tmp, l, 0.125, st2), "rb").read() != want
+
+
+# ------------------------------------------------------------------------------ input.dat reader
+
+def _flat(obj, prefix=""):
+    """(name, value) leaves of a ctypes struct, arrays and nested structs expanded."""
+    import ctypes as C
+    if isinstance(obj, C.Structure):
+        for name, _ in obj._fields_:
+            yield from _flat(getattr(obj, name), f"{prefix}{name}.")
+    elif isinstance(obj, C.Array):
+        for i, v in enumerate(obj):
+            yield from _flat(v, f"{prefix}{i}.")
+    else:
+        yield prefix.rstrip("."), obj
+
+
+def _same_struct(a, b):
+    import struct as st
+    bad = []
+    for (na, va), (nb, vb) in zip(_flat(a), _flat(b)):
+        assert na == nb
+        if isinstance(va, float):
+            if st.pack("<d", va) != st.pack("<d", vb):
+                bad.append((na, va, vb))
+        elif va != vb:
+            bad.append((na, va, vb))
+    return bad
+
+
+def _reader_decks():
+    from golden.make_golden import decks
+    from sphgpy import column_collapse_2d
+    out = {name: d for name, (d, _, _) in decks().items()}
+    # flags no golden deck sets: air phase, TS-transform material line, moving walls
+    d = column_collapse_2d(nx=4, ny=5, wall_h=7, floor_len=9, steps=3, seed=4)
+    d.fla = 1
+    d.air = dict(d.air, dens=1.2, paa0=101325.0, vis=1.8e-5, vpu=34.0, press_type=1, acc_type=1, pre_coe=0.5)
+    d.flts = 1
+    d.alphats, d.damp_coe = 0.25, 0.125
+    d.if_move, d.move_vx = 1, (0.1, -0.2, 0.3)
+    out["air_ts_move"] = d
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(_reader_decks()))
+def test_read_input_gives_the_structs_the_reference_runs_on(name, tmp_path):
+    # The deck file written for the reference and read back by sphg_read_input gives exactly the
+    # problem/material structs that problem_from_deck / materials_from_deck build, on which the
+    # library's runs match the reference's runs of the same file (the GPU parity tests), and the
+    # same particles.
+    from sphgpy.capi import read_input, problem_from_deck, materials_from_deck
+    deck = _reader_decks()[name]
+    path = str(tmp_path / "input.dat")
+    deck.write_input_dat(path)
+    pro, mat, parts = read_input(path)
+    assert _same_struct(pro, problem_from_deck(deck)) == []
+    assert _same_struct(mat, materials_from_deck(deck)) == []
+    xp = np.zeros((deck.n, 3))
+    xp[:, :deck.ndim] = deck.xp[:, :deck.ndim]
+    assert np.array_equal(parts["xp"].view(np.int64), xp.view(np.int64))
+    assert np.array_equal(parts["vxp"][:, :deck.ndim], deck.vxp[:, :deck.ndim])
+    for k, v in (("type", deck.type), ("matype", deck.matype), ("permtype", deck.permtype_p),
+                 ("etype", deck.etype)):
+        assert np.array_equal(parts[k], v), k
+
+
+def test_read_input_section_order_and_errors(tmp_path):
+    import ctypes as C
+    from sphgpy.capi import read_input, problem_from_deck, Problem, Materials
+    from sphgpy import column_collapse_2d
+    deck = column_collapse_2d(nx=3, ny=3, wall_h=4, floor_len=5, steps=2, seed=9)
+    path = tmp_path / "input.dat"
+    deck.write_input_dat(str(path))
+    lines = path.read_text().splitlines()
+    # sections are found from the top of the file, so their order does not matter
+    pa = lines.index("PA section")
+    gf = lines.index("GF section")
+    moved = lines[:gf] + lines[gf + 4:pa] + lines[gf:gf + 4] + lines[pa:]
+    p2 = tmp_path / "moved.dat"
+    p2.write_text("\n".join(moved) + "\n")
+    pro, _, _ = read_input(str(p2))
+    assert _same_struct(pro, problem_from_deck(deck)) == []
+    L = product_lib().lib
+    pr, mt, n = Problem(), Materials(), C.c_int(0)
+    assert L.sphg_read_input(str(tmp_path / "none.dat").encode(), C.byref(pr), C.byref(mt), C.byref(n), 0,
+                             None, None, None, None, None, None) == 100
+    # ntotal 0: the reference's error 2
+    zero = [ln for ln in lines]
+    i = zero.index("PR section") + 3
+    f = zero[i].split()
+    f[3] = "0"
+    zero[i] = " ".join(f)
+    p3 = tmp_path / "zero.dat"
+    p3.write_text("\n".join(zero) + "\n")
+    assert L.sphg_read_input(str(p3).encode(), C.byref(pr), C.byref(mt), C.byref(n), 0,
+                             None, None, None, None, None, None) == 2
+    # capacity below ntotal, then a truncated particle section
+    arr = np.zeros((deck.n, 3))
+    ints = [np.zeros(deck.n, np.int32) for _ in range(4)]
+    ip = [i_.ctypes.data_as(C.POINTER(C.c_int)) for i_ in ints]
+    dp = arr.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.sphg_read_input(str(path).encode(), C.byref(pr), C.byref(mt), C.byref(n), deck.n - 1, dp, dp,
+                             *ip) == 103
+    p4 = tmp_path / "short.dat"
+    p4.write_text("\n".join(lines[:-3]) + "\n")
+    assert L.sphg_read_input(str(p4).encode(), C.byref(pr), C.byref(mt), C.byref(n), deck.n, dp, dp, *ip) == 100
