@@ -250,6 +250,14 @@ int sphg_write_odb_fields(const char *dir, int l, double t, int n, const sphg_od
  * realisation of a batch; 0 and the particle count for a single simulation). l and t label the
  * file as the reference's output_total call does (pPPro->l, pPPro->t at the output step). */
 int sphg_write_odb(sphg_ctx *ctx, const char *dir, int l, double t, int first, int count);
+/* One project directory run the way the reference's executable runs it (main and
+ * SPH_Calculation, CPP_0-main.cpp, CPP_1-SPHLoops.cpp:67-782): <dir>/input.dat read with
+ * sphg_read_input, the initial state of sphg_setting_par, one context of the given arithmetic
+ * form, the time loop of sphg_run, and <dir>/data-<l>.odb written at the reference's output steps
+ * (and, after a step error, for the state the run stopped in). Returns the run's error code and
+ * the last step written in *last_step (may be null). Random-field decks (flspv >= 1) return
+ * SPHG_E_UNSUPPORTED: the K-L entry points serve them through the API. */
+int sphg_solve_project(const char *dir, int precision, int *last_step);
 /* Checkpoint and resume (no counterpart in the reference, which restarts from input.dat only):
  * sphg_save_state writes every device array of the context except the neighbour lists, which the
  * next step's search rebuilds, together with the step counter and the buffer parity;

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libsphg.so")
+SOLVE = os.path.join(OUT_DIR, "sphg_solve")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
@@ -24,14 +25,16 @@ def build(verbose: bool = False) -> str:
     obj_cpp = os.path.join(OUT_DIR, "sph_init.o")
     obj_kl = os.path.join(OUT_DIR, "sph_kl_host.o")
     obj_io = os.path.join(OUT_DIR, "sph_io.o")
+    obj_drv = os.path.join(OUT_DIR, "sph_driver.o")
     obj_hip = os.path.join(OUT_DIR, "sphg_capi.o")
     host = ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-c"]
     cmds = [
         host + [os.path.join(CSRC, "sph_init.cpp"), "-o", obj_cpp],
         host + [os.path.join(CSRC, "sph_kl_host.cpp"), "-o", obj_kl],
         host + [os.path.join(CSRC, "sph_io.cpp"), "-o", obj_io],
+        host + [os.path.join(CSRC, "sph_driver.cpp"), "-o", obj_drv],
         [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "sphg_capi.hip"), "-o", obj_hip],
-        [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp, obj_kl, obj_io],
+        [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp, obj_kl, obj_io, obj_drv],
     ]
     for c in cmds:
         if verbose:
@@ -41,6 +44,12 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"build failed: {' '.join(c[:3])} ...")
     os.replace(LIB + ".tmp", LIB)
+    # the project runner executable, linked against the library beside it
+    r = subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "sphg_main.cpp"), "-o", SOLVE,
+                        "-L" + OUT_DIR, "-lsphg", "-Wl,-rpath,$ORIGIN"], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: sphg_solve")
     return LIB
 
 

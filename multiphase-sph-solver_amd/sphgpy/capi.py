@@ -20,6 +20,7 @@ _I = C.c_int
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "_build", "libsphg.so")
+SOLVE_PATH = os.path.join(PKG_DIR, "_build", "sphg_solve")  # project runner (sphg_solve_project)
 MAX_NB = 99
 
 

@@ -109,3 +109,45 @@ def test_checkpoint_refused_by_other_deck_or_form(tmp_path):
     with open(path, "r+b") as f:  # a damaged header
         f.write(b"XXXX")
     assert a.restore_state(path) == 100
+
+
+def test_project_runner_writes_the_reference_outputs(tmp_path):
+    # sphg_solve on a project directory (input.dat -> reader -> setting_par -> device loop ->
+    # .odb at steps 1 and every fop-th) writes, at each output step, the table the host writer
+    # gives for the oracle's state after the same steps, and the same files as the API path
+    import subprocess
+    from sphgpy.capi import SOLVE_PATH, odb_path
+    deck, states = load_golden("ssi_2d_t1")
+    deck = deck.copy()
+    deck.loop, deck.fop = 6, 3
+    proj = tmp_path / "proj"
+    proj.mkdir()
+    deck.write_input_dat(str(proj / "input.dat"))
+    r = subprocess.run([SOLVE_PATH, str(proj)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "last output step 6" in r.stdout
+    assert sorted(os.listdir(proj)) == ["data-0000000001.odb", "data-0000000003.odb", "data-0000000006.odb",
+                                        "input.dat"]
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    done = 0
+    for l in (1, 3, 6):
+        assert o.step(l - done) == 0
+        done = l
+        st = {k: o.get(k) for k in ODB_FIELDS}
+        st["type"], st["matype"] = deck.type, deck.matype
+        d = tmp_path / f"o{l}"
+        d.mkdir()
+        want = open(write_odb_fields(str(d), l, odb_time(deck.dt, l), st), "rb").read()
+        assert open(odb_path(str(proj), l), "rb").read() == want, l
+    # the API path: same deck through GpuSim and sphg_run with the writer as output callback
+    api = tmp_path / "api"
+    api.mkdir()
+    g = GpuSim(deck)
+    g.load_state(setting_par(deck))
+    g.upload_done()
+    g.set_step(1)
+    rc, t = g.run(deck.loop, deck.fop, deck.dt * (deck.loop + 1),
+                  output=lambda l, tm: g.write_odb(str(api), l, tm) and 0)
+    assert rc == 0
+    for l in (1, 3, 6):
+        assert open(odb_path(str(api), l), "rb").read() == open(odb_path(str(proj), l), "rb").read()

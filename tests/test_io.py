@@ -172,3 +172,21 @@ def test_read_input_section_order_and_errors(tmp_path):
     p4 = tmp_path / "short.dat"
     p4.write_text("\n".join(lines[:-3]) + "\n")
     assert L.sphg_read_input(str(p4).encode(), C.byref(pr), C.byref(mt), C.byref(n), deck.n, dp, dp, *ip) == 100
+
+
+def test_project_runner_argument_errors(tmp_path):
+    # checks that end before any device call: usage, a directory without input.dat, and a
+    # random-field deck (flspv), whose SF/SV sections the reader does not take
+    import ctypes as C
+    import subprocess
+    from sphgpy.capi import SOLVE_PATH
+    from sphgpy import column_collapse_2d
+    assert subprocess.run([SOLVE_PATH], capture_output=True, timeout=60).returncode == 2
+    L = product_lib().lib
+    L.sphg_solve_project.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    last = C.c_int(-1)
+    assert L.sphg_solve_project(str(tmp_path).encode(), 0, C.byref(last)) == 100 and last.value == 0
+    d = column_collapse_2d(nx=3, ny=3, wall_h=4, floor_len=5, steps=2, seed=9)
+    d.flspv = 1
+    d.write_input_dat(str(tmp_path / "input.dat"))
+    assert L.sphg_solve_project(str(tmp_path).encode(), 0, None) == 102
