@@ -318,6 +318,41 @@ int sphg_kl_synthesize_part(sphg_ctx *ctx, const sphg_kl_field *f, const double 
 int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const double *xp, const int *type, int nreal,
                              const double *xi, double fill, double *value, double *gemm_ms);
 
+/* ---- Correlation-matrix random fields (flspv 3, CPP_17-3-Nonstationary-RF.cpp) ------------- */
+
+/* One spatially variable property generated by the correlation-matrix method
+ * (Flag_SpatialVariability and Para_SpatialVariability, Class_spatial_variability.h:10-66, as
+ * nonstationary_random_2D/3D use them). sphg_rf_cells fills the cell partition. */
+typedef struct {
+    int ndim;
+    int dist;                 /* 1 normal, 2 lognormal */
+    int field;                /* 0 soil fai, 1 soil c, 2 soil cop, 3 soil ds, 4 fluid fai, 5 fluid c */
+    int acf;                  /* flag_auto: 1 exponential, 2 Gaussian, otherwise rotated exponential */
+    double mean_value, sd_value, coe_depth;
+    double rl[3];             /* correlation lengths rl_value */
+    double beta, beta_1;      /* rotation angles in degrees (rotated exponential) */
+    double cell_ndx, dr;      /* cell edge cell_ndx * dr */
+    double box_nb[6];         /* Cell_Con xmin_nb xmax_nb ymin_nb ymax_nb zmin_nb zmax_nb */
+    double soil_max, water_max; /* Cell_Con soil_max / water_max on the vertical axis (y in 2D, z in 3D) */
+    int nc[3], cell_total;    /* set by sphg_rf_cells */
+    double base[3], cell_dx;
+} sphg_rf_field;
+/* Cell partition (CPP_17-3-Nonstationary-RF.cpp:97-160 2D, :479-565 3D): the non-boundary box
+ * widened by 10 % per side, cells of cell_ndx * dr, and the cell of every particle mapped as the
+ * reference maps them (2D: water and soil particles; 3D: all particles), -1 otherwise. Host only.
+ * Returns 41 when the cell count is not in (0, 15000), 42 when a cell would take a 40th particle
+ * (the reference's codes), SPHG_E_ARG for a 3D particle whose cell index falls outside the grid
+ * (the reference writes out of bounds there). cell_of may be null. */
+int sphg_rf_cells(sphg_rf_field *f, int n, const double *xp, const int *type, int *cell_of);
+/* The field for nreal realisations on the device: correlation matrix, Cholesky factor (FP64,
+ * blocked, matrix cores for the trailing updates), E = L * zeta (matrix cores), per-particle
+ * values value[nreal][n] (the property's phase only; other particles keep fill). zeta holds
+ * nreal x cell_total standard normals (the reference draws them from a clock-seeded generator,
+ * CPP_17-3-Nonstationary-RF.cpp:32-42). L (may be null) receives the cell_total^2 factor,
+ * row-major with zeros above the diagonal; factor_ms (may be null) the Cholesky's device time. */
+int sphg_rf_generate(const sphg_rf_field *f, int n, const double *xp, const int *type, int nreal,
+                     const double *zeta, double fill, double *value, double *L, double *factor_ms);
+
 const char *sphg_error_string(int code);
 const char *sphg_build_info(void);
 

@@ -16,6 +16,7 @@
 #include "sphg_kernels.hip"
 #include "sphg_kl.hip"
 #include "sphg_ssi.hip"
+#include "sphg_rf.hip"
 
 using namespace sphg;
 
@@ -1185,6 +1186,135 @@ extern "C" int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const dou
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
         *gemm_ms = ms;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    hipFree(pool);
+    return e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
+}
+
+// ------------------------------------------------------------- correlation-matrix random fields
+static const double kRefPi = 3.141592653;  // Header_Parameters.h:10
+
+extern "C" int sphg_rf_cells(sphg_rf_field *f, int n, const double *xp, const int *type, int *cell_of) {
+    if (!f || n <= 0 || !xp || !type || (f->ndim != 2 && f->ndim != 3) || !(f->cell_ndx > 0.0) || !(f->dr > 0.0))
+        return SPHG_E_ARG;
+    const int nd = f->ndim;
+    const double *b = f->box_nb;
+    double xmin[3] = {0.0, 0.0, 0.0}, xmax[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < nd; d++) {
+        xmin[d] = b[2 * d] - 0.1 * (b[2 * d + 1] - b[2 * d]);
+        xmax[d] = b[2 * d + 1] + 0.1 * (b[2 * d + 1] - b[2 * d]);
+    }
+    const double cell_dx = f->cell_ndx * f->dr;
+    long long nc[3] = {1, 1, 1};
+    for (int d = 0; d < nd; d++) nc[d] = (long long)((xmax[d] - xmin[d]) / cell_dx) + 1;
+    const long long total = nc[0] * nc[1] * nc[2];
+    if (!(total > 0 && total < 15000)) return 41;
+    f->cell_dx = cell_dx;
+    f->cell_total = (int)total;
+    for (int d = 0; d < 3; d++) {
+        f->nc[d] = (int)nc[d];
+        f->base[d] = d < nd ? -(nc[d] * cell_dx - (xmax[d] - xmin[d])) / 2.0 + 0.01 * f->dr + xmin[d] : 0.0;
+    }
+    std::vector<int> cnt((size_t)total, 0);
+    for (int i = 0; i < n; i++) {
+        if (cell_of) cell_of[i] = -1;
+        if (nd == 2 && type[i] != 1 && type[i] != 2) continue;  // 2D maps water and soil only
+        long long np[3] = {0, 0, 0};
+        for (int d = 0; d < nd; d++) np[d] = (long long)(fabs(xp[3 * (size_t)i + d] - f->base[d]) / cell_dx);
+        const long long c = nd == 2 ? np[1] * nc[0] + np[0] : np[2] * nc[0] * nc[1] + np[1] * nc[0] + np[0];
+        if (c < 0 || c >= total) return SPHG_E_ARG;
+        if (cnt[(size_t)c] >= 39) return 42;
+        cnt[(size_t)c]++;
+        if (cell_of) cell_of[i] = (int)c;
+    }
+    return SPHG_OK;
+}
+
+extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *xp, const int *type, int nreal,
+                                const double *zeta, double fill, double *value, double *L, double *factor_ms) {
+    if (!fin || !zeta || !value || nreal <= 0 || (fin->dist != 1 && fin->dist != 2) || fin->field < 0 ||
+        fin->field > 5)
+        return SPHG_E_ARG;
+    sphg_rf_field f = *fin;
+    std::vector<int> cell_of(n > 0 ? n : 0);
+    int rc = sphg_rf_cells(&f, n, xp, type, cell_of.data());
+    if (rc) return rc;
+    using namespace sphg;
+    const int total = f.cell_total, npad = (total + kRfB - 1) / kRfB * kRfB, rpad = (nreal + kRfB - 1) / kRfB * kRfB;
+    RfParams p{};
+    p.ndim = f.ndim;
+    p.acf = f.acf;
+    p.total = total;
+    p.npad = npad;
+    p.cell_dx = f.cell_dx;
+    p.pi = kRefPi;
+    for (int d = 0; d < 3; d++) {
+        p.nc[d] = f.nc[d];
+        p.base[d] = f.base[d];
+        p.rl[d] = f.rl[d];
+    }
+    const double bt = f.beta * kRefPi / 180.0, bt1 = f.beta_1 * kRefPi / 180.0;  // beta_temp, beta1_temp
+    p.cb = cos(bt); p.sb = sin(bt); p.cb1 = cos(bt1); p.sb1 = sin(bt1);
+    RfValues q{};
+    q.ndim = f.ndim; q.dist = f.dist; q.field = f.field; q.total = total; q.rpad = rpad; q.n = n; q.nreal = nreal;
+    q.mean_value = f.mean_value; q.sd_value = f.sd_value; q.coe_depth = f.coe_depth;
+    q.soil_max = f.soil_max; q.water_max = f.water_max;
+    q.e1 = exp(1.0);
+    std::vector<double> zh((size_t)rpad * npad, 0.0);
+    for (int r = 0; r < nreal; r++)
+        for (int j = 0; j < total; j++) zh[(size_t)r * npad + j] = zeta[(size_t)r * total + j];
+    const size_t bA = align256((size_t)npad * npad * 8), bZ = align256(zh.size() * 8),
+                 bE = align256((size_t)npad * rpad * 8), bC = align256((size_t)n * 4), bT = align256((size_t)n * 4),
+                 bV = align256((size_t)nreal * n * 8);
+    char *pool = nullptr;
+    HIPCHK(hipMalloc(&pool, bA + bZ + bE + bC + bT + bV));
+    double *dA = (double *)pool, *dZ = (double *)(pool + bA), *dE = (double *)(pool + bA + bZ);
+    int *dC = (int *)(pool + bA + bZ + bE), *dT = (int *)(pool + bA + bZ + bE + bC);
+    double *dV = (double *)(pool + bA + bZ + bE + bC + bT);
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemcpyAsync(dZ, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dC, cell_of.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dT, type, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const size_t tot = (size_t)npad * npad;
+        const int gcorr = (int)std::min<size_t>((tot + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_rf_corr, dim3(gcorr), dim3(256), 0, st, p, dA);
+        hipEventRecord(e0, st);
+        for (int k0 = 0; k0 < npad; k0 += kRfB) {
+            hipLaunchKernelGGL(k_rf_potrf, dim3(1), dim3(kRfB), 0, st, dA, npad, k0);
+            const int below = (npad - k0 - kRfB) / kRfB;
+            if (below > 0) {
+                hipLaunchKernelGGL(k_rf_trsm, dim3(below), dim3(kRfB), 0, st, dA, npad, k0);
+                hipLaunchKernelGGL(k_rf_gemm<true>, dim3(below * (below + 1) / 2), dim3(256), 0, st,
+                                   (const double *)dA, npad, (const double *)dA, npad, dA, npad, k0 + kRfB, k0, kRfB);
+            }
+        }
+        hipEventRecord(e1, st);
+        hipLaunchKernelGGL(k_rf_gemm<false>, dim3(npad / kRfB, rpad / kRfB), dim3(256), 0, st, (const double *)dA,
+                           npad, (const double *)dZ, npad, dE, rpad, 0, 0, npad);
+        const long long nv = (long long)n * nreal;
+        hipLaunchKernelGGL(k_rf_values, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, p, q,
+                           (const double *)dE, (const int *)dC, (const int *)dT, fill, dV);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(value, dV, (size_t)nreal * n * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && L) {
+        for (int r = 0; r < total && e == hipSuccess; r++) {
+            e = hipMemcpy(L + (size_t)r * total, dA + (size_t)r * npad, (size_t)total * 8, hipMemcpyDeviceToHost);
+        }
+    }
+    if (e == hipSuccess && factor_ms) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        *factor_ms = ms;
     }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);

@@ -1,0 +1,236 @@
+// Correlation-matrix random fields on the device (nonstationary_random_2D / _3D,
+// CPP_17-3-Nonstationary-RF.cpp:76-849): the cells' correlation matrix, its Cholesky factor,
+// E = L * zeta for one or many realisations, and the per-particle property.
+//
+// The matrix is dense, symmetric and up to 15,000 cells on a side, so the factor is the one
+// cubic-cost step of the set-up. It runs as a blocked right-looking Cholesky in FP64 with 64-wide
+// panels: the diagonal block is factored by one wave in LDS (column by column, the reference's
+// order within the block, with its clamp sqrt(max(d, 1e-6))), the panel below by a triangular
+// solve per row, and the trailing lower triangle is updated by an FP64 matrix-core kernel
+// (v_mfma_f64_16x16x4_f64). E = L * Z^T uses the same matrix-core tile, skipping the zero blocks
+// above the diagonal. Only the lower triangle of the matrix is stored (zeros above), padded to a
+// multiple of 64 with an identity block that leaves the factor of the real cells unchanged.
+#include <hip/hip_runtime.h>
+
+namespace sphg {
+
+constexpr int kRfB = 64;  // panel width and tile edge
+
+struct RfParams {
+    int ndim, acf, nc[3], total, npad;
+    double base[3], cell_dx;
+    double rl[3];
+    double cb, sb, cb1, sb1;  // cos/sin of beta and beta_1 (radians as the reference forms them)
+    double pi;
+};
+
+__device__ __forceinline__ void rf_centre(const RfParams &p, int c, double x[3]) {
+    const int nx = p.nc[0], ny = p.nc[1];
+    int i, j, k = 0;
+    if (p.ndim == 2) {
+        j = c / nx;
+        i = c - j * nx;
+    } else {
+        k = c / (nx * ny);
+        const int r = c - k * nx * ny;
+        j = r / nx;
+        i = r - j * nx;
+    }
+    x[0] = p.base[0] + ((double)i + 0.5) * p.cell_dx;
+    x[1] = p.base[1] + ((double)j + 0.5) * p.cell_dx;
+    x[2] = p.ndim == 2 ? 0.0 : p.base[2] + ((double)k + 0.5) * p.cell_dx;
+}
+
+// the autocorrelation between two cell centres (CPP_17-3-Nonstationary-RF.cpp:209-222, :599-614)
+__device__ __forceinline__ double rf_acf(const RfParams &p, const double a[3], const double b[3]) {
+    const double d0 = fabs(a[0] - b[0]), d1 = fabs(a[1] - b[1]);
+    const double lx = p.rl[0], ly = p.rl[1];
+    if (p.ndim == 2) {
+        if (p.acf == 1) return exp(-2.0 * (d0 / lx + d1 / ly));
+        if (p.acf == 2) return exp(-p.pi * (pow(d0 / lx, 2) + pow(d1 / ly, 2)));
+        return exp(-2.0 * ((d0 * p.cb + d1 * p.sb) / lx + (-d0 * p.sb + d1 * p.cb) / ly));
+    }
+    const double d2 = fabs(a[2] - b[2]), lz = p.rl[2];
+    if (p.acf == 1) return exp(-2.0 * (d0 / lx + d1 / ly + d2 / lz));
+    if (p.acf == 2) return exp(-p.pi * (pow(d0 / lx, 2) + pow(d1 / ly, 2) + pow(d2 / lz, 2)));
+    const double t0 = d0 * p.cb1 + d1 * p.sb1;
+    const double t1 = (-d0 * p.sb1 + d1 * p.cb1) * p.cb + d2 * p.sb;
+    const double t2 = -(-d0 * p.sb1 + d1 * p.cb1) * p.sb + d2 * p.cb;
+    return exp(-2.0 * (t0 / lx + t1 / ly + t2 / lz));
+}
+
+// A[r][c], lower triangle of the correlation matrix (zeros above), identity on the padding
+__global__ __launch_bounds__(256) void k_rf_corr(RfParams p, double *A) {
+    const size_t np = (size_t)p.npad, tot = np * np;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / np), c = (int)(e - (size_t)r * np);
+        double v = 0.0;
+        if (c <= r) {
+            if (r < p.total) {
+                double a[3], b[3];
+                rf_centre(p, r, a);
+                rf_centre(p, c, b);
+                v = rf_acf(p, a, b);
+            } else if (r == c) {
+                v = 1.0;
+            }
+        }
+        A[e] = v;
+    }
+}
+
+// Diagonal block A[k0.., k0..] (64 x 64, already updated by the earlier panels), one wave:
+// column by column, d = a_kk - sum_p l_kp^2 in ascending p, l_kk = sqrt(max(d, 1e-6)), then
+// l_ik = (a_ik - sum_p l_ip l_kp) / l_kk; the strict upper part is set to zero.
+__global__ __launch_bounds__(64) void k_rf_potrf(double *A, int ld, int k0) {
+    __shared__ double S[kRfB][kRfB + 1];
+    const int t = threadIdx.x;
+    double *blk = A + (size_t)k0 * ld + k0;
+    for (int r = 0; r < kRfB; r++) S[r][t] = blk[(size_t)r * ld + t];
+    __syncthreads();
+    for (int k = 0; k < kRfB; k++) {
+        if (t == k) {
+            double s = 0.0;
+            for (int p = 0; p < k; p++) s += S[k][p] * S[k][p];
+            s = S[k][k] - s;
+            S[k][k] = sqrt(s > 1.0e-6 ? s : 1.0e-6);
+        }
+        __syncthreads();
+        if (t > k) {
+            double s = 0.0;
+            for (int p = 0; p < k; p++) s += S[t][p] * S[k][p];
+            S[t][k] = (S[t][k] - s) / S[k][k];
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < kRfB; r++) blk[(size_t)r * ld + t] = t <= r ? S[r][t] : 0.0;
+}
+
+// Panel below the diagonal block: for 64 rows per workgroup, x L_kk^T = a row by row,
+// x_k = (a_k - sum_{p<k} x_p l_kp) / l_kk in ascending k and p.
+__global__ __launch_bounds__(64) void k_rf_trsm(double *A, int ld, int k0) {
+    __shared__ double Lk[kRfB][kRfB + 1];
+    __shared__ double X[kRfB][kRfB + 1];
+    const int t = threadIdx.x;
+    const int r0 = k0 + kRfB + blockIdx.x * kRfB;
+    const double *lk = A + (size_t)k0 * ld + k0;
+    double *rows = A + (size_t)r0 * ld + k0;
+    for (int r = 0; r < kRfB; r++) {
+        Lk[r][t] = lk[(size_t)r * ld + t];
+        X[r][t] = rows[(size_t)r * ld + t];
+    }
+    __syncthreads();
+    for (int k = 0; k < kRfB; k++) {
+        double s = 0.0;
+        for (int p = 0; p < k; p++) s += X[t][p] * Lk[k][p];
+        X[t][k] = (X[t][k] - s) / Lk[k][k];
+    }
+    __syncthreads();
+    for (int r = 0; r < kRfB; r++) rows[(size_t)r * ld + t] = X[r][t];
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// C tile (64 x 64) op= A rows x B rows^T over k in [0, kend), four waves of 32 x 32, each 2 x 2
+// v_mfma_f64_16x16x4_f64 tiles (lane l supplies A[l % 16][l / 16] and B^T[l / 16][l % 16] and
+// holds D[l / 16 + 4 v][l % 16] in accumulator v). kLower: the grid walks the lower tiles
+// (ti >= tj) of the trailing matrix from row r0 and subtracts (the Cholesky update, K = 64);
+// otherwise a full grid (M / 64, N / 64) writes C = A B^T with K limited to the nonzero blocks
+// of a lower-triangular A (E = L Z^T).
+template <bool kLower>
+__global__ __launch_bounds__(256) void k_rf_gemm(const double *A, int lda, const double *B, int ldb, double *C,
+                                                 int ldc, int r0, int k0, int K) {
+    __shared__ double As[kRfB][kRfB + 1];
+    __shared__ double Bs[kRfB][kRfB + 1];
+    int ti, tj;
+    if (kLower) {
+        const int b = blockIdx.x;
+        ti = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+        while ((ti + 1) * (ti + 2) / 2 <= b) ti++;
+        while (ti * (ti + 1) / 2 > b) ti--;
+        tj = b - ti * (ti + 1) / 2;
+    } else {
+        ti = blockIdx.x;
+        tj = blockIdx.y;
+    }
+    const int row0 = r0 + ti * kRfB, col0 = kLower ? r0 + tj * kRfB : tj * kRfB;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    f64x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const int kend = kLower ? K : min(K, (ti + 1) * kRfB);  // A lower triangular: blocks past the diagonal are zero
+    for (int kc = 0; kc < kend; kc += kRfB) {
+        // stage the 64 x 64 chunks: thread t loads row t / 4, 16 consecutive k
+        const int lr = t >> 2, lc = (t & 3) * 16;
+        const double *ga = A + (size_t)(row0 + lr) * lda + k0 + kc + lc;
+        const double *gb = B + (size_t)(col0 + lr) * ldb + k0 + kc + lc;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            As[lr][lc + q] = ga[q];
+            Bs[lr][lc + q] = gb[q];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kRfB; kk += 4) {
+            const double a0 = As[wr + li][kk + lk], a1 = As[wr + 16 + li][kk + lk];
+            const double b0 = Bs[wc + li][kk + lk], b1 = Bs[wc + 16 + li][kk + lk];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = row0 + wr + 16 * a + lk + 4 * v, c = col0 + wc + 16 * b + li;
+                double *dst = C + (size_t)r * ldc + c;
+                if (kLower) *dst = *dst - acc[a][b][v];
+                else *dst = acc[a][b][v];
+            }
+}
+
+struct RfValues {
+    int ndim, dist, field, total, rpad, n, nreal;
+    double mean_value, sd_value, coe_depth, soil_max, water_max, e1;
+};
+
+// per-particle property (CPP_17-3-Nonstationary-RF.cpp:227-450, :620-846): the cell's E with the
+// depth trend below soil_max / water_max (the lognormal fluid branches take soil_max, as the
+// reference does) and the lognormal map pow(exp(1), v); particles of the other phase, or in no
+// cell, keep fill
+__global__ __launch_bounds__(256) void k_rf_values(RfParams p, RfValues q, const double *E, const int *cell_of,
+                                                   const int *type, double fill, double *value) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)q.n * q.nreal) return;
+    const int r = (int)(g / q.n), i = (int)(g - (long long)r * q.n);
+    const int c = cell_of[i];
+    const int want = q.field <= 3 ? 2 : 1;
+    double out = fill;
+    if (c >= 0 && type[i] == want) {
+        double x[3];
+        rf_centre(p, c, x);
+        const double y = q.ndim == 2 ? x[1] : x[2];
+        const double e = E[(size_t)c * q.rpad + r];
+        const double top_n = q.field <= 3 ? q.soil_max : q.water_max;
+        if (q.dist == 1) {
+            out = top_n > y ? q.mean_value + q.coe_depth * (top_n - y) + q.sd_value * e : q.mean_value + q.sd_value * e;
+        } else {
+            double tm = q.soil_max > y ? q.mean_value + q.coe_depth * (q.soil_max - y) : q.mean_value;
+            const double st = sqrt(log(1.0 + (q.sd_value / tm) * (q.sd_value / tm)));
+            tm = log(tm) - 0.5 * st * st;
+            out = pow(q.e1, tm + st * e);
+        }
+    }
+    value[g] = out;
+}
+
+}  // namespace sphg

@@ -79,6 +79,66 @@ class KLField(C.Structure):
         return wi, ev, od
 
 
+class RfField(C.Structure):
+    """sphg_rf_field: one property of a correlation-matrix random field."""
+    _fields_ = [("ndim", _I), ("dist", _I), ("field", _I), ("acf", _I), ("mean_value", _D), ("sd_value", _D),
+                ("coe_depth", _D), ("rl", _D * 3), ("beta", _D), ("beta_1", _D), ("cell_ndx", _D), ("dr", _D),
+                ("box_nb", _D * 6), ("soil_max", _D), ("water_max", _D), ("nc", _I * 3), ("cell_total", _I),
+                ("base", _D * 3), ("cell_dx", _D)]
+
+
+def rf_field(deck: Deck, dist: int, mean: float, sd: float, rl, field: int = 1, acf: int = 1, cell_ndx: float = 4.0,
+             coe_depth: float = 0.0, beta: float = 0.0, beta_1: float = 0.0) -> RfField:
+    """RfField of a deck: the non-boundary box and phase maxima as input_par forms them
+    (CPP_2-initialf.cpp:911-1017) on the vertical axis (y in 2D, z in 3D)."""
+    f = RfField()
+    f.ndim, f.dist, f.field, f.acf = deck.ndim, dist, field, acf
+    f.mean_value, f.sd_value, f.coe_depth = mean, sd, coe_depth
+    for k in range(3):
+        f.rl[k] = rl[k] if k < len(rl) else 0.0
+    f.beta, f.beta_1, f.cell_ndx, f.dr = beta, beta_1, cell_ndx, deck.dr
+    nb = (deck.type != 0) & (deck.type != 7)
+    box = []
+    for k in range(3):
+        mx = max(-10000.0, float(deck.xp[nb, k].max())) if nb.any() else -10000.0
+        mn = min(10000.0, float(deck.xp[nb, k].min())) if nb.any() else 10000.0
+        box += [mn, mx]
+    for k in range(6):
+        f.box_nb[k] = box[k]
+    _, _, ph = deck.domain()
+    v = 1 if deck.ndim == 2 else 2
+    f.soil_max = float(ph["soil"][1][v])
+    f.water_max = float(ph["water"][1][v])
+    return f
+
+
+def rf_cells(f: RfField, xp: np.ndarray, ty: np.ndarray):
+    """sphg_rf_cells (host): (rc, cell_of[n]); fills f's cell partition."""
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    ty = np.ascontiguousarray(ty, np.int32)
+    cell_of = np.zeros(xp.shape[0], np.int32)
+    rc = product_lib().lib.sphg_rf_cells(C.byref(f), xp.shape[0], _dptr(xp), _iptr(ty), _iptr(cell_of))
+    return int(rc), cell_of
+
+
+def rf_generate(f: RfField, xp: np.ndarray, ty: np.ndarray, zeta: np.ndarray, fill: float = 0.0,
+                want_l: bool = False):
+    """sphg_rf_generate: (rc, value[nreal][n], L or None, factor_ms); zeta is (nreal, cell_total)."""
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    ty = np.ascontiguousarray(ty, np.int32)
+    zeta = np.ascontiguousarray(np.atleast_2d(zeta), dtype=np.float64)
+    nreal = zeta.shape[0]
+    value = np.zeros((nreal, xp.shape[0]))
+    g = RfField.from_buffer_copy(f)
+    rc0, _ = rf_cells(g, xp, ty)
+    Lm = np.zeros((g.cell_total, g.cell_total)) if (want_l and rc0 == 0) else None
+    ms = C.c_double(0.0)
+    rc = product_lib().lib.sphg_rf_generate(C.byref(f), xp.shape[0], _dptr(xp), _iptr(ty), nreal, _dptr(zeta),
+                                            fill, _dptr(value), _dptr(Lm) if Lm is not None else None,
+                                            C.byref(ms))
+    return int(rc), value, Lm, ms.value
+
+
 def kl_field(deck: Deck, dist: int, mean: float, sd: float, rl, field: int = 1, coe_depth: float = 0.0) -> KLField:
     """K-L field for `deck` over the reference's domain (sphg_kl_domain) with its modes
     (sphg_kl_modes); the depth-trend references are the deck's soil / water maxima on the
@@ -332,6 +392,13 @@ def product_lib() -> _Lib:
         L.sphg_read_input.argtypes = [C.c_char_p, C.POINTER(Problem), C.POINTER(Materials), C.POINTER(C.c_int), _I,
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)] + [C.POINTER(C.c_int)] * 4
         L.sphg_read_input.restype = _I
+        L.sphg_rf_cells.argtypes = [C.POINTER(RfField), _I, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]
+        L.sphg_rf_cells.restype = _I
+        L.sphg_rf_generate.argtypes = [C.POINTER(RfField), _I, C.POINTER(C.c_double), C.POINTER(C.c_int), _I,
+                                       C.POINTER(C.c_double), _D, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        L.sphg_rf_generate.restype = _I
         for nm in ("save_state", "load_state"):
             f = getattr(L, "sphg_" + nm)
             f.argtypes = [C.c_void_p, C.c_char_p]

@@ -219,10 +219,123 @@ int kl_mode(const char *dir) {
     return 0;
 }
 
+// The reference's Cholesky of the correlation-matrix random fields (CPP_17-3-Nonstationary-RF.cpp:45-74),
+// an inline function of that unit; oracle/Makefile compiles the unit with -fkeep-inline-functions
+// so that its out-of-line copy exists to be called here.
+void Cholesky(double *A, double *L, int n);
+
+// Cholesky mode: chol_in.bin: int32 n; double A[n][n] -> chol_out.bin: double L[n][n]
+int chol_mode(const char *dir) {
+    std::string in = std::string(dir) + "/chol_in.bin", out = std::string(dir) + "/chol_out.bin";
+    FILE *f = fopen(in.c_str(), "rb");
+    if (!f) return 2;
+    int n = 0;
+    bool ok = fread(&n, 4, 1, f) == 1 && n > 0;
+    std::vector<double> A(ok ? (size_t)n * n : 0), L(A.size(), 0.0);
+    ok = ok && fread(A.data(), 8, A.size(), f) == A.size();
+    fclose(f);
+    if (!ok) return 2;
+    Cholesky(A.data(), L.data(), n);
+    FILE *o = fopen(out.c_str(), "wb");
+    if (!o) return 2;
+    fwrite(L.data(), 8, L.size(), o);
+    fclose(o);
+    return 0;
+}
+
+// Correlation-matrix random-field mode: runs the reference's nonstationary_random_2D/3D
+// (CPP_17-3-Nonstationary-RF.cpp:76-849) on given particles. Its standard normals come from a
+// clock-seeded generator, so the outputs are reproducible only where sd_value is 0.
+// rf_in.bin: int32 ndim, n; double dr, box_nb[6] (xmin xmax ymin ymax zmin zmax of the non-boundary
+//            particles), soil_max[3], water_max[3]; int32 flags[6]; double cell_ndx; int32 flag_auto;
+//            double beta, beta_1; 6 x {int32 type; double mean, sd, rl[3], coe_depth};
+//            double xp[n][3]; int32 type[n]; double fai[n], c[n], cop[n], ds[n], fai_w[n], c_w[n]
+// rf_out.bin: double fai[n], c[n], cop[n], ds[n], fai_w[n], c_w[n]; int32 rc
+int rf_mode(const char *dir) {
+    std::string in = std::string(dir) + "/rf_in.bin", out = std::string(dir) + "/rf_out.bin";
+    FILE *f = fopen(in.c_str(), "rb");
+    if (!f) return 2;
+    int ndim, n, flags[6], fauto;
+    double dr, box[6], smax[3], wmax[3], cndx, beta, beta1;
+    bool ok = fread(&ndim, 4, 1, f) == 1 && fread(&n, 4, 1, f) == 1 && fread(&dr, 8, 1, f) == 1 &&
+              fread(box, 8, 6, f) == 6 && fread(smax, 8, 3, f) == 3 && fread(wmax, 8, 3, f) == 3 &&
+              fread(flags, 4, 6, f) == 6 && fread(&cndx, 8, 1, f) == 1 && fread(&fauto, 4, 1, f) == 1 &&
+              fread(&beta, 8, 1, f) == 1 && fread(&beta1, 8, 1, f) == 1 && n > 0;
+    Para_SpatialVariability psv[6];
+    for (int k = 0; ok && k < 6; k++) {
+        double v[6];
+        ok = fread(&psv[k].type, 4, 1, f) == 1 && fread(v, 8, 6, f) == 6;
+        psv[k].no = k;
+        psv[k].mean_value = v[0];
+        psv[k].sd_value = v[1];
+        for (int q = 0; q < 3; q++) psv[k].rl_value[q] = v[2 + q];
+        psv[k].coe_depth = v[5];
+    }
+    std::vector<double> xp(ok ? (size_t)n * 3 : 0), vals(ok ? (size_t)n * 6 : 0);
+    std::vector<int> type(ok ? n : 0);
+    ok = ok && fread(xp.data(), 8, xp.size(), f) == xp.size() && fread(type.data(), 4, n, f) == (size_t)n &&
+         fread(vals.data(), 8, vals.size(), f) == vals.size();
+    fclose(f);
+    if (!ok) return 2;
+    Particle *par = new Particle[n];
+    Para_Soil *ps = new Para_Soil[n];
+    Para_Fluid *pf = new Para_Fluid[n];
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) par[i].xp[k] = xp[3 * i + k];
+        par[i].type = type[i];
+        ps[i].fai = vals[i];
+        ps[i].c = vals[n + i];
+        ps[i].cop = vals[2 * (size_t)n + i];
+        ps[i].ds = vals[3 * (size_t)n + i];
+        pf[i].fai = vals[4 * (size_t)n + i];
+        pf[i].c = vals[5 * (size_t)n + i];
+    }
+    Para_Pro pro;
+    pro.ntotal = n;
+    pro.ndim = ndim;
+    pro.dr = dr;
+    Flag_SpatialVariability fsv;
+    fsv.flag_fai = flags[0]; fsv.flag_c = flags[1]; fsv.flag_cop = flags[2]; fsv.flag_ds = flags[3];
+    fsv.flag_faiw = flags[4]; fsv.flag_cw = flags[5];
+    fsv.cell_ndx = cndx;
+    fsv.flag_auto = fauto;
+    fsv.beta = beta;
+    fsv.beta_1 = beta1;
+    Cell_Con cc;
+    cc.xmin_nb = box[0]; cc.xmax_nb = box[1]; cc.ymin_nb = box[2]; cc.ymax_nb = box[3];
+    cc.zmin_nb = box[4]; cc.zmax_nb = box[5];
+    for (int k = 0; k < 3; k++) {
+        cc.soil_max[k] = smax[k];
+        cc.water_max[k] = wmax[k];
+    }
+    clFSpatialVariability_Fun fn;
+    int rc = ndim == 2 ? fn.nonstationary_random_2D(par, pf, ps, pro, fsv, psv, cc)
+                       : fn.nonstationary_random_3D(par, pf, ps, pro, fsv, psv, cc);
+    for (int i = 0; i < n; i++) {
+        vals[i] = ps[i].fai;
+        vals[n + i] = ps[i].c;
+        vals[2 * (size_t)n + i] = ps[i].cop;
+        vals[3 * (size_t)n + i] = ps[i].ds;
+        vals[4 * (size_t)n + i] = pf[i].fai;
+        vals[5 * (size_t)n + i] = pf[i].c;
+    }
+    FILE *o = fopen(out.c_str(), "wb");
+    if (!o) return 2;
+    fwrite(vals.data(), 8, vals.size(), o);
+    fwrite(&rc, 4, 1, o);
+    fclose(o);
+    delete[] par;
+    delete[] ps;
+    delete[] pf;
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--kl") == 0) return kl_mode(argv[2]);
+    if (argc >= 3 && strcmp(argv[1], "--chol") == 0) return chol_mode(argv[2]);
+    if (argc >= 3 && strcmp(argv[1], "--rf") == 0) return rf_mode(argv[2]);
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir>\n", argv[0]);
+        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf <dir>\n", argv[0]);
         return 2;
     }
     return SPH_Calculation(3, argv[1]);

@@ -55,4 +55,16 @@ int or_kl_synthesize(int ndim, int n, const double *xp, const int *type, const d
 #ifdef __cplusplus
 }
 #endif
+/* Correlation-matrix random fields (rf_oracle.c, CPP_17-3-Nonstationary-RF.cpp). */
+int or_rf_cells(int ndim, int n, const double *xp, const int *type, double cell_ndx, double dr,
+                const double box_nb[6], int nc[3], double base[3], double *cell_dx_out, int *cell_total,
+                int *cell_of);
+void or_rf_corr(int ndim, int acf_type, const double rl[3], double beta, double beta_1, const int nc[3],
+                const double base[3], double cell_dx, int total, double *C);
+void or_rf_cholesky(const double *A, double *L, int n);
+void or_rf_values(int ndim, int dist, int field, double mean, double std_dev, double coe_depth, double soil_max,
+                  double water_max, const int nc[3], const double base[3], double cell_dx, int total, const double *L,
+                  const double *zeta, int n, const int *type, const int *cell_of, double fill, double *E_out,
+                  double *value);
+
 #endif

@@ -1,0 +1,140 @@
+"""Writes the correlation-matrix random-field fixtures tests/golden/rf_*.npz from the reference.
+
+- rf_chol.npz: for each case, the reference's own Cholesky (CPP_17-3-Nonstationary-RF.cpp:45-74,
+  through `oracle/_ref/ref_harness --chol`, one thread) of the case's correlation matrix, with a
+  digest of that matrix so a test knows it factors the same input.
+- rf_pipeline.npz: the reference's nonstationary_random_2D/3D (`ref_harness --rf`) on seeded
+  decks with sd_value 0, where its clock-seeded normals drop out: the per-particle properties
+  and return codes it leaves.
+
+    python tests/golden/make_rf.py
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from helpers import oracle_rf, run_ref_chol, run_ref_rf  # noqa: E402
+from sphgpy import slope_2d, tran_box  # noqa: E402
+from sphgpy.capi import rf_field  # noqa: E402
+
+
+def without_walls(d):
+    """The deck's water and soil particles only: in 3D the reference maps every particle to a
+    cell of the widened non-boundary box, and wall particles outside it index past the grid."""
+    d = d.copy()
+    keep = d.type != 0
+    for k in ("xp", "vxp", "type", "matype", "permtype_p", "etype"):
+        setattr(d, k, getattr(d, k)[keep])
+    return d
+
+
+def rf_decks():
+    return {
+        "slope": slope_2d(1.0, dx=0.1, seed=3),
+        "box2d": tran_box(2, 3, nx=10, ny=3, nz_soil=5, nz_water=4, seed=41),
+        "box3d": without_walls(tran_box(3, 3, nx=6, ny=4, nz_soil=4, nz_water=3, seed=43)),
+    }
+
+
+# Cholesky cases: (deck, acf, rl, cell_ndx, beta, beta_1)
+CHOL_CASES = {
+    "exp2d": ("slope", 1, (2.0, 0.6, 0.0), 5.0, 0.0, 0.0),
+    "gauss2d": ("slope", 2, (2.0, 0.6, 0.0), 5.0, 0.0, 0.0),
+    "gauss2d_smooth": ("slope", 2, (8.0, 3.0, 0.0), 5.0, 0.0, 0.0),  # nearly singular: the clamp acts
+    "rot2d": ("box2d", 3, (0.01, 0.004, 0.0), 1.0, 30.0, 0.0),         # not positive definite: clamps
+    "rot3d": ("box3d", 3, (0.008, 0.004, 0.006), 1.0, 20.0, 35.0),
+}
+
+# sd = 0 pipeline cases: (deck, field, dist, mean, coe_depth, cell_ndx)
+PIPE_CASES = {
+    "slope_c_normal": ("slope", 1, 1, 10000.0, 2000.0, 4.0),
+    "slope_fai_lognormal": ("slope", 0, 2, 25.0, 1.5, 4.0),
+    "box2d_cop_normal": ("box2d", 2, 1, 1e-4, 0.01, 2.0),
+    "box2d_ds_lognormal": ("box2d", 3, 2, 0.002, 0.05, 2.0),
+    "box2d_faiw_normal": ("box2d", 4, 1, 5.0, 100.0, 2.0),
+    "box2d_cw_lognormal": ("box2d", 5, 2, 3.0, 50.0, 2.0),
+    "box3d_c_normal": ("box3d", 1, 1, 800.0, 1e4, 1.0),
+    "box3d_fai_lognormal": ("box3d", 0, 2, 30.0, 200.0, 1.0),
+    "slope_too_many_cells": ("slope", 1, 1, 1.0, 0.0, 0.2),        # error 41
+    "slope_crowded_cell": ("slope", 1, 1, 1.0, 0.0, 40.0),          # error 42
+}
+
+
+def chol_matrix(name):
+    dk, acf, rl, cndx, beta, beta1 = CHOL_CASES[name]
+    d = rf_decks()[dk]
+    f = rf_field(d, 1, 1.0, 0.0, rl, acf=acf, cell_ndx=cndx, beta=beta, beta_1=beta1)
+    rc, _, cells, L, _ = oracle_rf(d.ndim, d.xp, d.type, f, np.zeros(1), want_l=False)
+    assert rc == 0, (name, rc)
+    return f, d, cells
+
+
+def corr_of(d, f, cells):
+    import ctypes as C
+    from helpers import oracle_lib
+    L = oracle_lib().lib
+    total = cells["total"]
+    A = np.zeros((total, total))
+    dp = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    nc = np.ascontiguousarray(cells["nc"], np.int32)
+    base = np.ascontiguousarray(cells["base"], np.float64)
+    rl = np.array(list(f.rl))
+    L.or_rf_corr(d.ndim, f.acf, dp(rl), C.c_double(f.beta), C.c_double(f.beta_1),
+                 nc.ctypes.data_as(C.POINTER(C.c_int)), dp(base), C.c_double(cells["cell_dx"]), total, dp(A))
+    return A
+
+
+def digest(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, np.float64).tobytes()).hexdigest()
+
+
+def pipe_inputs(name):
+    dk, field, dist, mean, cdep, cndx = PIPE_CASES[name]
+    d = rf_decks()[dk]
+    f = rf_field(d, dist, mean, 0.0, (1.0, 0.5, 0.7), field=field, acf=1, cell_ndx=cndx, coe_depth=cdep)
+    return d, f
+
+
+def main() -> None:
+    out = {}
+    for name in CHOL_CASES:
+        f, d, cells = chol_matrix(name)
+        A = corr_of(d, f, cells)
+        Lr = run_ref_chol(A)
+        out[f"{name}/L"] = Lr
+        out[f"{name}/digest"] = np.frombuffer(digest(A).encode(), np.uint8)
+        print(name, A.shape, "clamped diagonals:", int((np.diag(Lr) == np.sqrt(1e-6)).sum()))
+    np.savez_compressed(os.path.join(HERE, "rf_chol.npz"), **out)
+    out = {}
+    for name in PIPE_CASES:
+        d, f = pipe_inputs(name)
+        v = 1 if d.ndim == 2 else 2
+        _, _, ph = d.domain()
+        smax3 = ph["soil"][1]
+        wmax3 = ph["water"][1]
+        flags = [0] * 6
+        flags[f.field] = 1
+        psv = [(1, 0.0, 0.0, (1.0, 1.0, 1.0), 0.0)] * 6
+        psv[f.field] = (f.dist, f.mean_value, 0.0, tuple(f.rl), f.coe_depth)
+        rng = np.random.default_rng(5)
+        init = [rng.uniform(1.0, 2.0, d.n) for _ in range(6)]  # the properties before the field
+        vals, rc = run_ref_rf(d.ndim, d.xp, d.type, d.dr, list(f.box_nb), smax3, wmax3, flags, f.cell_ndx, 1, 0.0,
+                              0.0, psv, init)
+        assert float(smax3[v]) == f.soil_max and float(wmax3[v]) == f.water_max
+        out[f"{name}/init"] = init[f.field]
+        out[f"{name}/value"] = vals[f.field]
+        others = [k for k in range(6) if k != f.field]
+        assert all(np.array_equal(vals[k], init[k]) for k in others)  # only the flagged property changes
+        out[f"{name}/rc"] = np.array([rc], np.int32)
+        print(name, "rc", rc)
+    np.savez_compressed(os.path.join(HERE, "rf_pipeline.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

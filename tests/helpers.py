@@ -187,3 +187,101 @@ def run_ref_kl(ndim, xp, ty, a, rl, x_trans, xi, dist, mean, sd, coe_depth, soil
         return wi, ev, od, val, rc
     finally:
         shutil.rmtree(wd, ignore_errors=True)
+
+
+# ------------------------------------------------------------ correlation-matrix random fields
+def oracle_rf(ndim, xp, ty, f, zeta, fill=0.0, want_l=False):
+    """Oracle random field (oracle/rf_oracle.c) for an RfField f (the product's parameter struct,
+    read field by field): returns rc, value[n], cells dict, L (if want_l), E."""
+    L = oracle_lib().lib
+    dp = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    ip = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    ty = np.ascontiguousarray(ty, np.int32)
+    n = xp.shape[0]
+    nc = np.zeros(3, np.int32)
+    base = np.zeros(3)
+    cdx = C.c_double(0.0)
+    tot = C.c_int(0)
+    cell_of = np.zeros(n, np.int32)
+    box = np.array(list(f.box_nb), dtype=np.float64)
+    L.or_rf_cells.restype = C.c_int
+    rc = L.or_rf_cells(ndim, n, dp(xp), ip(ty), C.c_double(f.cell_ndx), C.c_double(f.dr), dp(box), ip(nc),
+                       dp(base), C.byref(cdx), C.byref(tot), ip(cell_of))
+    cells = {"nc": nc, "base": base, "cell_dx": cdx.value, "total": tot.value, "cell_of": cell_of}
+    if rc:
+        return rc, None, cells, None, None
+    total = tot.value
+    A = np.zeros((total, total))
+    rl = np.array(list(f.rl), dtype=np.float64)
+    L.or_rf_corr(ndim, f.acf, dp(rl), C.c_double(f.beta), C.c_double(f.beta_1), ip(nc), dp(base), cdx, total,
+                 dp(A))
+    Lm = np.zeros((total, total))
+    L.or_rf_cholesky(dp(A), dp(Lm), total)
+    zeta = np.ascontiguousarray(zeta, dtype=np.float64)
+    E = np.zeros(total)
+    value = np.zeros(n)
+    D = C.c_double
+    L.or_rf_values(ndim, f.dist, f.field, D(f.mean_value), D(f.sd_value), D(f.coe_depth), D(f.soil_max),
+                   D(f.water_max), ip(nc), dp(base), cdx, total, dp(Lm), dp(zeta), n, ip(ty), ip(cell_of), D(fill),
+                   dp(E), dp(value))
+    return 0, value, cells, (Lm if want_l else None), E
+
+
+def oracle_chol(A):
+    L = oracle_lib().lib
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    out = np.zeros_like(A)
+    L.or_rf_cholesky(A.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)),
+                     A.shape[0])
+    return out
+
+
+def run_ref_chol(A):
+    """The reference's own Cholesky (CPP_17-3-Nonstationary-RF.cpp:45-74) through the link harness,
+    serial (one OpenMP thread: its diagonal sum is an OpenMP reduction)."""
+    import struct
+    wd = tempfile.mkdtemp(prefix="sphchol_")
+    try:
+        A = np.ascontiguousarray(A, dtype="<f8")
+        with open(os.path.join(wd, "chol_in.bin"), "wb") as f:
+            f.write(struct.pack("<i", A.shape[0]) + A.tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([REF_HARNESS, "--chol", wd], capture_output=True, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference harness --chol failed rc={r.returncode}")
+        return np.fromfile(os.path.join(wd, "chol_out.bin"), "<f8").reshape(A.shape)
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+
+
+def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx, flag_auto, beta, beta_1, psv,
+               init):
+    """The reference's own nonstationary_random_2D/3D through the link harness. psv: six tuples
+    (type, mean, sd, rl[3], coe_depth); init: six n-arrays (soil fai c cop ds, fluid fai c).
+    Returns (six n-arrays after, rc). Reproducible only where the flagged fields have sd 0."""
+    import struct
+    wd = tempfile.mkdtemp(prefix="sphrf_")
+    try:
+        n = xp.shape[0]
+        with open(os.path.join(wd, "rf_in.bin"), "wb") as f:
+            f.write(struct.pack("<ii", ndim, n) + struct.pack("<d", dr))
+            f.write(np.asarray(box_nb, "<f8").tobytes() + np.asarray(soil_max3, "<f8").tobytes() +
+                    np.asarray(water_max3, "<f8").tobytes())
+            f.write(np.asarray(flags, "<i4").tobytes() + struct.pack("<d", cell_ndx) + struct.pack("<i", flag_auto) +
+                    struct.pack("<dd", beta, beta_1))
+            for (t, mean, sd, rl, cdep) in psv:
+                f.write(struct.pack("<i", t) + struct.pack("<6d", mean, sd, rl[0], rl[1], rl[2], cdep))
+            f.write(np.ascontiguousarray(xp, "<f8").tobytes() + np.ascontiguousarray(ty, "<i4").tobytes())
+            for a in init:
+                f.write(np.ascontiguousarray(a, "<f8").tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([REF_HARNESS, "--rf", wd], capture_output=True, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference harness --rf failed rc={r.returncode}")
+        raw = open(os.path.join(wd, "rf_out.bin"), "rb").read()
+        vals = np.frombuffer(raw, "<f8", 6 * n).reshape(6, n).copy()
+        rc = int(np.frombuffer(raw, "<i4", 1, 6 * n * 8)[0])
+        return vals, rc
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
