@@ -27,6 +27,8 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
 import time
 
@@ -96,6 +98,10 @@ WORKLOAD_DESC = {
                    "3-layer walls/floor, DP c=1 kPa phi=25, Darcy drag, water vpu=30 m/s",
     "column2d": "2D dry granular column collapse: soil 60x120 at dx=0.002 m + 3-layer floor/left wall, "
                 "DP c=0 phi=30 E=1.8 MPa, AV alpha=0.1",
+    "rf_cholesky": "correlation-matrix random field (flspv 3) of a 2D slope H=10 m at dx=0.1 m (46,970 "
+                   "particles): cells of 3 dr (11,200 cells), exponential ACF lx=10 m ly=2 m, lognormal "
+                   "cohesion (10 kPa, COV 0.3); FP64 Cholesky of the 11,200^2 correlation matrix, then "
+                   "E = L zeta and the particle values for 16 realisations",
 }
 
 
@@ -157,6 +163,100 @@ def cpu_baseline(deck, state, budget_s: float):
                       f"{el:.1f} s wall, OpenMP static schedule"}
 
 
+def rf_bench(args) -> None:
+    """One step = one random-field generation (correlation matrix, factor, E = L zeta, values);
+    the figure is the factor's rate, n^3 / 3 flops over its device time (HIP events around the
+    factor's launches)."""
+    from sphgpy import slope_2d
+    from sphgpy.capi import rf_field, rf_generate
+    deck = slope_2d(10.0, dx=0.1, seed=1)
+    f = rf_field(deck, 2, 10000.0, 3000.0, (10.0, 2.0), field=1, acf=1, cell_ndx=3.0)
+    rng = np.random.default_rng(7)
+    nreal = 16
+    steps, warm = max(1, min(args.steps, 5)), max(0, min(args.warmup, 2))
+    zeta = None
+    ms = []
+    for k in range(warm + steps):
+        if zeta is None:
+            from sphgpy.capi import rf_cells
+            rc, _ = rf_cells(f, deck.xp, deck.type)
+            if rc:
+                raise RuntimeError(f"sphg_rf_cells: {rc}")
+            zeta = rng.standard_normal((nreal, f.cell_total))
+        t0 = time.perf_counter()
+        rc, value, _, fms = rf_generate(f, deck.xp, deck.type, zeta, fill=0.0)
+        wall = time.perf_counter() - t0
+        if rc:
+            raise RuntimeError(f"sphg_rf_generate: {rc}")
+        if k >= warm:
+            ms.append((fms, wall))
+    n = f.cell_total
+    flops = n ** 3 / 3.0
+    fms = float(np.mean([m[0] for m in ms]))
+    tf = flops / (fms * 1e-3) / 1e12
+    line = {"metric": "Cholesky factor rate of the correlation-matrix random field (FP64)", "value": round(tf, 3),
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": round(fms, 3),
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic seeded slope deck and standard normals",
+            "config": {"workload": "rf_cholesky", "description": WORKLOAD_DESC["rf_cholesky"], "cells": n,
+                       "realisations": nreal, "generate_wall_s": round(float(np.mean([m[1] for m in ms])), 3)},
+            "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                         "frac": round(tf / 78.6, 4), "traffic": None, "kernel": "factor (potrf + trsm + mfma update)"},
+            "cpu_baseline": None}
+    if not args.no_cpu:
+        line["cpu_baseline"] = rf_cpu_baseline(deck, f, budget_s=args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+def rf_cpu_baseline(deck, f, budget_s: float):
+    """The reference's own Cholesky (ref_harness --chol, its OpenMP loops on up to 16 threads) on a
+    leading block of the same correlation matrix, sized to about budget_s; GFLOP/s = m^3 / 3 / t."""
+    if not os.path.exists(REF_HARNESS):
+        return None
+    import struct
+    import tempfile
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import oracle_rf
+    _, _, cells, _, _ = oracle_rf(deck.ndim, deck.xp, deck.type, f, np.zeros(1), cells_only=True)
+    threads = min(os.cpu_count() or 1, 16)
+    m = 1200
+    rate = None
+    while True:
+        sub = rf_field_matrix(deck, f, cells, m)
+        wd = tempfile.mkdtemp(prefix="rfcpu_")
+        with open(os.path.join(wd, "chol_in.bin"), "wb") as fh:
+            fh.write(struct.pack("<i", m) + np.ascontiguousarray(sub, "<f8").tobytes())
+        t0 = time.perf_counter()
+        r = subprocess.run([REF_HARNESS, "--chol", wd], capture_output=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+        dt = time.perf_counter() - t0
+        shutil.rmtree(wd, ignore_errors=True)
+        if r.returncode != 0:
+            return None
+        rate = m ** 3 / 3.0 / dt / 1e9
+        if dt > budget_s / 4 or m >= 4000:
+            break
+        m = min(4000, int(m * 1.6))
+    return {"value": round(rate, 3), "unit": "GFLOP/s", "cores": threads, "kind": "reference",
+            "sample": f"the reference's Cholesky on the leading {m} x {m} block of the same correlation matrix"}
+
+
+def rf_field_matrix(deck, f, cells, m):
+    """Leading m x m block of the field's correlation matrix (the oracle's restatement of the
+    reference's ACF loop)."""
+    import ctypes as C
+    from helpers import oracle_lib
+    L = oracle_lib().lib
+    nc = np.ascontiguousarray(cells["nc"], np.int32)
+    base = np.ascontiguousarray(cells["base"], np.float64)
+    rl = np.array(list(f.rl))
+    A = np.zeros((m, m))  # cells 0 .. m-1 of the same grid: the leading block
+    dp = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    L.or_rf_corr(deck.ndim, f.acf, dp(rl), C.c_double(f.beta), C.c_double(f.beta_1),
+                 nc.ctypes.data_as(C.POINTER(C.c_int)), dp(base), C.c_double(cells["cell_dx"]), m, dp(A))
+    return A
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +294,10 @@ def main():
                 "p99_time_to_result_s": round(stats["p99_ttr_s"], 3),
                 "roofline": None, "cpu_baseline": None}
         print(json.dumps(line), flush=True)
+        return
+
+    if args.workload == "rf_cholesky":
+        rf_bench(args)
         return
 
     def build(precision):

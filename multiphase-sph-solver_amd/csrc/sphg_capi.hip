@@ -1266,14 +1266,17 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
     std::vector<double> zh((size_t)rpad * npad, 0.0);
     for (int r = 0; r < nreal; r++)
         for (int j = 0; j < total; j++) zh[(size_t)r * npad + j] = zeta[(size_t)r * total + j];
+    // E = L Z^T in kRfSlices slices of K (multiples of 64), summed in order afterwards
+    constexpr int kRfSlices = 8;
+    const int kslice = ((npad / kRfB + kRfSlices - 1) / kRfSlices) * kRfB;
     const size_t bA = align256((size_t)npad * npad * 8), bZ = align256(zh.size() * 8),
                  bE = align256((size_t)npad * rpad * 8), bC = align256((size_t)n * 4), bT = align256((size_t)n * 4),
-                 bV = align256((size_t)nreal * n * 8);
+                 bV = align256((size_t)nreal * n * 8), bP = align256((size_t)kRfSlices * npad * rpad * 8);
     char *pool = nullptr;
-    HIPCHK(hipMalloc(&pool, bA + bZ + bE + bC + bT + bV));
+    HIPCHK(hipMalloc(&pool, bA + bZ + bE + bC + bT + bV + bP));
     double *dA = (double *)pool, *dZ = (double *)(pool + bA), *dE = (double *)(pool + bA + bZ);
     int *dC = (int *)(pool + bA + bZ + bE), *dT = (int *)(pool + bA + bZ + bE + bC);
-    double *dV = (double *)(pool + bA + bZ + bE + bC + bT);
+    double *dV = (double *)(pool + bA + bZ + bE + bC + bT), *dEp = (double *)(pool + bA + bZ + bE + bC + bT + bV);
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
@@ -1287,18 +1290,35 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
         const int gcorr = (int)std::min<size_t>((tot + 255) / 256, 8192);
         hipLaunchKernelGGL(k_rf_corr, dim3(gcorr), dim3(256), 0, st, p, dA);
         hipEventRecord(e0, st);
-        for (int k0 = 0; k0 < npad; k0 += kRfB) {
-            hipLaunchKernelGGL(k_rf_potrf, dim3(1), dim3(kRfB), 0, st, dA, npad, k0);
-            const int below = (npad - k0 - kRfB) / kRfB;
-            if (below > 0) {
+        // wide panels of 256 columns: inside one, each 64-column step factors its diagonal block,
+        // solves the rows below and updates the panel's remaining columns; the trailing matrix
+        // is then updated once with K = 256, which reads and writes it a quarter as often as
+        // 64-wide steps would
+        constexpr int kWide = 4 * kRfB;
+        const double *cA = dA;
+        for (int w0 = 0; w0 < npad; w0 += kWide) {
+            const int wend = std::min(npad, w0 + kWide);
+            for (int k0 = w0; k0 < wend; k0 += kRfB) {
+                hipLaunchKernelGGL(k_rf_potrf, dim3(1), dim3(kRfB), 0, st, dA, npad, k0);
+                const int below = (npad - k0 - kRfB) / kRfB;
+                if (below <= 0) continue;
                 hipLaunchKernelGGL(k_rf_trsm, dim3(below), dim3(kRfB), 0, st, dA, npad, k0);
-                hipLaunchKernelGGL(k_rf_gemm<true>, dim3(below * (below + 1) / 2), dim3(256), 0, st,
-                                   (const double *)dA, npad, (const double *)dA, npad, dA, npad, k0 + kRfB, k0, kRfB);
+                const int ncol = (wend - k0 - kRfB) / kRfB;
+                if (ncol > 0)
+                    hipLaunchKernelGGL(k_rf_gemm<kRect>, dim3(below, ncol), dim3(256), 0, st, cA, npad, cA, npad, dA,
+                                       npad, k0 + kRfB, k0 + kRfB, k0, kRfB, (size_t)0);
             }
+            const int T = (npad - wend) / kRfB;
+            if (T > 0)
+                hipLaunchKernelGGL(k_rf_gemm<kTri>, dim3(T * (T + 1) / 2), dim3(256), 0, st, cA, npad, cA, npad, dA,
+                                   npad, wend, wend, w0, wend - w0, (size_t)0);
         }
         hipEventRecord(e1, st);
-        hipLaunchKernelGGL(k_rf_gemm<false>, dim3(npad / kRfB, rpad / kRfB), dim3(256), 0, st, (const double *)dA,
-                           npad, (const double *)dZ, npad, dE, rpad, 0, 0, npad);
+        const size_t es = (size_t)npad * rpad;
+        hipLaunchKernelGGL(k_rf_gemm<kProd>, dim3(npad / kRfB, rpad / kRfB, kRfSlices), dim3(256), 0, st, cA, npad,
+                           (const double *)dZ, npad, dEp, rpad, 0, 0, 0, kslice, es);
+        hipLaunchKernelGGL(k_rf_sum_slices, dim3((unsigned)((es + 255) / 256)), dim3(256), 0, st, (const double *)dEp,
+                           kRfSlices, es, dE);
         const long long nv = (long long)n * nreal;
         hipLaunchKernelGGL(k_rf_values, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, p, q,
                            (const double *)dE, (const int *)dC, (const int *)dT, fill, dV);

@@ -79,71 +79,92 @@ __global__ __launch_bounds__(256) void k_rf_corr(RfParams p, double *A) {
     }
 }
 
-// Diagonal block A[k0.., k0..] (64 x 64, already updated by the earlier panels), one wave:
-// column by column, d = a_kk - sum_p l_kp^2 in ascending p, l_kk = sqrt(max(d, 1e-6)), then
-// l_ik = (a_ik - sum_p l_ip l_kp) / l_kk; the strict upper part is set to zero.
+// Diagonal block A[k0.., k0..] (64 x 64, already updated by the earlier panels), one wave,
+// right-looking with row t of the block in lane t's registers: for each column k, l_kk =
+// sqrt(max(d, 1e-6)) with d the diagonal left by the earlier columns (the reference's clamp),
+// the column below scaled by 1 / l_kk and shared through LDS, and every row below updated by the
+// rank-1 product; the strict upper part is set to zero.
 __global__ __launch_bounds__(64) void k_rf_potrf(double *A, int ld, int k0) {
-    __shared__ double S[kRfB][kRfB + 1];
+    __shared__ double col[kRfB];
+    __shared__ double diag;
     const int t = threadIdx.x;
-    double *blk = A + (size_t)k0 * ld + k0;
-    for (int r = 0; r < kRfB; r++) S[r][t] = blk[(size_t)r * ld + t];
-    __syncthreads();
+    double *row = A + (size_t)(k0 + t) * ld + k0;
+    double x[kRfB];
+#pragma unroll
+    for (int j = 0; j < kRfB; j++) x[j] = row[j];
+#pragma unroll
     for (int k = 0; k < kRfB; k++) {
         if (t == k) {
-            double s = 0.0;
-            for (int p = 0; p < k; p++) s += S[k][p] * S[k][p];
-            s = S[k][k] - s;
-            S[k][k] = sqrt(s > 1.0e-6 ? s : 1.0e-6);
+            const double d = x[k];
+            x[k] = sqrt(d > 1.0e-6 ? d : 1.0e-6);
+            diag = 1.0 / x[k];
         }
         __syncthreads();
+        if (t > k) x[k] = x[k] * diag;
+        col[t] = x[k];
+        __syncthreads();
         if (t > k) {
-            double s = 0.0;
-            for (int p = 0; p < k; p++) s += S[t][p] * S[k][p];
-            S[t][k] = (S[t][k] - s) / S[k][k];
+            const double lik = x[k];
+#pragma unroll
+            for (int j = k + 1; j < kRfB; j++) x[j] -= lik * col[j];
         }
         __syncthreads();
     }
-    for (int r = 0; r < kRfB; r++) blk[(size_t)r * ld + t] = t <= r ? S[r][t] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kRfB; j++) row[j] = j <= t ? x[j] : 0.0;
 }
 
-// Panel below the diagonal block: for 64 rows per workgroup, x L_kk^T = a row by row,
-// x_k = (a_k - sum_{p<k} x_p l_kp) / l_kk in ascending k and p.
+// Panel below the diagonal block: x L_kk^T = a for 64 rows per workgroup, one row per lane held
+// in registers, right-looking (x_k = a_k / l_kk, then a_j -= x_k l_jk for j > k).
 __global__ __launch_bounds__(64) void k_rf_trsm(double *A, int ld, int k0) {
     __shared__ double Lk[kRfB][kRfB + 1];
-    __shared__ double X[kRfB][kRfB + 1];
+    __shared__ double inv[kRfB];
     const int t = threadIdx.x;
     const int r0 = k0 + kRfB + blockIdx.x * kRfB;
     const double *lk = A + (size_t)k0 * ld + k0;
-    double *rows = A + (size_t)r0 * ld + k0;
-    for (int r = 0; r < kRfB; r++) {
-        Lk[r][t] = lk[(size_t)r * ld + t];
-        X[r][t] = rows[(size_t)r * ld + t];
-    }
+    for (int r = 0; r < kRfB; r++) Lk[r][t] = lk[(size_t)r * ld + t];
     __syncthreads();
+    inv[t] = 1.0 / Lk[t][t];
+    double *row = A + (size_t)(r0 + t) * ld + k0;
+    double x[kRfB];
+#pragma unroll
+    for (int k = 0; k < kRfB; k++) x[k] = row[k];
+    __syncthreads();
+#pragma unroll
     for (int k = 0; k < kRfB; k++) {
-        double s = 0.0;
-        for (int p = 0; p < k; p++) s += X[t][p] * Lk[k][p];
-        X[t][k] = (X[t][k] - s) / Lk[k][k];
+        x[k] = x[k] * inv[k];
+#pragma unroll
+        for (int j = k + 1; j < kRfB; j++) x[j] -= x[k] * Lk[j][k];
     }
-    __syncthreads();
-    for (int r = 0; r < kRfB; r++) rows[(size_t)r * ld + t] = X[r][t];
+#pragma unroll
+    for (int k = 0; k < kRfB; k++) row[k] = x[k];
 }
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// C tile (64 x 64) op= A rows x B rows^T over k in [0, kend), four waves of 32 x 32, each 2 x 2
+// C tile (64 x 64) op= A rows x B rows^T over K columns from k0, four waves of 32 x 32, each 2 x 2
 // v_mfma_f64_16x16x4_f64 tiles (lane l supplies A[l % 16][l / 16] and B^T[l / 16][l % 16] and
-// holds D[l / 16 + 4 v][l % 16] in accumulator v). kLower: the grid walks the lower tiles
-// (ti >= tj) of the trailing matrix from row r0 and subtracts (the Cholesky update, K = 64);
-// otherwise a full grid (M / 64, N / 64) writes C = A B^T with K limited to the nonzero blocks
-// of a lower-triangular A (E = L Z^T).
-template <bool kLower>
+// holds D[l / 16 + 4 v][l % 16] in accumulator v). Modes:
+//   kTri   the lower tiles (ti >= tj) of the square block from (r0, c0), walked by a 1D grid;
+//          C -= A A^T (the trailing update of the factor);
+//   kRect  a 2D grid of tiles from (r0, c0) that skips tiles above the diagonal; C -= A A^T (the
+//          update of the columns still inside the current wide panel);
+//   kProd  a 3D grid, C_s = A B^T over the K slice [s K, (s + 1) K) of blockIdx.z = s, limited to
+//          the nonzero blocks of a lower-triangular A; slice s is written at C + s * cstride and
+//          the slices are summed in order by k_rf_sum_slices (E = L Z^T, split so that enough
+//          workgroups share the long K of a product with few columns).
+enum { kTri = 0, kRect = 1, kProd = 2 };
+
+template <int kMode>
 __global__ __launch_bounds__(256) void k_rf_gemm(const double *A, int lda, const double *B, int ldb, double *C,
-                                                 int ldc, int r0, int k0, int K) {
-    __shared__ double As[kRfB][kRfB + 1];
-    __shared__ double Bs[kRfB][kRfB + 1];
+                                                 int ldc, int r0, int c0, int k0, int K, size_t cstride) {
+    constexpr bool kLower = kMode != kProd;
+    constexpr int kKc = 32;  // k columns staged per round: 2 x 18 KB of LDS, four workgroups per CU
+    // row stride 36: the operand reads (16 rows x 4 k per wave) fall on distinct bank pairs
+    __shared__ double As[kRfB][kKc + 4];
+    __shared__ double Bs[kRfB][kKc + 4];
     int ti, tj;
-    if (kLower) {
+    if (kMode == kTri) {
         const int b = blockIdx.x;
         ti = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
         while ((ti + 1) * (ti + 2) / 2 <= b) ti++;
@@ -153,30 +174,39 @@ __global__ __launch_bounds__(256) void k_rf_gemm(const double *A, int lda, const
         ti = blockIdx.x;
         tj = blockIdx.y;
     }
-    const int row0 = r0 + ti * kRfB, col0 = kLower ? r0 + tj * kRfB : tj * kRfB;
+    const int row0 = r0 + ti * kRfB, col0 = c0 + tj * kRfB;
+    if (kMode == kRect && col0 > row0) return;  // above the diagonal: never read, left alone
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
+    // the update starts from the C tile (read before the first staging, so the two overlap) and
+    // subtracts with a negated A operand; the product form starts from zero
     f64x4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-    const int kend = kLower ? K : min(K, (ti + 1) * kRfB);  // A lower triangular: blocks past the diagonal are zero
-    for (int kc = 0; kc < kend; kc += kRfB) {
-        // stage the 64 x 64 chunks: thread t loads row t / 4, 16 consecutive k
-        const int lr = t >> 2, lc = (t & 3) * 16;
-        const double *ga = A + (size_t)(row0 + lr) * lda + k0 + kc + lc;
-        const double *gb = B + (size_t)(col0 + lr) * ldb + k0 + kc + lc;
+        for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            As[lr][lc + q] = ga[q];
-            Bs[lr][lc + q] = gb[q];
+            for (int v = 0; v < 4; v++)
+                acc[a][b][v] = kLower ? C[(size_t)(row0 + wr + 16 * a + lk + 4 * v) * ldc + col0 + wc + 16 * b + li] : 0.0;
+    const double sgn = kLower ? -1.0 : 1.0;
+    // the product's slice; A lower triangular: blocks past the diagonal are zero
+    const int kbeg = kLower ? 0 : (int)blockIdx.z * K;
+    const int kend = kLower ? K : min(kbeg + K, (ti + 1) * kRfB);
+    if (!kLower) C += (size_t)blockIdx.z * cstride;
+    const int lc = t & (kKc - 1), lr0 = t / kKc;
+    for (int kc = kbeg; kc < kend; kc += kKc) {
+        // stage the 64 x 32 chunks: in load q each wave reads two row segments of 256 contiguous bytes
+#pragma unroll
+        for (int q = 0; q < kRfB * kKc / 256; q++) {
+            const int lr = q * (256 / kKc) + lr0;
+            As[lr][lc] = A[(size_t)(row0 + lr) * lda + k0 + kc + lc];
+            Bs[lr][lc] = B[(size_t)(col0 + lr) * ldb + k0 + kc + lc];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < kRfB; kk += 4) {
-            const double a0 = As[wr + li][kk + lk], a1 = As[wr + 16 + li][kk + lk];
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 4) {
+            const double a0 = sgn * As[wr + li][kk + lk], a1 = sgn * As[wr + 16 + li][kk + lk];
             const double b0 = Bs[wc + li][kk + lk], b1 = Bs[wc + 16 + li][kk + lk];
             acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
             acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
@@ -190,12 +220,17 @@ __global__ __launch_bounds__(256) void k_rf_gemm(const double *A, int lda, const
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int r = row0 + wr + 16 * a + lk + 4 * v, c = col0 + wc + 16 * b + li;
-                double *dst = C + (size_t)r * ldc + c;
-                if (kLower) *dst = *dst - acc[a][b][v];
-                else *dst = acc[a][b][v];
-            }
+            for (int v = 0; v < 4; v++)
+                C[(size_t)(row0 + wr + 16 * a + lk + 4 * v) * ldc + col0 + wc + 16 * b + li] = acc[a][b][v];
+}
+
+// E[e] = sum over the slices s = 0, 1, ... of C_s[e], in that order
+__global__ __launch_bounds__(256) void k_rf_sum_slices(const double *part, int nslice, size_t stride, double *E) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= stride) return;
+    double s = 0.0;
+    for (int q = 0; q < nslice; q++) s += part[(size_t)q * stride + e];
+    E[e] = s;
 }
 
 struct RfValues {

@@ -190,9 +190,10 @@ def run_ref_kl(ndim, xp, ty, a, rl, x_trans, xi, dist, mean, sd, coe_depth, soil
 
 
 # ------------------------------------------------------------ correlation-matrix random fields
-def oracle_rf(ndim, xp, ty, f, zeta, fill=0.0, want_l=False):
+def oracle_rf(ndim, xp, ty, f, zeta, fill=0.0, want_l=False, cells_only=False):
     """Oracle random field (oracle/rf_oracle.c) for an RfField f (the product's parameter struct,
-    read field by field): returns rc, value[n], cells dict, L (if want_l), E."""
+    read field by field): returns rc, value[n], cells dict, L (if want_l), E; with cells_only the
+    cell partition alone (rc, None, cells, None, None)."""
     L = oracle_lib().lib
     dp = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
     ip = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
@@ -209,7 +210,7 @@ def oracle_rf(ndim, xp, ty, f, zeta, fill=0.0, want_l=False):
     rc = L.or_rf_cells(ndim, n, dp(xp), ip(ty), C.c_double(f.cell_ndx), C.c_double(f.dr), dp(box), ip(nc),
                        dp(base), C.byref(cdx), C.byref(tot), ip(cell_of))
     cells = {"nc": nc, "base": base, "cell_dx": cdx.value, "total": tot.value, "cell_of": cell_of}
-    if rc:
+    if rc or cells_only:
         return rc, None, cells, None, None
     total = tot.value
     A = np.zeros((total, total))
