@@ -1277,11 +1277,15 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
     double *dA = (double *)pool, *dZ = (double *)(pool + bA), *dE = (double *)(pool + bA + bZ);
     int *dC = (int *)(pool + bA + bZ + bE), *dT = (int *)(pool + bA + bZ + bE + bC);
     double *dV = (double *)(pool + bA + bZ + bE + bC + bT), *dEp = (double *)(pool + bA + bZ + bE + bC + bT + bV);
-    hipStream_t st = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t st = nullptr, st2 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, ev_f = nullptr, ev_b = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_f, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_b, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ev_b, st2);  // nothing pending on the second stream yet
     if (e == hipSuccess) e = hipMemcpyAsync(dZ, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(dC, cell_of.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(dT, type, (size_t)n * 4, hipMemcpyHostToDevice, st);
@@ -1291,12 +1295,14 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
         hipLaunchKernelGGL(k_rf_corr, dim3(gcorr), dim3(256), 0, st, p, dA);
         hipEventRecord(e0, st);
         // wide panels of 256 columns: inside one, each 64-column step factors its diagonal block,
-        // solves the rows below and updates the panel's remaining columns; the trailing matrix
-        // is then updated once with K = 256, which reads and writes it a quarter as often as
-        // 64-wide steps would
+        // solves the rows below and updates the panel's remaining columns. The trailing update
+        // by a finished panel (K = 256, a quarter of the traffic of 64-wide steps) is split: the
+        // next panel's columns on the main stream, the rest on a second stream, where it runs
+        // while the main stream factors the next panel (one panel of look-ahead). The main
+        // stream waits for the second before it touches columns the second still updates.
         constexpr int kWide = 4 * kRfB;
         const double *cA = dA;
-        for (int w0 = 0; w0 < npad; w0 += kWide) {
+        for (int w0 = 0; w0 < npad && e == hipSuccess; w0 += kWide) {
             const int wend = std::min(npad, w0 + kWide);
             for (int k0 = w0; k0 < wend; k0 += kRfB) {
                 hipLaunchKernelGGL(k_rf_potrf, dim3(1), dim3(kRfB), 0, st, dA, npad, k0);
@@ -1309,10 +1315,25 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
                                        npad, k0 + kRfB, k0 + kRfB, k0, kRfB, (size_t)0);
             }
             const int T = (npad - wend) / kRfB;
-            if (T > 0)
-                hipLaunchKernelGGL(k_rf_gemm<kTri>, dim3(T * (T + 1) / 2), dim3(256), 0, st, cA, npad, cA, npad, dA,
-                                   npad, wend, wend, w0, wend - w0, (size_t)0);
+            if (T <= 0) break;
+            const int nnext = std::min(T, kWide / kRfB);  // the next panel's column tiles
+            // the rest: rows and columns past the next panel, on the second stream once this
+            // panel is factored
+            e = hipEventRecord(ev_f, st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st2, ev_f, 0);
+            const int Tr = T - nnext;
+            if (e == hipSuccess && Tr > 0)
+                hipLaunchKernelGGL(k_rf_gemm<kTri>, dim3(Tr * (Tr + 1) / 2), dim3(256), 0, st2, cA, npad, cA, npad, dA,
+                                   npad, wend + nnext * kRfB, wend + nnext * kRfB, w0, wend - w0, (size_t)0);
+            // the next panel's columns, after the second stream has finished the previous
+            // panel's update of those same columns
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, ev_b, 0);
+            if (e == hipSuccess)
+                hipLaunchKernelGGL(k_rf_gemm<kRect>, dim3(T, nnext), dim3(256), 0, st, cA, npad, cA, npad, dA, npad,
+                                   wend, wend, w0, wend - w0, (size_t)0);
+            if (e == hipSuccess) e = hipEventRecord(ev_b, st2);
         }
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, ev_b, 0);
         hipEventRecord(e1, st);
         const size_t es = (size_t)npad * rpad;
         hipLaunchKernelGGL(k_rf_gemm<kProd>, dim3(npad / kRfB, rpad / kRfB, kRfSlices), dim3(256), 0, st, cA, npad,
@@ -1338,7 +1359,11 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
     }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
+    if (ev_f) hipEventDestroy(ev_f);
+    if (ev_b) hipEventDestroy(ev_b);
+    if (st2) hipStreamSynchronize(st2);
     if (st) hipStreamDestroy(st);
+    if (st2) hipStreamDestroy(st2);
     hipFree(pool);
     return e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
 }
