@@ -353,6 +353,18 @@ int sphg_rf_cells(sphg_rf_field *f, int n, const double *xp, const int *type, in
 int sphg_rf_generate(const sphg_rf_field *f, int n, const double *xp, const int *type, int nreal,
                      const double *zeta, double fill, double *value, double *L, double *factor_ms);
 
+/* Correlated friction angle and cohesion of one phase (flspv 4, nonstationary_correlated_random_2D
+ * / _3D, CPP_17-5-Nonstationary-Correlated-RF.cpp:87-737): one correlation matrix (geometry,
+ * autocorrelation and lengths of f_fai), its factor, per cell two standard normals zeta
+ * [nreal][cell_total][2] mixed by [[1, rho], [0, sqrt(1 - rho^2)]] (rho = coe_frict_cohesion, or
+ * the correlation of the underlying normals when f_fai is lognormal), E = L * zeta, and the two
+ * properties: f_fai.field 0 (soil) or 4 (fluid), f_c.field one more; both take f_fai's
+ * distribution, lognormal values are exp(v), and the fluid pair measures its depth trend on y in
+ * 3D too (as the reference does). */
+int sphg_rf_generate_pair(const sphg_rf_field *f_fai, const sphg_rf_field *f_c, double coe_frict_cohesion,
+                          int n, const double *xp, const int *type, int nreal, const double *zeta, double fill,
+                          double *value_fai, double *value_c);
+
 const char *sphg_error_string(int code);
 const char *sphg_build_info(void);
 

@@ -1233,17 +1233,20 @@ extern "C" int sphg_rf_cells(sphg_rf_field *f, int n, const double *xp, const in
     return SPHG_OK;
 }
 
-extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *xp, const int *type, int nreal,
-                                const double *zeta, double fill, double *value, double *L, double *factor_ms) {
-    if (!fin || !zeta || !value || nreal <= 0 || (fin->dist != 1 && fin->dist != 2) || fin->field < 0 ||
-        fin->field > 5)
-        return SPHG_E_ARG;
+// One random-field generation: cell partition and correlation matrix of the geometry in fin,
+// its factor, E = L Z^T for the ncolz columns zcols[ncolz][total], and one value array per spec
+// (values[s][specs[s].nreal][n]).
+static int rf_run(const sphg_rf_field *fin, int n, const double *xp, const int *type, int ncolz, const double *zcols,
+                  int nspec, const sphg::RfValues *specs, double *const *values, double fill, double *L,
+                  double *factor_ms) {
     sphg_rf_field f = *fin;
     std::vector<int> cell_of(n > 0 ? n : 0);
     int rc = sphg_rf_cells(&f, n, xp, type, cell_of.data());
     if (rc) return rc;
     using namespace sphg;
-    const int total = f.cell_total, npad = (total + kRfB - 1) / kRfB * kRfB, rpad = (nreal + kRfB - 1) / kRfB * kRfB;
+    const int total = f.cell_total, npad = (total + kRfB - 1) / kRfB * kRfB, rpad = (ncolz + kRfB - 1) / kRfB * kRfB;
+    int vreal = 0;
+    for (int q = 0; q < nspec; q++) vreal = std::max(vreal, specs[q].nreal);
     RfParams p{};
     p.ndim = f.ndim;
     p.acf = f.acf;
@@ -1258,20 +1261,15 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
     }
     const double bt = f.beta * kRefPi / 180.0, bt1 = f.beta_1 * kRefPi / 180.0;  // beta_temp, beta1_temp
     p.cb = cos(bt); p.sb = sin(bt); p.cb1 = cos(bt1); p.sb1 = sin(bt1);
-    RfValues q{};
-    q.ndim = f.ndim; q.dist = f.dist; q.field = f.field; q.total = total; q.rpad = rpad; q.n = n; q.nreal = nreal;
-    q.mean_value = f.mean_value; q.sd_value = f.sd_value; q.coe_depth = f.coe_depth;
-    q.soil_max = f.soil_max; q.water_max = f.water_max;
-    q.e1 = exp(1.0);
     std::vector<double> zh((size_t)rpad * npad, 0.0);
-    for (int r = 0; r < nreal; r++)
-        for (int j = 0; j < total; j++) zh[(size_t)r * npad + j] = zeta[(size_t)r * total + j];
+    for (int r = 0; r < ncolz; r++)
+        for (int j = 0; j < total; j++) zh[(size_t)r * npad + j] = zcols[(size_t)r * total + j];
     // E = L Z^T in kRfSlices slices of K (multiples of 64), summed in order afterwards
     constexpr int kRfSlices = 8;
     const int kslice = ((npad / kRfB + kRfSlices - 1) / kRfSlices) * kRfB;
     const size_t bA = align256((size_t)npad * npad * 8), bZ = align256(zh.size() * 8),
                  bE = align256((size_t)npad * rpad * 8), bC = align256((size_t)n * 4), bT = align256((size_t)n * 4),
-                 bV = align256((size_t)nreal * n * 8), bP = align256((size_t)kRfSlices * npad * rpad * 8);
+                 bV = align256((size_t)vreal * n * 8), bP = align256((size_t)kRfSlices * npad * rpad * 8);
     char *pool = nullptr;
     HIPCHK(hipMalloc(&pool, bA + bZ + bE + bC + bT + bV + bP));
     double *dA = (double *)pool, *dZ = (double *)(pool + bA), *dE = (double *)(pool + bA + bZ);
@@ -1340,12 +1338,19 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
                            (const double *)dZ, npad, dEp, rpad, 0, 0, 0, kslice, es);
         hipLaunchKernelGGL(k_rf_sum_slices, dim3((unsigned)((es + 255) / 256)), dim3(256), 0, st, (const double *)dEp,
                            kRfSlices, es, dE);
-        const long long nv = (long long)n * nreal;
-        hipLaunchKernelGGL(k_rf_values, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, p, q,
-                           (const double *)dE, (const int *)dC, (const int *)dT, fill, dV);
         e = hipGetLastError();
+        for (int sidx = 0; sidx < nspec && e == hipSuccess; sidx++) {
+            RfValues q = specs[sidx];
+            q.total = total;
+            q.rpad = rpad;
+            q.n = n;
+            q.e1 = exp(1.0);
+            const long long nv = (long long)n * q.nreal;
+            hipLaunchKernelGGL(k_rf_values, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, p, q,
+                               (const double *)dE, (const int *)dC, (const int *)dT, fill, dV);
+            e = hipMemcpyAsync(values[sidx], dV, (size_t)q.nreal * n * 8, hipMemcpyDeviceToHost, st);
+        }
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(value, dV, (size_t)nreal * n * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess && L) {
         for (int r = 0; r < total && e == hipSuccess; r++) {
@@ -1366,4 +1371,86 @@ extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *x
     if (st2) hipStreamDestroy(st2);
     hipFree(pool);
     return e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
+}
+
+// the spec of one property of the correlation-matrix method (nonstationary_random_2D/3D)
+static sphg::RfValues rf_spec_single(const sphg_rf_field &f, int nreal) {
+    sphg::RfValues q{};
+    q.dist = f.dist;
+    q.want_type = f.field <= 3 ? 2 : 1;
+    q.nreal = nreal;
+    q.ncol = 1;
+    q.col = 0;
+    q.vaxis = f.ndim == 2 ? 1 : 2;
+    q.use_exp = 0;
+    q.mean_value = f.mean_value; q.sd_value = f.sd_value; q.coe_depth = f.coe_depth;
+    q.top_normal = f.field <= 3 ? f.soil_max : f.water_max;
+    q.top_log = f.soil_max;
+    return q;
+}
+
+extern "C" int sphg_rf_generate(const sphg_rf_field *fin, int n, const double *xp, const int *type, int nreal,
+                                const double *zeta, double fill, double *value, double *L, double *factor_ms) {
+    if (!fin || !zeta || !value || nreal <= 0 || (fin->dist != 1 && fin->dist != 2) || fin->field < 0 ||
+        fin->field > 5)
+        return SPHG_E_ARG;
+    const sphg::RfValues q = rf_spec_single(*fin, nreal);
+    double *vals[1] = {value};
+    return rf_run(fin, n, xp, type, nreal, zeta, 1, &q, vals, fill, L, factor_ms);
+}
+
+extern "C" int sphg_rf_generate_pair(const sphg_rf_field *f_fai, const sphg_rf_field *f_c, double coe_frict_cohesion,
+                                     int n, const double *xp, const int *type, int nreal, const double *zeta,
+                                     double fill, double *value_fai, double *value_c) {
+    if (!f_fai || !f_c || !zeta || !value_fai || !value_c || nreal <= 0 || (f_fai->dist != 1 && f_fai->dist != 2) ||
+        (f_fai->field != 0 && f_fai->field != 4) || f_c->field != f_fai->field + 1)
+        return SPHG_E_ARG;
+    sphg_rf_field g = *f_fai;
+    int rc = sphg_rf_cells(&g, n, xp, type, nullptr);
+    if (rc) return rc;
+    const int total = g.cell_total;
+    // correlation of the two standard normals (CPP_17-5-Nonstationary-Correlated-RF.cpp:215-231): the
+    // lognormal pair takes the correlation of the underlying normals
+    double coef = coe_frict_cohesion;
+    if (f_fai->dist == 2) {
+        const double m0 = f_fai->mean_value, s0 = f_fai->sd_value, m1 = f_c->mean_value, s1 = f_c->sd_value;
+        coef = log(1.0 + coe_frict_cohesion * (s0 / m0) * (s1 / m1));
+        coef = coef / sqrt(log(1.0 + s0 * s0 / m0 / m0) * log(1.0 + s1 * s1 / m1 / m1));
+    }
+    const double cov[4] = {1.0, coef, 0.0, sqrt(1.0 - coef * coef)};
+    // zeta_1 * cov per cell (matrix_multi, :22-36), realisation r's two columns at rows 2r, 2r + 1
+    std::vector<double> z((size_t)2 * nreal * total);
+    for (int r = 0; r < nreal; r++)
+        for (int j = 0; j < total; j++) {
+            const double *z1 = zeta + ((size_t)r * total + j) * 2;
+            for (int c = 0; c < 2; c++) {
+                double t = 0.0;
+                t = t + z1[0] * cov[0 * 2 + c];
+                t = t + z1[1] * cov[1 * 2 + c];
+                z[((size_t)2 * r + c) * total + j] = t;
+            }
+        }
+    sphg::RfValues q[2];
+    const sphg_rf_field *fs[2] = {f_fai, f_c};
+    for (int k = 0; k < 2; k++) {
+        sphg::RfValues &v = q[k];
+        v = sphg::RfValues{};
+        v.dist = f_fai->dist;  // both properties take the distribution of the friction angle
+        v.want_type = f_fai->field == 0 ? 2 : 1;
+        v.nreal = nreal;
+        v.ncol = 2;
+        v.col = k;
+        v.use_exp = 1;
+        v.mean_value = fs[k]->mean_value; v.sd_value = fs[k]->sd_value; v.coe_depth = fs[k]->coe_depth;
+        if (f_fai->field == 0) {  // soil: depth below soil_max on the vertical axis
+            v.vaxis = g.ndim == 2 ? 1 : 2;
+            v.top_normal = v.top_log = f_fai->soil_max;
+        } else {  // fluid: the normal branch below water_max, the lognormal below soil_max, on y in 2D and 3D
+            v.vaxis = 1;
+            v.top_normal = f_fai->water_max;
+            v.top_log = f_fai->soil_max;
+        }
+    }
+    double *vals[2] = {value_fai, value_c};
+    return rf_run(f_fai, n, xp, type, 2 * nreal, z.data(), 2, q, vals, fill, nullptr, nullptr);
 }

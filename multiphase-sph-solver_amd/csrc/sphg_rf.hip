@@ -233,36 +233,41 @@ __global__ __launch_bounds__(256) void k_rf_sum_slices(const double *part, int n
     E[e] = s;
 }
 
+// How the cell values E become one property: phase, column of E, the vertical axis and the tops
+// of the depth trend for the normal and the lognormal branch, and the lognormal map.
 struct RfValues {
-    int ndim, dist, field, total, rpad, n, nreal;
-    double mean_value, sd_value, coe_depth, soil_max, water_max, e1;
+    int dist, want_type, total, rpad, n, nreal;
+    int ncol, col;       // E column of realisation r: r * ncol + col
+    int vaxis;           // coordinate of the cell centre the trend is measured on
+    int use_exp;         // lognormal value as exp(v) (correlated pair) or pow(exp(1), v)
+    double mean_value, sd_value, coe_depth, top_normal, top_log, e1;
 };
 
-// per-particle property (CPP_17-3-Nonstationary-RF.cpp:227-450, :620-846): the cell's E with the
-// depth trend below soil_max / water_max (the lognormal fluid branches take soil_max, as the
-// reference does) and the lognormal map pow(exp(1), v); particles of the other phase, or in no
-// cell, keep fill
+// per-particle property (CPP_17-3-Nonstationary-RF.cpp:227-450, :620-846;
+// CPP_17-5-Nonstationary-Correlated-RF.cpp:244-395, :574-725): the cell's E with the depth trend
+// below the branch's top and the lognormal map; particles of the other phase, or in no cell, keep
+// fill
 __global__ __launch_bounds__(256) void k_rf_values(RfParams p, RfValues q, const double *E, const int *cell_of,
                                                    const int *type, double fill, double *value) {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)q.n * q.nreal) return;
     const int r = (int)(g / q.n), i = (int)(g - (long long)r * q.n);
     const int c = cell_of[i];
-    const int want = q.field <= 3 ? 2 : 1;
     double out = fill;
-    if (c >= 0 && type[i] == want) {
+    if (c >= 0 && type[i] == q.want_type) {
         double x[3];
         rf_centre(p, c, x);
-        const double y = q.ndim == 2 ? x[1] : x[2];
-        const double e = E[(size_t)c * q.rpad + r];
-        const double top_n = q.field <= 3 ? q.soil_max : q.water_max;
+        const double y = x[q.vaxis];
+        const double e = E[(size_t)c * q.rpad + (size_t)r * q.ncol + q.col];
         if (q.dist == 1) {
-            out = top_n > y ? q.mean_value + q.coe_depth * (top_n - y) + q.sd_value * e : q.mean_value + q.sd_value * e;
+            const double top = q.top_normal;
+            out = top > y ? q.mean_value + q.coe_depth * (top - y) + q.sd_value * e : q.mean_value + q.sd_value * e;
         } else {
-            double tm = q.soil_max > y ? q.mean_value + q.coe_depth * (q.soil_max - y) : q.mean_value;
+            double tm = q.top_log > y ? q.mean_value + q.coe_depth * (q.top_log - y) : q.mean_value;
             const double st = sqrt(log(1.0 + (q.sd_value / tm) * (q.sd_value / tm)));
             tm = log(tm) - 0.5 * st * st;
-            out = pow(q.e1, tm + st * e);
+            const double v = tm + st * e;
+            out = q.use_exp ? exp(v) : pow(q.e1, v);
         }
     }
     value[g] = out;

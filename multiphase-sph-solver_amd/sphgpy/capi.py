@@ -139,6 +139,23 @@ def rf_generate(f: RfField, xp: np.ndarray, ty: np.ndarray, zeta: np.ndarray, fi
     return int(rc), value, Lm, ms.value
 
 
+def rf_generate_pair(f_fai: RfField, f_c: RfField, coe: float, xp: np.ndarray, ty: np.ndarray, zeta: np.ndarray,
+                     fill: float = 0.0):
+    """sphg_rf_generate_pair: (rc, value_fai[nreal][n], value_c[nreal][n]); zeta is
+    (nreal, cell_total, 2)."""
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    ty = np.ascontiguousarray(ty, np.int32)
+    zeta = np.ascontiguousarray(zeta, dtype=np.float64)
+    if zeta.ndim == 2:
+        zeta = zeta[None]
+    nreal = zeta.shape[0]
+    va = np.zeros((nreal, xp.shape[0]))
+    vc = np.zeros((nreal, xp.shape[0]))
+    rc = product_lib().lib.sphg_rf_generate_pair(C.byref(f_fai), C.byref(f_c), coe, xp.shape[0], _dptr(xp),
+                                                 _iptr(ty), nreal, _dptr(zeta), fill, _dptr(va), _dptr(vc))
+    return int(rc), va, vc
+
+
 def kl_field(deck: Deck, dist: int, mean: float, sd: float, rl, field: int = 1, coe_depth: float = 0.0) -> KLField:
     """K-L field for `deck` over the reference's domain (sphg_kl_domain) with its modes
     (sphg_kl_modes); the depth-trend references are the deck's soil / water maxima on the
@@ -399,6 +416,10 @@ def product_lib() -> _Lib:
                                        C.POINTER(C.c_double), _D, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
         L.sphg_rf_generate.restype = _I
+        L.sphg_rf_generate_pair.argtypes = [C.POINTER(RfField), C.POINTER(RfField), _D, _I, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int), _I, C.POINTER(C.c_double), _D,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.sphg_rf_generate_pair.restype = _I
         for nm in ("save_state", "load_state"):
             f = getattr(L, "sphg_" + nm)
             f.argtypes = [C.c_void_p, C.c_char_p]

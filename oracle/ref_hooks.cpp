@@ -243,15 +243,17 @@ int chol_mode(const char *dir) {
     return 0;
 }
 
-// Correlation-matrix random-field mode: runs the reference's nonstationary_random_2D/3D
-// (CPP_17-3-Nonstationary-RF.cpp:76-849) on given particles. Its standard normals come from a
+// Correlation-matrix random-field modes: --rf runs the reference's nonstationary_random_2D/3D
+// (CPP_17-3-Nonstationary-RF.cpp:76-849), --rfc its nonstationary_correlated_random_2D/3D
+// (CPP_17-5-Nonstationary-Correlated-RF.cpp:87-737, the coefficient coe_frict_cohesion read after
+// the properties) on given particles. Its standard normals come from a
 // clock-seeded generator, so the outputs are reproducible only where sd_value is 0.
 // rf_in.bin: int32 ndim, n; double dr, box_nb[6] (xmin xmax ymin ymax zmin zmax of the non-boundary
 //            particles), soil_max[3], water_max[3]; int32 flags[6]; double cell_ndx; int32 flag_auto;
 //            double beta, beta_1; 6 x {int32 type; double mean, sd, rl[3], coe_depth};
 //            double xp[n][3]; int32 type[n]; double fai[n], c[n], cop[n], ds[n], fai_w[n], c_w[n]
 // rf_out.bin: double fai[n], c[n], cop[n], ds[n], fai_w[n], c_w[n]; int32 rc
-int rf_mode(const char *dir) {
+int rf_mode(const char *dir, bool correlated) {
     std::string in = std::string(dir) + "/rf_in.bin", out = std::string(dir) + "/rf_out.bin";
     FILE *f = fopen(in.c_str(), "rb");
     if (!f) return 2;
@@ -275,6 +277,8 @@ int rf_mode(const char *dir) {
     std::vector<int> type(ok ? n : 0);
     ok = ok && fread(xp.data(), 8, xp.size(), f) == xp.size() && fread(type.data(), 4, n, f) == (size_t)n &&
          fread(vals.data(), 8, vals.size(), f) == vals.size();
+    double coe = 0.0;
+    if (correlated) ok = ok && fread(&coe, 8, 1, f) == 1;
     fclose(f);
     if (!ok) return 2;
     Particle *par = new Particle[n];
@@ -301,6 +305,7 @@ int rf_mode(const char *dir) {
     fsv.flag_auto = fauto;
     fsv.beta = beta;
     fsv.beta_1 = beta1;
+    fsv.coe_frict_cohesion = coe;
     Cell_Con cc;
     cc.xmin_nb = box[0]; cc.xmax_nb = box[1]; cc.ymin_nb = box[2]; cc.ymax_nb = box[3];
     cc.zmin_nb = box[4]; cc.zmax_nb = box[5];
@@ -309,7 +314,12 @@ int rf_mode(const char *dir) {
         cc.water_max[k] = wmax[k];
     }
     clFSpatialVariability_Fun fn;
-    int rc = ndim == 2 ? fn.nonstationary_random_2D(par, pf, ps, pro, fsv, psv, cc)
+    int rc;
+    if (correlated)
+        rc = ndim == 2 ? fn.nonstationary_correlated_random_2D(par, pf, ps, pro, fsv, psv, cc)
+                       : fn.nonstationary_correlated_random_3D(par, pf, ps, pro, fsv, psv, cc);
+    else
+        rc = ndim == 2 ? fn.nonstationary_random_2D(par, pf, ps, pro, fsv, psv, cc)
                        : fn.nonstationary_random_3D(par, pf, ps, pro, fsv, psv, cc);
     for (int i = 0; i < n; i++) {
         vals[i] = ps[i].fai;
@@ -333,9 +343,10 @@ int rf_mode(const char *dir) {
 int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--kl") == 0) return kl_mode(argv[2]);
     if (argc >= 3 && strcmp(argv[1], "--chol") == 0) return chol_mode(argv[2]);
-    if (argc >= 3 && strcmp(argv[1], "--rf") == 0) return rf_mode(argv[2]);
+    if (argc >= 3 && strcmp(argv[1], "--rf") == 0) return rf_mode(argv[2], false);
+    if (argc >= 3 && strcmp(argv[1], "--rfc") == 0) return rf_mode(argv[2], true);
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf <dir>\n", argv[0]);
+        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] <dir>\n", argv[0]);
         return 2;
     }
     return SPH_Calculation(3, argv[1]);

@@ -1,6 +1,8 @@
 /* Test infrastructure only: CPU restatement of the correlation-matrix random fields of the
- * reference (nonstationary_random_2D / _3D, CPP_17-3-Nonstationary-RF.cpp:76-849), the checker
- * for the library's sphg_rf_* entry points.
+ * reference (nonstationary_random_2D / _3D, CPP_17-3-Nonstationary-RF.cpp:76-849, and the
+ * correlated pair of nonstationary_correlated_random_2D / _3D,
+ * CPP_17-5-Nonstationary-Correlated-RF.cpp:87-737), the checker for the library's sphg_rf_*
+ * entry points.
  *
  * Each function follows the reference code named in its comment, in the same operation order.
  * The standard normals zeta are an input: the reference draws them from a wall-clock seeded
@@ -185,4 +187,73 @@ void or_rf_values(int ndim, int dist, int field, double mean, double std_dev, do
     if (E_out) memcpy(E_out, E, sizeof(double) * (size_t)total);
     free(E);
     free(cv);
+}
+
+/* correlated friction angle and cohesion of one phase (nonstationary_correlated_random_2D/3D,
+ * CPP_17-5-Nonstationary-Correlated-RF.cpp:87-737): the mixing of the two normals per cell
+ * (:215-239, matrix_multi of :22-36), E = L zeta (:242), and the two properties (:244-395 2D,
+ * :574-725 3D). phase 0: soil (fai, c), trend below soil_max on the vertical axis; phase 1:
+ * fluid, normal branch below water_max and lognormal below soil_max, both on y. Both properties
+ * use dist; lognormal values are exp(v). */
+void or_rf_pair_values(int ndim, int phase, int dist, double coe, const double mean[2], const double sd[2],
+                       const double cdep[2], double soil_max, double water_max, const int nc[3], const double base[3],
+                       double cell_dx, int total, const double *L, const double *zeta_1, int n, const int *type,
+                       const int *cell_of, double fill, double *out_fai, double *out_c) {
+    double coefficient, cov[4], *zeta, *E, *cv[2];
+    int i, j, k, want = phase == 0 ? 2 : 1;
+    if (dist == 2) {
+        coefficient = log(1.0 + coe * (sd[0] / mean[0]) * (sd[1] / mean[1]));
+        coefficient = coefficient / sqrt(log(1.0 + sd[0] * sd[0] / mean[0] / mean[0]) *
+                                         log(1.0 + sd[1] * sd[1] / mean[1] / mean[1]));
+    } else {
+        coefficient = coe;
+    }
+    cov[0] = 1.0;
+    cov[1] = coefficient;
+    cov[2] = 0.0;
+    cov[3] = sqrt(1.0 - coefficient * coefficient);
+    zeta = (double *)malloc(sizeof(double) * (size_t)total * 2);
+    E = (double *)malloc(sizeof(double) * (size_t)total * 2);
+    for (i = 0; i < total; i++)
+        for (j = 0; j < 2; j++) {
+            double temp = 0.0;
+            for (k = 0; k < 2; k++) temp = temp + zeta_1[i * 2 + k] * cov[k * 2 + j];
+            zeta[i * 2 + j] = temp;
+        }
+    for (i = 0; i < total; i++)
+        for (j = 0; j < 2; j++) {
+            double temp = 0.0;
+            for (k = 0; k < total; k++) temp = temp + L[(size_t)i * total + k] * zeta[k * 2 + j];
+            E[i * 2 + j] = temp;
+        }
+    for (k = 0; k < 2; k++) {
+        cv[k] = (double *)malloc(sizeof(double) * (size_t)total);
+        for (j = 0; j < total; j++) {
+            double c[3], y, v;
+            rf_cent(ndim, nc, base, cell_dx, j, c);
+            y = (phase == 0 && ndim == 3) ? c[2] : c[1];
+            if (dist == 1) {
+                double top = phase == 0 ? soil_max : water_max;
+                if (top > y) v = mean[k] + cdep[k] * (top - y) + sd[k] * E[j * 2 + k];
+                else v = mean[k] + sd[k] * E[j * 2 + k];
+            } else {
+                double temp_mean, std_temp;
+                if (soil_max > y) temp_mean = mean[k] + cdep[k] * (soil_max - y);
+                else temp_mean = mean[k];
+                std_temp = sqrt(log(1.0 + (sd[k] / temp_mean) * (sd[k] / temp_mean)));
+                temp_mean = log(temp_mean) - 0.5 * std_temp * std_temp;
+                v = exp(temp_mean + std_temp * E[j * 2 + k]);
+            }
+            cv[k][j] = v;
+        }
+    }
+    for (i = 0; i < n; i++) {
+        int ok = cell_of[i] >= 0 && type[i] == want;
+        out_fai[i] = ok ? cv[0][cell_of[i]] : fill;
+        out_c[i] = ok ? cv[1][cell_of[i]] : fill;
+    }
+    free(cv[0]);
+    free(cv[1]);
+    free(zeta);
+    free(E);
 }

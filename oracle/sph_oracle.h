@@ -67,4 +67,9 @@ void or_rf_values(int ndim, int dist, int field, double mean, double std_dev, do
                   const double *zeta, int n, const int *type, const int *cell_of, double fill, double *E_out,
                   double *value);
 
+void or_rf_pair_values(int ndim, int phase, int dist, double coe, const double mean[2], const double sd[2],
+                       const double cdep[2], double soil_max, double water_max, const int nc[3], const double base[3],
+                       double cell_dx, int total, const double *L, const double *zeta_1, int n, const int *type,
+                       const int *cell_of, double fill, double *out_fai, double *out_c);
+
 #endif

@@ -6,6 +6,8 @@
 - rf_pipeline.npz: the reference's nonstationary_random_2D/3D (`ref_harness --rf`) on seeded
   decks with sd_value 0, where its clock-seeded normals drop out: the per-particle properties
   and return codes it leaves.
+- rf_pair.npz: the same for its correlated friction angle and cohesion
+  (nonstationary_correlated_random_2D/3D, `ref_harness --rfc`), normal distribution.
 
     python tests/golden/make_rf.py
 """
@@ -64,6 +66,23 @@ PIPE_CASES = {
     "slope_too_many_cells": ("slope", 1, 1, 1.0, 0.0, 0.2),        # error 41
     "slope_crowded_cell": ("slope", 1, 1, 1.0, 0.0, 40.0),          # error 42
 }
+
+
+# correlated pairs at sd = 0, normal: (deck, phase field of the friction angle, means, coe_depths, cell_ndx, coe)
+PAIR_CASES = {
+    "slope_soil_pair": ("slope", 0, (25.0, 9000.0), (1.5, 2500.0), 4.0, 0.4),
+    "box2d_fluid_pair": ("box2d", 4, (3.0, 0.5), (80.0, 20.0), 2.0, -0.3),
+    "box3d_soil_pair": ("box3d", 0, (30.0, 700.0), (300.0, 9000.0), 1.0, 0.6),
+    "box3d_fluid_pair": ("box3d", 4, (2.0, 0.7), (150.0, 40.0), 1.0, 0.2),
+}
+
+
+def pair_inputs(name):
+    dk, ffai, means, cdeps, cndx, coe = PAIR_CASES[name]
+    d = rf_decks()[dk]
+    fa = rf_field(d, 1, means[0], 0.0, (1.0, 0.5, 0.7), field=ffai, acf=1, cell_ndx=cndx, coe_depth=cdeps[0])
+    fc = rf_field(d, 1, means[1], 0.0, (1.0, 0.5, 0.7), field=ffai + 1, acf=1, cell_ndx=cndx, coe_depth=cdeps[1])
+    return d, fa, fc, coe
 
 
 def chol_matrix(name):
@@ -134,6 +153,26 @@ def main() -> None:
         out[f"{name}/rc"] = np.array([rc], np.int32)
         print(name, "rc", rc)
     np.savez_compressed(os.path.join(HERE, "rf_pipeline.npz"), **out)
+    out = {}
+    for name in PAIR_CASES:
+        d, fa, fc, coe = pair_inputs(name)
+        _, _, ph = d.domain()
+        flags = [0] * 6
+        flags[fa.field] = flags[fc.field] = 1
+        psv = [(1, 0.0, 0.0, (1.0, 1.0, 1.0), 0.0)] * 6
+        psv[fa.field] = (1, fa.mean_value, 0.0, tuple(fa.rl), fa.coe_depth)
+        psv[fc.field] = (1, fc.mean_value, 0.0, tuple(fc.rl), fc.coe_depth)
+        rng = np.random.default_rng(6)
+        init = [rng.uniform(1.0, 2.0, d.n) for _ in range(6)]
+        vals, rc = run_ref_rf(d.ndim, d.xp, d.type, d.dr, list(fa.box_nb), ph["soil"][1], ph["water"][1], flags,
+                              fa.cell_ndx, 1, 0.0, 0.0, psv, init, correlated=True, coe=coe)
+        out[f"{name}/init_fai"] = init[fa.field]
+        out[f"{name}/init_c"] = init[fc.field]
+        out[f"{name}/fai"] = vals[fa.field]
+        out[f"{name}/c"] = vals[fc.field]
+        out[f"{name}/rc"] = np.array([rc], np.int32)
+        print(name, "rc", rc)
+    np.savez_compressed(os.path.join(HERE, "rf_pair.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -257,10 +257,12 @@ def run_ref_chol(A):
 
 
 def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx, flag_auto, beta, beta_1, psv,
-               init):
-    """The reference's own nonstationary_random_2D/3D through the link harness. psv: six tuples
-    (type, mean, sd, rl[3], coe_depth); init: six n-arrays (soil fai c cop ds, fluid fai c).
-    Returns (six n-arrays after, rc). Reproducible only where the flagged fields have sd 0."""
+               init, correlated=False, coe=0.0):
+    """The reference's own nonstationary_random_2D/3D (or, correlated, its
+    nonstationary_correlated_random_2D/3D with coe_frict_cohesion = coe) through the link harness.
+    psv: six tuples (type, mean, sd, rl[3], coe_depth); init: six n-arrays (soil fai c cop ds,
+    fluid fai c). Returns (six n-arrays after, rc). Reproducible only where the flagged fields
+    have sd 0."""
     import struct
     wd = tempfile.mkdtemp(prefix="sphrf_")
     try:
@@ -276,8 +278,10 @@ def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx,
             f.write(np.ascontiguousarray(xp, "<f8").tobytes() + np.ascontiguousarray(ty, "<i4").tobytes())
             for a in init:
                 f.write(np.ascontiguousarray(a, "<f8").tobytes())
+            if correlated:
+                f.write(struct.pack("<d", coe))
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        r = subprocess.run([REF_HARNESS, "--rf", wd], capture_output=True, env=env)
+        r = subprocess.run([REF_HARNESS, "--rfc" if correlated else "--rf", wd], capture_output=True, env=env)
         if r.returncode != 0:
             raise RuntimeError(f"reference harness --rf failed rc={r.returncode}")
         raw = open(os.path.join(wd, "rf_out.bin"), "rb").read()
@@ -286,3 +290,29 @@ def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx,
         return vals, rc
     finally:
         shutil.rmtree(wd, ignore_errors=True)
+
+
+def oracle_rf_pair(ndim, xp, ty, f_fai, f_c, coe, zeta1, fill=0.0):
+    """Oracle correlated pair (or_rf_pair_values) with the geometry of f_fai: returns rc, value_fai,
+    value_c. zeta1 is (cell_total, 2)."""
+    L = oracle_lib().lib
+    dp = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    ip = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    rc, _, cells, Lm, _ = oracle_rf(ndim, xp, ty, f_fai, np.zeros(15000), want_l=True)
+    if rc:
+        return rc, None, None
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    ty = np.ascontiguousarray(ty, np.int32)
+    n = xp.shape[0]
+    phase = 0 if f_fai.field == 0 else 1
+    D = C.c_double
+    D2 = C.c_double * 2
+    zeta1 = np.ascontiguousarray(zeta1, dtype=np.float64)
+    va, vc = np.zeros(n), np.zeros(n)
+    nc = np.ascontiguousarray(cells["nc"], np.int32)
+    base = np.ascontiguousarray(cells["base"], np.float64)
+    L.or_rf_pair_values(ndim, phase, f_fai.dist, D(coe), D2(f_fai.mean_value, f_c.mean_value),
+                        D2(f_fai.sd_value, f_c.sd_value), D2(f_fai.coe_depth, f_c.coe_depth), D(f_fai.soil_max),
+                        D(f_fai.water_max), ip(nc), dp(base), D(cells["cell_dx"]), cells["total"], dp(Lm), dp(zeta1),
+                        n, ip(ty), ip(cells["cell_of"]), D(fill), dp(va), dp(vc))
+    return 0, va, vc

@@ -85,3 +85,43 @@ def test_values_match_oracle_with_spread(name):
     # each realisation of the batch is what it is alone
     rc1, v1, _, _ = rf_generate(f, d.xp, d.type, zeta[2, :total], fill=-7.0)
     assert rc1 == 0 and np.array_equal(v1[0], v[2])
+
+
+# ------------------------------------------------------------ correlated pair (flspv 4)
+from golden.make_rf import PAIR_CASES, pair_inputs  # noqa: E402
+from helpers import oracle_rf_pair  # noqa: E402
+from sphgpy.capi import rf_generate_pair, rf_cells  # noqa: E402
+
+
+@pytest.mark.parametrize("dist", [1, 2])
+@pytest.mark.parametrize("name", sorted(PAIR_CASES))
+def test_pair_matches_oracle(name, dist):
+    d, fa, fc, coe = pair_inputs(name)
+    for f in (fa, fc):
+        f.dist = dist
+        f.sd_value = 0.25 * f.mean_value
+    g = rf_field(d, 1, 1.0, 0.0, (1.0, 0.5, 0.7), field=fa.field, cell_ndx=fa.cell_ndx)
+    assert rf_cells(g, d.xp, d.type)[0] == 0
+    z1 = np.random.default_rng(11).standard_normal((2, g.cell_total, 2))
+    rc, va, vc = rf_generate_pair(fa, fc, coe, d.xp, d.type, z1, fill=-5.0)
+    assert rc == 0
+    for r in range(2):
+        rco, oa, oc = oracle_rf_pair(d.ndim, d.xp, d.type, fa, fc, coe, z1[r], fill=-5.0)
+        assert rco == 0
+        assert max_rel(va[r], oa) <= 1e-9 and max_rel(vc[r], oc) <= 1e-9, (max_rel(va[r], oa), max_rel(vc[r], oc))
+
+
+def test_pair_without_correlation_is_two_single_fields():
+    # coefficient 0, normal: each property is the single-field generation of its own normals
+    d, fa, fc, _ = pair_inputs("slope_soil_pair")
+    for f in (fa, fc):
+        f.sd_value = 0.3 * f.mean_value
+    g = rf_field(d, 1, 1.0, 0.0, (1.0, 0.5), field=0, cell_ndx=fa.cell_ndx)
+    assert rf_cells(g, d.xp, d.type)[0] == 0
+    z1 = np.random.default_rng(12).standard_normal((g.cell_total, 2))
+    rc, va, vc = rf_generate_pair(fa, fc, 0.0, d.xp, d.type, z1)
+    assert rc == 0
+    rc1, a1, _, _ = rf_generate(fa, d.xp, d.type, np.ascontiguousarray(z1[:, 0]))
+    rc2, c1, _, _ = rf_generate(fc, d.xp, d.type, np.ascontiguousarray(z1[:, 1]))
+    assert rc1 == 0 and rc2 == 0
+    assert np.array_equal(va[0], a1[0]) and np.array_equal(vc[0], c1[0])

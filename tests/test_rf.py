@@ -152,3 +152,39 @@ def test_value_maps_with_spread():
             st = np.sqrt(np.log(1.0 + (10.0 / trend) ** 2))
             assert np.abs(v0[m] * np.exp(st * st / 2) / trend - 1.0).max() < 1e-12
             assert np.abs(np.log(v[m]) - np.log(v0[m]) - st * e).max() < 1e-12
+
+
+# ------------------------------------------------------------ correlated pair (flspv 4)
+from golden.make_rf import PAIR_CASES, pair_inputs  # noqa: E402
+from helpers import oracle_rf_pair  # noqa: E402
+
+
+@pytest.mark.parametrize("name", sorted(PAIR_CASES))
+def test_oracle_pair_is_the_references_at_sd_zero(name):
+    # nonstationary_correlated_random_2D/3D with sd 0 (normal): the mixing and the normals drop
+    # out, the cell partition, the trend axes and tops (the fluid pair on y in 3D too) remain
+    z = np.load(os.path.join(GOLDEN, "rf_pair.npz"), allow_pickle=False)
+    d, fa, fc, coe = pair_inputs(name)
+    assert int(z[f"{name}/rc"][0]) == 0
+    rc, va, vc = oracle_rf_pair(d.ndim, d.xp, d.type, fa, fc, coe,
+                                np.random.default_rng(2).standard_normal((15000, 2)), fill=np.nan)
+    assert rc == 0
+    for got, init, want in ((va, "init_fai", "fai"), (vc, "init_c", "c")):
+        got = np.where(np.isnan(got), z[f"{name}/{init}"], got)
+        assert np.array_equal(_bits(got), _bits(z[f"{name}/{want}"])), want
+        assert not np.array_equal(got, z[f"{name}/{init}"])
+
+
+def test_oracle_pair_mixing():
+    # coefficient 1: the cohesion sees the friction angle's normals (equal values for equal
+    # parameters); coefficient 0: the cohesion sees the second normals alone
+    d, fa, fc, _ = pair_inputs("slope_soil_pair")
+    for f in (fa, fc):
+        f.sd_value, f.mean_value, f.coe_depth = 3.0, 20.0, 1.0
+    rc, _, cells, _, _ = oracle_rf(2, d.xp, d.type, fa, np.zeros(1), cells_only=True)
+    z1 = np.random.default_rng(4).standard_normal((cells["total"], 2))
+    _, va, vc = oracle_rf_pair(2, d.xp, d.type, fa, fc, 1.0, z1)
+    assert np.array_equal(va, vc)
+    _, va0, vc0 = oracle_rf_pair(2, d.xp, d.type, fa, fc, 0.0, z1)
+    _, v1, _ = oracle_rf_pair(2, d.xp, d.type, fa, fc, 0.0, z1[:, ::-1].copy())
+    assert np.array_equal(vc0, v1)
