@@ -2,13 +2,16 @@
 """Benchmark: particle-steps/s of the per-step SPH particle loop on one MI355X.
 
 Default workload: 3D saturated soil-water column, 64x32x64 soil lattice under a 64x32x16 water
-layer at dx = 0.005 m plus 3-layer walls and floor (172,032 particles), Drucker-Prager soil,
+layer at dx = 0.005 m plus 3-layer walls and floor (223,228 particles), Drucker-Prager soil,
 Darcy drag, plain FP64. Synthetic seeded lattice (no external data).
 
 One step = one whole reference time step (NNPS + all pair passes + constitutive + update) over
-all particles, state resident in HBM before the timed region starts. The timed region launches
-the step kernels eagerly with a HIP event pair around every kernel on the library's stream, so
-per-kernel durations are measured live; `value` is particle-steps/s over exactly --steps steps.
+all particles, state resident in HBM before the timed region starts. The neighbour batch widths
+are autotuned for the deck first (sphg_autotune; the state is restored afterwards). `value` is
+particle-steps/s over exactly --steps steps of sphg_step, which replays the step as a hipGraph.
+A second timed pass of the same length launches the step kernels eagerly with a HIP event pair
+around every kernel on the library's stream: the per-kernel durations (kernels, roofline) are
+measured live there, and its rate is reported as ms_per_step_eager_with_events.
 
 Extra fields on the JSON line:
   roofline      dominant kernel: algorithmic bytes per launch / measured average duration vs
@@ -16,8 +19,9 @@ Extra fields on the JSON line:
                 (profiles/pmc_<workload>.json) when present, else null.
   mixed         the same workload in the mixed arithmetic form (SPHG_MIXED: FP32 kernel values and
                 hardware reciprocals in the pair terms, FP64 sums and state), same warm-up and
-                steps from the same initial state: its rate, speed-up over FP64 and field-max
-                relative error against the FP64 context after the same number of steps.
+                steps from the same initial state, same two passes: its graph-replay rate,
+                speed-up over FP64 and field-max relative error against the FP64 context after the
+                same number of steps.
   cpu_baseline  the reference solver itself (compiled from its sources into oracle/_ref) timed on
                 this host's cores on a bounded sample of the same deck; where that build is absent,
                 the FP64 oracle (a restatement of the reference loop, validated bit-exact against it).
@@ -265,9 +269,9 @@ def main():
     ap.add_argument("--workload", default="soilwater3d", choices=sorted(WORKLOAD_DESC))
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="time the hipGraph replay path (no per-kernel events)")
     ap.add_argument("--nreal", type=int, default=256, help="realisations of the slope_batch workload")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default neighbour batch widths")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
@@ -323,6 +327,13 @@ def main():
             g.load_state(state)
             g.upload_done()
         g.set_step(1)
+        if not args.no_tune:
+            # neighbour batch widths of this deck (the state is restored after the trials)
+            rc, widths = g.autotune(5)
+            if rc:
+                raise SystemExit(f"autotune failed with error {rc}")
+            extra["batch_widths"] = {"density": widths[0], "norm_eos": widths[1], "strain": widths[2],
+                                     "xsph": widths[3]}
         return g, deck, state, extra
 
     def timed(g, per_kernel):
@@ -339,14 +350,18 @@ def main():
         return t1 - t0
 
     g, deck, state, extra = build(SPHG_FP64)
-    el = timed(g, not args.graph)
+    # value: the step as sphg_step runs it (hipGraph replay); the per-kernel durations of the
+    # roofline come from a second timed pass of the same length with eager launches bracketed by
+    # HIP events on the context's stream
+    el = timed(g, False)
+    el_events = timed(g, True)
     cnt, _ = g.neighbors()
     nbar = float(cnt.mean())
     n = g.n
     value = n * args.steps / el
     ms_per_step = el * 1e3 / args.steps
 
-    rows = g.kernel_times() if not args.graph else []
+    rows = g.kernel_times()
     roof = None
     kernels = {}
     if rows:
@@ -375,17 +390,19 @@ def main():
     if not args.no_mixed:
         # the same workload in the mixed form, same warm-up and step count from the same initial
         # state; its error is taken against the FP64 context after the same number of steps
-        gm, *_ = build(SPHG_MIXED)
-        elm = timed(gm, not args.graph)
+        gm, _, _, mextra = build(SPHG_MIXED)
+        elm = timed(gm, False)
+        timed(gm, True)
         err = {}
         for k in ("xp", "vx", "sig", "rho"):
             a, b = gm.get(k), g.get(k)
             err[k] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
-        mrows = gm.kernel_times() if not args.graph else []
+        mrows = gm.kernel_times()
         mixed = {"value": round(n * args.steps / elm, 1), "ms_per_step": round(elm * 1e3 / args.steps, 4),
                  "speedup_vs_fp64": round(el / elm, 3),
                  "max_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()},
-                 "err_horizon_steps": args.warmup + args.steps,
+                 "err_horizon_steps": 2 * (args.warmup + args.steps),
+                 "batch_widths": mextra.get("batch_widths"),
                  "kernels": {name: {"avg_us": ms * 1e3 / la, "launches": la} for name, ms, la in mrows}}
         gm.close()
 
@@ -404,7 +421,9 @@ def main():
         "data": "synthetic seeded lattice deck",
         "config": dict({"workload": args.workload, "description": WORKLOAD_DESC[args.workload], "particles": n,
                         "precision": "FP64", "mean_neighbours": round(nbar, 2),
-                        "timing": "graph replay" if args.graph else "eager launches with per-kernel HIP events"},
+                        "timing": "value: hipGraph replay of the step; kernels and roofline: a second timed pass "
+                                  "of the same length, eager launches with per-kernel HIP events",
+                        "ms_per_step_eager_with_events": round(el_events * 1e3 / args.steps, 4)},
                        **extra),
         "roofline": roof,
         "pair_interactions_per_s": round(float(cnt.sum()) * args.steps / el, 1),

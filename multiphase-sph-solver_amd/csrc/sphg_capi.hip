@@ -59,6 +59,9 @@ struct sphg_ctx {
     bool regu_on = false;  // regularisation passes are in the launch sequence
     bool pre_on = false;   // artificial-stress / geostress-damping pair pass is in the launch sequence
     bool full_models = false;  // some soil particle uses Cam-Clay, unsaturated Cam-Clay or FOCC
+    // neighbours per load round trip of the density, normalisation, strain and XSPH passes
+    // (sphg_autotune); each has the widths kBwChoices[k]
+    int bw[4] = {2, 6, 2, 2};
     std::vector<Grid> grids;   // one per realisation (host copy)
     std::vector<int> counts;   // particles per realisation (batch contexts)
     Grid *d_grids = nullptr;
@@ -682,6 +685,17 @@ static void tend(sphg_ctx *c, Timing &t) {
         tend(c, t_);                                                           \
     } while (0)
 
+// widths each tunable pass is built for (index 0 density, 1 normalisation, 2 strain, 3 XSPH)
+static const int kBwChoices[4][3] = {{1, 2, 4}, {2, 4, 6}, {1, 2, 4}, {1, 2, 4}};
+
+// launch of a pass instantiated for the context's width
+#define LAUNCH_BW(name, k, slot, W0, W1, W2, grid, ...)                                      \
+    do {                                                                                    \
+        if (c->bw[slot] == W0) LAUNCH(name, (k<NDIM, M, W0>), grid, __VA_ARGS__);           \
+        else if (c->bw[slot] == W1) LAUNCH(name, (k<NDIM, M, W1>), grid, __VA_ARGS__);      \
+        else LAUNCH(name, (k<NDIM, M, W2>), grid, __VA_ARGS__);                             \
+    } while (0)
+
 template <int NDIM>
 static void enqueue_nnps(sphg_ctx *c) {
     const int cur = c->cur, n = c->n, M = c->P.ncell + 2;
@@ -706,12 +720,12 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
     const int cur = c->cur, n = c->n;
     if (phase_mask & 1) enqueue_nnps<NDIM>(c);
     if (phase_mask & 2) {
-        LAUNCH("density", (k_density<NDIM, M>), grid_for(n), c->S, c->P, cur);
-        LAUNCH("norm_eos", (k_norm<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        LAUNCH_BW("density", k_density, 0, 1, 2, 4, grid_for(n), c->S, c->P, cur);
+        LAUNCH_BW("norm_eos", k_norm, 1, 2, 4, 6, grid_for(n), c->S, c->P, cur);
     }
     if (phase_mask & 4) {
         if (c->P.flagum == 1) LAUNCH("saturation", (k_saturation<NDIM, M>), grid_for(n), c->S, c->P, cur);
-        LAUNCH("strain", (k_strain<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        LAUNCH_BW("strain", k_strain, 2, 1, 2, 4, grid_for(n), c->S, c->P, cur);
         if (c->regu_on) {
             LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 0);
             LAUNCH("regu", (k_regu<NDIM, M>), grid_for(n), c->S, c->P, cur, 0);
@@ -740,7 +754,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
             LAUNCH("accel_tail", k_accel_tail<NDIM>, grid_for(n), c->S, c->P, cur);
     }
     if (phase_mask & 32) {
-        LAUNCH("xsph", (k_xsph<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        LAUNCH_BW("xsph", k_xsph, 3, 1, 2, 4, grid_for(n), c->S, c->P, cur);
         if (tran_bndy(c->P)) LAUNCH("slip_tran", (k_slip_tran<NDIM, M>), grid_for(n), c->S, c->P, cur);
     }
     if (phase_mask & 64) {
@@ -1058,6 +1072,78 @@ extern "C" int sphg_enable_timing(sphg_ctx *c, int on) {
     if (!c) return SPHG_E_ARG;
     c->timing = on != 0;
     c->tsum.clear();
+    return SPHG_OK;
+}
+
+// Autotuner of the neighbour batch widths: each of the three width sets is timed over `steps`
+// steps from the context's current state (per-pass HIP events), the state is restored after each
+// trial, and every pass keeps its fastest width. The widths change launch shapes only: every
+// width gives the same results bit for bit (the summation order is the list order).
+extern "C" int sphg_autotune(sphg_ctx *c, int steps, int *chosen) {
+    if (!c || !c->uploaded || steps <= 0 || c->timing) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // the whole device state but the neighbour lists (the next step's search rebuilds them)
+    const size_t skip_off = (size_t)((char *)c->S.nbr - c->pool), skip_len = (size_t)c->cap * kMaxNb * 4;
+    const size_t tail = c->pool_bytes - skip_off - skip_len;
+    char *bak = nullptr;
+    HIPCHK(hipMalloc(&bak, skip_off + tail));
+    auto copy_state = [&](bool save) -> hipError_t {
+        char *a = save ? bak : c->pool, *b = save ? c->pool : bak;
+        const size_t a2 = save ? skip_off : skip_off + skip_len, b2 = save ? skip_off + skip_len : skip_off;
+        hipError_t e = hipMemcpyAsync(a, b, skip_off, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(a + a2, b + b2, tail, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        return e;
+    };
+    const int step0 = c->step, cur0 = c->cur;
+    const bool stepped0 = c->stepped;
+    const int saved_bw[4] = {c->bw[0], c->bw[1], c->bw[2], c->bw[3]};
+    static const char *kNames[4] = {"density", "norm_eos", "strain", "xsph"};
+    double best[4] = {1e300, 1e300, 1e300, 1e300};
+    int pick[4] = {saved_bw[0], saved_bw[1], saved_bw[2], saved_bw[3]};
+    hipError_t e = copy_state(true);
+    int rc = e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
+    for (int t = 0; t < 3 && rc == SPHG_OK; t++) {
+        for (int k = 0; k < 4; k++) c->bw[k] = kBwChoices[k][t];
+        sphg_enable_timing(c, 1);
+        rc = sphg_step(c, 1);  // first launches of these instantiations, untimed
+        sphg_enable_timing(c, 1);
+        if (rc == SPHG_OK) rc = sphg_step(c, steps);
+        finish_timing(c);
+        for (auto &r : c->tsum)
+            for (int k = 0; k < 4; k++)
+                if (r.first == kNames[k] && r.second.second > 0) {
+                    const double avg = r.second.first / r.second.second;
+                    if (avg < best[k]) {
+                        best[k] = avg;
+                        pick[k] = kBwChoices[k][t];
+                    }
+                }
+        sphg_enable_timing(c, 0);
+        // back to the state before the trial
+        e = copy_state(false);
+        if (e != hipSuccess && rc == SPHG_OK) rc = SPHG_E_HIP;
+        c->step = step0;
+        c->cur = cur0;
+        c->stepped = stepped0;
+        if (hipMemcpy(c->S.step, &step0, 4, hipMemcpyHostToDevice) != hipSuccess && rc == SPHG_OK) rc = SPHG_E_HIP;
+    }
+    hipFree(bak);
+    for (int k = 0; k < 4; k++) c->bw[k] = rc == SPHG_OK ? pick[k] : saved_bw[k];
+    drop_graphs(c);
+    if (chosen)
+        for (int k = 0; k < 4; k++) chosen[k] = c->bw[k];
+    return rc;
+}
+
+extern "C" int sphg_set_batch_widths(sphg_ctx *c, const int *widths) {
+    if (!c || !widths) return SPHG_E_ARG;
+    for (int k = 0; k < 4; k++) {
+        const int *ch = kBwChoices[k];
+        if (widths[k] != ch[0] && widths[k] != ch[1] && widths[k] != ch[2]) return SPHG_E_ARG;
+    }
+    for (int k = 0; k < 4; k++) c->bw[k] = widths[k];
+    drop_graphs(c);
     return SPHG_OK;
 }
 

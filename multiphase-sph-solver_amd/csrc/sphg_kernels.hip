@@ -98,7 +98,10 @@ struct NbCursor {
 // first issues the gathers of all B neighbours and only then consumes them in list order: one
 // memory round trip serves B neighbours, and the summation order is the list order as before.
 // The ids of the next batch are loaded while a batch is processed. Slots past the end of the
-// list repeat the last entry (valid addresses) and are reported as inactive (b >= act).
+// list repeat the last entry (valid addresses) and are reported as inactive (b >= act). The
+// density, normalisation, strain and XSPH passes take B as a template argument (BW): wider
+// batches hide more latency but hold more registers, and the best width depends on the deck
+// (neighbour count, dimension), so each context picks its own (sphg_autotune).
 template <int B, class F>
 __device__ __forceinline__ void nb_batches(const State &S, const Params &P, int i, int nc, F &&f) {
     if (nc <= 0) return;
@@ -482,7 +485,7 @@ __device__ __forceinline__ void nb_vel(const State &S, int j, bool bnd, int mode
 // density loops 1-2 (CPP_6-density.cpp:22-166); water porosity (loop 1) and continuity (loop 2)
 // accumulate side by side in one walk over the list.
 // ------------------------------------------------------------------------------------------
-template <int NDIM, class M>
+template <int NDIM, class M, int BW>
 __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -506,7 +509,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     const bool solid = (ti == 2 || ti == 4), water = ti == 1;
     const int mode = bvel_mode(P, ti), ph = ti - 1;
     double up[3] = {0.0, 0.0, 0.0}, dn[3] = {0.0, 0.0, 0.0}, pup = 0.0, pdn = 0.0;
-    constexpr int B = 2;
+    constexpr int B = BW;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xj[B][3], vj[B][3], hlj[B], mj[B], porj[B], rj[B];
 #pragma unroll
@@ -574,7 +577,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
 }
 
 // density loop 3 (Jacobi form of CPP_6-density.cpp:168-188) and eos loop 1 (CPP_7-1-eos_CR.cpp:20-43)
-template <int NDIM, class M>
+template <int NDIM, class M, int BW>
 __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
     const int nc = nb_count(S, i);
     double up = 0.0, dn = 0.0;
-    constexpr int B = 6;
+    constexpr int B = BW;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xj[B][3], hlj[B], mj[B], rmj[B];
 #pragma unroll
@@ -685,7 +688,7 @@ __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) 
     }
 }
 
-template <int NDIM, class M>
+template <int NDIM, class M, int BW>
 __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -749,7 +752,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
             }
         });
     } else if (ti != 7) {
-        constexpr int B = 2;
+        constexpr int B = BW;
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
             double xj[B][3], vj[B][3], hlj[B], mj[B], rj[B], pj[B];
 #pragma unroll
@@ -2162,7 +2165,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 // XSPH + velocity_set_domain + tm_boundary first loop
 // ------------------------------------------------------------------------------------------
-template <int NDIM, class M>
+template <int NDIM, class M, int BW>
 __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
@@ -2192,7 +2195,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     int cur_id[3] = {-1, -1, -1};
     double d0[3] = {0, 0, 0};
     int total = 0, idm = 0;
-    constexpr int B = 2;
+    constexpr int B = BW;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xj[B][3], hlj[B], mj[B], rj[B], vxj[B][3];
         int mtj[B];
@@ -2565,26 +2568,58 @@ template __global__ void k_bin<2>(State, Params, int);
 template __global__ void k_bin<3>(State, Params, int);
 template __global__ void k_search<2>(State, Params, int);
 template __global__ void k_search<3>(State, Params, int);
-template __global__ void k_density<2, Exact>(State, Params, int);
-template __global__ void k_density<2, Fast>(State, Params, int);
-template __global__ void k_density<3, Exact>(State, Params, int);
-template __global__ void k_density<3, Fast>(State, Params, int);
-template __global__ void k_norm<2, Exact>(State, Params, int);
-template __global__ void k_norm<2, Fast>(State, Params, int);
-template __global__ void k_norm<3, Exact>(State, Params, int);
-template __global__ void k_norm<3, Fast>(State, Params, int);
-template __global__ void k_strain<2, Exact>(State, Params, int);
-template __global__ void k_strain<2, Fast>(State, Params, int);
-template __global__ void k_strain<3, Exact>(State, Params, int);
-template __global__ void k_strain<3, Fast>(State, Params, int);
+template __global__ void k_density<2, Exact, 1>(State, Params, int);
+template __global__ void k_density<2, Exact, 2>(State, Params, int);
+template __global__ void k_density<2, Exact, 4>(State, Params, int);
+template __global__ void k_density<2, Fast, 1>(State, Params, int);
+template __global__ void k_density<2, Fast, 2>(State, Params, int);
+template __global__ void k_density<2, Fast, 4>(State, Params, int);
+template __global__ void k_density<3, Exact, 1>(State, Params, int);
+template __global__ void k_density<3, Exact, 2>(State, Params, int);
+template __global__ void k_density<3, Exact, 4>(State, Params, int);
+template __global__ void k_density<3, Fast, 1>(State, Params, int);
+template __global__ void k_density<3, Fast, 2>(State, Params, int);
+template __global__ void k_density<3, Fast, 4>(State, Params, int);
+template __global__ void k_norm<2, Exact, 2>(State, Params, int);
+template __global__ void k_norm<2, Exact, 4>(State, Params, int);
+template __global__ void k_norm<2, Exact, 6>(State, Params, int);
+template __global__ void k_norm<2, Fast, 2>(State, Params, int);
+template __global__ void k_norm<2, Fast, 4>(State, Params, int);
+template __global__ void k_norm<2, Fast, 6>(State, Params, int);
+template __global__ void k_norm<3, Exact, 2>(State, Params, int);
+template __global__ void k_norm<3, Exact, 4>(State, Params, int);
+template __global__ void k_norm<3, Exact, 6>(State, Params, int);
+template __global__ void k_norm<3, Fast, 2>(State, Params, int);
+template __global__ void k_norm<3, Fast, 4>(State, Params, int);
+template __global__ void k_norm<3, Fast, 6>(State, Params, int);
+template __global__ void k_strain<2, Exact, 1>(State, Params, int);
+template __global__ void k_strain<2, Exact, 2>(State, Params, int);
+template __global__ void k_strain<2, Exact, 4>(State, Params, int);
+template __global__ void k_strain<2, Fast, 1>(State, Params, int);
+template __global__ void k_strain<2, Fast, 2>(State, Params, int);
+template __global__ void k_strain<2, Fast, 4>(State, Params, int);
+template __global__ void k_strain<3, Exact, 1>(State, Params, int);
+template __global__ void k_strain<3, Exact, 2>(State, Params, int);
+template __global__ void k_strain<3, Exact, 4>(State, Params, int);
+template __global__ void k_strain<3, Fast, 1>(State, Params, int);
+template __global__ void k_strain<3, Fast, 2>(State, Params, int);
+template __global__ void k_strain<3, Fast, 4>(State, Params, int);
 template __global__ void k_accel<2, Exact>(State, Params, int);
 template __global__ void k_accel<2, Fast>(State, Params, int);
 template __global__ void k_accel<3, Exact>(State, Params, int);
 template __global__ void k_accel<3, Fast>(State, Params, int);
-template __global__ void k_xsph<2, Exact>(State, Params, int);
-template __global__ void k_xsph<2, Fast>(State, Params, int);
-template __global__ void k_xsph<3, Exact>(State, Params, int);
-template __global__ void k_xsph<3, Fast>(State, Params, int);
+template __global__ void k_xsph<2, Exact, 1>(State, Params, int);
+template __global__ void k_xsph<2, Exact, 2>(State, Params, int);
+template __global__ void k_xsph<2, Exact, 4>(State, Params, int);
+template __global__ void k_xsph<2, Fast, 1>(State, Params, int);
+template __global__ void k_xsph<2, Fast, 2>(State, Params, int);
+template __global__ void k_xsph<2, Fast, 4>(State, Params, int);
+template __global__ void k_xsph<3, Exact, 1>(State, Params, int);
+template __global__ void k_xsph<3, Exact, 2>(State, Params, int);
+template __global__ void k_xsph<3, Exact, 4>(State, Params, int);
+template __global__ void k_xsph<3, Fast, 1>(State, Params, int);
+template __global__ void k_xsph<3, Fast, 2>(State, Params, int);
+template __global__ void k_xsph<3, Fast, 4>(State, Params, int);
 template __global__ void k_update<2, Exact>(State, Params, int);
 template __global__ void k_update<2, Fast>(State, Params, int);
 template __global__ void k_update<3, Exact>(State, Params, int);

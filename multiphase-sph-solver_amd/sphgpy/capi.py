@@ -400,6 +400,10 @@ def product_lib() -> _Lib:
         L.sphg_poll.restype = _I
         L.sphg_run.argtypes = [C.c_void_p, _I, _I, C.c_double, C.POINTER(C.c_double), OUTPUT_FN, C.c_void_p]
         L.sphg_run.restype = _I
+        L.sphg_autotune.argtypes = [C.c_void_p, _I, C.POINTER(C.c_int)]
+        L.sphg_autotune.restype = _I
+        L.sphg_set_batch_widths.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.sphg_set_batch_widths.restype = _I
         L.sphg_particle_count.argtypes = [C.c_void_p]
         L.sphg_particle_count.restype = _I
         L.sphg_write_odb_fields.argtypes = [C.c_char_p, _I, C.c_double, _I, C.POINTER(OdbFields)]
@@ -534,6 +538,16 @@ class GpuSim(Sim):
         if rc:
             raise RuntimeError(f"sphg_write_odb: error {rc}")
         return odb_path(dirname, l)
+
+    def autotune(self, steps: int = 5):
+        """sphg_autotune: returns (error code, widths of density, normalisation, strain, XSPH)."""
+        w = (C.c_int * 4)()
+        rc = self.L.lib.sphg_autotune(self.ctx, steps, w)
+        return int(rc), list(w)
+
+    def set_batch_widths(self, widths) -> int:
+        """sphg_set_batch_widths (density, normalisation, strain, XSPH)."""
+        return int(self.L.lib.sphg_set_batch_widths(self.ctx, (C.c_int * 4)(*widths)))
 
     def save_state(self, path: str) -> int:
         """sphg_save_state: checkpoint of the device state (returns the error code)."""
