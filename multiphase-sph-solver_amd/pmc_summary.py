@@ -30,8 +30,15 @@ def counters(d):
 
 
 def short(name):
+    """Pass name as the bench reports it; the neighbour batch width (third template argument of
+    the tunable passes) is kept as a suffix, e.g. strain/w2."""
     s = name.split("(")[0].replace("void ", "").replace("sphg::", "")
-    return s.split("<")[0].replace("k_", "")
+    base = s.split("<")[0].replace("k_", "")
+    if "<" in s:
+        args = [a.strip() for a in s[s.index("<") + 1:s.rindex(">")].split(",")]
+        if base in ("density", "norm", "strain", "xsph") and len(args) == 3:
+            return f"{base}/w{args[2]}"
+    return base
 
 
 def main(kt, fe, wr, cfe, cwr, workload, out_md, out_json):
@@ -51,7 +58,7 @@ def main(kt, fe, wr, cfe, cwr, workload, out_md, out_json):
         if "sphg" not in name:
             continue
         k = short(name)
-        kname = {"norm": "norm_eos"}.get(k, k)
+        kname = "norm_eos" + k[4:] if k.startswith("norm") else k
         fk = f.get(name, [0.0])
         wk = w.get(name, [0.0])
         fb = sum(fk) / len(fk) * 1024 * fscale
