@@ -130,8 +130,10 @@ int sphg_device_bytes(const sphg_ctx *ctx, size_t *bytes);
  * (Class_particle.h:13-103): x xp vx vxp ax axp ux strep divx interf interfss rho rhop mass hl
  * pre prep e eps deps sig dsig weps adps satu dsatu suct cop vsig veps divq divr meps roue rous
  * prew prea porosity cd poro_cop beta beta_ts type matype permtype etype; the per-particle
- * material copies pParti_ConsPara / pVFluid as "ps_<field>" / "pf_<field>" (only ps_fai, ps_c,
- * ps_cop, ps_ds differ between particles, CPP_17-1_K-L_expansion_non.cpp:575-600); and the
+ * material copies pParti_ConsPara / pVFluid as "ps_<field>" / "pf_<field>" (ps_fai, ps_c, ps_cop and
+ * ps_ds, the properties a random field varies, CPP_17-1_K-L_expansion_non.cpp:575-600, are kept
+ * per particle; every other copy must equal its material-table value, or sphg_upload_done returns
+ * SPHG_E_UNSUPPORTED); and the
  * boundary records clVar_Boundary vx_water vx_soil vx_air vx_struct (Class_particle.h:190-207)
  * and cl_bndy_pair vel_bndy (:168-177). Values are row-major [n][ncomp] (ncomp 1, 3, 6 or 9);
  * int fields (type matype permtype etype) use the *_i variants. sphg_upload_done() derives the
@@ -178,7 +180,8 @@ int sphg_strain_to_stress(sphg_ctx *ctx);
 /* clAcce_Fun fluid/soil/structure accelerations (CPP_9-4-derivation_stress.cpp:12-505),
  * artificial_viscosity (CPP_9-3-artificialvis.cpp:12-99), the artificial-stress pair sum
  * (CPP_9-5-artificialstress.cpp:92-140, :230-317), boundary stress effect
- * (CPP_9-4-derivation_stress.cpp:631-721), interact1 (CPP_10-interact1.cpp:15-110),
+ * (CPP_9-4-derivation_stress.cpp:631-721), interact1 / interact2 / interact3 by fli 1 / 2 / 3
+ * (CPP_10-interact1.cpp:15-110, :115-267, :270-420; CPP_1-SPHLoops.cpp:649-660),
  * soil_damping_geostress (CPP_9-4-derivation_stress.cpp:850-897) and veocity_update for
  * if_move 0-3 (CPP_11-xsph.cpp:11-142). */
 int sphg_accel(sphg_ctx *ctx);

@@ -8,7 +8,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -54,6 +56,11 @@ struct sphg_ctx {
     FluidSlot *d_fluid = nullptr;
     double *d_psc = nullptr;
     bool has_psc = false;
+    // ps_ds / ps_cop / ps_fai set by the caller: upload_done keeps them instead of the slot values
+    bool has_psds = false, has_pscop = false, has_psfai = false;
+    // other per-particle material copies the caller set (ps_e, pf_fai, ...): the step reads these
+    // from the material slots, so upload_done refuses them unless they equal the slot values
+    std::map<std::string, std::vector<double>> pinned_mat;
     int scan_blocks = 0;
     bool stepped = false;
     bool regu_on = false;  // regularisation passes are in the launch sequence
@@ -168,6 +175,8 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
                        &S.porosity, &S.cd, &S.poro_cop, &S.color})
         D(*f);
     plan.push_back({(void **)&c->d_psc, cap * 8});
+    D(S.ps_ds); D(S.ps_cop); D(S.ps_fai);
+    for (int k = 0; k < 3; k++) D(S.ps_dp[k]);
     I(S.type, cap); I(S.matype, cap); I(S.permtype, cap); I(S.etype, cap);
     plan.push_back({(void **)&S.soil_slot, cap});
     plan.push_back({(void **)&S.fluid_slot, cap});
@@ -330,7 +339,8 @@ static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, b
         {"e", S.e}, {"satu", S.satu}, {"dsatu", S.dsatu}, {"suct", S.suct}, {"cop", S.cop}, {"vsig", S.vsig},
         {"veps", S.veps}, {"divq", S.divq}, {"divr", S.divr}, {"meps", S.meps}, {"roue", S.roue},
         {"prew", S.prew}, {"prea", S.prea}, {"porosity", S.porosity}, {"cd", S.cd}, {"poro_cop", S.poro_cop},
-        {"color", S.color}, {"ps_c", c->d_psc}, {"rho_mid", S.rho_mid}};
+        {"color", S.color}, {"ps_c", c->d_psc}, {"ps_ds", S.ps_ds}, {"ps_cop", S.ps_cop},
+        {"ps_fai", S.ps_fai}, {"rho_mid", S.rho_mid}};
     for (auto &e : sc)
         if (nm == e.n) return s1(e.p);
     return -1;
@@ -397,6 +407,48 @@ extern "C" int sphg_get_field(sphg_ctx *c, const char *name, double *host, int n
     return SPHG_OK;
 }
 
+// Para_Soil / Para_Fluid members as per-particle copies "ps_<field>" / "pf_<field>"
+// (Class_ParticleProperty.h:10-133) and where each sits in sphg_soil / sphg_fluid
+#define SPHG_SOIL_MEMBER(m) {"ps_" #m, offsetof(sphg_soil, m)}
+#define SPHG_FLUID_MEMBER(m) {"pf_" #m, offsetof(sphg_fluid, m)}
+static const std::pair<const char *, size_t> kSoilMembers[] = {
+    SPHG_SOIL_MEMBER(e), SPHG_SOIL_MEMBER(poi), SPHG_SOIL_MEMBER(eps0), SPHG_SOIL_MEMBER(dens),
+    SPHG_SOIL_MEMBER(damp1), SPHG_SOIL_MEMBER(damp2), SPHG_SOIL_MEMBER(ramda), SPHG_SOIL_MEMBER(kapa),
+    SPHG_SOIL_MEMBER(zmf), SPHG_SOIL_MEMBER(ann), SPHG_SOIL_MEMBER(bnn), SPHG_SOIL_MEMBER(beta),
+    SPHG_SOIL_MEMBER(sitas), SPHG_SOIL_MEMBER(sitar), SPHG_SOIL_MEMBER(epsr), SPHG_SOIL_MEMBER(epss),
+    SPHG_SOIL_MEMBER(sd), SPHG_SOIL_MEMBER(sw), SPHG_SOIL_MEMBER(kse), SPHG_SOIL_MEMBER(c1),
+    SPHG_SOIL_MEMBER(c2), SPHG_SOIL_MEMBER(c3), SPHG_SOIL_MEMBER(ocr), SPHG_SOIL_MEMBER(sme0),
+    SPHG_SOIL_MEMBER(mzm), SPHG_SOIL_MEMBER(azm), SPHG_SOIL_MEMBER(r0), SPHG_SOIL_MEMBER(bl),
+    SPHG_SOIL_MEMBER(bz), SPHG_SOIL_MEMBER(zet), SPHG_SOIL_MEMBER(vpu), SPHG_SOIL_MEMBER(porosity),
+    SPHG_SOIL_MEMBER(fs), SPHG_SOIL_MEMBER(etao), SPHG_SOIL_MEMBER(g0), SPHG_SOIL_MEMBER(damp_coe)};
+static const std::pair<const char *, size_t> kFluidMembers[] = {
+    SPHG_FLUID_MEMBER(vis), SPHG_FLUID_MEMBER(vpu), SPHG_FLUID_MEMBER(dens), SPHG_FLUID_MEMBER(paa0),
+    SPHG_FLUID_MEMBER(porosity), SPHG_FLUID_MEMBER(fai), SPHG_FLUID_MEMBER(c), SPHG_FLUID_MEMBER(pre_coe),
+    SPHG_FLUID_MEMBER(strain_min)};
+#undef SPHG_SOIL_MEMBER
+#undef SPHG_FLUID_MEMBER
+
+// 0 when every pinned per-particle material value equals the value of the particle's slot
+static int check_pinned_materials(const sphg_ctx *c, const std::vector<sphg_soil> &soil_of_slot,
+                                  const std::vector<sphg_fluid> &fluid_of_slot, const std::vector<int8_t> &ss,
+                                  const std::vector<int8_t> &fs) {
+    for (const auto &e : c->pinned_mat) {
+        for (const auto &m : kSoilMembers)
+            if (e.first == m.first)
+                for (int i = 0; i < c->n; i++) {
+                    const double v = *(const double *)((const char *)&soil_of_slot[ss[i]] + m.second);
+                    if (memcmp(&v, &e.second[i], 8) != 0) return SPHG_E_UNSUPPORTED;
+                }
+        for (const auto &m : kFluidMembers)
+            if (e.first == m.first)
+                for (int i = 0; i < c->n; i++) {
+                    const double v = *(const double *)((const char *)&fluid_of_slot[fs[i]] + m.second);
+                    if (memcmp(&v, &e.second[i], 8) != 0) return SPHG_E_UNSUPPORTED;
+                }
+    }
+    return 0;
+}
+
 extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host, int n, int ncomp) {
     if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
     if (c->n == 0) c->n = n;
@@ -404,8 +456,18 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
     std::string nm(name);
     const size_t N = (size_t)n;
     if (nm == "x" || nm == "divx" || nm == "dsig" || nm == "beta_ts" || nm == "weps" || nm == "rous" ||
-        nm.rfind("pf_", 0) == 0 || (nm.rfind("ps_", 0) == 0 && nm != "ps_c"))
-        return SPHG_OK;  // derived, per-step temporary, or taken from the material slots
+        nm.rfind("pf_", 0) == 0 ||
+        (nm.rfind("ps_", 0) == 0 && nm != "ps_c" && nm != "ps_ds" && nm != "ps_cop" && nm != "ps_fai")) {
+        // derived, per-step temporary, or taken from the material slots
+        bool member = false;
+        for (const auto &m : kSoilMembers) member = member || nm == m.first;
+        for (const auto &m : kFluidMembers) member = member || nm == m.first;
+        if (member) {
+            if (ncomp != 1) return SPHG_E_ARG;
+            c->pinned_mat[nm].assign(host, host + n);
+        }
+        return SPHG_OK;
+    }
     double *ptr[9];
     int nc = 0;
     std::vector<double *> mirror;
@@ -417,6 +479,9 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
         if (!mirror.empty()) HIPCHK(hipMemcpy(mirror[k], buf.data(), N * 8, hipMemcpyHostToDevice));
     }
     if (nm == "ps_c") c->has_psc = true;
+    if (nm == "ps_ds") c->has_psds = true;
+    if (nm == "ps_cop") c->has_pscop = true;
+    if (nm == "ps_fai") c->has_psfai = true;
     return SPHG_OK;
 }
 
@@ -467,7 +532,7 @@ static int fluid_slot_of(int type, int matype, int etype) {
 static int check_supported(const sphg_problem &p) {
     if (p.flknl > 2) return SPHG_E_UNSUPPORTED;  // kernel_gradient_correction
     if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
-    if (p.fli > 1 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
+    if (p.fli < 0 || p.fli > 3 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
     if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 5 && p.flbndy != 13 &&
         p.flbndy != 14)
         return SPHG_E_UNSUPPORTED;
@@ -588,32 +653,32 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     }
     // material slots with host-evaluated transcendental constants
     std::vector<SoilSlot> soil(kNSlots);
+    std::vector<sphg_soil> soil_src(kNSlots);
     for (int s = 0; s < kNSlots; s++) {
         sphg_soil d{};
         if (s < SPHG_NSOIL) d = c->mat.soil[s];
         else { d.fs = 1.0; d.ds = 1.0; }
+        soil_src[s] = d;
         SoilSlot &o = soil[s];
         o.e = d.e; o.poi = d.poi; o.eps0 = d.eps0; o.dens = d.dens; o.kapa = d.kapa; o.zmf = d.zmf; o.ann = d.ann;
         o.ramda = d.ramda; o.fs = d.fs; o.c = d.c; o.sme0 = d.sme0; o.porosity = d.porosity; o.vpu = d.vpu;
-        o.cop = d.cop; o.sitas = d.sitas; o.sitar = d.sitar; o.damp_coe = d.damp_coe; o.beta = d.beta;
+        o.cop = d.cop; o.sitas = d.sitas; o.sitar = d.sitar; o.damp_coe = d.damp_coe; o.beta = d.beta; o.ds = d.ds;
         o.bnn = d.bnn; o.epsr = d.epsr; o.epss = d.epss; o.sd = d.sd; o.sw = d.sw; o.c1 = d.c1; o.c2 = d.c2;
         o.g0 = d.g0; o.etao = d.etao; o.zet = d.zet; o.damp1 = d.damp1; o.damp2 = d.damp2;
         o.exp_c2sw = exp(d.c2 * d.sw);  // CPP_7-2-constitutive.cpp:333-336
         o.exp_c1sd = exp(d.c1 * d.sd);
-        double fai = atan(tan(d.fai * 0.0174532925166667) / d.fs);  // CPP_8-1-consfactor.cpp:294
-        o.dp_sq = sqrt(9.0 + 3.0 * sin(fai) * sin(fai));
-        o.dp_yalf = sin(fai) / o.dp_sq;
-        o.dp_cosf = cos(fai);
         // dilatancy angle of dpmodel_bui (CPP_8-1-consfactor.cpp:462-463, :511)
         double fai_dil = atan(tan(d.beta * 0.0174532925166667) / d.fs);
         o.dp_yalf_dil = sin(fai_dil) / sqrt(9.0 + 3.0 * sin(fai_dil) * sin(fai_dil));
     }
     std::vector<FluidSlot> fl(8);
+    std::vector<sphg_fluid> fluid_src(8);
     for (int s = 0; s < 8; s++) {
         sphg_fluid d{};
         d.strain_min = 1.0;
         if (s < 3) d = c->mat.water[s];
         else if (s == 3) d = c->mat.air;
+        fluid_src[s] = d;
         FluidSlot &o = fl[s];
         o.vis = d.vis; o.vpu = d.vpu; o.dens = d.dens; o.porosity = d.porosity; o.c = d.c; o.pre_coe = d.pre_coe;
         o.strain_min = d.strain_min;
@@ -628,6 +693,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
         ss[i] = (int8_t)soil_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
         fs[i] = (int8_t)fluid_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
     }
+    if ((rc = check_pinned_materials(c, soil_src, fluid_src, ss, fs))) return rc;
     HIPCHK(hipMemcpy(c->S.soil_slot, ss.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->S.fluid_slot, fs.data(), n, hipMemcpyHostToDevice));
     // per-particle fluid density (drag partner), and the uniformity flags the pair passes use
@@ -645,6 +711,36 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
         }
     }
     HIPCHK(hipMemcpy(c->S.fdens, fd.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    {
+        // per-particle material values pParti_ConsPara[i].ds / .cop / .fai: the soil slot's unless
+        // the caller set them (random fields), and the Drucker-Prager constants of each particle's
+        // friction angle, evaluated here with the host's libm as the slot constants are
+        std::vector<double> v(n), fai(n), dp(3 * (size_t)n);
+        auto put = [&](double *d, const double *h) { return hipMemcpy(d, h, (size_t)n * 8, hipMemcpyHostToDevice); };
+        if (!c->has_psds) {
+            for (int i = 0; i < n; i++) v[i] = soil[ss[i]].ds;
+            HIPCHK(put(c->S.ps_ds, v.data()));
+        }
+        if (!c->has_pscop) {
+            for (int i = 0; i < n; i++) v[i] = soil[ss[i]].cop;
+            HIPCHK(put(c->S.ps_cop, v.data()));
+        }
+        if (c->has_psfai) {
+            HIPCHK(hipMemcpy(fai.data(), c->S.ps_fai, (size_t)n * 8, hipMemcpyDeviceToHost));
+        } else {
+            for (int i = 0; i < n; i++) fai[i] = ss[i] < SPHG_NSOIL ? c->mat.soil[ss[i]].fai : 0.0;
+            HIPCHK(put(c->S.ps_fai, fai.data()));
+        }
+        for (int i = 0; i < n; i++) {
+            const double fs = soil[ss[i]].fs;
+            const double f = atan(tan(fai[i] * 0.0174532925166667) / fs);  // CPP_8-1-consfactor.cpp:294, :783
+            const double sq = sqrt(9.0 + 3.0 * sin(f) * sin(f));
+            dp[i] = sin(f) / sq;
+            dp[n + i] = cos(f);
+            dp[2 * (size_t)n + i] = sq;
+        }
+        for (int k = 0; k < 3; k++) HIPCHK(put(c->S.ps_dp[k], dp.data() + (size_t)k * n));
+    }
     HIPCHK(hipMemcpy(hl.data(), c->S.hl, (size_t)n * 8, hipMemcpyDeviceToHost));
     P.hl_uniform = 1;
     for (int i = 1; i < n; i++)

@@ -21,13 +21,10 @@ constexpr int kNSlots = 32;  // 30 soil slots + default row (index 30) + padding
 // arithmetic that consumes them reproduces the reference's FP64 values exactly.
 struct SoilSlot {
     double e, poi, eps0, dens, kapa, zmf, ann, ramda, fs, c, sme0, porosity, vpu, cop, sitas, sitar;
-    double damp_coe, beta;
+    double damp_coe, beta, ds;
     double bnn, epsr, epss, sd, sw, c1, c2, g0, etao, zet, damp1, damp2;  // UMCC / FOCC parameters
     double exp_c2sw, exp_c1sd;  // exp(c2 * sw), exp(c1 * sd) of moisturesuct (CPP_7-2-constitutive.cpp:333-336)
     double dp_yalf_dil;  // sin(psi') / sqrt(9 + 3 sin^2 psi'), psi' = atan(tan(beta)/fs) (Bui DP dilatancy)
-    double dp_yalf;   // sin(phi') / sqrt(9 + 3 sin^2 phi'), phi' = atan(tan(phi)/fs)  (CPP_8-1-consfactor.cpp:346)
-    double dp_cosf;   // cos(phi')
-    double dp_sq;     // sqrt(9 + 3 sin^2 phi')
 };
 
 struct FluidSlot {
@@ -85,6 +82,11 @@ struct State {
     double *vb[4][3];     // clVar_Boundary vx_water/soil/air/struct
     double *vel_bndy[3];  // cl_bndy_pair
     double *ps_c;         // per-particle cohesion (random fields), or null -> slot value
+    double *ps_ds;        // per-particle grain size of the drag laws (slot value unless set per particle)
+    double *ps_cop;       // per-particle permeability of permtype 0 (slot value unless set per particle)
+    double *ps_fai;       // per-particle friction angle (slot value unless set per particle)
+    double *ps_dp[3];     // from ps_fai on the host: sin(phi') / sqrt(9 + 3 sin^2 phi'), cos(phi'),
+                          // sqrt(9 + 3 sin^2 phi') with phi' = atan(tan(fai) / fs)
     int *type, *matype, *permtype, *etype;
     int8_t *soil_slot, *fluid_slot;
     // cell list

@@ -841,7 +841,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     if (P.fli == 1 && ti == 2) {
         const SoilSlot &sl = S.soil[S.soil_slot[i]];
         double cop = S.cop[i];
-        if (P.permtype == 0) cop = sl.cop;
+        if (P.permtype == 0) cop = S.ps_cop[i];
         else if (P.permtype == 1) cop = P.pers;
         else if (P.permtype == 2) {
             double sa = S.satu[i];
@@ -1090,7 +1090,8 @@ __device__ void focc_tangent(const double sigin[6], const double deps[6], double
 
 // one DP-family stress update (dpmodel1 / dpmodel_bui + DPModel_check), CPP_8-1-consfactor.cpp:284-615, :779-819
 __device__ __forceinline__ void dp_update(const double sig[6], const double deps[6], const SoilSlot &s, double ci,
-                                          double yalf_dil, bool bui, double dt, double dsig[6], double adps[6]) {
+                                          double yalf, double cosf, double sq, double yalf_dil, bool bui, double dt,
+                                          double dsig[6], double adps[6]) {
     double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * 0.3333333333;
     if (sm < 1.0) sm = 1.0;
     double e0 = soil_e0(s, sm);
@@ -1101,8 +1102,7 @@ __device__ __forceinline__ void dp_update(const double sig[6], const double deps
     double dxy = -sig[3], dyz = -sig[4], dzx = -sig[5];
     double sjj = 0.5 * (dxv * dxv + dyv * dyv + dzv * dzv) + dxy * dxy + dyz * dyz + dzx * dzx;
     double sj = sqrt(sjj);
-    double yalf = s.dp_yalf;
-    double ykap = 3.0 * ci * s.dp_cosf / s.dp_sq;
+    double ykap = 3.0 * ci * cosf / sq;
     bool plastic = false;
     double d = 0.0, wv[6], zv[6], ramda = 0.0, fv[6];
     if ((sj - yalf * sm * 3.0 - ykap) >= 0.0) {
@@ -1276,7 +1276,8 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
                 tangent_apply(dsig, De, false, 0.0, z6, z6, deps);
             } else if (mt == 2 || mt == 5) {
                 double ci = (S.ps_c ? S.ps_c[i] : s.c) / s.fs;
-                dp_update(sig, deps, s, ci, s.dp_yalf_dil, mt == 5, dt, dsig, adps);
+                dp_update(sig, deps, s, ci, S.ps_dp[0][i], S.ps_dp[1][i], S.ps_dp[2][i], s.dp_yalf_dil, mt == 5, dt,
+                          dsig, adps);
             } else if (kFull && (mt == 1 || mt == 3)) {
                 camclay_update(sig, deps, s, mt == 3, S.satu[i], S.dsatu[i], dsig, adps, erfs);
             } else if (mt == 4) {
@@ -1556,6 +1557,38 @@ __device__ __constant__ int kSidx[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
 
 // artificial_viscosity pair term (CPP_9-3-artificialvis.cpp:36-89); dst is sqrt of the same
 // squared distance the kernel value was built from.
+template <int NDIM>
+__device__ __forceinline__ double mag3(const double v[3]) {
+    double m = 0.0;
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) m = m + v[k] * v[k];
+    return sqrt(m);
+}
+
+// Drag coefficient beta of a soil / water pair. fli 2 (interact2, Tsuji et al.): Ergun at soil
+// porosity <= 0.8, Wen-Yu above with the Schiller-Naumann cd of the pair's Reynolds number
+// (CPP_10-interact1.cpp:162-182). fli 3 (interact3, Peng et al.): Kozeny-Carman permeability
+// cd, turbulent term above Reynolds number 60 (:317-333). por and ds are the soil particle's,
+// vis and dens the water's parameters, rhow the water particle's density.
+__device__ __noinline__ double drag_beta(int fli, double mag_v, double por, double ds, double vis, double dens,
+                                         double rhow) {
+    if (fli == 2) {
+        if (por <= 0.8) {
+            double beta = 150.0 * (1.0 - por) * (1.0 - por) / por / por / ds / ds * vis;
+            return beta + 1.75 * (1.0 - por) / ds / por * dens * mag_v;
+        }
+        const double re = mag_v * dens * por * ds / vis;
+        double cd = 0.0;
+        if (re > 1000.0) cd = 0.43;
+        else if (fabs(re) > 0.0001) cd = 24.0 * (1.0 + 0.15 * pow(re, 0.687)) / re;
+        return 0.75 * cd * mag_v * dens * (1.0 - por) / ds * pow(por, -2.7);
+    }
+    const double re = mag_v * dens * ds / vis;
+    const double cd = por * por * por * ds * ds * 0.0066666667 / (1.0 - por) / (1.0 - por);
+    if (re > 60.0) return vis / cd + 1.75 / sqrt(150.0) * rhow * mag_v / sqrt(cd) / pow(por, 1.5);
+    return vis / cd;
+}
+
 template <int NDIM, class M>
 __device__ __forceinline__ void av_pair(const double vi[3], const double vj[3], const double xi[3],
                                         const double xj[3], double dst, double hli, double hlj, double rhoi,
@@ -1923,17 +1956,17 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     // artificial stress (ti == 2) and geostress damping (ti 2 / 4, l <= inip): pair sums from k_accel_pre
     const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
-    const bool drag_on = P.fli == 1;
+    const bool drag_on = P.fli >= 1;
     if (ti == 1 || ti == 3) {
         // fluid_acceleration_water/air (+ minor variants), CPP_9-4-derivation_stress.cpp:12-351;
         // qsig of a fluid neighbour holds (porosity * sig) / rho / rho, the j factor of the sum
         double fi[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) fi[k] = (fpor * sigi[k]) / rhoi / rhoi;
-        const double fdi = S.fluid[S.fluid_slot[i]].dens;
+        const double fdi = S.fluid[S.fluid_slot[i]].dens, fvisi = S.fluid[S.fluid_slot[i]].vis;
         constexpr int B = 1;  // one neighbour per round trip: 3 waves per SIMD instead of 2
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
-        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satjb[B], porjb[B], copjb[B];
+        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satjb[B], porjb[B], copjb[B], dsjb[B];
 #pragma unroll
         for (int b = 0; b < B; b++) {
             const int j = nb_id(e[b]);
@@ -1948,10 +1981,12 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             satjb[b] = 0.0;
             porjb[b] = 0.0;
             copjb[b] = 1.0;
+            dsjb[b] = 1.0;
             if (drag_on) {
                 satjb[b] = S.satu[j];
                 porjb[b] = S.porosity[j];
                 copjb[b] = S.cop[j];
+                if (P.fli >= 2) dsjb[b] = S.ps_ds[j];
             }
         }
 #pragma unroll
@@ -1960,7 +1995,8 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             const int tj = nb_type(e[b]);
             const double *xj = xjb[b], *vj = vjb[b];
             double *qj = qjb[b];
-            const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satj = satjb[b], porj = porjb[b], copj = copjb[b];
+            const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satj = satjb[b], porj = porjb[b], copj = copjb[b],
+                         dsj = dsjb[b];
             double w[4], d2, dst;
             pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
             if (tj == ti || tj == fother) {
@@ -1987,7 +2023,12 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 }
                 if (inprod < 0) {
                     const double w3 = w[3];
-                    if (ti == 1) {
+                    if (ti == 1 && P.fli >= 2) {
+                        // interact2 / interact3 water against soil (CPP_10-interact1.cpp:184-218, :343-372)
+                        const double beta = drag_beta(P.fli, mag3<NDIM>(dv), porj, dsj, fvisi, fdi, rhoi);
+#pragma unroll
+                        for (int k = 0; k < NDIM; k++) drag[k] = drag[k] + M::div(mj, rj) * w3 * beta * dv[k];
+                    } else if (ti == 1) {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
                             drag[k] = drag[k] + M::div(M::div(satj * satj * porj * porj * fdi * P.gacc * dv[k], copj) * mj, rj) * w3;
@@ -2005,10 +2046,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         double qi[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
-        const double copi = S.cop[i], pori = S.porosity[i];
+        const double copi = S.cop[i], pori = S.porosity[i], dsi = (P.fli >= 2 && ti == 2) ? S.ps_ds[i] : 1.0;
         constexpr int B = 1;  // one neighbour per round trip: 3 waves per SIMD instead of 2
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
-        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B];
+        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B], fvjb[B];
 #pragma unroll
         for (int b = 0; b < B; b++) {
             const int j = nb_id(e[b]);
@@ -2024,6 +2065,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             prewb[b] = S.prew[j];
             preab[b] = S.prea[j];
             fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
+            fvjb[b] = (P.fli >= 2 && ti == 2) ? S.fluid[S.fluid_slot[j]].vis : 1.0;
         }
 #pragma unroll
         for (int b = 0; b < B; b++) {
@@ -2031,7 +2073,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             const int tj = nb_type(e[b]);
             const double *xj = xjb[b], *vj = vjb[b], *qj = qjb[b];
             const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satu = satub[b], prewj = prewb[b],
-                         preaj = preab[b], fdj = fdjb[b];
+                         preaj = preab[b], fdj = fdjb[b], fvj = fvjb[b];
             double w[4], d2, dst;
             pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
             if (tj == ti) {
@@ -2069,7 +2111,12 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 }
                 if (inprod < 0) {
                     const double w3 = w[3];
-                    if (tj == 1) {
+                    if (tj == 1 && P.fli >= 2) {
+                        // interact2 / interact3 soil against water (CPP_10-interact1.cpp:158-182, :311-340)
+                        const double beta = drag_beta(P.fli, mag3<NDIM>(dv), pori, dsi, fvj, fdj, rj);
+#pragma unroll
+                        for (int k = 0; k < NDIM; k++) drag[k] = drag[k] + M::div(mj, rj) * w3 * beta * dv[k];
+                    } else if (tj == 1) {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
                             drag[k] = drag[k] + M::div(M::div(satui * satui * pori * pori * fdj * P.gacc * dv[k], copi) * mj, rj) * w3;
@@ -2134,8 +2181,8 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     }
     const double dt = P.dt;
     const double axp[3] = {S.axp[0][i], S.axp[1][i], S.axp[2][i]};
-    if (P.fli == 1 && ti != 0 && ti != 7) {
-        // interact1 clamp (CPP_10-interact1.cpp:93-108)
+    if (P.fli >= 1 && ti != 0 && ti != 7) {
+        // velocity-reversal clamp of interact1-3 (CPP_10-interact1.cpp:93-108, :249-264, :404-419)
         #pragma unroll
         for (int k = 0; k < NDIM; k++) {
             double itf = drag[k];

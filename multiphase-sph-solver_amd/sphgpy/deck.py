@@ -566,6 +566,43 @@ def tran_box(ndim=2, flbndy=3, nx=8, ny=3, nz_soil=4, nz_water=3, dx=0.002, step
     return deck
 
 
+def drag_box(ndim=2, fli=2, nx=8, ny=3, nz_soil=4, nz_water=3, dx=0.002, steps=6, seed=51, eps0=0.6, ds=4e-3,
+             vis=1e-3, v0=0.3, c_water=30.0) -> Deck:
+    """Soil under water in a rigid-walled box (flbndy 2) with the fluid-solid drag of Tsuji et al.
+    (fli 2: Ergun below porosity 0.8, Wen-Yu above; CPP_10-interact1.cpp:115-267) or of Peng et al.
+    (fli 3: Kozeny-Carman, turbulent term above Reynolds number 60; :270-420). Soil particles
+    start at porosity eps0 / (1 + eps0) with grain size ds; every water particle slides along x at
+    its own speed, uniform in [0, 2 v0), over soil sliding at -v0 / 2, so the pairs' Reynolds
+    numbers spread over the branches of the drag law."""
+    rng = np.random.default_rng(seed)
+    grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
+    d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, fli=fli, interc=1.0, flbndy=2,
+             art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"drag_box_{ndim}d_fli{fli}")
+    d.model_flags[2] = 1
+    layer = dp_layer(c=1000.0, fai=25.0, e=1.8e6, poi=0.3, dens=2650.0, eps0=eps0, cop=1e-4)
+    layer["ds"] = ds
+    d.soil_layers[2] = [layer]
+    w = default_fluid()
+    por = eps0 / (1 + eps0)
+    B = 1000.0 * c_water ** 2 / 7.0
+    w.update(dens=1000.0, vis=vis, vpu=c_water, press_type=0, acc_type=0, bhtype=0,
+             pre_coe=B / (por * 1000.0 * 9.81 * dx), strain_min=1.0)
+    d.water = [dict(w), dict(w), dict(w)]
+    top = nz_soil + nz_water
+    if ndim == 2:
+        parts = [(_lattice2d(0, nx, 0, nz_soil, dx), 2, 2), (_lattice2d(0, nx, nz_soil, top, dx), 1, 0),
+                 (_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0), (_lattice2d(-3, 0, 0, top + 3, dx), 0, 0),
+                 (_lattice2d(nx, nx + 3, 0, top + 3, dx), 0, 0)]
+    else:
+        parts = [(_lattice3d(0, nx, 0, ny, 0, nz_soil, dx), 2, 2), (_lattice3d(0, nx, 0, ny, nz_soil, top, dx), 1, 0),
+                 (_box_walls_3d(nx, ny, top + 3, dx), 0, 0)]
+    deck = _assemble(d, parts, rng)
+    water = deck.type == 1
+    deck.vxp[water, 0] = v0 * rng.uniform(0.0, 2.0, int(water.sum()))
+    deck.vxp[deck.type == 2, 0] = -0.5 * v0
+    return deck
+
+
 def slope_2d(height=5.0, dx=0.1, steps=3000, seed=0, crest=1.5, toe=1.5, base=4.0, walls=3) -> Deck:
     """2D soil slope for the random-field batch: a crest of `crest`*H, a 1:2 (vertical:horizontal)
     face, a toe of `toe`*H and a `base` m layer under all of it, Drucker-Prager soil (mean

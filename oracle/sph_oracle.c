@@ -1676,6 +1676,89 @@ static void or_interact1(or_ctx *c) {
     }
 }
 
+/* drag coefficient beta of a soil / water pair: interact2 (fli 2, Ergun below porosity 0.8,
+ * Wen-Yu above, CPP_10-interact1.cpp:162-182) or interact3 (fli 3, Kozeny-Carman permeability
+ * cd, turbulent term above Reynolds number 60, CPP_10-interact1.cpp:317-333). por and ds are the
+ * soil particle's, vis and dens the water's parameter, rhow the water particle's density. */
+static double drag_beta(int fli, double mag_v, double por, double ds, double vis, double dens, double rhow) {
+    double re, cd, beta;
+    if (fli == 2) {
+        re = mag_v * dens * por * ds / vis;
+        if (re > 1000.0) cd = 0.43;
+        else if (fabs(re) > 0.0001) cd = 24.0 * (1.0 + 0.15 * pow(re, 0.687)) / re;
+        else cd = 0.0;
+        if (por <= 0.8) {
+            beta = 150.0 * (1.0 - por) * (1.0 - por) / por / por / ds / ds * vis;
+            beta = beta + 1.75 * (1.0 - por) / ds / por * dens * mag_v;
+        } else {
+            beta = 0.75 * cd * mag_v * dens * (1.0 - por) / ds * pow(por, -2.7);
+        }
+        return beta;
+    }
+    re = mag_v * dens * ds / vis;
+    cd = por * por * por * ds * ds * 0.0066666667 / (1.0 - por) / (1.0 - por);
+    if (re > 60.0) return vis / cd + 1.75 / sqrt(150.0) * rhow * mag_v / sqrt(cd) / pow(por, 1.5);
+    return vis / cd;
+}
+
+/* interact2 / interact3: drag with laminar and turbulent parts between soil and water, the
+ * interact1 air terms, and the interact1 velocity-reversal clamp (CPP_10-interact1.cpp:115-267,
+ * :270-420) */
+static void or_interact23(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, fli = c->pro.fli, i;
+    double xishu = c->pro.interc, dt = c->pro.dt;
+    double gacc = sqrt(c->pro.gx * c->pro.gx + c->pro.gy * c->pro.gy + c->pro.gz * c->pro.gz);
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double tu[3] = {0, 0, 0}, dx[3], dvx[3], inprod, mag_v, beta, vx1, vx2;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double w3 = c->pc[i].wij[nj][3];
+            if (pi->type == pj->type) continue;
+            inprod = 0.0;
+            mag_v = 0.0;
+            for (k = 0; k < ndim; k++) {
+                dx[k] = pi->xp[k] - pj->xp[k];
+                dvx[k] = pi->vxp[k] - pj->vxp[k];
+                mag_v = mag_v + dvx[k] * dvx[k];
+                inprod = inprod + dx[k] * dvx[k];
+            }
+            mag_v = sqrt(mag_v);
+            if (!(inprod < 0)) continue;
+            if (pi->type == 2 && pj->type == 1) {
+                beta = drag_beta(fli, mag_v, pi->porosity, c->ps[i].ds, c->pf[j].vis, c->pf[j].dens, pj->rho);
+                for (k = 0; k < ndim; k++) tu[k] = tu[k] + pj->mass / pj->rho * w3 * beta * (pi->vxp[k] - pj->vxp[k]);
+            } else if (pi->type == 1 && pj->type == 2) {
+                beta = drag_beta(fli, mag_v, pj->porosity, c->ps[j].ds, c->pf[i].vis, c->pf[i].dens, pi->rho);
+                for (k = 0; k < ndim; k++) tu[k] = tu[k] + pj->mass / pj->rho * w3 * beta * (pi->vxp[k] - pj->vxp[k]);
+            } else if (pi->type == 2 && pj->type == 3) {
+                for (k = 0; k < ndim; k++)
+                    tu[k] = tu[k] + (1 - pi->satu) * (1 - pi->satu) * pi->porosity * pi->porosity * c->pf[j].dens *
+                                        gacc * dvx[k] * 2000 * pj->mass / pj->rho * w3;
+            } else if (pi->type == 3 && pj->type == 2) {
+                for (k = 0; k < ndim; k++)
+                    tu[k] = tu[k] + (1 - pj->satu) * (1 - pj->satu) * pj->porosity * pj->porosity * c->pf[i].dens *
+                                        gacc * dvx[k] * 2000 * pj->mass / pj->rho * w3;
+            }
+        }
+        if (pi->type != 0 && pi->type != 7) {
+            for (k = 0; k < ndim; k++) {
+                pi->interf[k] = tu[k];
+                vx1 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k]) * dt;
+                vx2 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k] - pi->interf[k] * xishu / pi->rho) * dt;
+                if (vx1 * vx2 <= 0.0) {
+                    pi->interf[k] = 0.0;
+                    pi->ax[k] = -2.0 * pi->vxp[k] / dt - pi->axp[k];
+                } else
+                    pi->ax[k] -= pi->interf[k] * xishu / pi->rho;
+            }
+        }
+    }
+}
+
 
 /* ------------------------------------------------------------------ regularisation, saturation */
 /* strain_regu / stress_regu: Shepard smoothing of eps or sig over same-type neighbours
@@ -2620,7 +2703,7 @@ static void or_free_solid(or_ctx *c) {
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
     if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
-    if (P->fli > 1 || P->ss_type < 0 || P->ss_type > 3) return 1;
+    if (P->fli < 0 || P->fli > 3 || P->ss_type < 0 || P->ss_type > 3) return 1;
     if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 5 && P->flbndy != 13 &&
         P->flbndy != 14)
         return 1;
@@ -2667,6 +2750,7 @@ int or_phase(or_ctx *c, int mask) {
         else or_bndy_stress_tran(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3 && c->pro.ss_acc_type == 2) or_coupled(c);
         if (c->pro.fli == 1) or_interact1(c);
+        else if (c->pro.fli == 2 || c->pro.fli == 3) or_interact23(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3) or_ssi(c);
         if (c->l <= c->pro.inip) or_damping_geostress(c);
         if (c->pro.flbndy == 5) or_free_solid(c);

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 from helpers import run_ref, run_ref_kl, have_ref  # noqa: E402
 from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d,  # noqa: E402
-                    art_stress_block, slope_2d, ssi_block, tran_box)
+                    art_stress_block, slope_2d, ssi_block, tran_box, drag_box)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype"})
@@ -49,6 +49,11 @@ def decks():
         "tran_3d_b14": (tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 5, (1, 5)),
         "free_solid_2d": (tran_box(2, 5, steps=6, gap=0.4, vdown=0.3), 6, (1, 3, 6)),
         "free_solid_3d": (tran_box(3, 5, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, gap=0.4, vdown=0.3), 5, (1, 5)),
+        "drag_2d_fli2": (drag_box(2, 2), 6, (1, 3, 6)),
+        "drag_2d_fli2_wenyu": (drag_box(2, 2, eps0=5.0), 6, (1, 3, 6)),
+        "drag_2d_fli3": (drag_box(2, 3, vis=0.02), 6, (1, 3, 6)),
+        "drag_3d_fli2": (drag_box(3, 2, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, eps0=5.0), 5, (1, 5)),
+        "drag_3d_fli3": (drag_box(3, 3, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, vis=0.02), 5, (1, 5)),
     }
 
 

@@ -276,7 +276,87 @@ def test_tran_boundaries_bit_exact_vs_oracle(name):
     assert np.abs(g.get("vx_soil")[deck.type == 0]).max() > 0.0
 
 
-@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN + FREE_SOLID)
+DRAG = ["drag_2d_fli2", "drag_2d_fli2_wenyu", "drag_2d_fli3", "drag_3d_fli2", "drag_3d_fli3"]
+
+
+@pytest.mark.parametrize("name", DRAG)
+def test_drag_laws_vs_oracle(name):
+    # interact2 (Ergun / Wen-Yu with the Schiller-Naumann cd) and interact3 (Kozeny-Carman with the
+    # turbulent term): pair sums in both phases' momentum passes and the velocity-reversal clamp.
+    # The Wen-Yu and turbulent branches call pow, whose device and C-library results may differ
+    # by an ulp: 1e-9 field-max relative over five steps
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare_tol(g, o, 1e-9) == {}, f"step {s + 1}"
+        if name == "drag_2d_fli2":  # the Ergun law calls no transcendental function: bit-exact
+            assert compare(g, o) == {}, f"step {s + 1}"
+    print(name, "bit-exact fields:", sorted(set(GPU_FIELDS) - set(compare(g, o))))
+    f = g.get("interf")
+    assert np.abs(f[deck.type == 2]).max() > 0.0 and np.abs(f[deck.type == 1]).max() > 0.0
+
+
+def test_drag_with_per_particle_grain_size():
+    # a random field on the grain size (ps_ds, pParti_ConsPara[i].ds) reaches the drag laws
+    deck, states = load_golden("drag_2d_fli2")
+    ds = states[0]["ps_ds"] * np.random.default_rng(9).uniform(0.5, 1.5, deck.n)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    g.set("ps_ds", ds)
+    g.upload_done()
+    o.set("ps_ds", ds)
+    assert np.array_equal(g.get("ps_ds"), ds)
+    g0 = make_gpu(deck, states[0])
+    for s in range(3):
+        assert g.step(1) == 0 and o.step(1) == 0 and g0.step(1) == 0
+        assert compare_tol(g, o, 1e-9) == {}, f"step {s + 1}"
+    assert not np.array_equal(g.get("interf"), g0.get("interf"))
+
+
+def test_per_particle_friction_angle_and_permeability():
+    # random fields on the friction angle (ps_fai: the Drucker-Prager constants of each particle,
+    # CPP_8-1-consfactor.cpp:294, :783) and on the permeability of permtype 0 (ps_cop,
+    # CPP_7-2-constitutive.cpp:362) reach the step
+    deck, states = load_golden("soilwater3d_small")
+    deck = deck.copy()
+    deck.permtype = 0
+    rng = np.random.default_rng(10)
+    soil = deck.type == 2
+    fai = states[0]["ps_fai"] * np.where(soil, rng.uniform(0.8, 1.2, deck.n), 1.0)
+    cop = states[0]["ps_cop"] * np.where(soil, rng.uniform(0.5, 2.0, deck.n), 1.0)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for k, v in (("ps_fai", fai), ("ps_cop", cop)):
+        g.set(k, v)
+        o.set(k, v)
+    g.upload_done()
+    g0 = make_gpu(deck, states[0])
+    for s in range(4):
+        assert g.step(1) == 0 and o.step(1) == 0 and g0.step(1) == 0
+        assert compare_tol(g, o, 1e-9) == {}, f"step {s + 1}"
+    assert np.array_equal(g.get("ps_fai"), fai) and np.array_equal(g.get("ps_cop"), cop)
+    assert not np.array_equal(g.get("cop")[soil], g0.get("cop")[soil])
+    assert not np.array_equal(g.get("sig"), g0.get("sig"))
+
+
+def test_other_per_particle_materials_must_equal_their_slot():
+    # per-particle copies the step takes from the material slots: equal values are accepted,
+    # different ones refused (102) instead of being dropped
+    deck, states = load_golden("drag_2d_fli2")
+    g = GpuSim(deck)
+    g.load_state(states[0])
+    g.set("ps_e", states[0]["ps_e"])
+    g.set("pf_vis", states[0]["pf_vis"])
+    g.upload_done()
+    g.set("pf_fai", states[0]["pf_fai"] + (deck.type == 1) * 5.0)
+    with pytest.raises(RuntimeError, match="error 102"):
+        g.upload_done()
+
+
+@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN + FREE_SOLID + DRAG)
 def test_gpu_vs_reference_golden_models(name):
     # against the reference itself: within the oracle-Jacobi distance plus the device
     # transcendental allowance

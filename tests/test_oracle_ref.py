@@ -33,7 +33,8 @@ def _mismatches(sim, ref):
 GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1", "moving_box_m2", "moving_box_m3",
           "art_stress_2d", "art_stress_3d", "ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3",
           "ssi_cpl_2d", "ssi_cpl_3d", "tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3",
-          "tran_3d_b14", "free_solid_2d", "free_solid_3d"]
+          "tran_3d_b14", "free_solid_2d", "free_solid_3d", "drag_2d_fli2", "drag_2d_fli2_wenyu", "drag_2d_fli3",
+          "drag_3d_fli2", "drag_3d_fli3"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -164,3 +165,31 @@ def test_free_solid_wall_contact_closed_form(name):
         assert deck.type[i] in (1, 2)
         assert dax[i, nrm] == pytest.approx(zeta * 0.3 / dt, rel=2e-5)
         assert dax[i, 0] == pytest.approx(-zeta * min(0.2, myu * 0.3) / dt, rel=2e-5)
+
+
+@pytest.mark.parametrize("name", ["drag_2d_fli2", "drag_2d_fli2_wenyu", "drag_2d_fli3", "drag_3d_fli2", "drag_3d_fli3"])
+def test_drag_fixtures_are_not_trivial(name):
+    # the reference's interact2 / interact3 act on both phases and move them apart from the same
+    # deck without drag
+    deck, states = load_golden(name)
+    last = max(states)
+    f = states[last]["interf"]
+    assert np.abs(f[deck.type == 2]).max() > 0.0 and np.abs(f[deck.type == 1]).max() > 0.0
+    d0 = deck.copy()
+    d0.fli = 0
+    a = OracleSim(d0, states[0], mode=GS)
+    a.step(last)
+    assert not bitequal(a.get("vx"), states[last]["vx"])
+
+
+def test_drag_laws_differ():
+    # the same deck under fli 1, 2 and 3 gives three different states
+    deck, states = load_golden("drag_2d_fli2")
+    vx = []
+    for fli in (1, 2, 3):
+        d = deck.copy()
+        d.fli = fli
+        s = OracleSim(d, states[0], mode=GS)
+        assert s.step(3) == 0
+        vx.append(s.get("vx"))
+    assert not bitequal(vx[0], vx[1]) and not bitequal(vx[1], vx[2]) and not bitequal(vx[0], vx[2])
