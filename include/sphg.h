@@ -160,6 +160,11 @@ int sphg_setting_par(const sphg_problem *pro, const sphg_materials *mat, int n,
 int sphg_parttocell(sphg_ctx *ctx);
 /* clNNPS_Fun::partisearching with weight() (CPP_3-NNPS.cpp:265-374, :17-48). */
 int sphg_partisearching(sphg_ctx *ctx);
+/* clNNPS_Fun::kernel_gradient_correction (CPP_3-NNPS.cpp:376-559), called after the search when
+ * flknl > 2 (CPP_1-SPHLoops.cpp:544-546; a no-op otherwise): the correction matrix L^-1 of every
+ * soil and structure particle, which the later passes apply to those particles' kernel gradients.
+ * sphg_step and the graphs include it. */
+int sphg_kernel_gradient_correction(sphg_ctx *ctx);
 /* clDensity_Fun::density (CPP_6-density.cpp:11-189). The normalisation pass reads every
  * neighbour's pre-normalisation density (Jacobi form), where the reference's serial loop reads
  * the already normalised densities of lower-index neighbours in place (:168-188). */

@@ -176,6 +176,8 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
         D(*f);
     plan.push_back({(void **)&c->d_psc, cap * 8});
     D(S.ps_ds); D(S.ps_cop); D(S.ps_fai);
+    S.kgc = nullptr;
+    if (pro->flknl > 2) plan.push_back({(void **)&S.kgc, 9 * cap * 8});
     for (int k = 0; k < 3; k++) D(S.ps_dp[k]);
     I(S.type, cap); I(S.matype, cap); I(S.permtype, cap); I(S.etype, cap);
     plan.push_back({(void **)&S.soil_slot, cap});
@@ -530,7 +532,7 @@ static int fluid_slot_of(int type, int matype, int etype) {
 }
 
 static int check_supported(const sphg_problem &p) {
-    if (p.flknl > 2) return SPHG_E_UNSUPPORTED;  // kernel_gradient_correction
+    if (p.flknl < 1 || p.flknl > 4) return SPHG_E_UNSUPPORTED;  // 3 / 4: with kernel_gradient_correction
     if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
     if (p.fli < 0 || p.fli > 3 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
     if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 5 && p.flbndy != 13 &&
@@ -809,6 +811,7 @@ static void enqueue_nnps(sphg_ctx *c) {
     LAUNCH("scatter", k_scatter, grid_for(n), c->S, c->P);
     LAUNCH("cell_order", k_cell_order, grid_for(M), c->S, M);
     LAUNCH("search", k_search<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (c->P.flknl > 2) LAUNCH("kgc", k_kgc<NDIM>, grid_for(n), c->S, c->P, cur);
 }
 
 template <int NDIM, class M>
@@ -936,6 +939,15 @@ extern "C" int sphg_partisearching(sphg_ctx *c) {
     const int n = c->n, cur = c->cur;
     if (c->P.ndim == 2) LAUNCH("search", k_search<2>, grid_for(n), c->S, c->P, cur);
     else LAUNCH("search", k_search<3>, grid_for(n), c->S, c->P, cur);
+    finish_timing(c);
+    return read_err(c);
+}
+extern "C" int sphg_kernel_gradient_correction(sphg_ctx *c) {
+    if (!c || !c->uploaded) return SPHG_E_ARG;
+    if (c->P.flknl <= 2) return SPHG_OK;  // the reference calls it only for flknl > 2
+    const int n = c->n, cur = c->cur;
+    if (c->P.ndim == 2) LAUNCH("kgc", k_kgc<2>, grid_for(n), c->S, c->P, cur);
+    else LAUNCH("kgc", k_kgc<3>, grid_for(n), c->S, c->P, cur);
     finish_timing(c);
     return read_err(c);
 }

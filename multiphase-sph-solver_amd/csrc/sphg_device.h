@@ -85,6 +85,8 @@ struct State {
     double *ps_ds;        // per-particle grain size of the drag laws (slot value unless set per particle)
     double *ps_cop;       // per-particle permeability of permtype 0 (slot value unless set per particle)
     double *ps_fai;       // per-particle friction angle (slot value unless set per particle)
+    double *kgc;          // flknl 3 / 4: kernel-gradient correction L^-1 of particle i, row-major entry r
+                          // at kgc[r * cap + i] (soil and structure particles; null otherwise)
     double *ps_dp[3];     // from ps_fai on the host: sin(phi') / sqrt(9 + 3 sin^2 phi'), cos(phi'),
                           // sqrt(9 + 3 sin^2 phi') with phi' = atan(tan(fai) / fs)
     int *type, *matype, *permtype, *etype;

@@ -138,6 +138,8 @@ struct KernelCtx {
     double ra, a1;
     double r2;             // (2 ra)^2, the squared support radius (mixed form)
     float ra_f, inv_ra_f, a1_f;
+    const double *kg;      // kernel-gradient correction rows (flknl 3 / 4) and the particle whose
+    int kcap, ci;          // matrix corrects the gradients (-1: none)
 };
 
 // Arithmetic of the pair passes.
@@ -175,7 +177,37 @@ __device__ __forceinline__ KernelCtx kernel_ctx(const Params &P, double hli) {
     k.ra_f = (float)k.ra;
     k.inv_ra_f = (float)(1.0 / k.ra);
     k.a1_f = (float)k.a1;
+    k.kg = nullptr;
+    k.kcap = P.cap;
+    k.ci = -1;
     return k;
+}
+
+// The same for the pair passes of particle i: with flknl 3 / 4 the gradients of a soil or
+// structure particle's pairs are the corrected ones (kernel_gradient_correction,
+// CPP_3-NNPS.cpp:376-496, rewrites wij of those particles before every later pass reads them).
+template <int NDIM>
+__device__ __forceinline__ KernelCtx kernel_ctx(const Params &P, double hli, const State &S, int i) {
+    KernelCtx k = kernel_ctx<NDIM>(P, hli);
+    if (P.flknl > 2) {
+        const int t = S.type[i];
+        if (t == 2 || t == 4) {
+            k.kg = S.kgc;
+            k.ci = i;
+        }
+    }
+    return k;
+}
+
+// c = L^-1 w for the gradient part of w (mat_multiply, CPP_3-NNPS.cpp:85-90)
+__device__ __forceinline__ void kgc_apply(const KernelCtx &kc, double w[4]) {
+    const double *b = kc.kg + kc.ci;
+    const size_t s = (size_t)kc.kcap;
+    double c[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) c[r] = b[(3 * r) * s] * w[0] + b[(3 * r + 1) * s] * w[1] + b[(3 * r + 2) * s] * w[2];
+#pragma unroll
+    for (int r = 0; r < 3; r++) w[r] = c[r];
 }
 
 // Mixed form of pair_kernel below: same branches, FP32 arithmetic; d2 stays the FP64 sum.
@@ -227,6 +259,7 @@ __device__ __forceinline__ void pair_kernel(const Params &P, const KernelCtx &kc
                                             double &dst) {
     if constexpr (M::kFast) {
         pair_kernel_fast<NDIM>(P, kc, xi, xj, w, d2, dst);
+        if (kc.ci >= 0) kgc_apply(kc, w);
         return;
     }
     double dx[3];
@@ -257,6 +290,7 @@ __device__ __forceinline__ void pair_kernel(const Params &P, const KernelCtx &kc
     if (s == 0.0 || near) w[3] = vn;
     else if (far) w[3] = bspline ? vf / 6.0 : vf;
     else w[3] = 0.0;
+    if (kc.ci >= 0) kgc_apply(kc, w);
 }
 
 // hl of neighbour j: equal to hl_i for every pair (uniform smoothing length, checked by the host)
@@ -447,6 +481,86 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     for (int k = 0; k < nk; k++) S.nbr[(size_t)k * P.cap + i] = stage[k][t];
 }
 
+// kernel_gradient_correction (CPP_3-NNPS.cpp:376-496, flknl 3 / 4), right after the search: for a
+// soil or structure particle L = sum over its soil, boundary and structure neighbours of
+// (m_j / rho_j) (x_j - x_i) (x) grad W (uncorrected), and L^-1 by cofactors (inverse_mat_2D/3D,
+// :498-559: the identity when |det L| <= 1e-5; in 2D row and column 2 stay zero). The pair passes
+// apply L^-1 to every gradient of that particle's list (kgc_apply).
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_kgc(State S, Params P, int cur) {
+    const int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    if (ti != 2 && ti != 4) return;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const int nc = nb_count(S, i);
+    double A[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    constexpr int B = 2;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        double xj[B][3], vol[B];
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = nb_id(e[b]);
+            load_xj<NDIM>(S, cur, j, xj[b]);
+            vol[b] = S.mass[j] / S.rho[j];
+        }
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int tj = nb_type(e[b]);
+            if (b >= act || !(tj == 2 || tj == 0 || tj == 4)) continue;
+            double w[4], d2, dst;
+            pair_kernel<NDIM, Exact>(P, kc, xi, xj[b], hli, hli, w, d2, dst);
+            double x[3];
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) x[k] = xj[b][k] - xi[k];
+#pragma unroll
+            for (int k = 0; k < NDIM; k++)
+#pragma unroll
+                for (int l = 0; l < NDIM; l++) A[k][l] = A[k][l] + vol[b] * x[k] * w[l];
+        }
+    });
+    double R[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    if (NDIM == 2) {
+        const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+        if (fabs(det) > 1.0e-5) {
+            R[0][0] = A[1][1] / det;
+            R[0][1] = -A[0][1] / det;
+            R[1][0] = -A[1][0] / det;
+            R[1][1] = A[0][0] / det;
+        } else {
+            R[0][0] = 1.0;
+            R[1][1] = 1.0;
+        }
+    } else {
+        const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                           A[1][0] * (A[0][1] * A[2][2] - A[0][2] * A[2][1]) +
+                           A[2][0] * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+        if (fabs(det) > 1.0e-5) {
+            R[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+            R[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+            R[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+            R[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+            R[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+            R[1][2] = (A[1][0] * A[0][2] - A[0][0] * A[1][2]) / det;
+            R[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+            R[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+            R[2][2] = (A[0][0] * A[1][1] - A[1][0] * A[0][1]) / det;
+        } else {
+            R[0][0] = 1.0;
+            R[1][1] = 1.0;
+            R[2][2] = 1.0;
+        }
+    }
+    const size_t cap = (size_t)P.cap;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) S.kgc[(3 * r + l) * cap + i] = R[r][l];
+}
+
 // boundary treatment of Tran et al. (no slip 3, free slip 4, each after velocity_set_domain 13 / 14)
 __host__ __device__ __forceinline__ bool tran_bndy(const Params &P) {
     return P.flbndy == 3 || P.flbndy == 4 || P.flbndy == 13 || P.flbndy == 14;
@@ -503,7 +617,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
     const bool solid = (ti == 2 || ti == 4), water = ti == 1;
@@ -585,7 +699,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double up = 0.0, dn = 0.0;
     constexpr int B = BW;
@@ -642,7 +756,7 @@ __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) 
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     const SoilSlot &sl = S.soil[S.soil_slot[i]];
     const double sat_cof = sl.sitas - sl.sitar;
@@ -697,7 +811,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     if (l == P.inip)
         for (int k = 0; k < 6; k++) S.eps[k][i] = 0.0;
@@ -1520,7 +1634,7 @@ __global__ __launch_bounds__(256) void k_regu(State S, Params P, int cur, int wh
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     double up[6] = {0, 0, 0, 0, 0, 0}, dn = 0.0;
     constexpr int B = 4;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
@@ -1629,7 +1743,7 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
     double Ri[9], tu_art[3] = {0, 0, 0}, damp[3] = {0, 0, 0};
@@ -1875,7 +1989,7 @@ __global__ __launch_bounds__(256) void k_coupled(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i], rhoi = S.rho[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double si[6], t[3][3];
 #pragma unroll
@@ -1917,7 +2031,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     double xi[3], xj[3], dx[3], w[4];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double sigi[6];
     #pragma unroll
@@ -2227,7 +2341,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double tu[3] = {0, 0, 0}, dn = 0.0;
     const double xsame = ti == 1 ? P.xsph[0] : ti == 2 ? P.xsph[1] : ti == 3 ? P.xsph[2] : P.xsph[1];
@@ -2434,7 +2548,7 @@ __global__ __launch_bounds__(256) void k_slip_tran(State S, Params P, int cur) {
     load_x<NDIM>(S, cur, i, xi);
     const double x3[3] = {S.xp[cur][0][i], S.xp[cur][1][i], S.xp[cur][2][i]};
     const double hli = S.hl[i], dr = P.dr;
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double ts[4][6], tv[4][3], td[4];
 #pragma unroll
@@ -2515,7 +2629,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     load_x<NDIM>(S, cur, i, xi);
     if (ti == 0 && P.flbndy == 2) {
         const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
+    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
         const int nc = nb_count(S, i);
         double tv[4][3], td[4] = {0, 0, 0, 0};
         for (int a = 0; a < 4; a++) tv[a][0] = tv[a][1] = tv[a][2] = 0.0;

@@ -471,6 +471,62 @@ static int or_partisearching(or_ctx *c) {
 /* neighbours actually stored (ninflu may exceed the 99 slots before error 5 is raised) */
 static inline int nbcount(const OCell *pc) { return pc->ninflu < 99 ? pc->ninflu : 99; }
 
+/* kernel_gradient_correction (CPP_3-NNPS.cpp:376-496, flknl 3 / 4): for soil and structure
+ * particles, L = sum over soil, boundary and structure neighbours of V_j (x_j - x_i) (x) grad W,
+ * each listed gradient replaced by L^-1 grad W; L^-1 by cofactors, the identity when
+ * |det L| <= 1e-5 (inverse_mat_2D/3D, :498-559); in 2D row and column 2 of L^-1 stay zero. */
+static void or_kgc(or_ctx *c) {
+    int n = c->n, dim = c->pro.ndim, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double A[3][3] = {{0}}, B[3][3] = {{0}}, det, x[3], w[3];
+        int nc = nbcount(&c->pc[i]), nj, k, l;
+        if (pi->type != 2 && pi->type != 4) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            const OPar *pj = &c->p[c->pc[i].influ[nj]];
+            const double *g = c->pc[i].wij[nj];
+            if (!(pj->type == 2 || pj->type == 0 || pj->type == 4)) continue;
+            for (k = 0; k < dim; k++) x[k] = pj->xp[k] - pi->xp[k];
+            for (k = 0; k < dim; k++)
+                for (l = 0; l < dim; l++) A[k][l] = A[k][l] + pj->mass / pj->rho * x[k] * g[l];
+        }
+        if (dim == 2) {
+            det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+            if (fabs(det) > 1.0e-5) {
+                B[0][0] = A[1][1] / det;
+                B[0][1] = -A[0][1] / det;
+                B[1][0] = -A[1][0] / det;
+                B[1][1] = A[0][0] / det;
+            } else {
+                B[0][0] = B[1][1] = 1.0;
+            }
+        } else {
+            det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[1][0] * (A[0][1] * A[2][2] - A[0][2] * A[2][1]) +
+                  A[2][0] * (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+            if (fabs(det) > 1.0e-5) {
+                B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+                B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+                B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+                B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+                B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+                B[1][2] = (A[1][0] * A[0][2] - A[0][0] * A[1][2]) / det;
+                B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+                B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+                B[2][2] = (A[0][0] * A[1][1] - A[1][0] * A[0][1]) / det;
+            } else {
+                B[0][0] = B[1][1] = B[2][2] = 1.0;
+            }
+        }
+        for (nj = 1; nj <= nc; nj++) {
+            double *g = c->pc[i].wij[nj];
+            for (k = 0; k < 3; k++) w[k] = B[k][0] * g[0] + B[k][1] * g[1] + B[k][2] * g[2];
+            for (k = 0; k < 3; k++) g[k] = w[k];
+        }
+    }
+}
+
+
 /* virtual velocity of boundary neighbour j as seen by particle i (CPP_6-density.cpp:86-120;
  * the same selection is repeated in CPP_7-1-strain.cpp:92-125 and CPP_11-xsph.cpp:187-219). */
 static void bndy_vel(const or_ctx *c, int i, int j, int use_vx, double vxj[3]) {
@@ -2702,7 +2758,7 @@ static void or_free_solid(or_ctx *c) {
 /* ------------------------------------------------------------------ one step */
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
-    if (P->flknl > 2 || P->flr || P->fleob || P->flv || P->flts) return 1;
+    if (P->flknl < 1 || P->flknl > 4 || P->flr || P->fleob || P->flv || P->flts) return 1;
     if (P->fli < 0 || P->fli > 3 || P->ss_type < 0 || P->ss_type > 3) return 1;
     if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 5 && P->flbndy != 13 &&
         P->flbndy != 14)
@@ -2720,6 +2776,7 @@ int or_phase(or_ctx *c, int mask) {
         c->t = c->t + c->pro.dt;
         if ((e = or_parttocell(c))) return e;
         if ((e = or_partisearching(c))) return e;
+        if (c->pro.flknl > 2) or_kgc(c);
     }
     if (mask & 2) {
         or_density(c);

@@ -24,6 +24,14 @@ from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype"})
 
 
+def with_flknl(deck, flknl):
+    """The deck with kernel function flknl (3: B-spline, 4: Yang-Liu, both with the kernel
+    gradient correction, CPP_1-SPHLoops.cpp:544-546)."""
+    d = deck.copy()
+    d.flknl = flknl
+    return d
+
+
 def decks():
     return {
         "column2d_small": (column_collapse_2d(nx=10, ny=20, wall_h=30, floor_len=30, steps=8), 8, (1, 4, 8)),
@@ -54,6 +62,12 @@ def decks():
         "drag_2d_fli3": (drag_box(2, 3, vis=0.02), 6, (1, 3, 6)),
         "drag_3d_fli2": (drag_box(3, 2, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, eps0=5.0), 5, (1, 5)),
         "drag_3d_fli3": (drag_box(3, 3, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, vis=0.02), 5, (1, 5)),
+        "kgc_column2d_k3": (with_flknl(column_collapse_2d(nx=10, ny=20, wall_h=30, floor_len=30, steps=8), 3), 8,
+                            (1, 4, 8)),
+        "kgc_zoo2d_k4": (with_flknl(model_zoo_2d(steps=6), 4), 6, (1, 3, 6)),
+        "kgc_ssi2d_k3": (with_flknl(ssi_block(2, 1, steps=6), 3), 6, (1, 3, 6)),
+        "kgc_art3d_k3": (with_flknl(art_stress_block(3, nx=4, ny=3, nz=4, steps=6), 3), 6, (1, 3, 6)),
+        "kgc_tran3d_k4": (with_flknl(tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 4), 5, (1, 5)),
     }
 
 

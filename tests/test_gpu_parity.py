@@ -356,7 +356,39 @@ def test_other_per_particle_materials_must_equal_their_slot():
         g.upload_done()
 
 
-@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN + FREE_SOLID + DRAG)
+KGC = ["kgc_column2d_k3", "kgc_zoo2d_k4", "kgc_ssi2d_k3", "kgc_art3d_k3", "kgc_tran3d_k4"]
+
+
+@pytest.mark.parametrize("name", KGC)
+def test_kernel_gradient_correction_vs_oracle(name):
+    # flknl 3 / 4: the correction matrix of every soil and structure particle after the search,
+    # applied to those particles' gradients in every later pass
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare_tol(g, o, 1e-9) == {}, f"step {s + 1}"
+    print(name, "bit-exact fields:", sorted(set(GPU_FIELDS) - set(compare(g, o))))
+    if name in ("kgc_column2d_k3", "kgc_ssi2d_k3"):  # no transcendental call on these paths
+        assert compare(g, o) == {}
+
+
+def test_kernel_gradient_correction_phase_entry_point():
+    # parttocell, partisearching, kernel_gradient_correction, then the passes: the fused step
+    deck, states = load_golden("kgc_column2d_k3")
+    a = make_gpu(deck, states[0])
+    b = make_gpu(deck, states[0])
+    for _ in range(3):
+        assert a.step(1) == 0
+        for ph in ("parttocell", "partisearching", "kernel_gradient_correction", "density", "strain",
+                   "strain_to_stress", "accel", "xsph", "update"):
+            assert b.call(ph) == 0, ph
+    assert compare(a, b) == {}
+
+
+@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN + FREE_SOLID + DRAG + KGC)
 def test_gpu_vs_reference_golden_models(name):
     # against the reference itself: within the oracle-Jacobi distance plus the device
     # transcendental allowance

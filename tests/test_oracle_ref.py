@@ -34,7 +34,8 @@ GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1"
           "art_stress_2d", "art_stress_3d", "ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3",
           "ssi_cpl_2d", "ssi_cpl_3d", "tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3",
           "tran_3d_b14", "free_solid_2d", "free_solid_3d", "drag_2d_fli2", "drag_2d_fli2_wenyu", "drag_2d_fli3",
-          "drag_3d_fli2", "drag_3d_fli3"]
+          "drag_3d_fli2", "drag_3d_fli3", "kgc_column2d_k3", "kgc_zoo2d_k4", "kgc_ssi2d_k3", "kgc_art3d_k3",
+          "kgc_tran3d_k4"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -193,3 +194,44 @@ def test_drag_laws_differ():
         assert s.step(3) == 0
         vx.append(s.get("vx"))
     assert not bitequal(vx[0], vx[1]) and not bitequal(vx[1], vx[2]) and not bitequal(vx[0], vx[2])
+
+
+def _corrected_moments(name):
+    """Per soil particle, M[k][m] = sum_j V_j (x_j - x_i)_k g'_m over the soil / boundary /
+    structure neighbours, g' the gradients the oracle holds after the search and the kernel
+    gradient correction of the fixture's first step."""
+    import ctypes as C
+    from helpers import oracle_lib
+    deck, states = load_golden(name)
+    st = states[0]
+    sim = OracleSim(deck, st, mode=GS)
+    assert sim.phase(1) == 0
+    L = oracle_lib().lib
+    L.or_get_neighbors.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]
+    L.or_get_wij.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int]
+    n, nd = deck.n, deck.ndim
+    cnt = np.zeros(n, np.int32)
+    ids = np.zeros((n, 99), np.int32)
+    L.or_get_neighbors(sim.ctx, cnt.ctypes.data_as(C.POINTER(C.c_int)), ids.ctypes.data_as(C.POINTER(C.c_int)), n)
+    vol = st["mass"] / st["rho"]
+    out = {}
+    buf = np.zeros((99, 4))
+    for i in np.flatnonzero(st["type"] == 2):
+        nc = L.or_get_wij(sim.ctx, int(i), buf.ctypes.data_as(C.POINTER(C.c_double)), 99)
+        js = ids[i, :nc]
+        keep = np.isin(st["type"][js], (0, 2, 4))
+        x = st["xp"][js][:, :nd] - st["xp"][i, :nd]
+        out[int(i)] = np.einsum("j,jk,jm->km", vol[js][keep], x[keep], buf[:nc][keep][:, :nd])
+    return deck, out
+
+
+@pytest.mark.parametrize("name", ["kgc_column2d_k3", "kgc_art3d_k3"])
+def test_kernel_gradient_correction_restores_first_order_consistency(name):
+    # the corrected gradients differentiate linear fields exactly: sum_j V_j x_ij (x) g'_ij = I
+    # (the correction matrix is built from the same sum with the uncorrected gradients, which is
+    # symmetric up to rounding), wherever the matrix was invertible
+    deck, mom = _corrected_moments(name)
+    eye = np.eye(deck.ndim)
+    err = [np.abs(m - eye).max() for m in mom.values()]
+    assert len(err) > 40 and max(err) < 1e-9, max(err)
+
