@@ -844,6 +844,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
     if (phase_mask & 16) {
         if (c->pre_on) LAUNCH("accel_pre", (k_accel_pre<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.coupled) LAUNCH("coupled", (k_coupled<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if (c->P.fli >= 2) LAUNCH("drag", (k_drag<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("accel", (k_accel<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.ss_type != 0) {
             LAUNCH("ssi_soil", k_ssi_soil<NDIM>, grid_for(n), c->S, c->P, cur);
@@ -864,6 +865,11 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
 
 static void enqueue(sphg_ctx *c, int mask) {
     const bool mixed = c->precision == SPHG_MIXED;
+    if (c->P.flknl > 2) {  // passes with the kernel gradient correction
+        if (c->P.ndim == 2) mixed ? enqueue_step<2, FastKgc>(c, mask) : enqueue_step<2, ExactKgc>(c, mask);
+        else mixed ? enqueue_step<3, FastKgc>(c, mask) : enqueue_step<3, ExactKgc>(c, mask);
+        return;
+    }
     if (c->P.ndim == 2) mixed ? enqueue_step<2, Fast>(c, mask) : enqueue_step<2, Exact>(c, mask);
     else mixed ? enqueue_step<3, Fast>(c, mask) : enqueue_step<3, Exact>(c, mask);
 }

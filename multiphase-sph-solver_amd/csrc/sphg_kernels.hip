@@ -149,17 +149,25 @@ struct KernelCtx {
 //   root and reciprocal; divisions inside the pair terms use the FP64 hardware reciprocal with
 //   one Newton step (relative error ~1e-15 instead of correctly rounded). Every sum over the
 //   neighbours, and all per-particle state, stay FP64.
+// kKgc: the passes apply the kernel gradient correction (flknl 3 / 4); a separate instantiation,
+// so that the correction costs the other kernel functions no registers.
 struct Exact {
-    static constexpr bool kFast = false;
+    static constexpr bool kFast = false, kKgc = false;
     __device__ static __forceinline__ double div(double a, double b) { return a / b; }
 };
 struct Fast {
-    static constexpr bool kFast = true;
+    static constexpr bool kFast = true, kKgc = false;
     __device__ static __forceinline__ double div(double a, double b) {
         double r = __builtin_amdgcn_rcp(b);
         r = fma(fma(-b, r, 1.0), r, r);
         return a * r;
     }
+};
+struct ExactKgc : Exact {
+    static constexpr bool kKgc = true;
+};
+struct FastKgc : Fast {
+    static constexpr bool kKgc = true;
 };
 
 template <int NDIM>
@@ -186,10 +194,10 @@ __device__ __forceinline__ KernelCtx kernel_ctx(const Params &P, double hli) {
 // The same for the pair passes of particle i: with flknl 3 / 4 the gradients of a soil or
 // structure particle's pairs are the corrected ones (kernel_gradient_correction,
 // CPP_3-NNPS.cpp:376-496, rewrites wij of those particles before every later pass reads them).
-template <int NDIM>
+template <int NDIM, class M>
 __device__ __forceinline__ KernelCtx kernel_ctx(const Params &P, double hli, const State &S, int i) {
     KernelCtx k = kernel_ctx<NDIM>(P, hli);
-    if (P.flknl > 2) {
+    if constexpr (M::kKgc) {
         const int t = S.type[i];
         if (t == 2 || t == 4) {
             k.kg = S.kgc;
@@ -259,7 +267,8 @@ __device__ __forceinline__ void pair_kernel(const Params &P, const KernelCtx &kc
                                             double &dst) {
     if constexpr (M::kFast) {
         pair_kernel_fast<NDIM>(P, kc, xi, xj, w, d2, dst);
-        if (kc.ci >= 0) kgc_apply(kc, w);
+        if constexpr (M::kKgc)
+            if (kc.ci >= 0) kgc_apply(kc, w);
         return;
     }
     double dx[3];
@@ -290,7 +299,8 @@ __device__ __forceinline__ void pair_kernel(const Params &P, const KernelCtx &kc
     if (s == 0.0 || near) w[3] = vn;
     else if (far) w[3] = bspline ? vf / 6.0 : vf;
     else w[3] = 0.0;
-    if (kc.ci >= 0) kgc_apply(kc, w);
+    if constexpr (M::kKgc)
+        if (kc.ci >= 0) kgc_apply(kc, w);
 }
 
 // hl of neighbour j: equal to hl_i for every pair (uniform smoothing length, checked by the host)
@@ -617,7 +627,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
     const bool solid = (ti == 2 || ti == 4), water = ti == 1;
@@ -699,7 +709,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double up = 0.0, dn = 0.0;
     constexpr int B = BW;
@@ -756,7 +766,7 @@ __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) 
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     const SoilSlot &sl = S.soil[S.soil_slot[i]];
     const double sat_cof = sl.sitas - sl.sitar;
@@ -811,7 +821,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     if (l == P.inip)
         for (int k = 0; k < 6; k++) S.eps[k][i] = 0.0;
@@ -1634,7 +1644,7 @@ __global__ __launch_bounds__(256) void k_regu(State S, Params P, int cur, int wh
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     double up[6] = {0, 0, 0, 0, 0, 0}, dn = 0.0;
     constexpr int B = 4;
     nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
@@ -1743,7 +1753,7 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
     double Ri[9], tu_art[3] = {0, 0, 0}, damp[3] = {0, 0, 0};
@@ -1989,7 +1999,7 @@ __global__ __launch_bounds__(256) void k_coupled(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i], rhoi = S.rho[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double si[6], t[3][3];
 #pragma unroll
@@ -2023,6 +2033,63 @@ __global__ __launch_bounds__(256) void k_coupled(State S, Params P, int cur) {
     for (int a = 0; a < 3; a++) S.cpl[a][i] = t[a][0] + t[a][1] + t[a][2];
 }
 
+// interact2 / interact3 pair sums (CPP_10-interact1.cpp:131-247, :286-402), fli 2 / 3: water and
+// soil (Ergun / Wen-Yu or Kozeny-Carman, drag_beta) and the interact1 air terms, in list order,
+// into interf; k_accel then applies the shared velocity-reversal clamp to them. A pass of its own
+// keeps the drag law's registers (and its pow calls) out of the momentum pass.
+template <int NDIM, class M>
+__global__ __launch_bounds__(256) void k_drag(State S, Params P, int cur) {
+    const int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    if (ti != 1 && ti != 2 && ti != 3) return;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
+    const int nc = nb_count(S, i);
+    const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    const double rhoi = S.rho[i], pori = S.porosity[i], satui = S.satu[i], fdi = S.fdens[i];
+    double tu[3] = {0.0, 0.0, 0.0};
+    constexpr int B = 1;
+    nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        const int j = nb_id(e[0]), tj = nb_type(e[0]);
+        if (act < 1 || tj == ti) return;
+        const bool water_soil = (ti == 2 && tj == 1) || (ti == 1 && tj == 2);
+        const bool air_soil = (ti == 2 && tj == 3) || (ti == 3 && tj == 2);
+        if (!water_soil && !air_soil) return;
+        double xj[3];
+        load_xj<NDIM>(S, cur, j, xj);
+        const double vj[3] = {S.vxp[0][j], S.vxp[1][j], S.vxp[2][j]};
+        double inprod = 0.0, dv[3];
+#pragma unroll
+        for (int k = 0; k < NDIM; k++) {
+            dv[k] = vi[k] - vj[k];
+            inprod = inprod + (xi[k] - xj[k]) * dv[k];
+        }
+        if (!(inprod < 0)) return;
+        double w[4], d2, dst;
+        pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hli, w, d2, dst);
+        const double mj = S.mass[j], rj = S.rho[j], w3 = w[3];
+        if (water_soil) {
+            const double beta = ti == 2
+                ? drag_beta(P.fli, mag3<NDIM>(dv), pori, S.ps_ds[i], S.fluid[S.fluid_slot[j]].vis, S.fdens[j], rj)
+                : drag_beta(P.fli, mag3<NDIM>(dv), S.porosity[j], S.ps_ds[j], S.fluid[S.fluid_slot[i]].vis, fdi, rhoi);
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) tu[k] = tu[k] + M::div(mj, rj) * w3 * beta * dv[k];
+        } else {
+            // the soil particle's saturation and porosity, the air particle's density
+            const double sat = ti == 2 ? satui : S.satu[j], por = ti == 2 ? pori : S.porosity[j];
+            const double fd = ti == 2 ? S.fdens[j] : fdi;
+#pragma unroll
+            for (int k = 0; k < NDIM; k++)
+                tu[k] = tu[k] + M::div((1 - sat) * (1 - sat) * por * por * fd * P.gacc * dv[k] * 2000 * mj, rj) * w3;
+        }
+    });
+#pragma unroll
+    for (int k = 0; k < NDIM; k++) S.interf[k][i] = tu[k];
+}
+
 template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     int i = pidx();
@@ -2031,7 +2098,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     double xi[3], xj[3], dx[3], w[4];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double sigi[6];
     #pragma unroll
@@ -2070,17 +2137,17 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     // artificial stress (ti == 2) and geostress damping (ti 2 / 4, l <= inip): pair sums from k_accel_pre
     const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
-    const bool drag_on = P.fli >= 1;
+    const bool drag_on = P.fli == 1;  // fli 2 / 3: the pair sums come from k_drag
     if (ti == 1 || ti == 3) {
         // fluid_acceleration_water/air (+ minor variants), CPP_9-4-derivation_stress.cpp:12-351;
         // qsig of a fluid neighbour holds (porosity * sig) / rho / rho, the j factor of the sum
         double fi[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) fi[k] = (fpor * sigi[k]) / rhoi / rhoi;
-        const double fdi = S.fluid[S.fluid_slot[i]].dens, fvisi = S.fluid[S.fluid_slot[i]].vis;
+        const double fdi = S.fluid[S.fluid_slot[i]].dens;
         constexpr int B = 1;  // one neighbour per round trip: 3 waves per SIMD instead of 2
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
-        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satjb[B], porjb[B], copjb[B], dsjb[B];
+        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satjb[B], porjb[B], copjb[B];
 #pragma unroll
         for (int b = 0; b < B; b++) {
             const int j = nb_id(e[b]);
@@ -2095,12 +2162,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             satjb[b] = 0.0;
             porjb[b] = 0.0;
             copjb[b] = 1.0;
-            dsjb[b] = 1.0;
             if (drag_on) {
                 satjb[b] = S.satu[j];
                 porjb[b] = S.porosity[j];
                 copjb[b] = S.cop[j];
-                if (P.fli >= 2) dsjb[b] = S.ps_ds[j];
             }
         }
 #pragma unroll
@@ -2109,8 +2174,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             const int tj = nb_type(e[b]);
             const double *xj = xjb[b], *vj = vjb[b];
             double *qj = qjb[b];
-            const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satj = satjb[b], porj = porjb[b], copj = copjb[b],
-                         dsj = dsjb[b];
+            const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satj = satjb[b], porj = porjb[b], copj = copjb[b];
             double w[4], d2, dst;
             pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
             if (tj == ti || tj == fother) {
@@ -2137,12 +2201,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 }
                 if (inprod < 0) {
                     const double w3 = w[3];
-                    if (ti == 1 && P.fli >= 2) {
-                        // interact2 / interact3 water against soil (CPP_10-interact1.cpp:184-218, :343-372)
-                        const double beta = drag_beta(P.fli, mag3<NDIM>(dv), porj, dsj, fvisi, fdi, rhoi);
-#pragma unroll
-                        for (int k = 0; k < NDIM; k++) drag[k] = drag[k] + M::div(mj, rj) * w3 * beta * dv[k];
-                    } else if (ti == 1) {
+                    if (ti == 1) {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
                             drag[k] = drag[k] + M::div(M::div(satj * satj * porj * porj * fdi * P.gacc * dv[k], copj) * mj, rj) * w3;
@@ -2160,10 +2219,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         double qi[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
-        const double copi = S.cop[i], pori = S.porosity[i], dsi = (P.fli >= 2 && ti == 2) ? S.ps_ds[i] : 1.0;
+        const double copi = S.cop[i], pori = S.porosity[i];
         constexpr int B = 1;  // one neighbour per round trip: 3 waves per SIMD instead of 2
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
-        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B], fvjb[B];
+        double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B];
 #pragma unroll
         for (int b = 0; b < B; b++) {
             const int j = nb_id(e[b]);
@@ -2179,7 +2238,6 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             prewb[b] = S.prew[j];
             preab[b] = S.prea[j];
             fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
-            fvjb[b] = (P.fli >= 2 && ti == 2) ? S.fluid[S.fluid_slot[j]].vis : 1.0;
         }
 #pragma unroll
         for (int b = 0; b < B; b++) {
@@ -2187,7 +2245,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             const int tj = nb_type(e[b]);
             const double *xj = xjb[b], *vj = vjb[b], *qj = qjb[b];
             const double hlj = hljb[b], mj = mjb[b], rj = rjb[b], satu = satub[b], prewj = prewb[b],
-                         preaj = preab[b], fdj = fdjb[b], fvj = fvjb[b];
+                         preaj = preab[b], fdj = fdjb[b];
             double w[4], d2, dst;
             pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
             if (tj == ti) {
@@ -2225,12 +2283,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 }
                 if (inprod < 0) {
                     const double w3 = w[3];
-                    if (tj == 1 && P.fli >= 2) {
-                        // interact2 / interact3 soil against water (CPP_10-interact1.cpp:158-182, :311-340)
-                        const double beta = drag_beta(P.fli, mag3<NDIM>(dv), pori, dsi, fvj, fdj, rj);
-#pragma unroll
-                        for (int k = 0; k < NDIM; k++) drag[k] = drag[k] + M::div(mj, rj) * w3 * beta * dv[k];
-                    } else if (tj == 1) {
+                    if (tj == 1) {
 #pragma unroll
                         for (int k = 0; k < NDIM; k++)
                             drag[k] = drag[k] + M::div(M::div(satui * satui * pori * pori * fdj * P.gacc * dv[k], copi) * mj, rj) * w3;
@@ -2299,7 +2352,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         // velocity-reversal clamp of interact1-3 (CPP_10-interact1.cpp:93-108, :249-264, :404-419)
         #pragma unroll
         for (int k = 0; k < NDIM; k++) {
-            double itf = drag[k];
+            double itf = P.fli == 1 ? drag[k] : S.interf[k][i];
             double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * dt;
             double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - itf * P.interc / rhoi) * dt;
             if (vx1 * vx2 <= 0.0) {
@@ -2341,7 +2394,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double tu[3] = {0, 0, 0}, dn = 0.0;
     const double xsame = ti == 1 ? P.xsph[0] : ti == 2 ? P.xsph[1] : ti == 3 ? P.xsph[2] : P.xsph[1];
@@ -2548,7 +2601,7 @@ __global__ __launch_bounds__(256) void k_slip_tran(State S, Params P, int cur) {
     load_x<NDIM>(S, cur, i, xi);
     const double x3[3] = {S.xp[cur][0][i], S.xp[cur][1][i], S.xp[cur][2][i]};
     const double hli = S.hl[i], dr = P.dr;
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
     const int nc = nb_count(S, i);
     double ts[4][6], tv[4][3], td[4];
 #pragma unroll
@@ -2629,7 +2682,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     load_x<NDIM>(S, cur, i, xi);
     if (ti == 0 && P.flbndy == 2) {
         const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM>(P, hli, S, i);
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
         const int nc = nb_count(S, i);
         double tv[4][3], td[4] = {0, 0, 0, 0};
         for (int a = 0; a < 4; a++) tv[a][0] = tv[a][1] = tv[a][2] = 0.0;
