@@ -180,7 +180,9 @@ int sphg_strain(sphg_ctx *ctx);
  * UMCC, AMCC, Bui DP, FOCC: CPP_8-1/8-2/8-4/8-5) + stress_regu every step_regu steps (:385-418,
  * Jacobi form) + free-surface stress correction (:481-505) + Stress_eigValue
  * (CPP_7-1-strain.cpp:283-332) + the per-particle tensor of artificial_stress_2d/3d
- * (CPP_9-5-artificialstress.cpp:42-89, :161-227). */
+ * (CPP_9-5-artificialstress.cpp:42-89, :161-227); with flts >= 1 the TS transform
+ * TSSMP_before / TSSMP_after around the soil models (CPP_7-3-TS.cpp:12-106), its state readable
+ * as the fields "tssig" (6) and "tsqc". */
 int sphg_strain_to_stress(sphg_ctx *ctx);
 /* clAcce_Fun fluid/soil/structure accelerations (CPP_9-4-derivation_stress.cpp:12-505),
  * artificial_viscosity (CPP_9-3-artificialvis.cpp:12-99), the artificial-stress pair sum

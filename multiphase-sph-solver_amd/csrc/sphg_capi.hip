@@ -178,6 +178,12 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     D(S.ps_ds); D(S.ps_cop); D(S.ps_fai);
     S.kgc = nullptr;
     if (pro->flknl > 2) plan.push_back({(void **)&S.kgc, 9 * cap * 8});
+    for (int k = 0; k < 6; k++) S.tssig[k] = nullptr;
+    S.tsqc = nullptr;
+    if (pro->flts >= 1) {
+        for (int k = 0; k < 6; k++) D(S.tssig[k]);
+        D(S.tsqc);
+    }
     for (int k = 0; k < 3; k++) D(S.ps_dp[k]);
     I(S.type, cap); I(S.matype, cap); I(S.permtype, cap); I(S.etype, cap);
     plan.push_back({(void **)&S.soil_slot, cap});
@@ -329,6 +335,14 @@ static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, b
         for (int k = 0; k < 9; k++) out[k] = S.beta[k];
         ncomp = 9;
         return 0;
+    }
+    if (nm == "tssig") {
+        if (!S.tssig[0]) return -1;
+        return v6(S.tssig);
+    }
+    if (nm == "tsqc") {
+        if (!S.tsqc) return -1;
+        return s1(S.tsqc);
     }
     if (nm == "Rij") {
         if (!S.rij[0]) return -1;
@@ -533,7 +547,7 @@ static int fluid_slot_of(int type, int matype, int etype) {
 
 static int check_supported(const sphg_problem &p) {
     if (p.flknl < 1 || p.flknl > 4) return SPHG_E_UNSUPPORTED;  // 3 / 4: with kernel_gradient_correction
-    if (p.flr || p.fleob || p.flv || p.flts) return SPHG_E_UNSUPPORTED;
+    if (p.flr || p.fleob || p.flv) return SPHG_E_UNSUPPORTED;
     if (p.fli < 0 || p.fli > 3 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
     if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 5 && p.flbndy != 13 &&
         p.flbndy != 14)
@@ -596,6 +610,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.coupled = (pro.ss_type >= 1 && pro.ss_type <= 3 && pro.ss_acc_type == 2) ? 1 : 0;
     P.ss_zeta = pro.ss_zeta;
     P.flts = pro.flts;
+    P.alphats = pro.alphats;
     P.dt = pro.dt;
     P.dr = pro.dr;
     P.gx = pro.gx; P.gy = pro.gy; P.gz = pro.gz;

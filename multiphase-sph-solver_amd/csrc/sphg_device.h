@@ -60,6 +60,7 @@ struct Params {
     double struct_e, struct_poi;
     int flagum, step_regu;
     double art_stre_coe;
+    double alphats;       // alpha of the TS transform (flts >= 1)
     double move_vx[3];
     int hl_uniform;          // every particle has the same smoothing length (hl_j == hl_i bit for bit)
     int nreal;               // realisations (cell grids) in the context
@@ -87,6 +88,7 @@ struct State {
     double *ps_fai;       // per-particle friction angle (slot value unless set per particle)
     double *kgc;          // flknl 3 / 4: kernel-gradient correction L^-1 of particle i, row-major entry r
                           // at kgc[r * cap + i] (soil and structure particles; null otherwise)
+    double *tssig[6], *tsqc;  // TS transform (flts >= 1; null otherwise): stress after the transform, qc
     double *ps_dp[3];     // from ps_fai on the host: sin(phi') / sqrt(9 + 3 sin^2 phi'), cos(phi'),
                           // sqrt(9 + 3 sin^2 phi') with phi' = atan(tan(fai) / fs)
     int *type, *matype, *permtype, *etype;

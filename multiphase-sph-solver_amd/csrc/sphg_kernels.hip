@@ -1346,7 +1346,47 @@ __device__ __forceinline__ void camclay_update(const double sig[6], const double
 // caller's registers), eps, adps, e, porosity and the invariants of the soil/structure particles.
 // kFull adds the Cam-Clay family (1, 3) and FOCC (8); the host selects it only when a soil
 // particle uses one of them, so the common elastic / DP instantiation keeps a small register budget.
+
+// TSSMP_before (CPP_7-3-TS.cpp:12-64), flts >= 1, soil: Yao's transformed stress with the
+// Matsuoka-Nakai SMP criterion from last step's principal stresses strep, mean stress vsig and
+// deviatoric stress divq. The original stress goes to tssig until ts_after. Quirk kept: I2 adds
+// strep[2]*strep[1] where the invariant has strep[2]*strep[0].
+__device__ __noinline__ void ts_before(const State &S, const Params &P, int i, double sig[6]) {
+    const double q = S.divq[i], vs = S.vsig[i], p0 = S.strep[0][i], p1 = S.strep[1][i], p2 = S.strep[2][i];
+    const double I1 = -(p0 + p1 + p2), I2 = p0 * p1 + p1 * p2 + p2 * p1, I3 = -p0 * p1 * p2;
+    const double alpha = P.alphats;
+    double qc = 0.0;
+    if ((I1 * I1 - 3.0 * I2) >= 1.0e-7) {
+        const double temp = 3.0 * sqrt((I1 * I2 - I3) / (I1 * I2 - 9.0 * I3)) - 1.0;
+        if (fabs(temp) >= 1.0e-7)
+            qc = alpha * sqrt(I1 * I1 - 3.0 * I2) + 2.0 * (1.0 - alpha) * I1 / (3.0 * sqrt((I1 * I2 - I3) / (I1 * I2 - 9.0 * I3)) - 1.0);
+    }
+    S.tsqc[i] = qc;
+    double sx[6];
+    for (int k = 0; k < 6; k++) {
+        S.tssig[k][i] = sig[k];
+        sx[k] = k < 3 ? -sig[k] + vs : -sig[k];
+    }
+    if (q != 0.0)
+        for (int k = 0; k < 6; k++) sig[k] = k < 3 ? -I1 * 0.33333333 - qc / q * sx[k] : -qc / q * sx[k];
+}
+
+// TSSMP_after (CPP_7-3-TS.cpp:66-106): the stress rate back to the real space, the original stress
+// restored, tssig left holding the transformed one
+__device__ __noinline__ void ts_after(const State &S, int i, double sig[6], double dsig[6]) {
+    const double q = S.divq[i], qc = S.tsqc[i];
+    const double dv = (dsig[0] + dsig[1] + dsig[2]) / 3.0;
+    if (qc != 0.0)
+        for (int k = 0; k < 6; k++) dsig[k] = k < 3 ? dv + q / qc * (dsig[k] - dv) : q / qc * dsig[k];
+    for (int k = 0; k < 6; k++) {
+        const double t = S.tssig[k][i];
+        S.tssig[k][i] = sig[k];
+        sig[k] = t;
+    }
+}
+
 template <bool kFull>
+
 __device__ __forceinline__ void constitutive(const State &S, const Params &P, int i, int ti, int l, double sig[6],
                                              const double deps[6]) {
     const double dt = P.dt;
@@ -1390,6 +1430,7 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
         if (ti == 2) {
             const int m8 = S.matype[i];
             focc = (m8 == 8 || m8 == 18 || m8 == 28);
+            if (P.flts >= 1) ts_before(S, P, i, sig);
             const SoilSlot &s = S.soil[S.soil_slot[i]];
             int mt = P.inip > l ? 0 : model_of(S.matype[i]);
             double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * 0.3333333333;
@@ -1443,6 +1484,7 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
                 adps[a] = deps[a] - ee;
             }
         }
+        if (ti == 2 && P.flts >= 1) ts_after(S, i, sig, dsig);
         double eps[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];

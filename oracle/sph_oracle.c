@@ -27,6 +27,7 @@ typedef struct {
     double porosity, cd, poro_cop;
     double beta[3][3], beta_ts[3][3];
     double Rij[3][3]; /* clRaij (Class_particle.h:181-187) */
+    double tssig[6], tsqc; /* TS transform (Class_particle.h:54-55) */
     int type, matype, permtype, etype;
 } OPar;
 
@@ -185,6 +186,7 @@ static const FieldDesc PFIELDS[] = {
     FD(weps, 9), FD(adps, 6), FD(satu, 1), FD(dsatu, 1), FD(suct, 1), FD(cop, 1), FD(vsig, 1),
     FD(veps, 1), FD(divq, 1), FD(divr, 1), FD(meps, 1), FD(roue, 1), FD(rous, 1), FD(prew, 1),
     FD(prea, 1), FD(porosity, 1), FD(cd, 1), FD(poro_cop, 1), FD(beta, 9), FD(beta_ts, 9), FD(Rij, 9),
+    FD(tssig, 6), FD(tsqc, 1),
 };
 #undef FD
 #define SD(nm) {"ps_" #nm, offsetof(sphg_soil, nm), 1}
@@ -827,6 +829,48 @@ static void or_shearstrain(OPar *p) {
                 p->eps[5] * p->eps[5];
     p->meps = sqrt(2.0 * fabs(j2));
 }
+/* TSSMP_before (CPP_7-3-TS.cpp:12-64): transformed stress of Yao's TS method with the
+ * Matsuoka-Nakai SMP criterion from last step's principal stresses strep, mean stress vsig and
+ * deviatoric stress divq; the original stress is kept in tssig. Quirk: I2 adds strep[2]*strep[1]
+ * where the invariant has strep[2]*strep[0]. */
+static void or_ts_before(OPar *p, double alpha) {
+    double q = p->divq, qc = 0.0, sx[6], I1, I2, I3, temp;
+    int k;
+    p->tsqc = 0.0;
+    for (k = 0; k < 3; k++) sx[k] = -p->sig[k] + p->vsig;
+    for (k = 3; k < 6; k++) sx[k] = -p->sig[k];
+    I1 = -(p->strep[0] + p->strep[1] + p->strep[2]);
+    I2 = p->strep[0] * p->strep[1] + p->strep[1] * p->strep[2] + p->strep[2] * p->strep[1];
+    I3 = -p->strep[0] * p->strep[1] * p->strep[2];
+    if ((I1 * I1 - 3.0 * I2) >= 1.0e-7) {
+        temp = 3.0 * sqrt((I1 * I2 - I3) / (I1 * I2 - 9.0 * I3)) - 1.0;
+        if (fabs(temp) >= 1.0e-7)
+            qc = alpha * sqrt(I1 * I1 - 3.0 * I2) + 2.0 * (1.0 - alpha) * I1 / (3.0 * sqrt((I1 * I2 - I3) / (I1 * I2 - 9.0 * I3)) - 1.0);
+    }
+    p->tsqc = qc;
+    for (k = 0; k < 6; k++) p->tssig[k] = p->sig[k];
+    if (q != 0.0) {
+        for (k = 0; k < 3; k++) p->sig[k] = -I1 * 0.33333333 - qc / q * sx[k];
+        for (k = 3; k < 6; k++) p->sig[k] = -qc / q * sx[k];
+    }
+}
+
+/* TSSMP_after (CPP_7-3-TS.cpp:66-106): stress rate back to the real space; sig restored, tssig
+ * left holding the transformed stress */
+static void or_ts_after(OPar *p) {
+    double q = p->divq, qc = p->tsqc, dv = (p->dsig[0] + p->dsig[1] + p->dsig[2]) / 3.0, t[6];
+    int k;
+    if (qc != 0.0) {
+        for (k = 0; k < 3; k++) p->dsig[k] = dv + q / qc * (p->dsig[k] - dv);
+        for (k = 3; k < 6; k++) p->dsig[k] = q / qc * p->dsig[k];
+    }
+    for (k = 0; k < 6; k++) {
+        t[k] = p->tssig[k];
+        p->tssig[k] = p->sig[k];
+        p->sig[k] = t[k];
+    }
+}
+
 static void or_poro_cal(OPar *p, double dt) {
     double de = (1 + p->e) * (p->deps[0] + p->deps[1] + p->deps[2]) * dt, et = p->e + de;
     if (et < 0.0) et = 0.0;
@@ -1302,7 +1346,7 @@ static int or_strain_to_stress(or_ctx *c) {
                 erfs[a] = adps[a] = 0.0;
                 for (b = 0; b < 6; b++) D[a][b] = 0.0;
             }
-            if (c->pro.flts >= 1) known = 0; /* TS transform not restated */
+            if (c->pro.flts >= 1) or_ts_before(pi, c->pro.alphats);
             if (c->l < c->pro.inip) {
                 or_elastic(D, pi, s);
                 apply_tangent(pi, D);
@@ -1335,6 +1379,7 @@ static int or_strain_to_stress(or_ctx *c) {
             jaumann(pi);
             if (pi->matype == 8 || pi->matype == 18 || pi->matype == 28)
                 or_fractional_plastic(pi->dsig, pi->deps, adps, Cf);
+            if (c->pro.flts >= 1) or_ts_after(pi);
             for (k = 0; k < 6; k++) {
                 pi->adps[k] = pi->adps[k] + adps[k] * dt;
                 pi->eps[k] = pi->eps[k] + pi->deps[k] * dt;
@@ -2758,7 +2803,7 @@ static void or_free_solid(or_ctx *c) {
 /* ------------------------------------------------------------------ one step */
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
-    if (P->flknl < 1 || P->flknl > 4 || P->flr || P->fleob || P->flv || P->flts) return 1;
+    if (P->flknl < 1 || P->flknl > 4 || P->flr || P->fleob || P->flv) return 1;
     if (P->fli < 0 || P->fli > 3 || P->ss_type < 0 || P->ss_type > 3) return 1;
     if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 5 && P->flbndy != 13 &&
         P->flbndy != 14)

@@ -21,7 +21,7 @@ from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, movi
                     art_stress_block, slope_2d, ssi_block, tran_box, drag_box)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
-KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype"})
+KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype", "tssig", "tsqc"})
 
 
 def with_flknl(deck, flknl):
@@ -29,6 +29,13 @@ def with_flknl(deck, flknl):
     gradient correction, CPP_1-SPHLoops.cpp:544-546)."""
     d = deck.copy()
     d.flknl = flknl
+    return d
+
+
+def with_ts(deck, alphats):
+    """The deck with the TS transform (flts 1, TSSMP_before / _after around the soil models)."""
+    d = deck.copy()
+    d.flts, d.alphats = 1, alphats
     return d
 
 
@@ -67,6 +74,10 @@ def decks():
         "kgc_zoo2d_k4": (with_flknl(model_zoo_2d(steps=6), 4), 6, (1, 3, 6)),
         "kgc_ssi2d_k3": (with_flknl(ssi_block(2, 1, steps=6), 3), 6, (1, 3, 6)),
         "kgc_art3d_k3": (with_flknl(art_stress_block(3, nx=4, ny=3, nz=4, steps=6), 3), 6, (1, 3, 6)),
+        "ts_column2d": (with_ts(column_collapse_2d(nx=10, ny=20, wall_h=30, floor_len=30, steps=8), 0.5), 8,
+                        (1, 4, 8)),
+        "ts_art3d": (with_ts(art_stress_block(3, nx=4, ny=3, nz=4, steps=6), 0.3), 6, (1, 3, 6)),
+        "ts_drag3d": (with_ts(drag_box(3, 2, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 0.4), 5, (1, 5)),
         "kgc_tran3d_k4": (with_flknl(tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 4), 5, (1, 5)),
     }
 

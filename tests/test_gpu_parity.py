@@ -388,7 +388,25 @@ def test_kernel_gradient_correction_phase_entry_point():
     assert compare(a, b) == {}
 
 
-@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN + FREE_SOLID + DRAG + KGC)
+TS = ["ts_column2d", "ts_art3d", "ts_drag3d"]
+
+
+@pytest.mark.parametrize("name", TS)
+def test_ts_transform_vs_oracle(name):
+    # flts 1: TSSMP_before / _after around the soil models, the transformed stress kept in tssig
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare_tol(g, o, 1e-9) == {}, f"step {s + 1}"
+        assert nan_rel(g.get("tssig", 6), o.get("tssig", 6)) <= 1e-9 and nan_rel(g.get("tsqc"), o.get("tsqc")) <= 1e-9
+    # the transform calls sqrt and divides only: bit-exact
+    assert compare(g, o) == {} and bitequal(g.get("tssig", 6), o.get("tssig", 6))
+
+
+@pytest.mark.parametrize("name", ["model_zoo_2d", "art_stress_3d"] + SSI + TRAN + FREE_SOLID + DRAG + KGC + TS)
 def test_gpu_vs_reference_golden_models(name):
     # against the reference itself: within the oracle-Jacobi distance plus the device
     # transcendental allowance

@@ -35,7 +35,7 @@ GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1"
           "ssi_cpl_2d", "ssi_cpl_3d", "tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3",
           "tran_3d_b14", "free_solid_2d", "free_solid_3d", "drag_2d_fli2", "drag_2d_fli2_wenyu", "drag_2d_fli3",
           "drag_3d_fli2", "drag_3d_fli3", "kgc_column2d_k3", "kgc_zoo2d_k4", "kgc_ssi2d_k3", "kgc_art3d_k3",
-          "kgc_tran3d_k4"]
+          "kgc_tran3d_k4", "ts_column2d", "ts_art3d", "ts_drag3d"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -234,4 +234,17 @@ def test_kernel_gradient_correction_restores_first_order_consistency(name):
     eye = np.eye(deck.ndim)
     err = [np.abs(m - eye).max() for m in mom.values()]
     assert len(err) > 40 and max(err) < 1e-9, max(err)
+
+
+@pytest.mark.parametrize("name", ["ts_column2d", "ts_art3d", "ts_drag3d"])
+def test_ts_transform_state_is_the_references(name):
+    # TSSMP_before / _after: the transformed stress (tssig) and qc (tsqc) the reference leaves
+    deck, states = load_golden(name)
+    sim = OracleSim(deck, states[0], mode=GS)
+    done = 0
+    for s in sorted(k for k in states if k > 0):
+        assert sim.step(s - done) == 0
+        done = s
+        assert bitequal(sim.get("tssig", 6), states[s]["tssig"]) and bitequal(sim.get("tsqc"), states[s]["tsqc"])
+    assert (states[done]["tssig"] != states[done]["sig"]).any()
 
