@@ -38,7 +38,9 @@ enum {
     SPHG_E_HIP = 101,        /* HIP runtime error */
     SPHG_E_UNSUPPORTED = 102,/* flag combination this build does not implement */
     SPHG_E_CAPACITY = 103,   /* exceeds the context's capacity */
-    SPHG_E_NOCONVERGE = 104  /* K-L root bisection cannot reach |f| <= 1e-6 in FP64 (the reference never returns) */
+    SPHG_E_NOCONVERGE = 104, /* K-L root bisection cannot reach |f| <= 1e-6 in FP64 (the reference never returns) */
+    SPHG_E_NONFINITE = 105   /* random-field Cholesky factor not finite: the correlation matrix is indefinite beyond
+                                what the reference's clamp absorbs (the reference goes on with NaN properties) */
 };
 
 /* Arithmetic form of a context. SPHG_FP64 reproduces the reference's FP64 operations in their
@@ -369,7 +371,9 @@ int sphg_rf_cells(sphg_rf_field *f, int n, const double *xp, const int *type, in
  * values value[nreal][n] (the property's phase only; other particles keep fill). zeta holds
  * nreal x cell_total standard normals (the reference draws them from a clock-seeded generator,
  * CPP_17-3-Nonstationary-RF.cpp:32-42). L (may be null) receives the cell_total^2 factor,
- * row-major with zeros above the diagonal; factor_ms (may be null) the Cholesky's device time. */
+ * row-major with zeros above the diagonal; factor_ms (may be null) the Cholesky's device time.
+ * Returns SPHG_E_NONFINITE (the outputs are still written) when the factor holds an inf or NaN,
+ * where the reference would assign NaN properties without notice. */
 int sphg_rf_generate(const sphg_rf_field *f, int n, const double *xp, const int *type, int nreal,
                      const double *zeta, double fill, double *value, double *L, double *factor_ms);
 

@@ -6,6 +6,7 @@ device and host code: the FP64 kernels are meant to reproduce the reference's CP
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
 
@@ -18,6 +19,33 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def kernel_resources(remarks: str) -> dict:
+    """Per-kernel VGPRs / scratch bytes per lane / occupancy from the compiler's resource remarks."""
+    out, cur = {}, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            cur = m.group(1)
+            out[cur] = {}
+            continue
+        m = re.search(r"remark:\s+(VGPRs|ScratchSize \[bytes/lane\]|Occupancy \[waves/SIMD\]): (\d+)", line)
+        if m and cur:
+            out[cur][m.group(1).split()[0]] = int(m.group(2))
+    return out
+
+
+def check_no_scratch(remarks: str) -> None:
+    """Every device function must stay in registers: a kernel that spills to per-lane scratch
+    memory (a stack array whose address escapes, an oversized unrolled body) runs several times
+    slower, so such a build is refused."""
+    res = kernel_resources(remarks)
+    if not res:
+        raise RuntimeError("build: no kernel resource remarks found")
+    bad = {k: v["ScratchSize"] for k, v in res.items() if v.get("ScratchSize", 0) > 0}
+    if bad:
+        raise RuntimeError("build: kernels use scratch memory: " + ", ".join(f"{k} {b} B/lane" for k, b in bad.items()))
 
 
 def build(verbose: bool = False) -> str:
@@ -33,7 +61,8 @@ def build(verbose: bool = False) -> str:
         host + [os.path.join(CSRC, "sph_kl_host.cpp"), "-o", obj_kl],
         host + [os.path.join(CSRC, "sph_io.cpp"), "-o", obj_io],
         host + [os.path.join(CSRC, "sph_driver.cpp"), "-o", obj_drv],
-        [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "sphg_capi.hip"), "-o", obj_hip],
+        [HIPCC] + HIP_FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-c", os.path.join(CSRC, "sphg_capi.hip"),
+                               "-o", obj_hip],
         [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp, obj_kl, obj_io, obj_drv],
     ]
     for c in cmds:
@@ -43,6 +72,8 @@ def build(verbose: bool = False) -> str:
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"build failed: {' '.join(c[:3])} ...")
+        if any("kernel-resource-usage" in a for a in c):
+            check_no_scratch(r.stderr)
     os.replace(LIB + ".tmp", LIB)
     # the project runner executable, linked against the library beside it
     r = subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "sphg_main.cpp"), "-o", SOLVE,

@@ -846,7 +846,8 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         }
     }
     if (phase_mask & 8) {
-        auto ks = c->full_models ? k_stress<true> : k_stress<false>;
+        auto ks = c->P.flts >= 1 ? (c->full_models ? k_stress<3> : k_stress<2>)
+                                 : (c->full_models ? k_stress<1> : k_stress<0>);
         if (c->regu_on) {
             LAUNCH("stress", ks, grid_for(n), c->S, c->P, 1);
             LAUNCH("regu_copy", k_regu_copy, grid_for(n), c->S, c->P, 1);
@@ -1302,6 +1303,7 @@ extern "C" const char *sphg_error_string(int code) {
     case SPHG_E_UNSUPPORTED: return "flag combination not implemented";
     case SPHG_E_CAPACITY: return "capacity exceeded";
     case SPHG_E_NOCONVERGE: return "K-L root bisection did not converge";
+    case SPHG_E_NONFINITE: return "random-field factor not finite (indefinite correlation matrix)";
     default: return "unknown error";
     }
 }
@@ -1486,10 +1488,12 @@ static int rf_run(const sphg_rf_field *fin, int n, const double *xp, const int *
                  bE = align256((size_t)npad * rpad * 8), bC = align256((size_t)n * 4), bT = align256((size_t)n * 4),
                  bV = align256((size_t)vreal * n * 8), bP = align256((size_t)kRfSlices * npad * rpad * 8);
     char *pool = nullptr;
-    HIPCHK(hipMalloc(&pool, bA + bZ + bE + bC + bT + bV + bP));
+    HIPCHK(hipMalloc(&pool, bA + bZ + bE + bC + bT + bV + bP + 256));
     double *dA = (double *)pool, *dZ = (double *)(pool + bA), *dE = (double *)(pool + bA + bZ);
     int *dC = (int *)(pool + bA + bZ + bE), *dT = (int *)(pool + bA + bZ + bE + bC);
     double *dV = (double *)(pool + bA + bZ + bE + bC + bT), *dEp = (double *)(pool + bA + bZ + bE + bC + bT + bV);
+    int *dFlag = (int *)(pool + bA + bZ + bE + bC + bT + bV + bP);
+    int nonfinite = 0;
     hipStream_t st = nullptr, st2 = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr, ev_f = nullptr, ev_b = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
@@ -1548,6 +1552,10 @@ static int rf_run(const sphg_rf_field *fin, int n, const double *xp, const int *
         }
         if (e == hipSuccess) e = hipStreamWaitEvent(st, ev_b, 0);
         hipEventRecord(e1, st);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, dFlag, 1, 0);
+        hipLaunchKernelGGL(k_rf_nonfinite, dim3((unsigned)(((size_t)total * total + 255) / 256)), dim3(256), 0, st,
+                           (const double *)dA, npad, total, dFlag);
+        if (e == hipSuccess) e = hipMemcpyAsync(&nonfinite, dFlag, 4, hipMemcpyDeviceToHost, st);
         const size_t es = (size_t)npad * rpad;
         hipLaunchKernelGGL(k_rf_gemm<kProd>, dim3(npad / kRfB, rpad / kRfB, kRfSlices), dim3(256), 0, st, cA, npad,
                            (const double *)dZ, npad, dEp, rpad, 0, 0, 0, kslice, es);
@@ -1585,7 +1593,8 @@ static int rf_run(const sphg_rf_field *fin, int n, const double *xp, const int *
     if (st) hipStreamDestroy(st);
     if (st2) hipStreamDestroy(st2);
     hipFree(pool);
-    return e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
+    if (e != hipSuccess) return SPHG_E_HIP;
+    return nonfinite ? SPHG_E_NONFINITE : SPHG_OK;
 }
 
 // the spec of one property of the correlation-matrix method (nonstationary_random_2D/3D)

@@ -1351,7 +1351,7 @@ __device__ __forceinline__ void camclay_update(const double sig[6], const double
 // Matsuoka-Nakai SMP criterion from last step's principal stresses strep, mean stress vsig and
 // deviatoric stress divq. The original stress goes to tssig until ts_after. Quirk kept: I2 adds
 // strep[2]*strep[1] where the invariant has strep[2]*strep[0].
-__device__ __noinline__ void ts_before(const State &S, const Params &P, int i, double sig[6]) {
+__device__ __forceinline__ void ts_before(const State &S, const Params &P, int i, double sig[6]) {
     const double q = S.divq[i], vs = S.vsig[i], p0 = S.strep[0][i], p1 = S.strep[1][i], p2 = S.strep[2][i];
     const double I1 = -(p0 + p1 + p2), I2 = p0 * p1 + p1 * p2 + p2 * p1, I3 = -p0 * p1 * p2;
     const double alpha = P.alphats;
@@ -1373,7 +1373,7 @@ __device__ __noinline__ void ts_before(const State &S, const Params &P, int i, d
 
 // TSSMP_after (CPP_7-3-TS.cpp:66-106): the stress rate back to the real space, the original stress
 // restored, tssig left holding the transformed one
-__device__ __noinline__ void ts_after(const State &S, int i, double sig[6], double dsig[6]) {
+__device__ __forceinline__ void ts_after(const State &S, int i, double sig[6], double dsig[6]) {
     const double q = S.divq[i], qc = S.tsqc[i];
     const double dv = (dsig[0] + dsig[1] + dsig[2]) / 3.0;
     if (qc != 0.0)
@@ -1385,10 +1385,12 @@ __device__ __noinline__ void ts_after(const State &S, int i, double sig[6], doub
     }
 }
 
-template <bool kFull>
-
+// kVar bit 0 (kFull) adds the Cam-Clay family and FOCC; bit 1 (kTs) the TS transform (flts >= 1).
+// Each is its own instantiation, so a deck without them carries none of their registers.
+template <int kVar>
 __device__ __forceinline__ void constitutive(const State &S, const Params &P, int i, int ti, int l, double sig[6],
                                              const double deps[6]) {
+    constexpr bool kFull = (kVar & 1) != 0, kTs = (kVar & 2) != 0;
     const double dt = P.dt;
     // weps[a][b] from the three stored components (antisymmetric, zero diagonal)
     const double w01 = S.weps[0][i], w02 = S.weps[1][i], w12 = S.weps[2][i];
@@ -1430,7 +1432,7 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
         if (ti == 2) {
             const int m8 = S.matype[i];
             focc = (m8 == 8 || m8 == 18 || m8 == 28);
-            if (P.flts >= 1) ts_before(S, P, i, sig);
+            if (kTs) ts_before(S, P, i, sig);
             const SoilSlot &s = S.soil[S.soil_slot[i]];
             int mt = P.inip > l ? 0 : model_of(S.matype[i]);
             double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * 0.3333333333;
@@ -1484,7 +1486,7 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
                 adps[a] = deps[a] - ee;
             }
         }
-        if (ti == 2 && P.flts >= 1) ts_after(S, i, sig, dsig);
+        if (kTs && ti == 2) ts_after(S, i, sig, dsig);
         double eps[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
@@ -1646,7 +1648,7 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
 
 // part 0: constitutive update and post-processing in one pass; part 1: constitutive update only;
 // part 2: post-processing of the stress in memory (a stress regularisation pass sits between 1 and 2)
-template <bool kFull>
+template <int kVar>
 __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
     int i = pidx();
     if (i >= P.n) return;
@@ -1658,7 +1660,7 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
         double deps[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) deps[k] = S.deps[k][i];
-        constitutive<kFull>(S, P, i, ti, step_of(S), sig, deps);
+        constitutive<kVar>(S, P, i, ti, step_of(S), sig, deps);
         if (part == 1) {
             if (ti != 0 && ti != 7)
                 for (int k = 0; k < 6; k++) S.sig[k][i] = sig[k];
@@ -2882,8 +2884,10 @@ template __global__ void k_update<3, Exact>(State, Params, int);
 template __global__ void k_update<3, Fast>(State, Params, int);
 template __global__ void k_regu<2, Exact>(State, Params, int, int);
 template __global__ void k_regu<2, Fast>(State, Params, int, int);
-template __global__ void k_stress<false>(State, Params, int);
-template __global__ void k_stress<true>(State, Params, int);
+template __global__ void k_stress<0>(State, Params, int);
+template __global__ void k_stress<1>(State, Params, int);
+template __global__ void k_stress<2>(State, Params, int);
+template __global__ void k_stress<3>(State, Params, int);
 template __global__ void k_saturation<2, Exact>(State, Params, int);
 template __global__ void k_saturation<2, Fast>(State, Params, int);
 template __global__ void k_saturation<3, Exact>(State, Params, int);

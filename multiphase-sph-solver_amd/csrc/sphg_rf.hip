@@ -233,6 +233,16 @@ __global__ __launch_bounds__(256) void k_rf_sum_slices(const double *part, int n
     E[e] = s;
 }
 
+// Sets *flag when any entry of the n x n lower triangle of the factor is not finite: beyond what the
+// clamp sqrt(max(d, 1e-6)) absorbs (an indefinite correlation matrix) the factor degenerates to
+// inf / NaN, and every property value drawn from it would be NaN.
+__global__ __launch_bounds__(256) void k_rf_nonfinite(const double *A, int ld, int n, int *flag) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const int r = (int)(e / n), c = (int)(e % n);
+    if (c <= r && !isfinite(A[(size_t)r * ld + c])) atomicOr(flag, 1);
+}
+
 // How the cell values E become one property: phase, column of E, the vertical axis and the tops
 // of the depth trend for the normal and the lognormal branch, and the lognormal map.
 struct RfValues {

@@ -44,7 +44,8 @@ def errors(gm, gd, fields=FIELDS):
 
 
 @pytest.mark.parametrize("name", ["column2d_small", "soilwater3d_small", "model_zoo_2d", "art_stress_3d",
-                                  "moving_box_m1", "ssi_2d_t2", "ssi_cpl_3d", "tran_2d_b14", "free_solid_3d"])
+                                  "moving_box_m1", "ssi_2d_t2", "ssi_cpl_3d", "tran_2d_b14", "free_solid_3d",
+                                  "kgc_zoo2d_k4", "kgc_tran3d_k4", "drag_3d_fli2", "ts_drag3d"])
 def test_mixed_one_step_from_identical_state(name):
     deck, states = load_golden(name)
     gd = make(deck, states[0], 0)

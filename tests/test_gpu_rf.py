@@ -4,9 +4,12 @@ Bars (FP64; the device factor is blocked and sums in another order than the refe
 column-by-column Cholesky, so it is not bit-equal):
 - factor of a positive definite correlation matrix (no clamped diagonal): max |L_gpu - L_oracle|
   <= 1e-12 (entries of L are at most 1); closed form of a separable field <= 1e-11 at 1,600 cells;
-- where the clamp sqrt(max(d, 1e-6)) acts (nearly singular or indefinite matrices) the factor is
-  not a continuous function of rounding, so only the clamp pattern and L L^T on the unclamped
-  columns are compared (<= 1e-9);
+- where the clamp sqrt(max(d, 1e-6)) acts on a nearly singular matrix the factor is not a
+  continuous function of rounding past the first clamped column, so the clamp pattern (which
+  diagonals are clamped) must be equal and the columns before the first clamp agree (<= 1e-9);
+- an indefinite matrix whose factor degenerates to inf / NaN despite the clamp (the reference then
+  assigns NaN properties) is refused with SPHG_E_NONFINITE (105); the oracle's factor of the same
+  matrix is confirmed non-finite;
 - per-particle values <= 1e-9 field-max relative (device log/pow/exp);
 - every realisation of a batch equals that realisation generated alone, bit for bit;
 - the reference's error codes 41 and 42.
@@ -20,7 +23,9 @@ from sphgpy.capi import rf_field, rf_generate
 
 pytestmark = pytest.mark.gpu
 
-CLAMPED = ("gauss2d_smooth", "rot2d")
+CLAMPED = ("gauss2d_smooth",)
+NONFINITE = ("rot2d",)
+SPHG_E_NONFINITE = 105
 
 
 @pytest.mark.parametrize("name", sorted(CHOL_CASES))
@@ -29,17 +34,24 @@ def test_factor_matches_oracle(name):
     total = cells["total"]
     zeta = np.random.default_rng(3).standard_normal(total)
     rc, value, Lg, ms = rf_generate(f, d.xp, d.type, zeta, want_l=True)
-    assert rc == 0
     rco, vo, _, Lo, _ = oracle_rf(d.ndim, d.xp, d.type, f, zeta, want_l=True)
     assert rco == 0
+    if name in NONFINITE:
+        assert not np.isfinite(Lo).all()
+        assert rc == SPHG_E_NONFINITE
+        return
+    assert rc == 0
+    assert np.isfinite(Lg).all() and np.isfinite(Lo).all()
     assert np.all(np.triu(Lg, 1) == 0.0)
     clamp_g = np.diag(Lg) == np.sqrt(1e-6)
     clamp_o = np.diag(Lo) == np.sqrt(1e-6)
     print(name, total, "max |dL|", np.abs(Lg - Lo).max(), "clamped", int(clamp_o.sum()), int(clamp_g.sum()))
     if name in CLAMPED:
         assert clamp_o.any()
+        assert np.array_equal(clamp_g, clamp_o)
         first = int(np.argmax(clamp_o))  # before the first clamped column the factors agree
         assert np.abs(Lg[:, :first] - Lo[:, :first]).max() <= 1e-9
+        assert np.isfinite(value).all()
         return
     assert not clamp_o.any()
     assert np.abs(Lg - Lo).max() <= 1e-12
