@@ -1391,7 +1391,8 @@ extern "C" int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const dou
         const int tot = npad * kKlTermsHost;
         hipLaunchKernelGGL(sphg::k_kl_phi, dim3((tot + 255) / 256), dim3(256), 0, st, dxp, dty, dm, kp, npad, dphi);
         hipEventRecord(e0, st);
-        hipLaunchKernelGGL(sphg::k_kl_gemm, dim3(npad / 32, rpad / 32), dim3(64), 0, st, dphi, dxi, rpad, dh);
+        hipLaunchKernelGGL(sphg::k_kl_gemm, dim3(npad / 32, (rpad + sphg::kKlCols - 1) / sphg::kKlCols), dim3(256), 0,
+                           st, dphi, dxi, rpad, dh);
         hipEventRecord(e1, st);
         const int totv = n * nreal;
         hipLaunchKernelGGL(sphg::k_kl_values, dim3((totv + 255) / 256), dim3(256), 0, st, dh, dxp, dty, kp, nreal,

@@ -81,28 +81,56 @@ __global__ __launch_bounds__(256) void k_kl_phi(const double *xp, const int *typ
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// H[npad][rpad] = Phi[npad][200] * Xi[200][rpad]; one 64-lane wave per 32 x 32 tile, K in steps of
-// 2 (v_mfma_f32_32x32x2_f32: lane l supplies A[l % 32][l / 32] and B[l / 32][l % 32] and holds
-// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] in accumulator r). npad and rpad are multiples of 32.
-__global__ __launch_bounds__(64) void k_kl_gemm(const float *phi, const float *xi, int rpad, float *h) {
-    const int lane = threadIdx.x;
-    const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+// H[npad][rpad] = Phi[npad][200] * Xi[200][rpad] on the matrix cores (v_mfma_f32_32x32x2_f32:
+// lane l supplies A[l % 32][k0 + l / 32] and B[k0 + l / 32][l % 32] and holds
+// D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32] in accumulator q). A workgroup of four waves owns 32
+// rows of H and kKlCols columns: the 32 x 200 block of Phi is one contiguous 25.6 KB stretch, staged
+// into LDS with coalesced loads, and each wave multiplies it with its kKlCols / 4 columns
+// of Xi (read from L2, 128 contiguous bytes per half-wave). Phi is read from HBM once and every
+// A fragment serves kKlTiles tiles. npad and rpad are multiples of 32; grid.y covers the columns
+// in groups of kKlCols.
+constexpr int kKlTiles = 2;                 // 32-column tiles per wave
+constexpr int kKlCols = 4 * 32 * kKlTiles;  // columns per workgroup
+
+__global__ __launch_bounds__(256) void k_kl_gemm(const float *phi, const float *xi, int rpad, float *h) {
+    // rows padded to kKlTerms + 1 floats: the 32 lanes of a half-wave read 32 rows at one k, and an
+    // odd row stride puts them in 32 different banks
+    constexpr int kLd = kKlTerms + 1;
+    __shared__ float a_lds[32 * kLd];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int m0 = blockIdx.x * 32;
+    const float *src = phi + (size_t)m0 * kKlTerms;
+    for (int q = tid; q < 32 * kKlTerms; q += 256) a_lds[(q / kKlTerms) * kLd + q % kKlTerms] = src[q];
+    __syncthreads();
     const int r = lane & 31, kk = lane >> 5;
-    f32x16 acc;
+    const int c0 = blockIdx.y * kKlCols + wv * 32 * kKlTiles;
+    f32x16 acc[kKlTiles];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] = 0.0f;
-    const float *arow = phi + (size_t)(m0 + r) * kKlTerms + kk;
-    const float *bcol = xi + (size_t)kk * rpad + n0 + r;
+    for (int t = 0; t < kKlTiles; t++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[t][q] = 0.0f;
+    bool live[kKlTiles];
+#pragma unroll
+    for (int t = 0; t < kKlTiles; t++) live[t] = c0 + 32 * t < rpad;
+    if (!live[0]) return;  // no barrier follows
 #pragma unroll 4
     for (int k0 = 0; k0 < kKlTerms; k0 += 2) {
-        const float a = arow[k0];
-        const float b = bcol[(size_t)k0 * rpad];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        const float a = a_lds[r * kLd + k0 + kk];
+#pragma unroll
+        for (int t = 0; t < kKlTiles; t++) {
+            if (!live[t]) continue;
+            const float b = xi[(size_t)(k0 + kk) * rpad + c0 + 32 * t + r];
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
+        }
     }
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int row = 8 * (q / 4) + 4 * kk + (q % 4);
-        h[(size_t)(m0 + row) * rpad + n0 + r] = acc[q];
+    for (int t = 0; t < kKlTiles; t++) {
+        if (!live[t]) continue;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int row = 8 * (q / 4) + 4 * kk + (q % 4);
+            h[(size_t)(m0 + row) * rpad + c0 + 32 * t + r] = acc[t][q];
+        }
     }
 }
 

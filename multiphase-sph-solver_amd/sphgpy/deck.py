@@ -603,6 +603,115 @@ def drag_box(ndim=2, fli=2, nx=8, ny=3, nz_soil=4, nz_water=3, dx=0.002, steps=6
     return deck
 
 
+def air_water_box(ndim=2, acc_type=0, nx=8, ny=3, nz_soil=3, nz_fluid=3, dx=0.002, steps=6, seed=61,
+                  eps0=0.6, perm=1e-4, v0=0.2, paa0=200.0) -> Deck:
+    """Three phases in a rigid-walled box (flbndy 2): unsaturated Cam-Clay soil (matype 3, void
+    ratio eps0, saturation starting at sitar) under water (type 1) on the left half and air
+    (type 3, fla 1) on the right half, with Darcy drag (fli 1, permtype 1) for both fluids
+    (CPP_10-interact1.cpp:15-110; the air term, x2000, scales with (1 - satu)^2 and so needs an
+    unsaturated soil). acc_type selects the fluids' momentum sums for water and air together
+    (0: fluid_acceleration_water / _air, 1: the _minorwater / _minorair forms,
+    CPP_9-4-derivation_stress.cpp:12-351). The left wall's particles are matype 1, the walls whose
+    pressure follows the Tait equation against their own rhop (CPP_7-1-eos_CR.cpp:37-41); air takes
+    the reference's equation of state with its comma operator (:31-35) from its initial pressure
+    paa0 (press_type 0). Fluids slide along x at v0 and the soil at -v0/2, so drag and viscosity
+    act from the first step."""
+    rng = np.random.default_rng(seed)
+    grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
+    d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, fla=1, fli=1, permtype=1,
+             pers=perm, perl=perm, perm=perm, interc=1.0, flbndy=2,
+             art_vis=((0.1, 0.0), (0.1, 0.0), (0.1, 0.0)), name=f"air_water_box_{ndim}d_a{acc_type}")
+    d.model_flags[3] = 1
+    d.soil_layers[3] = [_layer(3, zmf=1.2, sme0=4000.0, ramda=0.2, kapa=0.04, ann=0.5, bnn=0.5, beta=1.0, epss=0.6,
+                               epsr=0.9, sitas=1.0, sitar=0.3, sd=1.0, sw=1.0, kse=1.0, c1=1.0, c2=1.0, c3=1.0,
+                               e=0.5, eps0=eps0, cop=perm)]
+    w = default_fluid()
+    w.update(dens=1000.0, vis=1e-3, vpu=30.0, press_type=0, acc_type=acc_type, bhtype=0, pre_coe=1.0,
+             strain_min=1.0)
+    d.water = [dict(w), dict(w), dict(w)]
+    a = default_fluid()
+    a.update(dens=1.293, paa0=paa0, vis=1.8e-5, vpu=30.0, press_type=0, acc_type=acc_type, pre_coe=1.0)
+    d.air = a
+    top, hx = nz_soil + nz_fluid, nx // 2
+    if ndim == 2:
+        parts = [(_lattice2d(0, nx, 0, nz_soil, dx), 2, 3), (_lattice2d(0, hx, nz_soil, top, dx), 1, 0),
+                 (_lattice2d(hx, nx, nz_soil, top, dx), 3, 0),
+                 (_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0), (_lattice2d(-3, 0, 0, top + 3, dx), 0, 1),
+                 (_lattice2d(nx, nx + 3, 0, top + 3, dx), 0, 0)]
+    else:
+        parts = [(_lattice3d(0, nx, 0, ny, 0, nz_soil, dx), 2, 3), (_lattice3d(0, hx, 0, ny, nz_soil, top, dx), 1, 0),
+                 (_lattice3d(hx, nx, 0, ny, nz_soil, top, dx), 3, 0),
+                 (_lattice3d(-3, nx + 3, -3, ny + 3, -3, 0, dx), 0, 0),
+                 (_lattice3d(-3, 0, -3, ny + 3, 0, top + 3, dx), 0, 1),
+                 (_lattice3d(nx, nx + 3, -3, ny + 3, 0, top + 3, dx), 0, 0),
+                 (_lattice3d(0, nx, -3, 0, 0, top + 3, dx), 0, 0),
+                 (_lattice3d(0, nx, ny, ny + 3, 0, top + 3, dx), 0, 0)]
+    deck = _assemble(d, parts, rng)
+    fluid = (deck.type == 1) | (deck.type == 3)
+    deck.vxp[fluid, 0] = v0
+    deck.vxp[deck.type == 2, 0] = -0.5 * v0
+    return deck
+
+
+def slope_pile_3d(nx=200, ny=50, nz=100, dx=0.1, model=4, steps=1000, seed=71, pile=(10, 12), pile_x=0.6,
+                  cut=(0.5, 0.75, 0.1), pile_shift=(0.6, 0.0, 0.0), ss_type=2, myu=0.3, zeta=0.5, walls=3,
+                  v0=(0.0, 0.0, 0.0)) -> Deck:
+    """3D slope with an embedded pile (soil-structure interaction). Soil on an nx by ny by nz
+    lattice (x along the slope, y across, z up) with a 1:2 (vertical:horizontal) cut: the surface
+    is at nz for x below cut[0] nx, falls by one cell per two along x, and stays at cut[1] nz past
+    the face; soil is modified Cam-Clay, model 4 (AMCC, the reference's no-op tangent: lambda 0.2,
+    kappa 0.04, M 1.2, OCR 1.5, e0 0.8) or model 1 (SB sub-loading Cam-Clay with the same
+    constants). The pile is a pile[0] by pile[1] column of elastic structure particles
+    (type 4, E = 30 GPa, nu 0.2, rho 2500) centred across the slope at pile_x nx, from the base to
+    cut[2] nz above the surface there (below it for cut[2] < 0), in place of the soil it displaces,
+    shifted by pile_shift dx (by default 0.6 dx downslope, so that its downslope face sits 0.4 dx
+    from the soil and the contact acts from the first step); the soil-pile contact is Para_SSInter
+    type ss_type
+    (2: deformable, CPP_10-1-interact_ss.cpp:352-691). Rigid base and four side walls of `walls`
+    layers; seeded 1e-6 dx jitter; the soil starts at velocity v0 (the contact force is
+    m v_rel / dt, so a soil at rest against a pile at rest feels none). The defaults
+    (200 x 50 x 100 at dx = 0.1) give about 0.9 M soil and 12 k pile particles."""
+    rng = np.random.default_rng(seed)
+    d = Deck(ndim=3, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=steps, gravity=(0.0, 0.0, -9.81), flbndy=2,
+             ss_type=ss_type, ss_myu=myu, ss_zeta=zeta, ss_acc_type=1,
+             art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"slope_pile3d_m{model}_{nx}x{ny}x{nz}")
+    d.model_flags[model] = 1
+    cam = dict(poi=0.3, dens=2000.0, eps0=0.8, zmf=1.2, sme0=20000.0, fs=1.0, ramda=0.2, kapa=0.04, cop=1e-4,
+               e=0.8, ds=1.0, ocr=1.5, ann=0.5, mzm=1.0, azm=0.5, r0=1.0, bl=0.95, bz=1.0, zet=0.0)
+    d.soil_layers[model] = [_layer(model, **cam)]
+    d.structure = {"dens": 2500.0, "eps0": 0.0, "cop": 0.0, "e": 3.0e10, "poi": 0.2}
+    ix = np.arange(nx)
+    x0, ztoe = cut[0] * nx, int(round(cut[1] * nz))
+    surf = np.where(ix < x0, nz, np.maximum(ztoe, nz - np.floor((ix - x0) / 2.0).astype(np.int64)))
+    px0 = int(round(pile_x * nx)) - pile[0] // 2
+    py0 = ny // 2 - pile[1] // 2
+    ptop = int(surf[px0 + pile[0] // 2]) + int(round(cut[2] * nz))
+    inpile = np.zeros((nx, ny), bool)
+    inpile[px0:px0 + pile[0], py0:py0 + pile[1]] = True
+    cols = []
+    for i in range(nx):
+        for j in range(ny):
+            if not inpile[i, j]:
+                cols.append((i, j, int(surf[i])))
+    ci = np.array([c[0] for c in cols]); cj = np.array([c[1] for c in cols]); ch = np.array([c[2] for c in cols])
+    I = np.repeat(ci, ch); J = np.repeat(cj, ch)
+    K = np.concatenate([np.arange(h) for h in ch])
+    soil = (np.stack([I, J, K], axis=1) + 0.5) * dx
+    pile_p = _lattice3d(px0, px0 + pile[0], py0, py0 + pile[1], 0, ptop, dx)
+    pile_p += np.asarray(pile_shift, dtype=np.float64) * dx
+    hw = nz + walls
+    wl = walls
+    parts = [(soil, 2, model), (pile_p, 4, 0),
+             (_lattice3d(-wl, nx + wl, -wl, ny + wl, -wl, 0, dx), 0, 0),
+             (_lattice3d(-wl, 0, -wl, ny + wl, 0, hw, dx), 0, 0),
+             (_lattice3d(nx, nx + wl, -wl, ny + wl, 0, hw, dx), 0, 0),
+             (_lattice3d(0, nx, -wl, 0, 0, hw, dx), 0, 0),
+             (_lattice3d(0, nx, ny, ny + wl, 0, hw, dx), 0, 0)]
+    deck = _assemble(d, parts, rng)
+    deck.vxp[deck.type == 2] = v0
+    return deck
+
+
 def slope_2d(height=5.0, dx=0.1, steps=3000, seed=0, crest=1.5, toe=1.5, base=4.0, walls=3) -> Deck:
     """2D soil slope for the random-field batch: a crest of `crest`*H, a 1:2 (vertical:horizontal)
     face, a toe of `toe`*H and a `base` m layer under all of it, Drucker-Prager soil (mean

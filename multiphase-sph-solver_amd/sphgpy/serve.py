@@ -14,7 +14,7 @@ from typing import Callable, Dict, List, Optional, Tuple
 import numpy as np
 
 from .capi import GpuSim, kl_field, setting_par
-from .deck import Deck, column_collapse_2d, slope_2d, soil_water_column_3d
+from .deck import Deck, column_collapse_2d, slope_2d, slope_pile_3d, soil_water_column_3d
 
 SPHG_BUSY = -1
 BYTES_PER_PARTICLE = {2: 1200, 3: 1700}  # resident state + 99-entry lists, rounded up
@@ -141,17 +141,22 @@ class Server:
                 "p99_ttr_s": float(np.percentile(ttr, 99)), "max_ttr_s": float(ttr.max())}
 
 
+KINDS = ("column2d", "slope2d", "soilwater3d", "slopepile3d")
+
+
 def request_stream(n: int, rate: float, seed: int = 0, steps: Tuple[int, int] = (500, 5000),
-                   scale: float = 1.0) -> List[Request]:
-    """Poisson arrivals at `rate` per second of requests drawn by seed from three families:
-    2D column collapse (5k-50k particles), 2D random-field slope (20k-80k) and 3D soil-water
-    column (100k-500k). `scale` shrinks the particle counts (tests)."""
+                   scale: float = 1.0, cycle: bool = False) -> List[Request]:
+    """Poisson arrivals at `rate` per second of requests drawn by seed from four families:
+    2D column collapse (5k-50k particles), 2D random-field slope (20k-80k), 3D soil-water column
+    (100k-500k) and 3D slope with an embedded pile (0.5M-2M). `scale` shrinks the particle counts
+    (tests); `cycle` takes the families in turn instead of at random."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for rid in range(n):
         t += rng.exponential(1.0 / rate)
-        kind = ["column2d", "slope2d", "soilwater3d"][int(rng.integers(0, 3))]
+        pick = int(rng.integers(0, len(KINDS)))
+        kind = KINDS[rid % len(KINDS)] if cycle else KINDS[pick]
         st = int(rng.integers(steps[0], steps[1] + 1))
         s = int(rng.integers(0, 1 << 30))
         if kind == "column2d":
@@ -166,11 +171,17 @@ def request_stream(n: int, rate: float, seed: int = 0, steps: Tuple[int, int] = 
             dx = float(np.sqrt(slope_2d(h, dx=0.1).n * 0.01 / target))
             make = (lambda h=h, dx=dx, s=s: slope_2d(h, dx=dx, steps=10, seed=s))
             kl = (2, 10000.0, 3000.0, (10.0, 2.0), rng.standard_normal(200))
-        else:
+        elif kind == "soilwater3d":
             target = rng.uniform(1e5, 5e5) * scale
             k = max(2, int(round((target / 5.0) ** (1.0 / 3.0))))  # soil 4k^3 + water k^3 + walls
             make = (lambda k=k, s=s: soil_water_column_3d(nx=2 * k, ny=k, nz_soil=2 * k, nz_water=max(1, k // 2),
                                                           steps=10, seed=s))
+            kl = None
+        else:
+            target = rng.uniform(5e5, 2e6) * scale
+            k = max(4, int(round((target / 8.6) ** (1.0 / 3.0))))  # 4k x k x 2k lattice, 1:2 cut, walls
+            make = (lambda k=k, s=s: slope_pile_3d(nx=4 * k, ny=k, nz=2 * k, pile=(max(2, k // 5), max(2, k // 4)),
+                                                   steps=10, seed=s, v0=(-0.3, 0.1, -0.05)))
             kl = None
         out.append(Request(rid, kind, t, st, make, kl))
     return out

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 from helpers import run_ref, run_ref_kl, have_ref  # noqa: E402
 from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d,  # noqa: E402
-                    art_stress_block, slope_2d, ssi_block, tran_box, drag_box)
+                    art_stress_block, slope_2d, ssi_block, tran_box, drag_box, air_water_box,
+                    slope_pile_3d)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype", "tssig", "tsqc"})
@@ -37,6 +38,30 @@ def with_ts(deck, alphats):
     d = deck.copy()
     d.flts, d.alphats = 1, alphats
     return d
+
+
+def with_permtype(deck, permtype):
+    """The deck with the permeability law permtype (2: linear in the saturation between sitar and
+    sitas, 3: per-particle layers 1 / 2 / 3 -> pers / perm / perl; CPP_7-2-constitutive.cpp:373-378)."""
+    d = deck.copy()
+    d.permtype = permtype
+    return d
+
+
+def all_umcc(deck):
+    """The deck with every soil particle on the unsaturated Cam-Clay model (matype 3), the only
+    soil block that reads the saturation bounds sitas / sitar which permtype 2 interpolates between
+    (elsewhere both stay 0 and the reference's linear law divides 0 by 0)."""
+    d = deck.copy()
+    d.matype = d.matype.copy()
+    d.matype[d.type == 2] = 3
+    # bounds inside the saturation range, so that permtype 2 takes all three of its branches
+    d.soil_layers[3][0].update(sitar=0.5, sitas=0.9)
+    return d
+
+
+# soil velocity of the slope-with-pile fixtures: towards the pile's near face, with a sliding part
+PILE_V0 = (-0.3, 0.1, -0.05)
 
 
 def decks():
@@ -79,6 +104,17 @@ def decks():
         "ts_art3d": (with_ts(art_stress_block(3, nx=4, ny=3, nz=4, steps=6), 0.3), 6, (1, 3, 6)),
         "ts_drag3d": (with_ts(drag_box(3, 2, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 0.4), 5, (1, 5)),
         "kgc_tran3d_k4": (with_flknl(tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5), 4), 5, (1, 5)),
+        "air_water_2d_a0": (air_water_box(2, 0), 6, (1, 3, 6)),
+        "air_water_2d_a1": (air_water_box(2, 1), 6, (1, 3, 6)),
+        "air_water_3d_a0": (air_water_box(3, 0, nx=5, ny=3, steps=5), 5, (1, 5)),
+        "air_water_3d_a1": (air_water_box(3, 1, nx=5, ny=3, steps=5), 5, (1, 5)),
+        "tm_2d_b1": (tran_box(2, 1, steps=6, vdown=0.3), 6, (1, 3, 6)),
+        "tm_3d_b1": (tran_box(3, 1, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, vdown=0.3), 5, (1, 5)),
+        "umcc_perm2": (with_permtype(all_umcc(model_zoo_2d(steps=6)), 2), 6, (1, 3, 6)),
+        "zoo_perm3": (with_permtype(model_zoo_2d(steps=6), 3), 6, (1, 3, 6)),
+        "ssi_3d_t2": (ssi_block(3, 2, nx=5, ny=3, nz=3, steps=5), 5, (1, 5)),
+        "pile3d_amcc": (slope_pile_3d(nx=16, ny=8, nz=10, pile=(3, 3), model=4, steps=5, v0=PILE_V0), 5, (1, 3, 5)),
+        "pile3d_sb": (slope_pile_3d(nx=16, ny=8, nz=10, pile=(3, 3), model=1, steps=5, v0=PILE_V0), 5, (1, 3, 5)),
     }
 
 

@@ -225,7 +225,7 @@ def test_artificial_stress_vs_oracle(name):
     assert np.abs(o.get("Rij")).max() > 0.0  # the artificial stress is active in this deck
 
 
-SSI = ["ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t3", "ssi_cpl_2d", "ssi_cpl_3d"]
+SSI = ["ssi_2d_t1", "ssi_2d_t2", "ssi_2d_t3", "ssi_3d_t1", "ssi_3d_t2", "ssi_3d_t3", "ssi_cpl_2d", "ssi_cpl_3d"]
 
 
 @pytest.mark.parametrize("name", SSI)
@@ -507,3 +507,100 @@ def test_time_loop_driver_outputs_and_matches_stepping():
     seen2 = []
     rc, t2 = g2.run(8, 4, deck.dt * 2.5, 0.0, lambda l, tm: seen2.append(l))
     assert rc == 0 and seen2 == [1, 4] and g2.L.lib.sphg_current_step(g2.ctx) == 4
+
+
+AIR = ["air_water_2d_a0", "air_water_2d_a1", "air_water_3d_a0", "air_water_3d_a1"]
+
+
+@pytest.mark.parametrize("name", AIR)
+def test_air_water_vs_oracle(name):
+    # water + air: the air equation of state with its comma operator, matype-1 (Tait) walls,
+    # fluid_acceleration_water / _air (acc_type 0) or _minorwater / _minorair (acc_type 1), and
+    # interact1's air drag. The water EOS calls pow on the device: 1e-9 field-max relative
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare_tol(g, o, TRANSCENDENTAL_TOL) == {}, f"step {s + 1}"
+    air = deck.type == 3
+    assert bitequal(g.get("pre")[air], o.get("pre")[air])
+    assert np.abs(g.get("interf")[air]).max() > 0.0
+
+
+@pytest.mark.parametrize("name", ["tm_2d_b1", "tm_3d_b1"])
+def test_takeda_morris_boundary_bit_exact_vs_oracle(name):
+    # flbndy 1: tmboundary's repulsion on particles approaching wall particles
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare(g, o) == {}, f"step {s + 1}"
+
+
+@pytest.mark.parametrize("name", ["umcc_perm2", "zoo_perm3"])
+def test_permeability_laws_vs_oracle(name):
+    # permtype 2 (saturation-linear, on unsaturated Cam-Clay soil) and 3 (per-particle layers)
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(6):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare_tol(g, o, TRANSCENDENTAL_TOL) == {}, f"step {s + 1}"
+    soil = deck.type == 2
+    assert bitequal(g.get("cop")[soil], o.get("cop")[soil])
+
+
+@pytest.mark.parametrize("name", ["pile3d_amcc", "pile3d_sb"])
+def test_slope_pile_vs_oracle(name):
+    # the 3D slope-with-pile flag set at fixture size: modified Cam-Clay soil (AMCC no-op tangent,
+    # or SB with its log hardening), elastic pile, deformable soil-structure contact in 3D
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        if name == "pile3d_amcc":  # no transcendental call on this path
+            assert compare(g, o) == {}, f"step {s + 1}"
+        else:
+            assert compare_tol(g, o, TRANSCENDENTAL_TOL) == {}, f"step {s + 1}"
+        if s == 0:
+            f = g.get("interfss")
+            assert np.abs(f[deck.type == 2]).max() > 0.0 and np.abs(f[deck.type == 4]).max() > 0.0
+
+
+def test_slope_pile_full_size_properties():
+    # the full 1.1 M-particle 3D slope with the embedded pile (0.9 M AMCC soil, 11 k pile, walls):
+    # one step against the oracle from the same state (neighbour lists of an x-slab bit-exact,
+    # fields within 1e-9), then more steps with the state finite and the contact reactions balanced
+    # (the structure receives minus what the soil receives)
+    from golden.make_golden import PILE_V0
+    from sphgpy import slope_pile_3d
+    deck = slope_pile_3d(v0=PILE_V0, steps=4)
+    st = setting_par(deck)
+    g = make_gpu(deck, st)
+    o = OracleSim(deck, st, mode=JACOBI, nthreads=16)
+    assert g.step(1) == 0 and o.step(1) == 0
+    cg, ig = g.neighbors()
+    co, io = o.neighbors()
+    assert np.array_equal(cg, co)
+    x = deck.xp[:, 0]
+    slab = (x > 0.55 * x.max()) & (x < 0.65 * x.max())
+    assert np.array_equal(ig[slab], io[slab])
+    del ig, io
+    assert compare_tol(g, o, 1e-9, ["xp", "vx", "rho", "sig", "ax", "interfss"]) == {}
+    print("bit-exact fields after one step:", sorted(set(["xp", "vx", "rho", "sig", "ax"]) -
+                                                     set(compare(g, o, ["xp", "vx", "rho", "sig", "ax"]))))
+    assert g.step(3) == 0
+    for k in ("xp", "vx", "sig", "rho"):
+        assert np.isfinite(g.get(k)).all(), k
+    f = g.get("interfss")
+    soil, pile = deck.type == 2, deck.type == 4
+    tot_s, tot_p = f[soil].sum(axis=0), f[pile].sum(axis=0)
+    print("soil particles in contact:", int((np.abs(f[soil]).sum(axis=1) > 0).sum()), tot_s, tot_p)
+    np.testing.assert_allclose(tot_p, -tot_s, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(tot_s).max()))

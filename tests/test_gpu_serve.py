@@ -31,3 +31,20 @@ def test_admission_waits_for_memory():
     assert stats["failed"] == 0
     order = sorted(reqs, key=lambda r: r.admitted)
     assert all(b.admitted >= a.finished for a, b in zip(order, order[1:]))
+
+
+def test_real_sizes_all_four_families_same_alone_and_in_the_mix():
+    # one request of each family at its real size (2D column 5k-50k, 2D random-field slope
+    # 20k-80k, 3D soil-water column 100k-500k, 3D slope with pile 0.5M-2M), few steps each,
+    # arriving together so that all four run at once on the device
+    reqs = request_stream(4, rate=1000.0, seed=7, steps=(3, 8), cycle=True)
+    assert sorted(r.kind for r in reqs) == ["column2d", "slope2d", "slopepile3d", "soilwater3d"]
+    alone = {r.rid: run_alone(r) for r in reqs}
+    stats = Server(hbm_budget=64e9, chunk=2, max_active=4).run(reqs)
+    assert stats["requests"] == 4 and stats["failed"] == 0
+    print({r.kind: r.n for r in reqs}, stats)
+    order = sorted(reqs, key=lambda r: r.admitted)
+    assert any(b.admitted < a.finished for a, b in zip(order, order[1:]))
+    for r in reqs:
+        for k, v in alone[r.rid].items():
+            assert bitequal(r.result[k], v), (r.rid, r.kind, k)

@@ -35,7 +35,9 @@ GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1"
           "ssi_cpl_2d", "ssi_cpl_3d", "tran_2d_b3", "tran_2d_b4", "tran_2d_b13", "tran_2d_b14", "tran_3d_b3",
           "tran_3d_b14", "free_solid_2d", "free_solid_3d", "drag_2d_fli2", "drag_2d_fli2_wenyu", "drag_2d_fli3",
           "drag_3d_fli2", "drag_3d_fli3", "kgc_column2d_k3", "kgc_zoo2d_k4", "kgc_ssi2d_k3", "kgc_art3d_k3",
-          "kgc_tran3d_k4", "ts_column2d", "ts_art3d", "ts_drag3d"]
+          "kgc_tran3d_k4", "ts_column2d", "ts_art3d", "ts_drag3d", "air_water_2d_a0", "air_water_2d_a1",
+          "air_water_3d_a0", "air_water_3d_a1", "tm_2d_b1", "tm_3d_b1", "umcc_perm2", "zoo_perm3", "ssi_3d_t2",
+          "pile3d_amcc", "pile3d_sb"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -248,3 +250,59 @@ def test_ts_transform_state_is_the_references(name):
         assert bitequal(sim.get("tssig", 6), states[s]["tssig"]) and bitequal(sim.get("tsqc"), states[s]["tsqc"])
     assert (states[done]["tssig"] != states[done]["sig"]).any()
 
+
+
+@pytest.mark.parametrize("name", ["air_water_2d_a0", "air_water_3d_a1"])
+def test_air_and_tait_wall_pressures_closed_form(name):
+    # Air keeps the reference's comma operator (CPP_7-1-eos_CR.cpp:34): the pow call is never made,
+    # so p = 1.403 prep (-1) + prep = -0.403 prep whatever the density; matype-1 walls follow the
+    # Tait law prep ((rho / rhop)^7 - 1) + prep against their own rhop (:37-41). Both hold in the
+    # reference's values at every stored step.
+    deck, states = load_golden(name)
+    air = deck.type == 3
+    tait = (deck.type == 0) & (deck.matype == 1)
+    assert air.sum() > 0 and tait.sum() > 0
+    for s in sorted(k for k in states if k > 0):
+        st = states[s]
+        np.testing.assert_array_equal(st["pre"][air], 1.403 * st["prep"][air] * -1.0 + st["prep"][air])
+        r = st["rho"][tait] / st["rhop"][tait]
+        np.testing.assert_allclose(st["pre"][tait], st["prep"][tait] * (r ** 7 - 1.0) + st["prep"][tait],
+                                   rtol=1e-13)
+        assert np.abs(st["interf"][air]).max() > 0.0  # the x2000 air drag acts
+
+
+def test_permeability_laws_closed_form():
+    # permtype 3: per-particle layers 1 / 2 / 3 take pers / perm / perl (Class_particle.cpp:162-173);
+    # permtype 2: (perl - pers) satu / (sitas - sitar) + pers between the bounds (:147-158)
+    deck, states = load_golden("zoo_perm3")
+    soil = deck.type == 2
+    st = states[1]
+    want = np.select([deck.permtype_p == 1, deck.permtype_p == 2, deck.permtype_p == 3],
+                     [deck.pers, deck.perm, deck.perl], np.nan)
+    np.testing.assert_array_equal(st["cop"][soil], want[soil])
+    assert len(np.unique(st["cop"][soil])) == 3
+    deck, states = load_golden("umcc_perm2")
+    soil = deck.type == 2
+    st = states[3]
+    sitas, sitar = states[0]["ps_sitas"][soil], states[0]["ps_sitar"][soil]
+    sat = st["satu"][soil]
+    lin = (deck.perl - deck.pers) * sat / (sitas - sitar) + deck.pers
+    np.testing.assert_allclose(st["cop"][soil], lin, rtol=1e-15)
+
+
+def test_takeda_morris_wall_slows_approach():
+    # flbndy 1 (tmboundary, CPP_13-boundary.cpp:125-177): for a particle approaching a wall particle
+    # (v . x < 0 with x = x_i - x_j) the velocity gains (1 - beta) x (v . x) / |x|^2 with
+    # beta = min(1.5, 1 + 0.3 dr / (|x| - 0.3 dr)) >= 1, i.e. it loses approach speed; running the
+    # same deck with flbndy 2 (no such correction) must give a larger approach speed at the floor
+    deck, states = load_golden("tm_2d_b1")
+    st1 = OracleSim(deck, states[0], mode=GS)
+    assert st1.step(1) == 0
+    d2 = deck.copy()
+    d2.flbndy = 2
+    st2 = OracleSim(d2, states[0], mode=GS)
+    assert st2.step(1) == 0
+    low = ((deck.type == 1) | (deck.type == 2)) & (deck.xp[:, 1] < 1.5 * deck.dr)
+    assert low.sum() > 0
+    assert np.all(st1.get("vx")[low, 1] >= st2.get("vx")[low, 1])
+    assert np.any(st1.get("vx")[low, 1] > st2.get("vx")[low, 1])
