@@ -340,6 +340,15 @@ int sphg_kl_synthesize_part(sphg_ctx *ctx, const sphg_kl_field *f, const double 
 int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const double *xp, const int *type, int nreal,
                              const double *xi, double fill, double *value, double *gemm_ms);
 
+/* The same synthesis as sphg_kl_synthesize_batch with no device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). All device buffers live in one static array of about 32 MiB per
+ * GPU, held until the process ends, through which the particles pass 8192 at a time; nreal <= 256
+ * (SPHG_E_CAPACITY beyond). Results equal sphg_kl_synthesize_batch's bit for bit. Runs on the
+ * default stream of the current device; calls that overlap on one GPU share the array and race. */
+int sphg_kl_synthesize_batch_noalloc(const sphg_kl_field *f, int n, const double *xp, const int *type, int nreal,
+                                     const double *xi, double fill, double *value);
+
 /* ---- Correlation-matrix random fields (flspv 3, CPP_17-3-Nonstationary-RF.cpp) ------------- */
 
 /* One spatially variable property generated by the correlation-matrix method

@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -88,6 +89,7 @@ static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
 // -------------------------------------------------------------------------------- creation
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+static constexpr size_t align256c(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Cell grid of one simulation: celllist_ini1 (CPP_3-NNPS.cpp:96-122) and the nncell stencil of
 // celllist_ini2 (:125-171) in reference order. Returns false when the grid is empty or too large.
@@ -1409,6 +1411,102 @@ extern "C" int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const dou
     if (e1) hipEventDestroy(e1);
     if (st) hipStreamDestroy(st);
     hipFree(pool);
+    return e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
+}
+
+// ------------------------------------------------------ K-L batch synthesis without allocation
+// The same synthesis as sphg_kl_synthesize_batch for callers that cannot allocate device memory
+// (code under stream capture, tight serving loops): every buffer lives in one file-scope device
+// array, `scratch`, of kNaBytes, and the particles pass through it kNaChunk at a time. Each GPU
+// holds its own copy of the array; its address is looked up once per device, with that device
+// current, and only that address is handed to kernels and copies. Calls that overlap on one GPU
+// share the array and race.
+namespace {
+constexpr int kNaChunk = 8192;  // particles per pass (a multiple of 32)
+constexpr int kNaMaxReal = 256; // realisations per call
+constexpr size_t kNaXp = 0, kNaTy = kNaXp + align256c((size_t)kNaChunk * 24),
+                 kNaModes = kNaTy + align256c((size_t)kNaChunk * 4),
+                 kNaPhi = kNaModes + align256c(sizeof(sphg::KlModes)),
+                 kNaXiD = kNaPhi + align256c((size_t)kNaChunk * SPHG_KL_TERMS * 4),
+                 kNaXi = kNaXiD + align256c((size_t)kNaMaxReal * SPHG_KL_TERMS * 8),
+                 kNaH = kNaXi + align256c((size_t)SPHG_KL_TERMS * kNaMaxReal * 4),
+                 kNaV = kNaH + align256c((size_t)kNaChunk * kNaMaxReal * 4),
+                 kNaBytes = kNaV + align256c((size_t)kNaMaxReal * kNaChunk * 8);
+constexpr int kNaMaxDevices = 64;
+char *g_na_ptr[kNaMaxDevices];
+std::mutex g_na_mutex;
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNaBytes];
+
+namespace sphg {
+// Xi[200][rpad] (FP32, zero past nreal) from the callers' standard normals xi[nreal][200] (FP64)
+__global__ __launch_bounds__(256) void k_kl_xi(const double *xi, int nreal, int rpad, float *out) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= SPHG_KL_TERMS * rpad) return;
+    const int j = g / rpad, r = g % rpad;
+    out[g] = r < nreal ? (float)xi[(size_t)r * SPHG_KL_TERMS + j] : 0.0f;
+}
+}  // namespace sphg
+
+// Device address of this GPU's scratch array (looked up on the first call on each device).
+static int na_scratch(char **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDevices) return SPHG_E_HIP;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SPHG_E_HIP;
+        g_na_ptr[dev] = (char *)p;
+    }
+    *out = g_na_ptr[dev];
+    return SPHG_OK;
+}
+
+extern "C" int sphg_kl_synthesize_batch_noalloc(const sphg_kl_field *f, int n, const double *xp, const int *type,
+                                                int nreal, const double *xi, double fill, double *value) {
+    if (!xp || !type || !xi || !value || n <= 0 || nreal <= 0) return SPHG_E_ARG;
+    if (nreal > kNaMaxReal) return SPHG_E_CAPACITY;
+    int rc = kl_check(f);
+    if (rc) return rc;
+    char *base = nullptr;
+    rc = na_scratch(&base);
+    if (rc) return rc;
+    double *dxp = (double *)(base + kNaXp);
+    int *dty = (int *)(base + kNaTy);
+    sphg::KlModes *dm = (sphg::KlModes *)(base + kNaModes);
+    float *dphi = (float *)(base + kNaPhi), *dxi = (float *)(base + kNaXi), *dh = (float *)(base + kNaH);
+    double *dxid = (double *)(base + kNaXiD), *dv = (double *)(base + kNaV);
+    const int rpad = (nreal + 31) / 32 * 32;
+    sphg::KlModes m;
+    kl_modes_device(*f, nullptr, m);
+    const hipStream_t st = nullptr;
+    hipError_t e = hipMemcpyAsync(dm, &m, sizeof(m), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dxid, xi, (size_t)nreal * SPHG_KL_TERMS * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        hipLaunchKernelGGL(sphg::k_kl_xi, dim3((SPHG_KL_TERMS * rpad + 255) / 256), dim3(256), 0, st,
+                           (const double *)dxid, nreal, rpad, dxi);
+    for (int first = 0; first < n && e == hipSuccess; first += kNaChunk) {
+        const int cn = std::min(kNaChunk, n - first), cpad = (cn + 31) / 32 * 32;
+        sphg::KlParams kp = kl_params(*f, cn);
+        e = hipMemcpyAsync(dxp, xp + (size_t)first * 3, (size_t)cn * 24, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dty, type + first, (size_t)cn * 4, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) break;
+        const int tot = cpad * kKlTermsHost;
+        hipLaunchKernelGGL(sphg::k_kl_phi, dim3((tot + 255) / 256), dim3(256), 0, st, (const double *)dxp,
+                           (const int *)dty, (const sphg::KlModes *)dm, kp, cpad, dphi);
+        hipLaunchKernelGGL(sphg::k_kl_gemm, dim3(cpad / 32, (rpad + sphg::kKlCols - 1) / sphg::kKlCols), dim3(256), 0,
+                           st, (const float *)dphi, (const float *)dxi, rpad, dh);
+        hipLaunchKernelGGL(sphg::k_kl_values, dim3((cn * nreal + 255) / 256), dim3(256), 0, st, (const float *)dh,
+                           (const double *)dxp, (const int *)dty, kp, nreal, rpad, fill, dv);
+        e = hipGetLastError();
+        // value[r][first .. first + cn) from the chunk's value[r][0 .. cn)
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(value + first, (size_t)n * 8, dv, (size_t)cn * 8, (size_t)cn * 8, nreal,
+                                 hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);  // the next chunk reuses the buffers
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
 }
 
