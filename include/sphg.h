@@ -284,6 +284,14 @@ int sphg_load_state(sphg_ctx *ctx, const char *path);
  * as the true count (the reference keeps counting past the cap before raising error 5). */
 int sphg_get_neighbors(sphg_ctx *ctx, int *counts, int *ids, int n);
 
+/* Measurement tap with no counterpart in the reference: reversed = 1 reverses every neighbour list
+ * right after the search, so every pair sum of the following steps runs in the opposite order (and
+ * sphg_get_neighbors reports the reversed lists); 0 restores the reference order. The distance
+ * between the two FP64 runs is the noise floor of the summation order. Meant for decks whose
+ * results depend on the list only through its sums (not the soil-structure contact or Takeda-Morris
+ * wall searches, which pick the first listed structure or wall particles). */
+int sphg_set_list_order(sphg_ctx *ctx, int reversed);
+
 /* Per-kernel timing of the last sphg_step call set (HIP events on the context stream):
  * fills up to cap (name, total ms, launches) rows, returns the row count. */
 int sphg_kernel_times(sphg_ctx *ctx, char (*names)[32], double *ms, int *launches, int cap);

@@ -67,6 +67,7 @@ struct sphg_ctx {
     bool regu_on = false;  // regularisation passes are in the launch sequence
     bool pre_on = false;   // artificial-stress / geostress-damping pair pass is in the launch sequence
     bool full_models = false;  // some soil particle uses Cam-Clay, unsaturated Cam-Clay or FOCC
+    bool rev_lists = false;    // every neighbour list reversed after the search (noise-floor runs)
     // neighbours per load round trip of the density, normalisation, strain and XSPH passes
     // (sphg_autotune); each has the widths kBwChoices[k]
     int bw[4] = {2, 6, 2, 2};
@@ -828,6 +829,7 @@ static void enqueue_nnps(sphg_ctx *c) {
     LAUNCH("scatter", k_scatter, grid_for(n), c->S, c->P);
     LAUNCH("cell_order", k_cell_order, grid_for(M), c->S, M);
     LAUNCH("search", k_search<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (c->rev_lists) LAUNCH("reverse_lists", k_reverse_lists, grid_for(n), c->S, c->P);
     if (c->P.flknl > 2) LAUNCH("kgc", k_kgc<NDIM>, grid_for(n), c->S, c->P, cur);
 }
 
@@ -1276,6 +1278,15 @@ extern "C" int sphg_set_batch_widths(sphg_ctx *c, const int *widths) {
     }
     for (int k = 0; k < 4; k++) c->bw[k] = widths[k];
     drop_graphs(c);
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_list_order(sphg_ctx *c, int reversed) {
+    if (!c || (reversed != 0 && reversed != 1)) return SPHG_E_ARG;
+    if (c->rev_lists != (reversed == 1)) {
+        c->rev_lists = reversed == 1;
+        drop_graphs(c);
+    }
     return SPHG_OK;
 }
 

@@ -496,6 +496,22 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
 // (m_j / rho_j) (x_j - x_i) (x) grad W (uncorrected), and L^-1 by cofactors (inverse_mat_2D/3D,
 // :498-559: the identity when |det L| <= 1e-5; in 2D row and column 2 stay zero). The pair passes
 // apply L^-1 to every gradient of that particle's list (kgc_apply).
+// Reverses every particle's neighbour list in place (entries 0 .. count - 1). Every later pair
+// sum then runs in the opposite order: the same step with another valid summation order, whose
+// distance to the list-order run is the floating-point noise floor of the FP64 form.
+__global__ __launch_bounds__(256) void k_reverse_lists(State S, Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int c = nb_count(S, i);
+    int *lst = S.nbr + i;
+    const size_t cap = (size_t)P.cap;
+    for (int k = 0; k < c / 2; k++) {
+        const int a = lst[(size_t)k * cap], b = lst[(size_t)(c - 1 - k) * cap];
+        lst[(size_t)k * cap] = b;
+        lst[(size_t)(c - 1 - k) * cap] = a;
+    }
+}
+
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_kgc(State S, Params P, int cur) {
     const int i = pidx();

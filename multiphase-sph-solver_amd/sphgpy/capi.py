@@ -403,6 +403,8 @@ def product_lib() -> _Lib:
         L.sphg_autotune.argtypes = [C.c_void_p, _I, C.POINTER(C.c_int)]
         L.sphg_autotune.restype = _I
         L.sphg_set_batch_widths.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.sphg_set_list_order.argtypes = [C.c_void_p, _I]
+        L.sphg_set_list_order.restype = _I
         L.sphg_set_batch_widths.restype = _I
         L.sphg_particle_count.argtypes = [C.c_void_p]
         L.sphg_particle_count.restype = _I
@@ -544,6 +546,10 @@ class GpuSim(Sim):
         w = (C.c_int * 4)()
         rc = self.L.lib.sphg_autotune(self.ctx, steps, w)
         return int(rc), list(w)
+
+    def set_list_order(self, reversed_: bool) -> int:
+        """sphg_set_list_order: every neighbour sum in reversed list order (noise-floor runs)."""
+        return int(self.L.lib.sphg_set_list_order(self.ctx, 1 if reversed_ else 0))
 
     def set_batch_widths(self, widths) -> int:
         """sphg_set_batch_widths (density, normalisation, strain, XSPH)."""

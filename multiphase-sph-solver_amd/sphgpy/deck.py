@@ -628,7 +628,8 @@ def air_water_box(ndim=2, acc_type=0, nx=8, ny=3, nz_soil=3, nz_fluid=3, dx=0.00
     w = default_fluid()
     w.update(dens=1000.0, vis=1e-3, vpu=30.0, press_type=0, acc_type=acc_type, bhtype=0, pre_coe=1.0,
              strain_min=1.0)
-    d.water = [dict(w), dict(w), dict(w)]
+    # the deck file carries acc_type for the first water material only (CPP_2-initialf.cpp:417-420)
+    d.water = [dict(w), dict(w, acc_type=0), dict(w, acc_type=0)]
     a = default_fluid()
     a.update(dens=1.293, paa0=paa0, vis=1.8e-5, vpu=30.0, press_type=0, acc_type=acc_type, pre_coe=1.0)
     d.air = a
@@ -672,7 +673,10 @@ def slope_pile_3d(nx=200, ny=50, nz=100, dx=0.1, model=4, steps=1000, seed=71, p
     m v_rel / dt, so a soil at rest against a pile at rest feels none). The defaults
     (200 x 50 x 100 at dx = 0.1) give about 0.9 M soil and 12 k pile particles."""
     rng = np.random.default_rng(seed)
-    d = Deck(ndim=3, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=steps, gravity=(0.0, 0.0, -9.81), flbndy=2,
+    # time step from the pile's elastic wave speed sqrt(E / rho) = 3.5 km/s (the soils' 600 m/s
+    # would let the pile's stress waves grow without bound)
+    c_pile = float(np.sqrt(3.0e10 / 2500.0))
+    d = Deck(ndim=3, dr=dx, dt=0.2 * dx / c_pile, loop=steps, fop=steps, gravity=(0.0, 0.0, -9.81), flbndy=2,
              ss_type=ss_type, ss_myu=myu, ss_zeta=zeta, ss_acc_type=1,
              art_vis=((0.0, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"slope_pile3d_m{model}_{nx}x{ny}x{nz}")
     d.model_flags[model] = 1

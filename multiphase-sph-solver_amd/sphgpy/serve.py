@@ -136,7 +136,8 @@ class Server:
         makespan = now()
         ttr = np.array([r.ttr for r in done])
         work = float(sum(r.n * r.steps for r in done if not r.error))
-        return {"requests": len(done), "failed": int(sum(1 for r in done if r.error)), "makespan_s": makespan,
+        return {"requests": len(done), "failed": int(sum(1 for r in done if r.error)),
+                "errors": [(r.rid, r.kind, r.n, r.error) for r in done if r.error], "makespan_s": makespan,
                 "particle_steps_per_s": work / makespan, "p50_ttr_s": float(np.percentile(ttr, 50)),
                 "p99_ttr_s": float(np.percentile(ttr, 99)), "max_ttr_s": float(ttr.max())}
 
