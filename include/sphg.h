@@ -146,6 +146,47 @@ int sphg_set_field_i(sphg_ctx *ctx, const char *name, const int *host, int n);
 int sphg_get_field_i(sphg_ctx *ctx, const char *name, int *host, int n);
 int sphg_upload_done(sphg_ctx *ctx, int n);
 
+/* The reference's particle record, member for member and in the same order, so that an array of
+ * the reference's Particle (Class_particle.h:13-103, 131 doubles and 4 ints, 1064 bytes) can be
+ * passed where an array of sphg_particle is expected. */
+typedef struct {
+    double x[3], xp[3], vx[3], vxp[3], ax[3], axp[3], ux[3], strep[3];
+    double divx[3][3];
+    double interf[3], interfss[3];
+    double rho, rhop, mass, hl, pre, prep, e;
+    double eps[6], deps[6], sig[6], dsig[6], ff_sig[3];
+    double weps[3][3];
+    double adps[6], tssig[6];
+    double tsqc, satu, dsatu, suct, cop, vsig, veps, divq, divr, meps;
+    double roue, rous, zeta, rsta, prew, prea, porosity, cd, poro_cop;
+    double beta[3][3], beta_ts[3][3];
+    int type, matype, permtype, etype;
+} sphg_particle;
+
+/* Groups of Particle members for sphg_download_particles. */
+enum {
+    SPHG_PART_KINEMATICS = 1,   /* x xp vx vxp ax axp ux */
+    SPHG_PART_DENSITY = 2,      /* rho rhop mass hl pre prep prew prea e porosity poro_cop */
+    SPHG_PART_STRESS = 4,       /* eps deps sig adps strep vsig veps divq divr meps weps beta beta_ts tssig tsqc */
+    SPHG_PART_STATE = 8,        /* satu dsatu suct cop roue cd interf interfss */
+    SPHG_PART_IDS = 16,         /* type matype permtype etype */
+    SPHG_PART_ALL = 31
+};
+
+/* Upload of the host particle array as SPH_Calculation holds it (pPar, allocated at
+ * CPP_1-SPHLoops.cpp:354-378 and filled by input_par / setting_par): every member the device keeps
+ * (all but divx dsig ff_sig weps rous zeta rsta, which the reference recomputes each step or this
+ * library does not use; tssig / tsqc only where the TS transform is on), the same as one
+ * sphg_set_field / sphg_set_field_i call per member. sphg_upload_done() follows, as after
+ * sphg_set_field. The per-particle material copies (pParti_ConsPara, pVFluid) keep their
+ * "ps_<field>" / "pf_<field>" fields. */
+int sphg_upload_particles(sphg_ctx *ctx, const sphg_particle *p, int n);
+/* Download into the host particle array (what output_total, CPP_14-2-output.cpp:15-135, and the
+ * reference's other writers read): the members of the groups in `mask` that the device keeps
+ * (weps and beta_ts derived as sphg_get_field derives them); members it does not keep are left
+ * as they are. */
+int sphg_download_particles(sphg_ctx *ctx, sphg_particle *p, int n, unsigned mask);
+
 /* Host-side initial state: setting_vis_cons + setting_par (CPP_2-initialf.cpp:1021-1380) on host
  * arrays (row-major, n particles): fills rho rhop mass hl satu e porosity axp cd roue sig pre prep
  * suct and the per-particle material copies; pro->nboun etc. are not needed. Reads xp, type,

@@ -54,6 +54,7 @@ def build(verbose: bool = False) -> str:
     obj_kl = os.path.join(OUT_DIR, "sph_kl_host.o")
     obj_io = os.path.join(OUT_DIR, "sph_io.o")
     obj_drv = os.path.join(OUT_DIR, "sph_driver.o")
+    obj_par = os.path.join(OUT_DIR, "sph_particles.o")
     obj_hip = os.path.join(OUT_DIR, "sphg_capi.o")
     host = ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-c"]
     cmds = [
@@ -61,9 +62,11 @@ def build(verbose: bool = False) -> str:
         host + [os.path.join(CSRC, "sph_kl_host.cpp"), "-o", obj_kl],
         host + [os.path.join(CSRC, "sph_io.cpp"), "-o", obj_io],
         host + [os.path.join(CSRC, "sph_driver.cpp"), "-o", obj_drv],
+        host + [os.path.join(CSRC, "sph_particles.cpp"), "-o", obj_par],
         [HIPCC] + HIP_FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-c", os.path.join(CSRC, "sphg_capi.hip"),
                                "-o", obj_hip],
-        [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp, obj_kl, obj_io, obj_drv],
+        [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", obj_hip, obj_cpp, obj_kl, obj_io, obj_drv,
+                                                                                 obj_par],
     ]
     for c in cmds:
         if verbose:

@@ -491,11 +491,6 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     for (int k = 0; k < nk; k++) S.nbr[(size_t)k * P.cap + i] = stage[k][t];
 }
 
-// kernel_gradient_correction (CPP_3-NNPS.cpp:376-496, flknl 3 / 4), right after the search: for a
-// soil or structure particle L = sum over its soil, boundary and structure neighbours of
-// (m_j / rho_j) (x_j - x_i) (x) grad W (uncorrected), and L^-1 by cofactors (inverse_mat_2D/3D,
-// :498-559: the identity when |det L| <= 1e-5; in 2D row and column 2 stay zero). The pair passes
-// apply L^-1 to every gradient of that particle's list (kgc_apply).
 // Reverses every particle's neighbour list in place (entries 0 .. count - 1). Every later pair
 // sum then runs in the opposite order: the same step with another valid summation order, whose
 // distance to the list-order run is the floating-point noise floor of the FP64 form.
@@ -512,6 +507,11 @@ __global__ __launch_bounds__(256) void k_reverse_lists(State S, Params P) {
     }
 }
 
+// kernel_gradient_correction (CPP_3-NNPS.cpp:376-496, flknl 3 / 4), right after the search: for a
+// soil or structure particle L = sum over its soil, boundary and structure neighbours of
+// (m_j / rho_j) (x_j - x_i) (x) grad W (uncorrected), and L^-1 by cofactors (inverse_mat_2D/3D,
+// :498-559: the identity when |det L| <= 1e-5; in 2D row and column 2 stay zero). The pair passes
+// apply L^-1 to every gradient of that particle's list (kgc_apply).
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_kgc(State S, Params P, int cur) {
     const int i = pidx();
