@@ -68,6 +68,7 @@ struct sphg_ctx {
     bool pre_on = false;   // artificial-stress / geostress-damping pair pass is in the launch sequence
     bool full_models = false;  // some soil particle uses Cam-Clay, unsaturated Cam-Clay or FOCC
     bool rev_lists = false;    // every neighbour list reversed after the search (noise-floor runs)
+    bool eig_stale = false;    // strep / beta not written by the last stress pass (k_eig on read)
     // neighbours per load round trip of the density, normalisation, strain and XSPH passes
     // (sphg_autotune); each has the widths kBwChoices[k]
     int bw[4] = {2, 6, 2, 2};
@@ -195,6 +196,8 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     I(S.nbr, cap * kMaxNb);
     for (int k = 0; k < 6; k++) D(S.qsig[k]);
     D(S.fdens);
+    S.mrec = nullptr;
+    if (precision == SPHG_MIXED) plan.push_back({(void **)&S.mrec, cap * 4 * sizeof(float4)});
     // artificial-stress tensor (kept between steps) and the Jacobi regularisation copy
     for (int k = 0; k < 9; k++) {
         if (pro->art_stre_coe > 0.0) D(S.rij[k]);
@@ -365,12 +368,26 @@ static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, b
     return -1;
 }
 
+// strep / beta of the current stress when the last stress pass left them out (no consumer in the
+// step): the same Stress_eigValue of the same stress, so the values are those of the pass.
+static int ensure_eig(sphg_ctx *c) {
+    if (!c->eig_stale) return SPHG_OK;
+    hipLaunchKernelGGL(k_eig, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->S, c->P);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) return SPHG_E_HIP;
+    c->eig_stale = false;
+    return SPHG_OK;
+}
+
 // Fields derived on download from what the device keeps (weps: 3 stored components; beta_ts:
 // transpose of beta; x: equal to xp for every particle the update moves).
 extern "C" int sphg_get_field(sphg_ctx *c, const char *name, double *host, int n, int ncomp) {
     if (!c || !name || !host || n != c->n) return SPHG_E_ARG;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return SPHG_E_HIP;
     std::string nm(name);
+    if (nm == "strep" || nm == "beta" || nm == "beta_ts") {
+        const int rc = ensure_eig(c);
+        if (rc) return rc;
+    }
     const size_t N = (size_t)n;
     std::vector<double> buf(N);
     auto fetch = [&](double *d, std::vector<double> &h) {
@@ -635,6 +652,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.tmax = pro.threshold_max;
     P.struct_e = c->mat.structure.e;
     P.struct_poi = c->mat.structure.poi;
+    P.eig_in_step = (pro.flts >= 1 || pro.art_stre_coe > 0.0) ? 1 : 0;
     P.water_acc_minor = c->mat.water[0].acc_type != 0;
     P.air_acc_minor = c->mat.air.acc_type != 0;
     // realisation of every particle (batch contexts: realisation-major ranges of counts[r]) and
@@ -931,6 +949,7 @@ static int phase(sphg_ctx *c, int mask) {
     ensure_regu(c, 1);
     plan_pre(c);
     enqueue(c, mask);
+    if ((mask & 8) && !c->P.eig_in_step) c->eig_stale = true;
     if (mask & 64) {
         c->cur ^= 1;
         c->step++;
@@ -1011,6 +1030,7 @@ extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
             c->step++;
         }
         c->stepped = c->stepped || nsteps > 0;
+        if (nsteps > 0 && !c->P.eig_in_step) c->eig_stale = true;
         finish_timing(c);
         return read_err(c);
     }
@@ -1026,6 +1046,7 @@ extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
         c->step++;
     }
     c->stepped = c->stepped || nsteps > 0;
+    if (nsteps > 0 && !c->P.eig_in_step) c->eig_stale = true;
     return read_err(c);
 }
 
@@ -1080,6 +1101,7 @@ extern "C" int sphg_step_async(sphg_ctx *c, int nsteps) {
         c->step++;
     }
     c->stepped = c->stepped || nsteps > 0;
+    if (nsteps > 0 && !c->P.eig_in_step) c->eig_stale = true;
     return SPHG_OK;
 }
 
@@ -1144,6 +1166,7 @@ bool ckpt_stream(sphg_ctx *c, FILE *f, size_t off, size_t len, bool save) {
 }  // namespace
 
 extern "C" int sphg_save_state(sphg_ctx *c, const char *path) {
+    if (c && ensure_eig(c)) return SPHG_E_HIP;
     if (!c || !path || !c->uploaded) return SPHG_E_ARG;
     HIPCHK(hipStreamSynchronize(c->stream));
     const CkptHeader h = ckpt_header(c);
@@ -1176,6 +1199,7 @@ extern "C" int sphg_load_state(sphg_ctx *c, const char *path) {
     c->step = h.step;
     c->cur = h.cur;
     c->stepped = h.stepped != 0;
+    c->eig_stale = false;  // the file was written with strep / beta of its stress
     c->has_psc = h.has_psc != 0;
     return SPHG_OK;
 }

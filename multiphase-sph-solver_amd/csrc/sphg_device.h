@@ -67,6 +67,7 @@ struct Params {
     int fluid_por_uniform;   // every fluid particle's slot has the same porosity
     double ss_myu, ss_zeta;  // Para_SSInter friction coefficient and force factor
     int coupled;             // Para_SSInter acc_type 2 with an interaction type 1-3
+    int eig_in_step;         // principal stresses needed within the step (TS transform, artificial stress)
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -100,6 +101,11 @@ struct State {
     double *qsig[6];      // the j factor of the momentum sums: sig / rho / rho (solids),
                           // (porosity * sig) / rho / rho (fluids)
     double *fdens;        // density of the particle's fluid slot (drag partner density)
+    // mixed form only (null in FP64): what the momentum pass gathers of a neighbour, packed in
+    // FP32 as four 16-byte words per particle {m, rho, vxp0, vxp1} {vxp2, q0, q1, q2}
+    // {q3, q4, q5, satu} {prew, prea, porosity, cop}, q the j factor of the stress sums; written
+    // by the stress pass, so one neighbour costs four vector loads instead of fifteen scalar ones
+    float4 *mrec;
     double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
     double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
     // soil-structure interaction (ss_type 1-3): per soil / water particle the structure particles

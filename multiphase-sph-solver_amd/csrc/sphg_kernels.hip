@@ -1533,6 +1533,32 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
     }
 }
 
+// Stress_eigValue (CPP_7-1-strain.cpp:283-332) of particle i's final stress: cyclic Jacobi on the
+// 3 x 3 stress, principal stresses into strep, principal directions into beta; a and v return the
+// rotated matrix and the eigenvectors.
+__device__ __forceinline__ void principal_axes(const State &S, int i, const double sig[6], double a[9], double v[9]) {
+    const double m[9] = {sig[0], sig[3], sig[5], sig[3], sig[1], sig[4], sig[5], sig[4], sig[2]};
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = m[k];
+    eejcb3(a, v, 0.001, 7);
+    #pragma unroll
+    for (int k = 0; k < 3; k++) {
+        S.strep[k][i] = a[k * 3 + k];
+        for (int c = 0; c < 3; c++) S.beta[c * 3 + k][i] = v[k * 3 + c];  // beta[c][k] = v[3k+c]
+    }
+}
+
+// principal stresses and directions of every particle from the stress in memory (the values the
+// stress pass would have written at the end of the last step)
+__global__ __launch_bounds__(256) void k_eig(State S, Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    double sig[6], a[9], v[9];
+#pragma unroll
+    for (int k = 0; k < 6; k++) sig[k] = S.sig[k][i];
+    principal_axes(S, i, sig, a, v);
+}
+
 // free_surface_check (+ stress_adjust), the per-particle factor of the solid momentum sums,
 // principal stresses, and the artificial-stress tensor of particle i.
 __device__ __forceinline__ void stress_post(const State &S, const Params &P, int i, int ti, double sig[6]) {
@@ -1584,23 +1610,36 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
     // fluid sum multiplies by (the i particle's; otherwise the fluid pass forms it per pair)
     {
         const double r = S.rho[i];
+        double q[6];
         if ((ti == 1 || ti == 3) && P.fluid_por_uniform) {
             const double fp = S.fluid[S.fluid_slot[i]].porosity;
             #pragma unroll
-            for (int k = 0; k < 6; k++) S.qsig[k][i] = (fp * sig[k]) / r / r;
+            for (int k = 0; k < 6; k++) q[k] = (fp * sig[k]) / r / r;
         } else {
             #pragma unroll
-            for (int k = 0; k < 6; k++) S.qsig[k][i] = sig[k] / r / r;
+            for (int k = 0; k < 6; k++) q[k] = sig[k] / r / r;
+        }
+        #pragma unroll
+        for (int k = 0; k < 6; k++) S.qsig[k][i] = q[k];
+        if (S.mrec) {
+            // the fluid pass forms (porosity * sig) / rho / rho per pair when the fluids' porosities differ
+            const bool raw = (ti == 1 || ti == 3) && !P.fluid_por_uniform;
+            float qf[6];
+            #pragma unroll
+            for (int k = 0; k < 6; k++) qf[k] = (float)(raw ? sig[k] : q[k]);
+            float4 *o = S.mrec + 4 * (size_t)i;
+            o[0] = make_float4((float)S.mass[i], (float)r, (float)S.vxp[0][i], (float)S.vxp[1][i]);
+            o[1] = make_float4((float)S.vxp[2][i], qf[0], qf[1], qf[2]);
+            o[2] = make_float4(qf[3], qf[4], qf[5], (float)S.satu[i]);
+            o[3] = make_float4((float)S.prew[i], (float)S.prea[i], (float)S.porosity[i], (float)S.cop[i]);
         }
     }
-    // principal stresses (CPP_7-1-strain.cpp:283-332)
-    double a[9] = {sig[0], sig[3], sig[5], sig[3], sig[1], sig[4], sig[5], sig[4], sig[2]}, v[9];
-    eejcb3(a, v, 0.001, 7);
-    #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        S.strep[k][i] = a[k * 3 + k];
-        for (int c = 0; c < 3; c++) S.beta[c * 3 + k][i] = v[k * 3 + c];  // beta[c][k] = v[3k+c]
-    }
+    // principal stresses (CPP_7-1-strain.cpp:283-332). Only the TS transform (next step) and the
+    // artificial stress (below) read them inside the step; otherwise they are left to k_eig, run
+    // on the final stress when the host reads them, which gives the same values.
+    if (!P.eig_in_step) return;
+    double a[9], v[9];
+    principal_axes(S, i, sig, a, v);
     // artificial-stress tensor (CPP_9-5-artificialstress.cpp:42-89 in 2D, :161-227 in 3D)
     if (P.art_stre_coe > 0.0 && ti == 2) {
         const double pi = 3.141592653, xs = P.art_stre_coe, rho = S.rho[i];
@@ -2041,6 +2080,21 @@ __device__ __forceinline__ void bndy_stress_rows(const State &S, const Params &P
     for (int a = 0; a < 3; a++) rows[a] = t[a][0] + t[a][1] + t[a][2];
 }
 
+// A neighbour's packed momentum-pass record (mixed form), widened to FP64 for the arithmetic
+struct NbRec {
+    double m, r, v[3], q[6], satu, prew, prea, por, cop;
+};
+__device__ __forceinline__ NbRec load_nbrec(const State &S, int j) {
+    const float4 *p = S.mrec + 4 * (size_t)j;
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    NbRec o;
+    o.m = a.x; o.r = a.y; o.v[0] = a.z; o.v[1] = a.w;
+    o.v[2] = b.x; o.q[0] = b.y; o.q[1] = b.z; o.q[2] = b.w;
+    o.q[3] = c.x; o.q[4] = c.y; o.q[5] = c.z; o.satu = c.w;
+    o.prew = d.x; o.prea = d.y; o.por = d.z; o.cop = d.w;
+    return o;
+}
+
 // coupled_Water_Soil_Structure (CPP_9-4-derivation_stress.cpp:507-629), Para_SSInter acc_type 2:
 // stress-gradient sums across structure-soil and structure-water pairs, kept per row
 // (t[a][0] + t[a][1]) + t[a][2] for the momentum kernel to add after the boundary-stress term
@@ -2213,6 +2267,19 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xjb[b]);
             hljb[b] = hl_of(S, P, j, hli);
+            if constexpr (M::kFast) {
+                const NbRec rc = load_nbrec(S, j);
+                mjb[b] = rc.m;
+                rjb[b] = rc.r;
+#pragma unroll
+                for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? rc.v[k] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 6; k++) qjb[b][k] = rc.q[k];
+                satjb[b] = drag_on ? rc.satu : 0.0;
+                porjb[b] = drag_on ? rc.por : 0.0;
+                copjb[b] = drag_on ? rc.cop : 1.0;
+                continue;
+            }
             mjb[b] = S.mass[j];
             rjb[b] = S.rho[j];
             #pragma unroll
@@ -2288,6 +2355,20 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xjb[b]);
             hljb[b] = hl_of(S, P, j, hli);
+            fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
+            if constexpr (M::kFast) {
+                const NbRec rc = load_nbrec(S, j);
+                mjb[b] = rc.m;
+                rjb[b] = rc.r;
+#pragma unroll
+                for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? rc.v[k] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 6; k++) qjb[b][k] = rc.q[k];
+                satub[b] = rc.satu;
+                prewb[b] = rc.prew;
+                preab[b] = rc.prea;
+                continue;
+            }
             mjb[b] = S.mass[j];
             rjb[b] = S.rho[j];
             #pragma unroll
@@ -2297,7 +2378,6 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             satub[b] = S.satu[j];
             prewb[b] = S.prew[j];
             preab[b] = S.prea[j];
-            fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
         }
 #pragma unroll
         for (int b = 0; b < B; b++) {

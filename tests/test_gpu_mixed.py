@@ -1,11 +1,13 @@
-"""Mixed arithmetic form (SPHG_MIXED) against the FP64 form of the same library.
+"""Mixed arithmetic form (SPHG_MIXED) against the FP64 form of the same library and the oracle.
 
 The FP64 form is pinned bit-exact (or within 1e-9 where device transcendentals enter) to the
 oracle and through it to the reference, so the distance mixed -> FP64 is the distance mixed ->
 reference up to those bounds. Error metric per field: max_i |f_mixed - f_fp64| / max_i |f_fp64|
 (field-max normalised). Bars:
-  - one step from an identical state: <= 1e-6 on every state field (kernel values carry FP32
-    rounding, ~6e-8 relative; sums and state stay FP64);
+  - one step from an identical state: <= 1e-5 on every state field (kernel values carry FP32
+    rounding, ~6e-8 relative, and so do the neighbour fields the momentum sums gather from their
+    packed FP32 records; the stress-divergence sums cancel by up to two orders of magnitude, so
+    the acceleration carries up to ~2e-6 of its field maximum; sums and state stay FP64);
   - 100 steps of the 2D column collapse: <= 1e-4 on positions, velocities, stresses, densities;
   - neighbour lists after one step: identical (the search is the FP64 code in both forms);
   - a mixed realisation batch equals its realisations run alone in mixed form, bit for bit.
@@ -57,7 +59,7 @@ def test_mixed_one_step_from_identical_state(name):
         d = np.abs(gm.get(k) - gd.get(k)).reshape(deck.n, -1).max(axis=1)
         worst = np.argsort(d)[-4:]
         print(k, [(int(i), int(deck.type[i]), float(d[i])) for i in worst])
-    assert max(e.values()) <= 1e-6, e
+    assert max(e.values()) <= 1e-5, e
     assert 0.0 < max(e.values())  # the mixed form is in use (FP32 kernel values differ from FP64)
     cd, idd = gd.neighbors()
     cm, im = gm.neighbors()
@@ -100,3 +102,32 @@ def test_mixed_batch_equals_mixed_standalone():
         assert g.step(5) == 0
         for k in ("xp", "vx", "sig", "rho", "eps"):
             assert bitequal(b.part(k, r), g.get(k)), (r, k)
+
+
+@pytest.mark.parametrize("name", ["column2d_small", "soilwater3d_small", "model_zoo_2d", "ssi_cpl_3d",
+                                  "air_water_2d_a1", "pile3d_amcc"])
+def test_mixed_one_step_vs_oracle(name):
+    # the mixed form against the FP64 oracle (Jacobi density form) from the same state: the same
+    # 1e-5 field-max bar as against the library's FP64 form
+    from helpers import OracleSim, JACOBI
+    deck, states = load_golden(name)
+    gm = make(deck, states[0], MIXED)
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    assert gm.step(1) == 0 and o.step(1) == 0
+    e = {k: rel(gm.get(k), o.get(k)) for k in FIELDS}
+    print(name, {k: f"{v:.2e}" for k, v in e.items()})
+    assert max(e.values()) <= 1e-5, e
+    cd, idd = gm.neighbors()
+    co, io = o.neighbors()
+    assert np.array_equal(cd, co) and np.array_equal(idd, io)
+
+
+def test_mixed_column_collapse_100_steps_vs_oracle():
+    from helpers import OracleSim, JACOBI
+    deck, states = load_golden("column2d_small")
+    gm = make(deck, states[0], MIXED)
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    assert gm.step(100) == 0 and o.step(100) == 0
+    e = {k: rel(gm.get(k), o.get(k)) for k in ("xp", "vx", "sig", "rho")}
+    print({k: f"{v:.2e}" for k, v in e.items()})
+    assert max(e.values()) <= 1e-4, e
