@@ -196,8 +196,14 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     I(S.nbr, cap * kMaxNb);
     for (int k = 0; k < 6; k++) D(S.qsig[k]);
     D(S.fdens);
-    S.mrec = nullptr;
-    if (precision == SPHG_MIXED) plan.push_back({(void **)&S.mrec, cap * 4 * sizeof(float4)});
+    S.mra = S.mrc = S.mrv = nullptr;
+    S.mrn = nullptr;
+    if (precision == SPHG_MIXED) {
+        plan.push_back({(void **)&S.mra, cap * sizeof(float4)});
+        plan.push_back({(void **)&S.mrc, cap * 3 * sizeof(float4)});
+        plan.push_back({(void **)&S.mrv, cap * sizeof(float4)});
+        plan.push_back({(void **)&S.mrn, cap * sizeof(float2)});
+    }
     // artificial-stress tensor (kept between steps) and the Jacobi regularisation copy
     for (int k = 0; k < 9; k++) {
         if (pro->art_stre_coe > 0.0) D(S.rij[k]);
@@ -653,6 +659,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.struct_e = c->mat.structure.e;
     P.struct_poi = c->mat.structure.poi;
     P.eig_in_step = (pro.flts >= 1 || pro.art_stre_coe > 0.0) ? 1 : 0;
+    P.mrv_ok = (pro.ss_type == 0 && pro.flbndy != 5) ? 1 : 0;
     P.water_acc_minor = c->mat.water[0].acc_type != 0;
     P.air_acc_minor = c->mat.air.acc_type != 0;
     // realisation of every particle (batch contexts: realisation-major ranges of counts[r]) and
@@ -790,6 +797,11 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     int zero = 0;
     HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->S.step, &c->step, 4, hipMemcpyHostToDevice));
+    if (c->S.mra) {  // mixed form: the packed records from the uploaded state
+        hipLaunchKernelGGL(k_mrec_init, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->S, c->P, c->cur);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     drop_graphs(c);
     c->regu_on = false;
     c->pre_on = false;

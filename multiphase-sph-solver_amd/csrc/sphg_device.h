@@ -68,6 +68,7 @@ struct Params {
     double ss_myu, ss_zeta;  // Para_SSInter friction coefficient and force factor
     int coupled;             // Para_SSInter acc_type 2 with an interaction type 1-3
     int eig_in_step;         // principal stresses needed within the step (TS transform, artificial stress)
+    int mrv_ok;              // mixed form: the momentum kernel writes every particle's mrv (no contact tail)
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -101,11 +102,16 @@ struct State {
     double *qsig[6];      // the j factor of the momentum sums: sig / rho / rho (solids),
                           // (porosity * sig) / rho / rho (fluids)
     double *fdens;        // density of the particle's fluid slot (drag partner density)
-    // mixed form only (null in FP64): what the momentum pass gathers of a neighbour, packed in
-    // FP32 as four 16-byte words per particle {m, rho, vxp0, vxp1} {vxp2, q0, q1, q2}
-    // {q3, q4, q5, satu} {prew, prea, porosity, cop}, q the j factor of the stress sums; written
-    // by the stress pass, so one neighbour costs four vector loads instead of fifteen scalar ones
-    float4 *mrec;
+    // Mixed form only (null in FP64): the neighbour fields the pair passes gather, packed in FP32
+    // by the pass that produces them, so one neighbour costs one 16-byte load per record instead
+    // of one 8-byte load per field:
+    //   mra {m, vxp0, vxp1, vxp2}            update (and the upload)   -> density, strain, accel, XSPH
+    //   mrn {rho, pre}                       normalisation             -> strain
+    //   mrc {rho, q0, q1, q2} {q3, q4, q5, satu} {prew, prea, porosity, cop}
+    //                                         stress (q: the j factor of the stress sums) -> accel
+    //   mrv {vx0, vx1, vx2, rho}             momentum (velocity after the update) -> XSPH
+    float4 *mra, *mrc, *mrv;
+    float2 *mrn;
     double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
     double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
     // soil-structure interaction (ss_type 1-3): per soil / water particle the structure particles

@@ -621,6 +621,33 @@ __device__ __forceinline__ void nb_vel(const State &S, int j, bool bnd, int mode
         for (int k = 0; k < NDIM; k++) vj[k] = 0.0;
 }
 
+// Mixed form: the packed records that a step reads before their producer has run in it (mra
+// before the first update; mrn and mrv for completeness), from the uploaded state.
+__global__ __launch_bounds__(256) void k_mrec_init(State S, Params P, int cur) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    S.mra[i] = make_float4((float)S.mass[i], (float)S.vxp[0][i], (float)S.vxp[1][i], (float)S.vxp[2][i]);
+    S.mrn[i] = make_float2((float)S.rho[i], (float)S.pre[i]);
+    S.mrv[i] = make_float4((float)S.vx[cur][0][i], (float)S.vx[cur][1][i], (float)S.vx[cur][2][i], (float)S.rho[i]);
+}
+
+// Neighbour velocity of the mixed form: own-phase neighbours (and walls of mode 1 / 0, as nb_vel)
+// from the packed record, virtual wall velocities (mode 2) from memory.
+template <int NDIM>
+__device__ __forceinline__ void nb_vel_mixed(const State &S, int j, bool bnd, int mode, int ph, const double vi[3],
+                                             const float4 ra, double vj[3]) {
+    if (bnd && mode == 2) {
+#pragma unroll
+        for (int k = 0; k < NDIM; k++) vj[k] = S.vb[ph][k][j];
+        return;
+    }
+    vj[0] = ra.y; vj[1] = ra.z; vj[2] = ra.w;
+    if (bnd && mode == 1)
+        for (int k = 0; k < NDIM; k++) vj[k] = vi[k];
+    else if (bnd && mode == 0)
+        for (int k = 0; k < NDIM; k++) vj[k] = 0.0;
+}
+
 // ------------------------------------------------------------------------------------------
 // density loops 1-2 (CPP_6-density.cpp:22-166); water porosity (loop 1) and continuity (loop 2)
 // accumulate side by side in one walk over the list.
@@ -657,8 +684,14 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xj[b]);
             hlj[b] = hl_of(S, P, j, hli);
-            mj[b] = S.mass[j];
-            nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
+            if constexpr (M::kFast) {
+                const float4 ra = S.mra[j];
+                mj[b] = ra.x;
+                nb_vel_mixed<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, ra, vj[b]);
+            } else {
+                mj[b] = S.mass[j];
+                nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
+            }
             porj[b] = 0.0;
             rj[b] = 1.0;
             if (water) {
@@ -766,6 +799,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
         double p = prep * (pow(r / S.rhop[i], 7.0) - 1.0);
         S.pre[i] = p + prep;
     }
+    if constexpr (M::kFast) S.mrn[i] = make_float2((float)r, (float)S.pre[i]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -900,10 +934,19 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                 const int j = nb_id(e[b]);
                 load_xj<NDIM>(S, cur, j, xj[b]);
                 hlj[b] = hl_of(S, P, j, hli);
-                mj[b] = S.mass[j];
-                rj[b] = S.rho[j];
-                nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
-                pj[b] = solid ? S.pre[j] : 0.0;
+                if constexpr (M::kFast) {
+                    const float4 ra = S.mra[j];
+                    const float2 rn = S.mrn[j];
+                    mj[b] = ra.x;
+                    rj[b] = rn.x;
+                    nb_vel_mixed<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, ra, vj[b]);
+                    pj[b] = solid ? (double)rn.y : 0.0;
+                } else {
+                    mj[b] = S.mass[j];
+                    rj[b] = S.rho[j];
+                    nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
+                    pj[b] = solid ? S.pre[j] : 0.0;
+                }
             }
 #pragma unroll
             for (int b = 0; b < B; b++) {
@@ -1621,17 +1664,16 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
         }
         #pragma unroll
         for (int k = 0; k < 6; k++) S.qsig[k][i] = q[k];
-        if (S.mrec) {
+        if (S.mrc) {
             // the fluid pass forms (porosity * sig) / rho / rho per pair when the fluids' porosities differ
             const bool raw = (ti == 1 || ti == 3) && !P.fluid_por_uniform;
             float qf[6];
             #pragma unroll
             for (int k = 0; k < 6; k++) qf[k] = (float)(raw ? sig[k] : q[k]);
-            float4 *o = S.mrec + 4 * (size_t)i;
-            o[0] = make_float4((float)S.mass[i], (float)r, (float)S.vxp[0][i], (float)S.vxp[1][i]);
-            o[1] = make_float4((float)S.vxp[2][i], qf[0], qf[1], qf[2]);
-            o[2] = make_float4(qf[3], qf[4], qf[5], (float)S.satu[i]);
-            o[3] = make_float4((float)S.prew[i], (float)S.prea[i], (float)S.porosity[i], (float)S.cop[i]);
+            float4 *o = S.mrc + 3 * (size_t)i;
+            o[0] = make_float4((float)r, qf[0], qf[1], qf[2]);
+            o[1] = make_float4(qf[3], qf[4], qf[5], (float)S.satu[i]);
+            o[2] = make_float4((float)S.prew[i], (float)S.prea[i], (float)S.porosity[i], (float)S.cop[i]);
         }
     }
     // principal stresses (CPP_7-1-strain.cpp:283-332). Only the TS transform (next step) and the
@@ -2085,11 +2127,12 @@ struct NbRec {
     double m, r, v[3], q[6], satu, prew, prea, por, cop;
 };
 __device__ __forceinline__ NbRec load_nbrec(const State &S, int j) {
-    const float4 *p = S.mrec + 4 * (size_t)j;
-    const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    const float4 a = S.mra[j];
+    const float4 *p = S.mrc + 3 * (size_t)j;
+    const float4 b = p[0], c = p[1], d = p[2];
     NbRec o;
-    o.m = a.x; o.r = a.y; o.v[0] = a.z; o.v[1] = a.w;
-    o.v[2] = b.x; o.q[0] = b.y; o.q[1] = b.z; o.q[2] = b.w;
+    o.m = a.x; o.v[0] = a.y; o.v[1] = a.z; o.v[2] = a.w;
+    o.r = b.x; o.q[0] = b.y; o.q[1] = b.z; o.q[2] = b.w;
     o.q[3] = c.x; o.q[4] = c.y; o.q[5] = c.z; o.satu = c.w;
     o.prew = d.x; o.prea = d.y; o.por = d.z; o.cop = d.w;
     return o;
@@ -2514,6 +2557,8 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     }
     accel_damp<NDIM>(S, P, i, ax, vi, axp, damp_on);
     accel_vel<NDIM>(S, P, i, ti, cur, ax, vi, axp);
+    if constexpr (M::kFast)
+        S.mrv[i] = make_float4((float)S.vx[cur][0][i], (float)S.vx[cur][1][i], (float)S.vx[cur][2][i], (float)rhoi);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2558,10 +2603,19 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xj[b]);
             hlj[b] = hl_of(S, P, j, hli);
-            mj[b] = S.mass[j];
-            rj[b] = S.rho[j];
+            if (M::kFast && P.mrv_ok) {
+                const float4 rv = S.mrv[j];
+                mj[b] = S.mra[j].x;
+                rj[b] = rv.w;
+                vxj[b][0] = rv.x;
+                vxj[b][1] = rv.y;
+                if (NDIM == 3) vxj[b][2] = rv.z;
+            } else {
+                mj[b] = S.mass[j];
+                rj[b] = S.rho[j];
 #pragma unroll
-            for (int k = 0; k < NDIM; k++) vxj[b][k] = S.vx[cur][k][j];
+                for (int k = 0; k < NDIM; k++) vxj[b][k] = S.vx[cur][k][j];
+            }
             mtj[b] = tm ? S.matype[j] : 0;
         }
 #pragma unroll
@@ -2894,6 +2948,8 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
         }
         if (NDIM == 2) S.xp[nxt][2][i] = S.xp[cur][2][i];
         S.rhop[i] = S.rho[i];
+        if constexpr (M::kFast)
+            S.mra[i] = make_float4((float)S.mass[i], (float)S.vxp[0][i], (float)S.vxp[1][i], (float)S.vxp[2][i]);
     } else {
         int mt = S.matype[i];
         if (mt >= 2 && mt <= 5) {
