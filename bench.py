@@ -301,7 +301,17 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error-vs-step curves")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default neighbour batch widths")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (scales the HBM byte counters of a profile)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch  # device memory and the runtime's copy kernel only
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
               "run with --gpus 1", file=sys.stderr)

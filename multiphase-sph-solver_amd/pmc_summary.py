@@ -5,7 +5,8 @@
     python multiphase-sph-solver_amd/pmc_summary.py --counters <out.md> <pmc_dir> [<pmc_dir> ...]
 
 Inputs are the output directories of: rocprofv3 --kernel-trace --stats (bench), --pmc FETCH_SIZE
-(bench), --pmc WRITE_SIZE (bench), and the two PMC passes over copy_calibration.py (1 GiB copy).
+(bench), --pmc WRITE_SIZE (bench), and the same two PMC passes over `bench.py --copy-calibration`
+(three device copies of 1 GiB), whose counts give the bytes-per-count scale of each counter.
 """
 import collections
 import csv
