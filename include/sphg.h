@@ -376,12 +376,32 @@ int sphg_kl_domain(int ndim, int n, const double *xp, const int *type, double a[
 int sphg_kl_modes(sphg_kl_field *f);
 /* solve_h_2D / solve_h_3D (CPP_17-1_K-L_expansion_non.cpp:499-713) on the device for the
  * context's particles (FP64), with the standard normals xi[200] supplied by the caller (the
- * reference draws them from a clock-seeded generator, :24-34). Field 1 (soil cohesion) only: the
- * result becomes the per-particle cohesion of the soil models. */
+ * reference draws them from a clock-seeded generator, :24-34). The result becomes the per-particle
+ * property f->field of the field's phase (0 soil fai, 1 c, 2 cop, 3 ds: soil particles; 4 fluid fai,
+ * 5 c: water particles), which the step reads in place of the material slot's value; the other
+ * particles keep their slot values. */
 int sphg_kl_synthesize(sphg_ctx *ctx, const sphg_kl_field *f, const double *xi);
 /* The same for the particles [first, first + count) only: one realisation of a batch context
  * (no counterpart in the reference, which holds one realisation per run). */
 int sphg_kl_synthesize_part(sphg_ctx *ctx, const sphg_kl_field *f, const double *xi, int first, int count);
+/* K-L expansion with correlated friction angle and cohesion (flspv 2,
+ * SpatialVariables_Generate_2D_corr / _3D_corr, CPP_17-2_K-L_expansion_corr.cpp:55-473): the
+ * standard normals of the two fields are u[200][2] (the reference draws them from a clock-seeded
+ * generator, :39-52) mixed by [[1, rho], [0, sqrt(1 - rho^2)]] with rho = coe_frict_cohesion, or for
+ * a lognormal f_a the correlation of the underlying normals (:114-140); xi_a / xi_b receive the
+ * mixed normals and coefficient (may be null) rho. Host only. */
+int sphg_kl_corr_normals(const sphg_kl_field *f_a, const sphg_kl_field *f_b, double coe_frict_cohesion,
+                         const double *u, double *xi_a, double *xi_b, double *coefficient);
+/* Both fields of the pair into the context's per-particle properties: f_a.field 0 and f_b.field 1
+ * (soil fai, c) or 4 and 5 (fluid fai, c), each with its own modes, mean, sd and distribution (the
+ * reference takes the mixing coefficient's distribution from f_a). The K-L domain is that of
+ * sphg_kl_domain (the same 20 % widening as flspv 1). SPHG_E_ARG for any other field pair. */
+int sphg_kl_synthesize_corr(sphg_ctx *ctx, const sphg_kl_field *f_a, const sphg_kl_field *f_b,
+                            double coe_frict_cohesion, const double *u);
+/* The same for the particles [first, first + count) only (one realisation of a batch). */
+int sphg_kl_synthesize_corr_part(sphg_ctx *ctx, const sphg_kl_field *f_a, const sphg_kl_field *f_b,
+                                 double coe_frict_cohesion, const double *u, int first, int count);
+
 /* Realisation batch in a common K-L domain: for nreal realisations xi[nreal][200] over the same
  * particles, H = Phi (n x 200) * Xi (200 x nreal) as one FP32 matrix-core GEMM, then the field
  * values value[nreal][n] (FP64 from the FP32 H; particles of other types keep fill). gemm_ms
@@ -446,6 +466,31 @@ int sphg_rf_generate(const sphg_rf_field *f, int n, const double *xp, const int 
 int sphg_rf_generate_pair(const sphg_rf_field *f_fai, const sphg_rf_field *f_c, double coe_frict_cohesion,
                           int n, const double *xp, const int *type, int nreal, const double *zeta, double fill,
                           double *value_fai, double *value_c);
+
+/* ---- Correlated random variables (flspv 5, CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863) -- */
+
+/* Flag_SpatialVariability members random_variable_generate reads (Class_spatial_variability.h:10-42) */
+typedef struct {
+    int flag_fai, flag_c, flag_faiw, flag_cw;
+    double coe_frict_cohesion;
+} sphg_rv_flags;
+/* Para_SpatialVariability members it reads (Class_spatial_variability.h:44-66): type 1 normal,
+ * 2 lognormal; psv[6] indexed as the reference (0 soil fai, 1 soil c, 4 fluid fai, 5 fluid c) */
+typedef struct {
+    int type;
+    double mean_value, sd_value;
+} sphg_rv_param;
+/* The homogeneous values of random_variable_generate for the standard normals zeta[2] (the reference
+ * draws them from a clock-seeded generator): the distribution is psv[0]'s when flag_fai is set, else
+ * psv[4]'s when flag_faiw is set, else normal; the pair is mixed as in flspv 2 and fai takes the
+ * mixed normal, c the first. values[k] is set for each flagged property k (0, 1, 4, 5), NaN
+ * otherwise. Host only. */
+int sphg_random_variable_values(const sphg_rv_flags *fsv, const sphg_rv_param *psv, const double zeta[2],
+                                double values[6]);
+/* The same values written to every soil (type 2: fai, c) or water (type 1: fai, c) particle in
+ * [first, first + count) of the context; nothing is written when the distribution is neither 1 nor 2. */
+int sphg_random_variable_generate(sphg_ctx *ctx, const sphg_rv_flags *fsv, const sphg_rv_param *psv,
+                                  const double zeta[2], int first, int count);
 
 const char *sphg_error_string(int code);
 const char *sphg_build_info(void);

@@ -59,6 +59,8 @@ struct sphg_ctx {
     bool has_psc = false;
     // ps_ds / ps_cop / ps_fai set by the caller: upload_done keeps them instead of the slot values
     bool has_psds = false, has_pscop = false, has_psfai = false;
+    // pf_fai / pf_c (Bingham water strength) set by the caller or by a random-field call
+    bool has_pffai = false, has_pfc = false;
     // other per-particle material copies the caller set (ps_e, pf_fai, ...): the step reads these
     // from the material slots, so upload_done refuses them unless they equal the slot values
     std::map<std::string, std::vector<double>> pinned_mat;
@@ -180,6 +182,7 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
         D(*f);
     plan.push_back({(void **)&c->d_psc, cap * 8});
     D(S.ps_ds); D(S.ps_cop); D(S.ps_fai);
+    D(S.pf_fai); D(S.pf_c); D(S.pf_tan);
     S.kgc = nullptr;
     if (pro->flknl > 2) plan.push_back({(void **)&S.kgc, 9 * cap * 8});
     for (int k = 0; k < 6; k++) S.tssig[k] = nullptr;
@@ -368,7 +371,7 @@ static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, b
         {"veps", S.veps}, {"divq", S.divq}, {"divr", S.divr}, {"meps", S.meps}, {"roue", S.roue},
         {"prew", S.prew}, {"prea", S.prea}, {"porosity", S.porosity}, {"cd", S.cd}, {"poro_cop", S.poro_cop},
         {"color", S.color}, {"ps_c", c->d_psc}, {"ps_ds", S.ps_ds}, {"ps_cop", S.ps_cop},
-        {"ps_fai", S.ps_fai}, {"rho_mid", S.rho_mid}};
+        {"ps_fai", S.ps_fai}, {"pf_fai", S.pf_fai}, {"pf_c", S.pf_c}, {"rho_mid", S.rho_mid}};
     for (auto &e : sc)
         if (nm == e.n) return s1(e.p);
     return -1;
@@ -491,6 +494,35 @@ static int check_pinned_materials(const sphg_ctx *c, const std::vector<sphg_soil
     return 0;
 }
 
+// Constants the step derives from per-particle strength, evaluated with the host's libm as the slot
+// constants are: the Drucker-Prager factors of ps_fai (phi' = atan(tan(fai) / fs),
+// CPP_8-1-consfactor.cpp:294, :783) and tan of the Bingham friction angle pf_fai
+// (CPP_7-2-constitutive.cpp:42-43). Called at upload and whenever either array changes after it.
+static int soil_slot_of(int type, int matype, int etype);
+static int fluid_slot_of(int type, int matype, int etype);
+
+static int derive_strength_constants(sphg_ctx *c) {
+    const int n = c->n;
+    std::vector<double> fai(n), dp(3 * (size_t)n);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(fai.data(), c->S.ps_fai, (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        const int s = soil_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
+        const double fs = s < SPHG_NSOIL ? c->mat.soil[s].fs : 1.0;
+        const double f = atan(tan(fai[i] * 0.0174532925166667) / fs);
+        const double sq = sqrt(9.0 + 3.0 * sin(f) * sin(f));
+        dp[i] = sin(f) / sq;
+        dp[n + i] = cos(f);
+        dp[2 * (size_t)n + i] = sq;
+    }
+    for (int k = 0; k < 3; k++)
+        HIPCHK(hipMemcpy(c->S.ps_dp[k], dp.data() + (size_t)k * n, (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(fai.data(), c->S.pf_fai, (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) fai[i] = tan(fai[i] * 0.0174532925199);
+    HIPCHK(hipMemcpy(c->S.pf_tan, fai.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    return SPHG_OK;
+}
+
 extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host, int n, int ncomp) {
     if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
     if (c->n == 0) c->n = n;
@@ -498,7 +530,7 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
     std::string nm(name);
     const size_t N = (size_t)n;
     if (nm == "x" || nm == "divx" || nm == "dsig" || nm == "beta_ts" || nm == "weps" || nm == "rous" ||
-        nm.rfind("pf_", 0) == 0 ||
+        (nm.rfind("pf_", 0) == 0 && nm != "pf_fai" && nm != "pf_c") ||
         (nm.rfind("ps_", 0) == 0 && nm != "ps_c" && nm != "ps_ds" && nm != "ps_cop" && nm != "ps_fai")) {
         // derived, per-step temporary, or taken from the material slots
         bool member = false;
@@ -524,6 +556,9 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
     if (nm == "ps_ds") c->has_psds = true;
     if (nm == "ps_cop") c->has_pscop = true;
     if (nm == "ps_fai") c->has_psfai = true;
+    if (nm == "pf_fai") c->has_pffai = true;
+    if (nm == "pf_c") c->has_pfc = true;
+    if (c->uploaded && (nm == "ps_fai" || nm == "pf_fai")) return derive_strength_constants(c);
     return SPHG_OK;
 }
 
@@ -760,7 +795,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
         // per-particle material values pParti_ConsPara[i].ds / .cop / .fai: the soil slot's unless
         // the caller set them (random fields), and the Drucker-Prager constants of each particle's
         // friction angle, evaluated here with the host's libm as the slot constants are
-        std::vector<double> v(n), fai(n), dp(3 * (size_t)n);
+        std::vector<double> v(n), fai(n);
         auto put = [&](double *d, const double *h) { return hipMemcpy(d, h, (size_t)n * 8, hipMemcpyHostToDevice); };
         if (!c->has_psds) {
             for (int i = 0; i < n; i++) v[i] = soil[ss[i]].ds;
@@ -770,21 +805,20 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
             for (int i = 0; i < n; i++) v[i] = soil[ss[i]].cop;
             HIPCHK(put(c->S.ps_cop, v.data()));
         }
-        if (c->has_psfai) {
-            HIPCHK(hipMemcpy(fai.data(), c->S.ps_fai, (size_t)n * 8, hipMemcpyDeviceToHost));
-        } else {
+        if (!c->has_psfai) {
             for (int i = 0; i < n; i++) fai[i] = ss[i] < SPHG_NSOIL ? c->mat.soil[ss[i]].fai : 0.0;
             HIPCHK(put(c->S.ps_fai, fai.data()));
         }
-        for (int i = 0; i < n; i++) {
-            const double fs = soil[ss[i]].fs;
-            const double f = atan(tan(fai[i] * 0.0174532925166667) / fs);  // CPP_8-1-consfactor.cpp:294, :783
-            const double sq = sqrt(9.0 + 3.0 * sin(f) * sin(f));
-            dp[i] = sin(f) / sq;
-            dp[n + i] = cos(f);
-            dp[2 * (size_t)n + i] = sq;
+        // Para_Fluid fai / c of each particle (Bingham water): its fluid slot's unless set
+        if (!c->has_pffai) {
+            for (int i = 0; i < n; i++) v[i] = fluid_src[fs[i]].fai;
+            HIPCHK(put(c->S.pf_fai, v.data()));
         }
-        for (int k = 0; k < 3; k++) HIPCHK(put(c->S.ps_dp[k], dp.data() + (size_t)k * n));
+        if (!c->has_pfc) {
+            for (int i = 0; i < n; i++) v[i] = fluid_src[fs[i]].c;
+            HIPCHK(put(c->S.pf_c, v.data()));
+        }
+        if ((rc = derive_strength_constants(c))) return rc;
     }
     HIPCHK(hipMemcpy(hl.data(), c->S.hl, (size_t)n * 8, hipMemcpyDeviceToHost));
     P.hl_uniform = 1;
@@ -1362,23 +1396,58 @@ extern "C" const char *sphg_build_info(void) {
 }
 
 // -------------------------------------------------------------------------------- K-L fields
+// The per-particle array of spatially variable property `field` (0 soil fai, 1 soil c, 2 soil cop,
+// 3 soil ds, 4 fluid fai, 5 fluid c), initialised from the material slots (setting_vis_cons copies,
+// CPP_2-initialf.cpp:1021-1058) the first time a random field writes it, so the particles of the
+// other phase keep their slot values.
+static int property_array(sphg_ctx *c, int field, double **out) {
+    bool *has = field == 0 ? &c->has_psfai : field == 1 ? &c->has_psc : field == 2 ? &c->has_pscop
+              : field == 3 ? &c->has_psds : field == 4 ? &c->has_pffai : &c->has_pfc;
+    double *d = field == 0 ? c->S.ps_fai : field == 1 ? c->d_psc : field == 2 ? c->S.ps_cop
+              : field == 3 ? c->S.ps_ds : field == 4 ? c->S.pf_fai : c->S.pf_c;
+    if (!*has) {
+        std::vector<double> base(c->n);
+        for (int i = 0; i < c->n; i++) {
+            if (field <= 3) {
+                const int s = soil_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
+                sphg_soil m{};
+                m.fs = 1.0;
+                m.ds = 1.0;
+                if (s < SPHG_NSOIL) m = c->mat.soil[s];
+                base[i] = field == 0 ? m.fai : field == 1 ? m.c : field == 2 ? m.cop : m.ds;
+            } else {
+                const int s = fluid_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
+                sphg_fluid m{};
+                if (s < 3) m = c->mat.water[s];
+                else if (s == 3) m = c->mat.air;
+                base[i] = field == 4 ? m.fai : m.c;
+            }
+        }
+        HIPCHK(hipMemcpy(d, base.data(), (size_t)c->n * 8, hipMemcpyHostToDevice));
+        *has = true;
+    }
+    *out = d;
+    return SPHG_OK;
+}
+
+// After a random field changed a property: the derived constants, and the cohesion pointer the
+// captured graphs hold.
+static int property_changed(sphg_ctx *c, int field) {
+    if (field == 1 && c->S.ps_c != c->d_psc) {
+        c->S.ps_c = c->d_psc;
+        for (int b = 0; b < 2; b++) c->graph_ok[b] = false;  // the State copy in the graphs changed
+    }
+    if (c->uploaded && (field == 0 || field == 4)) return derive_strength_constants(c);
+    return SPHG_OK;
+}
+
 extern "C" int sphg_kl_synthesize_part(sphg_ctx *c, const sphg_kl_field *f, const double *xi, int first,
                                        int count) {
     if (!c || !xi || first < 0 || count <= 0 || first + count > c->n) return SPHG_E_ARG;
     int rc = kl_check(f);
     if (rc) return rc;
-    if (f->field != 1) return SPHG_E_UNSUPPORTED;
-    // per-particle cohesion starts from the material slots (setting_vis_cons copies), then the
-    // field overwrites the soil particles
-    if (!c->has_psc) {
-        std::vector<double> base(c->n);
-        for (int i = 0; i < c->n; i++) {
-            int s = soil_slot_of(c->h_type[i], c->h_matype[i], c->h_etype[i]);
-            base[i] = s < SPHG_NSOIL ? c->mat.soil[s].c : 0.0;
-        }
-        HIPCHK(hipMemcpy(c->d_psc, base.data(), (size_t)c->n * 8, hipMemcpyHostToDevice));
-        c->has_psc = true;
-    }
+    double *dst = nullptr;
+    if ((rc = property_array(c, f->field, &dst))) return rc;
     sphg::KlModes m;
     kl_modes_device(*f, xi, m);
     sphg::KlModes *dm = nullptr;
@@ -1388,13 +1457,130 @@ extern "C" int sphg_kl_synthesize_part(sphg_ctx *c, const sphg_kl_field *f, cons
         sphg::KlParams kp = kl_params(*f, count);
         const int cur = c->cur;
         hipLaunchKernelGGL(sphg::k_kl_fp64, dim3(grid_for(count)), dim3(kBlock), 0, c->stream, c->S.xp[cur][0],
-                           c->S.xp[cur][1], c->S.xp[cur][2], c->S.type, dm, kp, first, c->d_psc);
+                           c->S.xp[cur][1], c->S.xp[cur][2], c->S.type, dm, kp, first, dst);
         e = hipStreamSynchronize(c->stream);
     }
     hipFree(dm);
     if (e != hipSuccess) return SPHG_E_HIP;
-    c->S.ps_c = c->d_psc;
-    for (int b = 0; b < 2; b++) c->graph_ok[b] = false;  // the State copy in the graphs changed
+    return property_changed(c, f->field);
+}
+
+// Correlation coefficient of the two fields' underlying normals and the mixed standard normals of
+// SpatialVariables_Generate_2D_corr / _3D_corr (CPP_17-2_K-L_expansion_corr.cpp:114-140): rho =
+// coe_frict_cohesion, or for a lognormal first field log(1 + rho cv_a cv_b) /
+// sqrt(log(1 + cv_a^2) log(1 + cv_b^2)); zeta = u [[1, rho], [0, sqrt(1 - rho^2)]] row by row with
+// matrix_multi's summation order (:22-37).
+extern "C" int sphg_kl_corr_normals(const sphg_kl_field *f_a, const sphg_kl_field *f_b, double coe_frict_cohesion,
+                                    const double *u, double *xi_a, double *xi_b, double *coefficient) {
+    if (!f_a || !f_b || !u || !xi_a || !xi_b) return SPHG_E_ARG;
+    double coe = coe_frict_cohesion;
+    if (f_a->dist == 2) {
+        coe = log(1.0 + coe_frict_cohesion * (f_a->sd_value / f_a->mean_value) * (f_b->sd_value / f_b->mean_value));
+        coe = coe / sqrt(log(1.0 + f_a->sd_value * f_a->sd_value / f_a->mean_value / f_a->mean_value) *
+                         log(1.0 + f_b->sd_value * f_b->sd_value / f_b->mean_value / f_b->mean_value));
+    }
+    const double cov[4] = {1.0, coe, 0.0, sqrt(1.0 - coe * coe)};
+    for (int i = 0; i < SPHG_KL_TERMS; i++) {
+        double z[2];
+        for (int j = 0; j < 2; j++) {
+            double t = 0.0;
+            for (int k = 0; k < 2; k++) t = t + u[i * 2 + k] * cov[k * 2 + j];
+            z[j] = t;
+        }
+        xi_a[i] = z[0];
+        xi_b[i] = z[1];
+    }
+    if (coefficient) *coefficient = coe;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_kl_synthesize_corr_part(sphg_ctx *c, const sphg_kl_field *f_a, const sphg_kl_field *f_b,
+                                            double coe_frict_cohesion, const double *u, int first, int count) {
+    if (!c || !f_a || !f_b || !u) return SPHG_E_ARG;
+    if (!((f_a->field == 0 && f_b->field == 1) || (f_a->field == 4 && f_b->field == 5))) return SPHG_E_ARG;
+    std::vector<double> xa(SPHG_KL_TERMS), xb(SPHG_KL_TERMS);
+    int rc = sphg_kl_corr_normals(f_a, f_b, coe_frict_cohesion, u, xa.data(), xb.data(), nullptr);
+    if (!rc) rc = sphg_kl_synthesize_part(c, f_a, xa.data(), first, count);
+    if (!rc) rc = sphg_kl_synthesize_part(c, f_b, xb.data(), first, count);
+    return rc;
+}
+
+extern "C" int sphg_kl_synthesize_corr(sphg_ctx *c, const sphg_kl_field *f_a, const sphg_kl_field *f_b,
+                                       double coe_frict_cohesion, const double *u) {
+    if (!c) return SPHG_E_ARG;
+    return sphg_kl_synthesize_corr_part(c, f_a, f_b, coe_frict_cohesion, u, 0, c->n);
+}
+
+// random_variable_generate (CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863): one correlated pair of
+// standard normals for the whole domain, mixed as above with the distribution of psv[0] (flag_fai)
+// or else psv[4] (flag_faiw), or else the constructor default (normal); fai takes the mixed normal
+// and c the first one.
+extern "C" int sphg_random_variable_values(const sphg_rv_flags *fsv, const sphg_rv_param *psv,
+                                           const double zeta[2], double values[6]) {
+    if (!fsv || !psv || !zeta || !values) return SPHG_E_ARG;
+    sphg_rv_param t{1, 0.0, 0.0};
+    if (fsv->flag_fai == 1) t = psv[0];
+    else if (fsv->flag_faiw == 1) t = psv[4];
+    const int dist = t.type;
+    double coe;
+    if (dist == 2 && fsv->flag_fai == 1) {
+        coe = log(1.0 + fsv->coe_frict_cohesion * (psv[0].sd_value / psv[0].mean_value) *
+                            (psv[1].sd_value / psv[1].mean_value));
+        coe = coe / sqrt(log(1.0 + psv[0].sd_value * psv[0].sd_value / psv[0].mean_value / psv[0].mean_value) *
+                         log(1.0 + psv[1].sd_value * psv[1].sd_value / psv[1].mean_value / psv[1].mean_value));
+    } else if (dist == 2 && fsv->flag_faiw == 1) {
+        coe = log(1.0 + fsv->coe_frict_cohesion * (psv[4].sd_value / psv[4].mean_value) *
+                            (psv[5].sd_value / psv[5].mean_value));
+        coe = coe / sqrt(log(1.0 + psv[4].sd_value * psv[4].sd_value / psv[4].mean_value / psv[4].mean_value) *
+                         log(1.0 + psv[5].sd_value * psv[5].sd_value / psv[5].mean_value / psv[5].mean_value));
+    } else {
+        coe = fsv->coe_frict_cohesion;
+    }
+    const double cov[4] = {1.0, coe, 0.0, sqrt(1.0 - coe * coe)};
+    double z1[2];
+    for (int j = 0; j < 2; j++) {
+        double tt = 0.0;
+        for (int k = 0; k < 2; k++) tt = tt + zeta[k] * cov[k * 2 + j];
+        z1[j] = tt;
+    }
+    // property slot (psv index) and which mixed normal it takes
+    const int flag[4] = {fsv->flag_fai, fsv->flag_c, fsv->flag_faiw, fsv->flag_cw};
+    const int slot[4] = {0, 1, 4, 5}, zi[4] = {1, 0, 1, 0};
+    for (int k = 0; k < 6; k++) values[k] = NAN;
+    for (int q = 0; q < 4; q++) {
+        if (flag[q] != 1) continue;
+        double mean = psv[slot[q]].mean_value, sd = psv[slot[q]].sd_value;
+        if (dist == 1) {
+            values[slot[q]] = mean + sd * z1[zi[q]];
+        } else if (dist == 2) {
+            sd = sqrt(log(1.0 + (sd / mean) * (sd / mean)));
+            mean = log(mean) - 0.5 * sd * sd;
+            values[slot[q]] = exp(mean + sd * z1[zi[q]]);
+        }
+    }
+    return SPHG_OK;
+}
+
+extern "C" int sphg_random_variable_generate(sphg_ctx *c, const sphg_rv_flags *fsv, const sphg_rv_param *psv,
+                                             const double zeta[2], int first, int count) {
+    if (!c || first < 0 || count <= 0 || first + count > c->n) return SPHG_E_ARG;
+    double v[6];
+    int rc = sphg_random_variable_values(fsv, psv, zeta, v);
+    if (rc) return rc;
+    const int t = fsv->flag_fai == 1 ? psv[0].type : fsv->flag_faiw == 1 ? psv[4].type : 1;
+    if (t != 1 && t != 2) return SPHG_OK;  // the reference assigns nothing for another distribution
+    const int flag[4] = {fsv->flag_fai, fsv->flag_c, fsv->flag_faiw, fsv->flag_cw};
+    const int slot[4] = {0, 1, 4, 5};
+    for (int q = 0; q < 4; q++) {
+        if (flag[q] != 1) continue;
+        double *dst = nullptr;
+        if ((rc = property_array(c, slot[q], &dst))) return rc;
+        hipLaunchKernelGGL(k_fill_phase, dim3(grid_for(count)), dim3(kBlock), 0, c->stream, dst, c->S.type,
+                           slot[q] <= 1 ? 2 : 1, v[slot[q]], first, count);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if ((rc = property_changed(c, slot[q]))) return rc;
+    }
     return SPHG_OK;
 }
 

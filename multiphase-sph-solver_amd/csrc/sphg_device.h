@@ -88,6 +88,9 @@ struct State {
     double *ps_ds;        // per-particle grain size of the drag laws (slot value unless set per particle)
     double *ps_cop;       // per-particle permeability of permtype 0 (slot value unless set per particle)
     double *ps_fai;       // per-particle friction angle (slot value unless set per particle)
+    double *pf_fai, *pf_c;  // per-particle fluid friction angle and cohesion (Bingham water; slot values
+                            // unless set per particle)
+    double *pf_tan;       // from pf_fai on the host: tan(fai * 0.0174532925199)
     double *kgc;          // flknl 3 / 4: kernel-gradient correction L^-1 of particle i, row-major entry r
                           // at kgc[r * cap + i] (soil and structure particles; null otherwise)
     double *tssig[6], *tsqc;  // TS transform (flts >= 1; null otherwise): stress after the transform, qc

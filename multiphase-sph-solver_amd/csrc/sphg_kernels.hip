@@ -314,6 +314,13 @@ __global__ void k_fill_int(int *a, int n, int v) {
     if (i < n) a[i] = v;
 }
 
+// dst[i] = v for the particles of type t in [first, first + count): one random variable for a whole
+// phase (random_variable_generate, CPP_17-5-Nonstationary-Correlated-RF.cpp:786-860)
+__global__ void k_fill_phase(double *dst, const int *type, int t, double v, int first, int count) {
+    const int i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < first + count && type[i] == t) dst[i] = v;
+}
+
 template <int NDIM>
 __global__ __launch_bounds__(256) void k_bin(State S, Params P, int cur) {
     int i = pidx();
@@ -1469,7 +1476,7 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
                                              deps[4] * deps[4] * 0.25 + deps[5] * deps[5] * 0.25);
             de = sqrt(de);
             if (f.bhtype == 1) {
-                double tao = pre * f.tan_fai + f.c;
+                double tao = pre * S.pf_tan[i] + S.pf_c[i];
                 vis = de < f.strain_min ? f.vis + tao / f.strain_min : f.vis + tao / de;
             }
         }

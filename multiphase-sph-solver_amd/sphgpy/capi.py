@@ -181,6 +181,44 @@ def kl_field(deck: Deck, dist: int, mean: float, sd: float, rl, field: int = 1, 
     return f
 
 
+class RVFlags(C.Structure):
+    """sphg_rv_flags: the Flag_SpatialVariability members random_variable_generate reads."""
+    _fields_ = [("flag_fai", _I), ("flag_c", _I), ("flag_faiw", _I), ("flag_cw", _I), ("coe_frict_cohesion", _D)]
+
+
+class RVParam(C.Structure):
+    """sphg_rv_param: distribution type (1 normal, 2 lognormal), mean and standard deviation."""
+    _fields_ = [("type", _I), ("mean_value", _D), ("sd_value", _D)]
+
+
+def rv_args(flags, coe, psv):
+    """flags (fai, c, faiw, cw), coefficient and six (type, mean, sd) tuples as the C structs."""
+    fl = RVFlags(*[int(v) for v in flags], float(coe))
+    ps = (RVParam * 6)(*[RVParam(int(t), float(m), float(sd)) for (t, m, sd) in psv])
+    return fl, ps
+
+
+def random_variable_values(flags, coe, psv, zeta):
+    """sphg_random_variable_values: the six homogeneous values (NaN where not flagged)."""
+    fl, ps = rv_args(flags, coe, psv)
+    out = (_D * 6)()
+    rc = product_lib().lib.sphg_random_variable_values(C.byref(fl), ps, (_D * 2)(*zeta), out)
+    if rc:
+        raise RuntimeError(f"sphg_random_variable_values: error {rc}")
+    return np.array(list(out))
+
+
+def kl_corr_normals(fa: KLField, fb: KLField, coe: float, u: np.ndarray):
+    """sphg_kl_corr_normals: the mixed standard normals (xi_a, xi_b) and the coefficient."""
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(200, 2)
+    xa, xb, cf = np.zeros(200), np.zeros(200), C.c_double()
+    rc = product_lib().lib.sphg_kl_corr_normals(C.byref(fa), C.byref(fb), coe, _dptr(u), _dptr(xa), _dptr(xb),
+                                                C.byref(cf))
+    if rc:
+        raise RuntimeError(f"sphg_kl_corr_normals: error {rc}")
+    return xa, xb, cf.value
+
+
 def kl_synthesize_batch(f: KLField, xp: np.ndarray, ty: np.ndarray, xi: np.ndarray, fill: float = 0.0):
     """Field values [nreal][n] of a realisation batch in the common domain (FP32 matrix-core GEMM)
     and the GEMM kernel time in ms."""
@@ -335,7 +373,7 @@ class Sim:
             if k in state:
                 self.set(k, state[k])
         for k, v in state.items():
-            if k.startswith("ps_") and k in ("ps_fai", "ps_c", "ps_cop", "ps_ds"):
+            if k in ("ps_fai", "ps_c", "ps_cop", "ps_ds", "pf_fai", "pf_c"):
                 self.set(k, v)
 
     def step(self, nsteps: int = 1) -> int:
@@ -391,6 +429,17 @@ def product_lib() -> _Lib:
         L.sphg_kl_synthesize_batch.restype = _I
         L.sphg_kl_synthesize_part.argtypes = [C.c_void_p, C.POINTER(KLField), C.POINTER(C.c_double), _I, _I]
         L.sphg_kl_synthesize_part.restype = _I
+        L.sphg_kl_corr_normals.argtypes = [C.POINTER(KLField), C.POINTER(KLField), _D, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.sphg_kl_corr_normals.restype = _I
+        L.sphg_kl_synthesize_corr_part.argtypes = [C.c_void_p, C.POINTER(KLField), C.POINTER(KLField), _D,
+                                                   C.POINTER(C.c_double), _I, _I]
+        L.sphg_kl_synthesize_corr_part.restype = _I
+        L.sphg_random_variable_values.argtypes = [C.POINTER(RVFlags), C.POINTER(RVParam), _D * 2, _D * 6]
+        L.sphg_random_variable_values.restype = _I
+        L.sphg_random_variable_generate.argtypes = [C.c_void_p, C.POINTER(RVFlags), C.POINTER(RVParam), _D * 2,
+                                                    _I, _I]
+        L.sphg_random_variable_generate.restype = _I
         L.sphg_create_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Problem), C.POINTER(Materials), _I,
                                         C.POINTER(C.c_int), C.POINTER(Box), _I]
         L.sphg_create_batch.restype = _I
@@ -578,11 +627,29 @@ class GpuSim(Sim):
         return [(bytes(names[i]).split(b"\0", 1)[0].decode(), ms[i], cnt[i]) for i in range(k)]
 
     def kl_synthesize(self, f: "KLField", xi: np.ndarray) -> None:
-        """sphg_kl_synthesize: K-L field (soil cohesion) into the per-particle cohesion."""
+        """sphg_kl_synthesize: K-L field into the per-particle property f.field."""
         xi = np.ascontiguousarray(xi, dtype=np.float64)
         rc = self.L.lib.sphg_kl_synthesize(self.ctx, C.byref(f), _dptr(xi))
         if rc:
             raise RuntimeError(f"sphg_kl_synthesize: error {rc}")
+
+    def _range(self, r):
+        if r is None:
+            return 0, self.n
+        a, b = int(self.offsets[r]), int(self.offsets[r + 1])
+        return a, b - a
+
+    def kl_synthesize_corr(self, fa: "KLField", fb: "KLField", coe: float, u: np.ndarray, r=None) -> int:
+        """sphg_kl_synthesize_corr(_part): correlated fai / c pair from u[200][2]; returns the code."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(200, 2)
+        a, cnt = self._range(r)
+        return self.L.lib.sphg_kl_synthesize_corr_part(self.ctx, C.byref(fa), C.byref(fb), coe, _dptr(u), a, cnt)
+
+    def random_variable_generate(self, flags, coe, psv, zeta, r=None) -> int:
+        """sphg_random_variable_generate over the context (or realisation r); returns the code."""
+        fl, ps = rv_args(flags, coe, psv)
+        a, cnt = self._range(r)
+        return self.L.lib.sphg_random_variable_generate(self.ctx, C.byref(fl), ps, (_D * 2)(*zeta), a, cnt)
 
     def device_bytes(self) -> int:
         b = C.c_size_t()

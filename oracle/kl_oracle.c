@@ -168,3 +168,86 @@ int or_kl_synthesize(int ndim, int n, const double *xp, const int *type, const d
     }
     return 0;
 }
+
+/* Standard normals of the correlated pair of SpatialVariables_Generate_2D_corr / _3D_corr
+ * (CPP_17-2_K-L_expansion_corr.cpp:114-140 soil, :183-209 fluid): the coefficient is
+ * coe_frict_cohesion, or for a lognormal first field log(1 + coe cv_a cv_b) divided by
+ * sqrt(log(1 + cv_a^2) log(1 + cv_b^2)); u[200][2] times [[1, coef], [0, sqrt(1 - coef^2)]] in
+ * matrix_multi's order (:22-37) gives the fai normals (column 0) and the c normals (column 1). */
+void or_kl_corr_normals(int dist_a, double mean_a, double sd_a, double mean_b, double sd_b, double coe,
+                        const double *u, double *xi_a, double *xi_b, double *coef_out) {
+    double coef = coe, cov[4], zeta[2 * KL_TERMS];
+    int i, j, k;
+    if (dist_a == 2) {
+        coef = log(1.0 + coe * (sd_a / mean_a) * (sd_b / mean_b));
+        coef = coef / sqrt(log(1.0 + sd_a * sd_a / mean_a / mean_a) * log(1.0 + sd_b * sd_b / mean_b / mean_b));
+    }
+    cov[0] = 1.0;
+    cov[1] = coef;
+    cov[2] = 0.0;
+    cov[3] = sqrt(1.0 - coef * coef);
+    for (i = 0; i < KL_TERMS; i++)
+        for (j = 0; j < 2; j++) {
+            double temp = 0.0;
+            for (k = 0; k < 2; k++) temp = temp + u[i * 2 + k] * cov[k * 2 + j];
+            zeta[i * 2 + j] = temp;
+        }
+    for (i = 0; i < KL_TERMS; i++) {
+        xi_a[i] = zeta[2 * i];
+        xi_b[i] = zeta[2 * i + 1];
+    }
+    if (coef_out) *coef_out = coef;
+}
+
+/* random_variable_generate (CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863). flags: flag_fai,
+ * flag_c, flag_faiw, flag_cw; dist[6], mean[6], sd[6] indexed like the reference's psv (0 soil fai,
+ * 1 soil c, 4 fluid fai, 5 fluid c); zeta[2] the standard normals. The distribution is psv[0]'s
+ * when flag_fai is set, else psv[4]'s when flag_faiw is set, else the constructor's 1; the
+ * coefficient as above (the lognormal form only with flag_fai or flag_faiw); the pair
+ * zeta [[1, coef], [0, sqrt(1 - coef^2)]]; fai takes its second entry, c its first. vals[6][n]:
+ * each flagged property of every soil (type 2: 0, 1) or water (type 1: 4, 5) particle. */
+void or_random_variable(const int flags[4], double coe, const int dist_of[6], const double mean_of[6],
+                        const double sd_of[6], const double zeta[2], int n, const int *type, double *vals) {
+    int dist = 1, i, q, j, k;
+    double coef, cov[4], z1[2];
+    static const int slot[4] = {0, 1, 4, 5}, zi[4] = {1, 0, 1, 0}, ph[4] = {2, 2, 1, 1};
+    if (flags[0] == 1) dist = dist_of[0];
+    else if (flags[2] == 1) dist = dist_of[4];
+    if (dist == 2 && flags[0] == 1) {
+        coef = log(1.0 + coe * (sd_of[0] / mean_of[0]) * (sd_of[1] / mean_of[1]));
+        coef = coef / sqrt(log(1.0 + sd_of[0] * sd_of[0] / mean_of[0] / mean_of[0]) *
+                           log(1.0 + sd_of[1] * sd_of[1] / mean_of[1] / mean_of[1]));
+    } else if (dist == 2 && flags[2] == 1) {
+        coef = log(1.0 + coe * (sd_of[4] / mean_of[4]) * (sd_of[5] / mean_of[5]));
+        coef = coef / sqrt(log(1.0 + sd_of[4] * sd_of[4] / mean_of[4] / mean_of[4]) *
+                           log(1.0 + sd_of[5] * sd_of[5] / mean_of[5] / mean_of[5]));
+    } else {
+        coef = coe;
+    }
+    cov[0] = 1.0;
+    cov[1] = coef;
+    cov[2] = 0.0;
+    cov[3] = sqrt(1.0 - coef * coef);
+    for (j = 0; j < 2; j++) {
+        double temp = 0.0;
+        for (k = 0; k < 2; k++) temp = temp + zeta[k] * cov[k * 2 + j];
+        z1[j] = temp;
+    }
+    for (q = 0; q < 4; q++) {
+        double mean, sd, v;
+        if (flags[q] != 1) continue;
+        mean = mean_of[slot[q]];
+        sd = sd_of[slot[q]];
+        if (dist == 1) {
+            v = mean + sd * z1[zi[q]];
+        } else if (dist == 2) {
+            sd = sqrt(log(1.0 + (sd / mean) * (sd / mean)));
+            mean = log(mean) - 0.5 * sd * sd;
+            v = exp(mean + sd * z1[zi[q]]);
+        } else {
+            continue;
+        }
+        for (i = 0; i < n; i++)
+            if (type[i] == ph[q]) vals[(size_t)slot[q] * n + i] = v;
+    }
+}

@@ -243,17 +243,22 @@ int chol_mode(const char *dir) {
     return 0;
 }
 
-// Correlation-matrix random-field modes: --rf runs the reference's nonstationary_random_2D/3D
-// (CPP_17-3-Nonstationary-RF.cpp:76-849), --rfc its nonstationary_correlated_random_2D/3D
-// (CPP_17-5-Nonstationary-Correlated-RF.cpp:87-737, the coefficient coe_frict_cohesion read after
-// the properties) on given particles. Its standard normals come from a
-// clock-seeded generator, so the outputs are reproducible only where sd_value is 0.
+// Random-field modes on given particles, kind 0..4: --rf runs the reference's
+// nonstationary_random_2D/3D (CPP_17-3-Nonstationary-RF.cpp:76-849), --rfc its
+// nonstationary_correlated_random_2D/3D (CPP_17-5-Nonstationary-Correlated-RF.cpp:87-737), --kln
+// SpatialVariables_Generate_2D/3D (CPP_17-1_K-L_expansion_non.cpp:63-265), --klc
+// SpatialVariables_Generate_2D_corr/3D_corr (CPP_17-2_K-L_expansion_corr.cpp:55-473) and --rv
+// random_variable_generate (CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863); every kind but --rf
+// reads the coefficient coe_frict_cohesion after the properties. Their standard normals come from
+// a clock-seeded generator, so the outputs are reproducible only where sd_value is 0 (or where
+// the values do not depend on them). The K-L kinds append their parameters to
+// Para_Spatial_Dist.txt in the working directory.
 // rf_in.bin: int32 ndim, n; double dr, box_nb[6] (xmin xmax ymin ymax zmin zmax of the non-boundary
 //            particles), soil_max[3], water_max[3]; int32 flags[6]; double cell_ndx; int32 flag_auto;
 //            double beta, beta_1; 6 x {int32 type; double mean, sd, rl[3], coe_depth};
 //            double xp[n][3]; int32 type[n]; double fai[n], c[n], cop[n], ds[n], fai_w[n], c_w[n]
-// rf_out.bin: double fai[n], c[n], cop[n], ds[n], fai_w[n], c_w[n]; int32 rc
-int rf_mode(const char *dir, bool correlated) {
+// rf_out.bin: double fai[n], c[n], cop[n], ds[n], fai_w[n], c_w[n]; int32 rc; double a[6][3]
+int rf_mode(const char *dir, int kind) {
     std::string in = std::string(dir) + "/rf_in.bin", out = std::string(dir) + "/rf_out.bin";
     FILE *f = fopen(in.c_str(), "rb");
     if (!f) return 2;
@@ -278,7 +283,7 @@ int rf_mode(const char *dir, bool correlated) {
     ok = ok && fread(xp.data(), 8, xp.size(), f) == xp.size() && fread(type.data(), 4, n, f) == (size_t)n &&
          fread(vals.data(), 8, vals.size(), f) == vals.size();
     double coe = 0.0;
-    if (correlated) ok = ok && fread(&coe, 8, 1, f) == 1;
+    if (kind != 0) ok = ok && fread(&coe, 8, 1, f) == 1;
     fclose(f);
     if (!ok) return 2;
     Particle *par = new Particle[n];
@@ -315,9 +320,17 @@ int rf_mode(const char *dir, bool correlated) {
     }
     clFSpatialVariability_Fun fn;
     int rc;
-    if (correlated)
+    if (kind == 1)
         rc = ndim == 2 ? fn.nonstationary_correlated_random_2D(par, pf, ps, pro, fsv, psv, cc)
                        : fn.nonstationary_correlated_random_3D(par, pf, ps, pro, fsv, psv, cc);
+    else if (kind == 2)
+        rc = ndim == 2 ? fn.SpatialVariables_Generate_2D_corr(par, pf, ps, pro, fsv, psv, cc)
+                       : fn.SpatialVariables_Generate_3D_corr(par, pf, ps, pro, fsv, psv, cc);
+    else if (kind == 3)
+        rc = ndim == 2 ? fn.SpatialVariables_Generate_2D(par, pf, ps, pro, fsv, psv, cc)
+                       : fn.SpatialVariables_Generate_3D(par, pf, ps, pro, fsv, psv, cc);
+    else if (kind == 4)
+        rc = fn.random_variable_generate(par, pf, ps, pro, fsv, psv);
     else
         rc = ndim == 2 ? fn.nonstationary_random_2D(par, pf, ps, pro, fsv, psv, cc)
                        : fn.nonstationary_random_3D(par, pf, ps, pro, fsv, psv, cc);
@@ -333,6 +346,7 @@ int rf_mode(const char *dir, bool correlated) {
     if (!o) return 2;
     fwrite(vals.data(), 8, vals.size(), o);
     fwrite(&rc, 4, 1, o);
+    for (int k = 0; k < 6; k++) fwrite(psv[k].a, 8, 3, o);  // the K-L half-widths the generators set
     fclose(o);
     delete[] par;
     delete[] ps;
@@ -343,10 +357,13 @@ int rf_mode(const char *dir, bool correlated) {
 int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--kl") == 0) return kl_mode(argv[2]);
     if (argc >= 3 && strcmp(argv[1], "--chol") == 0) return chol_mode(argv[2]);
-    if (argc >= 3 && strcmp(argv[1], "--rf") == 0) return rf_mode(argv[2], false);
-    if (argc >= 3 && strcmp(argv[1], "--rfc") == 0) return rf_mode(argv[2], true);
+    if (argc >= 3 && strcmp(argv[1], "--rf") == 0) return rf_mode(argv[2], 0);
+    if (argc >= 3 && strcmp(argv[1], "--rfc") == 0) return rf_mode(argv[2], 1);
+    if (argc >= 3 && strcmp(argv[1], "--klc") == 0) return rf_mode(argv[2], 2);
+    if (argc >= 3 && strcmp(argv[1], "--kln") == 0) return rf_mode(argv[2], 3);
+    if (argc >= 3 && strcmp(argv[1], "--rv") == 0) return rf_mode(argv[2], 4);
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] <dir>\n", argv[0]);
+        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] | --klc | --kln | --rv <dir>\n", argv[0]);
         return 2;
     }
     return SPH_Calculation(3, argv[1]);

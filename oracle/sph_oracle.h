@@ -51,6 +51,12 @@ int or_kl_synthesize(int ndim, int n, const double *xp, const int *type, const d
                      const double *eigval, const int order_id[][3], const double *uu, const double x_trans[3],
                      const double a[3], int dist, double mean_value, double sd_value, double coe_depth,
                      double soil_max, double water_max, int field, double *value);
+/* Correlated pair of SpatialVariables_Generate_2D_corr / _3D_corr (CPP_17-2_K-L_expansion_corr.cpp). */
+void or_kl_corr_normals(int dist_a, double mean_a, double sd_a, double mean_b, double sd_b, double coe,
+                        const double *u, double *xi_a, double *xi_b, double *coef_out);
+/* random_variable_generate (CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863). */
+void or_random_variable(const int flags[4], double coe, const int dist_of[6], const double mean_of[6],
+                        const double sd_of[6], const double zeta[2], int n, const int *type, double *vals);
 
 #ifdef __cplusplus
 }

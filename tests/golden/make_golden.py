@@ -60,6 +60,16 @@ def all_umcc(deck):
     return d
 
 
+def with_bingham(deck, fai=5.0, c=2.0, strain_min=1.0):
+    """The deck with Bingham water (bhtype 1): viscosity vis + (pre tan(fai) + c) / max(strain rate,
+    strain_min) (CPP_7-2-constitutive.cpp:38-47)."""
+    d = deck.copy()
+    d.water = [dict(w) for w in d.water]
+    for w in d.water:
+        w.update(bhtype=1, fai=fai, c=c, strain_min=strain_min)
+    return d
+
+
 # soil velocity of the slope-with-pile fixtures: towards the pile's near face, with a sliding part
 PILE_V0 = (-0.3, 0.1, -0.05)
 
@@ -115,6 +125,9 @@ def decks():
         "ssi_3d_t2": (ssi_block(3, 2, nx=5, ny=3, nz=3, steps=5), 5, (1, 5)),
         "pile3d_amcc": (slope_pile_3d(nx=16, ny=8, nz=10, pile=(3, 3), model=4, steps=5, v0=PILE_V0), 5, (1, 3, 5)),
         "pile3d_sb": (slope_pile_3d(nx=16, ny=8, nz=10, pile=(3, 3), model=1, steps=5, v0=PILE_V0), 5, (1, 3, 5)),
+        "bingham_2d": (with_bingham(tran_box(2, 3, steps=6, v0=0.5)), 6, (1, 3, 6)),
+        "bingham_3d": (with_bingham(tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, v0=0.5), fai=8.0,
+                                    c=0.5, strain_min=50.0), 5, (1, 5)),
     }
 
 

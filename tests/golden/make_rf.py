@@ -8,8 +8,15 @@
   and return codes it leaves.
 - rf_pair.npz: the same for its correlated friction angle and cohesion
   (nonstationary_correlated_random_2D/3D, `ref_harness --rfc`), normal distribution.
+- kl_spv.npz: the reference's whole K-L generators on seeded decks with sd_value 0 -
+  SpatialVariables_Generate_2D/3D (flspv 1, `--kln`, every property flag) and
+  SpatialVariables_Generate_2D_corr/3D_corr (flspv 2, `--klc`) - and its homogeneous correlated
+  random variables (random_variable_generate, flspv 5, `--rv`): the properties and return codes
+  they leave. Where sd is 0 the normals drop out of the normal branch; in the lognormal branch the
+  correlation coefficient is 0 / 0 (NaN), which the properties fed by the mixed normal carry.
 
-    python tests/golden/make_rf.py
+    python tests/golden/make_rf.py          # all
+    python tests/golden/make_rf.py spv      # kl_spv.npz only
 """
 from __future__ import annotations
 
@@ -83,6 +90,55 @@ def pair_inputs(name):
     fa = rf_field(d, 1, means[0], 0.0, (1.0, 0.5, 0.7), field=ffai, acf=1, cell_ndx=cndx, coe_depth=cdeps[0])
     fc = rf_field(d, 1, means[1], 0.0, (1.0, 0.5, 0.7), field=ffai + 1, acf=1, cell_ndx=cndx, coe_depth=cdeps[1])
     return d, fa, fc, coe
+
+
+# flspv 1 at sd 0: (deck, {field: (dist, mean, coe_depth)})
+KLN_CASES = {
+    "kln_slope_c": ("slope", {1: (1, 9000.0, 2500.0)}),
+    "kln_box2d_six": ("box2d", {0: (1, 25.0, 1.5), 1: (2, 800.0, 30.0), 2: (1, 1e-4, 0.01), 3: (2, 0.002, 0.05),
+                                4: (1, 5.0, 100.0), 5: (2, 3.0, 50.0)}),
+    "kln_box3d_three": ("box3d", {0: (2, 30.0, 200.0), 1: (1, 700.0, 9000.0), 5: (1, 2.0, 40.0)}),
+}
+
+# flspv 2 at sd 0: (deck, {pair's first field: ((dist, mean, coe_depth) of fai, of c)}, coe)
+KLC_CASES = {
+    "klc_slope_soil": ("slope", {0: ((1, 25.0, 1.5), (1, 9000.0, 2500.0))}, 0.4),
+    "klc_box2d_fluid": ("box2d", {4: ((1, 3.0, 80.0), (1, 0.5, 20.0))}, -0.3),
+    "klc_box3d_soil_lognormal": ("box3d", {0: ((2, 30.0, 300.0), (2, 700.0, 9000.0))}, 0.6),
+    "klc_box2d_both": ("box2d", {0: ((1, 28.0, 2.0), (2, 600.0, 50.0)), 4: ((2, 4.0, 90.0), (1, 0.8, 10.0))}, 0.25),
+}
+
+# flspv 5 at sd 0: (deck, (flag_fai, flag_c, flag_faiw, flag_cw), {psv index: (dist, mean)}, coe)
+RV_CASES = {
+    "rv_soil_normal": ("box2d", (1, 1, 0, 0), {0: (1, 25.0), 1: (1, 9000.0)}, 0.5),
+    "rv_soil_lognormal": ("box2d", (1, 1, 0, 0), {0: (2, 25.0), 1: (2, 9000.3)}, 0.5),
+    "rv_fluid_lognormal": ("box3d", (0, 0, 1, 1), {4: (2, 3.3), 5: (2, 0.7)}, -0.2),
+    "rv_c_only_default_normal": ("box2d", (0, 1, 0, 0), {1: (2, 9000.3)}, 0.5),
+    "rv_all_from_soil": ("box3d", (1, 1, 1, 1), {0: (2, 31.0), 1: (2, 650.7), 4: (1, 5.0), 5: (1, 1.1)}, 0.3),
+}
+
+
+def spv_decks():
+    """Decks of the K-L and random-variable cases: the K-L roots need domains that are not small
+    against the correlation lengths (the reference's bisection never ends otherwise), so the
+    boxes use dx = 0.1 m; the 3D box keeps its walls (the K-L generators skip them)."""
+    return {
+        "slope": slope_2d(1.0, dx=0.1, seed=3),
+        "box2d": tran_box(2, 3, nx=10, ny=3, nz_soil=5, nz_water=4, dx=0.1, seed=41),
+        "box3d": tran_box(3, 3, nx=6, ny=4, nz_soil=4, nz_water=3, dx=0.1, seed=43),
+    }
+
+
+def spv_inputs(d):
+    """Harness arguments shared by the K-L and random-variable cases: the non-boundary box, the
+    phase tops as input_par computes them, and six properties to start from."""
+    nb = (d.type != 0) & (d.type != 7)
+    box = []
+    for k in range(3):
+        box += [min(10000.0, float(d.xp[nb, k].min())), max(-10000.0, float(d.xp[nb, k].max()))]
+    _, _, ph = d.domain()
+    init = [np.random.default_rng(9).uniform(1.0, 2.0, d.n) for _ in range(6)]
+    return box, ph["soil"][1], ph["water"][1], init
 
 
 def chol_matrix(name):
@@ -173,7 +229,50 @@ def main() -> None:
         out[f"{name}/rc"] = np.array([rc], np.int32)
         print(name, "rc", rc)
     np.savez_compressed(os.path.join(HERE, "rf_pair.npz"), **out)
+    main_spv()
+
+
+def main_spv() -> None:
+    out = {}
+    rl = (1.0, 0.5, 0.7)
+    for kind, cases in (("kln", KLN_CASES), ("klc", KLC_CASES), ("rv", RV_CASES)):
+        for name, case in cases.items():
+            d = spv_decks()[case[0]]
+            box, smax3, wmax3, init = spv_inputs(d)
+            flags = [0] * 6
+            psv = [(1, 0.0, 0.0, rl, 0.0)] * 6
+            coe = 0.0
+            if kind == "kln":
+                for fld, (dist, mean, cdep) in case[1].items():
+                    flags[fld] = 1
+                    psv[fld] = (dist, mean, 0.0, rl, cdep)
+            elif kind == "klc":
+                coe = case[2]
+                for fld, (pa, pb) in case[1].items():
+                    flags[fld] = flags[fld + 1] = 1
+                    psv[fld] = (pa[0], pa[1], 0.0, rl, pa[2])
+                    psv[fld + 1] = (pb[0], pb[1], 0.0, rl, pb[2])
+            else:
+                coe = case[3]
+                for q, fld in enumerate((0, 1, 4, 5)):
+                    flags[fld] = case[1][q]
+                for fld, (dist, mean) in case[2].items():
+                    psv[fld] = (dist, mean, 0.0, rl, 0.0)
+            res = run_ref_rf(d.ndim, d.xp, d.type, d.dr, box, smax3, wmax3, flags, 1.0, 1, 0.0, 0.0, psv, init,
+                             coe=coe, kind=kind)
+            vals, rc = res[0], res[1]
+            if kind != "rv":
+                out[f"{name}/a"] = res[2]
+            out[f"{name}/init"] = np.array(init)
+            out[f"{name}/value"] = vals
+            out[f"{name}/rc"] = np.array([rc], np.int32)
+            changed = [k for k in range(6) if not np.array_equal(vals[k], init[k])]
+            print(name, "rc", rc, "changed", changed, "nan", [k for k in range(6) if np.isnan(vals[k]).any()])
+    np.savez_compressed(os.path.join(HERE, "kl_spv.npz"), **out)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["spv"]:
+        main_spv()
+    else:
+        main()

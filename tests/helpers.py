@@ -257,12 +257,15 @@ def run_ref_chol(A):
 
 
 def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx, flag_auto, beta, beta_1, psv,
-               init, correlated=False, coe=0.0):
+               init, correlated=False, coe=0.0, kind=None):
     """The reference's own nonstationary_random_2D/3D (or, correlated, its
-    nonstationary_correlated_random_2D/3D with coe_frict_cohesion = coe) through the link harness.
-    psv: six tuples (type, mean, sd, rl[3], coe_depth); init: six n-arrays (soil fai c cop ds,
-    fluid fai c). Returns (six n-arrays after, rc). Reproducible only where the flagged fields
-    have sd 0."""
+    nonstationary_correlated_random_2D/3D with coe_frict_cohesion = coe) through the link harness;
+    kind "klc" runs SpatialVariables_Generate_2D/3D_corr, "kln" SpatialVariables_Generate_2D/3D and
+    "rv" random_variable_generate on the same inputs instead. psv: six tuples (type, mean, sd,
+    rl[3], coe_depth); init: six n-arrays (soil fai c cop ds, fluid fai c). Returns (six n-arrays
+    after, rc). Reproducible only where the values do not depend on the clock-seeded normals."""
+    if kind is None:
+        kind = "rfc" if correlated else "rf"
     import struct
     wd = tempfile.mkdtemp(prefix="sphrf_")
     try:
@@ -278,15 +281,17 @@ def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx,
             f.write(np.ascontiguousarray(xp, "<f8").tobytes() + np.ascontiguousarray(ty, "<i4").tobytes())
             for a in init:
                 f.write(np.ascontiguousarray(a, "<f8").tobytes())
-            if correlated:
+            if kind != "rf":
                 f.write(struct.pack("<d", coe))
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        r = subprocess.run([REF_HARNESS, "--rfc" if correlated else "--rf", wd], capture_output=True, env=env)
+        r = subprocess.run([REF_HARNESS, "--" + kind, wd], capture_output=True, env=env, cwd=wd)
         if r.returncode != 0:
             raise RuntimeError(f"reference harness --rf failed rc={r.returncode}")
         raw = open(os.path.join(wd, "rf_out.bin"), "rb").read()
         vals = np.frombuffer(raw, "<f8", 6 * n).reshape(6, n).copy()
         rc = int(np.frombuffer(raw, "<i4", 1, 6 * n * 8)[0])
+        if kind in ("klc", "kln"):  # and the half-widths a of the K-L domain it set, per property
+            return vals, rc, np.frombuffer(raw, "<f8", 18, 6 * n * 8 + 4).reshape(6, 3).copy()
         return vals, rc
     finally:
         shutil.rmtree(wd, ignore_errors=True)
@@ -316,3 +321,30 @@ def oracle_rf_pair(ndim, xp, ty, f_fai, f_c, coe, zeta1, fill=0.0):
                         D(f_fai.water_max), ip(nc), dp(base), D(cells["cell_dx"]), cells["total"], dp(Lm), dp(zeta1),
                         n, ip(ty), ip(cells["cell_of"]), D(fill), dp(va), dp(vc))
     return 0, va, vc
+
+
+def oracle_kl_corr_normals(dist_a, mean_a, sd_a, mean_b, sd_b, coe, u):
+    """Oracle mixing of the flspv 2 pair (or_kl_corr_normals): returns xi_a, xi_b, coefficient."""
+    L = oracle_lib().lib
+    D = C.c_double
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(200, 2)
+    xa, xb, cf = np.zeros(200), np.zeros(200), D()
+    dp = lambda a_: a_.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    L.or_kl_corr_normals(dist_a, D(mean_a), D(sd_a), D(mean_b), D(sd_b), D(coe), dp(u), dp(xa), dp(xb), C.byref(cf))
+    return xa, xb, cf.value
+
+
+def oracle_random_variable(flags, coe, psv, zeta, ty, init):
+    """Oracle random_variable_generate (or_random_variable): flags (fai, c, faiw, cw), psv six
+    (type, mean, sd), zeta[2]; init six n-arrays; returns the six n-arrays after."""
+    L = oracle_lib().lib
+    D = C.c_double
+    ty = np.ascontiguousarray(ty, np.int32)
+    n = ty.shape[0]
+    vals = np.ascontiguousarray(np.array(init, dtype=np.float64).reshape(6, n)).copy()
+    dist = (C.c_int * 6)(*[int(p[0]) for p in psv])
+    mean = (D * 6)(*[float(p[1]) for p in psv])
+    sd = (D * 6)(*[float(p[2]) for p in psv])
+    L.or_random_variable((C.c_int * 4)(*[int(v) for v in flags]), D(coe), dist, mean, sd, (D * 2)(*zeta), n,
+                         ty.ctypes.data_as(C.POINTER(C.c_int)), vals.ctypes.data_as(C.POINTER(C.c_double)))
+    return vals

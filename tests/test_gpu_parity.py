@@ -343,15 +343,16 @@ def test_per_particle_friction_angle_and_permeability():
 
 
 def test_other_per_particle_materials_must_equal_their_slot():
-    # per-particle copies the step takes from the material slots: equal values are accepted,
-    # different ones refused (102) instead of being dropped
+    # per-particle copies the step takes from the material slots (all but the random-field
+    # properties ps_fai, ps_c, ps_cop, ps_ds, pf_fai, pf_c): equal values are accepted, different
+    # ones refused (102) instead of being dropped
     deck, states = load_golden("drag_2d_fli2")
     g = GpuSim(deck)
     g.load_state(states[0])
     g.set("ps_e", states[0]["ps_e"])
     g.set("pf_vis", states[0]["pf_vis"])
     g.upload_done()
-    g.set("pf_fai", states[0]["pf_fai"] + (deck.type == 1) * 5.0)
+    g.set("pf_vis", states[0]["pf_vis"] + (deck.type == 1) * 5.0)
     with pytest.raises(RuntimeError, match="error 102"):
         g.upload_done()
 
@@ -553,6 +554,27 @@ def test_permeability_laws_vs_oracle(name):
         assert compare_tol(g, o, TRANSCENDENTAL_TOL) == {}, f"step {s + 1}"
     soil = deck.type == 2
     assert bitequal(g.get("cop")[soil], o.get("cop")[soil])
+
+
+@pytest.mark.parametrize("name", ["bingham_2d", "bingham_3d"])
+def test_bingham_water_vs_oracle(name):
+    # bhtype 1: the water's viscosity grows by (pre tan(fai) + c) / max(strain rate, strain_min),
+    # tan evaluated on the host per particle: bit-exact
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(5):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare(g, o) == {}, f"step {s + 1}"
+    last = max(states)
+    g2 = make_gpu(deck, states[0])
+    assert g2.step(last) == 0
+    oj = OracleSim(deck, states[0], mode=JACOBI)
+    assert oj.step(last) == 0
+    for k in ("xp", "vx", "sig", "rho"):
+        form = nan_rel(oj.get(k), states[last][k])
+        assert nan_rel(g2.get(k), states[last][k]) <= form, k
 
 
 @pytest.mark.parametrize("name", ["pile3d_amcc", "pile3d_sb"])

@@ -37,7 +37,7 @@ GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1"
           "drag_3d_fli2", "drag_3d_fli3", "kgc_column2d_k3", "kgc_zoo2d_k4", "kgc_ssi2d_k3", "kgc_art3d_k3",
           "kgc_tran3d_k4", "ts_column2d", "ts_art3d", "ts_drag3d", "air_water_2d_a0", "air_water_2d_a1",
           "air_water_3d_a0", "air_water_3d_a1", "tm_2d_b1", "tm_3d_b1", "umcc_perm2", "zoo_perm3", "ssi_3d_t2",
-          "pile3d_amcc", "pile3d_sb"]
+          "pile3d_amcc", "pile3d_sb", "bingham_2d", "bingham_3d"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
