@@ -467,6 +467,15 @@ int sphg_rf_generate_pair(const sphg_rf_field *f_fai, const sphg_rf_field *f_c, 
                           int n, const double *xp, const int *type, int nreal, const double *zeta, double fill,
                           double *value_fai, double *value_c);
 
+/* Per-particle random-field values from a file (flspv -1, random_field_input,
+ * CPP_17-1_K-L_expansion_non.cpp:718-772): <dir>/Parameters.txt with a header line and one line
+ * per particle (id, x, y, z, four values, type, matype). values[6][n] (soil fai, c, cop, ds, fluid
+ * fai, c) keeps what the caller put there except the values the file gives: fai, c, cop, ds of
+ * soil and structure particles, fai and c of water and air particles. Host only; SPHG_E_ARG for a
+ * file that cannot be opened (the reference reads from it before the check). The values enter a
+ * context with sphg_set_field ("ps_fai" ... "pf_c"). */
+int sphg_read_parameters(const char *dir, int n, const int *type, double *values);
+
 /* ---- Correlated random variables (flspv 5, CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863) -- */
 
 /* Flag_SpatialVariability members random_variable_generate reads (Class_spatial_variability.h:10-42) */

@@ -299,6 +299,38 @@ extern "C" int sphg_read_input(const char *path, sphg_problem *pro, sphg_materia
     return SPHG_OK;
 }
 
+// random_field_input (CPP_17-1_K-L_expansion_non.cpp:718-772): <dir>/Parameters.txt, a header line
+// then one line per particle in particle order; for water and air the 5th and 6th values are the
+// fluid fai and c, for soil and structure the 5th to 8th are fai, c, cop and ds; any other
+// particle's line is skipped with the reference's two-character scan, whose first character is
+// the newline the previous numeric line left (so one line is skipped either way). The reference
+// scans before testing the file pointer; a missing file is SPHG_E_ARG here.
+extern "C" int sphg_read_parameters(const char *dir, int n, const int *type, double *values) {
+    if (!dir || !type || !values || n <= 0) return SPHG_E_ARG;
+    std::string path = std::string(dir) + "/Parameters.txt";
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return SPHG_E_ARG;
+    char c[2];
+    int id, ty, ma;
+    double x[3], tmp[2];
+    const size_t N = (size_t)n;
+    int err = fscanf(f, "%c%c%*[^\n]%*c", &c[0], &c[1]);
+    for (int i = 0; i < n; i++) {
+        const int t = type[i];
+        if (t == 1 || t == 3)
+            err = fscanf(f, "%d%lf%lf%lf%lf%lf%lf%lf%d%d", &id, &x[0], &x[1], &x[2], &values[4 * N + i],
+                         &values[5 * N + i], &tmp[0], &tmp[1], &ty, &ma);
+        else if (t == 2 || t == 4)
+            err = fscanf(f, "%d%lf%lf%lf%lf%lf%lf%lf%d%d", &id, &x[0], &x[1], &x[2], &values[i], &values[N + i],
+                         &values[2 * N + i], &values[3 * N + i], &ty, &ma);
+        else
+            err = fscanf(f, "%c%c%*[^\n]%*c", &c[0], &c[1]);
+    }
+    (void)err;
+    fclose(f);
+    return SPHG_OK;
+}
+
 extern "C" int sphg_write_odb_fields(const char *dir, int l, double t, int n, const sphg_odb_fields *f) {
     if (!dir || !f || n < 0) return SPHG_E_ARG;
     const double *v3[] = {f->xp, f->ux, f->ax, f->vx, f->interf, f->interfss, f->strep};

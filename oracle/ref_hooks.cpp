@@ -247,8 +247,9 @@ int chol_mode(const char *dir) {
 // nonstationary_random_2D/3D (CPP_17-3-Nonstationary-RF.cpp:76-849), --rfc its
 // nonstationary_correlated_random_2D/3D (CPP_17-5-Nonstationary-Correlated-RF.cpp:87-737), --kln
 // SpatialVariables_Generate_2D/3D (CPP_17-1_K-L_expansion_non.cpp:63-265), --klc
-// SpatialVariables_Generate_2D_corr/3D_corr (CPP_17-2_K-L_expansion_corr.cpp:55-473) and --rv
-// random_variable_generate (CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863); every kind but --rf
+// SpatialVariables_Generate_2D_corr/3D_corr (CPP_17-2_K-L_expansion_corr.cpp:55-473), --rv
+// random_variable_generate (CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863) and --rfi
+// random_field_input (CPP_17-1_K-L_expansion_non.cpp:718-772, <dir>/Parameters.txt); every kind but --rf
 // reads the coefficient coe_frict_cohesion after the properties. Their standard normals come from
 // a clock-seeded generator, so the outputs are reproducible only where sd_value is 0 (or where
 // the values do not depend on them). The K-L kinds append their parameters to
@@ -331,6 +332,8 @@ int rf_mode(const char *dir, int kind) {
                        : fn.SpatialVariables_Generate_3D(par, pf, ps, pro, fsv, psv, cc);
     else if (kind == 4)
         rc = fn.random_variable_generate(par, pf, ps, pro, fsv, psv);
+    else if (kind == 5)
+        rc = fn.random_field_input(par, pf, ps, pro, (char *)dir);
     else
         rc = ndim == 2 ? fn.nonstationary_random_2D(par, pf, ps, pro, fsv, psv, cc)
                        : fn.nonstationary_random_3D(par, pf, ps, pro, fsv, psv, cc);
@@ -362,8 +365,9 @@ int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--klc") == 0) return rf_mode(argv[2], 2);
     if (argc >= 3 && strcmp(argv[1], "--kln") == 0) return rf_mode(argv[2], 3);
     if (argc >= 3 && strcmp(argv[1], "--rv") == 0) return rf_mode(argv[2], 4);
+    if (argc >= 3 && strcmp(argv[1], "--rfi") == 0) return rf_mode(argv[2], 5);
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] | --klc | --kln | --rv <dir>\n", argv[0]);
+        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] | --klc | --kln | --rv | --rfi <dir>\n", argv[0]);
         return 2;
     }
     return SPH_Calculation(3, argv[1]);

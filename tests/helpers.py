@@ -257,11 +257,12 @@ def run_ref_chol(A):
 
 
 def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx, flag_auto, beta, beta_1, psv,
-               init, correlated=False, coe=0.0, kind=None):
+               init, correlated=False, coe=0.0, kind=None, files=None):
     """The reference's own nonstationary_random_2D/3D (or, correlated, its
     nonstationary_correlated_random_2D/3D with coe_frict_cohesion = coe) through the link harness;
     kind "klc" runs SpatialVariables_Generate_2D/3D_corr, "kln" SpatialVariables_Generate_2D/3D and
-    "rv" random_variable_generate on the same inputs instead. psv: six tuples (type, mean, sd,
+    "rv" random_variable_generate and "rfi" random_field_input (files: {name: text} written into
+    the working directory first, Parameters.txt for "rfi") on the same inputs instead. psv: six tuples (type, mean, sd,
     rl[3], coe_depth); init: six n-arrays (soil fai c cop ds, fluid fai c). Returns (six n-arrays
     after, rc). Reproducible only where the values do not depend on the clock-seeded normals."""
     if kind is None:
@@ -270,6 +271,9 @@ def run_ref_rf(ndim, xp, ty, dr, box_nb, soil_max3, water_max3, flags, cell_ndx,
     wd = tempfile.mkdtemp(prefix="sphrf_")
     try:
         n = xp.shape[0]
+        for fname, text in (files or {}).items():
+            with open(os.path.join(wd, fname), "w") as f:
+                f.write(text)
         with open(os.path.join(wd, "rf_in.bin"), "wb") as f:
             f.write(struct.pack("<ii", ndim, n) + struct.pack("<d", dr))
             f.write(np.asarray(box_nb, "<f8").tobytes() + np.asarray(soil_max3, "<f8").tobytes() +

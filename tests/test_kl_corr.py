@@ -226,3 +226,93 @@ def test_reference_random_variable_live():
     assert rc == 0
     got = oracle_random_variable((1, 1, 0, 0), -0.35, [p[:3] for p in psv], (0.1, 0.2), d.type, init)
     assert np.array_equal(_bits(got), _bits(vals))
+
+
+# ------------------------------------------------------------------ Parameters.txt (flspv -1)
+def _parameters_text(d, vals, with_boundary_lines):
+    """A Parameters.txt in the layout of the reference's writer (CPP_18-spatialoutput.cpp:34-52),
+    which lists water, soil, air and structure particles only; with_boundary_lines adds a line for
+    every other particle too."""
+    rows = ["No.---- X----Y----Z----fai----c----cop----ds----type----matype----"]
+    n = d.n
+    for i in range(n):
+        t = int(d.type[i])
+        if t in (2, 4):
+            v = (vals[0][i], vals[1][i], vals[2][i], vals[3][i])
+        elif t in (1, 3):
+            v = (vals[4][i], vals[5][i], vals[2][i], vals[3][i])
+        elif with_boundary_lines:
+            v = (0.0, 0.0, 0.0, 0.0)
+        else:
+            continue
+        x = d.xp[i]
+        rows.append("   %6d   %10.6e   %10.6e   %10.6e   %10.6e   %10.6e   %10.6e   %10.6e   %2d   %2d"
+                    % (i + 1, x[0], x[1], x[2], *v, t, int(d.matype[i])))
+    rows.append(rows[0])
+    return "\n".join(rows) + "\n"
+
+
+@pytest.mark.parametrize("with_boundary_lines", [True, False])
+def test_parameters_file_reader(with_boundary_lines, tmp_path):
+    # the library's reader against the parse of the written values and, where the reference is
+    # built, against its own random_field_input; the box lists its walls last, so a file without
+    # boundary lines reads the same for every water and soil particle
+    import ctypes as C
+    from sphgpy.capi import product_lib
+    d = spv_decks()["box3d"]
+    rng = np.random.default_rng(12)
+    vals = rng.uniform(0.5, 50.0, (6, d.n))
+    text = _parameters_text(d, vals, with_boundary_lines)
+    (tmp_path / "Parameters.txt").write_text(text)
+    init = rng.uniform(-2.0, -1.0, (6, d.n))
+    got = np.ascontiguousarray(init.copy())
+    ty = np.ascontiguousarray(d.type, np.int32)
+    rc = product_lib().lib.sphg_read_parameters(str(tmp_path).encode(), d.n, ty.ctypes.data_as(C.POINTER(C.c_int)),
+                                                got.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    printed = np.array([[float("%10.6e" % v) for v in row] for row in vals])
+    for k, phase in ((0, (2, 4)), (1, (2, 4)), (2, (2, 4)), (3, (2, 4)), (4, (1, 3)), (5, (1, 3))):
+        m = np.isin(d.type, phase)
+        assert np.array_equal(got[k][m], printed[k][m]), k
+        assert np.array_equal(got[k][~m], init[k][~m]), k
+    if have_ref():
+        _, smax3, wmax3, _ = spv_inputs(d)
+        ref, rc_ref = run_ref_rf(d.ndim, d.xp, d.type, d.dr, [0.0] * 6, smax3, wmax3, [0] * 6, 1.0, 1, 0.0, 0.0,
+                                 [(1, 0.0, 0.0, RL, 0.0)] * 6, list(init), kind="rfi",
+                                 files={"Parameters.txt": text})
+        assert rc_ref == 0
+        assert np.array_equal(_bits(got), _bits(ref))
+    assert product_lib().lib.sphg_read_parameters(str(tmp_path / "none").encode(), d.n,
+                                                   ty.ctypes.data_as(C.POINTER(C.c_int)),
+                                                   got.ctypes.data_as(C.POINTER(C.c_double))) == 100
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference link harness not built")
+def test_parameters_file_with_leading_walls_reads_like_the_reference(tmp_path):
+    # walls first: a file in the writer's layout (no wall lines) is read one line per wall, so the
+    # values land on later particles than they were written for - in the reference and here alike
+    import ctypes as C
+    from sphgpy.capi import product_lib
+    d0 = spv_decks()["slope"]
+    perm = np.argsort(d0.type != 0, kind="stable")  # walls first, then the soil in its order
+    d = d0.copy()
+    for k in ("xp", "vxp", "type", "matype", "permtype_p", "etype"):
+        setattr(d, k, getattr(d0, k)[perm].copy())
+    assert d.type[0] == 0 and d.type[-1] == 2
+    rng = np.random.default_rng(13)
+    vals = rng.uniform(0.5, 50.0, (6, d.n))
+    text = _parameters_text(d, vals, False)
+    (tmp_path / "Parameters.txt").write_text(text)
+    init = rng.uniform(-2.0, -1.0, (6, d.n))
+    got = np.ascontiguousarray(init.copy())
+    ty = np.ascontiguousarray(d.type, np.int32)
+    assert product_lib().lib.sphg_read_parameters(str(tmp_path).encode(), d.n, ty.ctypes.data_as(C.POINTER(C.c_int)),
+                                                  got.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    _, smax3, wmax3, _ = spv_inputs(d)
+    ref, rc_ref = run_ref_rf(d.ndim, d.xp, d.type, d.dr, [0.0] * 6, smax3, wmax3, [0] * 6, 1.0, 1, 0.0, 0.0,
+                             [(1, 0.0, 0.0, RL, 0.0)] * 6, list(init), kind="rfi", files={"Parameters.txt": text})
+    assert rc_ref == 0
+    assert np.array_equal(_bits(got), _bits(ref))
+    soil = d.type == 2
+    printed = np.array([float("%10.6e" % v) for v in vals[0]])
+    assert not np.array_equal(got[0][soil], printed[soil])  # shifted, as in the reference
