@@ -265,6 +265,19 @@ int sphg_poll(sphg_ctx *ctx);
 int sphg_current_step(const sphg_ctx *ctx);
 /* Sets the step counter pPPro->l for the next step (CPP_1-SPHLoops.cpp:478). */
 int sphg_set_step(sphg_ctx *ctx, int l);
+/* The clock Para_Pro::t (CPP_1-SPHLoops.cpp:481-483): every step first advances it by dt; the
+ * vibration loads read it. Zero after sphg_create; checkpoints keep it. */
+int sphg_set_time(sphg_ctx *ctx, double t);
+int sphg_get_time(sphg_ctx *ctx, double *t);
+/* Vibration loads (flv 1, 2; Para_Vibra, Class_Problem.h:133-149, read by initial_pro's VL section
+ * and initial_vib from vibra.dat, CPP_2-initialf.cpp:250-265, :293-322): one period of ttime in
+ * nsteps loads loads[nsteps][3], applied while sttime <= t <= edtime. With flv 1 they are the
+ * acceleration of the matype-3 / 5 walls (vibration_acceleration), with flv 2 the velocity of the
+ * matype-4 / 5 walls (vibration_velocity, CPP_9-1-acceleration_vibration.cpp:11-164); flv > 0 also
+ * adds the free-field boundaries (CPP_19-FreeFieldParticles.cpp, CPP_19-FreeFieldParticles_soil.cpp)
+ * and soil_damping_vibration. Required before sphg_upload_done when flv > 0 (SPHG_E_ARG otherwise,
+ * where the reference stops for a missing vibra.dat). */
+int sphg_set_vibration(sphg_ctx *ctx, double ttime, int nsteps, double sttime, double edtime, const double *loads);
 /* Particles in the context (pPPro->ntotal, all realisations of a batch), -1 for a null context. */
 int sphg_particle_count(const sphg_ctx *ctx);
 

@@ -70,6 +70,9 @@ struct sphg_ctx {
     bool pre_on = false;   // artificial-stress / geostress-damping pair pass is in the launch sequence
     bool full_models = false;  // some soil particle uses Cam-Clay, unsaturated Cam-Clay or FOCC
     bool rev_lists = false;    // every neighbour list reversed after the search (noise-floor runs)
+    // Para_Vibra (flv 1, 2): period, loads per period, window; the table is in S.vib_loads
+    double vib_ttime = 0.0, vib_sttime = 0.0, vib_edtime = 0.0;
+    int vib_nsteps = 0;
     bool eig_stale = false;    // strep / beta not written by the last stress pass (k_eig on read)
     // neighbours per load round trip of the density, normalisation, strain and XSPH passes
     // (sphg_autotune); each has the widths kBwChoices[k]
@@ -235,6 +238,18 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
+    plan.push_back({(void **)&S.tnow, 8});
+    // vibration loads and free-field boundaries: the clock above, the tangent stiffness and the
+    // pair sums (the load table is allocated by sphg_set_vibration)
+    S.vib_loads = nullptr;
+    S.vstiff = nullptr;
+    for (int k = 0; k < 3; k++) S.ffb[k] = S.ffs[k] = nullptr;
+    for (int k = 0; k < 4; k++) S.vdamp[k] = nullptr;
+    if (pro->flv > 0) {
+        D(S.vstiff);
+        for (int k = 0; k < 3; k++) { D(S.ffb[k]); D(S.ffs[k]); }
+        for (int k = 0; k < 4; k++) D(S.vdamp[k]);
+    }
     plan.push_back({(void **)&c->d_grids, sizeof(Grid) * (size_t)nreal});
     I(c->d_rid, cap);
     plan.push_back({(void **)&c->d_soil, sizeof(SoilSlot) * kNSlots});
@@ -302,6 +317,7 @@ extern "C" void sphg_destroy(sphg_ctx *c) {
         hipEventDestroy(t.b);
     }
     if (c->pool) hipFree(c->pool);
+    if (c->S.vib_loads) hipFree(c->S.vib_loads);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -608,7 +624,7 @@ static int fluid_slot_of(int type, int matype, int etype) {
 
 static int check_supported(const sphg_problem &p) {
     if (p.flknl < 1 || p.flknl > 4) return SPHG_E_UNSUPPORTED;  // 3 / 4: with kernel_gradient_correction
-    if (p.flr || p.fleob || p.flv) return SPHG_E_UNSUPPORTED;
+    if (p.flr || p.fleob || p.flv < 0 || p.flv > 2) return SPHG_E_UNSUPPORTED;
     if (p.fli < 0 || p.fli > 3 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
     if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 5 && p.flbndy != 13 &&
         p.flbndy != 14)
@@ -694,7 +710,12 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.struct_e = c->mat.structure.e;
     P.struct_poi = c->mat.structure.poi;
     P.eig_in_step = (pro.flts >= 1 || pro.art_stre_coe > 0.0) ? 1 : 0;
-    P.mrv_ok = (pro.ss_type == 0 && pro.flbndy != 5) ? 1 : 0;
+    P.mrv_ok = (pro.ss_type == 0 && pro.flbndy != 5 && pro.flv == 0) ? 1 : 0;
+    if (pro.flv > 0 && !c->S.vib_loads) return SPHG_E_ARG;  // the reference stops when vibra.dat is missing
+    P.vib_ttime = c->vib_ttime;
+    P.vib_sttime = c->vib_sttime;
+    P.vib_edtime = c->vib_edtime;
+    P.vib_nsteps = c->vib_nsteps;
     P.water_acc_minor = c->mat.water[0].acc_type != 0;
     P.air_acc_minor = c->mat.air.acc_type != 0;
     // realisation of every particle (batch contexts: realisation-major ranges of counts[r]) and
@@ -900,7 +921,10 @@ static void enqueue_nnps(sphg_ctx *c) {
 template <int NDIM, class M>
 static void enqueue_step(sphg_ctx *c, int phase_mask) {
     const int cur = c->cur, n = c->n;
-    if (phase_mask & 1) enqueue_nnps<NDIM>(c);
+    if (phase_mask & 1) {
+        if (c->P.flv > 0) LAUNCH("time", k_time_step, 1, c->S, c->P);
+        enqueue_nnps<NDIM>(c);
+    }
     if (phase_mask & 2) {
         LAUNCH_BW("density", k_density, 0, 1, 2, 4, grid_for(n), c->S, c->P, cur);
         LAUNCH_BW("norm_eos", k_norm, 1, 2, 4, 6, grid_for(n), c->S, c->P, cur);
@@ -929,13 +953,16 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         if (c->pre_on) LAUNCH("accel_pre", (k_accel_pre<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.coupled) LAUNCH("coupled", (k_coupled<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.fli >= 2) LAUNCH("drag", (k_drag<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if (c->P.flv > 0) LAUNCH("vib", (k_vib<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("accel", (k_accel<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if (c->P.flv == 1) LAUNCH("vib_walls_acc", k_vib_walls_acc<NDIM>, grid_for(n), c->S, c->P);
         if (c->P.ss_type != 0) {
             LAUNCH("ssi_soil", k_ssi_soil<NDIM>, grid_for(n), c->S, c->P, cur);
             LAUNCH("ssi_struct", k_ssi_struct, grid_for(n), c->S, c->P);
         }
-        if (c->P.ss_type != 0 || c->P.flbndy == 5)
+        if (c->P.ss_type != 0 || c->P.flbndy == 5 || c->P.flv > 0)
             LAUNCH("accel_tail", k_accel_tail<NDIM>, grid_for(n), c->S, c->P, cur);
+        if (c->P.flv > 0) LAUNCH("vib_walls_vel", k_vib_walls_vel<NDIM>, grid_for(n), c->S, c->P, cur);
     }
     if (phase_mask & 32) {
         LAUNCH_BW("xsph", k_xsph, 3, 1, 2, 4, grid_for(n), c->S, c->P, cur);
@@ -1102,6 +1129,12 @@ extern "C" int sphg_run(sphg_ctx *c, int loop, int fop, double tt, double *t, sp
     if (!c || !c->uploaded || !t || fop <= 0) return SPHG_E_ARG;
     const double dt = c->P.dt;
     int l = c->step;
+    // the device clock the vibration loads read starts from the caller's t and then advances
+    // step by step exactly as *t does here
+    if (c->P.flv > 0) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->S.tnow, t, 8, hipMemcpyHostToDevice));
+    }
     while (l <= loop) {
         if (!(*t < tt)) {  // the reference's else branch: output and leave the loop
             if (out) {
@@ -1247,6 +1280,42 @@ extern "C" int sphg_load_state(sphg_ctx *c, const char *path) {
     c->stepped = h.stepped != 0;
     c->eig_stale = false;  // the file was written with strep / beta of its stress
     c->has_psc = h.has_psc != 0;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_vibration(sphg_ctx *c, double ttime, int nsteps, double sttime, double edtime,
+                                  const double *loads) {
+    if (!c || !loads || nsteps < 1) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->S.vib_loads) {
+        HIPCHK(hipFree(c->S.vib_loads));
+        c->S.vib_loads = nullptr;
+    }
+    HIPCHK(hipMalloc(&c->S.vib_loads, (size_t)nsteps * 3 * 8));
+    HIPCHK(hipMemcpy(c->S.vib_loads, loads, (size_t)nsteps * 3 * 8, hipMemcpyHostToDevice));
+    c->vib_ttime = ttime;
+    c->vib_nsteps = nsteps;
+    c->vib_sttime = sttime;
+    c->vib_edtime = edtime;
+    c->P.vib_ttime = ttime;
+    c->P.vib_nsteps = nsteps;
+    c->P.vib_sttime = sttime;
+    c->P.vib_edtime = edtime;
+    drop_graphs(c);  // the captured launches hold the old table pointer and parameters
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_time(sphg_ctx *c, double t) {
+    if (!c) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->S.tnow, &t, 8, hipMemcpyHostToDevice));
+    return SPHG_OK;
+}
+
+extern "C" int sphg_get_time(sphg_ctx *c, double *t) {
+    if (!c || !t) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(t, c->S.tnow, 8, hipMemcpyDeviceToHost));
     return SPHG_OK;
 }
 

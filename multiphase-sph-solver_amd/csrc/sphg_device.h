@@ -69,6 +69,8 @@ struct Params {
     int coupled;             // Para_SSInter acc_type 2 with an interaction type 1-3
     int eig_in_step;         // principal stresses needed within the step (TS transform, artificial stress)
     int mrv_ok;              // mixed form: the momentum kernel writes every particle's mrv (no contact tail)
+    double vib_ttime, vib_sttime, vib_edtime;  // Para_Vibra: period, start and end time (flv 1, 2)
+    int vib_nsteps;                            // loads per period
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -125,6 +127,12 @@ struct State {
     // boundary treatment of Tran et al. (flbndy 3/4/13/14): per boundary particle and phase
     // (water, soil, air, structure) the interpolated stress
     double *sigb[4][6];
+    // vibration loads and free-field boundaries (flv 1, 2; null otherwise)
+    double *vib_loads;    // Para_Vibra loads[nsteps][3]
+    double *tnow;         // Para_Pro::t, advanced by dt at the start of every step
+    double *vstiff;       // largest diagonal entry (with zero) of the particle's last tangent (depp0)
+    double *ffb[3], *ffs[3];  // free-field sums against matype-5 walls / between soil and its columns
+    double *vdamp[4];     // soil_damping_vibration sums: damp[3], tempdown
     int *err, *step;      // sticky error word; current step number l
     const Grid *grids;    // one per realisation
     const int *rid;       // realisation of each particle

@@ -1097,6 +1097,21 @@ __device__ __forceinline__ void tangent_apply(double dsig[6], const double De[6]
     }
 }
 
+// The largest diagonal entry, with zero, of the tangent tangent_apply forms: what
+// soil_damping_vibration reads from StiffMat::depp0 (CPP_9-1-acceleration_vibration.cpp:186-188,
+// depp0 = dept in CPP_7-2-constitutive.cpp:112-292).
+__device__ __forceinline__ double tangent_stiff(const double De[6][6], bool plastic, double d, const double wv[6],
+                                                const double zv[6]) {
+    double st = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double Daa = De[a][a];
+        if (plastic) Daa = Daa - d * wv[a] * zv[a];
+        st = fmax(st, Daa);
+    }
+    return st;
+}
+
 __device__ int eejcb3(double a[9], double v[9], double eps, int jt) {
     int p = 0, q = 0, l = 1;
     for (int x = 0; x < 3; x++)
@@ -1281,7 +1296,7 @@ __device__ void focc_tangent(const double sigin[6], const double deps[6], double
 // one DP-family stress update (dpmodel1 / dpmodel_bui + DPModel_check), CPP_8-1-consfactor.cpp:284-615, :779-819
 __device__ __forceinline__ void dp_update(const double sig[6], const double deps[6], const SoilSlot &s, double ci,
                                           double yalf, double cosf, double sq, double yalf_dil, bool bui, double dt,
-                                          double dsig[6], double adps[6]) {
+                                          double dsig[6], double adps[6], double &stiff) {
     double sm = (((-sig[0]) + (-sig[1])) + (-sig[2])) * 0.3333333333;
     if (sm < 1.0) sm = 1.0;
     double e0 = soil_e0(s, sm);
@@ -1318,6 +1333,7 @@ __device__ __forceinline__ void dp_update(const double sig[6], const double deps
         }
     }
     tangent_apply(dsig, De, plastic, d, wv, zv, deps);
+    stiff = tangent_stiff(De, plastic, d, wv, zv);
     if (plastic)
         for (int k = 0; k < 6; k++) adps[k] = ramda * d * fv[k];
     // DPModel_check: stress-scaling return; sj from the stress before the increment
@@ -1341,7 +1357,7 @@ __device__ __forceinline__ void dp_update(const double sig[6], const double deps
 // Cam-Clay family (sub_camclay, sub_camunsatu): CPP_8-1-consfactor.cpp:129-281, CPP_8-4-consfactor-umcc.cpp:17-180
 __device__ __forceinline__ void camclay_update(const double sig[6], const double deps[6], const SoilSlot &s, bool unsat,
                                                double satu, double dsatu, double dsig[6], double adps[6],
-                                               double erfs[6]) {
+                                               double erfs[6], double &stiff) {
     const double third = unsat ? 0.333333333 : 0.3333333333;
     double poi = s.poi, eps0 = s.eps0, zmf = s.zmf;
     double ymf = 3.0 * (zmf - 1.0) / (zmf + 2.0) / s.fs;
@@ -1401,6 +1417,7 @@ __device__ __forceinline__ void camclay_update(const double sig[6], const double
     double wv[6] = {dg + dmu2 * f11, dg + dmu2 * f22, dg + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
     double zv[6] = {df + dmu2 * f11, df + dmu2 * f22, df + dmu2 * f33, dmu * f12, dmu * f23, dmu * f31};
     tangent_apply(dsig, De, plastic, d, wv, zv, deps);
+    stiff = tangent_stiff(De, plastic, d, wv, zv);
     if (plastic) {
         double fv[6] = {f11, f22, f33, f12, f23, f31};
         #pragma unroll
@@ -1488,7 +1505,8 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
         for (int k = 0; k < 6; k++) sig[k] = k < 3 ? -pre - vis * dd[k] : -vis * dd[k];
     } else if (ti == 2 || ti == 4) {
         double dsig[6] = {0, 0, 0, 0, 0, 0}, adps[6] = {0, 0, 0, 0, 0, 0}, erfs[6] = {0, 0, 0, 0, 0, 0};
-        double De[6][6], dlam, dmu;
+        double De[6][6], dlam, dmu, stiff = 0.0;
+        bool has_stiff = true;  // the branch formed a tangent (depp0 keeps its old value otherwise)
         // fractional_plastic runs for matype 8/18/28 also before inip, where the reference passes
         // its (never assigned, in practice zero) OpenMP-private compliance; zero is used here
         bool focc = false;
@@ -1507,12 +1525,13 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
                 iso_tangent(De, soil_e0(s, sm), s.poi, dlam, dmu);
                 double z6[6] = {0, 0, 0, 0, 0, 0};
                 tangent_apply(dsig, De, false, 0.0, z6, z6, deps);
+                stiff = tangent_stiff(De, false, 0.0, z6, z6);
             } else if (mt == 2 || mt == 5) {
                 double ci = (S.ps_c ? S.ps_c[i] : s.c) / s.fs;
                 dp_update(sig, deps, s, ci, S.ps_dp[0][i], S.ps_dp[1][i], S.ps_dp[2][i], s.dp_yalf_dil, mt == 5, dt,
-                          dsig, adps);
+                          dsig, adps, stiff);
             } else if (kFull && (mt == 1 || mt == 3)) {
-                camclay_update(sig, deps, s, mt == 3, S.satu[i], S.dsatu[i], dsig, adps, erfs);
+                camclay_update(sig, deps, s, mt == 3, S.satu[i], S.dsatu[i], dsig, adps, erfs, stiff);
             } else if (mt == 4) {
                 // sub_camanisotropy: body commented out in the reference (quirk A13): zero tangent
                 double Z[6][6];
@@ -1525,17 +1544,23 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
                 focc_tangent(sig, deps, S.e[i], s, D, Cf);
                 double z6[6] = {0, 0, 0, 0, 0, 0};
                 tangent_apply(dsig, D, false, 0.0, z6, z6, deps);
+                stiff = tangent_stiff(D, false, 0.0, z6, z6);
             } else if (mt >= 0) {
                 // viscous-elastic (7) and Mohr-Coulomb (9) are not implemented (the host also
                 // refuses 1 / 3 / 8 in the reduced instantiation)
                 set_err(S, 4);
+                has_stiff = false;
+            } else {
+                has_stiff = false;
             }
         } else {
             // structure (CPP_7-2-constitutive.cpp:276-319)
             iso_tangent(De, P.struct_e, P.struct_poi, dlam, dmu);
             double z6[6] = {0, 0, 0, 0, 0, 0};
             tangent_apply(dsig, De, false, 0.0, z6, z6, deps);
+            stiff = tangent_stiff(De, false, 0.0, z6, z6);
         }
+        if (S.vstiff && has_stiff) S.vstiff[i] = stiff;
         // Jaumann correction (CPP_7-2-constitutive.cpp:236-253)
         const double *s = sig;
         dsig[0] = dsig[0] + (W[0][0] * s[0] + W[0][1] * s[3] + W[0][2] * s[5]) - (s[0] * W[0][0] + s[3] * W[1][0] + s[5] * W[2][0]);
@@ -2532,6 +2557,11 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         #pragma unroll
         for (int k = 0; k < 3; k++) ax[k] += brow[k];
     }
+    if (P.flv > 0 && ti == 2) {
+        // free-field boundaries (flv > 0), each sum added on its own as the reference does
+        for (int k = 0; k < NDIM; k++) ax[k] = ax[k] + S.ffb[k][i];
+        for (int k = 0; k < NDIM; k++) ax[k] = ax[k] + S.ffs[k][i];
+    }
     if (P.coupled && (ti == 4 || ti == 2 || ti == 1)) {
         #pragma unroll
         for (int k = 0; k < 3; k++) ax[k] += S.cpl[k][i];
@@ -2554,10 +2584,11 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             S.interf[k][i] = itf;
         }
     }
-    if (P.ss_type != 0 || P.flbndy == 5) {
+    if (P.ss_type != 0 || P.flbndy == 5 || P.flv > 0) {
         // soil-structure interaction adds to ax before the damping clamp, the free-solid wall
-        // contact (flbndy 5) after it, both before the velocity update (CPP_1-SPHLoops.cpp:662-705):
-        // k_ssi_* and k_accel_tail finish the pass
+        // contact (flbndy 5) after it, both before the velocity update (CPP_1-SPHLoops.cpp:662-705);
+        // with vibration loads the walls' own laws run in between: k_ssi_*, k_vib_walls and
+        // k_accel_tail finish the pass
         #pragma unroll
         for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
         return;
@@ -2566,6 +2597,181 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     accel_vel<NDIM>(S, P, i, ti, cur, ax, vi, axp);
     if constexpr (M::kFast)
         S.mrv[i] = make_float4((float)S.vx[cur][0][i], (float)S.vx[cur][1][i], (float)S.vx[cur][2][i], (float)rhoi);
+}
+
+// ------------------------------------------------------------------------------------------
+// Vibration loads and free-field boundaries (flv 1, 2; CPP_1-SPHLoops.cpp:612-711)
+// ------------------------------------------------------------------------------------------
+// Para_Pro::t advances by dt at the start of every step (CPP_1-SPHLoops.cpp:481-483)
+__global__ void k_time_step(State S, Params P) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *S.tnow = *S.tnow + P.dt;
+}
+
+// ff_stress_2d_solid / ff_stress_3d_solid (CPP_19-FreeFieldParticles.cpp:24-76; the soil unit has the
+// same two): the viscous stress of a free-field boundary from the velocity jump va - vb, a P wave
+// along the boundary normal `type` (1 x, 2 y, 3 z; 3D only for z) and S waves along the others
+template <int NDIM>
+__device__ __forceinline__ void ff_stress(double sig[3], const double va[3], const double vb[3], double e, double poi,
+                                          double dens, int type) {
+    const double g = e * 0.5 / (1 + poi), k = e * 0.333333333333333 / (1 - 2.0 * poi);
+    const double c_p = sqrt((k + 4.0 * g * 0.333333333333333) / dens), c_s = sqrt(g / dens);
+    const double cx = type == 1 ? c_p : c_s, cy = type == 2 ? c_p : c_s, cz = type == 3 ? c_p : c_s;
+    if (NDIM == 2 && (type == 1 || type == 2)) {
+        sig[0] = -cx * dens * (va[0] - vb[0]);
+        sig[1] = -cy * dens * (va[1] - vb[1]);
+    } else if (NDIM == 3 && type >= 1 && type <= 3) {
+        sig[0] = -cx * dens * (va[0] - vb[0]);
+        sig[1] = -cy * dens * (va[1] - vb[1]);
+        sig[2] = -cz * dens * (va[2] - vb[2]);
+    }
+}
+
+// Pair sums of flv > 0 for soil and structure particles, after the stress pass:
+//   ffb: clFreeFieldBndy_Fun::free_field_solid (CPP_19-FreeFieldParticles.cpp:79-141), soil against
+//        matype-5 walls moving at their interfss, with the elastic constants of the
+//        default-constructed Para_Soil the reference passes (e = poi = 0, CPP_1-SPHLoops.cpp:253);
+//   ffs: clFreeFieldBndy_Fun_soil::free_field_solid (CPP_19-FreeFieldParticles_soil.cpp:79-169), soil
+//        of matype < 50 against the free-field columns (matype >= 50) and back, with the column
+//        particle's e, poi, density and normal (permtype);
+//   vdamp: the sums of soil_damping_vibration (CPP_9-1-acceleration_vibration.cpp:167-205) over
+//        same-type neighbours, cd from the particle's last tangent (vstiff).
+// Both free-field routines read pPar[i].vx, which equals vxp there (update copies vx into vxp and
+// nothing writes vx before the velocity update). The momentum kernel adds ffb, then ffs, after
+// the boundary-stress term; k_accel_tail applies vdamp.
+template <int NDIM, class M>
+__global__ __launch_bounds__(256) void k_vib(State S, Params P, int cur) {
+    int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    if (ti != 2 && ti != 4) return;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
+    const int nc = nb_count(S, i);
+    const double rhoi = S.rho[i];
+    const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    const SoilSlot &si = S.soil[S.soil_slot[i]];
+    const double cd = si.damp1 + si.damp2 * sqrt(S.vstiff[i] / S.rhop[i] / hli / hli);
+    const bool col = S.matype[i] >= 50;
+    double fb[3] = {0.0, 0.0, 0.0}, fs[3] = {0.0, 0.0, 0.0}, dm[3] = {0.0, 0.0, 0.0}, down = 0.0;
+    nb_batches<1>(S, P, i, nc, [&](const int (&e)[1], int act) {
+        if (act < 1) return;
+        const int j = nb_id(e[0]), tj = nb_type(e[0]);
+        const bool damp_j = tj == ti;
+        const bool fb_j = ti == 2 && tj == 0 && S.matype[j] == 5;
+        const bool fs_j = ti == 2 && tj == 2 && (col ? S.matype[j] < 50 : S.matype[j] >= 50);
+        if (!damp_j && !fb_j && !fs_j) return;
+        double xj[3], w[4], d2, dst;
+        load_xj<NDIM>(S, cur, j, xj);
+        pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hl_of(S, P, j, hli), w, d2, dst);
+        const double mj = S.mass[j], rj = S.rho[j];
+        double vj[3] = {S.vxp[0][j], S.vxp[1][j], S.vxp[2][j]};
+        if (damp_j) {
+#pragma unroll
+            for (int k = 0; k < NDIM; k++) dm[k] = dm[k] + mj / rj * cd * (vi[k] - vj[k]) * w[3];
+            down = down + mj / rj * w[3];
+        }
+        if (fb_j || fs_j) {
+            double sg[3] = {0.0, 0.0, 0.0};
+            // the velocity jump, the elastic constants, density and normal of the free-field side
+            double va[3], vb[3], fe, fpoi, fdens;
+            int ftype;
+            if (fb_j) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) { va[k] = vi[k]; vb[k] = S.interfss[k][j]; }
+                fe = 0.0; fpoi = 0.0; fdens = rj; ftype = S.permtype[j];
+            } else if (!col) {
+                const SoilSlot &sj = S.soil[S.soil_slot[j]];
+#pragma unroll
+                for (int k = 0; k < 3; k++) { va[k] = vi[k]; vb[k] = vj[k]; }
+                fe = sj.e; fpoi = sj.poi; fdens = rj; ftype = S.permtype[j];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; k++) { va[k] = vj[k]; vb[k] = vi[k]; }
+                fe = si.e; fpoi = si.poi; fdens = rhoi; ftype = S.permtype[i];
+            }
+            ff_stress<NDIM>(sg, va, vb, fe, fpoi, fdens, ftype);
+            if (fb_j) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) fb[k] = fb[k] + mj / rj * sg[k] / rhoi * w[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; k++) fs[k] = fs[k] + mj / rj * sg[k] / rhoi * w[k];
+            }
+        }
+    });
+    if (ti == 2) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            S.ffb[k][i] = fb[k];
+            S.ffs[k][i] = fs[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) S.vdamp[k][i] = dm[k];
+    S.vdamp[3][i] = down;
+}
+
+// The period step of the vibration table at time t, and whether its load applies: sttime <= t <=
+// edtime and every component below 1000 in magnitude (CPP_9-1-acceleration_vibration.cpp:22-40)
+__device__ __forceinline__ bool vib_load(const State &S, const Params &P, double v[3]) {
+    const double t = *S.tnow, time1 = t - P.vib_sttime, time2 = t - P.vib_edtime;
+    if (!(time1 >= 0.0 && time2 <= 0.0)) return false;
+    int nstep = (int)(time1 / P.vib_ttime * (double)P.vib_nsteps);
+    if (nstep > P.vib_nsteps - 1) nstep = P.vib_nsteps - 1;
+    for (int k = 0; k < 3; k++) v[k] = S.vib_loads[3 * nstep + k];
+    double mx = fmax(0.0, fabs(v[0]));
+    mx = fmax(mx, fabs(v[1]));
+    mx = fmax(mx, fabs(v[2]));
+    return mx < 1000.0;
+}
+
+// vibration_acceleration (flv 1, CPP_9-1-acceleration_vibration.cpp:11-87): the acceleration of the
+// driving walls (matype 3, 5); outside the window (or for a load of 1000 or more) their
+// acceleration and velocity are zeroed. Between the momentum kernel and its tail, so the
+// free-solid contact of the tail sees the walls as the reference's boundary_free_solid does.
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_vib_walls_acc(State S, Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n || S.type[i] != 0) return;
+    const int mt = S.matype[i];
+    if (mt != 3 && mt != 5) return;
+    double v[3];
+    const bool live = vib_load(S, P, v);
+    for (int k = 0; k < NDIM; k++) {
+        if (live) {
+            S.interf[k][i] = v[k];
+        } else {
+            S.interf[k][i] = 0.0;
+            S.interfss[k][i] = 0.0;
+        }
+    }
+}
+
+// The walls' part of veocity_update (CPP_11-xsph.cpp:11-142) and, with flv 2, vibration_velocity
+// (CPP_9-1-acceleration_vibration.cpp:90-164: the velocity of the driving walls of matype 4, 5),
+// after the tail: the contact laws of the tail read the walls' velocities of the start of the step.
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_vib_walls_vel(State S, Params P, int cur) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n || S.type[i] != 0) return;
+    double ax[3] = {S.ax[0][i], S.ax[1][i], S.ax[2][i]};
+    const double vi[3] = {S.vxp[0][i], S.vxp[1][i], S.vxp[2][i]};
+    const double axp[3] = {S.axp[0][i], S.axp[1][i], S.axp[2][i]};
+    accel_vel<NDIM>(S, P, i, 0, cur, ax, vi, axp);
+    const int mt = S.matype[i];
+    if (P.flv != 2 || (mt != 4 && mt != 5)) return;
+    double v[3];
+    const bool live = vib_load(S, P, v);
+    for (int k = 0; k < NDIM; k++) {
+        if (live) {
+            S.interfss[k][i] = v[k];
+        } else {
+            S.interf[k][i] = 0.0;
+            S.interfss[k][i] = 0.0;
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------

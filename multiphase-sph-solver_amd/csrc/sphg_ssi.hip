@@ -412,6 +412,18 @@ __global__ __launch_bounds__(256) void k_accel_tail(State S, Params P, int cur) 
     const double axp[3] = {S.axp[0][i], S.axp[1][i], S.axp[2][i]};
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
     accel_damp<NDIM>(S, P, i, ax, vi, axp, damp_on);
+    if (P.flv > 0 && (ti == 2 || ti == 4)) {
+        // soil_damping_vibration's clamp (CPP_9-1-acceleration_vibration.cpp:207-221)
+        double tempdown = S.vdamp[3][i];
+        for (int k = 0; k < NDIM; k++) {
+            const double damp = S.vdamp[k][i];
+            if (fabs(tempdown) < 1.0e-7) tempdown = 1.0e-7;
+            const double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * P.dt;
+            const double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - damp / tempdown) * P.dt;
+            if (vx1 * vx2 <= 0.0) ax[k] = -2.0 * vi[k] / P.dt - axp[k];
+            else ax[k] -= damp / tempdown;
+        }
+    }
     if (P.flbndy == 5 && (NDIM == 3 || (ti >= 1 && ti <= 4))) {
         // boundary_free_solid (CPP_13-boundary.cpp:1239-1515): the rigid contact against the wall
         // (boundary particles, their interfss as velocity); the 3D loop has no type test
@@ -424,6 +436,12 @@ __global__ __launch_bounds__(256) void k_accel_tail(State S, Params P, int cur) 
             for (int k = 0; k < NDIM; k++) ax[k] += zeta * (ft[k] + fn[k]) / mi;
             if (NDIM == 2) ax[2] += 0.0;
         }
+    }
+    if (P.flv > 0 && ti == 0) {
+        // with vibration loads the walls move in k_vib_walls_vel, after this kernel
+#pragma unroll
+        for (int k = 0; k < 3; k++) S.ax[k][i] = ax[k];
+        return;
     }
     accel_vel<NDIM>(S, P, i, ti, cur, ax, vi, axp);
 }

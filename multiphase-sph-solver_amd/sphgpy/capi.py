@@ -407,6 +407,12 @@ def product_lib() -> _Lib:
         L.sphg_sync.restype = _I
         L.sphg_set_step.argtypes = [C.c_void_p, _I]
         L.sphg_set_step.restype = _I
+        L.sphg_set_time.argtypes = [C.c_void_p, _D]
+        L.sphg_set_time.restype = _I
+        L.sphg_get_time.argtypes = [C.c_void_p, C.POINTER(_D)]
+        L.sphg_get_time.restype = _I
+        L.sphg_set_vibration.argtypes = [C.c_void_p, _D, _I, _D, _D, C.POINTER(_D)]
+        L.sphg_set_vibration.restype = _I
         L.sphg_error_string.restype = C.c_char_p
         L.sphg_build_info.restype = C.c_char_p
         L.sphg_enable_timing.argtypes = [C.c_void_p, _I]
@@ -562,6 +568,21 @@ class GpuSim(Sim):
         if rc:
             raise RuntimeError(f"sphg_create: error {rc} ({lib.lib.sphg_error_string(rc).decode()})")
         super().__init__(lib, ctx, deck.n)
+        if deck.flv > 0:
+            self.set_vibration(deck)
+
+    def set_vibration(self, deck: Deck) -> None:
+        """sphg_set_vibration from the deck's Para_Vibra fields and loads."""
+        loads = np.ascontiguousarray(deck.vib_loads, dtype=np.float64)
+        rc = self.L.lib.sphg_set_vibration(self.ctx, deck.vib_ttime, deck.vib_nsteps, deck.vib_sttime,
+                                           deck.vib_edtime, _dptr(loads))
+        if rc:
+            raise RuntimeError(f"sphg_set_vibration: error {rc}")
+
+    def time(self) -> float:
+        t = C.c_double()
+        self.L.lib.sphg_get_time(self.ctx, C.byref(t))
+        return t.value
 
     def upload_done(self):
         rc = self.L.lib.sphg_upload_done(self.ctx, self.n)
@@ -692,6 +713,8 @@ class GpuBatch(GpuSim):
             raise RuntimeError(f"sphg_create_batch: error {rc}")
         self.offsets = np.concatenate([[0], np.cumsum([d.n for d in self.decks])]).astype(np.int64)
         Sim.__init__(self, lib, ctx, int(self.offsets[-1]))
+        if self.decks[0].flv > 0:
+            self.set_vibration(self.decks[0])
 
     def kl_synthesize_part(self, r: int, f: "KLField", xi: np.ndarray) -> None:
         xi = np.ascontiguousarray(xi, dtype=np.float64)

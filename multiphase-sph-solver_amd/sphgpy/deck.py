@@ -127,6 +127,13 @@ class Deck:
     permtype_p: np.ndarray = None
     etype: np.ndarray = None
     name: str = "deck"
+    # Para_Vibra (flv 1, 2): one period of vib_ttime in vib_nsteps loads, applied between
+    # vib_sttime and vib_edtime; vib_loads (vib_nsteps, 3) go to vibra.dat
+    vib_ttime: float = 0.0
+    vib_nsteps: int = 0
+    vib_sttime: float = 0.0
+    vib_edtime: float = 0.0
+    vib_loads: np.ndarray = None
 
     @property
     def n(self) -> int:
@@ -194,6 +201,14 @@ class Deck:
                              self.vxp[i, 2], int(self.type[i]), int(self.matype[i]), int(self.permtype_p[i]),
                              int(self.etype[i])])
         sec("PA", rows)
+        if self.flv >= 1:
+            # VL section (initial_pro, CPP_2-initialf.cpp:250-265) and the loads of vibra.dat
+            # beside input.dat (initial_vib, :293-322)
+            sec("VL", [[self.vib_ttime, self.vib_nsteps], [self.vib_sttime, self.vib_edtime]])
+            import os
+            with open(os.path.join(os.path.dirname(os.path.abspath(path)), "vibra.dat"), "w") as f:
+                for row in np.asarray(self.vib_loads, dtype=np.float64).reshape(-1, 3):
+                    f.write(" ".join(_g(float(v)) for v in row) + "\n")
         with open(path, "w") as f:
             f.write("\n".join(L) + "\n")
 
@@ -713,6 +728,53 @@ def slope_pile_3d(nx=200, ny=50, nz=100, dx=0.1, model=4, steps=1000, seed=71, p
              (_lattice3d(0, nx, ny, ny + wl, 0, hw, dx), 0, 0)]
     deck = _assemble(d, parts, rng)
     deck.vxp[deck.type == 2] = v0
+    return deck
+
+
+def vibration_box(ndim=2, flv=1, nx=8, ny=3, nz=5, dx=0.002, steps=6, seed=81, amp=None, v0=0.05,
+                  damp=(50.0, 1e-3)) -> Deck:
+    """Soil on a shaking base (flv 1: the base walls, matype 3, and the side walls, matype 5, take
+    the acceleration of the vibration table; flv 2: the base, matype 4, and the sides take its
+    velocity, CPP_9-1-acceleration_vibration.cpp). The sides are free-field boundaries: a column of
+    free-field soil (matype 52, its normal x as permtype 1) next to each side wall (matype 5,
+    normal x; in 3D the y walls take normal y), CPP_19-FreeFieldParticles*.cpp. The soil (DP with
+    damp1 / damp2 of soil_damping_vibration) starts moving at v0 along x. The table has eight loads
+    per period of 4 dt and applies from t = 1.5 dt to (steps - 0.5) dt; its sixth load exceeds 1000
+    in magnitude, which zeroes the walls' motion for that step."""
+    rng = np.random.default_rng(seed)
+    grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
+    d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, flbndy=2, coe_bndy=0.1,
+             flv=flv, art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"vibration_{ndim}d_v{flv}")
+    d.model_flags[2] = 1
+    lay = dp_layer(c=500.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0, eps0=0.6)
+    lay.update(damp1=damp[0], damp2=damp[1])
+    d.soil_layers[2] = [lay]
+    base = 3 if flv == 1 else 4
+    top = nz + 2
+    if ndim == 2:
+        parts = [(_lattice2d(1, nx - 1, 0, nz, dx), 2, 2), (_lattice2d(0, 1, 0, nz, dx), 2, 52, 1),
+                 (_lattice2d(nx - 1, nx, 0, nz, dx), 2, 52, 1), (_lattice2d(-3, nx + 3, -3, 0, dx), 0, base),
+                 (_lattice2d(-3, 0, 0, top, dx), 0, 5, 1), (_lattice2d(nx, nx + 3, 0, top, dx), 0, 5, 1)]
+    else:
+        parts = [(_lattice3d(1, nx - 1, 0, ny, 0, nz, dx), 2, 2), (_lattice3d(0, 1, 0, ny, 0, nz, dx), 2, 52, 1),
+                 (_lattice3d(nx - 1, nx, 0, ny, 0, nz, dx), 2, 52, 1),
+                 (_lattice3d(-3, nx + 3, -3, ny + 3, -3, 0, dx), 0, base),
+                 (_lattice3d(-3, 0, -3, ny + 3, 0, top, dx), 0, 5, 1),
+                 (_lattice3d(nx, nx + 3, -3, ny + 3, 0, top, dx), 0, 5, 1),
+                 (_lattice3d(0, nx, -3, 0, 0, top, dx), 0, 5, 2), (_lattice3d(0, nx, ny, ny + 3, 0, top, dx), 0, 5, 2)]
+    deck = _assemble(d, parts, rng)
+    soil = deck.type == 2
+    deck.vxp[soil, 0] = v0
+    if amp is None:
+        amp = 5.0 if flv == 1 else 0.05
+    ph = np.arange(8) * (2.0 * math.pi / 8.0)
+    loads = np.zeros((8, 3))
+    loads[:, 0] = amp * np.sin(ph)
+    loads[:, ndim - 1] = 0.5 * amp * np.cos(ph)
+    loads[5, 0] = 2000.0
+    deck.vib_ttime, deck.vib_nsteps = 4.0 * d.dt, 8
+    deck.vib_sttime, deck.vib_edtime = 1.5 * d.dt, (steps - 0.5) * d.dt
+    deck.vib_loads = loads
     return deck
 
 

@@ -63,6 +63,12 @@ struct or_ctx {
     int l;
     double t;
     int density_mode, nthreads;
+    /* StiffMat::depp0 (Class_particle.h:105-112): the diagonal of each particle's last tangent */
+    double (*depp0)[3];
+    /* Para_Vibra (Class_Problem.h:133-149) and the loads of vibra.dat */
+    double vib_ttime, vib_sttime, vib_edtime;
+    int vib_nsteps;
+    double (*vib_loads)[3];
 };
 
 /* ------------------------------------------------------------------ kernel function */
@@ -138,6 +144,7 @@ or_ctx *or_create(const sphg_problem *pro, const sphg_materials *mat, int n) {
     c->vel_bndy = (double(*)[3])calloc(n, sizeof(double[3]));
     c->vx_temp = (double(*)[3])calloc(n, sizeof(double[3]));
     c->rho_tmp = (double *)calloc(n, sizeof(double));
+    c->depp0 = (double(*)[3])calloc(n, sizeof(double[3]));
     for (i = 0; i < n; i++) {
         par_init(&c->p[i]);
         soil_default(&c->ps[i]);
@@ -167,7 +174,22 @@ void or_destroy(or_ctx *c) {
     free(c->p); free(c->pc); free(c->ps); free(c->pf); free(c->vb); free(c->vel_bndy);
     free(c->vx_temp); free(c->rho_tmp); free(c->cstart); free(c->cend); free(c->nncell);
     free(c->s_cell); free(c->s_id); free(c->t_cell); free(c->t_id);
+    free(c->depp0); free(c->vib_loads);
     free(c);
+}
+
+/* initial_vib (CPP_2-initialf.cpp:293-322) and the VL section (:250-265): one period of ttime
+ * in nsteps loads (three components each), applied between sttime and edtime. */
+int or_set_vibration(or_ctx *c, double ttime, int nsteps, double sttime, double edtime, const double *loads) {
+    if (nsteps < 1 || !loads) return SPHG_E_ARG;
+    free(c->vib_loads);
+    c->vib_loads = (double(*)[3])malloc(sizeof(double[3]) * nsteps);
+    memcpy(c->vib_loads, loads, sizeof(double[3]) * nsteps);
+    c->vib_ttime = ttime;
+    c->vib_nsteps = nsteps;
+    c->vib_sttime = sttime;
+    c->vib_edtime = edtime;
+    return 0;
 }
 
 void or_set_mode(or_ctx *c, int density_mode, int nthreads) {
@@ -175,6 +197,7 @@ void or_set_mode(or_ctx *c, int density_mode, int nthreads) {
     c->nthreads = nthreads > 0 ? nthreads : 1;
 }
 void or_set_step(or_ctx *c, int l) { c->l = l; }
+void or_set_time(or_ctx *c, double t) { c->t = t; }
 
 /* field access by Particle member name ------------------------------------------------ */
 typedef struct { const char *name; size_t off; int ncomp; } FieldDesc;
@@ -1304,6 +1327,13 @@ static void apply_tangent(OPar *p, double D[6][6]) {
         for (b = 0; b < 6; b++) p->dsig[a] += D[a][b] * p->deps[b];
 }
 
+/* pParStiff[i].depp0 = dept (CPP_7-2-constitutive.cpp:112-292): only its diagonal is read later */
+static void keep_depp0(or_ctx *c, int i, double D[6][6]) {
+    c->depp0[i][0] = D[0][0];
+    c->depp0[i][1] = D[1][1];
+    c->depp0[i][2] = D[2][2];
+}
+
 /* strain_to_stress (CPP_7-2-constitutive.cpp:13-321) for the models the workloads use:
  * Newtonian/Bingham fluid, elastic (0, 6), SB (1), DP (2), AMCC no-op (4, quirk A13), structure. */
 static int or_strain_to_stress(or_ctx *c) {
@@ -1350,28 +1380,36 @@ static int or_strain_to_stress(or_ctx *c) {
             if (c->l < c->pro.inip) {
                 or_elastic(D, pi, s);
                 apply_tangent(pi, D);
+                keep_depp0(c, i, D);
             } else if (mt == 0 || mt == 6) {
                 or_elastic(D, pi, s);
                 apply_tangent(pi, D);
+                keep_depp0(c, i, D);
             } else if (mt == 1) {
                 or_sub_camclay(D, pi, s, adps);
                 apply_tangent(pi, D);
+                keep_depp0(c, i, D);
             } else if (mt == 2) {
                 or_dpmodel1(D, pi, s, adps);
                 apply_tangent(pi, D);
+                keep_depp0(c, i, D);
                 or_dp_check(pi, s, dt);
             } else if (mt == 3) {
                 or_sub_camunsatu(D, erfs, pi, s, adps);
                 apply_tangent(pi, D);
+                keep_depp0(c, i, D);
             } else if (mt == 4) {
                 apply_tangent(pi, D); /* AMCC body is commented out: zero tangent, no erfs */
+                keep_depp0(c, i, D);
             } else if (mt == 5) {
                 or_dpmodel_bui(D, pi, s, adps);
                 apply_tangent(pi, D);
+                keep_depp0(c, i, D);
                 or_dp_check(pi, s, dt);
             } else if (mt == 8) {
                 or_fractional_mcc(D, Cf, pi, s);
                 apply_tangent(pi, D);
+                keep_depp0(c, i, D);
             } else if (mt >= 0) {
                 known = 0; /* 7 viscous-elastic and 9 Mohr-Coulomb are not restated */
             }
@@ -1393,6 +1431,7 @@ static int or_strain_to_stress(or_ctx *c) {
                 for (b = 0; b < 6; b++) D[a][b] = 0.0;
             iso_tangent(D, c->mat.structure.e, c->mat.structure.poi);
             apply_tangent(pi, D);
+            keep_depp0(c, i, D);
             jaumann(pi);
             for (k = 0; k < 6; k++) {
                 pi->eps[k] = pi->eps[k] + pi->deps[k] * dt;
@@ -2049,6 +2088,148 @@ static void or_damping_geostress(or_ctx *c) {
             vx2 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k] - damp[k]) * dt;
             if (vx1 * vx2 <= 0.0) pi->ax[k] = -2.0 * pi->vxp[k] / dt - pi->axp[k];
             else pi->ax[k] -= damp[k];
+        }
+    }
+}
+
+/* soil_damping_vibration (CPP_9-1-acceleration_vibration.cpp:167-224), flv > 0: a velocity damping
+ * of soil and structure particles against their own phase, its coefficient from the largest
+ * diagonal entry of the particle's last tangent */
+static void or_damping_vibration(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+    double dt = c->pro.dt;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double damp[3] = {0, 0, 0}, cd, stiff, tempdown = 0.0, vx1, vx2;
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (!(pi->type == 2 || pi->type == 4)) continue;
+        stiff = fmax(0.0, c->depp0[i][0]);
+        stiff = fmax(stiff, c->depp0[i][1]);
+        stiff = fmax(stiff, c->depp0[i][2]);
+        cd = c->ps[i].damp1 + c->ps[i].damp2 * sqrt(stiff / pi->rhop / pi->hl / pi->hl);
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            if (pi->type != pj->type) continue;
+            for (k = 0; k < ndim; k++)
+                damp[k] = damp[k] + pj->mass / pj->rho * cd * (pi->vxp[k] - pj->vxp[k]) * c->pc[i].wij[nj][3];
+            tempdown = tempdown + pj->mass / pj->rho * c->pc[i].wij[nj][3];
+        }
+        for (k = 0; k < ndim; k++) {
+            if (fabs(tempdown) < 1.0e-7) tempdown = 1.0e-7;
+            vx1 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k]) * dt;
+            vx2 = pi->vxp[k] + 0.5 * (pi->axp[k] + pi->ax[k] - damp[k] / tempdown) * dt;
+            if (vx1 * vx2 <= 0.0) pi->ax[k] = -2.0 * pi->vxp[k] / dt - pi->axp[k];
+            else pi->ax[k] -= damp[k] / tempdown;
+        }
+    }
+}
+
+/* ff_stress_2d_solid / ff_stress_3d_solid (CPP_19-FreeFieldParticles.cpp:24-76): viscous stresses
+ * of a free-field boundary from the velocity jump, P waves along the boundary normal `type`
+ * (1 x, 2 y, 3 z) and S waves along the others */
+static void ff_stress(int ndim, double sig[3], const double va[3], const double vb[3], double e, double poi,
+                      double dens, int type) {
+    double g = e * 0.5 / (1 + poi), k = e * 0.333333333333333 / (1 - 2.0 * poi);
+    double c_p = sqrt((k + 4.0 * g * 0.333333333333333) / dens), c_s = sqrt(g / dens);
+    if (ndim == 2) {
+        if (type == 1) {
+            sig[0] = -c_p * dens * (va[0] - vb[0]);
+            sig[1] = -c_s * dens * (va[1] - vb[1]);
+        } else if (type == 2) {
+            sig[0] = -c_s * dens * (va[0] - vb[0]);
+            sig[1] = -c_p * dens * (va[1] - vb[1]);
+        }
+        return;
+    }
+    if (type == 1) {
+        sig[0] = -c_p * dens * (va[0] - vb[0]);
+        sig[1] = -c_s * dens * (va[1] - vb[1]);
+        sig[2] = -c_s * dens * (va[2] - vb[2]);
+    } else if (type == 2) {
+        sig[0] = -c_s * dens * (va[0] - vb[0]);
+        sig[1] = -c_p * dens * (va[1] - vb[1]);
+        sig[2] = -c_s * dens * (va[2] - vb[2]);
+    } else if (type == 3) {
+        sig[0] = -c_s * dens * (va[0] - vb[0]);
+        sig[1] = -c_s * dens * (va[1] - vb[1]);
+        sig[2] = -c_p * dens * (va[2] - vb[2]);
+    }
+}
+
+/* clFreeFieldBndy_Fun::free_field_solid (CPP_19-FreeFieldParticles.cpp:79-141), flv > 0: soil
+ * particles against matype-5 boundary particles moving at their interfss, with the elastic
+ * constants of a default-constructed Para_Soil (e = poi = 0, SPHLoops.cpp:253) */
+static void or_free_field_bndy(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double t1[3] = {0, 0, 0};
+        int nc = nbcount(&c->pc[i]), nj, k;
+        if (pi->type != 2) continue;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double sig[3] = {0.0, 0.0, 0.0};
+            if (!(pj->type == 0 && pj->matype == 5)) continue;
+            ff_stress(ndim, sig, pi->vx, pj->interfss, 0.0, 0.0, pj->rho, pj->permtype);
+            for (k = 0; k < 3; k++) t1[k] = t1[k] + pj->mass / pj->rho * sig[k] / pi->rho * c->pc[i].wij[nj][k];
+        }
+        for (k = 0; k < ndim; k++) pi->ax[k] = pi->ax[k] + t1[k];
+    }
+}
+
+/* clFreeFieldBndy_Fun_soil::free_field_solid (CPP_19-FreeFieldParticles_soil.cpp:79-169), flv > 0:
+ * soil of matype < 50 against free-field soil columns (matype >= 50) and back, with the column
+ * particle's elastic constants, density and normal */
+static void or_free_field_soil(or_ctx *c) {
+    int n = c->n, ndim = c->pro.ndim, i;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        double t1[3] = {0, 0, 0};
+        int nc = nbcount(&c->pc[i]), nj, k, col;
+        if (pi->type != 2) continue;
+        col = pi->matype >= 50;
+        for (nj = 1; nj <= nc; nj++) {
+            int j = c->pc[i].influ[nj];
+            const OPar *pj = &c->p[j];
+            double sig[3] = {0.0, 0.0, 0.0};
+            if (!(pj->type == 2 && (col ? pj->matype < 50 : pj->matype >= 50))) continue;
+            if (!col) ff_stress(ndim, sig, pi->vx, pj->vx, c->ps[j].e, c->ps[j].poi, pj->rho, pj->permtype);
+            else ff_stress(ndim, sig, pj->vx, pi->vx, c->ps[i].e, c->ps[i].poi, pi->rho, pi->permtype);
+            for (k = 0; k < 3; k++) t1[k] = t1[k] + pj->mass / pj->rho * sig[k] / pi->rho * c->pc[i].wij[nj][k];
+        }
+        for (k = 0; k < ndim; k++) pi->ax[k] = pi->ax[k] + t1[k];
+    }
+}
+
+/* vibration_acceleration (flv 1, CPP_9-1-acceleration_vibration.cpp:11-87) and vibration_velocity
+ * (flv 2, :90-164): the load of the period step containing t - sttime, while sttime <= t <=
+ * edtime, becomes the acceleration (interf; walls of matype 3, 5) or velocity (interfss; matype 4,
+ * 5) of the driving walls; a load of magnitude 1000 or more, or a time outside the window, zeroes
+ * both instead */
+static void or_vibration(or_ctx *c, int velocity) {
+    int n = c->n, ndim = c->pro.ndim, i, k, nstep, live = 0;
+    double time1 = c->t - c->vib_sttime, time2 = c->t - c->vib_edtime, v[3] = {0, 0, 0}, mx;
+    if (time1 >= 0.0 && time2 <= 0.0) {
+        nstep = (int)(time1 / c->vib_ttime * (double)c->vib_nsteps);
+        if (nstep > c->vib_nsteps - 1) nstep = c->vib_nsteps - 1;
+        for (k = 0; k < 3; k++) v[k] = c->vib_loads[nstep][k];
+        mx = fmax(0.0, fabs(v[0]));
+        mx = fmax(mx, fabs(v[1]));
+        mx = fmax(mx, fabs(v[2]));
+        live = mx < 1000.0;
+    }
+    for (i = 0; i < n; i++) {
+        OPar *pi = &c->p[i];
+        if (pi->type != 0 || !(pi->matype == (velocity ? 4 : 3) || pi->matype == 5)) continue;
+        for (k = 0; k < ndim; k++) {
+            if (live && velocity) pi->interfss[k] = v[k];
+            else if (live) pi->interf[k] = v[k];
+            else pi->interf[k] = pi->interfss[k] = 0.0;
         }
     }
 }
@@ -2803,7 +2984,8 @@ static void or_free_solid(or_ctx *c) {
 /* ------------------------------------------------------------------ one step */
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
-    if (P->flknl < 1 || P->flknl > 4 || P->flr || P->fleob || P->flv) return 1;
+    if (P->flknl < 1 || P->flknl > 4 || P->flr || P->fleob || P->flv < 0 || P->flv > 2) return 1;
+    if (P->flv > 0 && !c->vib_loads) return 1;
     if (P->fli < 0 || P->fli > 3 || P->ss_type < 0 || P->ss_type > 3) return 1;
     if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 5 && P->flbndy != 13 &&
         P->flbndy != 14)
@@ -2850,13 +3032,20 @@ int or_phase(or_ctx *c, int mask) {
         if (c->pro.art_stre_coe > 0.0) or_art_stress(c);
         if (c->pro.flbndy == 1 || c->pro.flbndy == 2 || c->pro.flbndy == 5) or_bndy_stress(c);
         else or_bndy_stress_tran(c);
+        if (c->pro.flv > 0) {
+            or_free_field_bndy(c);
+            or_free_field_soil(c);
+        }
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3 && c->pro.ss_acc_type == 2) or_coupled(c);
+        if (c->pro.flv == 1) or_vibration(c, 0);
         if (c->pro.fli == 1) or_interact1(c);
         else if (c->pro.fli == 2 || c->pro.fli == 3) or_interact23(c);
         if (c->pro.ss_type >= 1 && c->pro.ss_type <= 3) or_ssi(c);
         if (c->l <= c->pro.inip) or_damping_geostress(c);
+        if (c->pro.flv > 0) or_damping_vibration(c);
         if (c->pro.flbndy == 5) or_free_solid(c);
         or_velocity(c);
+        if (c->pro.flv == 2) or_vibration(c, 1);
     }
     if (mask & 32) {
         if (c->pro.if_move != 2) or_xsph(c);

@@ -34,6 +34,10 @@ int or_setting_vis_cons(or_ctx *c);
 int or_upload_done(or_ctx *c);
 void or_set_mode(or_ctx *c, int density_mode, int nthreads);
 void or_set_step(or_ctx *c, int l);
+/* Para_Pro::t, the clock the vibration loads read (advanced by dt at the start of every step). */
+void or_set_time(or_ctx *c, double t);
+/* Para_Vibra and vibra.dat (flv 1, 2): loads[nsteps][3]. */
+int or_set_vibration(or_ctx *c, double ttime, int nsteps, double sttime, double edtime, const double *loads);
 /* Runs nsteps reference steps; returns 0 or the reference error code (3, 5). */
 int or_step(or_ctx *c, int nsteps);
 /* Phase groups of one step (bit 1 NNPS, 2 density+eos1, 4 strain+eos2, 8 stress, 16 accel+velocity,

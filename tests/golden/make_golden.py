@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 from helpers import run_ref, run_ref_kl, have_ref  # noqa: E402
 from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d,  # noqa: E402
                     art_stress_block, slope_2d, ssi_block, tran_box, drag_box, air_water_box,
-                    slope_pile_3d)
+                    slope_pile_3d, vibration_box)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype", "tssig", "tsqc"})
@@ -128,6 +128,10 @@ def decks():
         "bingham_2d": (with_bingham(tran_box(2, 3, steps=6, v0=0.5)), 6, (1, 3, 6)),
         "bingham_3d": (with_bingham(tran_box(3, 14, nx=5, ny=3, nz_soil=3, nz_water=2, steps=5, v0=0.5), fai=8.0,
                                     c=0.5, strain_min=50.0), 5, (1, 5)),
+        "vib_2d_acc": (vibration_box(2, 1), 6, (1, 3, 6)),
+        "vib_2d_vel": (vibration_box(2, 2), 6, (1, 3, 6)),
+        "vib_3d_acc": (vibration_box(3, 1, nx=5, ny=3, nz=3), 6, (1, 3, 6)),
+        "vib_3d_vel": (vibration_box(3, 2, nx=5, ny=3, nz=3), 6, (1, 3, 6)),
     }
 
 

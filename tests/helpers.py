@@ -45,6 +45,10 @@ def oracle_lib() -> _Lib:
         L.or_setting_vis_cons.argtypes = [C.c_void_p]
         L.or_set_mode.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.or_set_step.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.or_set_vibration.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double,
+                                       C.POINTER(C.c_double)]
+        L.or_set_vibration.restype = C.c_int
         L.or_phase.argtypes = [C.c_void_p, C.c_int]
         L.or_phase.restype = C.c_int
         L.or_pair_count.argtypes = [C.c_void_p]
@@ -63,6 +67,10 @@ class OracleSim(Sim):
         self._mat = materials_from_deck(deck)
         ctx = lib.lib.or_create(C.byref(self._pro), C.byref(self._mat), deck.n)
         super().__init__(lib, ctx, deck.n)
+        if deck.flv > 0:
+            loads = np.ascontiguousarray(deck.vib_loads, dtype=np.float64)
+            lib.lib.or_set_vibration(ctx, deck.vib_ttime, deck.vib_nsteps, deck.vib_sttime, deck.vib_edtime,
+                                     loads.ctypes.data_as(C.POINTER(C.c_double)))
         for k in INT_FIELDS:
             self.set(k, state[k])
         lib.lib.or_setting_vis_cons(ctx)

@@ -577,6 +577,75 @@ def test_bingham_water_vs_oracle(name):
         assert nan_rel(g2.get(k), states[last][k]) <= form, k
 
 
+VIB = ["vib_2d_acc", "vib_2d_vel", "vib_3d_acc", "vib_3d_vel"]
+
+
+@pytest.mark.parametrize("name", VIB)
+def test_vibration_and_free_field_vs_oracle(name):
+    # flv 1 / 2: the vibration table drives the base and side walls (acceleration or velocity,
+    # zeroed outside the window and under the oversized load), the free-field boundaries and
+    # soil_damping_vibration act on the soil; the device clock follows the reference's t
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    for s in range(6):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare(g, o) == {}, f"step {s + 1}"
+    assert g.time() == _clock(deck.dt, 6)
+    last = max(states)
+    for k in ("xp", "vx", "sig", "interfss"):
+        form = nan_rel(o.get(k), states[last][k])
+        assert nan_rel(g.get(k), states[last][k]) <= form, k
+
+
+def _clock(dt, steps):
+    t = 0.0
+    for _ in range(steps):
+        t = t + dt
+    return t
+
+
+def test_vibration_graph_replay_and_resume():
+    # six steps in one call (graph replay) equal six single steps; a checkpoint after three keeps
+    # the clock, and the resumed run ends where the uninterrupted one does
+    import tempfile
+    deck, states = load_golden("vib_2d_vel")
+    a = make_gpu(deck, states[0])
+    b = make_gpu(deck, states[0])
+    assert a.step(6) == 0
+    for _ in range(6):
+        assert b.step(1) == 0
+    assert compare(a, b) == {}
+    c = make_gpu(deck, states[0])
+    assert c.step(3) == 0
+    with tempfile.TemporaryDirectory() as wd:
+        path = wd + "/ck.bin"
+        assert c.save_state(path) == 0
+        r = make_gpu(deck, states[0])
+        assert r.restore_state(path) == 0
+        assert r.time() == _clock(deck.dt, 3)
+        assert r.step(3) == 0
+    assert compare(r, a) == {}
+
+
+def test_vibration_needs_its_table():
+    from sphgpy.capi import GpuSim
+    deck, states = load_golden("vib_2d_acc")
+    d = deck.copy()
+    d.vib_loads = None
+    g = GpuSim.__new__(GpuSim)
+    from sphgpy.capi import product_lib, problem_from_deck, materials_from_deck, Sim
+    import ctypes as C
+    lib = product_lib()
+    g._pro, g._mat = problem_from_deck(d), materials_from_deck(d)
+    ctx = C.c_void_p()
+    assert lib.lib.sphg_create(C.byref(ctx), C.byref(g._pro), C.byref(g._mat), d.n, 0) == 0
+    Sim.__init__(g, lib, ctx, d.n)
+    g.load_state(states[0])
+    assert lib.lib.sphg_upload_done(ctx, d.n) == 100  # SPHG_E_ARG: no vibration table
+
+
 @pytest.mark.parametrize("name", ["pile3d_amcc", "pile3d_sb"])
 def test_slope_pile_vs_oracle(name):
     # the 3D slope-with-pile flag set at fixture size: modified Cam-Clay soil (AMCC no-op tangent,

@@ -37,7 +37,8 @@ GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1"
           "drag_3d_fli2", "drag_3d_fli3", "kgc_column2d_k3", "kgc_zoo2d_k4", "kgc_ssi2d_k3", "kgc_art3d_k3",
           "kgc_tran3d_k4", "ts_column2d", "ts_art3d", "ts_drag3d", "air_water_2d_a0", "air_water_2d_a1",
           "air_water_3d_a0", "air_water_3d_a1", "tm_2d_b1", "tm_3d_b1", "umcc_perm2", "zoo_perm3", "ssi_3d_t2",
-          "pile3d_amcc", "pile3d_sb", "bingham_2d", "bingham_3d"]
+          "pile3d_amcc", "pile3d_sb", "bingham_2d", "bingham_3d", "vib_2d_acc", "vib_2d_vel", "vib_3d_acc",
+          "vib_3d_vel"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -306,3 +307,22 @@ def test_takeda_morris_wall_slows_approach():
     assert low.sum() > 0
     assert np.all(st1.get("vx")[low, 1] >= st2.get("vx")[low, 1])
     assert np.any(st1.get("vx")[low, 1] > st2.get("vx")[low, 1])
+
+
+def test_vibration_fixtures_exercise_every_branch():
+    # the vibration decks move their walls while the window is open and stop them outside it or
+    # under the oversized load; the free-field and damping terms change the soil's acceleration
+    from sphgpy.capi import setting_par  # noqa: F401
+    for name, wall_mt in (("vib_2d_acc", 3), ("vib_2d_vel", 4)):
+        deck, states = load_golden(name)
+        base = (deck.type == 0) & (deck.matype == wall_mt)
+        side = (deck.type == 0) & (deck.matype == 5)
+        assert np.abs(states[3]["interfss"][base]).max() > 0.0
+        assert np.abs(states[3]["interfss"][side]).max() > 0.0
+        assert np.abs(states[6]["interfss"][base]).max() == 0.0  # t past edtime
+        d0 = deck.copy()
+        d0.flv = 0
+        o = OracleSim(d0, states[0], mode=GS)
+        assert o.step(6) == 0
+        soil = deck.type == 2
+        assert np.abs(o.get("ax")[soil] - states[6]["ax"][soil]).max() > 1.0
