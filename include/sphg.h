@@ -278,6 +278,12 @@ int sphg_get_time(sphg_ctx *ctx, double *t);
  * and soil_damping_vibration. Required before sphg_upload_done when flv > 0 (SPHG_E_ARG otherwise,
  * where the reference stops for a missing vibra.dat). */
 int sphg_set_vibration(sphg_ctx *ctx, double ttime, int nsteps, double sttime, double edtime, const double *loads);
+/* The VL section of <dir>/input.dat and the loads of <dir>/vibra.dat, as initial_pro and initial_vib
+ * read them (CPP_2-initialf.cpp:250-265, :293-322). loads may be null (parameters only) or must hold
+ * 3 * cap values (SPHG_E_CAPACITY when cap < nsteps). SPHG_E_ARG for a missing file or section, or
+ * nsteps < 1. Host only. */
+int sphg_read_vibration(const char *dir, double *ttime, int *nsteps, double *sttime, double *edtime, double *loads,
+                        int cap);
 /* Particles in the context (pPPro->ntotal, all realisations of a batch), -1 for a null context. */
 int sphg_particle_count(const sphg_ctx *ctx);
 

@@ -62,6 +62,19 @@ extern "C" int sphg_solve_project(const char *dir, int precision, int *last_step
         return rc;
     sphg_ctx *ctx = nullptr;
     if (int rc = sphg_create(&ctx, &pro, &mat, n, precision)) return rc;
+    if (pro.flv >= 1) {
+        // the VL section and vibra.dat (CPP_1-SPHLoops.cpp:306-340)
+        double tt_, st_, et_;
+        int ns = 0;
+        int rv = sphg_read_vibration(dir, &tt_, &ns, &st_, &et_, nullptr, 0);
+        std::vector<double> loads(3 * (size_t)(ns > 0 ? ns : 1));
+        if (rv == SPHG_OK) rv = sphg_read_vibration(dir, &tt_, &ns, &st_, &et_, loads.data(), ns);
+        if (rv == SPHG_OK) rv = sphg_set_vibration(ctx, tt_, ns, st_, et_, loads.data());
+        if (rv) {
+            sphg_destroy(ctx);
+            return rv;
+        }
+    }
     int rc = upload(ctx, n,
                     {{"xp", {xp.data(), 3}}, {"vxp", {vxp.data(), 3}}, {"rho", {rho.data(), 1}},
                      {"rhop", {rhop.data(), 1}}, {"mass", {mass.data(), 1}}, {"hl", {hl.data(), 1}},

@@ -135,7 +135,8 @@ const std::vector<ModelBlock> &model_blocks() {
 }
 
 // initial_pro (CPP_2-initialf.cpp:22-290) for the sections this library's flag set uses; the
-// rain, excavation and vibration sections belong to flags it refuses (flr, fleob, flv).
+// vibration section is read by sphg_read_vibration, the rain and excavation sections belong to
+// flags it refuses (flr, fleob).
 void read_problem(FILE *f, sphg_problem &p) {
     problem_defaults(p);
     if (find_section(f, "FL", false))
@@ -328,6 +329,34 @@ extern "C" int sphg_read_parameters(const char *dir, int n, const int *type, dou
     }
     (void)err;
     fclose(f);
+    return SPHG_OK;
+}
+
+// The VL section of <dir>/input.dat (CPP_2-initialf.cpp:250-265: ttime, nsteps, sttime, edtime)
+// and the loads of <dir>/vibra.dat (initial_vib, :293-322: nsteps rows of three values). loads may
+// be null (then only the parameters are read) or must hold 3 * cap values.
+extern "C" int sphg_read_vibration(const char *dir, double *ttime, int *nsteps, double *sttime, double *edtime,
+                                   double *loads, int cap) {
+    if (!dir || !ttime || !nsteps || !sttime || !edtime) return SPHG_E_ARG;
+    FILE *f = fopen((std::string(dir) + "/input.dat").c_str(), "r");
+    if (!f) return SPHG_E_ARG;
+    *ttime = *sttime = *edtime = 0.0;  // Para_Vibra::Para_Vibra (Class_Problem.cpp:121-129)
+    *nsteps = 0;
+    const bool ok = find_section(f, "VL", false) && read_vals(f, {D(*ttime), I(*nsteps), D(*sttime), D(*edtime)});
+    fclose(f);
+    if (!ok || *nsteps < 1) return SPHG_E_ARG;  // the reference reports error 1 without loads
+    if (!loads) return SPHG_OK;
+    if (cap < *nsteps) return SPHG_E_CAPACITY;
+    FILE *v = fopen((std::string(dir) + "/vibra.dat").c_str(), "r");
+    if (!v) return SPHG_E_ARG;  // CPP_1-SPHLoops.cpp:306-330 stops with error 1
+    for (int i = 0; i < *nsteps; i++) {
+        // a short file leaves the rest of the table as it was; the reference's table is
+        // uninitialised there, so it is zeroed first
+        loads[3 * i] = loads[3 * i + 1] = loads[3 * i + 2] = 0.0;
+    }
+    for (int i = 0; i < *nsteps; i++)
+        if (fscanf(v, "%lf%lf%lf", &loads[3 * i], &loads[3 * i + 1], &loads[3 * i + 2]) != 3) break;
+    fclose(v);
     return SPHG_OK;
 }
 

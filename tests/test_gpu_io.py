@@ -151,3 +151,30 @@ def test_project_runner_writes_the_reference_outputs(tmp_path):
     assert rc == 0
     for l in (1, 3, 6):
         assert open(odb_path(str(api), l), "rb").read() == open(odb_path(str(proj), l), "rb").read()
+
+
+def test_project_runner_with_vibration_loads(tmp_path):
+    # flv 2 project: the runner reads the VL section and vibra.dat, and its tables at the output
+    # steps are those of the oracle with the same table (walls driven, free-field boundaries,
+    # soil_damping_vibration)
+    import subprocess
+    from sphgpy.capi import SOLVE_PATH, odb_path
+    deck, states = load_golden("vib_2d_vel")
+    deck = deck.copy()
+    deck.loop, deck.fop = 6, 3
+    proj = tmp_path / "proj"
+    proj.mkdir()
+    deck.write_input_dat(str(proj / "input.dat"))
+    r = subprocess.run([SOLVE_PATH, str(proj)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    done = 0
+    for l in (1, 3, 6):
+        assert o.step(l - done) == 0
+        done = l
+        st = {k: o.get(k) for k in ODB_FIELDS}
+        st["type"], st["matype"] = deck.type, deck.matype
+        d = tmp_path / f"o{l}"
+        d.mkdir()
+        want = open(write_odb_fields(str(d), l, odb_time(deck.dt, l), st), "rb").read()
+        assert open(odb_path(str(proj), l), "rb").read() == want, l

@@ -190,3 +190,28 @@ def test_project_runner_argument_errors(tmp_path):
     d.flspv = 1
     d.write_input_dat(str(tmp_path / "input.dat"))
     assert L.sphg_solve_project(str(tmp_path).encode(), 0, None) == 102
+
+
+def test_read_vibration_section_and_table(tmp_path):
+    # sphg_read_vibration: the VL section of input.dat and vibra.dat as the deck writer wrote them
+    import ctypes as C
+    from sphgpy import vibration_box
+    from sphgpy.capi import product_lib
+    L = product_lib().lib
+    deck = vibration_box(2, 1)
+    deck.write_input_dat(str(tmp_path / "input.dat"))
+    tt, st, et, ns = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+    D = C.POINTER(C.c_double)
+    assert L.sphg_read_vibration(str(tmp_path).encode(), C.byref(tt), C.byref(ns), C.byref(st), C.byref(et),
+                                 C.cast(None, D), 0) == 0
+    assert (tt.value, ns.value, st.value, et.value) == (deck.vib_ttime, deck.vib_nsteps, deck.vib_sttime,
+                                                        deck.vib_edtime)
+    loads = np.zeros((ns.value, 3))
+    assert L.sphg_read_vibration(str(tmp_path).encode(), C.byref(tt), C.byref(ns), C.byref(st), C.byref(et),
+                                 loads.ctypes.data_as(D), ns.value) == 0
+    assert np.array_equal(loads, deck.vib_loads)
+    assert L.sphg_read_vibration(str(tmp_path).encode(), C.byref(tt), C.byref(ns), C.byref(st), C.byref(et),
+                                 loads.ctypes.data_as(D), ns.value - 1) == 103  # SPHG_E_CAPACITY
+    (tmp_path / "vibra.dat").unlink()
+    assert L.sphg_read_vibration(str(tmp_path).encode(), C.byref(tt), C.byref(ns), C.byref(st), C.byref(et),
+                                 loads.ctypes.data_as(D), ns.value) == 100
