@@ -495,6 +495,30 @@ int sphg_rf_generate_pair(const sphg_rf_field *f_fai, const sphg_rf_field *f_c, 
  * context with sphg_set_field ("ps_fai" ... "pf_c"). */
 int sphg_read_parameters(const char *dir, int n, const int *type, double *values);
 
+/* ---- Dynamic particle sets (flr 1, fleob 1) ---------------------------------------------------- */
+
+/* Para_Rain (Class_Problem.h:83-100) as the RP section gives it (CPP_2-initialf.cpp:180-201):
+ * rtype 1 rain, 2 / 3 water-level decrease on the left / right, 4 cement injection (5-7 accepted,
+ * nothing happens); nrain reserve particles at the end of the array (type 7); nhor per row (0
+ * means 35); initial pressure factor; drop velocity of rain. */
+typedef struct {
+    int rtype, nrain, nhor;
+    double press_factor, drop_velocity;
+} sphg_rain;
+/* Rain and particle deletion (flr 1; rainini, rainpro, cementpro, waterdel_min / _max, partidel,
+ * CPP_15-1-rain.cpp, CPP_15-2-partidel.cpp, called from CPP_1-SPHLoops.cpp:455-511, :757-760).
+ * Set before sphg_upload_done, which derives what the reference derives from the initial
+ * particles (the water-level limits, rainini's period and velocity, the cement base). Upload
+ * returns SPHG_E_ARG for rtype outside 1-7 (the reference's error 6) or a period below one step
+ * (the reference divides by zero), SPHG_E_UNSUPPORTED in a batch context. */
+int sphg_set_rain(sphg_ctx *ctx, const sphg_rain *rain);
+/* Excavation or backfill (fleob 1; exca_or_back, CPP_16-EOB.cpp:13-74): ecp[ecount][3] = first and
+ * last particle (1-based, inclusive) and the step at which the stage's particles swap type and
+ * etype, one stage per step in order (CPP_1-SPHLoops.cpp:513-520). The reference's reader takes
+ * the first two per stage and never assigns the step (CPP_2-initialf.cpp:146), so there the step is
+ * indeterminate; here the caller gives it. Set before sphg_upload_done. */
+int sphg_set_excavation(sphg_ctx *ctx, int ecount, const int *ecp);
+
 /* ---- Correlated random variables (flspv 5, CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863) -- */
 
 /* Flag_SpatialVariability members random_variable_generate reads (Class_spatial_variability.h:10-42) */

@@ -73,6 +73,10 @@ struct sphg_ctx {
     // Para_Vibra (flv 1, 2): period, loads per period, window; the table is in S.vib_loads
     double vib_ttime = 0.0, vib_sttime = 0.0, vib_edtime = 0.0;
     int vib_nsteps = 0;
+    // Para_Rain (flr 1) and Para_EC (fleob 1) as the caller set them; eob_len: longest stage
+    bool rain_set = false;
+    sphg_rain rain{};
+    int ecount = 0, eob_len = 0;
     bool eig_stale = false;    // strep / beta not written by the last stress pass (k_eig on read)
     // neighbours per load round trip of the density, normalisation, strain and XSPH passes
     // (sphg_autotune); each has the widths kBwChoices[k]
@@ -239,6 +243,8 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
     plan.push_back({(void **)&S.tnow, 8});
+    I(S.dyn, 4);  // rain reserve, cement flag, excavation stage (dynamic particle sets)
+    S.ecp = nullptr;
     // vibration loads and free-field boundaries: the clock above, the tangent stiffness and the
     // pair sums (the load table is allocated by sphg_set_vibration)
     S.vib_loads = nullptr;
@@ -318,6 +324,7 @@ extern "C" void sphg_destroy(sphg_ctx *c) {
     }
     if (c->pool) hipFree(c->pool);
     if (c->S.vib_loads) hipFree(c->S.vib_loads);
+    if (c->S.ecp) hipFree(c->S.ecp);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -622,9 +629,96 @@ static int fluid_slot_of(int type, int matype, int etype) {
     return 4;  // default-constructed Para_Fluid
 }
 
+// Rain (flr 1) and excavation (fleob 1) set-up at upload: what the reference derives before its
+// loop from the initial particles - x_min / x_max of the water of etype 7 (setting_par,
+// CPP_2-initialf.cpp:1307-1311), rainini's period and velocity (CPP_15-1-rain.cpp:10-45; nrain /
+// nhor is an integer division there), the cement base (center_cement, :133-163) - and the device
+// counters. One realisation per context (the counters are per run).
+static int upload_dynamic_sets(sphg_ctx *c) {
+    Params &P = c->P;
+    const sphg_problem &pro = c->pro;
+    P.flr = pro.flr;
+    P.fleob = pro.fleob;
+    P.ecount = pro.fleob == 1 ? c->ecount : 0;
+    P.rtype = 0;
+    P.rain_lp = 0;
+    if (pro.flr != 1 && pro.fleob != 1) return SPHG_OK;
+    if (P.nreal != 1) return SPHG_E_UNSUPPORTED;
+    if (pro.fleob == 1 && c->ecount < 1) return SPHG_E_ARG;
+    int dyn[4] = {0, 1, 0, 0};
+    if (pro.flr == 1) {
+        if (!c->rain_set) return SPHG_E_ARG;
+        const sphg_rain &r = c->rain;
+        if (r.rtype < 1 || r.rtype > 7) return SPHG_E_ARG;  // the reference's error 6
+        const int n = c->n, dim = pro.ndim;
+        std::vector<double> xp(3 * (size_t)n);
+        for (int k = 0; k < 3; k++)
+            HIPCHK(hipMemcpy(xp.data() + (size_t)k * n, c->S.xp[c->cur][k], (size_t)n * 8, hipMemcpyDeviceToHost));
+        double xmax = -10000.0, xmin = 10000.0, mx[3] = {-1000.0, -1000.0, -1000.0}, mn[3] = {1000.0, 1000.0, 1000.0};
+        for (int i = 0; i < n; i++) {
+            if (c->h_type[i] == 1 && c->h_etype[i] == 7) {
+                xmax = fmax(xmax, xp[(size_t)(dim - 2) * n + i]);
+                xmin = fmin(xmin, xp[(size_t)(dim - 2) * n + i]);
+            }
+            if (c->h_type[i] == 7)
+                for (int k = 0; k < 3; k++) {
+                    mx[k] = fmax(mx[k], xp[(size_t)k * n + i]);
+                    mn[k] = fmin(mn[k], xp[(size_t)k * n + i]);
+                }
+        }
+        const int nhor = r.nhor == 0 ? 35 : r.nhor;
+        double temp = 0.0;
+        P.vrain = 0.0;
+        if (r.rtype == 1 || r.rtype == 4) {
+            temp = r.nrain / nhor + 1.0;
+            P.vrain = r.rtype == 1 ? r.drop_velocity : 2.0;
+        } else if (r.rtype == 2 || r.rtype == 3) {
+            temp = fabs(xmax - xmin) / pro.dr + 1.0;
+        }
+        if (r.rtype <= 4) {
+            P.rain_lp = (int)((pro.loop - pro.inip) / temp);
+            if (P.rain_lp < 1) return SPHG_E_ARG;  // the reference divides by zero
+        }
+        P.rtype = r.rtype;
+        P.press_factor = r.press_factor;
+        P.rx_min = xmin;
+        P.rx_max = xmax;
+        for (int k = 0; k < 3; k++) P.cement_base[k] = (mx[k] + mn[k]) * 0.5;
+        P.dom_xmin = pro.xmin;
+        P.dom_xmax = pro.xmax;
+        dyn[0] = r.nrain;
+    }
+    HIPCHK(hipMemcpy(c->S.dyn, dyn, sizeof dyn, hipMemcpyHostToDevice));
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_rain(sphg_ctx *c, const sphg_rain *r) {
+    if (!c || !r) return SPHG_E_ARG;
+    c->rain = *r;
+    c->rain_set = true;
+    c->uploaded = false;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_excavation(sphg_ctx *c, int ecount, const int *ecp) {
+    if (!c || !ecp || ecount < 1) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->S.ecp) {
+        HIPCHK(hipFree(c->S.ecp));
+        c->S.ecp = nullptr;
+    }
+    HIPCHK(hipMalloc(&c->S.ecp, (size_t)ecount * 3 * 4));
+    HIPCHK(hipMemcpy(c->S.ecp, ecp, (size_t)ecount * 3 * 4, hipMemcpyHostToDevice));
+    c->ecount = ecount;
+    c->eob_len = 0;
+    for (int s = 0; s < ecount; s++) c->eob_len = std::max(c->eob_len, ecp[3 * s + 1] - ecp[3 * s] + 1);
+    c->uploaded = false;
+    return SPHG_OK;
+}
+
 static int check_supported(const sphg_problem &p) {
     if (p.flknl < 1 || p.flknl > 4) return SPHG_E_UNSUPPORTED;  // 3 / 4: with kernel_gradient_correction
-    if (p.flr || p.fleob || p.flv < 0 || p.flv > 2) return SPHG_E_UNSUPPORTED;
+    if (p.flr < 0 || p.flr > 1 || p.fleob < 0 || p.fleob > 1 || p.flv < 0 || p.flv > 2) return SPHG_E_UNSUPPORTED;
     if (p.fli < 0 || p.fli > 3 || p.ss_type < 0 || p.ss_type > 3) return SPHG_E_UNSUPPORTED;
     if (p.flbndy != 1 && p.flbndy != 2 && p.flbndy != 3 && p.flbndy != 4 && p.flbndy != 5 && p.flbndy != 13 &&
         p.flbndy != 14)
@@ -716,6 +810,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.vib_sttime = c->vib_sttime;
     P.vib_edtime = c->vib_edtime;
     P.vib_nsteps = c->vib_nsteps;
+    if ((rc = upload_dynamic_sets(c))) return rc;
     P.water_acc_minor = c->mat.water[0].acc_type != 0;
     P.air_acc_minor = c->mat.air.acc_type != 0;
     // realisation of every particle (batch contexts: realisation-major ranges of counts[r]) and
@@ -750,7 +845,8 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     c->full_models = false;
     for (int i = 0; i < n; i++) {
         int m = c->h_matype[i], mt = ((m >= 0 && m < 30) || (m >= 50 && m < 60)) ? m % 10 : -1;
-        if (c->h_type[i] == 2 && (mt == 1 || mt == 3 || mt == 8)) c->full_models = true;
+        const bool soil = c->h_type[i] == 2 || (pro.fleob == 1 && c->h_etype[i] == 2);  // backfill
+        if (soil && (mt == 1 || mt == 3 || mt == 8)) c->full_models = true;
     }
     // material slots with host-evaluated transcendental constants
     std::vector<SoilSlot> soil(kNSlots);
@@ -805,7 +901,9 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     for (int i = 0; i < n; i++) {
         fd[i] = fl[fs[i]].dens;
         int t = c->h_type[i];
-        if (t == 1 || t == 3) {
+        // reserve particles rain or backfill may turn into water count as water
+        const bool dyn_water = (pro.flr == 1 || pro.fleob == 1) && t == 7 && c->h_etype[i] == 1;
+        if (t == 1 || t == 3 || dyn_water) {
             double pr = fl[fs[i]].porosity;
             if (!seen) { por0 = pr; seen = true; }
             else if (memcmp(&pr, &por0, 8) != 0) P.fluid_por_uniform = 0;
@@ -923,6 +1021,14 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
     const int cur = c->cur, n = c->n;
     if (phase_mask & 1) {
         if (c->P.flv > 0) LAUNCH("time", k_time_step, 1, c->S, c->P);
+        if (c->P.flr == 1) {
+            LAUNCH("rain", k_rain<NDIM>, 1, c->S, c->P, cur);
+            LAUNCH("waterdel", k_waterdel<NDIM>, grid_for(n), c->S, c->P, cur);
+        }
+        if (c->P.fleob == 1 && c->eob_len > 0) {
+            LAUNCH("eob", k_eob, grid_for(c->eob_len), c->S, c->P);
+            LAUNCH("eob_next", k_eob_next, 1, c->S, c->P);
+        }
         enqueue_nnps<NDIM>(c);
     }
     if (phase_mask & 2) {
@@ -970,6 +1076,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
     }
     if (phase_mask & 64) {
         LAUNCH("update", (k_update<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if (c->P.flr == 1) LAUNCH("partidel", k_partidel, grid_for(n), c->S, c->P, cur ^ 1);
         LAUNCH("step_inc", k_step_inc, 1, c->S);
     }
 }

@@ -71,6 +71,11 @@ struct Params {
     int mrv_ok;              // mixed form: the momentum kernel writes every particle's mrv (no contact tail)
     double vib_ttime, vib_sttime, vib_edtime;  // Para_Vibra: period, start and end time (flv 1, 2)
     int vib_nsteps;                            // loads per period
+    // dynamic particle sets: rain / cement / water-level decrease (flr 1) and excavation or
+    // backfill (fleob 1)
+    int flr, fleob, rtype, rain_lp, ecount;
+    double press_factor, vrain, rx_min, rx_max, cement_base[3];
+    double dom_xmin, dom_xmax;                 // Cell_Con xmin / xmax (partidel)
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -133,6 +138,8 @@ struct State {
     double *vstiff;       // largest diagonal entry (with zero) of the particle's last tangent (depp0)
     double *ffb[3], *ffs[3];  // free-field sums against matype-5 walls / between soil and its columns
     double *vdamp[4];     // soil_damping_vibration sums: damp[3], tempdown
+    // dynamic particle sets: dyn = {nrain, cement_flag, nstage}; ecp[ecount][3] = start, end, step
+    int *dyn, *ecp;
     int *err, *step;      // sticky error word; current step number l
     const Grid *grids;    // one per realisation
     const int *rid;       // realisation of each particle

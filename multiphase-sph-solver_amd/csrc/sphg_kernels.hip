@@ -3186,6 +3186,143 @@ __global__ void k_step_inc(State S) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *S.step += 1;
 }
 
+// ------------------------------------------------------------------------------------------
+// Dynamic particle sets (flr 1: CPP_15-1-rain.cpp, CPP_15-2-partidel.cpp; fleob 1: CPP_16-EOB.cpp)
+// ------------------------------------------------------------------------------------------
+// rain / cement production and water-level decrease fire when (l - inip) > 0 and (l - inip) is a
+// multiple of rainini's period (CPP_1-SPHLoops.cpp:495-511)
+__device__ __forceinline__ bool rain_due(const State &S, const Params &P) {
+    const int lr = step_of(S) - P.inip;
+    return P.flr == 1 && lr > 0 && P.rain_lp > 0 && lr % P.rain_lp == 0;
+}
+
+// the mixed form's packed neighbour record of a particle whose mass or velocity changed here
+__device__ __forceinline__ void refresh_mra(const State &S, int k) {
+    if (S.mra) S.mra[k] = make_float4((float)S.mass[k], (float)S.vxp[0][k], (float)S.vxp[1][k], (float)S.vxp[2][k]);
+}
+
+// rainpro (rtype 1, CPP_15-1-rain.cpp:48-98) and cementpro (rtype 4, :101-131): one thread walks
+// the reserve from the end of the array, a row (rain: the same height within 0.1 dr) or a group
+// (cement: the same matype as the call's cement flag) per call, as the reference's serial loops
+template <int NDIM>
+__global__ void k_rain(State S, Params P, int cur) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || !rain_due(S, P)) return;
+    if (P.rtype != 1 && P.rtype != 4) return;
+    const int n = P.n;
+    const double g = NDIM == 2 ? P.gy : P.gz;
+    int nrain = S.dyn[0];
+    if (P.rtype == 1) {
+        if (nrain <= 0) return;
+        int k = n - nrain;
+        for (;;) {
+            const double temp_xp = S.xp[cur][NDIM - 1][k];
+            nrain = nrain - 1;
+            S.xp[cur][NDIM - 1][k] = 0.95 * temp_xp;
+            S.type[k] = 1;
+            S.matype[k] = 0;
+            S.etype[k] = 7;
+            const FluidSlot &f = S.fluid[S.fluid_slot[k]];
+            const double rhop = f.porosity * f.dens, hl = P.dr;
+            S.rhop[k] = rhop;
+            S.hl[k] = hl;
+            S.mass[k] = NDIM == 2 ? rhop * hl * hl : rhop * hl * hl * hl;
+            for (int q = 0; q < 3; q++) {
+                S.vxp[q][k] = 0.0;
+                S.vx[cur][q][k] = 0.0;
+            }
+            S.vxp[NDIM - 1][k] = P.vrain;
+            S.prep[k] = rhop * P.dr * (-g) * P.press_factor;
+            S.satu[k] = 0.99;
+            refresh_mra(S, k);
+            k++;
+            if (k > n - 1 || !(fabs(S.xp[cur][NDIM - 1][k] - temp_xp) < 0.1 * P.dr)) break;
+        }
+    } else {
+        const int cf = S.dyn[1];
+        if (nrain > 0) {
+            int k = n - nrain;
+            do {
+                nrain = nrain - 1;
+                double drp = 0.0, xk[3];
+                for (int q = 0; q < 3; q++) xk[q] = S.xp[cur][q][k];
+                for (int q = 0; q < NDIM; q++) drp = drp + (xk[q] - P.cement_base[q]) * (xk[q] - P.cement_base[q]);
+                drp = sqrt(drp);
+                S.type[k] = 1;
+                S.matype[k] = 0;
+                const FluidSlot &f = S.fluid[S.fluid_slot[k]];
+                const double rhop = f.porosity * f.dens, hl = P.dr;
+                S.rhop[k] = rhop;
+                S.hl[k] = hl;
+                S.mass[k] = NDIM == 2 ? rhop * hl * hl : rhop * hl * hl * hl;
+                for (int q = 0; q < 3; q++) S.vxp[q][k] = P.vrain * (xk[q] - P.cement_base[q]) / drp;
+                S.prep[k] = rhop * P.dr * (-g) * P.press_factor;
+                S.satu[k] = 0.99;
+                refresh_mra(S, k);
+                k++;
+            } while (k < n - 1 && S.matype[k] == cf);  // the bound first: the reference reads one past the array
+        }
+        S.dyn[1] = cf + 1;
+    }
+    S.dyn[0] = nrain;
+}
+
+// waterdel_min / waterdel_max (rtype 2 / 3, CPP_15-2-partidel.cpp:52-130): water of etype 7 past
+// x_min + 3 dr (x_max - 3 dr) on the second-to-last axis goes back to the reserve
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_waterdel(State S, Params P, int cur) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n || !rain_due(S, P) || (P.rtype != 2 && P.rtype != 3)) return;
+    if (!(S.type[i] == 1 && S.etype[i] == 7)) return;
+    const double x = S.xp[cur][NDIM - 2][i];
+    const bool del = P.rtype == 2 ? x <= P.rx_min + 3.0 * P.dr : x >= P.rx_max - 3.0 * P.dr;
+    if (!del) return;
+    S.type[i] = 7;
+    S.etype[i] = 1;
+    for (int q = 0; q < 3; q++) {
+        S.vxp[q][i] = 0.0;
+        S.vx[cur][q][i] = 0.0;
+        S.ax[q][i] = 0.0;
+    }
+    refresh_mra(S, i);
+    atomicAdd(&S.dyn[0], 1);
+}
+
+// partidel (CPP_15-2-partidel.cpp:16-50), after the update (positions in buffer nxt): water 7 dr
+// beyond the domain box in x (within dr) and above y = 0.12 leaves the run
+__global__ __launch_bounds__(256) void k_partidel(State S, Params P, int nxt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n || S.type[i] != 1) return;
+    const double x = S.xp[nxt][0][i], y = S.xp[nxt][1][i], dr = P.dr;
+    const double dx1 = fabs(x - P.dom_xmax - 7.0 * dr), dx2 = fabs(x - P.dom_xmin + 7.0 * dr), dy1 = y - 0.12;
+    if (!((dx1 < dr || dx2 < dr) && dy1 > 0.0)) return;
+    S.type[i] = 7;
+    for (int q = 0; q < 3; q++) {
+        S.vxp[q][i] = 0.0;
+        S.vx[nxt][q][i] = 0.0;
+        S.ax[q][i] = 0.0;
+    }
+    refresh_mra(S, i);
+}
+
+// exca_or_back (CPP_16-EOB.cpp:13-74): at the step of stage nstage, its particles [start, end]
+// swap type and etype; k_eob_next then moves to the next stage (CPP_1-SPHLoops.cpp:513-520)
+__device__ __forceinline__ bool eob_due(const State &S, const Params &P) {
+    const int st = S.dyn[2];
+    return P.fleob == 1 && st < P.ecount && step_of(S) == S.ecp[3 * st + 2];
+}
+__global__ __launch_bounds__(256) void k_eob(State S, Params P) {
+    if (!eob_due(S, P)) return;
+    const int st = S.dyn[2];
+    const int i = S.ecp[3 * st] - 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < S.ecp[3 * st] - 1 || i >= S.ecp[3 * st + 1] || i >= P.n) return;
+    const int t = S.type[i];
+    S.type[i] = S.etype[i];
+    S.etype[i] = t;
+}
+__global__ void k_eob_next(State S, Params P) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && eob_due(S, P)) S.dyn[2] += 1;
+}
+
 // explicit instantiations used by the host side
 template __global__ void k_bin<2>(State, Params, int);
 template __global__ void k_bin<3>(State, Params, int);

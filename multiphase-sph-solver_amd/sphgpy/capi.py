@@ -181,6 +181,11 @@ def kl_field(deck: Deck, dist: int, mean: float, sd: float, rl, field: int = 1, 
     return f
 
 
+class Rain(C.Structure):
+    """sphg_rain: Para_Rain as the RP section gives it."""
+    _fields_ = [("rtype", _I), ("nrain", _I), ("nhor", _I), ("press_factor", _D), ("drop_velocity", _D)]
+
+
 class RVFlags(C.Structure):
     """sphg_rv_flags: the Flag_SpatialVariability members random_variable_generate reads."""
     _fields_ = [("flag_fai", _I), ("flag_c", _I), ("flag_faiw", _I), ("flag_cw", _I), ("coe_frict_cohesion", _D)]
@@ -413,6 +418,10 @@ def product_lib() -> _Lib:
         L.sphg_get_time.restype = _I
         L.sphg_set_vibration.argtypes = [C.c_void_p, _D, _I, _D, _D, C.POINTER(_D)]
         L.sphg_set_vibration.restype = _I
+        L.sphg_set_rain.argtypes = [C.c_void_p, C.POINTER(Rain)]
+        L.sphg_set_rain.restype = _I
+        L.sphg_set_excavation.argtypes = [C.c_void_p, _I, C.POINTER(_I)]
+        L.sphg_set_excavation.restype = _I
         L.sphg_error_string.restype = C.c_char_p
         L.sphg_build_info.restype = C.c_char_p
         L.sphg_enable_timing.argtypes = [C.c_void_p, _I]
@@ -570,6 +579,19 @@ class GpuSim(Sim):
         super().__init__(lib, ctx, deck.n)
         if deck.flv > 0:
             self.set_vibration(deck)
+        self.set_dynamic_sets(deck)
+
+    def set_dynamic_sets(self, deck: Deck) -> None:
+        """sphg_set_rain / sphg_set_excavation from the deck's Para_Rain and Para_EC."""
+        if deck.flr == 1:
+            rc = self.L.lib.sphg_set_rain(self.ctx, C.byref(Rain(*deck.rain)))
+            if rc:
+                raise RuntimeError(f"sphg_set_rain: error {rc}")
+        if deck.fleob == 1:
+            ecp = np.ascontiguousarray(np.array(deck.ecp, dtype=np.int32).reshape(-1, 3))
+            rc = self.L.lib.sphg_set_excavation(self.ctx, ecp.shape[0], _iptr(ecp))
+            if rc:
+                raise RuntimeError(f"sphg_set_excavation: error {rc}")
 
     def set_vibration(self, deck: Deck) -> None:
         """sphg_set_vibration from the deck's Para_Vibra fields and loads."""

@@ -134,6 +134,10 @@ class Deck:
     vib_sttime: float = 0.0
     vib_edtime: float = 0.0
     vib_loads: np.ndarray = None
+    # Para_Rain (flr 1): (rtype, nrain, nhor, press_factor, drop_velocity)
+    rain: tuple = None
+    # Para_EC (fleob 1): stages (first, last, step); input.dat carries first and last only
+    ecp: list = None
 
     @property
     def n(self) -> int:
@@ -201,6 +205,10 @@ class Deck:
                              self.vxp[i, 2], int(self.type[i]), int(self.matype[i]), int(self.permtype_p[i]),
                              int(self.etype[i])])
         sec("PA", rows)
+        if self.flr == 1:
+            sec("RP", [list(self.rain)])  # initial_pro, CPP_2-initialf.cpp:180-201
+        if self.fleob == 1:
+            sec("EC", [[len(self.ecp)]] + [[int(a), int(b)] for a, b, _ in self.ecp])  # :125-157
         if self.flv >= 1:
             # VL section (initial_pro, CPP_2-initialf.cpp:250-265) and the loads of vibra.dat
             # beside input.dat (initial_vib, :293-322)
@@ -775,6 +783,86 @@ def vibration_box(ndim=2, flv=1, nx=8, ny=3, nz=5, dx=0.002, steps=6, seed=81, a
     deck.vib_ttime, deck.vib_nsteps = 4.0 * d.dt, 8
     deck.vib_sttime, deck.vib_edtime = 1.5 * d.dt, (steps - 0.5) * d.dt
     deck.vib_loads = loads
+    return deck
+
+
+def rain_box(ndim=2, rtype=1, nx=8, ny=3, nz=4, dx=0.002, steps=6, seed=91, rows=2, vdrop=-0.5) -> Deck:
+    """Drucker-Prager soil in a rigid-walled box (flbndy 2) with the dynamic water sets of flr 1
+    (CPP_15-1-rain.cpp, CPP_15-2-partidel.cpp). rtype 1: `rows` rows of reserve particles (type 7,
+    etype 1) above the soil, last in the particle array, fall in as rain one row per period; rtype
+    4: two groups of reserve particles (matype 1 then 2) are injected radially from their centre as
+    cement, a group per period; rtype 2 / 3: a water layer of etype 7 over the soil loses its
+    left (right) columns back to the reserve."""
+    rng = np.random.default_rng(seed)
+    grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
+    d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, flbndy=2, coe_bndy=0.1, flr=1,
+             art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"rain_{ndim}d_r{rtype}")
+    d.model_flags[2] = 1
+    d.soil_layers[2] = [dp_layer(c=300.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0, eps0=0.6)]
+    w = default_fluid()
+    w.update(dens=1000.0, vis=1e-3, vpu=30.0, press_type=0, acc_type=0, bhtype=0, pre_coe=1.0, strain_min=1.0,
+             porosity=1.0)
+    w1, w2 = dict(w), dict(w)
+    w1.update(dens=1200.0, porosity=0.9)
+    w2.update(dens=1400.0, porosity=0.8)
+    d.water = [w, w1, w2]
+    H = nz + 10
+    lat = (lambda a, b, c_, e: _lattice2d(a, b, c_, e, dx)) if ndim == 2 else \
+        (lambda a, b, c_, e: _lattice3d(a, b, 0, ny, c_, e, dx))
+    parts = [(lat(0, nx, 0, nz), 2, 2)]
+    if rtype in (2, 3):
+        parts.append((lat(0, nx, nz, nz + 3), 1, 0))
+    if ndim == 2:
+        walls = [(_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0), (_lattice2d(-3, 0, 0, H + 3, dx), 0, 0),
+                 (_lattice2d(nx, nx + 3, 0, H + 3, dx), 0, 0)]
+    else:
+        walls = [(_box_walls_3d(nx, ny, H + 3, dx), 0, 0)]
+    parts += walls
+    reserve = []
+    if rtype == 1:
+        for r in range(rows):
+            reserve.append((lat(0, nx, H - 1 - 2 * r, H - 2 * r), 7, 0))
+    elif rtype == 4:
+        reserve.append((lat(2, nx - 2, H - 4, H - 2), 7, 1))
+        reserve.append((lat(2, nx - 2, H - 2, H), 7, 2))
+    parts += reserve
+    deck = _assemble(d, parts, rng)
+    if rtype in (2, 3):
+        deck.etype[deck.type == 1] = 7
+    nres = int(sum(p[0].shape[0] for p in reserve))
+    nhor = nx if ndim == 2 else nx * ny
+    if rtype == 4:
+        nhor = nres // 2
+    deck.rain = (rtype, nres, nhor, 1.0, vdrop)
+    return deck
+
+
+def eob_box(ndim=2, nx=8, ny=3, nz=6, dx=0.002, steps=6, seed=93) -> Deck:
+    """Excavation and backfill (fleob 1, CPP_16-EOB.cpp): a Drucker-Prager block in a rigid-walled
+    box whose top-right part is excavated at step 2 (type 2 -> 7) while a backfill next to it (type
+    7, etype 2) turns into soil at step 4."""
+    rng = np.random.default_rng(seed)
+    grav = (0.0, -9.81, 0.0) if ndim == 2 else (0.0, 0.0, -9.81)
+    d = Deck(ndim=ndim, dr=dx, dt=0.2 * dx / 600.0, loop=steps, fop=1, gravity=grav, flbndy=2, coe_bndy=0.1,
+             fleob=1, art_vis=((0.1, 0.0), (0.1, 0.0), (0.0, 0.0)), name=f"eob_{ndim}d")
+    d.model_flags[2] = 1
+    d.soil_layers[2] = [dp_layer(c=300.0, fai=30.0, e=1.8e6, poi=0.3, dens=2000.0, eps0=0.6)]
+    lat = (lambda a, b, c_, e: _lattice2d(a, b, c_, e, dx)) if ndim == 2 else \
+        (lambda a, b, c_, e: _lattice3d(a, b, 0, ny, c_, e, dx))
+    h = nx // 2
+    parts = [(lat(0, nx, 0, nz - 2), 2, 2), (lat(h, nx, nz - 2, nz), 2, 2), (lat(0, h, nz - 2, nz), 7, 2)]
+    if ndim == 2:
+        parts += [(_lattice2d(-3, nx + 3, -3, 0, dx), 0, 0), (_lattice2d(-3, 0, 0, nz + 3, dx), 0, 0),
+                  (_lattice2d(nx, nx + 3, 0, nz + 3, dx), 0, 0)]
+    else:
+        parts += [(_box_walls_3d(nx, ny, nz + 3, dx), 0, 0)]
+    deck = _assemble(d, parts, rng)
+    n0 = parts[0][0].shape[0]
+    n1 = parts[1][0].shape[0]
+    n2 = parts[2][0].shape[0]
+    deck.etype[n0:n0 + n1] = 7      # excavated: swaps to type 7
+    deck.etype[n0 + n1:n0 + n1 + n2] = 2  # backfill: swaps to soil
+    deck.ecp = [(n0 + 1, n0 + n1, 2), (n0 + n1 + 1, n0 + n1 + n2, 4)]
     return deck
 
 

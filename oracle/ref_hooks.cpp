@@ -357,6 +357,46 @@ int rf_mode(const char *dir, int kind) {
     return 0;
 }
 
+// Excavation-or-backfill mode: the reference's exca_or_back (CPP_16-EOB.cpp:13-74) for one stage.
+// eob_in.bin: int32 n, start, end; int32 type[n], etype[n] -> eob_out.bin: int32 type[n], etype[n]
+int eob_mode(const char *dir) {
+    std::string in = std::string(dir) + "/eob_in.bin", out = std::string(dir) + "/eob_out.bin";
+    FILE *f = fopen(in.c_str(), "rb");
+    if (!f) return 2;
+    int n = 0, se[2];
+    bool ok = fread(&n, 4, 1, f) == 1 && n > 0 && fread(se, 4, 2, f) == 2;
+    std::vector<int> ty(ok ? n : 0), et(ok ? n : 0);
+    ok = ok && fread(ty.data(), 4, n, f) == (size_t)n && fread(et.data(), 4, n, f) == (size_t)n;
+    fclose(f);
+    if (!ok) return 2;
+    Particle *par = new Particle[n];
+    for (int i = 0; i < n; i++) {
+        par[i].type = ty[i];
+        par[i].etype = et[i];
+    }
+    Para_Pro pro;
+    pro.ntotal = n;
+    Para_EC ec;
+    ec.ecount = 1;
+    ec.memloads();
+    ec.ecp[0][0] = se[0];
+    ec.ecp[0][1] = se[1];
+    ec.ecp[0][2] = 0;
+    clExcBac_Fun fn;
+    fn.exca_or_back(par, &pro, ec, 1, 0);
+    for (int i = 0; i < n; i++) {
+        ty[i] = par[i].type;
+        et[i] = par[i].etype;
+    }
+    delete[] par;
+    FILE *o = fopen(out.c_str(), "wb");
+    if (!o) return 2;
+    fwrite(ty.data(), 4, n, o);
+    fwrite(et.data(), 4, n, o);
+    fclose(o);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--kl") == 0) return kl_mode(argv[2]);
     if (argc >= 3 && strcmp(argv[1], "--chol") == 0) return chol_mode(argv[2]);
@@ -366,8 +406,9 @@ int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--kln") == 0) return rf_mode(argv[2], 3);
     if (argc >= 3 && strcmp(argv[1], "--rv") == 0) return rf_mode(argv[2], 4);
     if (argc >= 3 && strcmp(argv[1], "--rfi") == 0) return rf_mode(argv[2], 5);
+    if (argc >= 3 && strcmp(argv[1], "--eob") == 0) return eob_mode(argv[2]);
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] | --klc | --kln | --rv | --rfi <dir>\n", argv[0]);
+        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] | --klc | --kln | --rv | --rfi | --eob <dir>\n", argv[0]);
         return 2;
     }
     return SPH_Calculation(3, argv[1]);

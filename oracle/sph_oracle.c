@@ -69,6 +69,13 @@ struct or_ctx {
     double vib_ttime, vib_sttime, vib_edtime;
     int vib_nsteps;
     double (*vib_loads)[3];
+    /* Para_Rain (RP section) with rainini's period lp and velocity, setting_par's x_min / x_max of
+     * the water particles of etype 7 and center_cement's base (flr 1) */
+    int rain_set, rtype, nrain, nhor, cement_flag, lp;
+    double press_factor, drop_velocity, vrain, rx_min, rx_max, cement_base[3];
+    /* Para_EC (fleob 1): stages [start, end] and the step each fires at; nstage fired so far */
+    int ecount, nstage;
+    int (*ecp)[3];
 };
 
 /* ------------------------------------------------------------------ kernel function */
@@ -174,8 +181,68 @@ void or_destroy(or_ctx *c) {
     free(c->p); free(c->pc); free(c->ps); free(c->pf); free(c->vb); free(c->vel_bndy);
     free(c->vx_temp); free(c->rho_tmp); free(c->cstart); free(c->cend); free(c->nncell);
     free(c->s_cell); free(c->s_id); free(c->t_cell); free(c->t_id);
-    free(c->depp0); free(c->vib_loads);
+    free(c->depp0); free(c->vib_loads); free(c->ecp);
     free(c);
+}
+
+/* Para_Rain from the RP section (CPP_2-initialf.cpp:180-201, nhor 0 taken as 35), then what the
+ * reference derives before its loop from the initial state: x_min / x_max over the water particles
+ * of etype 7 (setting_par, :1307-1311), the period lp and velocity of rainini
+ * (CPP_15-1-rain.cpp:10-45; nrain / nhor is an integer division there) and the cement base, the
+ * centre of the type-7 particles' box (center_cement, :133-163). Call after the particles are set.
+ * Returns 6 for a type outside 1-7 (the reference's error), SPHG_E_ARG for lp < 1 where the
+ * reference would divide by zero. */
+int or_set_rain(or_ctx *c, int rtype, int nrain, int nhor, double press_factor, double drop_velocity) {
+    int i, dim = c->pro.ndim;
+    double temp, mx[3] = {-1000.0, -1000.0, -1000.0}, mn[3] = {1000.0, 1000.0, 1000.0};
+    if (rtype < 1 || rtype > 7) return 6;
+    c->rtype = rtype;
+    c->nrain = nrain;
+    c->nhor = nhor == 0 ? 35 : nhor;
+    c->press_factor = press_factor;
+    c->drop_velocity = drop_velocity;
+    c->cement_flag = 1;
+    c->rx_max = -10000.0;
+    c->rx_min = 10000.0;
+    c->vrain = 0.0;
+    c->lp = 0;
+    for (i = 0; i < c->n; i++)
+        if (c->p[i].type == 1 && c->p[i].etype == 7) {
+            c->rx_max = fmax(c->rx_max, c->p[i].xp[dim - 2]);
+            c->rx_min = fmin(c->rx_min, c->p[i].xp[dim - 2]);
+        }
+    if (rtype == 1 || rtype == 4) {
+        temp = c->nrain / c->nhor + 1.0;
+        c->lp = (int)((c->pro.loop - c->pro.inip) / temp);
+        c->vrain = rtype == 1 ? drop_velocity : 2.0;
+    } else if (rtype == 2 || rtype == 3) {
+        temp = fabs(c->rx_max - c->rx_min) / c->pro.dr + 1.0;
+        c->lp = (int)((c->pro.loop - c->pro.inip) / temp);
+    }
+    for (i = 0; i < c->n; i++)
+        if (c->p[i].type == 7) {
+            int k;
+            for (k = 0; k < 3; k++) {
+                mx[k] = fmax(mx[k], c->p[i].xp[k]);
+                mn[k] = fmin(mn[k], c->p[i].xp[k]);
+            }
+        }
+    for (i = 0; i < 3; i++) c->cement_base[i] = (mx[i] + mn[i]) * 0.5;
+    c->rain_set = 1;
+    if (rtype <= 4 && c->lp < 1) return SPHG_E_ARG;
+    return 0;
+}
+
+/* Para_EC: ecount stages, ecp[i] = {start, end, step} (1-based particle range; the reference's
+ * reader fills start and end only, CPP_2-initialf.cpp:146, and never the step). */
+int or_set_excavation(or_ctx *c, int ecount, const int *ecp) {
+    if (ecount < 1 || !ecp) return 8;  /* the reference's error for no stage */
+    free(c->ecp);
+    c->ecp = (int(*)[3])malloc(sizeof(int[3]) * ecount);
+    memcpy(c->ecp, ecp, sizeof(int[3]) * ecount);
+    c->ecount = ecount;
+    c->nstage = 0;
+    return 0;
 }
 
 /* initial_vib (CPP_2-initialf.cpp:293-322) and the VL section (:250-265): one period of ttime
@@ -2234,6 +2301,113 @@ static void or_vibration(or_ctx *c, int velocity) {
     }
 }
 
+/* ------------------------------------------------------------------ dynamic particle sets */
+/* rainpro (CPP_15-1-rain.cpp:48-98): the next reserved particles from the end of the array become
+ * water, one row at a time (while the next particle's height is within 0.1 dr of this one's
+ * before the move); the reference reads one particle past the array at the end of the loop, here
+ * the bound is tested first */
+static void or_rainpro(or_ctx *c) {
+    int k = c->n - c->nrain, ndim = c->pro.ndim;
+    double g = ndim == 2 ? c->pro.gy : c->pro.gz;
+    if (c->nrain <= 0) return;
+    do {
+        OPar *q = &c->p[k];
+        double temp_xp = q->xp[ndim - 1];
+        c->nrain = c->nrain - 1;
+        q->xp[ndim - 1] = 0.95 * q->xp[ndim - 1];
+        q->type = 1;
+        q->matype = 0;
+        q->etype = 7;
+        q->rhop = c->pf[k].porosity * c->pf[k].dens;
+        q->hl = c->pro.dr;
+        q->mass = ndim == 2 ? q->rhop * q->hl * q->hl : q->rhop * q->hl * q->hl * q->hl;
+        q->vxp[0] = q->vxp[1] = q->vxp[2] = 0.0;
+        q->vxp[ndim - 1] = c->vrain;
+        q->vx[0] = q->vx[1] = q->vx[2] = 0.0;
+        q->prep = q->rhop * c->pro.dr * (-g) * c->press_factor;
+        q->satu = 0.99;
+        k++;
+        if (k > c->n - 1) break;
+        if (!(fabs(c->p[k].xp[ndim - 1] - temp_xp) < 0.1 * c->pro.dr)) break;
+    } while (1);
+}
+
+/* cementpro (CPP_15-1-rain.cpp:101-131): as rainpro, radially from the cement base (vx and etype
+ * untouched), while the next particle's matype equals the call's cement flag; the flag advances
+ * after every call */
+static void or_cementpro(or_ctx *c) {
+    int k = c->n - c->nrain, ndim = c->pro.ndim, i, cf = c->cement_flag;
+    double g = ndim == 2 ? c->pro.gy : c->pro.gz;
+    if (c->nrain > 0) {
+        do {
+            OPar *q = &c->p[k];
+            double drp = 0.0;
+            c->nrain = c->nrain - 1;
+            for (i = 0; i < ndim; i++) drp = drp + (q->xp[i] - c->cement_base[i]) * (q->xp[i] - c->cement_base[i]);
+            drp = sqrt(drp);
+            q->type = 1;
+            q->matype = 0;
+            q->rhop = c->pf[k].porosity * c->pf[k].dens;
+            q->hl = c->pro.dr;
+            q->mass = ndim == 2 ? q->rhop * q->hl * q->hl : q->rhop * q->hl * q->hl * q->hl;
+            for (i = 0; i < 3; i++) q->vxp[i] = c->vrain * (q->xp[i] - c->cement_base[i]) / drp;
+            q->prep = q->rhop * c->pro.dr * (-g) * c->press_factor;
+            q->satu = 0.99;
+            k++;
+        } while (k < c->n - 1 && c->p[k].matype == cf);  /* the bound first: the reference reads p[n] */
+    }
+    c->cement_flag += 1;
+}
+
+/* waterdel_min / waterdel_max (CPP_15-2-partidel.cpp:52-130): water particles of etype 7 past
+ * x_min + 3 dr (or x_max - 3 dr) on the second-to-last axis return to the reserve */
+static void or_waterdel(or_ctx *c, int at_max) {
+    int i, ndim = c->pro.ndim;
+    double lim = at_max ? c->rx_max - 3.0 * c->pro.dr : c->rx_min + 3.0 * c->pro.dr;
+    for (i = 0; i < c->n; i++) {
+        OPar *q = &c->p[i];
+        double x;
+        if (!(q->type == 1 && q->etype == 7)) continue;
+        x = q->xp[ndim - 2];
+        if (at_max ? x >= lim : x <= lim) {
+            q->type = 7;
+            q->etype = 1;
+            q->vxp[0] = q->vxp[1] = q->vxp[2] = 0.0;
+            q->vx[0] = q->vx[1] = q->vx[2] = 0.0;
+            q->ax[0] = q->ax[1] = q->ax[2] = 0.0;
+            c->nrain = c->nrain + 1;
+        }
+    }
+}
+
+/* partidel (CPP_15-2-partidel.cpp:16-50): water particles 7 dr beyond the domain box in x (within
+ * dr) and above y = 0.12 leave the run */
+static void or_partidel(or_ctx *c) {
+    int i;
+    double dr = c->pro.dr;
+    for (i = 0; i < c->n; i++) {
+        OPar *q = &c->p[i];
+        double dx1 = fabs(q->xp[0] - c->pro.xmax - 7.0 * dr), dx2 = fabs(q->xp[0] - c->pro.xmin + 7.0 * dr);
+        double dy1 = q->xp[1] - 0.12;
+        if (q->type == 1 && (dx1 < dr || dx2 < dr) && dy1 > 0.0) {
+            q->type = 7;
+            q->vxp[0] = q->vxp[1] = q->vxp[2] = 0.0;
+            q->vx[0] = q->vx[1] = q->vx[2] = 0.0;
+            q->ax[0] = q->ax[1] = q->ax[2] = 0.0;
+        }
+    }
+}
+
+/* exca_or_back (CPP_16-EOB.cpp:13-74): type and etype of the stage's particles swap */
+static void or_eob(or_ctx *c) {
+    int i, start = c->ecp[c->nstage][0], end = c->ecp[c->nstage][1];
+    for (i = start - 1; i < end; i++) {
+        int t = c->p[i].type;
+        c->p[i].type = c->p[i].etype;
+        c->p[i].etype = t;
+    }
+}
+
 /* ------------------------------------------------------------------ velocity */
 /* veocity_update (CPP_11-xsph.cpp:11-142) */
 static void or_velocity(or_ctx *c) {
@@ -2984,8 +3158,12 @@ static void or_free_solid(or_ctx *c) {
 /* ------------------------------------------------------------------ one step */
 static int unsupported_flags(const or_ctx *c) {
     const sphg_problem *P = &c->pro;
-    if (P->flknl < 1 || P->flknl > 4 || P->flr || P->fleob || P->flv < 0 || P->flv > 2) return 1;
+    if (P->flknl < 1 || P->flknl > 4 || P->flr < 0 || P->flr > 1 || P->fleob < 0 || P->fleob > 1 || P->flv < 0 ||
+        P->flv > 2)
+        return 1;
     if (P->flv > 0 && !c->vib_loads) return 1;
+    if (P->flr == 1 && !c->rain_set) return 1;
+    if (P->fleob == 1 && !c->ecp) return 1;
     if (P->fli < 0 || P->fli > 3 || P->ss_type < 0 || P->ss_type > 3) return 1;
     if (P->flbndy != 1 && P->flbndy != 2 && P->flbndy != 3 && P->flbndy != 4 && P->flbndy != 5 && P->flbndy != 13 &&
         P->flbndy != 14)
@@ -3000,7 +3178,20 @@ int or_phase(or_ctx *c, int mask) {
     int e;
     if (unsupported_flags(c)) return SPHG_E_UNSUPPORTED;
     if (mask & 1) {
+        int lr = c->l - c->pro.inip;
         c->t = c->t + c->pro.dt;
+        /* rain / cement production and water-level decrease, then excavation or backfill
+         * (CPP_1-SPHLoops.cpp:495-520) */
+        if (c->pro.flr == 1 && lr > 0 && c->lp > 0 && lr % c->lp == 0) {
+            if (c->rtype == 1) or_rainpro(c);
+            else if (c->rtype == 2) or_waterdel(c, 0);
+            else if (c->rtype == 3) or_waterdel(c, 1);
+            else if (c->rtype == 4) or_cementpro(c);
+        }
+        if (c->pro.fleob == 1 && c->nstage < c->ecount && c->l == c->ecp[c->nstage][2]) {
+            or_eob(c);
+            c->nstage += 1;
+        }
         if ((e = or_parttocell(c))) return e;
         if ((e = or_partisearching(c))) return e;
         if (c->pro.flknl > 2) or_kgc(c);
@@ -3063,6 +3254,7 @@ int or_phase(or_ctx *c, int mask) {
     if (mask & 64) {
         or_check_domain(c);
         or_update(c);
+        if (c->pro.flr == 1) or_partidel(c);
         c->l++;
     }
     return 0;

@@ -38,6 +38,10 @@ void or_set_step(or_ctx *c, int l);
 void or_set_time(or_ctx *c, double t);
 /* Para_Vibra and vibra.dat (flv 1, 2): loads[nsteps][3]. */
 int or_set_vibration(or_ctx *c, double ttime, int nsteps, double sttime, double edtime, const double *loads);
+/* Para_Rain (flr 1), after the particles are set. */
+int or_set_rain(or_ctx *c, int rtype, int nrain, int nhor, double press_factor, double drop_velocity);
+/* Para_EC (fleob 1): ecp[ecount][3] = start, end, step. */
+int or_set_excavation(or_ctx *c, int ecount, const int *ecp);
 /* Runs nsteps reference steps; returns 0 or the reference error code (3, 5). */
 int or_step(or_ctx *c, int nsteps);
 /* Phase groups of one step (bit 1 NNPS, 2 density+eos1, 4 strain+eos2, 8 stress, 16 accel+velocity,

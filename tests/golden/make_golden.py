@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 from helpers import run_ref, run_ref_kl, have_ref  # noqa: E402
 from sphgpy import (column_collapse_2d, soil_water_column_3d, model_zoo_2d, moving_box_2d,  # noqa: E402
                     art_stress_block, slope_2d, ssi_block, tran_box, drag_box, air_water_box,
-                    slope_pile_3d, vibration_box)
+                    slope_pile_3d, vibration_box, rain_box)
 from sphgpy.capi import STATE_FIELDS, PARTICLE_FIELDS  # noqa: E402
 
 KEEP = sorted(set(STATE_FIELDS) | set(PARTICLE_FIELDS) | {"type", "matype", "permtype", "etype", "tssig", "tsqc"})
@@ -132,6 +132,12 @@ def decks():
         "vib_2d_vel": (vibration_box(2, 2), 6, (1, 3, 6)),
         "vib_3d_acc": (vibration_box(3, 1, nx=5, ny=3, nz=3), 6, (1, 3, 6)),
         "vib_3d_vel": (vibration_box(3, 2, nx=5, ny=3, nz=3), 6, (1, 3, 6)),
+        "rain_2d_r1": (rain_box(2, 1), 6, (1, 2, 4, 6)),
+        "rain_2d_r2": (rain_box(2, 2, steps=9), 9, (1, 3, 9)),
+        "rain_2d_r3": (rain_box(2, 3, steps=9), 9, (1, 3, 9)),
+        "rain_2d_r4": (rain_box(2, 4), 6, (1, 2, 4, 6)),
+        "rain_3d_r1": (rain_box(3, 1, nx=5, ny=3, nz=3), 6, (1, 2, 4, 6)),
+        "rain_3d_r4": (rain_box(3, 4, nx=5, ny=3, nz=3), 6, (1, 2, 4, 6)),
     }
 
 

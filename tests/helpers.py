@@ -49,6 +49,10 @@ def oracle_lib() -> _Lib:
         L.or_set_vibration.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double,
                                        C.POINTER(C.c_double)]
         L.or_set_vibration.restype = C.c_int
+        L.or_set_rain.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.or_set_rain.restype = C.c_int
+        L.or_set_excavation.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.or_set_excavation.restype = C.c_int
         L.or_phase.argtypes = [C.c_void_p, C.c_int]
         L.or_phase.restype = C.c_int
         L.or_pair_count.argtypes = [C.c_void_p]
@@ -78,6 +82,13 @@ class OracleSim(Sim):
         for k, v in state.items():
             if k.startswith("ps_") or k.startswith("pf_") or k in ("vx_water", "vx_soil", "vx_air", "vx_struct", "vel_bndy"):
                 self.set(k, v)
+        if deck.flr == 1:
+            rc = lib.lib.or_set_rain(ctx, *deck.rain)
+            if rc:
+                raise RuntimeError(f"or_set_rain: error {rc}")
+        if deck.fleob == 1:
+            ecp = np.ascontiguousarray(np.array(deck.ecp, dtype=np.int32).reshape(-1, 3))
+            lib.lib.or_set_excavation(ctx, ecp.shape[0], ecp.ctypes.data_as(C.POINTER(C.c_int)))
         lib.lib.or_upload_done(ctx)
         lib.lib.or_set_mode(ctx, mode, nthreads)
         lib.lib.or_set_step(ctx, step)

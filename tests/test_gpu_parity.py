@@ -646,6 +646,78 @@ def test_vibration_needs_its_table():
     assert lib.lib.sphg_upload_done(ctx, d.n) == 100  # SPHG_E_ARG: no vibration table
 
 
+RAIN = ["rain_2d_r1", "rain_2d_r2", "rain_2d_r3", "rain_2d_r4", "rain_3d_r1", "rain_3d_r4"]
+
+
+@pytest.mark.parametrize("name", RAIN)
+def test_rain_and_water_level_vs_oracle(name):
+    # flr 1: rain rows / cement groups leave the reserve one period at a time, the water-level
+    # decrease sends water back to it; the particle set changes on the device as in the oracle
+    # (the water EOS calls pow on the device: 1e-9 field-max relative, particle sets exact)
+    deck, states = load_golden(name)
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    last = max(states)
+    for s in range(last):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare_tol(g, o, TRANSCENDENTAL_TOL) == {}, f"step {s + 1}"
+        for k in ("type", "etype", "matype"):
+            assert np.array_equal(g.get(k), np.asarray(o.get(k), np.int32)), (s + 1, k)
+    assert np.array_equal(g.get("type"), states[last]["type"].astype(np.int32))
+    for k in ("xp", "vx", "rho", "mass"):
+        form = nan_rel(o.get(k), states[last][k])
+        assert nan_rel(g.get(k), states[last][k]) <= form, k
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_excavation_and_backfill_vs_oracle(ndim):
+    # fleob 1: stage 0 excavates at step 2, stage 1 backfills at step 4 (one stage per step)
+    from sphgpy import eob_box
+    kw = dict(nx=6, ny=3, nz=4) if ndim == 3 else {}
+    deck = eob_box(ndim, **kw)
+    st = setting_par(deck)
+    g = make_gpu(deck, st)
+    o = OracleSim(deck, st, mode=JACOBI)
+    for s in range(6):
+        assert g.step(1) == 0
+        assert o.step(1) == 0
+        assert compare(g, o) == {}, f"step {s + 1}"
+        assert np.array_equal(g.get("type"), np.asarray(o.get("type"), np.int32)), s + 1
+    (a1, b1, _), (a2, b2, _) = deck.ecp
+    t = g.get("type")
+    assert np.all(t[a1 - 1:b1] == 7) and np.all(t[a2 - 1:b2] == 2)
+
+
+def test_particle_deletion_beyond_the_domain():
+    # partidel: a water particle 7 dr past the domain box in x and above y = 0.12 leaves the run at
+    # the update (moved there between steps; the update phase alone, as the oracle's)
+    deck, states = load_golden("rain_2d_r2")
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    assert g.step(1) == 0 and o.step(1) == 0
+    bmin, bmax, _ = deck.domain()
+    water = np.nonzero(g.get("type") == 1)[0]  # the water-level decrease took some at step 1
+    assert np.array_equal(water, np.nonzero(np.asarray(o.get("type")) == 1)[0])
+    xp = g.get("xp")
+    xp[water[0]] = (bmax[0] + 7.3 * deck.dr, 0.2, 0.0)
+    xp[water[1]] = (bmin[0] - 7.2 * deck.dr, 0.2, 0.0)
+    xp[water[2]] = (bmax[0] + 7.3 * deck.dr, 0.1, 0.0)  # below y = 0.12: stays
+    g.set("xp", xp)
+    o.set("xp", xp)
+    assert g.call("update") == 0
+    assert o.phase(64) == 0
+    t = g.get("type")
+    assert t[water[0]] == 7 and t[water[1]] == 7 and t[water[2]] == 1
+    assert np.array_equal(t, np.asarray(o.get("type"), np.int32))
+    # the deleted particles stop (the rest of this lone update phase reads the velocities of a
+    # step that did not run on the device and is not compared)
+    gone = water[:2]
+    for sim in (g, o):
+        for k in ("vxp", "ax", "vx"):
+            assert np.all(sim.get(k)[gone] == 0.0), k
+
+
 @pytest.mark.parametrize("name", ["pile3d_amcc", "pile3d_sb"])
 def test_slope_pile_vs_oracle(name):
     # the 3D slope-with-pile flag set at fixture size: modified Cam-Clay soil (AMCC no-op tangent,

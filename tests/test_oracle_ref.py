@@ -5,6 +5,8 @@
 2. Where the reference link harness is built (oracle/_ref), against fresh reference runs on larger
    decks and more steps.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -38,7 +40,7 @@ GOLDEN = ["column2d_small", "soilwater3d_small", "model_zoo_2d", "moving_box_m1"
           "kgc_tran3d_k4", "ts_column2d", "ts_art3d", "ts_drag3d", "air_water_2d_a0", "air_water_2d_a1",
           "air_water_3d_a0", "air_water_3d_a1", "tm_2d_b1", "tm_3d_b1", "umcc_perm2", "zoo_perm3", "ssi_3d_t2",
           "pile3d_amcc", "pile3d_sb", "bingham_2d", "bingham_3d", "vib_2d_acc", "vib_2d_vel", "vib_3d_acc",
-          "vib_3d_vel"]
+          "vib_3d_vel", "rain_2d_r1", "rain_2d_r2", "rain_2d_r3", "rain_2d_r4", "rain_3d_r1", "rain_3d_r4"]
 
 
 @pytest.mark.parametrize("name", GOLDEN)
@@ -326,3 +328,54 @@ def test_vibration_fixtures_exercise_every_branch():
         assert o.step(6) == 0
         soil = deck.type == 2
         assert np.abs(o.get("ax")[soil] - states[6]["ax"][soil]).max() > 1.0
+
+
+def test_rain_fixtures_change_the_particle_set():
+    # rain and cement turn the reserve into water one row / group per period; the water-level
+    # decrease returns water to the reserve
+    for name, before, after in (("rain_2d_r1", 0, 16), ("rain_2d_r4", 0, 16), ("rain_3d_r1", 0, 30),
+                                ("rain_2d_r2", 24, 15), ("rain_2d_r3", 24, 15)):
+        deck, states = load_golden(name)
+        last = max(states)
+        assert int((states[0]["type"] == 1).sum()) == before, name
+        assert int((states[last]["type"] == 1).sum()) == after, name
+    deck, states = load_golden("rain_2d_r1")
+    assert int((states[2]["type"] == 1).sum()) == 8 and int((states[4]["type"] == 1).sum()) == 16
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference link harness not built")
+def test_excavation_swap_is_the_references():
+    # exca_or_back for one stage through the harness against the oracle's swap (or_eob)
+    import struct
+    import subprocess
+    import tempfile
+    from helpers import REF_HARNESS
+    from sphgpy import eob_box
+    d = eob_box()
+    for (a, b, _) in d.ecp:
+        wd = tempfile.mkdtemp(prefix="spheob_")
+        with open(os.path.join(wd, "eob_in.bin"), "wb") as f:
+            f.write(struct.pack("<iii", d.n, a, b))
+            f.write(np.ascontiguousarray(d.type, "<i4").tobytes() + np.ascontiguousarray(d.etype, "<i4").tobytes())
+        assert subprocess.run([REF_HARNESS, "--eob", wd], capture_output=True).returncode == 0
+        raw = np.frombuffer(open(os.path.join(wd, "eob_out.bin"), "rb").read(), "<i4")
+        ty, et = raw[:d.n], raw[d.n:]
+        want_t, want_e = d.type.copy(), d.etype.copy()
+        want_t[a - 1:b], want_e[a - 1:b] = d.etype[a - 1:b], d.type[a - 1:b]
+        assert np.array_equal(ty, want_t) and np.array_equal(et, want_e)
+
+
+def test_oracle_excavation_and_backfill_stages():
+    # the oracle fires stage s at its step, one stage per step, in order: the excavated part
+    # leaves (type 7) at step 2, the backfill turns into soil at step 4
+    from sphgpy import eob_box
+    from sphgpy.capi import setting_par
+    d = eob_box()
+    o = OracleSim(d, setting_par(d), mode=GS)
+    (a1, b1, _), (a2, b2, _) = d.ecp
+    seen = []
+    for _ in range(6):
+        assert o.step(1) == 0
+        t = o.get_i("type") if hasattr(o, "get_i") else o.get("type")
+        seen.append((int((np.asarray(t)[a1 - 1:b1] == 7).all()), int((np.asarray(t)[a2 - 1:b2] == 2).all())))
+    assert seen == [(0, 0), (1, 0), (1, 0), (1, 1), (1, 1), (1, 1)]
