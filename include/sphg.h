@@ -518,6 +518,14 @@ int sphg_set_rain(sphg_ctx *ctx, const sphg_rain *rain);
  * the first two per stage and never assigns the step (CPP_2-initialf.cpp:146), so there the step is
  * indeterminate; here the caller gives it. Set before sphg_upload_done. */
 int sphg_set_excavation(sphg_ctx *ctx, int ecount, const int *ecp);
+/* The RP section of <dir>/input.dat (CPP_2-initialf.cpp:180-201) after Para_Rain's defaults; nhor 0
+ * becomes 35. Host only. */
+int sphg_read_rain(const char *dir, sphg_rain *rain);
+/* The EC section of <dir>/input.dat (CPP_2-initialf.cpp:125-157): ecount stages of first and last
+ * particle into ecp[3 * s], ecp[3 * s + 1]; ecp[3 * s + 2] (the step) is set to 0, at which no stage
+ * fires, since the reference never assigns it. ecp may be null (ecount only) or holds 3 * cap ints
+ * (SPHG_E_CAPACITY beyond). Returns 8 (the reference's error) when no stage is given. Host only. */
+int sphg_read_excavation(const char *dir, int *ecount, int *ecp, int cap);
 
 /* ---- Correlated random variables (flspv 5, CPP_17-5-Nonstationary-Correlated-RF.cpp:738-863) -- */
 

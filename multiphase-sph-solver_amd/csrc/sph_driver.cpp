@@ -75,6 +75,28 @@ extern "C" int sphg_solve_project(const char *dir, int precision, int *last_step
             return rv;
         }
     }
+    if (pro.flr == 1) {
+        // the RP section (CPP_2-initialf.cpp:180-201)
+        sphg_rain r{};
+        int rv = sphg_read_rain(dir, &r);
+        if (rv == SPHG_OK) rv = sphg_set_rain(ctx, &r);
+        if (rv) {
+            sphg_destroy(ctx);
+            return rv;
+        }
+    }
+    if (pro.fleob == 1) {
+        // the EC section (CPP_2-initialf.cpp:125-157); no stage step is given there (see the reader)
+        int ec = 0;
+        int rv = sphg_read_excavation(dir, &ec, nullptr, 0);
+        std::vector<int> ecp(3 * (size_t)(ec > 0 ? ec : 1));
+        if (rv == SPHG_OK) rv = sphg_read_excavation(dir, &ec, ecp.data(), ec);
+        if (rv == SPHG_OK) rv = sphg_set_excavation(ctx, ec, ecp.data());
+        if (rv) {
+            sphg_destroy(ctx);
+            return rv;
+        }
+    }
     int rc = upload(ctx, n,
                     {{"xp", {xp.data(), 3}}, {"vxp", {vxp.data(), 3}}, {"rho", {rho.data(), 1}},
                      {"rhop", {rhop.data(), 1}}, {"mass", {mass.data(), 1}}, {"hl", {hl.data(), 1}},

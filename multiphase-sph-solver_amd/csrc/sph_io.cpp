@@ -332,6 +332,43 @@ extern "C" int sphg_read_parameters(const char *dir, int n, const int *type, dou
     return SPHG_OK;
 }
 
+// The RP section of <dir>/input.dat (CPP_2-initialf.cpp:180-201): rtype, nrain, nhor,
+// press_factor, drop_velocity after Para_Rain's defaults (Class_Problem.cpp:74-84); nhor 0 is 35.
+extern "C" int sphg_read_rain(const char *dir, sphg_rain *r) {
+    if (!dir || !r) return SPHG_E_ARG;
+    FILE *f = fopen((std::string(dir) + "/input.dat").c_str(), "r");
+    if (!f) return SPHG_E_ARG;
+    *r = sphg_rain{};
+    if (find_section(f, "RP", false))
+        read_vals(f, {I(r->rtype), I(r->nrain), I(r->nhor), D(r->press_factor), D(r->drop_velocity)});
+    fclose(f);
+    if (r->nhor == 0) r->nhor = 35;
+    return SPHG_OK;
+}
+
+// The EC section of <dir>/input.dat (CPP_2-initialf.cpp:125-157): ecount, then the first and last
+// particle of each stage. The reference never assigns a stage's step (its table is uninitialised
+// there); this reader gives step 0, at which no stage fires. ecp holds 3 * cap ints (may be null:
+// ecount only). Returns 8 (the reference's error) for no stage.
+extern "C" int sphg_read_excavation(const char *dir, int *ecount, int *ecp, int cap) {
+    if (!dir || !ecount) return SPHG_E_ARG;
+    FILE *f = fopen((std::string(dir) + "/input.dat").c_str(), "r");
+    if (!f) return SPHG_E_ARG;
+    *ecount = 0;
+    int rc = SPHG_OK;
+    if (find_section(f, "EC", false) && read_vals(f, {I(*ecount)}) && *ecount >= 1) {
+        if (ecp && cap < *ecount) rc = SPHG_E_CAPACITY;
+        for (int s = 0; ecp && rc == SPHG_OK && s < *ecount; s++) {
+            ecp[3 * s] = ecp[3 * s + 1] = ecp[3 * s + 2] = 0;
+            read_vals(f, {I(ecp[3 * s]), I(ecp[3 * s + 1])});
+        }
+    } else {
+        rc = 8;
+    }
+    fclose(f);
+    return rc;
+}
+
 // The VL section of <dir>/input.dat (CPP_2-initialf.cpp:250-265: ttime, nsteps, sttime, edtime)
 // and the loads of <dir>/vibra.dat (initial_vib, :293-322: nsteps rows of three values). loads may
 // be null (then only the parameters are read) or must hold 3 * cap values.

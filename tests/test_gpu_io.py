@@ -153,13 +153,13 @@ def test_project_runner_writes_the_reference_outputs(tmp_path):
         assert open(odb_path(str(api), l), "rb").read() == open(odb_path(str(proj), l), "rb").read()
 
 
-def test_project_runner_with_vibration_loads(tmp_path):
-    # flv 2 project: the runner reads the VL section and vibra.dat, and its tables at the output
-    # steps are those of the oracle with the same table (walls driven, free-field boundaries,
-    # soil_damping_vibration)
+@pytest.mark.parametrize("name", ["vib_2d_vel", "rain_2d_r1"])
+def test_project_runner_with_vibration_and_rain(name, tmp_path):
+    # flv 2 / flr 1 projects: the runner reads the VL section and vibra.dat, or the RP section, and
+    # its tables at the output steps are those of the oracle with the same parameters
     import subprocess
     from sphgpy.capi import SOLVE_PATH, odb_path
-    deck, states = load_golden("vib_2d_vel")
+    deck, states = load_golden(name)
     deck = deck.copy()
     deck.loop, deck.fop = 6, 3
     proj = tmp_path / "proj"
@@ -173,7 +173,8 @@ def test_project_runner_with_vibration_loads(tmp_path):
         assert o.step(l - done) == 0
         done = l
         st = {k: o.get(k) for k in ODB_FIELDS}
-        st["type"], st["matype"] = deck.type, deck.matype
+        # the particle set changes with rain: the table lists the types of the step
+        st["type"], st["matype"] = np.asarray(o.get("type")), np.asarray(o.get("matype"))
         d = tmp_path / f"o{l}"
         d.mkdir()
         want = open(write_odb_fields(str(d), l, odb_time(deck.dt, l), st), "rb").read()

@@ -215,3 +215,26 @@ def test_read_vibration_section_and_table(tmp_path):
     (tmp_path / "vibra.dat").unlink()
     assert L.sphg_read_vibration(str(tmp_path).encode(), C.byref(tt), C.byref(ns), C.byref(st), C.byref(et),
                                  loads.ctypes.data_as(D), ns.value) == 100
+
+
+def test_read_rain_and_excavation_sections(tmp_path):
+    import ctypes as C
+    from sphgpy import rain_box, eob_box
+    from sphgpy.capi import Rain
+    L = product_lib().lib
+    d = rain_box(2, 4)
+    d.write_input_dat(str(tmp_path / "input.dat"))
+    r = Rain()
+    assert L.sphg_read_rain(str(tmp_path).encode(), C.byref(r)) == 0
+    assert (r.rtype, r.nrain, r.nhor, r.press_factor, r.drop_velocity) == d.rain
+    e = eob_box()
+    (tmp_path / "e").mkdir()
+    e.write_input_dat(str(tmp_path / "e" / "input.dat"))
+    n = C.c_int()
+    assert L.sphg_read_excavation(str(tmp_path / "e").encode(), C.byref(n), None, 0) == 0
+    ecp = np.full((n.value, 3), -1, np.int32)
+    assert L.sphg_read_excavation(str(tmp_path / "e").encode(), C.byref(n),
+                                  ecp.ctypes.data_as(C.POINTER(C.c_int)), n.value) == 0
+    # first and last particle as written; the step is not in the file (the reference never sets it)
+    assert ecp.tolist() == [[a, b, 0] for a, b, _ in e.ecp]
+    assert L.sphg_read_excavation(str(tmp_path).encode(), C.byref(n), None, 0) == 8  # no EC section
