@@ -397,6 +397,61 @@ int eob_mode(const char *dir) {
     return 0;
 }
 
+// Particle-deletion mode: the reference's partidel (CPP_15-2-partidel.cpp:16-50) once.
+// del_in.bin: int32 n; double dr, xmin, xmax; double xp[n][3], vxp[n][3], vx[n][3], ax[n][3];
+//             int32 type[n] -> del_out.bin: int32 type[n]; double vxp[n][3], vx[n][3], ax[n][3]
+int del_mode(const char *dir) {
+    std::string in = std::string(dir) + "/del_in.bin", out = std::string(dir) + "/del_out.bin";
+    FILE *f = fopen(in.c_str(), "rb");
+    if (!f) return 2;
+    int n = 0;
+    double g[3];
+    bool ok = fread(&n, 4, 1, f) == 1 && n > 0 && fread(g, 8, 3, f) == 3;
+    std::vector<double> xp(ok ? 3 * (size_t)n : 0), vxp(xp.size()), vx(xp.size()), ax(xp.size());
+    std::vector<int> ty(ok ? n : 0);
+    ok = ok && fread(xp.data(), 8, xp.size(), f) == xp.size() && fread(vxp.data(), 8, xp.size(), f) == xp.size() &&
+         fread(vx.data(), 8, xp.size(), f) == xp.size() && fread(ax.data(), 8, xp.size(), f) == xp.size() &&
+         fread(ty.data(), 4, n, f) == (size_t)n;
+    fclose(f);
+    if (!ok) return 2;
+    Particle *par = new Particle[n];
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            par[i].xp[k] = xp[3 * i + k];
+            par[i].vxp[k] = vxp[3 * i + k];
+            par[i].vx[k] = vx[3 * i + k];
+            par[i].ax[k] = ax[3 * i + k];
+        }
+        par[i].type = ty[i];
+    }
+    Para_Pro pro;
+    pro.ntotal = n;
+    pro.dr = g[0];
+    Cell_Con cc;
+    cc.xmin = g[1];
+    cc.xmax = g[2];
+    Para_Rain rain;
+    clRain_Fun fn;
+    fn.partidel(par, &rain, &pro, cc, 1);
+    for (int i = 0; i < n; i++) {
+        ty[i] = par[i].type;
+        for (int k = 0; k < 3; k++) {
+            vxp[3 * i + k] = par[i].vxp[k];
+            vx[3 * i + k] = par[i].vx[k];
+            ax[3 * i + k] = par[i].ax[k];
+        }
+    }
+    delete[] par;
+    FILE *o = fopen(out.c_str(), "wb");
+    if (!o) return 2;
+    fwrite(ty.data(), 4, n, o);
+    fwrite(vxp.data(), 8, vxp.size(), o);
+    fwrite(vx.data(), 8, vx.size(), o);
+    fwrite(ax.data(), 8, ax.size(), o);
+    fclose(o);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--kl") == 0) return kl_mode(argv[2]);
     if (argc >= 3 && strcmp(argv[1], "--chol") == 0) return chol_mode(argv[2]);
@@ -407,8 +462,9 @@ int main(int argc, char **argv) {
     if (argc >= 3 && strcmp(argv[1], "--rv") == 0) return rf_mode(argv[2], 4);
     if (argc >= 3 && strcmp(argv[1], "--rfi") == 0) return rf_mode(argv[2], 5);
     if (argc >= 3 && strcmp(argv[1], "--eob") == 0) return eob_mode(argv[2]);
+    if (argc >= 3 && strcmp(argv[1], "--del") == 0) return del_mode(argv[2]);
     if (argc < 2) {
-        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] | --klc | --kln | --rv | --rfi | --eob <dir>\n", argv[0]);
+        fprintf(stderr, "usage: %s <project-dir-with-input.dat> | --kl <dir> | --chol <dir> | --rf[c] | --klc | --kln | --rv | --rfi | --eob | --del <dir>\n", argv[0]);
         return 2;
     }
     return SPH_Calculation(3, argv[1]);

@@ -2398,6 +2398,10 @@ static void or_partidel(or_ctx *c) {
     }
 }
 
+/* partidel on its own, for the checker's comparison with the reference's (the step calls it after
+ * every update when flr is 1) */
+void or_partidel_now(or_ctx *c) { or_partidel(c); }
+
 /* exca_or_back (CPP_16-EOB.cpp:13-74): type and etype of the stage's particles swap */
 static void or_eob(or_ctx *c) {
     int i, start = c->ecp[c->nstage][0], end = c->ecp[c->nstage][1];

@@ -42,6 +42,8 @@ int or_set_vibration(or_ctx *c, double ttime, int nsteps, double sttime, double 
 int or_set_rain(or_ctx *c, int rtype, int nrain, int nhor, double press_factor, double drop_velocity);
 /* Para_EC (fleob 1): ecp[ecount][3] = start, end, step. */
 int or_set_excavation(or_ctx *c, int ecount, const int *ecp);
+/* partidel (CPP_15-2-partidel.cpp:16-50) once, outside the step. */
+void or_partidel_now(or_ctx *c);
 /* Runs nsteps reference steps; returns 0 or the reference error code (3, 5). */
 int or_step(or_ctx *c, int nsteps);
 /* Phase groups of one step (bit 1 NNPS, 2 density+eos1, 4 strain+eos2, 8 stress, 16 accel+velocity,

@@ -379,3 +379,54 @@ def test_oracle_excavation_and_backfill_stages():
         t = o.get_i("type") if hasattr(o, "get_i") else o.get("type")
         seen.append((int((np.asarray(t)[a1 - 1:b1] == 7).all()), int((np.asarray(t)[a2 - 1:b2] == 2).all())))
     assert seen == [(0, 0), (1, 0), (1, 0), (1, 1), (1, 1), (1, 1)]
+
+
+def test_oracle_particle_deletion_is_the_references():
+    # partidel never fires in the fixture runs (no particle reaches 7 dr past the box); its rule is
+    # pinned on a particle set through the harness: the water beyond +-7 dr of the box in x and above
+    # y = 0.12 stops and leaves, other particles stay
+    import ctypes as C
+    import struct
+    import subprocess
+    import tempfile
+    from helpers import REF_HARNESS, oracle_lib
+    if not have_ref():
+        pytest.skip("reference link harness not built")
+    deck, states = load_golden("rain_2d_r2")
+    st = {k: v.copy() for k, v in states[0].items()}
+    bmin, bmax, _ = deck.domain()
+    dr = deck.dr
+    rng = np.random.default_rng(3)
+    n = deck.n
+    xs = np.concatenate([bmax[0] + 7.0 * dr + rng.uniform(-1.5, 1.5, n // 2) * dr,
+                         bmin[0] - 7.0 * dr + rng.uniform(-1.5, 1.5, n - n // 2) * dr])
+    st["xp"][:, 0] = xs
+    st["xp"][:, 1] = rng.uniform(0.10, 0.14, n)
+    st["vxp"] = rng.uniform(-1, 1, (n, 3))
+    st["vx"] = rng.uniform(-1, 1, (n, 3))
+    st["ax"] = rng.uniform(-1, 1, (n, 3))
+    ty = np.where(rng.uniform(0, 1, n) < 0.7, 1, 2).astype(np.int32)
+    st["type"] = ty
+    p = problem_of(deck)
+    wd = tempfile.mkdtemp(prefix="sphdel_")
+    with open(os.path.join(wd, "del_in.bin"), "wb") as f:
+        f.write(struct.pack("<i", n) + struct.pack("<3d", dr, p.xmin, p.xmax))
+        for k in ("xp", "vxp", "vx", "ax"):
+            f.write(np.ascontiguousarray(st[k], "<f8").tobytes())
+        f.write(ty.astype("<i4").tobytes())
+    assert subprocess.run([REF_HARNESS, "--del", wd], capture_output=True).returncode == 0
+    raw = open(os.path.join(wd, "del_out.bin"), "rb").read()
+    rty = np.frombuffer(raw, "<i4", n)
+    rest = np.frombuffer(raw, "<f8", 9 * n, 4 * n).reshape(3, n, 3)
+    o = OracleSim(deck, st, mode=GS)
+    oracle_lib().lib.or_partidel_now.argtypes = [C.c_void_p]
+    oracle_lib().lib.or_partidel_now(o.ctx)
+    assert np.array_equal(np.asarray(o.get("type")), rty)
+    for idx, k in enumerate(("vxp", "vx", "ax")):
+        assert bitequal(o.get(k), rest[idx]), k
+    assert 0 < int((rty == 7).sum()) < int((ty == 1).sum())  # some water left, some stayed
+
+
+def problem_of(deck):
+    from sphgpy.capi import problem_from_deck
+    return problem_from_deck(deck)
