@@ -269,6 +269,19 @@ int sphg_set_step(sphg_ctx *ctx, int l);
  * vibration loads read it. Zero after sphg_create; checkpoints keep it. */
 int sphg_set_time(sphg_ctx *ctx, double t);
 int sphg_get_time(sphg_ctx *ctx, double *t);
+/* Time step (timestep, CPP_12-2-dt.cpp:13-20: the constant dt = 0.25 dr / 600 that setting_par
+ * stores in Para_Pro::dt). SPHG_DT_CONSTANT (default): every step uses sphg_problem.dt, as the
+ * reference does. SPHG_DT_CFL: before every step the device computes
+ *   dt = min(c_cour dr / (c_wave + max_i |v_i|), 0.25 sqrt(dr / max_i |a_i|), dt_max)
+ * in FP64 as a min-reduction over the particles (v, a: the velocity and acceleration of the last
+ * update; type-7 particles left out), which is the reference's constant at rest with c_cour 0.25
+ * and c_wave 600. The clock Para_Pro::t advances by each step's dt; sphg_run then takes one step
+ * at a time. SPHG_E_ARG for another mode or non-positive coefficients. sphg_get_dt returns the dt
+ * of the last step (before any step: sphg_problem.dt). */
+#define SPHG_DT_CONSTANT 0
+#define SPHG_DT_CFL 1
+int sphg_set_dt_mode(sphg_ctx *ctx, int mode, double c_cour, double c_wave, double dt_max);
+int sphg_get_dt(sphg_ctx *ctx, double *dt);
 /* Vibration loads (flv 1, 2; Para_Vibra, Class_Problem.h:133-149, read by initial_pro's VL section
  * and initial_vib from vibra.dat, CPP_2-initialf.cpp:250-265, :293-322): one period of ttime in
  * nsteps loads loads[nsteps][3], applied while sttime <= t <= edtime. With flv 1 they are the

@@ -242,6 +242,8 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
+    plan.push_back({(void **)&S.dtp, 8});
+    plan.push_back({(void **)&S.dt_red, 16});
     plan.push_back({(void **)&S.tnow, 8});
     I(S.dyn, 4);  // rain reserve, cement flag, excavation stage (dynamic particle sets)
     S.ecp = nullptr;
@@ -950,6 +952,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     int zero = 0;
     HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->S.step, &c->step, 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->S.dtp, &pro.dt, 8, hipMemcpyHostToDevice));
     if (c->S.mra) {  // mixed form: the packed records from the uploaded state
         hipLaunchKernelGGL(k_mrec_init, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->S, c->P, c->cur);
         HIPCHK(hipGetLastError());
@@ -1020,7 +1023,11 @@ template <int NDIM, class M>
 static void enqueue_step(sphg_ctx *c, int phase_mask) {
     const int cur = c->cur, n = c->n;
     if (phase_mask & 1) {
-        if (c->P.flv > 0) LAUNCH("time", k_time_step, 1, c->S, c->P);
+        if (c->P.dt_mode == SPHG_DT_CFL) {
+            LAUNCH("dt_reduce", k_dt_reduce<NDIM>, grid_for(n), c->S, c->P);
+            LAUNCH("dt_set", k_dt_set, 1, c->S, c->P);
+        }
+        if (c->P.flv > 0 || c->P.dt_mode == SPHG_DT_CFL) LAUNCH("time", k_time_step, 1, c->S, c->P);
         if (c->P.flr == 1) {
             LAUNCH("rain", k_rain<NDIM>, 1, c->S, c->P, cur);
             LAUNCH("waterdel", k_waterdel<NDIM>, grid_for(n), c->S, c->P, cur);
@@ -1230,10 +1237,67 @@ extern "C" int sphg_step(sphg_ctx *c, int nsteps) {
     return read_err(c);
 }
 
+// The time loop with the CFL time step: dt is only known on the device once a step has started,
+// so the loop takes one step at a time and reads the device clock Para_Pro::t after each (the
+// output steps and the t < tt test are those of sphg_run).
+static int run_cfl(sphg_ctx *c, int loop, int fop, double tt, double *t, sphg_output_fn out, void *user) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->S.tnow, t, 8, hipMemcpyHostToDevice));
+    int l = c->step;
+    while (l <= loop) {
+        if (!(*t < tt)) {
+            if (out) {
+                int r = out(user, l, *t);
+                if (r) return r;
+            }
+            break;
+        }
+        int rc = sphg_step(c, 1);
+        HIPCHK(hipMemcpy(t, c->S.tnow, 8, hipMemcpyDeviceToHost));
+        if (rc) return rc;
+        const int last = l++;
+        if (out && (last % fop == 0 || last == 1)) {
+            int r = out(user, last, *t);
+            if (r) return r;
+        }
+    }
+    return SPHG_OK;
+}
+
+extern "C" int sphg_set_dt_mode(sphg_ctx *c, int mode, double c_cour, double c_wave, double dt_max) {
+    if (!c) return SPHG_E_ARG;
+    if (mode == SPHG_DT_CONSTANT) {
+        if (c->P.dt_mode != SPHG_DT_CONSTANT) {
+            drop_graphs(c);
+            c->P.dt_mode = SPHG_DT_CONSTANT;
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->S.dtp, &c->pro.dt, 8, hipMemcpyHostToDevice));
+        return SPHG_OK;
+    }
+    if (mode != SPHG_DT_CFL || !(c_cour > 0.0) || !(c_wave > 0.0) || !(dt_max > 0.0) || !std::isfinite(c_cour) ||
+        !std::isfinite(c_wave))
+        return SPHG_E_ARG;
+    drop_graphs(c);
+    c->P.dt_mode = SPHG_DT_CFL;
+    c->P.dt_cour = c_cour;
+    c->P.dt_cwave = c_wave;
+    c->P.dt_max = dt_max;
+    return SPHG_OK;
+}
+
+extern "C" int sphg_get_dt(sphg_ctx *c, double *dt) {
+    if (!c || !dt) return SPHG_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(dt, c->S.dtp, 8, hipMemcpyDeviceToHost));
+    return SPHG_OK;
+}
+
 // Serving: steps are enqueued on the context's stream without waiting, so several contexts'
 // work overlaps on the device; sphg_poll reports whether the stream has drained.
 extern "C" int sphg_run(sphg_ctx *c, int loop, int fop, double tt, double *t, sphg_output_fn out, void *user) {
     if (!c || !c->uploaded || !t || fop <= 0) return SPHG_E_ARG;
+    if (c->P.dt_mode == SPHG_DT_CFL) return run_cfl(c, loop, fop, tt, t, out, user);
     const double dt = c->P.dt;
     int l = c->step;
     // the device clock the vibration loads read starts from the caller's t and then advances

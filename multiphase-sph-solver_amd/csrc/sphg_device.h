@@ -76,6 +76,10 @@ struct Params {
     int flr, fleob, rtype, rain_lp, ecount;
     double press_factor, vrain, rx_min, rx_max, cement_base[3];
     double dom_xmin, dom_xmax;                 // Cell_Con xmin / xmax (partidel)
+    // time step: 0 constant (Para_Pro::dt, the reference's timestep()), 1 CFL min-reduction:
+    // dt = min(dt_cour dr / (dt_cwave + max|v|), 0.25 sqrt(dr / max|a|), dt_max)
+    int dt_mode;
+    double dt_cour, dt_cwave, dt_max;
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -141,6 +145,8 @@ struct State {
     // dynamic particle sets: dyn = {nrain, cement_flag, nstage}; ecp[ecount][3] = start, end, step
     int *dyn, *ecp;
     int *err, *step;      // sticky error word; current step number l
+    double *dtp;          // time step of the current step (Para_Pro::dt; the CFL mode rewrites it)
+    unsigned long long *dt_red;  // CFL mode: bit patterns of max |v|^2 and max |a|^2 over particles
     const Grid *grids;    // one per realisation
     const int *rid;       // realisation of each particle
     const SoilSlot *soil;

@@ -671,7 +671,7 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
     }
     if (ti == 0) {  // no continuity sum, but the reference still assigns rho = rhop + drhot*dt (:160-163)
         double drhot = 0.0;
-        S.rho_mid[i] = S.rhop[i] + drhot * P.dt;
+        S.rho_mid[i] = S.rhop[i] + drhot * *S.dtp;
         return;
     }
     double xi[3];
@@ -751,8 +751,9 @@ __global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
         for (int k = 0; k < NDIM; k++) drhot += up[k];
     }
     double rhop = S.rhop[i], r;
-    if (ti == 1 && fabs(por) > 0.0001) r = rhop + drhot * P.dt + rhop * (por - pcop) / (1.0 - pcop);
-    else r = rhop + drhot * P.dt;
+    const double dt = *S.dtp;
+    if (ti == 1 && fabs(por) > 0.0001) r = rhop + drhot * dt + rhop * (por - pcop) / (1.0 - pcop);
+    else r = rhop + drhot * dt;
     S.rho_mid[i] = r;
 }
 
@@ -1474,7 +1475,7 @@ template <int kVar>
 __device__ __forceinline__ void constitutive(const State &S, const Params &P, int i, int ti, int l, double sig[6],
                                              const double deps[6]) {
     constexpr bool kFull = (kVar & 1) != 0, kTs = (kVar & 2) != 0;
-    const double dt = P.dt;
+    const double dt = *S.dtp;
     // weps[a][b] from the three stored components (antisymmetric, zero diagonal)
     const double w01 = S.weps[0][i], w02 = S.weps[1][i], w12 = S.weps[2][i];
     // the reference forms weps[b][a] = 0.5*(divx[b][a]-divx[a][b]) separately; that equals -weps[a][b]
@@ -1993,7 +1994,7 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
 template <int NDIM>
 __device__ __forceinline__ void accel_damp(const State &S, const Params &P, int i, double ax[3], const double vi[3],
                                            const double axp[3], bool damp_on) {
-    const double dt = P.dt;
+    const double dt = *S.dtp;
     if (damp_on) {
         // soil_damping_geostress clamp (CPP_9-4-derivation_stress.cpp:874-895)
         double damp[3];
@@ -2012,7 +2013,7 @@ __device__ __forceinline__ void accel_damp(const State &S, const Params &P, int 
 template <int NDIM>
 __device__ __forceinline__ void accel_vel(const State &S, const Params &P, int i, int ti, int cur, double ax[3],
                                           const double vi[3], const double axp[3]) {
-    const double dt = P.dt;
+    const double dt = *S.dtp;
     // veocity_update (CPP_11-xsph.cpp:11-142)
     if (P.if_move == 1) {
         if (ti == 0) {
@@ -2566,7 +2567,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         #pragma unroll
         for (int k = 0; k < 3; k++) ax[k] += S.cpl[k][i];
     }
-    const double dt = P.dt;
+    const double dt = *S.dtp;
     const double axp[3] = {S.axp[0][i], S.axp[1][i], S.axp[2][i]};
     if (P.fli >= 1 && ti != 0 && ti != 7) {
         // velocity-reversal clamp of interact1-3 (CPP_10-interact1.cpp:93-108, :249-264, :404-419)
@@ -2604,7 +2605,50 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 // ------------------------------------------------------------------------------------------
 // Para_Pro::t advances by dt at the start of every step (CPP_1-SPHLoops.cpp:481-483)
 __global__ void k_time_step(State S, Params P) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *S.tnow = *S.tnow + P.dt;
+    if (threadIdx.x == 0 && blockIdx.x == 0) *S.tnow = *S.tnow + *S.dtp;
+}
+
+// Time step of the CFL mode (dt_mode 1): an FP64 min-reduction of the step restriction over the
+// particles, as max |v|^2 and max |a|^2 of the velocity and acceleration the last update left
+// (vxp, axp). Non-negative doubles order like their bit patterns, so each maximum is an integer
+// atomicMax of one value per wavefront. The reference's timestep() (CPP_12-2-dt.cpp:13-20) is the
+// constant dt = 0.25 dr / 600, which is this formula at rest with dt_cour 0.25, dt_cwave 600.
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_dt_reduce(State S, Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v2 = 0.0, a2 = 0.0;
+    if (i < P.n && S.type[i] != 7) {
+#pragma unroll
+        for (int k = 0; k < NDIM; k++) {
+            const double v = S.vxp[k][i], a = S.axp[k][i];
+            v2 = v2 + v * v;
+            a2 = a2 + a * a;
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        v2 = fmax(v2, __shfl_xor(v2, d));
+        a2 = fmax(a2, __shfl_xor(a2, d));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&S.dt_red[0], (unsigned long long)__double_as_longlong(v2));
+        atomicMax(&S.dt_red[1], (unsigned long long)__double_as_longlong(a2));
+    }
+}
+
+__global__ void k_dt_set(State S, Params P) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double v2 = __longlong_as_double((long long)S.dt_red[0]);
+    const double a2 = __longlong_as_double((long long)S.dt_red[1]);
+    double dt = P.dt_cour * P.dr / (P.dt_cwave + sqrt(v2));
+    if (a2 > 0.0) {
+        const double df = 0.25 * sqrt(P.dr / sqrt(a2));
+        if (df < dt) dt = df;
+    }
+    if (P.dt_max < dt) dt = P.dt_max;
+    *S.dtp = dt;
+    S.dt_red[0] = 0ull;
+    S.dt_red[1] = 0ull;
 }
 
 // ff_stress_2d_solid / ff_stress_3d_solid (CPP_19-FreeFieldParticles.cpp:24-76; the soil unit has the
@@ -3084,7 +3128,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     const int ti = S.type[i];
     const int nxt = cur ^ 1;
     const int l = step_of(S);
-    const double dt = P.dt;
+    const double dt = *S.dtp;
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     if (ti == 0 && P.flbndy == 2) {

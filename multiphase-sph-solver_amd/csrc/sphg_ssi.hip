@@ -249,7 +249,7 @@ __device__ Contact find_contact(const State &S, const Params &P, int cur, int i,
 template <int NDIM>
 __device__ __forceinline__ void contact_force(const State &S, const Params &P, int i, const Contact &c, bool scale,
                                               bool penalty, double fn[3], double ft[3]) {
-    const double dt = P.dt, dr = P.dr, myu = P.ss_myu, zeta = P.ss_zeta, mi = S.mass[i];
+    const double dt = *S.dtp, dr = P.dr, myu = P.ss_myu, zeta = P.ss_zeta, mi = S.mass[i];
     double vs[3] = {c.vs[0], c.vs[1], c.vs[2]};
 #pragma unroll
     for (int k = 0; k < NDIM; k++) vs[k] -= S.vxp[k][i];
@@ -418,9 +418,9 @@ __global__ __launch_bounds__(256) void k_accel_tail(State S, Params P, int cur) 
         for (int k = 0; k < NDIM; k++) {
             const double damp = S.vdamp[k][i];
             if (fabs(tempdown) < 1.0e-7) tempdown = 1.0e-7;
-            const double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * P.dt;
-            const double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - damp / tempdown) * P.dt;
-            if (vx1 * vx2 <= 0.0) ax[k] = -2.0 * vi[k] / P.dt - axp[k];
+            const double vx1 = vi[k] + 0.5 * (axp[k] + ax[k]) * *S.dtp;
+            const double vx2 = vi[k] + 0.5 * (axp[k] + ax[k] - damp / tempdown) * *S.dtp;
+            if (vx1 * vx2 <= 0.0) ax[k] = -2.0 * vi[k] / *S.dtp - axp[k];
             else ax[k] -= damp / tempdown;
         }
     }

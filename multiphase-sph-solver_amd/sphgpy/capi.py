@@ -416,6 +416,10 @@ def product_lib() -> _Lib:
         L.sphg_set_time.restype = _I
         L.sphg_get_time.argtypes = [C.c_void_p, C.POINTER(_D)]
         L.sphg_get_time.restype = _I
+        L.sphg_set_dt_mode.argtypes = [C.c_void_p, _I, _D, _D, _D]
+        L.sphg_set_dt_mode.restype = _I
+        L.sphg_get_dt.argtypes = [C.c_void_p, C.POINTER(_D)]
+        L.sphg_get_dt.restype = _I
         L.sphg_set_vibration.argtypes = [C.c_void_p, _D, _I, _D, _D, C.POINTER(_D)]
         L.sphg_set_vibration.restype = _I
         L.sphg_set_rain.argtypes = [C.c_void_p, C.POINTER(Rain)]
@@ -600,6 +604,24 @@ class GpuSim(Sim):
                                            deck.vib_edtime, _dptr(loads))
         if rc:
             raise RuntimeError(f"sphg_set_vibration: error {rc}")
+
+    def set_dt_mode(self, mode: int, c_cour: float = 0.25, c_wave: float = 600.0, dt_max: float = 1.0) -> int:
+        """sphg_set_dt_mode: 0 the constant Para_Pro::dt, 1 the device CFL min-reduction."""
+        return int(self.L.lib.sphg_set_dt_mode(self.ctx, mode, c_cour, c_wave, dt_max))
+
+    def dt(self) -> float:
+        """sphg_get_dt: the time step of the last step."""
+        v = C.c_double()
+        rc = self.L.lib.sphg_get_dt(self.ctx, C.byref(v))
+        if rc:
+            raise RuntimeError(f"sphg_get_dt: error {rc}")
+        return v.value
+
+    def set_time(self, t: float) -> None:
+        """sphg_set_time: the clock Para_Pro::t."""
+        rc = self.L.lib.sphg_set_time(self.ctx, t)
+        if rc:
+            raise RuntimeError(f"sphg_set_time: error {rc}")
 
     def time(self) -> float:
         t = C.c_double()
