@@ -1695,9 +1695,10 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
             #pragma unroll
             for (int k = 0; k < 6; k++) q[k] = sig[k] / r / r;
         }
-        #pragma unroll
-        for (int k = 0; k < 6; k++) S.qsig[k][i] = q[k];
-        if (S.mrc) {
+        if (!S.mrc) {  // the mixed form's momentum pass reads the packed record instead
+            #pragma unroll
+            for (int k = 0; k < 6; k++) S.qsig[k][i] = q[k];
+        } else {
             // the fluid pass forms (porosity * sig) / rho / rho per pair when the fluids' porosities differ
             const bool raw = (ti == 1 || ti == 3) && !P.fluid_por_uniform;
             float qf[6];
@@ -2386,7 +2387,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 #pragma unroll
                 for (int a = 0; a < 3; a++)
 #pragma unroll
-                    for (int b = 0; b < 3; b++) {
+                    for (int b = 0; b < NDIM; b++) {  // 2D: w[2] is +0, the b = 2 sums stay +0
                         const int sx = kSidx[a][b];
                         if (fsgn > 0) mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] + fi[sx]) * w[b];
                         else mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] - fi[sx]) * w[b];
@@ -2468,7 +2469,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
 #pragma unroll
                 for (int a = 0; a < 3; a++)
 #pragma unroll
-                    for (int b = 0; b < 3; b++) {
+                    for (int b = 0; b < NDIM; b++) {  // 2D: w[2] is +0, the b = 2 sums stay +0
                         const int sx = kSidx[a][b];
                         mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] + qi[sx]) * w[b];
                     }
@@ -2477,13 +2478,13 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                     double prei = (1 - spor) * (-satu * prewi + (1 - satu) * preai);
                     double prej = (1 - spor) * (-satu * prewj + (1 - satu) * preaj);
 #pragma unroll
-                    for (int k = 0; k < 3; k++)
+                    for (int k = 0; k < NDIM; k++)
                         mom_t[k][k] = mom_t[k][k] + mj * (M::div(M::div(prej, rj), rj) + M::div(M::div(prei, sdens), sdens)) * w[k];  // quirk A12
                 } else {
                     double prei = -satu * prewi + (1 - satu) * preai;
                     double prej = -satu * prewj + (1 - satu) * preaj;
 #pragma unroll
-                    for (int k = 0; k < 3; k++)
+                    for (int k = 0; k < NDIM; k++)
                         mom_t[k][k] = mom_t[k][k] + mj * (M::div(M::div(prej, rj), rj) - M::div(M::div(prei, rhoi), rhoi)) * w[k];
                 }
                 av_pair<NDIM, M>(vi, vj, xi, xj, dst, hli, hlj, rhoi, rj, mj, av_a, av_b, av_c, w, av);
