@@ -1829,6 +1829,11 @@ extern "C" int sphg_kl_synthesize(sphg_ctx *c, const sphg_kl_field *f, const dou
     return sphg_kl_synthesize_part(c, f, xi, 0, c->n);
 }
 
+// one workgroup per 128 x 128 tile of H[npad][rpad]
+static dim3 kl_gemm_grid(int npad, int rpad) {
+    return dim3((npad + sphg::kGm - 1) / sphg::kGm, (rpad + sphg::kGn - 1) / sphg::kGn);
+}
+
 extern "C" int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const double *xp, const int *type, int nreal,
                                         const double *xi, double fill, double *value, double *gemm_ms) {
     if (!xp || !type || !xi || !value || n <= 0 || nreal <= 0) return SPHG_E_ARG;
@@ -1866,8 +1871,7 @@ extern "C" int sphg_kl_synthesize_batch(const sphg_kl_field *f, int n, const dou
         const int tot = npad * kKlTermsHost;
         hipLaunchKernelGGL(sphg::k_kl_phi, dim3((tot + 255) / 256), dim3(256), 0, st, dxp, dty, dm, kp, npad, dphi);
         hipEventRecord(e0, st);
-        hipLaunchKernelGGL(sphg::k_kl_gemm, dim3(npad / 32, (rpad + sphg::kKlCols - 1) / sphg::kKlCols), dim3(256), 0,
-                           st, dphi, dxi, rpad, dh);
+        hipLaunchKernelGGL(sphg::k_kl_gemm, kl_gemm_grid(npad, rpad), dim3(256), 0, st, dphi, dxi, npad, rpad, dh);
         hipEventRecord(e1, st);
         const int totv = n * nreal;
         hipLaunchKernelGGL(sphg::k_kl_values, dim3((totv + 255) / 256), dim3(256), 0, st, dh, dxp, dty, kp, nreal,
@@ -1968,8 +1972,8 @@ extern "C" int sphg_kl_synthesize_batch_noalloc(const sphg_kl_field *f, int n, c
         const int tot = cpad * kKlTermsHost;
         hipLaunchKernelGGL(sphg::k_kl_phi, dim3((tot + 255) / 256), dim3(256), 0, st, (const double *)dxp,
                            (const int *)dty, (const sphg::KlModes *)dm, kp, cpad, dphi);
-        hipLaunchKernelGGL(sphg::k_kl_gemm, dim3(cpad / 32, (rpad + sphg::kKlCols - 1) / sphg::kKlCols), dim3(256), 0,
-                           st, (const float *)dphi, (const float *)dxi, rpad, dh);
+        hipLaunchKernelGGL(sphg::k_kl_gemm, kl_gemm_grid(cpad, rpad), dim3(256), 0, st, (const float *)dphi,
+                           (const float *)dxi, cpad, rpad, dh);
         hipLaunchKernelGGL(sphg::k_kl_values, dim3((cn * nreal + 255) / 256), dim3(256), 0, st, (const float *)dh,
                            (const double *)dxp, (const int *)dty, kp, nreal, rpad, fill, dv);
         e = hipGetLastError();

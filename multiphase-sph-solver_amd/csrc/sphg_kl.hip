@@ -5,7 +5,7 @@
 // eigenfunctions, sums sqrt(lambda_j) phi_j(x) xi_j in mode order, and maps H to the property.
 // Batch path (many realisations in a common domain): Phi(x) scaled by sqrt(lambda) is formed
 // once in FP64 and stored as FP32, the realisations' standard normals form Xi, and H = Phi * Xi
-// runs on the matrix cores (v_mfma_f32_32x32x2_f32), one wave per 32 x 32 tile of H.
+// runs on the matrix cores (v_mfma_f32_32x32x2_f32), one workgroup per 128 x 128 tile of H.
 // Mode constants (roots, normalisations 1/sqrt(sin(2wa)/2w +- a), sqrt(lambda)) are evaluated
 // on the host with the C library, so only cos/sin(w x) and the lognormal map run on the device.
 #include <hip/hip_runtime.h>
@@ -82,56 +82,93 @@ __global__ __launch_bounds__(256) void k_kl_phi(const double *xp, const int *typ
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // H[npad][rpad] = Phi[npad][200] * Xi[200][rpad] on the matrix cores (v_mfma_f32_32x32x2_f32:
-// lane l supplies A[l % 32][k0 + l / 32] and B[k0 + l / 32][l % 32] and holds
-// D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32] in accumulator q). A workgroup of four waves owns 32
-// rows of H and kKlCols columns: the 32 x 200 block of Phi is one contiguous 25.6 KB stretch, staged
-// into LDS with coalesced loads, and each wave multiplies it with its kKlCols / 4 columns
-// of Xi (read from L2, 128 contiguous bytes per half-wave). Phi is read from HBM once and every
-// A fragment serves kKlTiles tiles. npad and rpad are multiples of 32; grid.y covers the columns
-// in groups of kKlCols.
-constexpr int kKlTiles = 2;                 // 32-column tiles per wave
-constexpr int kKlCols = 4 * 32 * kKlTiles;  // columns per workgroup
+// lane l supplies A[l % 32][k + l / 32] and B[k + l / 32][l % 32] and holds
+// D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32] in accumulator q). A workgroup of four waves owns a
+// 128 x 128 tile of H, each wave a 64 x 64 quarter (2 x 2 MFMA tiles). K runs in chunks of 40:
+// the 128 x 40 block of Phi and the 40 x 128 block of Xi are staged in LDS with 16-byte coalesced
+// loads, so every element of Phi and Xi is read from memory once per tile it feeds, and each
+// fragment read from LDS feeds two MFMAs. The LDS rows are padded (A: 42 floats, B: 160) so that
+// the two half-waves' fragment reads, at k and k + 1, fall in 64 distinct banks.
+constexpr int kGm = 128, kGn = 128, kGk = 40, kGlda = 42, kGldb = 160;
+static_assert(kKlTerms % kGk == 0, "K chunks");
 
-__global__ __launch_bounds__(256) void k_kl_gemm(const float *phi, const float *xi, int rpad, float *h) {
-    // rows padded to kKlTerms + 1 floats: the 32 lanes of a half-wave read 32 rows at one k, and an
-    // odd row stride puts them in 32 different banks
-    constexpr int kLd = kKlTerms + 1;
-    __shared__ float a_lds[32 * kLd];
+__global__ __launch_bounds__(256) void k_kl_gemm(const float *phi, const float *xi, int npad, int rpad, float *h) {
+    __shared__ float a_s[kGm * kGlda];  // [row][k]
+    __shared__ float b_s[kGk * kGldb];  // [k][column]
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int m0 = blockIdx.x * 32;
-    const float *src = phi + (size_t)m0 * kKlTerms;
-    for (int q = tid; q < 32 * kKlTerms; q += 256) a_lds[(q / kKlTerms) * kLd + q % kKlTerms] = src[q];
-    __syncthreads();
+    const int m0 = blockIdx.x * kGm, n0 = blockIdx.y * kGn;
+    const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
     const int r = lane & 31, kk = lane >> 5;
-    const int c0 = blockIdx.y * kKlCols + wv * 32 * kKlTiles;
-    f32x16 acc[kKlTiles];
+    f32x16 acc[2][2];
 #pragma unroll
-    for (int t = 0; t < kKlTiles; t++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int q = 0; q < 16; q++) acc[t][q] = 0.0f;
-    bool live[kKlTiles];
+        for (int j = 0; j < 2; j++)
 #pragma unroll
-    for (int t = 0; t < kKlTiles; t++) live[t] = c0 + 32 * t < rpad;
-    if (!live[0]) return;  // no barrier follows
+            for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
+    // each thread stages kPer float4 of A and of B per chunk; the next chunk's loads are issued
+    // before the current chunk's MFMAs, so their latency overlaps the matrix work
+    constexpr int kPer = kGm * kGk / 4 / 256;
+    static_assert(kPer * 256 * 4 == kGm * kGk && kGk * kGn == kGm * kGk, "staging split");
+    float4 ra[kPer], rb[kPer];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < kPer; p++) {
+            const int q = tid + 256 * p;
+            const int row = q / (kGk / 4), c4 = q % (kGk / 4), m = m0 + row;
+            ra[p] = m < npad ? *(const float4 *)(phi + (size_t)m * kKlTerms + k0 + 4 * c4)
+                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const int kr = q / (kGn / 4), d4 = q % (kGn / 4), col = n0 + 4 * d4;
+            rb[p] = col < rpad ? *(const float4 *)(xi + (size_t)(k0 + kr) * rpad + col)
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < kPer; p++) {
+            const int q = tid + 256 * p;
+            const int row = q / (kGk / 4), c4 = q % (kGk / 4);
+            float *d = a_s + row * kGlda + 4 * c4;
+            d[0] = ra[p].x;
+            d[1] = ra[p].y;
+            d[2] = ra[p].z;
+            d[3] = ra[p].w;
+            const int kr = q / (kGn / 4), d4 = q % (kGn / 4);
+            *(float4 *)(b_s + kr * kGldb + 4 * d4) = rb[p];
+        }
+    };
+    load(0);
+    store();
+    __syncthreads();
+    for (int k0 = 0; k0 < kKlTerms; k0 += kGk) {
+        const bool more = k0 + kGk < kKlTerms;
+        if (more) load(k0 + kGk);
 #pragma unroll 4
-    for (int k0 = 0; k0 < kKlTerms; k0 += 2) {
-        const float a = a_lds[r * kLd + k0 + kk];
-#pragma unroll
-        for (int t = 0; t < kKlTiles; t++) {
-            if (!live[t]) continue;
-            const float b = xi[(size_t)(k0 + kk) * rpad + c0 + 32 * t + r];
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
+        for (int k = 0; k < kGk; k += 2) {
+            const float a0 = a_s[(wm + r) * kGlda + k + kk], a1 = a_s[(wm + 32 + r) * kGlda + k + kk];
+            const float b0 = b_s[(k + kk) * kGldb + wn + r], b1 = b_s[(k + kk) * kGldb + wn + 32 + r];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
         }
     }
 #pragma unroll
-    for (int t = 0; t < kKlTiles; t++) {
-        if (!live[t]) continue;
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int row = 8 * (q / 4) + 4 * kk + (q % 4);
-            h[(size_t)(m0 + row) * rpad + c0 + 32 * t + r] = acc[t][q];
+        for (int j = 0; j < 2; j++) {
+            const int col = n0 + wn + 32 * j + r;
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int row = m0 + wm + 32 * i + 8 * (q / 4) + 4 * kk + (q % 4);
+                if (row < npad && col < rpad) h[(size_t)row * rpad + col] = acc[i][j][q];
+            }
         }
-    }
 }
 
 // value[rr][i] from H[i][rr]; particles of other types keep fill
