@@ -288,12 +288,14 @@ class Deck:
             bmin[j] = bmin[j] - 2.0 * self.dr
         ph = {}
         for t, nm in ((1, "water"), (2, "soil")):
+            # per particle in order: mn = min(mn, x), then mx = max(mn, x) = x, so the maximum
+            # slot ends up holding the last particle's coordinates
             mn = np.full(3, 100000.0)
             mx = np.full(3, -100000.0)
-            for i in np.nonzero(self.type == t)[0]:
-                for j in range(3):
-                    mn[j] = min(mn[j], xp[i, j])
-                    mx[j] = max(mn[j], xp[i, j])
+            idx = np.nonzero(self.type == t)[0]
+            if idx.size:
+                mn = np.minimum(mn, xp[idx].min(axis=0))
+                mx = xp[idx[-1]].astype(np.float64).copy()
             ph[nm] = (mn, mx)
         return bmin, bmax, ph
 
