@@ -380,7 +380,7 @@ def main():
             if rc:
                 raise SystemExit(f"autotune failed with error {rc}")
             extra["batch_widths"] = {"density": widths[0], "norm_eos": widths[1], "strain": widths[2],
-                                     "xsph": widths[3]}
+                                     "xsph": widths[3], "search_stage": g.search_stage()}
         return g, deck, state, extra
 
     def timed(g, per_kernel):

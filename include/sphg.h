@@ -370,15 +370,21 @@ int sphg_set_list_order(sphg_ctx *ctx, int reversed);
 int sphg_kernel_times(sphg_ctx *ctx, char (*names)[32], double *ms, int *launches, int cap);
 int sphg_enable_timing(sphg_ctx *ctx, int on);
 /* Autotuner (no counterpart in the reference): the density, normalisation, strain and XSPH
- * passes load 1-6 neighbours per memory round trip, and the best width depends on the deck. Each
- * width set is timed over `steps` steps from the current state, the state is restored after each
- * trial, and each pass keeps its fastest width (chosen[4]: density, normalisation, strain, XSPH;
- * may be null). Results do not depend on the widths (bit for bit). Allocates a copy of the
- * device state for the duration of the call. */
+ * passes load 1-6 neighbours per memory round trip, and the best width depends on the deck; the
+ * search stages the first 16 or 32 entries of each list in LDS before writing them out, and the
+ * better depth depends on the lists' length and the deck's size. Each width set (with a staging
+ * depth) is timed over `steps` steps from the current state, the state is restored after each
+ * trial, and each pass keeps its fastest choice (chosen[4]: density, normalisation, strain, XSPH;
+ * may be null; sphg_get_search_stage reports the depth). Results do not depend on these choices
+ * (bit for bit). Allocates a copy of the device state for the duration of the call. */
 int sphg_autotune(sphg_ctx *ctx, int steps, int *chosen);
 /* Sets the widths directly (density 1/2/4, normalisation 2/4/6, strain 1/2/4, XSPH 1/2/4);
  * SPHG_E_ARG for any other value. */
 int sphg_set_batch_widths(sphg_ctx *ctx, const int *widths);
+/* The search's list staging depth directly (16 or 32; default 16 in 2D, 32 in 3D); SPHG_E_ARG for
+ * any other value. */
+int sphg_set_search_stage(sphg_ctx *ctx, int stage);
+int sphg_get_search_stage(const sphg_ctx *ctx, int *stage);
 
 /* ---- K-L random fields (CPP_17-1_K-L_expansion_non.cpp) ---------------------------------- */
 

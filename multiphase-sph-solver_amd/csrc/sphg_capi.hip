@@ -81,6 +81,8 @@ struct sphg_ctx {
     // neighbours per load round trip of the density, normalisation, strain and XSPH passes
     // (sphg_autotune); each has the widths kBwChoices[k]
     int bw[4] = {2, 6, 2, 2};
+    // list entries the search stages in LDS per thread (16 or 32; sphg_autotune, sphg_set_search_stage)
+    int search_stage = 32;
     std::vector<Grid> grids;   // one per realisation (host copy)
     std::vector<int> counts;   // particles per realisation (batch contexts)
     Grid *d_grids = nullptr;
@@ -140,6 +142,7 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
     c->pro = *pro;
     c->mat = *mat;
     c->cap = capacity;
+    c->search_stage = pro->ndim == 2 ? 16 : 32;  // 2D lists hold about 10-20 entries, 3D about 30
     c->precision = precision;
     Params &P = c->P;
     P.cell_size = 2.0 * pro->dr;
@@ -1014,7 +1017,8 @@ static void enqueue_nnps(sphg_ctx *c) {
     LAUNCH("scan_add", k_scan_add, c->scan_blocks, c->S.cell_start, c->S.block_sums, M);
     LAUNCH("scatter", k_scatter, grid_for(n), c->S, c->P);
     LAUNCH("cell_order", k_cell_order, grid_for(M), c->S, M);
-    LAUNCH("search", k_search<NDIM>, grid_for(n), c->S, c->P, cur);
+    if (c->search_stage == 16) LAUNCH("search", (k_search<NDIM, 16>), grid_for(n), c->S, c->P, cur);
+    else LAUNCH("search", (k_search<NDIM, 32>), grid_for(n), c->S, c->P, cur);
     if (c->rev_lists) LAUNCH("reverse_lists", k_reverse_lists, grid_for(n), c->S, c->P);
     if (c->P.flknl > 2) LAUNCH("kgc", k_kgc<NDIM>, grid_for(n), c->S, c->P, cur);
 }
@@ -1169,8 +1173,13 @@ extern "C" int sphg_parttocell(sphg_ctx *c) {
 extern "C" int sphg_partisearching(sphg_ctx *c) {
     if (!c || !c->uploaded) return SPHG_E_ARG;
     const int n = c->n, cur = c->cur;
-    if (c->P.ndim == 2) LAUNCH("search", k_search<2>, grid_for(n), c->S, c->P, cur);
-    else LAUNCH("search", k_search<3>, grid_for(n), c->S, c->P, cur);
+    auto go = [&](auto ndim_tag) {
+        constexpr int NDIM = decltype(ndim_tag)::value;
+        if (c->search_stage == 16) LAUNCH("search", (k_search<NDIM, 16>), grid_for(n), c->S, c->P, cur);
+        else LAUNCH("search", (k_search<NDIM, 32>), grid_for(n), c->S, c->P, cur);
+    };
+    if (c->P.ndim == 2) go(std::integral_constant<int, 2>());
+    else go(std::integral_constant<int, 3>());
     finish_timing(c);
     return read_err(c);
 }
@@ -1541,7 +1550,10 @@ extern "C" int sphg_autotune(sphg_ctx *c, int steps, int *chosen) {
     };
     const int step0 = c->step, cur0 = c->cur;
     const bool stepped0 = c->stepped;
-    const int saved_bw[4] = {c->bw[0], c->bw[1], c->bw[2], c->bw[3]};
+    const int saved_bw[4] = {c->bw[0], c->bw[1], c->bw[2], c->bw[3]}, saved_stage = c->search_stage;
+    static const int kStageChoices[3] = {16, 32, 16};
+    double best_search = 1e300;
+    int pick_stage = saved_stage;
     static const char *kNames[4] = {"density", "norm_eos", "strain", "xsph"};
     double best[4] = {1e300, 1e300, 1e300, 1e300};
     int pick[4] = {saved_bw[0], saved_bw[1], saved_bw[2], saved_bw[3]};
@@ -1549,12 +1561,17 @@ extern "C" int sphg_autotune(sphg_ctx *c, int steps, int *chosen) {
     int rc = e == hipSuccess ? SPHG_OK : SPHG_E_HIP;
     for (int t = 0; t < 3 && rc == SPHG_OK; t++) {
         for (int k = 0; k < 4; k++) c->bw[k] = kBwChoices[k][t];
+        c->search_stage = kStageChoices[t];
         sphg_enable_timing(c, 1);
         rc = sphg_step(c, 1);  // first launches of these instantiations, untimed
         sphg_enable_timing(c, 1);
         if (rc == SPHG_OK) rc = sphg_step(c, steps);
         finish_timing(c);
-        for (auto &r : c->tsum)
+        for (auto &r : c->tsum) {
+            if (r.first == "search" && r.second.second > 0 && r.second.first / r.second.second < best_search) {
+                best_search = r.second.first / r.second.second;
+                pick_stage = kStageChoices[t];
+            }
             for (int k = 0; k < 4; k++)
                 if (r.first == kNames[k] && r.second.second > 0) {
                     const double avg = r.second.first / r.second.second;
@@ -1563,6 +1580,7 @@ extern "C" int sphg_autotune(sphg_ctx *c, int steps, int *chosen) {
                         pick[k] = kBwChoices[k][t];
                     }
                 }
+        }
         sphg_enable_timing(c, 0);
         // back to the state before the trial
         e = copy_state(false);
@@ -1574,10 +1592,24 @@ extern "C" int sphg_autotune(sphg_ctx *c, int steps, int *chosen) {
     }
     hipFree(bak);
     for (int k = 0; k < 4; k++) c->bw[k] = rc == SPHG_OK ? pick[k] : saved_bw[k];
+    c->search_stage = rc == SPHG_OK ? pick_stage : saved_stage;
     drop_graphs(c);
     if (chosen)
         for (int k = 0; k < 4; k++) chosen[k] = c->bw[k];
     return rc;
+}
+
+extern "C" int sphg_set_search_stage(sphg_ctx *c, int stage) {
+    if (!c || (stage != 16 && stage != 32)) return SPHG_E_ARG;
+    c->search_stage = stage;
+    drop_graphs(c);
+    return SPHG_OK;
+}
+
+extern "C" int sphg_get_search_stage(const sphg_ctx *c, int *stage) {
+    if (!c || !stage) return SPHG_E_ARG;
+    *stage = c->search_stage;
+    return SPHG_OK;
 }
 
 extern "C" int sphg_set_batch_widths(sphg_ctx *c, const int *widths) {

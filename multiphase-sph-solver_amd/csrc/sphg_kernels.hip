@@ -442,15 +442,17 @@ __global__ __launch_bounds__(256) void k_cell_order(State S, int ncells) {
 // order, predicate sqrt(d2) <= 2*ra && i != j with d2 accumulated in FP64 exactly as the reference.
 // sqrt is monotone and correctly rounded, so candidates whose d2 lies outside a +-1e-14 relative
 // band around (2*ra)^2 are decided without it; only the band (on-cut-off pairs of the jittered
-// lattice) takes the exact sqrt test. The first kStage entries of each thread are staged in LDS
+// lattice) takes the exact sqrt test. The first STAGE entries of each thread are staged in LDS
 // and written out row by row afterwards, so the list rows are stored coalesced. Kernel values are
 // not stored: each pair pass recomputes them from the positions (pair_kernel), which costs a
 // square root and a few divisions per pair instead of 32 B of HBM traffic per pair and pass.
-constexpr int kSearchBlock = 256, kStage = 32;
+constexpr int kSearchBlock = 256;
 
-template <int NDIM>
+// STAGE: list entries staged per thread (16 or 32). The LDS it takes bounds the resident waves (8 or 5
+// per SIMD), so the better depth depends on the deck's list length and size (sphg_autotune picks it).
+template <int NDIM, int STAGE>
 __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int cur) {
-    __shared__ int stage[kStage][kSearchBlock];
+    __shared__ int stage[STAGE][kSearchBlock];
     const int i = cell_pidx(S, P);
     const int t = threadIdx.x;
     if (i >= P.n) return;
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
                 else if (d2 > R2 * 1.00000000000001) in = false;
                 else in = sqrt(d2) <= R;
                 if (in && i != j) {
-                    if (cnt < kStage) stage[cnt][t] = e;
+                    if (cnt < STAGE) stage[cnt][t] = e;
                     else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = e;
                     tmask |= 1 << nb_type(e);
                     cnt++;
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     }
     S.nb_cnt[i] = cnt;
     S.nb_types[i] = tmask;
-    const int nk = cnt < kStage ? cnt : kStage;
+    const int nk = cnt < STAGE ? cnt : STAGE;
     for (int k = 0; k < nk; k++) S.nbr[(size_t)k * P.cap + i] = stage[k][t];
 }
 
@@ -3371,8 +3373,10 @@ __global__ void k_eob_next(State S, Params P) {
 // explicit instantiations used by the host side
 template __global__ void k_bin<2>(State, Params, int);
 template __global__ void k_bin<3>(State, Params, int);
-template __global__ void k_search<2>(State, Params, int);
-template __global__ void k_search<3>(State, Params, int);
+template __global__ void k_search<2, 16>(State, Params, int);
+template __global__ void k_search<2, 32>(State, Params, int);
+template __global__ void k_search<3, 16>(State, Params, int);
+template __global__ void k_search<3, 32>(State, Params, int);
 template __global__ void k_density<2, Exact, 1>(State, Params, int);
 template __global__ void k_density<2, Exact, 2>(State, Params, int);
 template __global__ void k_density<2, Exact, 4>(State, Params, int);

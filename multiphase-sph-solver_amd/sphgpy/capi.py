@@ -471,6 +471,10 @@ def product_lib() -> _Lib:
         L.sphg_autotune.argtypes = [C.c_void_p, _I, C.POINTER(C.c_int)]
         L.sphg_autotune.restype = _I
         L.sphg_set_batch_widths.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.sphg_set_search_stage.argtypes = [C.c_void_p, _I]
+        L.sphg_set_search_stage.restype = _I
+        L.sphg_get_search_stage.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.sphg_get_search_stage.restype = _I
         L.sphg_set_list_order.argtypes = [C.c_void_p, _I]
         L.sphg_set_list_order.restype = _I
         L.sphg_set_batch_widths.restype = _I
@@ -668,6 +672,18 @@ class GpuSim(Sim):
     def set_batch_widths(self, widths) -> int:
         """sphg_set_batch_widths (density, normalisation, strain, XSPH)."""
         return int(self.L.lib.sphg_set_batch_widths(self.ctx, (C.c_int * 4)(*widths)))
+
+    def set_search_stage(self, stage: int) -> int:
+        """sphg_set_search_stage: list entries the search stages in LDS (16 or 32)."""
+        return int(self.L.lib.sphg_set_search_stage(self.ctx, stage))
+
+    def search_stage(self) -> int:
+        """sphg_get_search_stage."""
+        v = C.c_int()
+        rc = self.L.lib.sphg_get_search_stage(self.ctx, C.byref(v))
+        if rc:
+            raise RuntimeError(f"sphg_get_search_stage: error {rc}")
+        return v.value
 
     def save_state(self, path: str) -> int:
         """sphg_save_state: checkpoint of the device state (returns the error code)."""

@@ -1,8 +1,10 @@
-"""Neighbour batch widths and the autotuner (sphg_set_batch_widths, sphg_autotune).
+"""Neighbour batch widths, the search's list staging depth and the autotuner
+(sphg_set_batch_widths, sphg_set_search_stage, sphg_autotune).
 
-Every width set gives the same fields and neighbour lists bit for bit (the summation order is
-the list order whatever the width); the autotuner leaves the state as it found it and picks
-widths from the built set.
+Every width set and both staging depths give the same fields and neighbour lists bit for bit (the
+summation order is the list order whatever the width, and the depth only decides which entries
+pass through LDS before they are stored); the autotuner leaves the state as it found it and picks
+choices from the built set.
 """
 import numpy as np
 import pytest
@@ -29,9 +31,10 @@ def test_every_width_gives_the_same_results(name):
     deck, states = load_golden(name)
     ref = make(deck, states[0])
     assert ref.step(4) == 0
-    for w in WIDTHS:
+    for w, stage in zip(WIDTHS, (16, 32, 16, 32)):
         g = make(deck, states[0])
         assert g.set_batch_widths(w) == 0
+        assert g.set_search_stage(stage) == 0 and g.search_stage() == stage
         assert g.step(4) == 0
         for k in FIELDS:
             assert bitequal(g.get(k), ref.get(k)), (w, k)
@@ -58,3 +61,5 @@ def test_autotune_restores_state_and_picks_built_widths():
     for k in FIELDS:
         assert bitequal(g.get(k), ref.get(k)), k
     assert g.set_batch_widths([3, 2, 2, 2]) == 100
+    assert g.search_stage() in (16, 32)
+    assert g.set_search_stage(24) == 100
