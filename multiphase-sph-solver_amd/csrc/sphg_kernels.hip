@@ -662,7 +662,7 @@ __device__ __forceinline__ void nb_vel_mixed(const State &S, int j, bool bnd, in
 // accumulate side by side in one walk over the list.
 // ------------------------------------------------------------------------------------------
 template <int NDIM, class M, int BW>
-__global__ __launch_bounds__(256) void k_density(State S, Params P, int cur) {
+__global__ __launch_bounds__(256, (NDIM == 2 && BW == 1 && !M::kKgc) ? 7 : 1) void k_density(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
