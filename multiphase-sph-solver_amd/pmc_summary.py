@@ -32,13 +32,16 @@ def counters(d):
 
 def short(name):
     """Pass name as the bench reports it; the neighbour batch width (third template argument of
-    the tunable passes) is kept as a suffix, e.g. strain/w2."""
+    the tunable passes) is kept as a suffix, e.g. strain/w2, and so is the search's list staging
+    depth, e.g. search/s16."""
     s = name.split("(")[0].replace("void ", "").replace("sphg::", "")
     base = s.split("<")[0].replace("k_", "")
     if "<" in s:
         args = [a.strip() for a in s[s.index("<") + 1:s.rindex(">")].split(",")]
         if base in ("density", "norm", "strain", "xsph") and len(args) == 3:
             return f"{base}/w{args[2]}"
+        if base == "search" and len(args) == 2:
+            return f"{base}/s{args[1]}"
     return base
 
 
