@@ -242,7 +242,7 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
             if (tran) D(S.sigb[t][k]);
             else S.sigb[t][k] = nullptr;
         }
-    I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_cursor, ncellp);
+    I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_rank, cap);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
     I(S.err, 1); I(S.step, 1);
     plan.push_back({(void **)&S.dtp, 8});
@@ -1005,7 +1005,6 @@ template <int NDIM>
 static void enqueue_nnps(sphg_ctx *c) {
     const int cur = c->cur, n = c->n, M = c->P.ncell + 2;
     LAUNCH("fill_counts", k_fill_int, grid_for(M), c->S.cell_count, M, 0);
-    LAUNCH("fill_cursor", k_fill_int, grid_for(M), c->S.cell_cursor, M, 0);
     LAUNCH("bin", k_bin<NDIM>, grid_for(n), c->S, c->P, cur);
     LAUNCH("scan_tiles", k_scan_tiles, c->scan_blocks, c->S.cell_count, c->S.cell_start, c->S.block_sums, M);
     {
@@ -1157,7 +1156,6 @@ extern "C" int sphg_parttocell(sphg_ctx *c) {
     auto go = [&](auto ndim_tag) {
         constexpr int NDIM = decltype(ndim_tag)::value;
         LAUNCH("fill_counts", k_fill_int, grid_for(M), c->S.cell_count, M, 0);
-        LAUNCH("fill_cursor", k_fill_int, grid_for(M), c->S.cell_cursor, M, 0);
         LAUNCH("bin", k_bin<NDIM>, grid_for(n), c->S, c->P, cur);
         LAUNCH("scan_tiles", k_scan_tiles, c->scan_blocks, c->S.cell_count, c->S.cell_start, c->S.block_sums, M);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, c->stream, c->S.block_sums, c->scan_blocks);

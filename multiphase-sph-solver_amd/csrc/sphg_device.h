@@ -110,7 +110,8 @@ struct State {
     int *type, *matype, *permtype, *etype;
     int8_t *soil_slot, *fluid_slot;
     // cell list
-    int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *block_sums;
+    int *cell, *cell_count, *cell_start, *sorted, *block_sums;
+    int *cell_rank;       // particle's arrival rank in its cell at binning (its slot in the cell's run)
     int *nb_cnt, *nbr;    // list entries: id | type << 28, column-major nbr[k * cap + i]
     int *nb_types;        // bit t set when the list holds a neighbour of type t
     double *qsig[6];      // the j factor of the momentum sums: sig / rho / rho (solids),

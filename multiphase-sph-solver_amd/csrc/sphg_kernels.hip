@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_bin(State S, Params P, int cur) {
         c = 0;
     }
     S.cell[i] = c;
-    atomicAdd(&S.cell_count[c], 1);
+    S.cell_rank[i] = atomicAdd(&S.cell_count[c], 1);
 }
 
 // Exclusive scan of cell counts: block-local pass, top pass over block sums, add pass.
@@ -415,8 +415,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_add(int *out, const int *bl
 __global__ __launch_bounds__(256) void k_scatter(State S, Params P) {
     int i = pidx();
     if (i >= P.n) return;
-    int c = S.cell[i];
-    int pos = S.cell_start[c] + atomicAdd(&S.cell_cursor[c], 1);
+    // the rank the binning's counter gave the particle is its slot in the cell's run (the run is
+    // put in id order by k_cell_order next)
+    const int pos = S.cell_start[S.cell[i]] + S.cell_rank[i];
     S.sorted[pos] = i | (S.type[i] << kTypeShift);
 }
 
