@@ -968,17 +968,19 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                 pair_kernel<NDIM, M>(P, kc, xi, xj[b], hli, hlj[b], w, d2, dst);
                 // strain (CPP_7-1-strain.cpp:67-150)
                 if (ti == tj || (bnd && ti <= 4)) {
+                    // 2D: the velocity difference and the gradient have no third component (+0), so
+                    // the sums with a = 2 or c = 2 stay +0 and are not formed
                     double dvx[3];
                     dvx[0] = vi[0] - vj[b][0];
                     dvx[1] = vi[1] - vj[b][1];
                     dvx[2] = NDIM == 3 ? vi[2] - vj[b][2] : 0.0;
 #pragma unroll
-                    for (int a = 0; a < 3; a++)
+                    for (int a = 0; a < NDIM; a++)
 #pragma unroll
-                        for (int c = 0; c < 3; c++) up[3 * a + c] = up[3 * a + c] + M::div(mj[b], rj[b]) * dvx[a] * w[c];
+                        for (int c = 0; c < NDIM; c++) up[3 * a + c] = up[3 * a + c] + M::div(mj[b], rj[b]) * dvx[a] * w[c];
                     if (solid) {
 #pragma unroll
-                        for (int k = 0; k < 3; k++) dnm[k] = dnm[k] + M::div(mj[b], rj[b]) * (xi[k] - xj[b][k]) * w[k];
+                        for (int k = 0; k < NDIM; k++) dnm[k] = dnm[k] + M::div(mj[b], rj[b]) * (xi[k] - xj[b][k]) * w[k];
                     }
                 }
                 // eos loop 2 (CPP_7-1-eos_CR.cpp:62-118): soil and structure particles
