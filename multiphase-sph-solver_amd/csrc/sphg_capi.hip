@@ -915,6 +915,10 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
         }
     }
     HIPCHK(hipMemcpy(c->S.fdens, fd.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+    // dry context: no water or air particle now, and none can appear (rain or backfill)
+    bool fluid = false;
+    for (int i = 0; i < n && !fluid; i++) fluid = c->h_type[i] == 1 || c->h_type[i] == 3;
+    P.dry = (!fluid && pro.flr == 0 && pro.fleob == 0) ? 1 : 0;
     {
         // per-particle material values pParti_ConsPara[i].ds / .cop / .fai: the soil slot's unless
         // the caller set them (random fields), and the Drucker-Prager constants of each particle's

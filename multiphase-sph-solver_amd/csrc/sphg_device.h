@@ -79,6 +79,10 @@ struct Params {
     // time step: 0 constant (Para_Pro::dt, the reference's timestep()), 1 CFL min-reduction:
     // dt = min(dt_cour dr / (dt_cwave + max|v|), 0.25 sqrt(dr / max|a|), dt_max)
     int dt_mode;
+    // no water or air particle can exist in the context (none uploaded, no rain, no excavation):
+    // every solid's prew / prea is then +0 after the eos pass, so the pore-pressure terms of the
+    // momentum sums are +-0 and their gathers are skipped
+    int dry;
     double dt_cour, dt_cwave, dt_max;
 };
 

@@ -956,7 +956,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                     mj[b] = S.mass[j];
                     rj[b] = S.rho[j];
                     nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
-                    pj[b] = solid ? S.pre[j] : 0.0;
+                    pj[b] = solid && !P.dry ? S.pre[j] : 0.0;  // read by the eos sums of fluid neighbours only
                 }
             }
 #pragma unroll
@@ -2165,10 +2165,10 @@ __device__ __forceinline__ void bndy_stress_rows(const State &S, const Params &P
 struct NbRec {
     double m, r, v[3], q[6], satu, prew, prea, por, cop;
 };
-__device__ __forceinline__ NbRec load_nbrec(const State &S, int j) {
+__device__ __forceinline__ NbRec load_nbrec(const State &S, int j, bool pore = true) {
     const float4 a = S.mra[j];
     const float4 *p = S.mrc + 3 * (size_t)j;
-    const float4 b = p[0], c = p[1], d = p[2];
+    const float4 b = p[0], c = p[1], d = pore ? p[2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     NbRec o;
     o.m = a.x; o.v[0] = a.y; o.v[1] = a.z; o.v[2] = a.w;
     o.r = b.x; o.q[0] = b.y; o.q[1] = b.z; o.q[2] = b.w;
@@ -2439,7 +2439,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             hljb[b] = hl_of(S, P, j, hli);
             fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
             if constexpr (M::kFast) {
-                const NbRec rc = load_nbrec(S, j);
+                const NbRec rc = load_nbrec(S, j, !P.dry || drag_on);
                 mjb[b] = rc.m;
                 rjb[b] = rc.r;
 #pragma unroll
@@ -2457,9 +2457,14 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
 #pragma unroll
             for (int k = 0; k < 6; k++) qjb[b][k] = S.qsig[k][j];  // sig_j / rho_j / rho_j
-            satub[b] = S.satu[j];
-            prewb[b] = S.prew[j];
-            preab[b] = S.prea[j];
+            satub[b] = 0.0;
+            prewb[b] = 0.0;
+            preab[b] = 0.0;
+            if (!P.dry) {
+                satub[b] = S.satu[j];
+                prewb[b] = S.prew[j];
+                preab[b] = S.prea[j];
+            }
         }
 #pragma unroll
         for (int b = 0; b < B; b++) {
@@ -2478,8 +2483,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                         const int sx = kSidx[a][b];
                         mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] + qi[sx]) * w[b];
                     }
-                // pore-pressure term; quirk: the neighbour's saturation is used for both terms
-                if (ti == 2) {
+                // pore-pressure term; quirk: the neighbour's saturation is used for both terms. In a
+                // dry context both pressures are +0, the term is +-0 and leaves the sums unchanged.
+                if (P.dry) {
+                } else if (ti == 2) {
                     double prei = (1 - spor) * (-satu * prewi + (1 - satu) * preai);
                     double prej = (1 - spor) * (-satu * prewj + (1 - satu) * preaj);
 #pragma unroll
