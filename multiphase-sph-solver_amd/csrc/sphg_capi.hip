@@ -244,7 +244,7 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
         }
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_rank, cap);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
-    I(S.err, 1); I(S.step, 1);
+    I(S.err, 1); I(S.step, 1); I(S.shear3, 1);
     plan.push_back({(void **)&S.dtp, 8});
     plan.push_back({(void **)&S.dt_red, 16});
     plan.push_back({(void **)&S.tnow, 8});
@@ -579,6 +579,12 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
         for (size_t i = 0; i < N; i++) buf[i] = host[nc * i + k];
         HIPCHK(hipMemcpy(ptr[k], buf.data(), N * 8, hipMemcpyHostToDevice));
         if (!mirror.empty()) HIPCHK(hipMemcpy(mirror[k], buf.data(), N * 8, hipMemcpyHostToDevice));
+    }
+    if (nm == "sig" && nc == 6 && c->uploaded && c->P.ndim == 2) {
+        // new stresses after sphg_upload_done: out-of-plane shear raises the flag the stress pass keeps
+        int sh3 = 0;
+        for (size_t i = 0; i < N && !sh3; i++) sh3 = host[6 * i + 4] != 0.0 || host[6 * i + 5] != 0.0;
+        if (sh3) HIPCHK(hipMemcpy(c->S.shear3, &sh3, 4, hipMemcpyHostToDevice));
     }
     if (nm == "ps_c") c->has_psc = true;
     if (nm == "ps_ds") c->has_psds = true;
@@ -955,6 +961,15 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     // the pair passes form the kernel normalisation once per particle from a common smoothing
     // length; setting_par gives every particle hl = dr and the step never changes it
     if (!P.hl_uniform) return SPHG_E_UNSUPPORTED;
+    // out-of-plane shear stresses of the uploaded state (2D); the stress pass keeps the flag after
+    int sh3 = 0;
+    if (P.ndim == 2) {
+        for (int k = 4; k < 6 && !sh3; k++) {
+            HIPCHK(hipMemcpy(hl.data(), c->S.sig[k], (size_t)n * 8, hipMemcpyDeviceToHost));
+            for (int i = 0; i < n && !sh3; i++) sh3 = hl[i] != 0.0;
+        }
+    }
+    HIPCHK(hipMemcpy(c->S.shear3, &sh3, 4, hipMemcpyHostToDevice));
     c->S.ps_c = c->has_psc ? c->d_psc : nullptr;
     int zero = 0;
     HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));

@@ -150,6 +150,7 @@ struct State {
     // dynamic particle sets: dyn = {nrain, cement_flag, nstage}; ecp[ecount][3] = start, end, step
     int *dyn, *ecp;
     int *err, *step;      // sticky error word; current step number l
+    int *shear3;          // 2D: set (and kept) once some particle's sig_yz or sig_xz is nonzero
     double *dtp;          // time step of the current step (Para_Pro::dt; the CFL mode rewrites it)
     unsigned long long *dt_red;  // CFL mode: bit patterns of max |v|^2 and max |a|^2 over particles
     const Grid *grids;    // one per realisation

@@ -1700,6 +1700,9 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
             #pragma unroll
             for (int k = 0; k < 6; k++) q[k] = sig[k] / r / r;
         }
+        // 2D: out-of-plane shear stresses stay zero from a zero start (no out-of-plane strain rate or
+        // spin); the flag records the first particle where they do not
+        if (P.ndim == 2 && (sig[4] != 0.0 || sig[5] != 0.0) && *S.shear3 == 0) atomicOr(S.shear3, 1);
         if (!S.mrc) {  // the mixed form's momentum pass reads the packed record instead
             #pragma unroll
             for (int k = 0; k < 6; k++) S.qsig[k][i] = q[k];
@@ -1709,9 +1712,9 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
             float qf[6];
             #pragma unroll
             for (int k = 0; k < 6; k++) qf[k] = (float)(raw ? sig[k] : q[k]);
-            float4 *o = S.mrc + 3 * (size_t)i;
-            o[0] = make_float4((float)r, qf[0], qf[1], qf[2]);
-            o[1] = make_float4(qf[3], qf[4], qf[5], (float)S.satu[i]);
+            float4 *o = S.mrc + 3 * (size_t)i;  // layout: see NbRec
+            o[0] = make_float4((float)r, qf[0], qf[1], qf[3]);
+            o[1] = make_float4(qf[2], qf[4], qf[5], (float)S.satu[i]);
             o[2] = make_float4((float)S.prew[i], (float)S.prea[i], (float)S.porosity[i], (float)S.cop[i]);
         }
     }
@@ -2161,18 +2164,22 @@ __device__ __forceinline__ void bndy_stress_rows(const State &S, const Params &P
     for (int a = 0; a < 3; a++) rows[a] = t[a][0] + t[a][1] + t[a][2];
 }
 
-// A neighbour's packed momentum-pass record (mixed form), widened to FP64 for the arithmetic
+// A neighbour's packed momentum-pass record (mixed form), widened to FP64 for the arithmetic. The
+// three float4 are {rho, q_xx, q_yy, q_xy}, {q_zz, q_yz, q_xz, satu}, {prew, prea, porosity, cop}:
+// the first holds all a 2D particle needs when the out-of-plane factors are zero (rest: false) and
+// the pore terms drop out (pore: false); the records not read are taken as zero.
 struct NbRec {
     double m, r, v[3], q[6], satu, prew, prea, por, cop;
 };
-__device__ __forceinline__ NbRec load_nbrec(const State &S, int j, bool pore = true) {
+__device__ __forceinline__ NbRec load_nbrec(const State &S, int j, bool pore = true, bool rest = true) {
     const float4 a = S.mra[j];
     const float4 *p = S.mrc + 3 * (size_t)j;
-    const float4 b = p[0], c = p[1], d = pore ? p[2] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 b = p[0], c = rest ? p[1] : z, d = pore ? p[2] : z;
     NbRec o;
     o.m = a.x; o.v[0] = a.y; o.v[1] = a.z; o.v[2] = a.w;
-    o.r = b.x; o.q[0] = b.y; o.q[1] = b.z; o.q[2] = b.w;
-    o.q[3] = c.x; o.q[4] = c.y; o.q[5] = c.z; o.satu = c.w;
+    o.r = b.x; o.q[0] = b.y; o.q[1] = b.z; o.q[3] = b.w;
+    o.q[2] = c.x; o.q[4] = c.y; o.q[5] = c.z; o.satu = c.w;
     o.prew = d.x; o.prea = d.y; o.por = d.z; o.cop = d.w;
     return o;
 }
@@ -2429,6 +2436,9 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         #pragma unroll
         for (int k = 0; k < 6; k++) qi[k] = sigi[k] / rhoi / rhoi;
         const double copi = S.cop[i], pori = S.porosity[i];
+        // the xz / yz factors (and the zz one, whose gradient component is +0) are needed in 3D and
+        // once a 2D particle carries out-of-plane shear stress
+        const bool sh3 = NDIM == 3 || *S.shear3 != 0;
         constexpr int B = 1;  // one neighbour per round trip: 3 waves per SIMD instead of 2
         nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
         double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B];
@@ -2439,7 +2449,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             hljb[b] = hl_of(S, P, j, hli);
             fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
             if constexpr (M::kFast) {
-                const NbRec rc = load_nbrec(S, j, !P.dry || drag_on);
+                const NbRec rc = load_nbrec(S, j, !P.dry || drag_on, sh3 || !P.dry || drag_on);
                 mjb[b] = rc.m;
                 rjb[b] = rc.r;
 #pragma unroll
@@ -2456,7 +2466,10 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             #pragma unroll
             for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
 #pragma unroll
-            for (int k = 0; k < 6; k++) qjb[b][k] = S.qsig[k][j];  // sig_j / rho_j / rho_j
+            for (int k = 0; k < 6; k++) {  // sig_j / rho_j / rho_j
+                const bool used = k == 0 || k == 1 || k == 3 || sh3;
+                qjb[b][k] = used ? S.qsig[k][j] : 0.0;
+            }
             satub[b] = 0.0;
             prewb[b] = 0.0;
             preab[b] = 0.0;
@@ -2477,12 +2490,14 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
             pair_kernel<NDIM, M>(P, kc, xi, xj, hli, hlj, w, d2, dst);
             if (tj == ti) {
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                for (int a = 0; a < 3; a++) {
+                    if (a == 2 && !sh3) continue;  // 2D, no out-of-plane shear: the row's terms are +-0
 #pragma unroll
                     for (int b = 0; b < NDIM; b++) {  // 2D: w[2] is +0, the b = 2 sums stay +0
                         const int sx = kSidx[a][b];
                         mom_t[a][b] = mom_t[a][b] + mj * (qj[sx] + qi[sx]) * w[b];
                     }
+                }
                 // pore-pressure term; quirk: the neighbour's saturation is used for both terms. In a
                 // dry context both pressures are +0, the term is +-0 and leaves the sums unchanged.
                 if (P.dry) {

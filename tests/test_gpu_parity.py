@@ -84,6 +84,30 @@ def test_column2d_fields_bit_exact_vs_oracle():
         assert compare(g, o) == {}, f"step {s + 1}"
 
 
+def test_column2d_out_of_plane_shear_vs_oracle():
+    # a 2D state carrying out-of-plane shear stress (sig_yz, sig_xz nonzero on the soil): the
+    # momentum pass then keeps the third row of its stress-divergence sums (it drops them only while
+    # every particle's out-of-plane shear is zero); uploaded, or set after the upload
+    deck, states = load_golden("column2d_small")
+    st = {k: v.copy() for k, v in states[0].items()}
+    soil = deck.type == 2
+    rng = np.random.default_rng(7)
+    st["sig"][soil, 4] = rng.uniform(-50.0, 50.0, soil.sum())
+    st["sig"][soil, 5] = rng.uniform(-50.0, 50.0, soil.sum())
+    for late in (False, True):
+        g = GpuSim(deck)
+        g.load_state(st if not late else states[0])
+        g.upload_done()
+        if late:
+            g.set("sig", st["sig"])
+        g.set_step(1)
+        o = OracleSim(deck, st, mode=JACOBI)
+        for s in range(3):
+            assert g.step(1) == 0 and o.step(1) == 0
+            assert compare(g, o) == {}, (late, s + 1)
+        assert np.abs(g.get("ax")[soil, 2]).max() > 0.0  # the third row is live
+
+
 def test_soilwater3d_vs_oracle():
     deck, states = load_golden("soilwater3d_small")
     g = make_gpu(deck, states[0])
