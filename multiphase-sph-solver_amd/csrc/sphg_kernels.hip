@@ -2444,9 +2444,18 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         double xjb[B][3], hljb[B], mjb[B], rjb[B], vjb[B][3], qjb[B][6], satub[B], prewb[B], preab[B], fdjb[B];
 #pragma unroll
         for (int b = 0; b < B; b++) {
-            const int j = nb_id(e[b]);
-            load_xj<NDIM>(S, cur, j, xjb[b]);
+            const int j = nb_id(e[b]), tjb = nb_type(e[b]);
             hljb[b] = hl_of(S, P, j, hli);
+            xjb[b][0] = xjb[b][1] = xjb[b][2] = mjb[b] = fdjb[b] = satub[b] = prewb[b] = preab[b] = 0.0;
+            rjb[b] = 1.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) vjb[b][k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; k++) qjb[b][k] = 0.0;
+            // wall neighbours enter through the boundary-stress sum after the loops: this loop reads
+            // the neighbours of the particle's own type (and, with interact1, the fluid ones)
+            if (!(tjb == ti || (drag_on && ti == 2 && (tjb == 1 || tjb == 3)))) continue;
+            load_xj<NDIM>(S, cur, j, xjb[b]);
             fdjb[b] = (drag_on && ti == 2) ? S.fdens[j] : 0.0;
             if constexpr (M::kFast) {
                 const NbRec rc = load_nbrec(S, j, !P.dry || drag_on, sh3 || !P.dry || drag_on);
@@ -2901,7 +2910,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
 #pragma unroll
                 for (int k = 0; k < NDIM; k++) vxj[b][k] = S.vx[cur][k][j];
             }
-            mtj[b] = tm ? S.matype[j] : 0;
+            mtj[b] = tm && nb_type(e[b]) == 0 ? S.matype[j] : 0;  // read for wall neighbours only
         }
 #pragma unroll
         for (int b = 0; b < B; b++) {
