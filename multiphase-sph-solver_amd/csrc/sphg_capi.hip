@@ -551,6 +551,8 @@ static int derive_strength_constants(sphg_ctx *c) {
     return SPHG_OK;
 }
 
+static int derive_mass_flags(sphg_ctx *c);
+
 extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host, int n, int ncomp) {
     if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
     if (c->n == 0) c->n = n;
@@ -586,6 +588,8 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
         for (size_t i = 0; i < N && !sh3; i++) sh3 = host[6 * i + 4] != 0.0 || host[6 * i + 5] != 0.0;
         if (sh3) HIPCHK(hipMemcpy(c->S.shear3, &sh3, 4, hipMemcpyHostToDevice));
     }
+    if (nm == "mass" && c->uploaded)
+        if (int rc = derive_mass_flags(c)) return rc;
     if (nm == "ps_c") c->has_psc = true;
     if (nm == "ps_ds") c->has_psds = true;
     if (nm == "ps_cop") c->has_pscop = true;
@@ -770,6 +774,49 @@ static void plan_pre(sphg_ctx *c) {
     if (want == c->pre_on) return;
     c->pre_on = want;
     drop_graphs(c);
+}
+
+// Neighbour-mass table of the pair passes (Params::mass_uniform / mass_t): per particle type 0-4,
+// whether every particle that is of that type, or can become it during the run, carries the same
+// mass bit for bit. Excavation / backfill swaps type and etype, so a particle counts for both;
+// rain gives reserve particles a water mass on the device, so water never qualifies with rain on.
+// Read from the device arrays, so that it also follows sphg_set_field and sphg_load_state.
+static int derive_mass_flags(sphg_ctx *c) {
+    const int n = c->n;
+    Params &P = c->P;
+    std::vector<double> m(n);
+    std::vector<int> ty(n), et(n);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(m.data(), c->S.mass, (size_t)n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ty.data(), c->S.type, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(et.data(), c->S.etype, (size_t)n * 4, hipMemcpyDeviceToHost));
+    int seen = 0, mixed = 0;
+    double mt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    auto add = [&](int t, double v) {
+        if (t < 0 || t > 4) return;
+        if (!((seen >> t) & 1)) {
+            mt[t] = v;
+            seen |= 1 << t;
+        } else if (memcmp(&v, &mt[t], 8) != 0) {
+            mixed |= 1 << t;
+        }
+    };
+    for (int i = 0; i < n; i++) {
+        add(ty[i], m[i]);
+        if (P.fleob) add(et[i], m[i]);
+    }
+    if (P.flr) mixed |= 1 << 1;
+    const int mask = seen & ~mixed;
+    bool same = mask == P.mass_uniform;
+    for (int t = 0; t < 5; t++) {
+        const double v = ((mask >> t) & 1) ? mt[t] : 0.0, vf = (double)(float)v;
+        same = same && memcmp(&v, &P.mass_t.t[t], 8) == 0;
+        P.mass_t.t[t] = v;
+        P.mass_tf.t[t] = vf;
+    }
+    P.mass_uniform = mask;
+    if (!same) drop_graphs(c);  // the captured launches hold the old parameters
+    return SPHG_OK;
 }
 
 extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
@@ -970,6 +1017,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
         }
     }
     HIPCHK(hipMemcpy(c->S.shear3, &sh3, 4, hipMemcpyHostToDevice));
+    if ((rc = derive_mass_flags(c))) return rc;
     c->S.ps_c = c->has_psc ? c->d_psc : nullptr;
     int zero = 0;
     HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));
@@ -1477,7 +1525,7 @@ extern "C" int sphg_load_state(sphg_ctx *c, const char *path) {
     c->stepped = h.stepped != 0;
     c->eig_stale = false;  // the file was written with strep / beta of its stress
     c->has_psc = h.has_psc != 0;
-    return SPHG_OK;
+    return derive_mass_flags(c);  // the file's masses
 }
 
 extern "C" int sphg_set_vibration(sphg_ctx *c, double ttime, int nsteps, double sttime, double edtime,

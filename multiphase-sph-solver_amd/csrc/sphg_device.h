@@ -45,6 +45,11 @@ struct Grid {
     int nb_off[27];          // stencil offsets in the reference's nncell order
 };
 
+// Common neighbour mass of each particle type 0-4.
+struct MassTable {
+    double t[5];
+};
+
 struct Params {
     int n, cap, ndim, flbndy, flknl, fli, if_move, flv, inip, ss_type, flts;
     int ncell;            // cells of all realisations (each grid's own ids run 1..ctotal)
@@ -84,6 +89,11 @@ struct Params {
     // momentum sums are +-0 and their gathers are skipped
     int dry;
     double dt_cour, dt_cwave, dt_max;
+    // neighbour masses: bit t set when every particle that is, or can become, of type t (0-4)
+    // carries the mass mass_t[t], bit for bit; the pair passes then use mass_t[t] (mass_tf[t], the
+    // FP64 value of its FP32 rounding, where they read the mixed form's FP32 record) with no gather
+    int mass_uniform;
+    MassTable mass_t, mass_tf;
 };
 
 // All device arrays of one context (pointers into one HBM allocation).

@@ -306,6 +306,59 @@ __device__ __forceinline__ void pair_kernel(const Params &P, const KernelCtx &kc
 // hl of neighbour j: equal to hl_i for every pair (uniform smoothing length, checked by the host)
 __device__ __forceinline__ double hl_of(const State &, const Params &, int, double hli) { return hli; }
 
+// Mass of neighbour j of type tj (from its list entry). Where the host found every particle that is
+// or can become of a type to carry the same mass bits (Params::mass_uniform), that common value
+// replaces the gather for the two types a particle mostly sums over: the wave's own type (that of
+// its first active lane; particles of one phase are stored together) and walls. The values are
+// chosen once per wave in scalar registers; the gather remains for the lanes that need it, and is
+// skipped when none of the wave's lanes does (a branch on a wave-wide vote).
+struct MassPick {
+    int ta, ua, ub;   // the wave's type, whether its mass and the walls' are common
+    double ma, mb;    // FP64 masses of type ta and of walls
+    double fa, fb;    // the same rounded to FP32 (the mixed form's records)
+};
+__device__ __forceinline__ MassPick mass_pick(const Params &P, int ti) {
+    MassPick p;
+    const int ta = __builtin_amdgcn_readfirstlane(ti);
+    double m0 = P.mass_t.t[0], m1 = P.mass_t.t[1], m2 = P.mass_t.t[2], m3 = P.mass_t.t[3], m4 = P.mass_t.t[4];
+    double f0 = P.mass_tf.t[0], f1 = P.mass_tf.t[1], f2 = P.mass_tf.t[2], f3 = P.mass_tf.t[3], f4 = P.mass_tf.t[4];
+    // opaque values: the selects below must not fold back into an indexed load of the kernel
+    // argument, which would need a private copy of it
+    asm("" : "+s"(m0), "+s"(m1), "+s"(m2), "+s"(m3), "+s"(m4));
+    asm("" : "+s"(f0), "+s"(f1), "+s"(f2), "+s"(f3), "+s"(f4));
+    p.ta = ta;
+    p.ua = ta >= 0 && ta <= 4 && ((P.mass_uniform >> ta) & 1);
+    p.ub = P.mass_uniform & 1;
+    p.ma = ta == 2 ? m2 : ta == 0 ? m0 : ta == 1 ? m1 : ta == 3 ? m3 : m4;
+    p.fa = ta == 2 ? f2 : ta == 0 ? f0 : ta == 1 ? f1 : ta == 3 ? f3 : f4;
+    p.mb = m0;
+    p.fb = f0;
+    return p;
+}
+__device__ __forceinline__ double nb_mass(const State &S, const MassPick &p, int j, int tj) {
+    const bool u = (tj == p.ta && p.ua) || (tj == 0 && p.ub);
+    double ma = p.ma, mb = p.mb;
+    asm("" : "+s"(ma), "+s"(mb));  // register values, so that the select is not folded into a load
+    double m = tj == 0 ? mb : ma;
+    if (__builtin_amdgcn_ballot_w64(!u) != 0) {
+        const double g = S.mass[j];
+        if (!u) m = g;
+    }
+    return m;
+}
+// The same for the mixed form's FP32 mass (the first member of mra).
+__device__ __forceinline__ double nb_mass_f(const State &S, const MassPick &p, int j, int tj) {
+    const bool u = (tj == p.ta && p.ua) || (tj == 0 && p.ub);
+    double fa = p.fa, fb = p.fb;
+    asm("" : "+s"(fa), "+s"(fb));
+    double m = tj == 0 ? fb : fa;
+    if (__builtin_amdgcn_ballot_w64(!u) != 0) {
+        const double g = S.mra[j].x;
+        if (!u) m = g;
+    }
+    return m;
+}
+
 // ------------------------------------------------------------------------------------------
 // NNPS: cell binning (parttocell, CPP_3-NNPS.cpp:175-263)
 // ------------------------------------------------------------------------------------------
@@ -667,6 +720,7 @@ __global__ __launch_bounds__(256, (NDIM == 2 && BW == 1 && !M::kKgc) ? 7 : 1) vo
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     const int l = step_of(S);
     if (ti == 7) {
         S.rho_mid[i] = S.rho[i];
@@ -699,7 +753,7 @@ __global__ __launch_bounds__(256, (NDIM == 2 && BW == 1 && !M::kKgc) ? 7 : 1) vo
                 mj[b] = ra.x;
                 nb_vel_mixed<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, ra, vj[b]);
             } else {
-                mj[b] = S.mass[j];
+                mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
                 nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
             }
             porj[b] = 0.0;
@@ -766,6 +820,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
@@ -780,7 +835,7 @@ __global__ __launch_bounds__(256) void k_norm(State S, Params P, int cur) {
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xj[b]);
             hlj[b] = hl_of(S, P, j, hli);
-            mj[b] = S.mass[j];
+            mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
             rmj[b] = S.rho_mid[j];
         }
 #pragma unroll
@@ -824,6 +879,7 @@ template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n || S.type[i] != 2) return;
+    const MassPick mp = mass_pick(P, 2);
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
@@ -840,7 +896,7 @@ __global__ __launch_bounds__(256) void k_saturation(State S, Params P, int cur) 
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xj[b]);
             hlj[b] = hl_of(S, P, j, hli);
-            mj[b] = S.mass[j];
+            mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
             rj[b] = S.rho[j];
         }
 #pragma unroll
@@ -878,6 +934,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     const int l = step_of(S);
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
@@ -900,18 +957,28 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
         // boundary particles: eos loop 2 only (CPP_7-1-eos_CR.cpp:62-118): water / air pressure and
         // soil stress interpolated onto the wall
         constexpr int B = 2;
-        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        // a wall whose list holds no water, soil or air particle has empty sums; the others read
+        // the fluid neighbours' pressure and the soil neighbours' stress only
+        const int nc_wall = (S.nb_types[i] & 0xE) ? nc : 0;
+        nb_batches<B>(S, P, i, nc_wall, [&](const int (&e)[B], int act) {
             double xj[B][3], hlj[B], mj[B], rj[B], pj[B], sgj[B][6];
 #pragma unroll
             for (int b = 0; b < B; b++) {
-                const int j = nb_id(e[b]);
-                load_xj<NDIM>(S, cur, j, xj[b]);
+                const int j = nb_id(e[b]), tj = nb_type(e[b]);
                 hlj[b] = hl_of(S, P, j, hli);
-                mj[b] = S.mass[j];
-                rj[b] = S.rho[j];
-                pj[b] = S.pre[j];
+                xj[b][0] = xj[b][1] = xj[b][2] = mj[b] = pj[b] = 0.0;
+                rj[b] = 1.0;
                 #pragma unroll
-                for (int k = 0; k < 6; k++) sgj[b][k] = S.sig[k][j];
+                for (int k = 0; k < 6; k++) sgj[b][k] = 0.0;
+                if (tj != 1 && tj != 2 && tj != 3) continue;
+                load_xj<NDIM>(S, cur, j, xj[b]);
+                mj[b] = nb_mass(S, mp, j, tj);
+                rj[b] = S.rho[j];
+                if (tj != 2) pj[b] = S.pre[j];
+                else {
+                    #pragma unroll
+                    for (int k = 0; k < 6; k++) sgj[b][k] = S.sig[k][j];
+                }
             }
 #pragma unroll
             for (int b = 0; b < B; b++) {
@@ -953,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
                     nb_vel_mixed<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, ra, vj[b]);
                     pj[b] = solid ? (double)rn.y : 0.0;
                 } else {
-                    mj[b] = S.mass[j];
+                    mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
                     rj[b] = S.rho[j];
                     nb_vel<NDIM>(S, j, nb_type(e[b]) == 0, mode, ph, vi, vj[b]);
                     pj[b] = solid && !P.dry ? S.pre[j] : 0.0;  // read by the eos sums of fluid neighbours only
@@ -1823,6 +1890,7 @@ __global__ __launch_bounds__(256) void k_regu(State S, Params P, int cur, int wh
     int i = pidx();
     if (i >= P.n || (step_of(S) % P.step_regu) != 0) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     const int nc = nb_count(S, i);
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
@@ -1837,7 +1905,7 @@ __global__ __launch_bounds__(256) void k_regu(State S, Params P, int cur, int wh
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xj[b]);
             hlj[b] = hl_of(S, P, j, hli);
-            mj[b] = S.mass[j];
+            mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
             rj[b] = S.rho[j];
             #pragma unroll
             for (int k = 0; k < 6; k++) vj[b][k] = S.tmp6[k][j];
@@ -1930,6 +1998,7 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     const bool art_on = P.art_stre_coe > 0.0 && ti == 2;
     const bool damp_on = (ti == 2 || ti == 4) && step_of(S) <= P.inip;
     if (!art_on && !damp_on) return;
@@ -1951,7 +2020,7 @@ __global__ __launch_bounds__(256) void k_accel_pre(State S, Params P, int cur) {
         const int j = nb_id(e[b]);
         load_xj<NDIM>(S, cur, j, xjb[b]);
         hljb[b] = hl_of(S, P, j, hli);
-        mjb[b] = S.mass[j];
+        mjb[b] = nb_mass(S, mp, j, nb_type(e[b]));
         rjb[b] = S.rho[j];
         if (art_on)
             for (int k = 0; k < 9; k++) Rjb[b][k] = S.rij[k][j];
@@ -2105,6 +2174,7 @@ __device__ __forceinline__ void accel_vel(const State &S, const Params &P, int i
 template <int NDIM, class M>
 __device__ __forceinline__ void bndy_stress_rows(const State &S, const Params &P, int cur, int i, int ti,
                                                  const double xi[3], const KernelCtx &kc, double rows[3]) {
+    const MassPick mp = mass_pick(P, ti);
     const bool tran = tran_bndy(P);
     double ai[6];
 #pragma unroll
@@ -2137,7 +2207,7 @@ __device__ __forceinline__ void bndy_stress_rows(const State &S, const Params &P
         for (int b = 0; b < B; b++) {
             const int j = nb_id(e[b]);
             load_xj<NDIM>(S, cur, j, xj[b]);
-            mj[b] = S.mass[j];
+            mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
             rj[b] = S.rho[j];
         }
 #pragma unroll
@@ -2298,6 +2368,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     double xi[3], xj[3], dx[3], w[4];
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
@@ -2369,7 +2440,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 copjb[b] = drag_on ? rc.cop : 1.0;
                 continue;
             }
-            mjb[b] = S.mass[j];
+            mjb[b] = nb_mass(S, mp, j, nb_type(e[b]));
             rjb[b] = S.rho[j];
             #pragma unroll
             for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
@@ -2470,7 +2541,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 preab[b] = rc.prea;
                 continue;
             }
-            mjb[b] = S.mass[j];
+            mjb[b] = nb_mass(S, mp, j, nb_type(e[b]));
             rjb[b] = S.rho[j];
             #pragma unroll
             for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
@@ -2559,7 +2630,7 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
                 const int j = nb_id(e[b]);
                 load_xj<NDIM>(S, cur, j, xj[b]);
                 hlj[b] = hl_of(S, P, j, hli);
-                mj[b] = S.mass[j];
+                mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
                 rj[b] = S.rho[j];
                 #pragma unroll
                 for (int k = 0; k < 3; k++) vj[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
@@ -2863,6 +2934,7 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     const int nxt = cur ^ 1;
     double vi[3] = {S.vx[cur][0][i], S.vx[cur][1][i], S.vx[cur][2][i]};
     if (ti == 0 || ti == 7) {
@@ -2899,13 +2971,13 @@ __global__ __launch_bounds__(256) void k_xsph(State S, Params P, int cur) {
             hlj[b] = hl_of(S, P, j, hli);
             if (M::kFast && P.mrv_ok) {
                 const float4 rv = S.mrv[j];
-                mj[b] = S.mra[j].x;
+                mj[b] = nb_mass_f(S, mp, j, nb_type(e[b]));
                 rj[b] = rv.w;
                 vxj[b][0] = rv.x;
                 vxj[b][1] = rv.y;
                 if (NDIM == 3) vxj[b][2] = rv.z;
             } else {
-                mj[b] = S.mass[j];
+                mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
                 rj[b] = S.rho[j];
 #pragma unroll
                 for (int k = 0; k < NDIM; k++) vxj[b][k] = S.vx[cur][k][j];
@@ -3163,6 +3235,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    const MassPick mp = mass_pick(P, ti);
     const int nxt = cur ^ 1;
     const int l = step_of(S);
     const double dt = *S.dtp;
@@ -3182,7 +3255,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
                 const int j = nb_id(e[b]);
                 load_xj<NDIM>(S, cur, j, xj[b]);
                 hlj[b] = hl_of(S, P, j, hli);
-                mj[b] = S.mass[j];
+                mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
                 rj[b] = S.rho[j];
                 #pragma unroll
                 for (int k = 0; k < 3; k++) vbj[b][k] = S.vel_bndy[k][j];
