@@ -5,6 +5,7 @@ device and host code: the FP64 kernels are meant to reproduce the reference's CP
 """
 from __future__ import annotations
 
+import json
 import os
 import re
 import subprocess
@@ -77,6 +78,9 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"build failed: {' '.join(c[:3])} ...")
         if any("kernel-resource-usage" in a for a in c):
             check_no_scratch(r.stderr)
+            # per-kernel VGPRs / occupancy of this build, for reading beside the profiles
+            with open(os.path.join(OUT_DIR, "kernel_resources.json"), "w") as f:
+                json.dump(kernel_resources(r.stderr), f, indent=1, sort_keys=True)
     os.replace(LIB + ".tmp", LIB)
     # the project runner executable, linked against the library beside it
     r = subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "sphg_main.cpp"), "-o", SOLVE,

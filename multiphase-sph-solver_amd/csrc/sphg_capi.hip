@@ -1152,6 +1152,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         if (tran_bndy(c->P)) LAUNCH("slip_tran", (k_slip_tran<NDIM, M>), grid_for(n), c->S, c->P, cur);
     }
     if (phase_mask & 64) {
+        if (c->P.flbndy == 2) LAUNCH("wall_vb", (k_wall_vb<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("update", (k_update<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.flr == 1) LAUNCH("partidel", k_partidel, grid_for(n), c->S, c->P, cur ^ 1);
         LAUNCH("step_inc", k_step_inc, 1, c->S);

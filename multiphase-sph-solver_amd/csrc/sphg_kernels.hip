@@ -955,8 +955,9 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     const int mode = bvel_mode(P, ti), ph = ti - 1;
     if (ti == 0) {
         // boundary particles: eos loop 2 only (CPP_7-1-eos_CR.cpp:62-118): water / air pressure and
-        // soil stress interpolated onto the wall
-        constexpr int B = 2;
+        // soil stress interpolated onto the wall, one neighbour per round trip (the six stress
+        // components of a second one would set the register peak of the whole kernel)
+        constexpr int B = 1;
         // a wall whose list holds no water, soil or air particle has empty sums; the others read
         // the fluid neighbours' pressure and the soil neighbours' stress only
         const int nc_wall = (S.nb_types[i] & 0xE) ? nc : 0;
@@ -3228,34 +3229,42 @@ __global__ __launch_bounds__(256) void k_slip_tran(State S, Params P, int cur) {
 }
 
 // ------------------------------------------------------------------------------------------
-// tm_boundary second loop + check_domain + update
+// tm_boundary second loop; check_domain + update
 // ------------------------------------------------------------------------------------------
+// tm_boundary_2d / 3d second loop (CPP_13-boundary.cpp:282-385 and its 3D copy), flbndy 2: each wall particle's
+// per-phase virtual velocity vb from its neighbours' boundary velocities (vel_bndy, written by the
+// XSPH pass). Its own launch ahead of the update, which neither reads vb nor writes what this loop
+// reads, so that the per-particle update does not carry the registers of the neighbour loop.
 template <int NDIM, class M>
-__global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
+__global__ __launch_bounds__(256) void k_wall_vb(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    if (ti != 0) return;
     const MassPick mp = mass_pick(P, ti);
-    const int nxt = cur ^ 1;
-    const int l = step_of(S);
-    const double dt = *S.dtp;
     double xi[3];
     load_x<NDIM>(S, cur, i, xi);
-    if (ti == 0 && P.flbndy == 2) {
+    {
         const double hli = S.hl[i];
-    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
+        const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
         const int nc = nb_count(S, i);
         double tv[4][3], td[4] = {0, 0, 0, 0};
         for (int a = 0; a < 4; a++) tv[a][0] = tv[a][1] = tv[a][2] = 0.0;
         constexpr int B = 4;
-        nb_batches<B>(S, P, i, nc, [&](const int (&e)[B], int act) {
+        // only water, soil, air and structure neighbours enter the sums: a wall whose list holds
+        // none has empty sums, and the other walls gather nothing for their wall neighbours
+        const int nc_wall = (S.nb_types[i] & 0x1E) ? nc : 0;
+        nb_batches<B>(S, P, i, nc_wall, [&](const int (&e)[B], int act) {
             double xj[B][3], hlj[B], mj[B], rj[B], vbj[B][3];
 #pragma unroll
             for (int b = 0; b < B; b++) {
-                const int j = nb_id(e[b]);
-                load_xj<NDIM>(S, cur, j, xj[b]);
+                const int j = nb_id(e[b]), tj = nb_type(e[b]);
                 hlj[b] = hl_of(S, P, j, hli);
-                mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
+                xj[b][0] = xj[b][1] = xj[b][2] = mj[b] = vbj[b][0] = vbj[b][1] = vbj[b][2] = 0.0;
+                rj[b] = 1.0;
+                if (tj < 1 || tj > 4) continue;
+                load_xj<NDIM>(S, cur, j, xj[b]);
+                mj[b] = nb_mass(S, mp, j, tj);
                 rj[b] = S.rho[j];
                 #pragma unroll
                 for (int k = 0; k < 3; k++) vbj[b][k] = S.vel_bndy[k][j];
@@ -3278,6 +3287,18 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
             #pragma unroll
             for (int k = 0; k < 3; k++) S.vb[t][k][i] = fabs(td[t]) > 1.0e-6 ? tv[t][k] / td[t] : 0.0;
     }
+}
+
+template <int NDIM, class M>
+__global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
+    int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    const int nxt = cur ^ 1;
+    const int l = step_of(S);
+    const double dt = *S.dtp;
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
     double v[3] = {S.vx[nxt][0][i], S.vx[nxt][1][i], S.vx[nxt][2][i]};
     if (ti != 0 && ti != 7) {
         // check_domain with the serial meaning of its shared flag (quirk A3)
@@ -3315,8 +3336,8 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
         }
         if (NDIM == 2) S.xp[nxt][2][i] = S.xp[cur][2][i];
         S.rhop[i] = S.rho[i];
-        if constexpr (M::kFast)
-            S.mra[i] = make_float4((float)S.mass[i], (float)S.vxp[0][i], (float)S.vxp[1][i], (float)S.vxp[2][i]);
+        if constexpr (M::kFast)  // the velocity just stored (vxp's third component stays in 2D)
+            S.mra[i] = make_float4((float)S.mass[i], (float)v[0], (float)v[1], (float)(NDIM == 3 ? v[2] : S.vxp[2][i]));
     } else {
         int mt = S.matype[i];
         if (mt >= 2 && mt <= 5) {
