@@ -501,6 +501,11 @@ __global__ __launch_bounds__(256) void k_cell_order(State S, int ncells) {
 // not stored: each pair pass recomputes them from the positions (pair_kernel), which costs a
 // square root and a few divisions per pair instead of 32 B of HBM traffic per pair and pass.
 constexpr int kSearchBlock = 256;
+// Candidates per load round trip: 4 in 3D, where a cell holds about 8 particles (197 -> 190 us on
+// the 3D soil-water column); 1 in 2D, where a cell holds at most 4 and padded batches cost more
+// than they hide (slope batch 1.19 -> 1.30 ms at 4).
+template <int NDIM>
+constexpr int kSearchBatch = NDIM == 3 ? 4 : 1;
 
 // STAGE: list entries staged per thread (16 or 32). The LDS it takes bounds the resident waves (8 or 5
 // per SIMD), so the better depth depends on the deck's list length and size (sphg_autotune picks it).
@@ -522,27 +527,39 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
             const int nll = c + g.nb_off[s];
             if (nll <= 0 || nll > g.ncell) continue;
             const int nl = g.base + nll;
-            int st = S.cell_start[nl], en = st + S.cell_count[nl];
-            for (int q = st; q < en; q++) {
-                const int e = S.sorted[q];
-                const int j = nb_id(e);
-                double d2 = 0.0;
+            const int st = S.cell_start[nl], en = st + S.cell_count[nl];
+            // the cell's candidates kSearchBatch at a time: their ids in one round trip, then their
+            // positions in one more, then the tests in cell order
+            for (int q0 = st; q0 < en; q0 += kSearchBatch<NDIM>) {
+                int eb[kSearchBatch<NDIM>];
+                double xb[kSearchBatch<NDIM>][3];
 #pragma unroll
-                for (int k = 0; k < NDIM; k++) {
-                    double d = xi[k] - S.xp[cur][k][j];
-                    d2 = d2 + d * d;
-                }
-                double R = 2 * ((hli + hl_of(S, P, j, hli)) / 2);
-                double R2 = R * R;
-                bool in;
-                if (d2 < R2 * 0.99999999999999) in = true;
-                else if (d2 > R2 * 1.00000000000001) in = false;
-                else in = sqrt(d2) <= R;
-                if (in && i != j) {
-                    if (cnt < STAGE) stage[cnt][t] = e;
-                    else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = e;
-                    tmask |= 1 << nb_type(e);
-                    cnt++;
+                for (int u = 0; u < kSearchBatch<NDIM>; u++) eb[u] = S.sorted[q0 + u < en ? q0 + u : en - 1];
+#pragma unroll
+                for (int u = 0; u < kSearchBatch<NDIM>; u++) load_xj<NDIM>(S, cur, nb_id(eb[u]), xb[u]);
+#pragma unroll
+                for (int u = 0; u < kSearchBatch<NDIM>; u++) {
+                    if (q0 + u >= en) continue;
+                    const int e = eb[u];
+                    const int j = nb_id(e);
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int k = 0; k < NDIM; k++) {
+                        double d = xi[k] - xb[u][k];
+                        d2 = d2 + d * d;
+                    }
+                    double R = 2 * ((hli + hl_of(S, P, j, hli)) / 2);
+                    double R2 = R * R;
+                    bool in;
+                    if (d2 < R2 * 0.99999999999999) in = true;
+                    else if (d2 > R2 * 1.00000000000001) in = false;
+                    else in = sqrt(d2) <= R;
+                    if (in && i != j) {
+                        if (cnt < STAGE) stage[cnt][t] = e;
+                        else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = e;
+                        tmask |= 1 << nb_type(e);
+                        cnt++;
+                    }
                 }
             }
         }
