@@ -3267,7 +3267,7 @@ __global__ __launch_bounds__(256) void k_wall_vb(State S, Params P, int cur) {
         const int nc = nb_count(S, i);
         double tv[4][3], td[4] = {0, 0, 0, 0};
         for (int a = 0; a < 4; a++) tv[a][0] = tv[a][1] = tv[a][2] = 0.0;
-        constexpr int B = 4;
+        constexpr int B = 1;  // one neighbour per round trip: 6-7 waves per SIMD (B = 4: 3-4 waves)
         // only water, soil, air and structure neighbours enter the sums: a wall whose list holds
         // none has empty sums, and the other walls gather nothing for their wall neighbours
         const int nc_wall = (S.nb_types[i] & 0x1E) ? nc : 0;
