@@ -223,6 +223,16 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
         else S.rij[k] = nullptr;
     }
     for (int k = 0; k < 6; k++) D(S.tmp6[k]);
+    const bool split_brow = pro->ndim == 2 && precision != SPHG_MIXED && pro->flknl <= 2;
+    for (int k = 0; k < 3; k++) {
+        if (split_brow) D(S.brow[k]);
+        else S.brow[k] = nullptr;
+    }
+    S.nw_list = S.nw_count = nullptr;
+    if (split_brow) {
+        I(S.nw_list, cap);
+        I(S.nw_count, 1);
+    }
     for (int q = 0; q < 3; q++) {
         if (pro->ss_type >= 1 && pro->ss_type <= 3) {
             I(S.ssi_id[q], cap);
@@ -1069,6 +1079,15 @@ static const int kBwChoices[4][3] = {{1, 2, 4}, {2, 4, 6}, {1, 2, 4}, {1, 2, 4}}
     } while (0)
 
 template <int NDIM>
+static void enqueue_search(sphg_ctx *c) {
+    const int cur = c->cur, n = c->n;
+    // the search appends the near-wall particles of the split momentum pass to a list it starts empty
+    if (c->S.nw_count) LAUNCH("nw_reset", k_fill_int, 1, c->S.nw_count, 1, 0);
+    if (c->search_stage == 16) LAUNCH("search", (k_search<NDIM, 16>), grid_for(n), c->S, c->P, cur);
+    else LAUNCH("search", (k_search<NDIM, 32>), grid_for(n), c->S, c->P, cur);
+}
+
+template <int NDIM>
 static void enqueue_nnps(sphg_ctx *c) {
     const int cur = c->cur, n = c->n, M = c->P.ncell + 2;
     LAUNCH("fill_counts", k_fill_int, grid_for(M), c->S.cell_count, M, 0);
@@ -1083,8 +1102,7 @@ static void enqueue_nnps(sphg_ctx *c) {
     LAUNCH("scan_add", k_scan_add, c->scan_blocks, c->S.cell_start, c->S.block_sums, M);
     LAUNCH("scatter", k_scatter, grid_for(n), c->S, c->P);
     LAUNCH("cell_order", k_cell_order, grid_for(M), c->S, M);
-    if (c->search_stage == 16) LAUNCH("search", (k_search<NDIM, 16>), grid_for(n), c->S, c->P, cur);
-    else LAUNCH("search", (k_search<NDIM, 32>), grid_for(n), c->S, c->P, cur);
+    enqueue_search<NDIM>(c);
     if (c->rev_lists) LAUNCH("reverse_lists", k_reverse_lists, grid_for(n), c->S, c->P);
     if (c->P.flknl > 2) LAUNCH("kgc", k_kgc<NDIM>, grid_for(n), c->S, c->P, cur);
 }
@@ -1137,6 +1155,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         if (c->P.coupled) LAUNCH("coupled", (k_coupled<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.fli >= 2) LAUNCH("drag", (k_drag<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.flv > 0) LAUNCH("vib", (k_vib<NDIM, M>), grid_for(n), c->S, c->P, cur);
+        if constexpr (kSplitBrow<NDIM, M>) LAUNCH("bndy_rows", (k_bndy_rows<NDIM, M>), grid_for(n), c->S, c->P, cur);
         LAUNCH("accel", (k_accel<NDIM, M>), grid_for(n), c->S, c->P, cur);
         if (c->P.flv == 1) LAUNCH("vib_walls_acc", k_vib_walls_acc<NDIM>, grid_for(n), c->S, c->P);
         if (c->P.ss_type != 0) {
@@ -1238,14 +1257,8 @@ extern "C" int sphg_parttocell(sphg_ctx *c) {
 }
 extern "C" int sphg_partisearching(sphg_ctx *c) {
     if (!c || !c->uploaded) return SPHG_E_ARG;
-    const int n = c->n, cur = c->cur;
-    auto go = [&](auto ndim_tag) {
-        constexpr int NDIM = decltype(ndim_tag)::value;
-        if (c->search_stage == 16) LAUNCH("search", (k_search<NDIM, 16>), grid_for(n), c->S, c->P, cur);
-        else LAUNCH("search", (k_search<NDIM, 32>), grid_for(n), c->S, c->P, cur);
-    };
-    if (c->P.ndim == 2) go(std::integral_constant<int, 2>());
-    else go(std::integral_constant<int, 3>());
+    if (c->P.ndim == 2) enqueue_search<2>(c);
+    else enqueue_search<3>(c);
     finish_timing(c);
     return read_err(c);
 }

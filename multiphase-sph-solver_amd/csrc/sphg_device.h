@@ -143,6 +143,9 @@ struct State {
     float2 *mrn;
     double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
     double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
+    double *brow[3];      // 2D FP64: boundary-stress rows of the momentum sums, from their own pass
+    int *nw_list;         // ... over the particles the search found next to a wall, and their
+    int *nw_count;        // number (cleared before every search)
     // soil-structure interaction (ss_type 1-3): per soil / water particle the structure particles
     // its contact force is passed to (-1: none) and what each receives (f_n[3], f_t[3])
     int *ssi_id[3];

@@ -520,7 +520,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     load_x<NDIM>(S, cur, i, xi);
     const double hli = S.hl[i];
     const KernelCtx kc = kernel_ctx<NDIM>(P, hli);
-    if (S.type[i] != 7 && S.cell[i] != 0) {
+    const int ti = S.type[i];
+    if (ti != 7 && S.cell[i] != 0) {
         const Grid &g = S.grids[S.rid[i]];
         const int c = S.cell[i] - g.base;  // cell id within the particle's own grid
         for (int s = 0; s < g.nstencil; s++) {
@@ -567,6 +568,19 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     }
     S.nb_cnt[i] = cnt;
     S.nb_types[i] = tmask;
+    if (S.nw_list) {
+        // non-wall particles with a wall neighbour, appended with one atomic add per wave
+        const bool nw = ti != 0 && (tmask & 1);
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(nw);
+        if (m) {
+            const int lane = __lane_id(), first = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == first) base = atomicAdd(S.nw_count, __popcll(m));
+            base = __shfl(base, first);
+            const int pos = base + __popcll(m & ((1ull << lane) - 1));
+            if (nw && pos < P.n) S.nw_list[pos] = i;  // at most one entry per particle and search
+        }
+    }
     const int nk = cnt < STAGE ? cnt : STAGE;
     for (int k = 0; k < nk; k++) S.nbr[(size_t)k * P.cap + i] = stage[k][t];
 }
@@ -2223,9 +2237,12 @@ __device__ __forceinline__ void bndy_stress_rows(const State &S, const Params &P
         double xj[B][3], mj[B], rj[B];
 #pragma unroll
         for (int b = 0; b < B; b++) {
-            const int j = nb_id(e[b]);
+            const int j = nb_id(e[b]), tj = nb_type(e[b]);
+            xj[b][0] = xj[b][1] = xj[b][2] = mj[b] = 0.0;
+            rj[b] = 1.0;
+            if (tj != 0) continue;  // the sum runs over wall neighbours only
             load_xj<NDIM>(S, cur, j, xj[b]);
-            mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
+            mj[b] = nb_mass(S, mp, j, tj);
             rj[b] = S.rho[j];
         }
 #pragma unroll
@@ -2381,8 +2398,31 @@ __global__ __launch_bounds__(256) void k_drag(State S, Params P, int cur) {
     for (int k = 0; k < NDIM; k++) S.interf[k][i] = tu[k];
 }
 
+// 2D FP64 (without the kernel gradient correction): the boundary-stress rows come from their own
+// pass (k_bndy_rows), so that the momentum pass fits four waves per SIMD (128 VGPRs) instead of
+// three; elsewhere the momentum pass forms them itself, where the split would not raise its waves.
 template <int NDIM, class M>
-__global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
+constexpr bool kSplitBrow = NDIM == 2 && !M::kFast && !M::kKgc;
+
+template <int NDIM, class M>
+__global__ __launch_bounds__(256) void k_bndy_rows(State S, Params P, int cur) {
+    // over the search's list of non-wall particles with a wall neighbour, so that its waves are full
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= *S.nw_count || q >= P.n) return;
+    const int i = S.nw_list[q];
+    const int ti = S.type[i];
+    double xi[3];
+    load_x<NDIM>(S, cur, i, xi);
+    const double hli = S.hl[i];
+    const KernelCtx kc = kernel_ctx<NDIM, M>(P, hli, S, i);
+    double brow[3] = {0.0, 0.0, 0.0};
+    bndy_stress_rows<NDIM, M>(S, P, cur, i, ti, xi, kc, brow);
+#pragma unroll
+    for (int k = 0; k < 3; k++) S.brow[k][i] = brow[k];
+}
+
+template <int NDIM, class M>
+__global__ __launch_bounds__(256, (kSplitBrow<NDIM, M> ? 4 : 1)) void k_accel(State S, Params P, int cur) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
@@ -2663,7 +2703,14 @@ __global__ __launch_bounds__(256) void k_accel(State S, Params P, int cur) {
         });
     }
     double brow[3] = {0.0, 0.0, 0.0};
-    if (ti != 0 && ti != 7 && (S.nb_types[i] & 1)) bndy_stress_rows<NDIM, M>(S, P, cur, i, ti, xi, kc, brow);
+    if (ti != 0 && ti != 7 && (S.nb_types[i] & 1)) {
+        if constexpr (kSplitBrow<NDIM, M>) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) brow[k] = S.brow[k][i];
+        } else {
+            bndy_stress_rows<NDIM, M>(S, P, cur, i, ti, xi, kc, brow);
+        }
+    }
     // combine in the reference's call order (CPP_1-SPHLoops.cpp:586-660)
     double ax[3] = {S.ax[0][i], S.ax[1][i], S.ax[2][i]};
     const double g[3] = {P.gx, P.gy, P.gz};
