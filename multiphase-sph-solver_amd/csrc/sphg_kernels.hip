@@ -397,7 +397,23 @@ __global__ __launch_bounds__(256) void k_bin(State S, Params P, int cur) {
         c = 0;
     }
     S.cell[i] = c;
-    S.cell_rank[i] = atomicAdd(&S.cell_count[c], 1);
+    // A lattice puts runs of consecutive particles into one cell: each run of equal cells among the
+    // wave's lanes takes its ranks with one atomic add by its first lane (end: one past the run's
+    // last lane). The active lanes are a prefix of the wave (consecutive ids), so lane - 1 of an
+    // active lane is active.
+    const int lane = __lane_id();
+    const int prev = __shfl_up(c, 1);
+    const bool head = lane == 0 || prev != c;
+    const unsigned long long heads = __builtin_amdgcn_ballot_w64(head);
+    const unsigned long long active = __builtin_amdgcn_ballot_w64(true);
+    const int h = 63 - __clzll(heads & ((2ull << lane) - 1));  // this lane's run head
+    const unsigned long long above = heads & ~((2ull << h) - 1);
+    const int end = above ? __ffsll((long long)above) - 1 : __popcll(active);
+    int base = 0;
+    if (head) base = atomicAdd(&S.cell_count[c], end - lane);
+    base = __shfl(base, h);
+    // ranks descending in id within the run: the order k_cell_order establishes in every cell
+    S.cell_rank[i] = base + (end - 1 - lane);
 }
 
 // Exclusive scan of cell counts: block-local pass, top pass over block sums, add pass.
