@@ -254,7 +254,7 @@ static int create_common(sphg_ctx **out, const sphg_problem *pro, const sphg_mat
         }
     I(S.cell_count, ncellp); I(S.cell_start, ncellp); I(S.cell_rank, cap);
     I(S.block_sums, (size_t)c->scan_blocks + 1);
-    I(S.err, 1); I(S.step, 1); I(S.shear3, 1);
+    I(S.err, 1); I(S.step, 1); I(S.shear3, 1); I(S.plane_dirty, 1);
     plan.push_back({(void **)&S.dtp, 8});
     plan.push_back({(void **)&S.dt_red, 16});
     plan.push_back({(void **)&S.tnow, 8});
@@ -562,6 +562,8 @@ static int derive_strength_constants(sphg_ctx *c) {
 }
 
 static int derive_mass_flags(sphg_ctx *c);
+static int derive_plane_zero(sphg_ctx *c);
+static void drop_graphs(sphg_ctx *c);
 
 extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host, int n, int ncomp) {
     if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
@@ -600,6 +602,10 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
     }
     if (nm == "mass" && c->uploaded)
         if (int rc = derive_mass_flags(c)) return rc;
+    if ((nm == "sig" || nm == "eps" || nm == "adps" || nm == "deps") && c->uploaded && c->P.plane_zero) {
+        c->P.plane_zero = 0;  // the plane step's zero components are no longer known
+        drop_graphs(c);
+    }
     if (nm == "ps_c") c->has_psc = true;
     if (nm == "ps_ds") c->has_psds = true;
     if (nm == "ps_cop") c->has_pscop = true;
@@ -829,6 +835,35 @@ static int derive_mass_flags(sphg_ctx *c) {
     return SPHG_OK;
 }
 
+// Params::plane_zero: a 2D context whose out-of-plane components are all +0 (bit for bit) and
+// stay so under every model of the reduced stress instantiation (elastic, Drucker-Prager, the
+// zero tangent of model 4, the elastic structure), with no TS transform and no particle that can
+// change type. Read from the device arrays; clears the flag the stress pass raises when a skipped
+// component turns out not to be +0.
+static int derive_plane_zero(sphg_ctx *c) {
+    Params &P = c->P;
+    int pz = P.ndim == 2 && !c->full_models && P.flts == 0 && P.fleob == 0 && P.flr == 0;
+    if (pz) {
+        int sh3 = 0;
+        HIPCHK(hipMemcpy(&sh3, c->S.shear3, 4, hipMemcpyDeviceToHost));
+        pz = sh3 == 0;
+    }
+    const size_t n = (size_t)c->n;
+    std::vector<unsigned long long> b(n);
+    double *const arrays[] = {c->S.sig[4], c->S.sig[5], c->S.eps[2], c->S.eps[4], c->S.eps[5], c->S.adps[4],
+                              c->S.adps[5], c->S.deps[2], c->S.deps[4], c->S.deps[5], c->S.weps[1], c->S.weps[2]};
+    for (double *a : arrays) {
+        if (!pz) break;
+        HIPCHK(hipMemcpy(b.data(), a, n * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n && pz; i++) pz = b[i] == 0ull;
+    }
+    const int zero = 0;
+    HIPCHK(hipMemcpy(c->S.plane_dirty, &zero, 4, hipMemcpyHostToDevice));
+    if (pz != P.plane_zero) drop_graphs(c);
+    P.plane_zero = pz;
+    return SPHG_OK;
+}
+
 extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     if (!c || n != c->n) return SPHG_E_ARG;
     int rc = check_supported(c->pro);
@@ -1028,6 +1063,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     }
     HIPCHK(hipMemcpy(c->S.shear3, &sh3, 4, hipMemcpyHostToDevice));
     if ((rc = derive_mass_flags(c))) return rc;
+    if ((rc = derive_plane_zero(c))) return rc;
     c->S.ps_c = c->has_psc ? c->d_psc : nullptr;
     int zero = 0;
     HIPCHK(hipMemcpy(c->S.err, &zero, 4, hipMemcpyHostToDevice));
@@ -1539,7 +1575,8 @@ extern "C" int sphg_load_state(sphg_ctx *c, const char *path) {
     c->stepped = h.stepped != 0;
     c->eig_stale = false;  // the file was written with strep / beta of its stress
     c->has_psc = h.has_psc != 0;
-    return derive_mass_flags(c);  // the file's masses
+    if (int rc = derive_mass_flags(c)) return rc;  // the file's masses
+    return derive_plane_zero(c);                   // and out-of-plane components
 }
 
 extern "C" int sphg_set_vibration(sphg_ctx *c, double ttime, int nsteps, double sttime, double edtime,

@@ -94,6 +94,12 @@ struct Params {
     // FP64 value of its FP32 rounding, where they read the mixed form's FP32 record) with no gather
     int mass_uniform;
     MassTable mass_t, mass_tf;
+    // 2D: every soil / structure particle's out-of-plane components (sig yz, xz; eps zz, yz, xz;
+    // adps yz, xz) and every particle's deps zz, yz, xz and spin w02, w12 are +0 (host check at
+    // upload). A plane step keeps them +0 (each is +0 plus a zero increment, or the constant the
+    // strain pass writes), so the strain and stress passes neither load nor store them; a computed
+    // value that is not +0 is stored and raises State::plane_dirty, which ends the skip.
+    int plane_zero;
 };
 
 // All device arrays of one context (pointers into one HBM allocation).
@@ -144,6 +150,7 @@ struct State {
     double *rij[9];       // clRaij: artificial-stress tensor per particle (persists between steps)
     double *tmp6[6];      // pre-pass copy for the Jacobi strain / stress regularisation
     double *brow[3];      // 2D FP64: boundary-stress rows of the momentum sums, from their own pass
+    int *plane_dirty;     // set once a skipped out-of-plane component turned out not to be +0
     int *nw_list;         // ... over the particles the search found next to a wall, and their
     int *nw_count;        // number (cleared before every search)
     // soil-structure interaction (ss_type 1-3): per soil / water particle the structure particles

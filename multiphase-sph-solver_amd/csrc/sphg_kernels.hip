@@ -1131,12 +1131,15 @@ __global__ __launch_bounds__(256) void k_strain(State S, Params P, int cur) {
     double d0 = divx[0], d1_ = divx[4], d2_ = divx[8];
     double d3_ = divx[1] + divx[3], d4_ = divx[5] + divx[7], d5_ = divx[6] + divx[2];
     if (NDIM == 2) d2_ = d4_ = d5_ = 0.0;
-    S.deps[0][i] = d0; S.deps[1][i] = d1_; S.deps[2][i] = d2_;
-    S.deps[3][i] = d3_; S.deps[4][i] = d4_; S.deps[5][i] = d5_;
     double w01 = 0.5 * (divx[1] - divx[3]);
     double w02 = NDIM == 2 ? 0.0 : 0.5 * (divx[2] - divx[6]);
     double w12 = NDIM == 2 ? 0.0 : 0.5 * (divx[5] - divx[7]);
-    S.weps[0][i] = w01; S.weps[1][i] = w02; S.weps[2][i] = w12;
+    S.deps[0][i] = d0; S.deps[1][i] = d1_; S.deps[3][i] = d3_;
+    S.weps[0][i] = w01;
+    if (NDIM == 3 || !P.plane_zero) {  // in a plane step the stored +0 is what these would write
+        S.deps[2][i] = d2_; S.deps[4][i] = d4_; S.deps[5][i] = d5_;
+        S.weps[1][i] = w02; S.weps[2][i] = w12;
+    }
     // eos loop 2 results (CPP_7-1-eos_CR.cpp:121-176)
     if (ti == 2 || ti == 0) {
         S.prew[i] = dw != 0.0 ? uw / dw : uw;
@@ -1591,13 +1594,30 @@ __device__ __forceinline__ void ts_after(const State &S, int i, double sig[6], d
 
 // kVar bit 0 (kFull) adds the Cam-Clay family and FOCC; bit 1 (kTs) the TS transform (flts >= 1).
 // Each is its own instantiation, so a deck without them carries none of their registers.
+// Out-of-plane components a plane step keeps at +0 (Params::plane_zero): sig / adps yz, xz and
+// eps / deps zz, yz, xz.
+__device__ constexpr bool sig_z(int k) { return k == 4 || k == 5; }
+__device__ constexpr bool eps_z(int k) { return k == 2 || k == 4 || k == 5; }
+
+// Store of a component the plane step skips (z): only a value that is not +0 is written, and it
+// ends the skip for the particle (pz) and, through the flag, for every later launch.
+__device__ __forceinline__ void store_z(const State &S, double *a, int i, double v, bool z, bool &pz) {
+    if (z && __double_as_longlong(v) == 0) return;
+    a[i] = v;
+    if (z) {
+        atomicOr(S.plane_dirty, 1);
+        pz = false;
+    }
+}
+
 template <int kVar>
 __device__ __forceinline__ void constitutive(const State &S, const Params &P, int i, int ti, int l, double sig[6],
-                                             const double deps[6]) {
+                                             const double deps[6], bool &pz) {
     constexpr bool kFull = (kVar & 1) != 0, kTs = (kVar & 2) != 0;
     const double dt = *S.dtp;
     // weps[a][b] from the three stored components (antisymmetric, zero diagonal)
-    const double w01 = S.weps[0][i], w02 = S.weps[1][i], w12 = S.weps[2][i];
+    const double w01 = S.weps[0][i];
+    const double w02 = P.plane_zero ? 0.0 : S.weps[1][i], w12 = P.plane_zero ? 0.0 : S.weps[2][i];
     // the reference forms weps[b][a] = 0.5*(divx[b][a]-divx[a][b]) separately; that equals -weps[a][b]
     // bit for bit except that a zero difference is +0 on both sides, which neg0 reproduces
     auto neg0 = [](double x) { return x == 0.0 ? 0.0 : -x; };
@@ -1701,15 +1721,20 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
         if (kTs && ti == 2) ts_after(S, i, sig, dsig);
         double eps[6];
         #pragma unroll
-        for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
+        for (int k = 0; k < 6; k++) eps[k] = (pz && eps_z(k)) ? 0.0 : S.eps[k][i];
         #pragma unroll
         for (int k = 0; k < 6; k++) {
-            if (ti == 2) S.adps[k][i] = S.adps[k][i] + adps[k] * dt;
+            if (ti == 2) {
+                const bool z = pz && sig_z(k);
+                const double a0 = z ? 0.0 : S.adps[k][i];
+                store_z(S, S.adps[k], i, a0 + adps[k] * dt, z, pz);
+            }
             eps[k] = eps[k] + deps[k] * dt;
             sig[k] = sig[k] + dsig[k] * dt + erfs[k];
         }
+        const bool pz_eps = pz;
         #pragma unroll
-        for (int k = 0; k < 6; k++) S.eps[k][i] = eps[k];
+        for (int k = 0; k < 6; k++) store_z(S, S.eps[k], i, eps[k], pz_eps && eps_z(k), pz);
         double vsig, veps, divq, divr;
         deq6(sig, eps, vsig, veps, divq, divr);
         // shearstrain (Class_particle.cpp:175-192)
@@ -1757,7 +1782,7 @@ __global__ __launch_bounds__(256) void k_eig(State S, Params P) {
 
 // free_surface_check (+ stress_adjust), the per-particle factor of the solid momentum sums,
 // principal stresses, and the artificial-stress tensor of particle i.
-__device__ __forceinline__ void stress_post(const State &S, const Params &P, int i, int ti, double sig[6]) {
+__device__ __forceinline__ void stress_post(const State &S, const Params &P, int i, int ti, double sig[6], bool pz) {
     // free-surface correction (CPP_7-2-constitutive.cpp:481-505)
     const double color = S.color[i], tmin = P.tmin, tmax = P.tmax, incl = tmax - tmin;
     if (ti == 1) {
@@ -1790,7 +1815,7 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
         }
         double eps[6], veps, divq, divr;
         #pragma unroll
-        for (int k = 0; k < 6; k++) eps[k] = S.eps[k][i];
+        for (int k = 0; k < 6; k++) eps[k] = (pz && eps_z(k)) ? 0.0 : S.eps[k][i];
         deq6(sig, eps, vsig, veps, divq, divr);
         S.vsig[i] = vsig;
         S.veps[i] = veps;
@@ -1798,8 +1823,9 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
         S.divr[i] = divr;
     }
     if (ti != 0 && ti != 7) {
+        const bool pz_sig = pz;
         #pragma unroll
-        for (int k = 0; k < 6; k++) S.sig[k][i] = sig[k];
+        for (int k = 0; k < 6; k++) store_z(S, S.sig[k], i, sig[k], pz_sig && sig_z(k), pz);
     }
     // the j factor of the momentum sums, in the reference's operation order: sig / rho / rho for
     // solids, (porosity * sig) / rho / rho for fluids when all fluids share the porosity the
@@ -1820,7 +1846,8 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
         if (P.ndim == 2 && (sig[4] != 0.0 || sig[5] != 0.0) && *S.shear3 == 0) atomicOr(S.shear3, 1);
         if (!S.mrc) {  // the mixed form's momentum pass reads the packed record instead
             #pragma unroll
-            for (int k = 0; k < 6; k++) S.qsig[k][i] = q[k];
+            for (int k = 0; k < 6; k++)
+                if (!(pz && sig_z(k))) S.qsig[k][i] = q[k];  // +0 / rho / rho stays the stored +0
         } else {
             // the fluid pass forms (porosity * sig) / rho / rho per pair when the fluids' porosities differ
             const bool raw = (ti == 1 || ti == 3) && !P.fluid_por_uniform;
@@ -1907,21 +1934,24 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
     int i = pidx();
     if (i >= P.n) return;
     const int ti = S.type[i];
+    bool pz = P.plane_zero && (ti == 2 || ti == 4) && *S.plane_dirty == 0;
     double sig[6];
     #pragma unroll
-    for (int k = 0; k < 6; k++) sig[k] = S.sig[k][i];
+    for (int k = 0; k < 6; k++) sig[k] = (pz && sig_z(k)) ? 0.0 : S.sig[k][i];
     if (part != 2) {
         double deps[6];
         #pragma unroll
-        for (int k = 0; k < 6; k++) deps[k] = S.deps[k][i];
-        constitutive<kVar>(S, P, i, ti, step_of(S), sig, deps);
+        for (int k = 0; k < 6; k++) deps[k] = (P.plane_zero && eps_z(k)) ? 0.0 : S.deps[k][i];
+        constitutive<kVar>(S, P, i, ti, step_of(S), sig, deps, pz);
         if (part == 1) {
-            if (ti != 0 && ti != 7)
-                for (int k = 0; k < 6; k++) S.sig[k][i] = sig[k];
+            if (ti != 0 && ti != 7) {
+                const bool pz_sig = pz;
+                for (int k = 0; k < 6; k++) store_z(S, S.sig[k], i, sig[k], pz_sig && sig_z(k), pz);
+            }
             return;
         }
     }
-    stress_post(S, P, i, ti, sig);
+    stress_post(S, P, i, ti, sig, pz);
 }
 
 // Jacobi form of strain_regu / stress_regu (CPP_7-2-constitutive.cpp:385-450): every step the
