@@ -540,13 +540,9 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
     if (ti != 7 && S.cell[i] != 0) {
         const Grid &g = S.grids[S.rid[i]];
         const int c = S.cell[i] - g.base;  // cell id within the particle's own grid
-        for (int s = 0; s < g.nstencil; s++) {
-            const int nll = c + g.nb_off[s];
-            if (nll <= 0 || nll > g.ncell) continue;
-            const int nl = g.base + nll;
-            const int st = S.cell_start[nl], en = st + S.cell_count[nl];
-            // the cell's candidates kSearchBatch at a time: their ids in one round trip, then their
-            // positions in one more, then the tests in cell order
+        // the candidates of one cell run [st, en) in cell order, kSearchBatch at a time: their ids in
+        // one round trip, then their positions in one more, then the tests
+        auto walk = [&](const int st, const int en) {
             for (int q0 = st; q0 < en; q0 += kSearchBatch<NDIM>) {
                 int eb[kSearchBatch<NDIM>];
                 double xb[kSearchBatch<NDIM>][3];
@@ -576,6 +572,61 @@ __global__ __launch_bounds__(kSearchBlock) void k_search(State S, Params P, int 
                         else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = e;
                         tmask |= 1 << nb_type(e);
                         cnt++;
+                    }
+                }
+            }
+        };
+        if constexpr (NDIM == 2) {
+            // the nine stencil cells' runs are loaded together before their candidates are walked;
+            // offsets in nncell order as celllist_ini2 forms them (ky outer, kx inner)
+            int cst[9], cen[9];
+#pragma unroll
+            for (int s = 0; s < 9; s++) {
+                const int nll = c + (s % 3 - 1) * g.cym + (s / 3 - 1);
+                const bool ok = nll > 0 && nll <= g.ncell;
+                cst[s] = ok ? S.cell_start[g.base + nll] : 0;
+                cen[s] = ok ? cst[s] + S.cell_count[g.base + nll] : 0;
+            }
+#pragma unroll
+            for (int s = 0; s < 9; s++) walk(cst[s], cen[s]);
+        } else {
+            // 3D: one cell at a time (loading a row's or a layer's runs together costs waves; the
+            // loop is written out, the shared walk above costs this instantiation a wave too)
+            for (int s = 0; s < g.nstencil; s++) {
+                const int nll = c + g.nb_off[s];
+                if (nll <= 0 || nll > g.ncell) continue;
+                const int nl = g.base + nll;
+                const int st = S.cell_start[nl], en = st + S.cell_count[nl];
+                for (int q0 = st; q0 < en; q0 += kSearchBatch<NDIM>) {
+                    int eb[kSearchBatch<NDIM>];
+                    double xb[kSearchBatch<NDIM>][3];
+#pragma unroll
+                    for (int u = 0; u < kSearchBatch<NDIM>; u++) eb[u] = S.sorted[q0 + u < en ? q0 + u : en - 1];
+#pragma unroll
+                    for (int u = 0; u < kSearchBatch<NDIM>; u++) load_xj<NDIM>(S, cur, nb_id(eb[u]), xb[u]);
+#pragma unroll
+                    for (int u = 0; u < kSearchBatch<NDIM>; u++) {
+                        if (q0 + u >= en) continue;
+                        const int e = eb[u];
+                        const int j = nb_id(e);
+                        double d2 = 0.0;
+#pragma unroll
+                        for (int k = 0; k < NDIM; k++) {
+                            double d = xi[k] - xb[u][k];
+                            d2 = d2 + d * d;
+                        }
+                        double R = 2 * ((hli + hl_of(S, P, j, hli)) / 2);
+                        double R2 = R * R;
+                        bool in;
+                        if (d2 < R2 * 0.99999999999999) in = true;
+                        else if (d2 > R2 * 1.00000000000001) in = false;
+                        else in = sqrt(d2) <= R;
+                        if (in && i != j) {
+                            if (cnt < STAGE) stage[cnt][t] = e;
+                            else if (cnt < kMaxNb) S.nbr[(size_t)cnt * P.cap + i] = e;
+                            tmask |= 1 << nb_type(e);
+                            cnt++;
+                        }
                     }
                 }
             }
