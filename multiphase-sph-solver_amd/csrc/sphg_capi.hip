@@ -416,10 +416,12 @@ static int field_ptrs(sphg_ctx *c, const char *name, double **out, int &ncomp, b
 }
 
 // strep / beta of the current stress when the last stress pass left them out (no consumer in the
-// step): the same Stress_eigValue of the same stress, so the values are those of the pass.
+// step): the same Stress_eigValue of the same stress, so the values are those of the pass; with
+// them vsig / veps / divq / divr / meps where the step leaves those out too (Params::lazy_deq).
 static int ensure_eig(sphg_ctx *c) {
     if (!c->eig_stale) return SPHG_OK;
     hipLaunchKernelGGL(k_eig, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->S, c->P);
+    if (c->P.lazy_deq) hipLaunchKernelGGL(k_deq, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->S, c->P);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) return SPHG_E_HIP;
     c->eig_stale = false;
     return SPHG_OK;
@@ -431,7 +433,8 @@ extern "C" int sphg_get_field(sphg_ctx *c, const char *name, double *host, int n
     if (!c || !name || !host || n != c->n) return SPHG_E_ARG;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return SPHG_E_HIP;
     std::string nm(name);
-    if (nm == "strep" || nm == "beta" || nm == "beta_ts") {
+    if (nm == "strep" || nm == "beta" || nm == "beta_ts" || nm == "vsig" || nm == "veps" || nm == "divq" ||
+        nm == "divr" || nm == "meps") {
         const int rc = ensure_eig(c);
         if (rc) return rc;
     }
@@ -567,6 +570,9 @@ static void drop_graphs(sphg_ctx *c);
 
 extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host, int n, int ncomp) {
     if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
+    // outputs formed on read from the stress (k_eig, k_deq) are formed from the stress of the last
+    // step, before the caller changes anything
+    if (ensure_eig(c)) return SPHG_E_HIP;
     if (c->n == 0) c->n = n;
     if (n != c->n) return SPHG_E_ARG;
     std::string nm(name);
@@ -618,6 +624,7 @@ extern "C" int sphg_set_field(sphg_ctx *c, const char *name, const double *host,
 
 extern "C" int sphg_set_field_i(sphg_ctx *c, const char *name, const int *host, int n) {
     if (!c || !name || !host || n <= 0 || n > c->cap) return SPHG_E_ARG;
+    if (ensure_eig(c)) return SPHG_E_HIP;
     if (c->n == 0) c->n = n;
     if (n != c->n) return SPHG_E_ARG;
     std::string nm(name);
@@ -907,6 +914,7 @@ extern "C" int sphg_upload_done(sphg_ctx *c, int n) {
     P.struct_e = c->mat.structure.e;
     P.struct_poi = c->mat.structure.poi;
     P.eig_in_step = (pro.flts >= 1 || pro.art_stre_coe > 0.0) ? 1 : 0;
+    P.lazy_deq = (!P.eig_in_step && pro.fleob == 0) ? 1 : 0;
     P.mrv_ok = (pro.ss_type == 0 && pro.flbndy != 5 && pro.flv == 0) ? 1 : 0;
     if (pro.flv > 0 && !c->S.vib_loads) return SPHG_E_ARG;  // the reference stops when vibra.dat is missing
     P.vib_ttime = c->vib_ttime;

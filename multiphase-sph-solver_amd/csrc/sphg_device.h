@@ -100,6 +100,9 @@ struct Params {
     // strain pass writes), so the strain and stress passes neither load nor store them; a computed
     // value that is not +0 is stored and raises State::plane_dirty, which ends the skip.
     int plane_zero;
+    // vsig / veps / divq / divr / meps are not stored by the step (no consumer inside it: no TS
+    // transform, no artificial stress, fixed particle types); k_deq forms them when they are read
+    int lazy_deq;
 };
 
 // All device arrays of one context (pointers into one HBM allocation).

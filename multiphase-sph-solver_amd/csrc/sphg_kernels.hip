@@ -1240,6 +1240,36 @@ __device__ __forceinline__ void deq6(const double sig[6], const double eps[6], d
     divr = vsig != 0.0 ? fabs(divq / vsig) : 0.0;
 }
 
+// shearstrain (Class_particle.cpp:175-192): the equivalent shear strain of eps with volume strain veps
+__device__ __forceinline__ double meps_of(const double eps[6], double veps) {
+    double sxx = eps[0] - veps / 3.0, syy = eps[1] - veps / 3.0, szz = eps[2] - veps / 3.0;
+    double j2 = 0.5 * (sxx * sxx + syy * syy + szz * szz) + eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5];
+    return sqrt(2.0 * fabs(j2));
+}
+
+// vsig / veps / divq / divr and meps of soil and structure particles from the stored stress and
+// strain, when the step left them out (Params::lazy_deq): deq6 and shearstrain of the same values
+// the stress pass forms them from last (its final stress; the strain its update stored), so the
+// numbers are the pass's. Run on read, like k_eig.
+__global__ __launch_bounds__(256) void k_deq(State S, Params P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    if (ti != 2 && ti != 4) return;
+    double sig[6], eps[6], vsig, veps, divq, divr;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        sig[k] = S.sig[k][i];
+        eps[k] = S.eps[k][i];
+    }
+    deq6(sig, eps, vsig, veps, divq, divr);
+    S.meps[i] = meps_of(eps, veps);
+    S.veps[i] = veps;
+    S.vsig[i] = vsig;
+    S.divq[i] = divq;
+    S.divr[i] = divr;
+}
+
 __device__ __forceinline__ void iso_tangent(double D[6][6], double e0, double poi, double &dlam, double &dmu) {
     dlam = e0 * poi / (1.0 + poi) / (1.0 - 2.0 * poi);
     dmu = e0 * 0.5 / (1.0 + poi);
@@ -1663,7 +1693,7 @@ __device__ __forceinline__ void store_z(const State &S, double *a, int i, double
 
 template <int kVar>
 __device__ __forceinline__ void constitutive(const State &S, const Params &P, int i, int ti, int l, double sig[6],
-                                             const double deps[6], bool &pz) {
+                                             const double deps[6], bool &pz, double &vsig_out) {
     constexpr bool kFull = (kVar & 1) != 0, kTs = (kVar & 2) != 0;
     const double dt = *S.dtp;
     // weps[a][b] from the three stored components (antisymmetric, zero diagonal)
@@ -1788,20 +1818,21 @@ __device__ __forceinline__ void constitutive(const State &S, const Params &P, in
         for (int k = 0; k < 6; k++) store_z(S, S.eps[k], i, eps[k], pz_eps && eps_z(k), pz);
         double vsig, veps, divq, divr;
         deq6(sig, eps, vsig, veps, divq, divr);
-        // shearstrain (Class_particle.cpp:175-192)
-        double sxx = eps[0] - veps / 3.0, syy = eps[1] - veps / 3.0, szz = eps[2] - veps / 3.0;
-        double j2 = 0.5 * (sxx * sxx + syy * syy + szz * szz) + eps[3] * eps[3] + eps[4] * eps[4] + eps[5] * eps[5];
-        S.meps[i] = sqrt(2.0 * fabs(j2));
+        vsig_out = vsig;
         // poro_cal (Class_particle.cpp:194-203)
         double e = S.e[i];
         double de = (1 + e) * (deps[0] + deps[1] + deps[2]) * dt, et = e + de;
         if (et < 0.0) et = 0.0;
         S.porosity[i] = et / (1 + et);
         S.e[i] = et;
-        S.veps[i] = veps;
-        S.vsig[i] = vsig;
-        S.divq[i] = divq;
-        S.divr[i] = divr;
+        if (!P.lazy_deq) {  // otherwise k_deq forms them from the stored stress and strain on read
+            // shearstrain (Class_particle.cpp:175-192)
+            S.meps[i] = meps_of(eps, veps);
+            S.veps[i] = veps;
+            S.vsig[i] = vsig;
+            S.divq[i] = divq;
+            S.divr[i] = divr;
+        }
     }
 }
 
@@ -1833,7 +1864,8 @@ __global__ __launch_bounds__(256) void k_eig(State S, Params P) {
 
 // free_surface_check (+ stress_adjust), the per-particle factor of the solid momentum sums,
 // principal stresses, and the artificial-stress tensor of particle i.
-__device__ __forceinline__ void stress_post(const State &S, const Params &P, int i, int ti, double sig[6], bool pz) {
+__device__ __forceinline__ void stress_post(const State &S, const Params &P, int i, int ti, double sig[6], bool pz,
+                                            bool have_vsig, double vsig_c) {
     // free-surface correction (CPP_7-2-constitutive.cpp:481-505)
     const double color = S.color[i], tmin = P.tmin, tmax = P.tmax, incl = tmax - tmin;
     if (ti == 1) {
@@ -1856,7 +1888,7 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
             #pragma unroll
             for (int k = 0; k < 6; k++) sig[k] = xs * sig[k];
         }
-        double vsig = S.vsig[i];
+        double vsig = have_vsig ? vsig_c : S.vsig[i];  // the constitutive update's
         if (vsig > 0.0) {  // stress_adjust (Class_particle.cpp:205-215)
             double t = -vsig;
             sig[0] = sig[0] + t;
@@ -1867,11 +1899,13 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
         double eps[6], veps, divq, divr;
         #pragma unroll
         for (int k = 0; k < 6; k++) eps[k] = (pz && eps_z(k)) ? 0.0 : S.eps[k][i];
-        deq6(sig, eps, vsig, veps, divq, divr);
-        S.vsig[i] = vsig;
-        S.veps[i] = veps;
-        S.divq[i] = divq;
-        S.divr[i] = divr;
+        if (!P.lazy_deq) {
+            deq6(sig, eps, vsig, veps, divq, divr);
+            S.vsig[i] = vsig;
+            S.veps[i] = veps;
+            S.divq[i] = divq;
+            S.divr[i] = divr;
+        }
     }
     if (ti != 0 && ti != 7) {
         const bool pz_sig = pz;
@@ -1986,6 +2020,8 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
     if (i >= P.n) return;
     const int ti = S.type[i];
     bool pz = P.plane_zero && (ti == 2 || ti == 4) && *S.plane_dirty == 0;
+    bool have_vsig = false;
+    double vsig_c = 0.0;
     double sig[6];
     #pragma unroll
     for (int k = 0; k < 6; k++) sig[k] = (pz && sig_z(k)) ? 0.0 : S.sig[k][i];
@@ -1993,8 +2029,10 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
         double deps[6];
         #pragma unroll
         for (int k = 0; k < 6; k++) deps[k] = (P.plane_zero && eps_z(k)) ? 0.0 : S.deps[k][i];
-        constitutive<kVar>(S, P, i, ti, step_of(S), sig, deps, pz);
+        constitutive<kVar>(S, P, i, ti, step_of(S), sig, deps, pz, vsig_c);
+        have_vsig = ti == 2;
         if (part == 1) {
+            if (ti == 2 && P.lazy_deq) S.vsig[i] = vsig_c;  // read by the post-processing launch
             if (ti != 0 && ti != 7) {
                 const bool pz_sig = pz;
                 for (int k = 0; k < 6; k++) store_z(S, S.sig[k], i, sig[k], pz_sig && sig_z(k), pz);
@@ -2002,7 +2040,7 @@ __global__ __launch_bounds__(256) void k_stress(State S, Params P, int part) {
             return;
         }
     }
-    stress_post(S, P, i, ti, sig, pz);
+    stress_post(S, P, i, ti, sig, pz, have_vsig, vsig_c);
 }
 
 // Jacobi form of strain_regu / stress_regu (CPP_7-2-constitutive.cpp:385-450): every step the
