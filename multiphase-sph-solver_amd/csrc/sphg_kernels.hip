@@ -3533,7 +3533,7 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
             S.vxp[k][i] = v[k];
             S.axp[k][i] = S.ax[k][i];
         }
-        if (NDIM == 2) S.xp[nxt][2][i] = S.xp[cur][2][i];
+        // 2D: z is set by the host in both position buffers and nothing changes it, so no copy
         S.rhop[i] = S.rho[i];
         if constexpr (M::kFast)  // the velocity just stored (vxp's third component stays in 2D)
             S.mra[i] = make_float4((float)S.mass[i], (float)v[0], (float)v[1], (float)(NDIM == 3 ? v[2] : S.vxp[2][i]));
@@ -3546,10 +3546,11 @@ __global__ __launch_bounds__(256) void k_update(State S, Params P, int cur) {
                 S.xp[nxt][k][i] = xi[k] + (f + f) * dt * 0.5;
                 S.ux[k][i] = S.ux[k][i] + (f + f) * dt * 0.5;
             }
-            if (NDIM == 2) S.xp[nxt][2][i] = S.xp[cur][2][i];
             S.rhop[i] = S.rho[i];
             S.prep[i] = S.pre[i];
-        } else {
+        } else if (P.fleob) {
+            // a wall that has always been one holds the same position in both buffers (the host sets
+            // both); with excavation / backfill it may have moved as another type
             #pragma unroll
             for (int k = 0; k < 3; k++) S.xp[nxt][k][i] = S.xp[cur][k][i];
         }

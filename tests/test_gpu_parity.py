@@ -108,6 +108,19 @@ def test_column2d_out_of_plane_shear_vs_oracle():
         assert np.abs(g.get("ax")[soil, 2]).max() > 0.0  # the third row is live
 
 
+def test_outputs_formed_on_read_follow_the_last_step():
+    # vsig / veps / divq / divr / meps and the principal stresses are formed on read from the stored
+    # stress and strain; a host write after the step must not change what the step left
+    deck, states = load_golden("column2d_small")
+    g = make_gpu(deck, states[0])
+    o = OracleSim(deck, states[0], mode=JACOBI)
+    assert g.step(3) == 0 and o.step(3) == 0
+    g.set("sig", np.asarray(states[0]["sig"]) * 1.5)
+    g.set("eps", np.zeros_like(np.asarray(states[0]["sig"])))
+    for k in ("vsig", "veps", "divq", "divr", "meps", "strep"):
+        assert bitequal(g.get(k), o.get(k)), k
+
+
 def test_soilwater3d_vs_oracle():
     deck, states = load_golden("soilwater3d_small")
     g = make_gpu(deck, states[0])
