@@ -2539,6 +2539,14 @@ __global__ __launch_bounds__(256) void k_drag(State S, Params P, int cur) {
 template <int NDIM, class M>
 constexpr bool kSplitBrow = NDIM == 2 && !M::kFast && !M::kKgc;
 
+// Neighbour mass in the momentum pass: the common-mass table except in 3D FP64, where its selects
+// would cost the pass its third wave per SIMD (175 instead of 167 VGPRs).
+template <int NDIM, class M>
+__device__ __forceinline__ double accel_mass(const State &S, const MassPick &mp, int j, int tj) {
+    if constexpr (NDIM == 3 && !M::kFast) return S.mass[j];
+    else return nb_mass(S, mp, j, tj);
+}
+
 template <int NDIM, class M>
 __global__ __launch_bounds__(256) void k_bndy_rows(State S, Params P, int cur) {
     // over the search's list of non-wall particles with a wall neighbour, so that its waves are full
@@ -2633,7 +2641,7 @@ __global__ __launch_bounds__(256, (kSplitBrow<NDIM, M> ? 4 : 1)) void k_accel(St
                 copjb[b] = drag_on ? rc.cop : 1.0;
                 continue;
             }
-            mjb[b] = nb_mass(S, mp, j, nb_type(e[b]));
+            mjb[b] = accel_mass<NDIM, M>(S, mp, j, nb_type(e[b]));
             rjb[b] = S.rho[j];
             #pragma unroll
             for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
@@ -2734,7 +2742,7 @@ __global__ __launch_bounds__(256, (kSplitBrow<NDIM, M> ? 4 : 1)) void k_accel(St
                 preab[b] = rc.prea;
                 continue;
             }
-            mjb[b] = nb_mass(S, mp, j, nb_type(e[b]));
+            mjb[b] = accel_mass<NDIM, M>(S, mp, j, nb_type(e[b]));
             rjb[b] = S.rho[j];
             #pragma unroll
             for (int k = 0; k < 3; k++) vjb[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
@@ -2823,7 +2831,7 @@ __global__ __launch_bounds__(256, (kSplitBrow<NDIM, M> ? 4 : 1)) void k_accel(St
                 const int j = nb_id(e[b]);
                 load_xj<NDIM>(S, cur, j, xj[b]);
                 hlj[b] = hl_of(S, P, j, hli);
-                mj[b] = nb_mass(S, mp, j, nb_type(e[b]));
+                mj[b] = accel_mass<NDIM, M>(S, mp, j, nb_type(e[b]));
                 rj[b] = S.rho[j];
                 #pragma unroll
                 for (int k = 0; k < 3; k++) vj[b][k] = (k < NDIM) ? S.vxp[k][j] : 0.0;
