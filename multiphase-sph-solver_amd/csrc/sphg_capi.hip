@@ -1193,6 +1193,7 @@ static void enqueue_step(sphg_ctx *c, int phase_mask) {
         } else {
             LAUNCH("stress", ks, grid_for(n), c->S, c->P, 0);
         }
+        if (c->P.eig_in_step) LAUNCH("stress_eig", k_stress_eig, grid_for(n), c->S, c->P);
     }
     if (phase_mask & 16) {
         if (c->pre_on) LAUNCH("accel_pre", (k_accel_pre<NDIM, M>), grid_for(n), c->S, c->P, cur);

@@ -1945,10 +1945,20 @@ __device__ __forceinline__ void stress_post(const State &S, const Params &P, int
             o[2] = make_float4((float)S.prew[i], (float)S.prea[i], (float)S.porosity[i], (float)S.cop[i]);
         }
     }
-    // principal stresses (CPP_7-1-strain.cpp:283-332). Only the TS transform (next step) and the
-    // artificial stress (below) read them inside the step; otherwise they are left to k_eig, run
-    // on the final stress when the host reads them, which gives the same values.
-    if (!P.eig_in_step) return;
+}
+
+// principal stresses (CPP_7-1-strain.cpp:283-332) and the artificial-stress tensor of the final
+// stress, in a pass of their own after the stress pass when a pass of the step reads them (the TS
+// transform next step, the artificial stress); otherwise they are left to k_eig, run on the final
+// stress when the host reads them, which gives the same values. Apart, their registers do not hold
+// the stress pass back (133 -> 107 VGPRs, four waves per SIMD).
+__global__ __launch_bounds__(256) void k_stress_eig(State S, Params P) {
+    const int i = pidx();
+    if (i >= P.n) return;
+    const int ti = S.type[i];
+    double sig[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) sig[k] = S.sig[k][i];
     double a[9], v[9];
     principal_axes(S, i, sig, a, v);
     // artificial-stress tensor (CPP_9-5-artificialstress.cpp:42-89 in 2D, :161-227 in 3D)
